@@ -1,0 +1,256 @@
+"""GPUSolver mirror over the C ABI (Sources/PhysicsAVBD/GPU/GPUSolver.swift).
+
+Method names follow the reference's Swift surface (step, submitStep,
+synchronize, bodyPosition, bodyStates, setBodyStates, debugColors,
+maxConstraintError, lastNumPairs ...) so tests read like the reference's own.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from .scene import SimSettings
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class GPUSolver:
+    def __init__(self, scene, maxPairsPerBody=16, lanesPerBody=0, sweepMode=0, useGraph=True,
+                 pairCapacityForTesting=0):
+        self._lib = _capi.load()
+        self.scene = scene
+        desc, keep = scene.to_desc()
+        opts = _capi.Options()
+        self._lib.avbd_options_default(C.byref(opts))
+        opts.max_pairs_per_body = int(maxPairsPerBody)
+        opts.lanes_per_body = int(lanesPerBody)
+        opts.sweep_mode = int(sweepMode)
+        opts.use_graph = 1 if useGraph else 0
+        opts.pair_capacity_override = int(pairCapacityForTesting)
+        h = C.c_void_p()
+        st = self._lib.avbd_create(C.byref(desc), C.byref(opts), C.byref(h))
+        self._h = h
+        self.bodyCount = len(scene.bodies)
+        if st == _capi.ERR_RUNTIME_FAILURE and h:
+            f = self._failure()
+            self._lib.avbd_destroy(h)
+            self._h = None
+            raise f
+        if st != _capi.OK:
+            self._h = None
+            raise _capi.AVBDError(st, "avbd_create")
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.avbd_destroy(self._h)
+            self._h = None
+
+    # -- failures ---------------------------------------------------------------
+    def _failure(self):
+        f = _capi.Failure()
+        self._lib.avbd_last_failure(self._h, C.byref(f))
+        return _capi.RuntimeFailure(f.kind, f.frame, f.required, f.capacity, f.message.decode())
+
+    def _check(self, st, what):
+        if st == _capi.OK:
+            return
+        if st == _capi.ERR_RUNTIME_FAILURE:
+            raise self._failure()
+        raise _capi.AVBDError(st, what)
+
+    @property
+    def runtimeFailure(self):
+        f = _capi.Failure()
+        self._lib.avbd_last_failure(self._h, C.byref(f))
+        return None if f.kind == 0 else self._failure()
+
+    # -- stepping -----------------------------------------------------------------
+    def submitStep(self):
+        self._check(self._lib.avbd_submit_step(self._h), "submitStep")
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._check(self._lib.avbd_step(self._h), "step")
+
+    def synchronize(self):
+        self._check(self._lib.avbd_synchronize(self._h), "synchronize")
+
+    @property
+    def settings(self):
+        s = _capi.Settings()
+        self._lib.avbd_get_settings(self._h, C.byref(s))
+        return SimSettings.from_c(s)
+
+    @settings.setter
+    def settings(self, value):
+        s = value.to_c()
+        self._check(self._lib.avbd_set_settings(self._h, C.byref(s)), "settings")
+
+    # -- state --------------------------------------------------------------------
+    def bodyStates(self, idx=None):
+        """Returns (pos[n,3], rot[n,4], vel[n,3], angvel[n,3]) as float32 arrays."""
+        if idx is None:
+            n = self.bodyCount
+            ip = None
+        else:
+            idx = np.ascontiguousarray(idx, dtype=np.uint32)
+            n = len(idx)
+            ip = _u32p(idx)
+        p = np.zeros((n, 3), np.float32)
+        r = np.zeros((n, 4), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        w = np.zeros((n, 3), np.float32)
+        self._check(self._lib.avbd_get_body_states(self._h, ip, n, _fp(p), _fp(r), _fp(v), _fp(w)), "bodyStates")
+        return p, r, v, w
+
+    def positions(self):
+        return self.bodyStates()[0]
+
+    def bodyPosition(self, i):
+        return self.bodyStates([i])[0][0]
+
+    def bodyRotation(self, i):
+        return self.bodyStates([i])[1][0]
+
+    def bodyVelocity(self, i):
+        return self.bodyStates([i])[2][0]
+
+    def bodyAngularVelocity(self, i):
+        return self.bodyStates([i])[3][0]
+
+    def bodyMass(self, i=None):
+        m = np.zeros(self.bodyCount, np.float32)
+        inertia = np.zeros((self.bodyCount, 3), np.float32)
+        self._check(self._lib.avbd_get_body_mass(self._h, self.bodyCount, _fp(m), _fp(inertia)), "bodyMass")
+        return m if i is None else m[i]
+
+    def setBodyStates(self, idx, pos, rot, vel, angvel):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        rot = np.ascontiguousarray(rot, dtype=np.float32)
+        vel = np.ascontiguousarray(vel, dtype=np.float32)
+        angvel = np.ascontiguousarray(angvel, dtype=np.float32)
+        self._check(self._lib.avbd_set_body_states(self._h, _u32p(idx), len(idx), _fp(pos), _fp(rot), _fp(vel),
+                                                   _fp(angvel)), "setBodyStates")
+
+    def setMotorTargets(self, joints, targets):
+        j = np.ascontiguousarray(joints, dtype=np.uint32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        self._check(self._lib.avbd_set_motor_targets(self._h, _u32p(j), len(j), _fp(t)), "setMotorTargets")
+
+    def setMotorTorques(self, joints, torques):
+        j = np.ascontiguousarray(joints, dtype=np.uint32)
+        t = np.ascontiguousarray(torques, dtype=np.float32)
+        self._check(self._lib.avbd_set_motor_torques(self._h, _u32p(j), len(j), _fp(t)), "setMotorTorques")
+
+    # -- stats / debug --------------------------------------------------------------
+    def stats(self):
+        s = _capi.Stats()
+        self._check(self._lib.avbd_get_stats(self._h, C.byref(s)), "stats")
+        return s
+
+    @property
+    def lastNumPairs(self):
+        return self.stats().num_pairs
+
+    @property
+    def lastPairCandidates(self):
+        return self.stats().pair_candidates
+
+    @property
+    def lastColorCounts(self):
+        return list(self.stats().color_counts)
+
+    @property
+    def lastMaxColorUsed(self):
+        return self.stats().max_color_used
+
+    @property
+    def frameIndex(self):
+        return self.stats().frame_index
+
+    def debugColors(self):
+        c = np.zeros(self.bodyCount, np.uint32)
+        self._check(self._lib.avbd_get_colors(self._h, self.bodyCount, _u32p(c)), "debugColors")
+        return c
+
+    def cellLists(self):
+        hs, nh, cs = C.c_uint32(), C.c_uint32(), C.c_float()
+        self._check(self._lib.avbd_get_cell_lists(self._h, C.byref(hs), C.byref(nh), C.byref(cs), None, None, None),
+                    "cellLists")
+        start = np.zeros(hs.value, np.uint32)
+        count = np.zeros(hs.value, np.uint32)
+        bodies = np.zeros(max(1, nh.value), np.uint32)
+        self._check(self._lib.avbd_get_cell_lists(self._h, None, None, None, _u32p(start), _u32p(count),
+                                                  _u32p(bodies)), "cellLists")
+        return dict(hash_size=hs.value, cell_size=cs.value, start=start, count=count, bodies=bodies[:nh.value])
+
+    def pairs(self):
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_pairs(self._h, 0, C.byref(n), None), "pairs")
+        out = np.zeros((max(1, n.value), 2), np.uint32)
+        self._check(self._lib.avbd_get_pairs(self._h, n.value, C.byref(n), _u32p(out)), "pairs")
+        return out[:n.value]
+
+    MANIFOLD_DTYPE = np.dtype([("header", np.uint32, 4), ("colliderPair", np.uint32, 4), ("basisN", np.float32, 4),
+                               ("basisT1", np.float32, 4),
+                               ("c", [("rA", np.float32, 4), ("rB", np.float32, 4), ("C0", np.float32, 4),
+                                      ("lambda", np.float32, 4), ("penalty", np.float32, 4)], 8)])
+
+    def manifolds(self):
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_manifolds(self._h, 0, C.byref(n), None), "manifolds")
+        out = np.zeros(max(1, n.value), self.MANIFOLD_DTYPE)
+        self._check(self._lib.avbd_get_manifolds(self._h, n.value, C.byref(n), out.ctypes.data_as(C.c_void_p)),
+                    "manifolds")
+        return out[:n.value]
+
+    def mapFind(self, pairs):
+        q = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        out = np.zeros(len(q), np.int32)
+        self._check(self._lib.avbd_map_find(self._h, _u32p(q), len(q), out.ctypes.data_as(C.POINTER(C.c_int32))),
+                    "mapFind")
+        return out
+
+    def maxConstraintError(self):
+        v = C.c_float()
+        self._check(self._lib.avbd_max_constraint_error(self._h, C.byref(v)), "maxConstraintError")
+        return v.value
+
+    def residual(self):
+        v = C.c_float()
+        self._check(self._lib.avbd_residual(self._h, C.byref(v)), "residual")
+        return v.value
+
+    def timePrimal(self, reps=20):
+        us, by, la = C.c_float(), C.c_double(), C.c_int32()
+        self._check(self._lib.avbd_time_primal(self._h, int(reps), C.byref(us), C.byref(by), C.byref(la)),
+                    "timePrimal")
+        return dict(avg_launch_us=us.value, sweep_bytes=by.value, launches_per_sweep=la.value)
+
+    def setProfiling(self, on=True):
+        self._check(self._lib.avbd_get_profile(self._h, 1 if on else 0, None, 0, None, None), "profile")
+
+    def profile(self):
+        names = C.create_string_buffer(1024)
+        ms = np.zeros(32, np.float32)
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_profile(self._h, 1, names, 1024, _fp(ms), C.byref(n)), "profile")
+        keys = [k for k in names.value.decode().split(",") if k]
+        return dict(zip(keys, ms[:n.value].tolist()))
+
+    def denyStageForTesting(self, stage, frame=-1):
+        self._lib.avbd_test_deny_stage(self._h, stage.encode() if stage else None, int(frame))
+
+
+def version():
+    return _capi.load().avbd_version().decode()

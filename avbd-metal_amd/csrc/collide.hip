@@ -1,0 +1,731 @@
+// collide.hip -- spatial hash, deterministic rigid pair stream, CAS
+// persistence map and narrow phase for gfx950.
+//
+// Integer outputs of this file (hash cells, cell lists, pair sets, map
+// contents) must be bit-reproducible, so floating-point contraction is off
+// here: every product/sum rounds exactly as written.
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace avbd {
+
+// ---------------------------------------------------------------------------
+// spatial hash (20_broadphase.metal:12-32)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cell_coord(V3 p, float cellSize, int& cx, int& cy, int& cz) {
+  cx = (int)floorf(p.x / cellSize);
+  cy = (int)floorf(p.y / cellSize);
+  cz = (int)floorf(p.z / cellSize);
+}
+// Multiply-xor of three primes, salted by collision group, then an
+// avalanche finish (xorshift-multiply) so lattice-aligned cells spread.
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t group, uint32_t hashSize) {
+  uint32_t h = ((uint32_t)cx * 92837111u) ^ ((uint32_t)cy * 689287499u) ^ ((uint32_t)cz * 283923481u);
+  h ^= group * 0x9e3779b9u;
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  return h & (hashSize - 1u);
+}
+__device__ __forceinline__ V3 collider_center(const DeviceState& S, uint32_t c) {
+  uint32_t b = S.colliderOwner[c];
+  return xyz(S.posLin[b]) + q_rotate(S.posAng[b], xyz(S.colliderLocalPos[c]));
+}
+__device__ __forceinline__ bool groups_compatible(uint32_t a, uint32_t b) {
+  return a == 0u || b == 0u || a == b;
+}
+__device__ __forceinline__ bool domains_compatible(uint32_t ga, uint32_t gb, uint32_t sa, uint32_t sb) {
+  if (!groups_compatible(ga, gb)) return false;
+  if (ga == 0u && gb != 0u) return sb != 0u;
+  if (gb == 0u && ga != 0u) return sa != 0u;
+  return true;
+}
+
+// Count colliders per cell and remember each collider's (cell, slot).
+__global__ void __launch_bounds__(256) k_bp_count(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed) return;
+  uint32_t c = S.hashedIdx[gid];
+  V3 p = collider_center(S, c);
+  int cx, cy, cz;
+  cell_coord(p, P.cellSize, cx, cy, cz);
+  uint32_t h = cell_hash(cx, cy, cz, S.colliderGroup[c], P.gridHashSize);
+  uint32_t slot = atomicAdd(&S.cellCount[h], 1u);
+  S.bodyCellSlot[gid] = U2{h, slot};
+  if (S.colliderShape[c].w >= 0.0f) S.cellRigid[h] = 1u;
+}
+
+__global__ void __launch_bounds__(256) k_bp_scatter(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed) return;
+  U2 cs = S.bodyCellSlot[gid];
+  S.cellBodies[S.cellStart[cs.x] + cs.y] = S.hashedIdx[gid];
+}
+
+// Sort each bucket by collider id so pair enumeration order is independent
+// of the atomic scatter schedule.
+__global__ void __launch_bounds__(256) k_bp_sort_cells(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.gridHashSize) return;
+  uint32_t n = S.cellCount[gid];
+  if (n < 2) return;
+  uint32_t s = S.cellStart[gid];
+  for (uint32_t i = 1; i < n; i++) {
+    uint32_t v = S.cellBodies[s + i];
+    uint32_t j = i;
+    while (j > 0 && S.cellBodies[s + j - 1] > v) {
+      S.cellBodies[s + j] = S.cellBodies[s + j - 1];
+      j--;
+    }
+    S.cellBodies[s + j] = v;
+  }
+}
+
+__device__ __forceinline__ bool pair_excluded(const DeviceState& S, uint32_t count, uint32_t a, uint32_t b) {
+  uint32_t lo = 0, hi = count;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    U2 e = S.exclusions[mid];
+    if (e.x < a || (e.x == a && e.y < b)) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo >= count) return false;
+  U2 e = S.exclusions[lo];
+  return e.x == a && e.y == b;
+}
+
+// One producer (hashed collider, or global collider) counts -- or emits at
+// its scanned offset -- its accepted candidates.  The same predicate chain
+// runs twice so the output order is a pure function of the state
+// (20_broadphase.metal:287-424).
+__device__ uint32_t bp_producer(const DeviceState& S, const StepParams& P, uint32_t producer, bool emit,
+                                uint32_t base) {
+  uint32_t count = 0;
+  if (producer < P.numHashed) {
+    uint32_t a = S.hashedIdx[producer];
+    uint32_t ownerA = S.colliderOwner[a];
+    uint32_t grpA = S.colliderGroup[a];
+    uint32_t shA = S.colliderShared[a];
+    V3 pa = collider_center(S, a);
+    float ra = fabsf(S.colliderShape[a].w);
+    bool aDyn = S.posLin[ownerA].w > 0.0f;
+    uint32_t ga = S.clothGroup[ownerA];
+    bool scanCells = !(ga != 0 && P.numHashedRigid == 0);
+    int cx, cy, cz;
+    cell_coord(pa, P.cellSize, cx, cy, cz);
+    if (scanCells) {
+      uint32_t visited[27];
+      uint32_t nvis = 0;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            uint32_t h = cell_hash(cx + dx, cy + dy, cz + dz, grpA, P.gridHashSize);
+            bool dup = false;
+            for (uint32_t v = 0; v < nvis; v++) dup |= (visited[v] == h);
+            if (dup) continue;
+            visited[nvis++] = h;
+            if (ga != 0 && S.cellRigid[h] == 0) continue;
+            uint32_t s = S.cellStart[h], e = s + S.cellCount[h];
+            for (uint32_t k = s; k < e; k++) {
+              uint32_t b = S.cellBodies[k];
+              if (b <= a) continue;
+              if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
+              uint32_t ownerB = S.colliderOwner[b];
+              if (ownerA == ownerB) continue;
+              if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
+              if (ga != 0 && S.clothGroup[ownerB] != 0) continue;
+              float rb = fabsf(S.colliderShape[b].w);
+              V3 d = pa - collider_center(S, b);
+              float r = ra + rb;
+              if (dot(d, d) > r * r) continue;
+              uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
+              if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
+              uint32_t slot = base + count;
+              if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a, b};
+              count++;
+            }
+          }
+    }
+    for (uint32_t g = 0; g < P.numGlobals; g++) {
+      uint32_t b = S.globalIdx[g];
+      if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
+      uint32_t ownerB = S.colliderOwner[b];
+      if (ownerA == ownerB) continue;
+      if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
+      if (ga != 0 && S.clothGroup[ownerB] == ga) continue;
+      float rb = fabsf(S.colliderShape[b].w);
+      V3 d = pa - collider_center(S, b);
+      float r = ra + rb;
+      if (dot(d, d) > r * r) continue;
+      uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
+      if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
+      uint32_t slot = base + count;
+      if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a < b ? a : b, a < b ? b : a};
+      count++;
+    }
+  } else {
+    uint32_t gp = producer - P.numHashed;
+    if (gp >= P.numGlobals) return 0;
+    uint32_t a = S.globalIdx[gp];
+    uint32_t ownerA = S.colliderOwner[a];
+    uint32_t grpA = S.colliderGroup[a];
+    uint32_t shA = S.colliderShared[a];
+    V3 pa = collider_center(S, a);
+    float ra = fabsf(S.colliderShape[a].w);
+    bool aDyn = S.posLin[ownerA].w > 0.0f;
+    uint32_t ga = S.clothGroup[ownerA];
+    for (uint32_t g = gp + 1; g < P.numGlobals; g++) {
+      uint32_t b = S.globalIdx[g];
+      if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
+      uint32_t ownerB = S.colliderOwner[b];
+      if (ownerA == ownerB) continue;
+      if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
+      if (ga != 0 && S.clothGroup[ownerB] == ga) continue;
+      float rb = fabsf(S.colliderShape[b].w);
+      V3 d = pa - collider_center(S, b);
+      float r = ra + rb;
+      if (dot(d, d) > r * r) continue;
+      uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
+      if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
+      uint32_t slot = base + count;
+      if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a < b ? a : b, a < b ? b : a};
+      count++;
+    }
+  }
+  return count;
+}
+
+__global__ void __launch_bounds__(256) k_bp_count_pairs(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed + P.numGlobals) return;
+  S.pairCount[gid] = bp_producer(S, P, gid, false, 0);
+}
+__global__ void __launch_bounds__(256) k_bp_emit_pairs(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed + P.numGlobals) return;
+  (void)bp_producer(S, P, gid, true, S.pairStart[gid]);
+}
+__global__ void k_bp_finalize(DeviceState S, StepParams P) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t producers = P.numHashed + P.numGlobals;
+  uint32_t n = producers == 0 ? 0 : S.pairStart[producers - 1] + S.pairCount[producers - 1];
+  S.counters[CTR_PAIR_CANDIDATES] = n;
+  S.counters[CTR_PAIRS] = n < P.maxPairs ? n : P.maxPairs;
+}
+
+// ---------------------------------------------------------------------------
+// collider-pair keyed persistence map (20_broadphase.metal:547-608):
+// open addressing, linear probing; key A (= colliderA + 1) claimed by CAS,
+// key B and value written by the winner.  Built from this frame's
+// manifolds, queried by next frame's narrow phase.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pair_map_hash(uint32_t a, uint32_t b, uint32_t cap) {
+  uint32_t h = (a * 73856093u) ^ (b * 19349663u);
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  return h & (cap - 1u);
+}
+
+__global__ void __launch_bounds__(256) k_pm_insert(DeviceState S, StepParams P) {
+  uint32_t n = S.counters[CTR_PAIRS];
+  for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x) {
+    U4 header = S.manifolds[gid].header;
+    if (header.z == 0) continue;
+    uint32_t a = S.manifolds[gid].colliderPair.x;
+    uint32_t b = S.manifolds[gid].colliderPair.y;
+    uint32_t h = pair_map_hash(a, b, P.mapCapacity);
+    for (uint32_t probe = 0; probe < P.mapCapacity; probe++) {
+      if (atomicCAS(&S.mapKeyA[h], 0u, a + 1u) == 0u) {
+        S.mapKeyB[h] = b;
+        S.mapVal[h] = gid;
+        break;
+      }
+      h = (h + 1) & (P.mapCapacity - 1u);
+    }
+  }
+}
+
+__device__ __forceinline__ int pair_map_find(const DeviceState& S, uint32_t cap, uint32_t a, uint32_t b) {
+  uint32_t h = pair_map_hash(a, b, cap);
+  for (uint32_t probe = 0; probe < cap; probe++) {
+    uint32_t ka = S.mapKeyA[h];
+    if (ka == 0) return -1;
+    if (ka == a + 1 && S.mapKeyB[h] == b) return (int)S.mapVal[h];
+    h = (h + 1) & (cap - 1u);
+  }
+  return -1;
+}
+
+__global__ void k_map_find(DeviceState S, StepParams P, const U2* q, int32_t* out, uint32_t n) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n) return;
+  out[gid] = pair_map_find(S, P.mapCapacity, q[gid].x, q[gid].y);
+}
+
+// ---------------------------------------------------------------------------
+// narrow phase (30_narrowphase.metal:337-1322): sphere-sphere, sphere-box
+// and box-box SAT with reference-face clipping.  Torus / capsule / convex
+// hull colliders are rejected at create time.
+// ---------------------------------------------------------------------------
+constexpr int kMaxPoly = 16;
+constexpr float kSatEps = 1.0e-6f;
+constexpr float kPlaneEps = 1.0e-5f;
+constexpr float kMergeDistSq = 1.0e-6f;
+
+struct NPBox {
+  V3 c, h, ax[3];
+};
+struct NPBest {
+  int type, ia, ib;
+  float sep;
+  V3 n;
+  bool valid;
+};
+struct NPContact {
+  V3 xA, xB;
+  uint32_t feature;
+};
+
+__device__ __forceinline__ float np_half(const NPBox& b, int i) { return comp(b.h, i); }
+
+__device__ __forceinline__ V3 np_support(const NPBox& b, V3 d) {
+  float sx = dot(d, b.ax[0]) >= 0.0f ? 1.0f : -1.0f;
+  float sy = dot(d, b.ax[1]) >= 0.0f ? 1.0f : -1.0f;
+  float sz = dot(d, b.ax[2]) >= 0.0f ? 1.0f : -1.0f;
+  return b.c + b.ax[0] * (b.h.x * sx) + b.ax[1] * (b.h.y * sy) + b.ax[2] * (b.h.z * sz);
+}
+
+__device__ __forceinline__ float np_spec_cap(float ra, float rb, float margin) {
+  float r = fminf(fabsf(ra), fabsf(rb));
+  return fminf(0.25f, fmaxf(4.0f * margin, 3.0f * r));
+}
+__device__ __forceinline__ float np_detect_margin(V3 n, V3 relVel, const StepParams& P, float ra, float rb) {
+  float approach = fmaxf(0.0f, -dot(relVel, n));
+  float allowed = fminf(approach * P.dt, np_spec_cap(ra, rb, P.collisionMargin));
+  return P.collisionMargin + allowed;
+}
+__device__ __forceinline__ V3 np_penalty_floor(const DeviceState& S, uint32_t ba, uint32_t bb, const StepParams& P) {
+  float mA = S.posLin[ba].w, mB = S.posLin[bb].w;
+  float big = 3.402823466e38f;
+  float minMass = fminf(mA > 0.0f ? mA : big, mB > 0.0f ? mB : big);
+  if (minMass == big) return v3s(kPenaltyMin);
+  float k = fminf(kPenaltyMax, fmaxf(kPenaltyMin, minMass / fmaxf(P.dt * P.dt, 1e-12f)));
+  return v3(k, fminf(k, kPenaltyMaxT), fminf(k, kPenaltyMaxT));
+}
+__device__ __forceinline__ V3 clamp3(V3 p, V3 lo, V3 hi) { return vmin(vmax(p, lo), hi); }
+
+__device__ bool np_test_axis(const NPBox& A, const NPBox& B, V3 delta, V3 axis, int type, int ia, int ib, V3 relVel,
+                             float specCap, const StepParams& P, NPBest& best) {
+  float lenSq = dot(axis, axis);
+  if (lenSq < kSatEps) return true;
+  V3 n = axis * rsq(lenSq);
+  if (dot(n, delta) < 0.0f) n = -n;
+  float dist = fabsf(dot(delta, n));
+  float rA = A.h.x * fabsf(dot(n, A.ax[0])) + A.h.y * fabsf(dot(n, A.ax[1])) + A.h.z * fabsf(dot(n, A.ax[2]));
+  float rB = B.h.x * fabsf(dot(n, B.ax[0])) + B.h.y * fabsf(dot(n, B.ax[1])) + B.h.z * fabsf(dot(n, B.ax[2]));
+  float sep = dist - (rA + rB);
+  float approach = fmaxf(0.0f, -dot(relVel, n));
+  float allowed = fminf(approach * P.dt, specCap);
+  if (sep > allowed) return false;
+  if (!best.valid || sep > best.sep) {
+    best.valid = true;
+    best.type = type;
+    best.ia = ia;
+    best.ib = ib;
+    best.sep = sep;
+    best.n = n;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void np_face_axes(const NPBox& b, int k, V3& u, V3& v, float& eu, float& ev) {
+  if (k == 0) { u = b.ax[1]; v = b.ax[2]; eu = b.h.y; ev = b.h.z; }
+  else if (k == 1) { u = b.ax[0]; v = b.ax[2]; eu = b.h.x; ev = b.h.z; }
+  else { u = b.ax[0]; v = b.ax[1]; eu = b.h.x; ev = b.h.y; }
+}
+
+__device__ int np_clip(const V3* in, int inCount, V3 n, float off, V3* out) {
+  if (inCount <= 0) return 0;
+  int outCount = 0;
+  V3 a = in[inCount - 1];
+  float da = dot(n, a) - off;
+  for (int i = 0; i < inCount; i++) {
+    V3 b = in[i];
+    float db = dot(n, b) - off;
+    bool aIn = da <= kPlaneEps, bIn = db <= kPlaneEps;
+    if (aIn != bIn) {
+      float t = 0.0f, denom = da - db;
+      if (fabsf(denom) > kSatEps) t = clampf(da / denom, 0.0f, 1.0f);
+      if (outCount < kMaxPoly) out[outCount++] = a + (b - a) * t;
+    }
+    if (bIn && outCount < kMaxPoly) out[outCount++] = b;
+    a = b;
+    da = db;
+  }
+  return outCount;
+}
+
+__device__ void np_support_edge(const NPBox& b, int k, V3 dir, V3& e0, V3& e1) {
+  int a1 = (k + 1) % 3, a2 = (k + 2) % 3;
+  float s1 = dot(dir, b.ax[a1]) >= 0.0f ? 1.0f : -1.0f;
+  float s2 = dot(dir, b.ax[a2]) >= 0.0f ? 1.0f : -1.0f;
+  V3 c = b.c + b.ax[a1] * (np_half(b, a1) * s1) + b.ax[a2] * (np_half(b, a2) * s2);
+  e0 = c - b.ax[k] * np_half(b, k);
+  e1 = c + b.ax[k] * np_half(b, k);
+}
+
+__device__ void np_closest_seg_seg(V3 p0, V3 p1, V3 q0, V3 q1, V3& c0, V3& c1) {
+  V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+  float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  float s = 0.0f, t = 0.0f;
+  if (a <= kSatEps && e <= kSatEps) { c0 = p0; c1 = q0; return; }
+  if (a <= kSatEps) {
+    t = clampf(f / e, 0.0f, 1.0f);
+  } else {
+    float c = dot(d1, r);
+    if (e <= kSatEps) {
+      s = clampf(-c / a, 0.0f, 1.0f);
+    } else {
+      float b = dot(d1, d2);
+      float denom = a * e - b * b;
+      if (fabsf(denom) > kSatEps) s = clampf((b * f - c * e) / denom, 0.0f, 1.0f);
+      t = (b * s + f) / e;
+      if (t < 0.0f) { t = 0.0f; s = clampf(-c / a, 0.0f, 1.0f); }
+      else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / a, 0.0f, 1.0f); }
+    }
+  }
+  c0 = p0 + d1 * s;
+  c1 = q0 + d2 * t;
+}
+
+__device__ __forceinline__ bool np_add(NPContact* cs, int& count, V3* mids, V3 xA, V3 xB, uint32_t feature) {
+  V3 mid = (xA + xB) * 0.5f;
+  for (int i = 0; i < count; i++) {
+    V3 d = mid - mids[i];
+    if (dot(d, d) < kMergeDistSq) return false;
+  }
+  if (count >= kMaxContacts) return false;
+  cs[count].xA = xA;
+  cs[count].xB = xB;
+  cs[count].feature = feature;
+  mids[count] = mid;
+  count++;
+  return true;
+}
+
+__global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) {
+  uint32_t numPairs = S.counters[CTR_PAIRS];
+  for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < numPairs; gid += gridDim.x * blockDim.x) {
+    U2 pr = S.pairs[gid];
+    uint32_t ia = pr.x, ib = pr.y;
+    uint32_t ba = S.colliderOwner[ia], bb = S.colliderOwner[ib];
+    F4 bodyPA = S.posLin[ba], bodyPB = S.posLin[bb];
+    F4 qBodyA = S.posAng[ba], qBodyB = S.posAng[bb];
+    V3 centerA = xyz(bodyPA) + q_rotate(qBodyA, xyz(S.colliderLocalPos[ia]));
+    V3 centerB = xyz(bodyPB) + q_rotate(qBodyB, xyz(S.colliderLocalPos[ib]));
+    F4 qA = q_mul(qBodyA, S.colliderLocalRot[ia]);
+    F4 qB = q_mul(qBodyB, S.colliderLocalRot[ib]);
+    V3 vA = xyz(S.velLin[ba]) + cross(xyz(S.velAng[ba]), centerA - xyz(bodyPA));
+    V3 vB = xyz(S.velLin[bb]) + cross(xyz(S.velAng[bb]), centerB - xyz(bodyPB));
+    V3 relVel = vB - vA;
+    F4 shA = S.colliderShape[ia], shB = S.colliderShape[ib];
+
+    NPBox A, B;
+    A.c = centerA;
+    A.h = xyz(shA) * 0.5f;
+    A.ax[0] = q_rotate(qA, v3(1, 0, 0)); A.ax[1] = q_rotate(qA, v3(0, 1, 0)); A.ax[2] = q_rotate(qA, v3(0, 0, 1));
+    B.c = centerB;
+    B.h = xyz(shB) * 0.5f;
+    B.ax[0] = q_rotate(qB, v3(1, 0, 0)); B.ax[1] = q_rotate(qB, v3(0, 1, 0)); B.ax[2] = q_rotate(qB, v3(0, 0, 1));
+    V3 delta = B.c - A.c;
+
+    ManifoldRec& outM = S.manifolds[gid];
+    outM.colliderPair = U4{ia, ib, 0, 0};
+    uint32_t tA = S.colliderShapeType[ia], tB = S.colliderShapeType[ib];
+    bool sphA = (tA & kShapeKindMask) == 1, sphB = (tB & kShapeKindMask) == 1;
+    F4 fA = S.colliderFriction[ia], fB = S.colliderFriction[ib];
+    float staticFriction = combine_friction(fA.x, fB.x, P.frictionCombineMode);
+    float dynamicFriction = combine_friction(fA.y, fB.y, P.frictionCombineMode);
+    float warm = P.alpha * P.gamma;
+
+    NPContact cs[kMaxContacts];
+    int count = 0;
+    V3 nrm;  // from B toward A
+
+    if (sphA || sphB) {
+      float rA = shA.x * 0.5f, rB = shB.x * 0.5f;
+      if (sphA && sphB) {
+        V3 d = A.c - B.c;
+        float dist = length(d);
+        V3 nT = dist > 1e-9f ? d / dist : v3(0, 0, 1);
+        float detectM = np_detect_margin(nT, vA - vB, P, rA, rB);
+        if (dist > rA + rB + detectM) { outM.header = U4{ba, bb, 0, 0}; continue; }
+        nrm = nT;
+        cs[0].xA = A.c - nrm * rA;
+        cs[0].xB = B.c + nrm * rB;
+        cs[0].feature = 0;
+        count = 1;
+      } else {
+        bool sIsA = sphA;
+        float r = sIsA ? rA : rB;
+        const NPBox& box = sIsA ? B : A;
+        F4 qBox = sIsA ? qB : qA;
+        V3 sc = sIsA ? A.c : B.c;
+        V3 local = q_rotate(q_conj(qBox), sc - box.c);
+        V3 half = box.h;
+        V3 clamped = clamp3(local, -half, half);
+        V3 nLocal, qLocal;
+        if (clamped.x == local.x && clamped.y == local.y && clamped.z == local.z) {
+          V3 pen = half - vabs(local);
+          int axis = 0;
+          if (pen.y < comp(pen, axis)) axis = 1;
+          if (pen.z < comp(pen, axis)) axis = 2;
+          V3 n = v3s(0);
+          setcomp(n, axis, comp(local, axis) >= 0.0f ? 1.0f : -1.0f);
+          nLocal = n;
+          qLocal = local;
+          setcomp(qLocal, axis, comp(n, axis) * comp(half, axis));
+        } else {
+          V3 d = local - clamped;
+          float dist = length(d);
+          V3 nLT = d / fmaxf(dist, 1e-9f);
+          V3 nWT = q_rotate(qBox, nLT);
+          V3 vS = sIsA ? vA : vB, vBx = sIsA ? vB : vA;
+          float detectM = np_detect_margin(nWT, vS - vBx, P, r, sIsA ? shB.w : shA.w);
+          if (dist > r + detectM) { outM.header = U4{ba, bb, 0, 0}; continue; }
+          nLocal = nLT;
+          qLocal = clamped;
+        }
+        V3 nW = q_rotate(qBox, nLocal);
+        V3 qW = q_rotate(qBox, qLocal) + box.c;
+        V3 xS = sc - nW * r;
+        nrm = sIsA ? nW : -nW;
+        cs[0].xA = sIsA ? xS : qW;
+        cs[0].xB = sIsA ? qW : xS;
+        cs[0].feature = 0;
+        count = 1;
+      }
+      // round-shape tail: anchors stored as world offsets when flagged
+      V3 t1, t2;
+      orthonormal(nrm, t1, t2);
+      int prevIdx = pair_map_find(S, P.mapCapacity, ia, ib);
+      bool roundA = (tA & kShapeWorldRound) != 0, roundB = (tB & kShapeWorldRound) != 0;
+      uint32_t flags = 1u | (roundA ? 2u : 0u) | (roundB ? 4u : 0u);
+      outM.header = U4{ba, bb, (uint32_t)count, flags};
+      outM.basisN = f4(nrm, dynamicFriction);
+      outM.basisT1 = f4(t1, staticFriction);
+      F4 qAc = q_conj(qBodyA), qBc = q_conj(qBodyB);
+      for (int i = 0; i < count; i++) {
+        V3 rA_ = roundA ? (cs[i].xA - xyz(bodyPA)) : q_rotate(qAc, cs[i].xA - xyz(bodyPA));
+        V3 rB_ = roundB ? (cs[i].xB - xyz(bodyPB)) : q_rotate(qBc, cs[i].xB - xyz(bodyPB));
+        V3 lambda = v3s(0), penalty = v3s(0);
+        float stick = 0.0f;
+        if (prevIdx >= 0) {
+          const ManifoldRec& pm = S.prevManifolds[prevIdx];
+          uint32_t pn = pm.header.z;
+          float bestD = 0.5f * fmaxf(rA, rB);
+          int bestJ = -1;
+          for (uint32_t j = 0; j < pn; j++) {
+            float dj = length(xyz(pm.c[j].rA) - rA_);
+            if (dj < bestD) { bestD = dj; bestJ = (int)j; }
+          }
+          if (bestJ >= 0) {
+            lambda = xyz(pm.c[bestJ].lambda);
+            penalty = xyz(pm.c[bestJ].penalty);
+            V3 t1o = xyz(pm.basisT1);
+            V3 t2o = cross(xyz(pm.basisN), t1o);
+            V3 lt = t1o * lambda.y + t2o * lambda.z;
+            lambda.y = dot(t1, lt);
+            lambda.z = dot(t2, lt);
+            bool partA = shA.w < 0.0f, partB = shB.w < 0.0f;
+            if ((partA || !roundA) && (partB || !roundB)) {
+              stick = pm.c[bestJ].rB.w;
+              if (stick != 0.0f) {
+                rA_ = xyz(pm.c[bestJ].rA);
+                rB_ = xyz(pm.c[bestJ].rB);
+              }
+            }
+          }
+        }
+        V3 xAw = roundA ? xyz(bodyPA) + rA_ : xform(xyz(bodyPA), qBodyA, rA_);
+        V3 xBw = roundB ? xyz(bodyPB) + rB_ : xform(xyz(bodyPB), qBodyB, rB_);
+        V3 d = xAw - xBw;
+        V3 C0 = v3(dot(nrm, d) + P.collisionMargin, dot(t1, d), dot(t2, d));
+        lambda *= warm;
+        V3 fl = np_penalty_floor(S, ba, bb, P);
+        penalty = v3(clampf(penalty.x * P.gamma, fl.x, kPenaltyMax), clampf(penalty.y * P.gamma, fl.y, kPenaltyMaxT),
+                     clampf(penalty.z * P.gamma, fl.z, kPenaltyMaxT));
+        outM.c[i].rA = f4(rA_, as_float(cs[i].feature));
+        outM.c[i].rB = f4(rB_, stick);
+        outM.c[i].C0 = f4(C0, 0.0f);
+        outM.c[i].lambda = f4(lambda, 0.0f);
+        outM.c[i].penalty = f4(penalty, 0.0f);
+      }
+      continue;
+    }
+
+    // ---- box-box SAT ----
+    NPBest bestFace, bestEdge;
+    bestFace.valid = false; bestFace.sep = -3.402823466e38f;
+    bestEdge.valid = false; bestEdge.sep = -3.402823466e38f;
+    float cap = np_spec_cap(shA.w, shB.w, P.collisionMargin);
+    bool separated = false;
+    for (int i = 0; i < 3 && !separated; i++)
+      if (!np_test_axis(A, B, delta, A.ax[i], 0, i, -1, relVel, cap, P, bestFace)) separated = true;
+    for (int i = 0; i < 3 && !separated; i++)
+      if (!np_test_axis(A, B, delta, B.ax[i], 1, -1, i, relVel, cap, P, bestFace)) separated = true;
+    for (int i = 0; i < 3 && !separated; i++)
+      for (int j = 0; j < 3 && !separated; j++)
+        if (!np_test_axis(A, B, delta, cross(A.ax[i], B.ax[j]), 2, i, j, relVel, cap, P, bestEdge)) separated = true;
+    if (separated || !bestFace.valid) { outM.header = U4{ba, bb, 0, 0}; continue; }
+    NPBest best = bestFace;
+    if (bestEdge.valid && 0.95f * bestEdge.sep > bestFace.sep + 0.01f) best = bestEdge;
+
+    V3 mids[kMaxContacts];
+    if (best.type == 2) {
+      V3 a0, a1, b0, b1, xA, xB;
+      np_support_edge(A, best.ia, best.n, a0, a1);
+      np_support_edge(B, best.ib, -best.n, b0, b1);
+      np_closest_seg_seg(a0, a1, b0, b1, xA, xB);
+      uint32_t feature = (2u << 24) | ((uint32_t)(best.ia & 0xFF) << 8) | (uint32_t)(best.ib & 0xFF);
+      np_add(cs, count, mids, xA, xB, feature);
+      if (count == 0) np_add(cs, count, mids, np_support(A, best.n), np_support(B, -best.n), feature);
+    } else {
+      bool refIsA = best.type == 0;
+      int refAxis = refIsA ? best.ia : best.ib;
+      const NPBox& ref = refIsA ? A : B;
+      const NPBox& inc = refIsA ? B : A;
+      V3 refOut = refIsA ? best.n : -best.n;
+      float signR = dot(refOut, ref.ax[refAxis]) >= 0.0f ? 1.0f : -1.0f;
+      V3 refN = ref.ax[refAxis] * signR;
+      V3 refC = ref.c + refN * np_half(ref, refAxis);
+      V3 refU, refV;
+      float refEU, refEV;
+      np_face_axes(ref, refAxis, refU, refV, refEU, refEV);
+      int incAxis = 0;
+      float bestD = -3.402823466e38f;
+      for (int i = 0; i < 3; i++) {
+        float d = fabsf(dot(inc.ax[i], refN));
+        if (d > bestD) { bestD = d; incAxis = i; }
+      }
+      float signI = dot(inc.ax[incAxis], refN) > 0.0f ? -1.0f : 1.0f;
+      V3 incN = inc.ax[incAxis] * signI;
+      V3 incC = inc.c + incN * np_half(inc, incAxis);
+      V3 incU, incV;
+      float incEU, incEV;
+      np_face_axes(inc, incAxis, incU, incV, incEU, incEV);
+      V3 p0[kMaxPoly], p1[kMaxPoly];
+      p0[0] = incC + incU * incEU + incV * incEV;
+      p0[1] = incC - incU * incEU + incV * incEV;
+      p0[2] = incC - incU * incEU - incV * incEV;
+      p0[3] = incC + incU * incEU - incV * incEV;
+      int n = 4;
+      n = np_clip(p0, n, refU, dot(refU, refC) + refEU, p1);
+      n = np_clip(p1, n, -refU, dot(-refU, refC) + refEU, p0);
+      n = np_clip(p0, n, refV, dot(refV, refC) + refEV, p1);
+      n = np_clip(p1, n, -refV, dot(-refV, refC) + refEV, p0);
+      uint32_t prefix = ((refIsA ? 0u : 1u) << 24) | ((uint32_t)(refAxis & 0xFF) << 16) |
+                        ((uint32_t)(incAxis & 0xFF) << 8);
+      float slop = kPlaneEps + fmaxf(0.0f, best.sep);
+      for (int i = 0; i < n && count < kMaxContacts; i++) {
+        V3 pInc = p0[i];
+        float d = dot(pInc - refC, refN);
+        if (d > slop) continue;
+        V3 pRef = pInc - refN * d;
+        np_add(cs, count, mids, refIsA ? pRef : pInc, refIsA ? pInc : pRef, prefix | (uint32_t)(i & 0xFF));
+      }
+      if (count == 0) np_add(cs, count, mids, np_support(A, best.n), np_support(B, -best.n), prefix);
+    }
+    nrm = -best.n;
+    V3 t1, t2;
+    orthonormal(nrm, t1, t2);
+    int prevIdx = pair_map_find(S, P.mapCapacity, ia, ib);
+    outM.header = U4{ba, bb, (uint32_t)count, 1u};
+    outM.basisN = f4(nrm, dynamicFriction);
+    outM.basisT1 = f4(t1, staticFriction);
+    F4 qAc = q_conj(qBodyA), qBc = q_conj(qBodyB);
+    V3 fl = np_penalty_floor(S, ba, bb, P);
+    for (int i = 0; i < count; i++) {
+      V3 rA = q_rotate(qAc, cs[i].xA - xyz(bodyPA));
+      V3 rB = q_rotate(qBc, cs[i].xB - xyz(bodyPB));
+      V3 lambda = v3s(0), penalty = v3s(0);
+      float stick = 0.0f;
+      if (prevIdx >= 0) {
+        const ManifoldRec& pm = S.prevManifolds[prevIdx];
+        uint32_t pn = pm.header.z;
+        for (uint32_t j = 0; j < pn; j++) {
+          if (as_uint(pm.c[j].rA.w) == cs[i].feature) {
+            lambda = xyz(pm.c[j].lambda);
+            penalty = xyz(pm.c[j].penalty);
+            stick = pm.c[j].rB.w;
+            if (stick != 0.0f) {
+              rA = xyz(pm.c[j].rA);
+              rB = xyz(pm.c[j].rB);
+            }
+            break;
+          }
+        }
+      }
+      V3 xAw = xform(xyz(bodyPA), qBodyA, rA);
+      V3 xBw = xform(xyz(bodyPB), qBodyB, rB);
+      V3 d = xAw - xBw;
+      V3 C0 = v3(dot(nrm, d) + P.collisionMargin, dot(t1, d), dot(t2, d));
+      lambda *= warm;
+      penalty = v3(clampf(penalty.x * P.gamma, fl.x, kPenaltyMax), clampf(penalty.y * P.gamma, fl.y, kPenaltyMaxT),
+                   clampf(penalty.z * P.gamma, fl.z, kPenaltyMaxT));
+      outM.c[i].rA = f4(rA, as_float(cs[i].feature));
+      outM.c[i].rB = f4(rB, stick);
+      outM.c[i].C0 = f4(C0, 0.0f);
+      outM.c[i].lambda = f4(lambda, 0.0f);
+      outM.c[i].penalty = f4(penalty, 0.0f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t blocks_for(uint32_t n, uint32_t tpb) { return (n + tpb - 1) / tpb; }
+
+void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (P.numHashed > 0) {
+    hipLaunchKernelGGL(k_bp_count, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
+  }
+  launch_exclusive_scan(st, S.cellCount, S.cellStart, P.gridHashSize, S.scanScratch);
+  if (P.numHashed > 0) {
+    hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
+  }
+  hipLaunchKernelGGL(k_bp_sort_cells, dim3(blocks_for(P.gridHashSize, 256)), dim3(256), 0, st, S, P);
+  uint32_t producers = P.numHashed + P.numGlobals;
+  if (producers > 0) {
+    hipLaunchKernelGGL(k_bp_count_pairs, dim3(blocks_for(producers, 256)), dim3(256), 0, st, S, P);
+    launch_exclusive_scan(st, S.pairCount, S.pairStart, producers, S.scanScratch);
+    hipLaunchKernelGGL(k_bp_emit_pairs, dim3(blocks_for(producers, 256)), dim3(256), 0, st, S, P);
+  }
+  hipLaunchKernelGGL(k_bp_finalize, dim3(1), dim3(64), 0, st, S, P);
+}
+
+void launch_narrowphase(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  uint32_t blocks = blocks_for(P.maxPairs, 64);
+  if (blocks > gridCap) blocks = gridCap;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_np_collide, dim3(blocks), dim3(64), 0, st, S, P);
+}
+
+void launch_map_rebuild(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  launch_fill_u32(st, S.mapKeyA, P.mapCapacity, 0u);
+  uint32_t blocks = blocks_for(P.maxPairs, 256);
+  if (blocks > gridCap) blocks = gridCap;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_pm_insert, dim3(blocks), dim3(256), 0, st, S, P);
+}
+
+void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, const U2* q, int32_t* out,
+                     uint32_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_map_find, dim3(blocks_for(n, 256)), dim3(256), 0, st, S, P, q, out, n);
+}
+
+}  // namespace avbd

@@ -1,0 +1,1562 @@
+// solver.hip -- host runtime of the gfx950 AVBD backend: scene upload and
+// precompute, device buffers, per-step orchestration (captured in hipGraphs),
+// the lane-split / sweep-mode autotuner, failure latching and readback.
+//
+// Follows GPUSolver (Sources/PhysicsAVBD/GPU/GPUSolver.swift): init :338,
+// upload :901-2081, encodeAndSubmitStep :3138-4405, retire :2829,
+// synchronize :2924, setBodyStates :2409.
+#include <math.h>
+#include <string.h>
+#include <stdio.h>
+#include <algorithm>
+#include <array>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd.h"
+#include "kernels.hpp"
+
+// upload-time geometry must round exactly as written (radii feed the
+// hashed/global partition and the cell size)
+#pragma clang fp contract(off)
+
+using namespace avbd;
+
+namespace {
+
+uint32_t next_pow2(uint32_t v) {
+  uint32_t n = 1;
+  while (n < v) n <<= 1;
+  return n;
+}
+
+struct HostBody {
+  float mass;
+  V3 moment;
+  float radius;
+};
+
+}  // namespace
+
+struct avbd_solver {
+  avbd_settings settings{};
+  avbd_options opts{};
+  uint32_t nb = 0, nc = 0, nj = 0, ns = 0, nt = 0;
+  uint32_t nHashed = 0, nGlobal = 0, nHashedRigid = 0, nExcl = 0;
+  uint32_t maxPairs = 0, mapCapacity = 0, gridHashSize = 0;
+  float cellSize = 0.1f;
+  bool dynamicColoring = false;
+  uint32_t staticUsedColors = 1;
+  std::vector<uint32_t> staticColorCounts;  // per color
+  int lanes = 8, sweepMode = 1;
+  int persistentBlocks = 0;
+  uint32_t gridCap = 1024;
+  hipStream_t stream = nullptr;
+  DeviceState S{};
+  std::vector<void*> allocs;
+  // host mirrors
+  std::vector<float> mass;
+  std::vector<float> inertia;  // n*3
+  std::vector<uint8_t> isParticle;
+  std::vector<uint32_t> sAdjCount;
+  std::vector<uint32_t> sAdjListHost;
+  std::vector<U4> jointHeaders;
+  // frames in flight
+  uint32_t* pinnedCounters = nullptr;  // [2][CTR_TOTAL]
+  hipEvent_t frameDone[2] = {nullptr, nullptr};
+  int64_t slotFrame[2] = {-1, -1};
+  bool slotBusy[2] = {false, false};
+  int64_t frameIndex = 0;
+  // graphs (even / odd frame: manifold buffers alternate)
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  bool graphsValid = false;
+  bool graphsDisabled = false;
+  int64_t graphBaseFrame = 1;  // frame that replays graphs[0]
+  // failure latch
+  avbd_failure failure{};
+  bool failed = false;
+  // stats
+  avbd_stats stats{};
+  // test hook
+  std::string deniedStage;
+  int64_t deniedFrame = -1;
+  // profiling
+  bool profiling = false;
+  std::vector<std::string> profNames;
+  std::vector<float> profMs;
+  // scratch for queries
+  uint32_t* devScratchU32 = nullptr;
+  uint32_t devScratchWords = 0;
+  float* residualPartialHost = nullptr;
+  uint32_t residualBlocks = 0;
+};
+
+namespace {
+
+avbd_status latch(avbd_solver* s, int kind, int64_t frame, int64_t required, int64_t capacity, int hipErr,
+                  const char* msg) {
+  if (!s->failed) {
+    s->failed = true;
+    s->failure.kind = kind;
+    s->failure.frame = frame;
+    s->failure.required = required;
+    s->failure.capacity = capacity;
+    s->failure.hip_error = hipErr;
+    snprintf(s->failure.message, sizeof(s->failure.message), "%s", msg);
+  }
+  return AVBD_ERR_RUNTIME_FAILURE;
+}
+
+#define HIP_OR_LATCH(s, expr, frame)                                                              \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) {                                                                       \
+      char _m[160];                                                                               \
+      snprintf(_m, sizeof(_m), "%s: %s", #expr, hipGetErrorString(_e));                           \
+      return latch(s, AVBD_FAIL_COMMAND_EXECUTION, frame, 0, 0, (int)_e, _m);                     \
+    }                                                                                             \
+  } while (0)
+
+template <typename T>
+hipError_t dev_alloc(avbd_solver* s, T** p, size_t count) {
+  size_t bytes = count * sizeof(T);
+  if (bytes < 16) bytes = 16;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return e;
+  e = hipMemset(q, 0, bytes);
+  s->allocs.push_back(q);
+  *p = (T*)q;
+  return e;
+}
+
+template <typename T>
+hipError_t upload(T* dst, const std::vector<T>& src) {
+  if (src.empty()) return hipSuccess;
+  return hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+StepParams make_params(const avbd_solver* s) {
+  StepParams P{};
+  P.dt = s->settings.dt;
+  P.gravity = s->settings.gravity;
+  P.alpha = s->settings.alpha;
+  P.betaLin = s->settings.beta_lin;
+  P.betaAng = s->settings.beta_ang;
+  P.gamma = s->settings.gamma;
+  P.lambdaMax = s->settings.lambda_max;
+  P.cellSize = s->cellSize;
+  P.maxSpeed = fmaxf(30.0f, 1.5f * s->cellSize / s->settings.dt * 0.5f);
+  P.collisionMargin = fmaxf(s->settings.collision_margin, 0.0f);
+  P.particleDamping = s->settings.particle_damping;
+  P.rigidLinearDamping = fmaxf(s->settings.rigid_linear_damping, 0.0f);
+  P.rigidAngularDamping = fmaxf(s->settings.rigid_angular_damping, 0.0f);
+  P.numBodies = s->nb;
+  P.numColliders = s->nc;
+  P.numJoints = s->nj;
+  P.numSprings = s->ns;
+  P.numTets = s->nt;
+  P.mapCapacity = s->mapCapacity;
+  P.maxPairs = s->maxPairs;
+  P.gridHashSize = s->gridHashSize;
+  P.numHashed = s->nHashed;
+  P.numGlobals = s->nGlobal;
+  P.numHashedRigid = s->nHashedRigid;
+  P.frictionCombineMode = s->settings.friction_combine_mode;
+  P.numExclusions = s->nExcl;
+  P.iterations = (uint32_t)std::max(0, s->settings.iterations);
+  return P;
+}
+
+bool stage_denied(const avbd_solver* s, const char* stage, int64_t frame) {
+  return !s->deniedStage.empty() && s->deniedStage == stage && (s->deniedFrame < 0 || s->deniedFrame == frame);
+}
+
+// Enqueue one frame's kernels on the stream (eagerly or under capture).
+hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S) {
+  hipStream_t st = s->stream;
+  StepParams P = make_params(s);
+  launch_fill_u32(st, S.counters, CTR_TOTAL, 0u);
+  launch_fill_u32(st, S.cellCount, s->gridHashSize, 0u);
+  launch_fill_u32(st, S.cellRigid, s->gridHashSize, 0u);
+  launch_broadphase(st, S, P);
+  launch_narrowphase(st, S, P, s->gridCap);
+  launch_map_rebuild(st, S, P, s->gridCap);
+  launch_warmstart(st, S, P);
+  launch_contact_adjacency(st, S, P, s->gridCap);
+  if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
+  if (s->sweepMode == 2) {
+    hipError_t e = launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors,
+                                           s->lanes, (uint32_t)s->persistentBlocks, 0u);
+    if (e != hipSuccess) return e;
+  } else {
+    for (uint32_t it = 0; it < P.iterations; it++) {
+      if (s->dynamicColoring) {
+        for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+          launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes);
+      } else {
+        for (uint32_t c = 0; c < s->staticUsedColors; c++)
+          launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes);
+      }
+      launch_dual(st, S, P, s->gridCap);
+    }
+  }
+  launch_finalize(st, S, P);
+  return hipGetLastError();
+}
+
+void invalidate_graphs(avbd_solver* s) {
+  for (int i = 0; i < 2; i++) {
+    if (s->graphs[i]) hipGraphExecDestroy(s->graphs[i]);
+    s->graphs[i] = nullptr;
+  }
+  s->graphsValid = false;
+}
+
+hipError_t build_graphs(avbd_solver* s) {
+  invalidate_graphs(s);
+  DeviceState S = s->S;
+  for (int p = 0; p < 2; p++) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t eq = enqueue_frame(s, S);
+    e = hipStreamEndCapture(s->stream, &g);
+    if (eq != hipSuccess) { if (g) hipGraphDestroy(g); return eq; }
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&s->graphs[p], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+    std::swap(S.manifolds, S.prevManifolds);
+  }
+  s->graphsValid = true;
+  s->graphBaseFrame = s->frameIndex + 1;
+  return hipSuccess;
+}
+
+avbd_status retire_slot(avbd_solver* s, int slot) {
+  if (!s->slotBusy[slot]) return AVBD_OK;
+  s->slotBusy[slot] = false;
+  int64_t frame = s->slotFrame[slot];
+  hipError_t e = hipEventSynchronize(s->frameDone[slot]);
+  if (e != hipSuccess) {
+    char m[160];
+    snprintf(m, sizeof(m), "frame %lld did not complete: %s", (long long)frame, hipGetErrorString(e));
+    return latch(s, AVBD_FAIL_COMMAND_EXECUTION, frame, 0, 0, (int)e, m);
+  }
+  const uint32_t* c = s->pinnedCounters + slot * CTR_TOTAL;
+  s->stats.frame_index = frame;
+  s->stats.pair_candidates = (int32_t)c[CTR_PAIR_CANDIDATES];
+  s->stats.num_pairs = (int32_t)c[CTR_PAIRS];
+  s->stats.num_soft = (int32_t)c[CTR_SOFT];
+  s->stats.soft_candidates = (int32_t)c[CTR_SOFT_CANDIDATES];
+  s->stats.num_active_manifolds = (int32_t)c[CTR_ACTIVE_MANIFOLDS];
+  s->stats.color_conflicts = (int32_t)c[CTR_COLOR_CONFLICTS];
+  if (s->dynamicColoring) {
+    int maxUsed = -1;
+    for (int k = 0; k < kMaxColors; k++) {
+      s->stats.color_counts[k] = (int32_t)c[CTR_COLOR_BASE + k];
+      if (c[CTR_COLOR_BASE + k] > 0) maxUsed = k;
+    }
+    s->stats.max_color_used = maxUsed;
+    s->stats.num_colors = maxUsed + 1;
+  }
+  if (c[CTR_GBAR_FAIL] != 0) {
+    return latch(s, AVBD_FAIL_BARRIER_TIMEOUT, frame, (int64_t)c[CTR_GBAR_FAIL], 0, 0,
+                 "persistent sweep grid barrier exceeded its spin bound");
+  }
+  if ((int64_t)c[CTR_PAIR_CANDIDATES] > (int64_t)s->maxPairs) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: rigid-pair demand %u exceeds capacity %u", (long long)frame,
+             c[CTR_PAIR_CANDIDATES], s->maxPairs);
+    return latch(s, AVBD_FAIL_RIGID_PAIR_CAPACITY, frame, c[CTR_PAIR_CANDIDATES], s->maxPairs, 0, m);
+  }
+  if (c[CTR_COLOR_CONFLICTS] > 0) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: dynamic coloring retained %u conflicting bodies", (long long)frame,
+             c[CTR_COLOR_CONFLICTS]);
+    return latch(s, AVBD_FAIL_UNRESOLVED_COLORING, frame, c[CTR_COLOR_CONFLICTS], 0, 0, m);
+  }
+  return AVBD_OK;
+}
+
+avbd_status sync_all(avbd_solver* s) {
+  avbd_status first = AVBD_OK;
+  // retire in frame order
+  int order[2] = {0, 1};
+  if (s->slotBusy[0] && s->slotBusy[1] && s->slotFrame[1] < s->slotFrame[0]) { order[0] = 1; order[1] = 0; }
+  for (int i = 0; i < 2; i++) {
+    avbd_status st = retire_slot(s, order[i]);
+    if (st != AVBD_OK && first == AVBD_OK) first = st;
+  }
+  if (first != AVBD_OK) return first;
+  return s->failed ? AVBD_ERR_RUNTIME_FAILURE : AVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// upload precompute (GPUSolver.swift:901-2081)
+// ---------------------------------------------------------------------------
+V3 rd3(const float* p, uint32_t i) { return v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+F4 rd4(const float* p, uint32_t i) { return F4{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; }
+
+bool body_is_dynamic(const avbd_scene_desc* d, uint32_t i) {
+  if (d->body_mass && !isnan(d->body_mass[i])) return d->body_mass[i] > 0.0f;
+  return d->body_density[i] > 0.0f;
+}
+
+// Pre-validates everything this backend cannot run.
+avbd_status validate(const avbd_scene_desc* d) {
+  if (!d || d->abi_version != AVBD_ABI_VERSION) return AVBD_ERR_INVALID_ARGUMENT;
+  if (d->num_bodies == 0) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!d->body_size || !d->body_density || !d->body_friction || !d->body_position || !d->body_shape)
+    return AVBD_ERR_INVALID_ARGUMENT;
+  if (!(d->settings.dt > 0.0f) || d->settings.iterations < 0) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!(d->settings.collision_margin >= 0.0f) || !isfinite(d->settings.collision_margin))
+    return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < d->num_bodies; i++)
+    if (d->body_shape[i] > AVBD_SHAPE_CAPSULE) return AVBD_ERR_INVALID_ARGUMENT;
+  if (d->num_colliders && (!d->collider_body || !d->collider_size || !d->collider_friction || !d->collider_shape))
+    return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < d->num_colliders; i++) {
+    if (d->collider_body[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+    uint32_t sh = d->collider_shape[i];
+    bool enabled = !d->collider_enabled || d->collider_enabled[i];
+    if (enabled && sh != AVBD_SHAPE_BOX && sh != AVBD_SHAPE_SPHERE) return AVBD_ERR_UNSUPPORTED;
+  }
+  for (uint32_t i = 0; i < d->num_joints; i++) {
+    if (d->joint_body_b[i] < 0 || (uint32_t)d->joint_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+    if (d->joint_body_a[i] >= 0 && (uint32_t)d->joint_body_a[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  }
+  for (uint32_t i = 0; i < d->num_springs; i++)
+    if (d->spring_body_a[i] >= d->num_bodies || d->spring_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < d->num_tets * 4; i++)
+    if (d->tet_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  if (d->num_tris > 0) return AVBD_ERR_UNSUPPORTED;  // cloth elements are not in this backend yet
+  for (uint32_t i = 0; i < d->num_exclusions * 2; i++)
+    if (d->exclusion_pairs[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  return AVBD_OK;
+}
+
+// Boundary faces of the tet mesh (faces used by exactly one tet),
+// GPUSolver.swift:875-897; only their vertex membership matters here.
+std::vector<uint8_t> tet_surface_flags(const avbd_scene_desc* d) {
+  std::map<std::array<uint32_t, 3>, int> faces;
+  static const int fi[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  for (uint32_t t = 0; t < d->num_tets; t++) {
+    const uint32_t* ids = d->tet_ids + 4 * t;
+    for (int f = 0; f < 4; f++) {
+      std::array<uint32_t, 3> k{ids[fi[f][0]], ids[fi[f][1]], ids[fi[f][2]]};
+      std::sort(k.begin(), k.end());
+      faces[k]++;
+    }
+  }
+  std::vector<uint8_t> flag(d->num_bodies, 0);
+  for (auto& kv : faces)
+    if (kv.second == 1)
+      for (uint32_t v : kv.first) flag[v] = 1;
+  return flag;
+}
+
+}  // namespace
+
+extern "C" {
+
+void avbd_settings_default(avbd_settings* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->dt = 1.0f / 60.0f;
+  o->gravity = -10.0f;
+  o->iterations = 10;
+  o->collision_margin = 0.01f;
+  o->friction_combine_mode = 0;
+  o->alpha = 0.99f;
+  o->beta_lin = 10000.0f;
+  o->beta_ang = 100.0f;
+  o->gamma = 0.999f;
+  o->lambda_max = 1.0e6f;
+}
+
+void avbd_options_default(avbd_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->max_pairs_per_body = 16;
+  o->lanes_per_body = 0;
+  o->sweep_mode = 0;
+  o->use_graph = 1;
+  o->pair_capacity_override = 0;
+}
+
+const char* avbd_version(void) { return "avbd-mi355x 0.1 (gfx950, wave64)"; }
+
+}  // extern "C"
+
+namespace {
+
+avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d);
+avbd_status static_coloring(avbd_solver* s);
+void autotune(avbd_solver* s);
+
+}  // namespace
+
+extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options* opts, avbd_solver** out) {
+  if (!out) return AVBD_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  avbd_status v = validate(d);
+  if (v != AVBD_OK) return v;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AVBD_ERR_NO_DEVICE;
+  avbd_solver* s = new avbd_solver();
+  if (opts) s->opts = *opts;
+  else avbd_options_default(&s->opts);
+  if (s->opts.max_pairs_per_body <= 0) s->opts.max_pairs_per_body = 16;
+  s->settings = d->settings;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return AVBD_ERR_NO_DEVICE;
+  }
+  avbd_status st = build_static_and_upload(s, d);
+  if (st != AVBD_OK) {
+    avbd_destroy(s);
+    return st;
+  }
+  *out = s;
+  st = static_coloring(s);
+  if (st == AVBD_ERR_RUNTIME_FAILURE) return st;  // failure latched; caller reads it then destroys
+  if (st != AVBD_OK) {
+    avbd_destroy(s);
+    *out = nullptr;
+    return st;
+  }
+  autotune(s);
+  s->stats.lanes_per_body = s->lanes;
+  s->stats.sweep_mode = s->sweepMode;
+  s->stats.uses_dynamic_coloring = s->dynamicColoring ? 1 : 0;
+  return AVBD_OK;
+}
+
+extern "C" void avbd_destroy(avbd_solver* s) {
+  if (!s) return;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  invalidate_graphs(s);
+  for (void* p : s->allocs) hipFree(p);
+  for (int i = 0; i < 2; i++)
+    if (s->frameDone[i]) hipEventDestroy(s->frameDone[i]);
+  if (s->pinnedCounters) hipHostFree(s->pinnedCounters);
+  if (s->residualPartialHost) hipHostFree(s->residualPartialHost);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+namespace {
+
+avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
+  const uint32_t nb = d->num_bodies;
+  s->nb = nb;
+  s->nc = d->num_colliders;
+  s->nj = d->num_joints;
+  s->ns = d->num_springs;
+  s->nt = d->num_tets;
+  s->dynamicColoring = d->num_tets == 0 && d->num_tris == 0;
+
+  // ---- bodies (GPUSolver.swift:914-969)
+  std::vector<F4> pl(nb), pa(nb), vl(nb), va(nb), pv(nb), pr(nb), sh(nb);
+  std::vector<float> gs(nb);
+  s->mass.resize(nb);
+  s->inertia.resize(3 * nb);
+  s->isParticle.assign(nb, 0);
+  const float kPi = 3.14159265358979f;
+  for (uint32_t i = 0; i < nb; i++) {
+    V3 size = rd3(d->body_size, i);
+    uint32_t shape = d->body_shape[i];
+    if (shape == AVBD_SHAPE_SPHERE) size = v3s(size.x);
+    float density = d->body_density[i];
+    float m = 0.0f, rad = 0.0f;
+    V3 mom = v3s(0);
+    if (shape == AVBD_SHAPE_BOX) {
+      m = density > 0.0f ? size.x * size.y * size.z * density : 0.0f;
+      mom = v3((size.y * size.y + size.z * size.z) / 12.0f * m, (size.x * size.x + size.z * size.z) / 12.0f * m,
+               (size.x * size.x + size.y * size.y) / 12.0f * m);
+      V3 h = size * 0.5f;
+      rad = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z);
+    } else if (shape == AVBD_SHAPE_SPHERE) {
+      float r = size.x / 2.0f;
+      m = density > 0.0f ? 4.0f / 3.0f * kPi * r * r * r * density : 0.0f;
+      mom = v3s(0.4f * m * r * r);
+      rad = r;
+    } else if (shape == AVBD_SHAPE_TORUS) {
+      float R = size.x, r = size.y;
+      m = density > 0.0f ? 2.0f * kPi * kPi * R * r * r * density : 0.0f;
+      float iDia = m * (R * R / 2.0f + 5.0f * r * r / 8.0f), iAxis = m * (R * R + 3.0f * r * r / 4.0f);
+      mom = v3(iDia, iDia, iAxis);
+      rad = R + r;
+    } else {
+      float L = size.x, r = size.y;
+      m = density > 0.0f ? kPi * r * r * (L + 4.0f * r / 3.0f) * density : 0.0f;
+      float iAxis = 0.5f * m * r * r, iPerp = m * (L * L / 12.0f + r * r / 4.0f);
+      mom = v3(iPerp, iPerp, iAxis);
+      rad = L / 2.0f + r;
+    }
+    if (d->body_mass && !isnan(d->body_mass[i])) {
+      m = d->body_mass[i];
+      mom = d->body_inertia ? rd3(d->body_inertia, i) : v3s(0);
+    }
+    bool particle = d->body_is_particle && d->body_is_particle[i];
+    s->isParticle[i] = particle ? 1 : 0;
+    V3 pos = rd3(d->body_position, i);
+    F4 rot = d->body_rotation ? rd4(d->body_rotation, i) : F4{0, 0, 0, 1};
+    V3 vel = d->body_velocity ? rd3(d->body_velocity, i) : v3s(0);
+    pl[i] = f4(pos, m);
+    pa[i] = rot;
+    vl[i] = f4(vel, 0);
+    va[i] = F4{0, 0, 0, 0};
+    pv[i] = f4(vel, 0);
+    float fr = d->body_friction[i];
+    pr[i] = f4(mom, fr);
+    sh[i] = f4(size, particle ? -rad : rad);
+    gs[i] = d->body_gravity_scale ? d->body_gravity_scale[i] : 1.0f;
+    s->mass[i] = m;
+    s->inertia[3 * i] = mom.x;
+    s->inertia[3 * i + 1] = mom.y;
+    s->inertia[3 * i + 2] = mom.z;
+  }
+
+  // ---- colliders (GPUSolver.swift:971-1090)
+  const uint32_t nc = d->num_colliders;
+  std::vector<uint32_t> co(nc), ct(nc), cg(nc), csc(nc);
+  std::vector<F4> cs(nc), cp(nc), cq(nc), cf(nc);
+  std::vector<float> radii;
+  std::vector<float> crad(nc);
+  uint32_t enabledCount = 0;
+  bool hasIsolated = false;
+  for (uint32_t i = 0; i < nc; i++) {
+    uint32_t b = d->collider_body[i];
+    uint32_t shape = d->collider_shape[i];
+    V3 size = rd3(d->collider_size, i);
+    if (shape == AVBD_SHAPE_SPHERE) size = v3s(size.x);
+    float r;
+    if (shape == AVBD_SHAPE_SPHERE) r = size.x / 2.0f;
+    else if (shape == AVBD_SHAPE_TORUS) r = size.x + size.y;
+    else if (shape == AVBD_SHAPE_CAPSULE) r = size.x / 2.0f + size.y;
+    else { V3 h = size * 0.5f; r = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z); }
+    crad[i] = r;
+    bool particle = d->body_is_particle && d->body_is_particle[b];
+    bool enabled = !d->collider_enabled || d->collider_enabled[i];
+    float fr = d->collider_friction[i];
+    float dfr = d->collider_dynamic_friction ? d->collider_dynamic_friction[i] : fr;
+    co[i] = b;
+    cg[i] = d->collider_group ? d->collider_group[i] : 0u;
+    csc[i] = (!d->collider_shared || d->collider_shared[i]) ? 1u : 0u;
+    cs[i] = f4(size, particle ? -r : r);
+    cp[i] = f4(d->collider_local_position ? rd3(d->collider_local_position, i) : v3s(0), fr);
+    F4 lq = d->collider_local_rotation ? rd4(d->collider_local_rotation, i) : F4{0, 0, 0, 1};
+    float qn = sqrtf(lq.x * lq.x + lq.y * lq.y + lq.z * lq.z + lq.w * lq.w);
+    cq[i] = F4{lq.x / qn, lq.y / qn, lq.z / qn, lq.w / qn};
+    cf[i] = F4{fr, dfr, 0, 0};
+    uint32_t flags = shape;
+    if (particle) flags |= kShapeParticle;
+    if (d->collider_world_round_anchor && d->collider_world_round_anchor[i]) flags |= kShapeWorldRound;
+    ct[i] = flags;
+    if (enabled) {
+      enabledCount++;
+      if (cg[i] != 0) hasIsolated = true;
+      if (body_is_dynamic(d, b)) radii.push_back(r);
+    }
+  }
+  std::sort(radii.begin(), radii.end());
+  float median = radii.empty() ? 0.5f : radii[radii.size() / 2];
+  float threshold = median * 4.0f;
+  std::vector<uint32_t> hashed, globals;
+  for (uint32_t i = 0; i < nc; i++) {
+    bool enabled = !d->collider_enabled || d->collider_enabled[i];
+    if (!enabled) continue;
+    if (crad[i] > threshold || (hasIsolated && cg[i] == 0)) globals.push_back(i);
+    else hashed.push_back(i);
+  }
+  float maxHashedR = 0.05f;
+  for (uint32_t i : hashed) maxHashedR = fmaxf(maxHashedR, fabsf(cs[i].w));
+  s->cellSize = maxHashedR * 2.0f;
+  s->nHashed = (uint32_t)hashed.size();
+  s->nGlobal = (uint32_t)globals.size();
+  std::vector<uint32_t> hashedRigid;
+  for (uint32_t i : hashed)
+    if (!(d->body_is_particle && d->body_is_particle[co[i]])) hashedRigid.push_back(i);
+  s->nHashedRigid = (uint32_t)hashedRigid.size();
+  s->maxPairs = std::max<uint32_t>(4096u, enabledCount * (uint32_t)s->opts.max_pairs_per_body);
+  if (s->opts.pair_capacity_override > 0) s->maxPairs = (uint32_t)s->opts.pair_capacity_override;
+  s->mapCapacity = next_pow2(2 * s->maxPairs);
+  s->gridHashSize = next_pow2(std::max<uint32_t>(64u, 2u * std::max<uint32_t>(1u, enabledCount)));
+
+  // ---- joints (GPUSolver.swift:1092-1193)
+  const uint32_t nj = d->num_joints;
+  std::vector<JointRec> jr(nj);
+  s->jointHeaders.resize(nj);
+  for (uint32_t i = 0; i < nj; i++) {
+    JointRec g;
+    memset(&g, 0, sizeof(g));
+    int32_t a = d->joint_body_a[i], b = d->joint_body_b[i];
+    float kl = d->joint_stiffness_lin[i], ka = d->joint_stiffness_ang[i];
+    float fracture = d->joint_fracture ? d->joint_fracture[i] : INFINITY;
+    uint32_t flags = 0;
+    if (isinf(kl)) flags |= 1;
+    if (isinf(ka)) flags |= 2;
+    if (isfinite(fracture)) flags |= 4;
+    if (d->joint_fracture_linear && d->joint_fracture_linear[i]) flags |= 8;
+    g.header = U4{a >= 0 ? (uint32_t)a : kWorld, (uint32_t)b, 0, flags};
+    const float bigK = 3.0e10f;
+    g.rA = f4(rd3(d->joint_ra, i), fminf(kl, bigK));
+    g.rB = f4(rd3(d->joint_rb, i), fminf(ka, bigK));
+    V3 sizeA = a >= 0 ? rd3(d->body_size, (uint32_t)a) : v3s(0);
+    V3 sz = sizeA + rd3(d->body_size, (uint32_t)b);
+    g.C0Lin = F4{0, 0, 0, dot(sz, sz)};
+    g.C0Ang = F4{0, 0, 0, fminf(fracture, 3.0e18f)};
+    F4 qA0 = (a >= 0 && d->body_rotation) ? rd4(d->body_rotation, (uint32_t)a) : F4{0, 0, 0, 1};
+    F4 qB0 = d->body_rotation ? rd4(d->body_rotation, (uint32_t)b) : F4{0, 0, 0, 1};
+    F4 rel = q_mul(q_inv(qA0), qB0);
+    float rn = sqrtf(rel.x * rel.x + rel.y * rel.y + rel.z * rel.z + rel.w * rel.w);
+    g.restRel = F4{rel.x / rn, rel.y / rn, rel.z / rn, rel.w / rn};
+    bool hinge = d->joint_hinge_axis && !isnan(d->joint_hinge_axis[3 * i]);
+    const float* mo = d->joint_motor ? d->joint_motor + 6 * i : nullptr;
+    float motorTorque = mo ? mo[1] : 0.0f;
+    float lo = d->joint_limits ? d->joint_limits[2 * i] : 1.0f;
+    float hi = d->joint_limits ? d->joint_limits[2 * i + 1] : -1.0f;
+    if (hinge) {
+      V3 ax = normalize(rd3(d->joint_hinge_axis, i));
+      g.hingeAxis = f4(ax, 1.0f);
+      g.dynamics.x = mo ? mo[4] : 0.0f;
+      if (motorTorque > 0.0f) {
+        g.motor = F4{mo[0], mo[1], 0.0f, mo[2]};
+        g.limits = F4{lo, hi, mo[3], 0.0f};
+        int mode = (int)mo[5];
+        g.header.w |= (mode == 1 ? kMotorExplicitPD : (mode == 2 ? kMotorVelocity : kMotorImplicitPD));
+      }
+      if (lo < hi && motorTorque == 0.0f) g.limits = F4{lo, hi, 0.0f, 0.0f};
+    } else if (motorTorque > 0.0f) {
+      return AVBD_ERR_INVALID_ARGUMENT;  // a powered joint must define a hinge axis
+    }
+    float mA = a >= 0 ? pl[a].w : 0.0f, mB = pl[b].w;
+    float eff = 0.0f;
+    bool any = false;
+    if (mA > 0.0f) { eff = mA; any = true; }
+    if (mB > 0.0f) { eff = any ? fminf(eff, mB) : mB; any = true; }
+    float floorK = fminf(1.0e9f, fmaxf(1.0f, eff / fmaxf(s->settings.dt * s->settings.dt, 1.0e-12f)));
+    if (isinf(kl)) g.penaltyLin = F4{floorK, floorK, floorK, floorK};
+    if (isinf(ka)) g.penaltyAng = F4{floorK, floorK, floorK, floorK};
+    if (kl > 0.0f && isfinite(kl)) { float k = fminf(kl, 1e9f); g.penaltyLin = F4{k, k, k, k}; }
+    if (ka > 0.0f && isfinite(ka)) { float k = fminf(ka, 1e9f); g.penaltyAng = F4{k, k, k, k}; }
+    jr[i] = g;
+    s->jointHeaders[i] = g.header;
+  }
+
+  // ---- springs (GPUSolver.swift:1195-1212)
+  const uint32_t ns = d->num_springs;
+  std::vector<SpringRec> sr(ns);
+  for (uint32_t i = 0; i < ns; i++) {
+    SpringRec g;
+    memset(&g, 0, sizeof(g));
+    uint32_t a = d->spring_body_a[i], b = d->spring_body_b[i];
+    bool hard = d->spring_hard && d->spring_hard[i];
+    g.header = U4{a, b, hard ? 1u : 0u, 0};
+    float rest = d->spring_rest[i];
+    V3 rA = rd3(d->spring_ra, i), rB = rd3(d->spring_rb, i);
+    if (rest < 0.0f) {
+      V3 pA = xyz(pl[a]) + q_rotate(pa[a], rA), pB = xyz(pl[b]) + q_rotate(pa[b], rB);
+      rest = length(pA - pB);
+    }
+    g.rA = f4(rA, d->spring_stiffness[i]);
+    g.rB = f4(rB, rest);
+    sr[i] = g;
+  }
+
+  // ---- tets (GPUSolver.swift:1214-1236): Dm^-1 rows, 6V, mu V, lambda V
+  const uint32_t nt = d->num_tets;
+  std::vector<TetRec> tr(nt);
+  for (uint32_t t = 0; t < nt; t++) {
+    const uint32_t* id = d->tet_ids + 4 * t;
+    double x0[3], D[3][3];  // D columns d0 d1 d2
+    for (int k = 0; k < 3; k++) x0[k] = d->body_position[3 * id[0] + k];
+    for (int c = 0; c < 3; c++)
+      for (int k = 0; k < 3; k++) D[k][c] = (double)d->body_position[3 * id[c + 1] + k] - x0[k];
+    double det = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) - D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                 D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+    double inv[3][3];
+    inv[0][0] = (D[1][1] * D[2][2] - D[1][2] * D[2][1]) / det;
+    inv[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) / det;
+    inv[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) / det;
+    inv[1][0] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) / det;
+    inv[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det;
+    inv[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det;
+    inv[2][0] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) / det;
+    inv[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det;
+    inv[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det;
+    double vol = fabs(det) / 6.0;
+    TetRec g;
+    g.ids = U4{id[0], id[1], id[2], id[3]};
+    g.r0 = F4{(float)inv[0][0], (float)inv[0][1], (float)inv[0][2], (float)det};
+    g.r1 = F4{(float)inv[1][0], (float)inv[1][1], (float)inv[1][2], (float)(d->tet_mu[t] * vol)};
+    g.r2 = F4{(float)inv[2][0], (float)inv[2][1], (float)inv[2][2], (float)(d->tet_lambda[t] * vol)};
+    tr[t] = g;
+  }
+
+  // ---- exclusions (GPUSolver.swift:1265-1288): sorted unique (lo, hi)
+  std::vector<uint64_t> ex;
+  for (uint32_t i = 0; i < nj; i++) {
+    int32_t a = d->joint_body_a[i], b = d->joint_body_b[i];
+    if (a < 0) continue;
+    uint64_t lo = (uint64_t)std::min(a, b), hi = (uint64_t)std::max(a, b);
+    ex.push_back(lo << 32 | hi);
+  }
+  for (uint32_t i = 0; i < ns; i++) {
+    uint64_t lo = std::min(d->spring_body_a[i], d->spring_body_b[i]);
+    uint64_t hi = std::max(d->spring_body_a[i], d->spring_body_b[i]);
+    ex.push_back(lo << 32 | hi);
+  }
+  for (uint32_t i = 0; i < d->num_exclusions; i++) {
+    uint64_t a = d->exclusion_pairs[2 * i], b = d->exclusion_pairs[2 * i + 1];
+    ex.push_back(std::min(a, b) << 32 | std::max(a, b));
+  }
+  std::sort(ex.begin(), ex.end());
+  ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
+  std::vector<U2> exr(ex.size());
+  for (size_t i = 0; i < ex.size(); i++) exr[i] = U2{(uint32_t)(ex[i] >> 32), (uint32_t)(ex[i] & 0xFFFFFFFFu)};
+  s->nExcl = (uint32_t)ex.size();
+
+  // ---- soft collision groups: tet boundary particles of one connected
+  // surface share a group (GPUSolver.swift:1476-1522); one union-find over
+  // boundary faces
+  std::vector<uint32_t> clothGroup(nb, 0);
+  if (nt > 0) {
+    std::vector<uint8_t> surf = tet_surface_flags(d);
+    std::vector<uint32_t> parent(nb);
+    for (uint32_t i = 0; i < nb; i++) parent[i] = i;
+    auto find = [&](uint32_t x) {
+      while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
+      return x;
+    };
+    for (uint32_t t = 0; t < nt; t++) {
+      const uint32_t* id = d->tet_ids + 4 * t;
+      uint32_t r = find(id[0]);
+      for (int k = 1; k < 4; k++) parent[find(id[k])] = r;
+    }
+    std::map<uint32_t, uint32_t> gidx;
+    for (uint32_t i = 0; i < nb; i++) {
+      if (!surf[i]) continue;
+      bool particle = d->body_is_particle && d->body_is_particle[i];
+      if (!particle) continue;
+      uint32_t root = find(i);
+      auto it = gidx.find(root);
+      if (it == gidx.end()) it = gidx.emplace(root, (uint32_t)gidx.size() + 1).first;
+      clothGroup[i] = it->second;
+    }
+  }
+
+  // ---- static topology CSR (sorted by entry: joints, springs, tets)
+  std::vector<std::vector<uint32_t>> adj(nb);
+  auto dyn = [&](uint32_t b) { return b != kWorld && pl[b].w > 0.0f; };
+  for (uint32_t i = 0; i < nj; i++) {
+    const JointRec& g = jr[i];
+    if (g.rA.w == 0.0f && g.rB.w == 0.0f && g.motor.y == 0.0f) continue;  // inert slot
+    uint32_t e = (FK_JOINT << kAdjShift) | i;
+    if (dyn(g.header.x)) adj[g.header.x].push_back(e);
+    if (dyn(g.header.y)) adj[g.header.y].push_back(e);
+  }
+  for (uint32_t i = 0; i < ns; i++) {
+    uint32_t e = (FK_SPRING << kAdjShift) | i;
+    if (dyn(sr[i].header.x)) adj[sr[i].header.x].push_back(e);
+    if (dyn(sr[i].header.y)) adj[sr[i].header.y].push_back(e);
+  }
+  for (uint32_t t = 0; t < nt; t++) {
+    uint32_t e = (FK_TET << kAdjShift) | t;
+    const uint32_t* id = d->tet_ids + 4 * t;
+    for (int k = 0; k < 4; k++)
+      if (dyn(id[k])) adj[id[k]].push_back(e);
+  }
+  std::vector<uint32_t> sStart(nb), sCount(nb), sList;
+  for (uint32_t b = 0; b < nb; b++) {
+    sStart[b] = (uint32_t)sList.size();
+    sCount[b] = (uint32_t)adj[b].size();
+    sList.insert(sList.end(), adj[b].begin(), adj[b].end());
+  }
+  s->sAdjCount = sCount;
+  s->sAdjListHost = sList;
+
+  // ---- device buffers
+  DeviceState& S = s->S;
+  hipError_t e = hipSuccess;
+#define ALLOC(ptr, n) \
+  if ((e = dev_alloc(s, &ptr, (n))) != hipSuccess) return AVBD_ERR_ALLOC;
+  ALLOC(S.posLin, nb) ALLOC(S.posAng, nb) ALLOC(S.initLin, nb) ALLOC(S.initAng, nb) ALLOC(S.inertLin, nb)
+  ALLOC(S.inertAng, nb) ALLOC(S.velLin, nb) ALLOC(S.velAng, nb) ALLOC(S.prevVelLin, nb) ALLOC(S.props, nb)
+  ALLOC(S.shape, nb) ALLOC(S.gravityScale, nb) ALLOC(S.clothGroup, nb) ALLOC(S.posLinSnap, nb)
+  ALLOC(S.posAngSnap, nb)
+  ALLOC(S.colliderOwner, nc) ALLOC(S.colliderShapeType, nc) ALLOC(S.colliderGroup, nc) ALLOC(S.colliderShared, nc)
+  ALLOC(S.colliderShape, nc) ALLOC(S.colliderLocalPos, nc) ALLOC(S.colliderLocalRot, nc)
+  ALLOC(S.colliderFriction, nc)
+  ALLOC(S.hashedIdx, std::max<size_t>(1, hashed.size())) ALLOC(S.globalIdx, std::max<size_t>(1, globals.size()))
+  ALLOC(S.cellCount, s->gridHashSize) ALLOC(S.cellStart, s->gridHashSize) ALLOC(S.cellRigid, s->gridHashSize)
+  ALLOC(S.cellBodies, std::max<size_t>(1, hashed.size())) ALLOC(S.bodyCellSlot, std::max<size_t>(1, hashed.size()))
+  uint32_t producers = s->nHashed + s->nGlobal;
+  ALLOC(S.pairCount, std::max<uint32_t>(1, producers)) ALLOC(S.pairStart, std::max<uint32_t>(1, producers))
+  ALLOC(S.pairs, s->maxPairs) ALLOC(S.exclusions, std::max<size_t>(1, exr.size()))
+  ALLOC(S.manifolds, s->maxPairs) ALLOC(S.prevManifolds, s->maxPairs)
+  ALLOC(S.mapKeyA, s->mapCapacity) ALLOC(S.mapKeyB, s->mapCapacity) ALLOC(S.mapVal, s->mapCapacity)
+  ALLOC(S.tets, std::max<uint32_t>(1, nt)) ALLOC(S.joints, std::max<uint32_t>(1, nj))
+  ALLOC(S.springs, std::max<uint32_t>(1, ns))
+  ALLOC(S.sAdjStart, nb) ALLOC(S.sAdjCount, nb) ALLOC(S.sAdjList, std::max<size_t>(1, sList.size()))
+  ALLOC(S.dAdjCount, nb) ALLOC(S.dAdjStart, nb) ALLOC(S.dAdjCursor, nb)
+  S.dAdjCapacity = 2 * s->maxPairs;
+  ALLOC(S.dAdjList, S.dAdjCapacity)
+  ALLOC(S.colors, nb) ALLOC(S.colorsB, nb) ALLOC(S.colorList, nb) ALLOC(S.colorStart, kMaxColors + 2)
+  uint32_t bb = (nb + 255) / 256;
+  ALLOC(S.bodySlot, std::max<uint32_t>(nb, 2 * kMaxColors * bb))
+  ALLOC(S.colorFlags, 32) ALLOC(S.counters, CTR_TOTAL)
+  uint32_t scanMax = std::max<uint32_t>(std::max<uint32_t>(s->gridHashSize, producers), std::max(nb, kMaxColors * bb));
+  ALLOC(S.scanScratch, scan_scratch_words(scanMax) + 16)
+  s->residualBlocks = bb;
+  ALLOC(S.residualPartial, 2 * bb)
+  s->devScratchWords = std::max<uint32_t>(2 * s->maxPairs, 64);
+  ALLOC(s->devScratchU32, s->devScratchWords)
+#undef ALLOC
+
+  if (upload(S.posLin, pl) || upload(S.posAng, pa) || upload(S.velLin, vl) || upload(S.velAng, va) ||
+      upload(S.prevVelLin, pv) || upload(S.props, pr) || upload(S.shape, sh) || upload(S.gravityScale, gs) ||
+      upload(S.clothGroup, clothGroup) || upload(S.colliderOwner, co) || upload(S.colliderShapeType, ct) ||
+      upload(S.colliderGroup, cg) || upload(S.colliderShared, csc) || upload(S.colliderShape, cs) ||
+      upload(S.colliderLocalPos, cp) || upload(S.colliderLocalRot, cq) || upload(S.colliderFriction, cf) ||
+      upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
+      upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||
+      upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.posLinSnap, pl) ||
+      upload(S.posAngSnap, pa))
+    return AVBD_ERR_HIP;
+
+  if (hipHostMalloc((void**)&s->pinnedCounters, 2 * CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault) !=
+      hipSuccess)
+    return AVBD_ERR_ALLOC;
+  memset(s->pinnedCounters, 0, 2 * CTR_TOTAL * sizeof(uint32_t));
+  if (hipHostMalloc((void**)&s->residualPartialHost, 2 * bb * sizeof(float), hipHostMallocDefault) != hipSuccess)
+    return AVBD_ERR_ALLOC;
+  for (int i = 0; i < 2; i++)
+    if (hipEventCreateWithFlags(&s->frameDone[i], hipEventDisableTiming) != hipSuccess) return AVBD_ERR_HIP;
+  int dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  s->gridCap = (uint32_t)std::max(256, 8 * cus);
+  return AVBD_OK;
+}
+
+// Static palette: Jacobi fixed point of the greedy rule on the GPU, then
+// stable index-ordered buckets (GPUSolver.swift:1956-2050).
+avbd_status static_coloring(avbd_solver* s) {
+  DeviceState& S = s->S;
+  StepParams P = make_params(s);
+  hipStream_t st = s->stream;
+  launch_fill_u32(st, S.colors, s->nb, 0u);
+  uint32_t* in = S.colors;
+  uint32_t* out = S.colorsB;
+  uint32_t changed = 1;
+  uint32_t rounds = 0;
+  // The fixed point is reached within (longest increasing-index path + 1)
+  // rounds; nb + 1 bounds it for any graph.
+  while (changed && rounds <= s->nb + 1) {
+    launch_fill_u32(st, S.counters + CTR_COLOR_CHANGED, 1, 0u);
+    launch_static_color_pass(st, S, P, in, out);
+    std::swap(in, out);
+    rounds++;
+    if ((rounds & 7u) == 0 || rounds < 8) {
+      if (hipMemcpyAsync(&changed, S.counters + CTR_COLOR_CHANGED, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return AVBD_ERR_HIP;
+      if (hipStreamSynchronize(st) != hipSuccess) return AVBD_ERR_HIP;
+    }
+  }
+  std::vector<uint32_t> colors(s->nb);
+  if (hipMemcpy(colors.data(), in, 4 * s->nb, hipMemcpyDeviceToHost) != hipSuccess) return AVBD_ERR_HIP;
+  if (in != S.colors && hipMemcpy(S.colors, in, 4 * s->nb, hipMemcpyDeviceToDevice) != hipSuccess)
+    return AVBD_ERR_HIP;
+  uint32_t maxColor = 0;
+  std::vector<std::vector<uint32_t>> buckets(kMaxColors);
+  for (uint32_t v = 0; v < s->nb; v++) {
+    if (!(s->mass[v] > 0.0f)) continue;
+    if (colors[v] >= (uint32_t)kMaxColors) {
+      char m[160];
+      snprintf(m, sizeof(m), "greedy static solver coloring at body %u exhausted %d colors (next color %d)", v,
+               kMaxColors, kMaxColors + 1);
+      return latch(s, AVBD_FAIL_STATIC_COLOR_CAPACITY, -1, kMaxColors + 1, kMaxColors, 0, m);
+    }
+    maxColor = std::max(maxColor, colors[v]);
+    buckets[colors[v]].push_back(v);
+  }
+  s->staticUsedColors = maxColor + 1;
+  std::vector<uint32_t> clist, cstart;
+  for (int c = 0; c < kMaxColors; c++) {
+    cstart.push_back((uint32_t)clist.size());
+    clist.insert(clist.end(), buckets[c].begin(), buckets[c].end());
+  }
+  cstart.push_back((uint32_t)clist.size());
+  cstart.push_back(s->staticUsedColors);
+  if (upload(S.colorStart, cstart) || upload(S.colorList, clist)) return AVBD_ERR_HIP;
+  s->staticColorCounts.assign(kMaxColors, 0);
+  for (int c = 0; c < kMaxColors; c++) {
+    s->staticColorCounts[c] = (uint32_t)buckets[c].size();
+    s->stats.color_counts[c] = (int32_t)buckets[c].size();
+  }
+  s->stats.max_color_used = (int32_t)s->staticUsedColors - 1;
+  s->stats.num_colors = (int32_t)s->staticUsedColors;
+  return AVBD_OK;
+}
+
+// Times primal sweeps / iterations for each lane split and sweep mode on the
+// initial state, then restores every buffer the trials touched.
+void autotune(avbd_solver* s) {
+  DeviceState& S = s->S;
+  hipStream_t st = s->stream;
+  const uint32_t nb = s->nb;
+  int wantLanes = s->opts.lanes_per_body;
+  int wantMode = s->opts.sweep_mode;
+  std::vector<int> laneCands = {4, 8, 16, 32};
+  if (wantLanes == 4 || wantLanes == 8 || wantLanes == 16 || wantLanes == 32) laneCands = {wantLanes};
+  std::vector<int> modeCands = {1, 2};
+  if (wantMode == 1 || wantMode == 2) modeCands = {wantMode};
+  // dynamic coloring with per-color launches would dispatch all 64 colors
+  if (s->dynamicColoring && wantMode == 0) modeCands = {2, 1};
+  if (laneCands.size() == 1 && modeCands.size() == 1) {
+    s->lanes = laneCands[0];
+    s->sweepMode = modeCands[0];
+  }
+  // persistent grid size per lane count
+  auto pblocks = [&](int L) {
+    int maxB = persistent_max_blocks(L);
+    uint32_t maxCount = 0;
+    for (uint32_t c : s->staticColorCounts) maxCount = std::max(maxCount, c);
+    if (s->dynamicColoring) maxCount = nb;
+    uint32_t want = (maxCount * (uint32_t)L + 255) / 256;
+    want = (want + 7) / 8 * 8;
+    if (want < 8) want = 8;
+    if (maxB <= 0) return 0;
+    return (int)std::min<uint32_t>(want, (uint32_t)maxB);
+  };
+  if (laneCands.size() == 1 && modeCands.size() == 1) {
+    s->persistentBlocks = pblocks(s->lanes);
+    if (s->sweepMode == 2 && s->persistentBlocks <= 0) s->sweepMode = 1;
+    return;
+  }
+  // backup
+  std::vector<std::pair<void*, size_t>> bufs = {
+      {S.posLin, nb * 16}, {S.posAng, nb * 16}, {S.initLin, nb * 16}, {S.initAng, nb * 16},
+      {S.inertLin, nb * 16}, {S.inertAng, nb * 16}, {S.posLinSnap, nb * 16}, {S.posAngSnap, nb * 16},
+      {S.joints, std::max<uint32_t>(1, s->nj) * sizeof(JointRec)},
+      {S.springs, std::max<uint32_t>(1, s->ns) * sizeof(SpringRec)}, {S.counters, CTR_TOTAL * 4},
+      {S.dAdjCount, nb * 4}, {S.dAdjStart, nb * 4}, {S.colors, nb * 4}, {S.colorsB, nb * 4},
+      {S.colorList, nb * 4}, {S.colorStart, (kMaxColors + 2) * 4}};
+  std::vector<void*> backups;
+  for (auto& b : bufs) {
+    void* q = nullptr;
+    if (hipMalloc(&q, b.second) != hipSuccess) { backups.push_back(nullptr); continue; }
+    hipMemcpyAsync(q, b.first, b.second, hipMemcpyDeviceToDevice, st);
+    backups.push_back(q);
+  }
+  // a plausible solve state: inertial target = start = current pose
+  hipMemcpyAsync(S.initLin, S.posLin, nb * 16, hipMemcpyDeviceToDevice, st);
+  hipMemcpyAsync(S.inertLin, S.posLin, nb * 16, hipMemcpyDeviceToDevice, st);
+  hipMemcpyAsync(S.initAng, S.posAng, nb * 16, hipMemcpyDeviceToDevice, st);
+  hipMemcpyAsync(S.inertAng, S.posAng, nb * 16, hipMemcpyDeviceToDevice, st);
+  launch_fill_u32(st, S.counters, CTR_TOTAL, 0u);
+  launch_fill_u32(st, S.dAdjCount, nb, 0u);
+  launch_fill_u32(st, S.dAdjStart, nb, 0u);
+  if (s->dynamicColoring) {
+    StepParams P = make_params(s);
+    launch_dynamic_coloring(st, S, P);
+  }
+  StepParams P = make_params(s);
+  P.iterations = 2;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  float best = 3.4e38f;
+  int bestL = 8, bestM = 1, bestBlocks = 0;
+  for (int M : modeCands) {
+    for (int L : laneCands) {
+      int blocks = pblocks(L);
+      if (M == 2 && blocks <= 0) continue;
+      auto run = [&]() -> bool {
+        if (M == 2) {
+          launch_fill_u32(st, S.counters + CTR_GBAR, 2, 0u);
+          return launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors, L,
+                                         (uint32_t)blocks, 0u) == hipSuccess;
+        }
+        for (uint32_t it = 0; it < P.iterations; it++) {
+          if (s->dynamicColoring) {
+            for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++) launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L);
+          } else {
+            for (uint32_t c = 0; c < s->staticUsedColors; c++)
+              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L);
+          }
+          launch_dual(st, S, P, s->gridCap);
+        }
+        return hipGetLastError() == hipSuccess;
+      };
+      if (!run()) { hipGetLastError(); continue; }  // warm-up (and launchability check)
+      hipEventRecord(e0, st);
+      bool ok = true;
+      for (int r = 0; r < 3 && ok; r++) ok = run();
+      hipEventRecord(e1, st);
+      if (!ok || hipEventSynchronize(e1) != hipSuccess) { hipGetLastError(); continue; }
+      uint32_t fail = 0;
+      hipMemcpy(&fail, S.counters + CTR_GBAR_FAIL, 4, hipMemcpyDeviceToHost);
+      if (fail) continue;
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bestL = L; bestM = M; bestBlocks = blocks; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  for (size_t i = 0; i < bufs.size(); i++) {
+    if (!backups[i]) continue;
+    hipMemcpyAsync(bufs[i].first, backups[i], bufs[i].second, hipMemcpyDeviceToDevice, st);
+  }
+  hipStreamSynchronize(st);
+  for (void* q : backups)
+    if (q) hipFree(q);
+  s->lanes = bestL;
+  s->sweepMode = bestM;
+  s->persistentBlocks = bestBlocks ? bestBlocks : pblocks(bestL);
+  if (s->sweepMode == 2 && s->persistentBlocks <= 0) s->sweepMode = 1;
+  hipStreamSynchronize(st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------
+extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  if (s->failed) return AVBD_ERR_RUNTIME_FAILURE;
+  int64_t frame = s->frameIndex + 1;
+  int slot = (int)(frame & 1);
+  // throttle: at most two frames in flight; the slot's previous owner retires first
+  if (s->slotBusy[slot]) {
+    avbd_status st = retire_slot(s, slot);
+    if (st != AVBD_OK) return st;
+  }
+  static const char* stages[] = {"broadphase", "narrowphase", "warmstart", "adjacency", "coloring", "solve",
+                                 "finalize"};
+  for (const char* stg : stages) {
+    if (stage_denied(s, stg, frame)) {
+      sync_all(s);
+      char m[160];
+      snprintf(m, sizeof(m), "physics frame %lld: encoder creation failed at %s", (long long)frame, stg);
+      return latch(s, AVBD_FAIL_STAGE_DENIED, frame, 0, 0, 0, m);
+    }
+  }
+  bool useGraph = s->opts.use_graph && !s->graphsDisabled && !s->profiling;
+  if (useGraph && !s->graphsValid) {
+    hipError_t e = build_graphs(s);
+    if (e != hipSuccess) {
+      hipGetLastError();
+      invalidate_graphs(s);
+      s->graphsDisabled = true;
+      useGraph = false;
+    }
+  }
+  if (useGraph) {
+    // graphs[0] was captured with the buffer roles of frame graphBaseFrame
+    HIP_OR_LATCH(s, hipGraphLaunch(s->graphs[(frame - s->graphBaseFrame) & 1], s->stream), frame);
+  } else if (s->profiling) {
+    // eager, with per-stage events
+    DeviceState S = s->S;
+    StepParams P = make_params(s);
+    hipStream_t st = s->stream;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::string> names;
+    auto mark = [&](const char* name) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      hipEventRecord(e, st);
+      ev.push_back(e);
+      if (name) names.push_back(name);
+    };
+    mark(nullptr);
+    launch_fill_u32(st, S.counters, CTR_TOTAL, 0u);
+    launch_fill_u32(st, S.cellCount, s->gridHashSize, 0u);
+    launch_fill_u32(st, S.cellRigid, s->gridHashSize, 0u);
+    launch_broadphase(st, S, P);
+    mark("broadphase");
+    launch_narrowphase(st, S, P, s->gridCap);
+    launch_map_rebuild(st, S, P, s->gridCap);
+    mark("narrowphase");
+    launch_warmstart(st, S, P);
+    mark("warmstart");
+    launch_contact_adjacency(st, S, P, s->gridCap);
+    mark("adjacency");
+    if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
+    mark("coloring");
+    if (s->sweepMode == 2) {
+      HIP_OR_LATCH(s, launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors,
+                                              s->lanes, (uint32_t)s->persistentBlocks, 0u),
+                   frame);
+      mark("solve");
+    } else {
+      for (uint32_t it = 0; it < P.iterations; it++) {
+        if (s->dynamicColoring) {
+          for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++) launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes);
+        } else {
+          for (uint32_t c = 0; c < s->staticUsedColors; c++)
+            launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes);
+        }
+        launch_dual(st, S, P, s->gridCap);
+      }
+      mark("solve");
+    }
+    launch_finalize(st, S, P);
+    mark("finalize");
+    HIP_OR_LATCH(s, hipStreamSynchronize(st), frame);
+    s->profNames = names;
+    s->profMs.assign(names.size(), 0.0f);
+    for (size_t i = 0; i + 1 < ev.size(); i++) hipEventElapsedTime(&s->profMs[i], ev[i], ev[i + 1]);
+    for (auto e : ev) hipEventDestroy(e);
+    std::swap(s->S.manifolds, s->S.prevManifolds);
+  } else {
+    HIP_OR_LATCH(s, enqueue_frame(s, s->S), frame);
+  }
+  if (!s->profiling) std::swap(s->S.manifolds, s->S.prevManifolds);
+  HIP_OR_LATCH(s,
+               hipMemcpyAsync(s->pinnedCounters + slot * CTR_TOTAL, s->S.counters, CTR_TOTAL * 4,
+                              hipMemcpyDeviceToHost, s->stream),
+               frame);
+  HIP_OR_LATCH(s, hipEventRecord(s->frameDone[slot], s->stream), frame);
+  s->slotBusy[slot] = true;
+  s->slotFrame[slot] = frame;
+  s->frameIndex = frame;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_step(avbd_solver* s) { return avbd_submit_step(s); }
+
+extern "C" avbd_status avbd_synchronize(avbd_solver* s) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  return sync_all(s);
+}
+
+extern "C" avbd_status avbd_get_settings(const avbd_solver* s, avbd_settings* out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  *out = s->settings;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_set_settings(avbd_solver* s, const avbd_settings* in) {
+  if (!s || !in) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!(in->dt > 0.0f) || in->iterations < 0 || !(in->collision_margin >= 0.0f)) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  s->settings = *in;
+  invalidate_graphs(s);
+  return st == AVBD_ERR_RUNTIME_FAILURE ? st : AVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// readback / reset
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+avbd_status read_dev(avbd_solver* s, std::vector<T>& out, const T* src, size_t n) {
+  out.resize(n);
+  if (n == 0) return AVBD_OK;
+  if (hipMemcpy(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+}  // namespace
+
+extern "C" avbd_status avbd_get_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n, float* pos3,
+                                            float* rot4, float* vel3, float* angvel3) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  (void)st;  // state stays readable after a failure, as the reference's shared buffers do
+  std::vector<F4> pl, pa, vl, va;
+  if (read_dev(s, pl, s->S.posLin, s->nb) || read_dev(s, pa, s->S.posAng, s->nb) ||
+      read_dev(s, vl, s->S.velLin, s->nb) || read_dev(s, va, s->S.velAng, s->nb))
+    return AVBD_ERR_HIP;
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t i = idx ? idx[k] : k;
+    if (i >= s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+    if (pos3) { pos3[3 * k] = pl[i].x; pos3[3 * k + 1] = pl[i].y; pos3[3 * k + 2] = pl[i].z; }
+    if (rot4) { rot4[4 * k] = pa[i].x; rot4[4 * k + 1] = pa[i].y; rot4[4 * k + 2] = pa[i].z; rot4[4 * k + 3] = pa[i].w; }
+    if (vel3) { vel3[3 * k] = vl[i].x; vel3[3 * k + 1] = vl[i].y; vel3[3 * k + 2] = vl[i].z; }
+    if (angvel3) { angvel3[3 * k] = va[i].x; angvel3[3 * k + 1] = va[i].y; angvel3[3 * k + 2] = va[i].z; }
+  }
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_body_mass(avbd_solver* s, uint32_t n, float* mass, float* inertia3) {
+  if (!s || n > s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; i++) {
+    if (mass) mass[i] = s->mass[i];
+    if (inertia3) for (int k = 0; k < 3; k++) inertia3[3 * i + k] = s->inertia[3 * i + k];
+  }
+  return AVBD_OK;
+}
+
+namespace {
+// Clears the warm starts of every joint, spring and manifold (both frame
+// buffers) incident to a flagged body (GPUSolver.swift:2433-2472).
+__global__ void k_reset_incident(DeviceState S, StepParams P, const uint32_t* flag, const F4* initPenLin,
+                                 const F4* initPenAng, ManifoldRec* other, uint32_t maxPairs) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  auto flagged = [&](uint32_t b) { return b != kWorld && b < P.numBodies && flag[b] != 0u; };
+  if (gid < P.numJoints) {
+    JointRec& j = S.joints[gid];
+    if (flagged(j.header.y) || flagged(j.header.x)) {
+      j.lambdaLin = F4{0, 0, 0, 0};
+      j.lambdaAng = F4{0, 0, 0, 0};
+      j.motor.z = 0.0f;
+      j.dynamics.y = 0.0f;
+      j.dynamics.z = 0.0f;
+      j.penaltyLin = initPenLin[gid];
+      j.penaltyAng = initPenAng[gid];
+    }
+  }
+  if (gid < P.numSprings) {
+    SpringRec& sp = S.springs[gid];
+    if (flagged(sp.header.x) || flagged(sp.header.y)) sp.dual = F4{0, 0, 0, 0};
+  }
+  if (gid < maxPairs) {
+    ManifoldRec* bufs[2] = {S.manifolds, other};
+    for (int k = 0; k < 2; k++) {
+      ManifoldRec& m = bufs[k][gid];
+      if (flagged(m.header.x) || flagged(m.header.y)) {
+        uint32_t* w = (uint32_t*)&m;
+        for (uint32_t q = 0; q < sizeof(ManifoldRec) / 4; q++) w[q] = 0u;
+      }
+    }
+  }
+}
+__global__ void k_write_states(DeviceState S, const uint32_t* idx, uint32_t n, const float* p, const float* r,
+                               const float* v, const float* w) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t i = idx[k];
+  float m = S.posLin[i].w;
+  S.posLin[i] = F4{p[3 * k], p[3 * k + 1], p[3 * k + 2], m};
+  S.posAng[i] = F4{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
+  S.velLin[i] = F4{v[3 * k], v[3 * k + 1], v[3 * k + 2], 0.0f};
+  S.velAng[i] = F4{w[3 * k], w[3 * k + 1], w[3 * k + 2], 0.0f};
+  S.prevVelLin[i] = F4{v[3 * k], v[3 * k + 1], v[3 * k + 2], 0.0f};
+}
+}  // namespace
+
+extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n, const float* pos3,
+                                            const float* rot4, const float* vel3, const float* angvel3) {
+  if (!s || !pos3 || !rot4 || !vel3 || !angvel3) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  std::vector<uint32_t> ids(n), flag(s->nb, 0u);
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t i = idx ? idx[k] : k;
+    if (i >= s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+    ids[k] = i;
+    flag[i] = 1u;
+  }
+  // joint penalties at upload (initialJointPenaltyLin/Ang) are re-derived
+  // from the stored records' flags: keep a device copy made at create
+  static_assert(sizeof(F4) == 16, "F4");
+  uint32_t* dIdx = nullptr;
+  uint32_t* dFlag = nullptr;
+  float* dBuf = nullptr;
+  F4* dPen = nullptr;
+  size_t fl = (size_t)n * 13;
+  if (hipMalloc(&dIdx, 4 * n) || hipMalloc(&dFlag, 4 * s->nb) || hipMalloc(&dBuf, 4 * fl) ||
+      hipMalloc(&dPen, 32 * std::max<uint32_t>(1, s->nj)))
+    return AVBD_ERR_ALLOC;
+  std::vector<float> buf(fl);
+  memcpy(buf.data(), pos3, 12 * n);
+  memcpy(buf.data() + 3 * n, rot4, 16 * n);
+  memcpy(buf.data() + 7 * n, vel3, 12 * n);
+  memcpy(buf.data() + 10 * n, angvel3, 12 * n);
+  hipMemcpy(dIdx, ids.data(), 4 * n, hipMemcpyHostToDevice);
+  hipMemcpy(dFlag, flag.data(), 4 * s->nb, hipMemcpyHostToDevice);
+  hipMemcpy(dBuf, buf.data(), 4 * fl, hipMemcpyHostToDevice);
+  // initial joint penalties: recompute as upload does (floor or stiffness)
+  std::vector<F4> pen(2 * std::max<uint32_t>(1, s->nj));
+  if (s->nj) {
+    std::vector<JointRec> jr;
+    read_dev(s, jr, s->S.joints, s->nj);
+    for (uint32_t i = 0; i < s->nj; i++) {
+      const JointRec& g = jr[i];
+      uint32_t a = g.header.x, b = g.header.y;
+      float mA = a != kWorld ? s->mass[a] : 0.0f, mB = s->mass[b];
+      float eff = 0.0f;
+      bool any = false;
+      if (mA > 0.0f) { eff = mA; any = true; }
+      if (mB > 0.0f) { eff = any ? fminf(eff, mB) : mB; any = true; }
+      float floorK = fminf(1.0e9f, fmaxf(1.0f, eff / fmaxf(s->settings.dt * s->settings.dt, 1.0e-12f)));
+      F4 pl{0, 0, 0, 0}, pa{0, 0, 0, 0};
+      if (g.header.w & 1u) pl = F4{floorK, floorK, floorK, floorK};
+      else if (g.rA.w > 0.0f) { float k = fminf(g.rA.w, 1e9f); pl = F4{k, k, k, k}; }
+      if (g.header.w & 2u) pa = F4{floorK, floorK, floorK, floorK};
+      else if (g.rB.w > 0.0f) { float k = fminf(g.rB.w, 1e9f); pa = F4{k, k, k, k}; }
+      pen[i] = pl;
+      pen[s->nj + i] = pa;
+    }
+    hipMemcpy(dPen, pen.data(), 32 * s->nj, hipMemcpyHostToDevice);
+  }
+  StepParams P = make_params(s);
+  hipLaunchKernelGGL(k_write_states, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dIdx, n, dBuf,
+                     dBuf + 3 * n, dBuf + 7 * n, dBuf + 10 * n);
+  uint32_t total = std::max(std::max(s->nj, s->ns), s->maxPairs);
+  hipLaunchKernelGGL(k_reset_incident, dim3((total + 255) / 256), dim3(256), 0, s->stream, s->S, P, dFlag, dPen,
+                     dPen + s->nj, s->S.prevManifolds, s->maxPairs);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(dIdx);
+  hipFree(dFlag);
+  hipFree(dBuf);
+  hipFree(dPen);
+  if (e != hipSuccess) return latch(s, AVBD_FAIL_COMMAND_EXECUTION, s->frameIndex, 0, 0, (int)e, "set_body_states");
+  return AVBD_OK;
+}
+
+namespace {
+__global__ void k_set_motor(DeviceState S, const uint32_t* j, const float* v, uint32_t n, int field) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (field == 0) S.joints[j[k]].motor.x = v[k];
+  else S.joints[j[k]].motor.y = v[k];
+}
+avbd_status set_motor_field(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int field) {
+  if (!s || (n && (!joints || !v))) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  for (uint32_t k = 0; k < n; k++)
+    if (joints[k] >= s->nj) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  uint32_t* dj = nullptr;
+  float* dv = nullptr;
+  if (hipMalloc(&dj, 4 * n) || hipMalloc(&dv, 4 * n)) return AVBD_ERR_ALLOC;
+  hipMemcpy(dj, joints, 4 * n, hipMemcpyHostToDevice);
+  hipMemcpy(dv, v, 4 * n, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_set_motor, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dj, dv, n, field);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(dj);
+  hipFree(dv);
+  return e == hipSuccess ? AVBD_OK : AVBD_ERR_HIP;
+}
+}  // namespace
+
+extern "C" avbd_status avbd_set_motor_targets(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* t) {
+  return set_motor_field(s, joints, n, t, 0);
+}
+extern "C" avbd_status avbd_set_motor_torques(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* t) {
+  return set_motor_field(s, joints, n, t, 1);
+}
+
+// ---------------------------------------------------------------------------
+// stats / debug
+// ---------------------------------------------------------------------------
+extern "C" avbd_status avbd_get_stats(avbd_solver* s, avbd_stats* out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  s->stats.lanes_per_body = s->lanes;
+  s->stats.sweep_mode = s->sweepMode;
+  s->stats.uses_dynamic_coloring = s->dynamicColoring ? 1 : 0;
+  *out = s->stats;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_colors(avbd_solver* s, uint32_t n, uint32_t* colors) {
+  if (!s || !colors || n > s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  if (hipMemcpy(colors, s->S.colors, 4 * n, hipMemcpyDeviceToHost) != hipSuccess) return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_cell_lists(avbd_solver* s, uint32_t* hash_size, uint32_t* num_hashed, float* cell_size,
+                                           uint32_t* cell_start, uint32_t* cell_count, uint32_t* cell_bodies) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  if (hash_size) *hash_size = s->gridHashSize;
+  if (num_hashed) *num_hashed = s->nHashed;
+  if (cell_size) *cell_size = s->cellSize;
+  if (cell_start && hipMemcpy(cell_start, s->S.cellStart, 4 * s->gridHashSize, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  if (cell_count && hipMemcpy(cell_count, s->S.cellCount, 4 * s->gridHashSize, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  if (cell_bodies && s->nHashed && hipMemcpy(cell_bodies, s->S.cellBodies, 4 * s->nHashed, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  uint32_t cnt = 0;
+  if (hipMemcpy(&cnt, s->S.counters + CTR_PAIRS, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  *n = cnt;
+  uint32_t m = std::min(cnt, capacity);
+  if (pairs && m && hipMemcpy(pairs, s->S.pairs, 8 * m, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_manifolds(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  uint32_t cnt = 0;
+  if (hipMemcpy(&cnt, s->S.counters + CTR_PAIRS, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  *n = cnt;
+  uint32_t m = std::min(cnt, capacity);
+  // the frame just retired wrote into what is now prevManifolds
+  if (records && m && hipMemcpy(records, s->S.prevManifolds, sizeof(ManifoldRec) * m, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out) {
+  if (!s || (n && (!pairs || !out))) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  sync_all(s);
+  U2* dq = nullptr;
+  int32_t* dout = nullptr;
+  if (hipMalloc(&dq, 8 * n) || hipMalloc(&dout, 4 * n)) return AVBD_ERR_ALLOC;
+  hipMemcpy(dq, pairs, 8 * n, hipMemcpyHostToDevice);
+  StepParams P = make_params(s);
+  launch_map_find(s->stream, s->S, P, dq, dout, n);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost);
+  hipFree(dq);
+  hipFree(dout);
+  return e == hipSuccess ? AVBD_OK : AVBD_ERR_HIP;
+}
+
+extern "C" avbd_status avbd_max_constraint_error(avbd_solver* s, float* out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  // the last frame's manifolds are in prevManifolds after the swap
+  DeviceState S = s->S;
+  std::swap(S.manifolds, S.prevManifolds);
+  StepParams P = make_params(s);
+  launch_fill_u32(s->stream, s->devScratchU32, 1, 0u);
+  launch_diag_error(s->stream, S, P, s->devScratchU32, s->gridCap);
+  uint32_t bits = 0;
+  if (hipMemcpyAsync(&bits, s->devScratchU32, 4, hipMemcpyDeviceToHost, s->stream) ||
+      hipStreamSynchronize(s->stream))
+    return AVBD_ERR_HIP;
+  *out = as_float(bits);
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_residual(avbd_solver* s, float* out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  DeviceState S = s->S;
+  std::swap(S.manifolds, S.prevManifolds);
+  StepParams P = make_params(s);
+  launch_residual(s->stream, S, P, s->residualBlocks);
+  if (hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 8 * s->residualBlocks, hipMemcpyDeviceToHost,
+                     s->stream) ||
+      hipStreamSynchronize(s->stream))
+    return AVBD_ERR_HIP;
+  double g2 = 0.0, w2 = 0.0;
+  for (uint32_t b = 0; b < s->residualBlocks; b++) {
+    g2 += s->residualPartialHost[2 * b];
+    w2 += s->residualPartialHost[2 * b + 1];
+  }
+  *out = w2 > 0.0 ? (float)sqrt(g2 / w2) : (float)sqrt(g2);
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_last_failure(const avbd_solver* s, avbd_failure* out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  if (s->failed) *out = s->failure;
+  else { memset(out, 0, sizeof(*out)); out->frame = -1; }
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_profile(avbd_solver* s, int32_t enable, char* names, uint32_t names_cap, float* ms,
+                                        uint32_t* n_stages) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  s->profiling = enable != 0;
+  if (n_stages) *n_stages = (uint32_t)s->profNames.size();
+  if (names && names_cap) {
+    std::string joined;
+    for (size_t i = 0; i < s->profNames.size(); i++) {
+      if (i) joined += ",";
+      joined += s->profNames[i];
+    }
+    snprintf(names, names_cap, "%s", joined.c_str());
+  }
+  if (ms)
+    for (size_t i = 0; i < s->profMs.size(); i++) ms[i] = s->profMs[i];
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us, double* sweep_bytes,
+                                        int32_t* launches_per_sweep) {
+  if (!s || reps <= 0) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  if (s->dynamicColoring) return AVBD_ERR_UNSUPPORTED;
+  DeviceState S = s->S;
+  std::swap(S.manifolds, S.prevManifolds);  // the last frame's contacts
+  StepParams P = make_params(s);
+  // algorithmic bytes of one sweep: per body fixed + per stamp record model
+  std::vector<uint32_t> dc, dstart, dlist;
+  if (read_dev(s, dc, S.dAdjCount, s->nb) || read_dev(s, dstart, S.dAdjStart, s->nb)) return AVBD_ERR_HIP;
+  uint32_t dtotal = 0;
+  for (uint32_t b = 0; b < s->nb; b++) dtotal = std::max(dtotal, dstart[b] + dc[b]);
+  if (read_dev(s, dlist, S.dAdjList, dtotal)) return AVBD_ERR_HIP;
+  std::vector<ManifoldRec> mf;
+  uint32_t npairs = 0;
+  hipMemcpy(&npairs, S.counters + CTR_PAIRS, 4, hipMemcpyDeviceToHost);
+  if (read_dev(s, mf, (const ManifoldRec*)S.manifolds, npairs)) return AVBD_ERR_HIP;
+  double bytes = 0.0;
+  std::vector<uint32_t> sStart(s->nb);
+  {
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < s->nb; b++) { sStart[b] = run; run += s->sAdjCount[b]; }
+  }
+  for (uint32_t b = 0; b < s->nb; b++) {
+    if (!(s->mass[b] > 0.0f)) continue;
+    // particles: posLin r/w 32 + inertLin 16 + shape 16 + CSR 8; rigid adds posAng r/w 32, inertAng 16, props 16
+    bool rigid = !s->isParticle[b];
+    bytes += rigid ? 136.0 : 72.0;
+    for (uint32_t k = 0; k < s->sAdjCount[b]; k++) {
+      uint32_t e = s->sAdjListHost[sStart[b] + k], kind = e >> kAdjShift;
+      if (kind == FK_TET) bytes += 4 + 64 + 4 * 16;
+      else if (kind == FK_JOINT) bytes += 4 + 224 + 2 * 32 + 2 * 16;
+      else if (kind == FK_SPRING) bytes += 4 + 64 + 2 * 32;
+    }
+    for (uint32_t k = 0; k < dc[b]; k++) {
+      uint32_t m = dlist[dstart[b] + k] & kAdjIndexMask;
+      uint32_t nct = m < mf.size() ? mf[m].header.z : 0;
+      bytes += 4 + 64 + 80.0 * nct + 2 * 32 + 2 * 32;
+    }
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, s->stream);
+  for (int r = 0; r < reps; r++)
+    for (uint32_t c = 0; c < s->staticUsedColors; c++)
+      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes);
+  hipEventRecord(e1, s->stream);
+  hipError_t he = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (he != hipSuccess) return AVBD_ERR_HIP;
+  int32_t launches = (int32_t)s->staticUsedColors;
+  if (avg_launch_us) *avg_launch_us = ms * 1000.0f / (float)(reps * launches);
+  if (sweep_bytes) *sweep_bytes = bytes;
+  if (launches_per_sweep) *launches_per_sweep = launches;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t frame) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  s->deniedStage = stage ? stage : "";
+  s->deniedFrame = frame;
+  return AVBD_OK;
+}

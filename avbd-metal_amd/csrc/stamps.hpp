@@ -1,0 +1,328 @@
+// stamps.hpp -- per-element gradient / Hessian-block contributions to one
+// body's local Newton system (40_solver.metal:724-1360).
+//
+// A body's system is  [Hll Hla^T; Hla Haa] [dx; dth] = -[gl; ga].
+// Acc keeps the lower triangle of Hll and Haa (all the LDL^T reads), the full
+// angular-by-linear block Hla, and the two right-hand sides.
+#pragma once
+#include "state.hpp"
+
+namespace avbd {
+
+struct Acc {
+  float l00, l10, l11, l20, l21, l22;  // Hll lower triangle
+  float a00, a10, a11, a20, a21, a22;  // Haa lower triangle
+  M3 cr;                               // Hla (rows angular, columns linear)
+  V3 gl, ga;                           // gradients
+};
+
+AVBD_HD void acc_zero(Acc& a) {
+  a.l00 = a.l10 = a.l11 = a.l20 = a.l21 = a.l22 = 0.0f;
+  a.a00 = a.a10 = a.a11 = a.a20 = a.a21 = a.a22 = 0.0f;
+  a.cr = m3_zero();
+  a.gl = v3s(0);
+  a.ga = v3s(0);
+}
+// Hll += s * u u^T
+AVBD_HD void acc_lin_outer(Acc& a, V3 u, float s) {
+  V3 su = u * s;
+  a.l00 += su.x * u.x; a.l10 += su.y * u.x; a.l11 += su.y * u.y;
+  a.l20 += su.z * u.x; a.l21 += su.z * u.y; a.l22 += su.z * u.z;
+}
+AVBD_HD void acc_lin_diag(Acc& a, float d) { a.l00 += d; a.l11 += d; a.l22 += d; }
+AVBD_HD void acc_lin_m3(Acc& a, const M3& m) {
+  a.l00 += m.r0.x; a.l10 += m.r1.x; a.l11 += m.r1.y; a.l20 += m.r2.x; a.l21 += m.r2.y; a.l22 += m.r2.z;
+}
+AVBD_HD void acc_ang_outer(Acc& a, V3 u, float s) {
+  V3 su = u * s;
+  a.a00 += su.x * u.x; a.a10 += su.y * u.x; a.a11 += su.y * u.y;
+  a.a20 += su.z * u.x; a.a21 += su.z * u.y; a.a22 += su.z * u.z;
+}
+AVBD_HD void acc_ang_m3(Acc& a, const M3& m) {
+  a.a00 += m.r0.x; a.a10 += m.r1.x; a.a11 += m.r1.y; a.a20 += m.r2.x; a.a21 += m.r2.y; a.a22 += m.r2.z;
+}
+AVBD_HD void acc_ang_diag3(Acc& a, V3 d) { a.a00 += d.x; a.a11 += d.y; a.a22 += d.z; }
+// Hla += s * u v^T (u angular, v linear)
+AVBD_HD void acc_cross_outer(Acc& a, V3 u, V3 v, float s) { a.cr = m3_add(a.cr, m3_scale(m3_outer(u, v), s)); }
+
+AVBD_HD M3 lower_to_m3(float x00, float x10, float x11, float x20, float x21, float x22) {
+  return M3{v3(x00, x10, x20), v3(x10, x11, x21), v3(x20, x21, x22)};
+}
+
+// Pose of a contact partner: the start-of-iteration snapshot when the partner
+// shares self's color (Jacobi coupling), else the live pose.
+struct PoseView {
+  const F4* lin;
+  const F4* ang;
+  const F4* linSnap;
+  const F4* angSnap;
+  const uint32_t* colors;
+  uint32_t selfColor;
+  __device__ __forceinline__ F4 L(uint32_t b, uint32_t self) const {
+    return (b != self && colors[b] == selfColor) ? linSnap[b] : lin[b];
+  }
+  __device__ __forceinline__ F4 A(uint32_t b, uint32_t self) const {
+    return (b != self && colors[b] == selfColor) ? angSnap[b] : ang[b];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Stable Neo-Hookean tet (Smith et al. 2018) -- 40_solver.metal:972-1020.
+// Psi = mu/2 (Ic-3) + lambda/2 (J - aT)^2, aT = 1 + mu/lambda; per-vertex
+// Hessian  V(mu |w|^2 I + lambda g g^T)  with g = dJ/dx_self.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stamp_tet(const TetRec& t, uint32_t self, const F4* pos, Acc& acc) {
+  V3 x0 = xyz(pos[t.ids.x]), x1 = xyz(pos[t.ids.y]), x2 = xyz(pos[t.ids.z]), x3 = xyz(pos[t.ids.w]);
+  float muV = t.r1.w, lamV = t.r2.w;
+  float alphaT = 1.0f + muV / fmaxf(lamV, 1e-9f);
+  V3 d0 = x1 - x0, d1 = x2 - x0, d2 = x3 - x0;
+  V3 m0 = xyz(t.r0), m1 = xyz(t.r1), m2 = xyz(t.r2);
+  V3 f0 = d0 * m0.x + d1 * m1.x + d2 * m2.x;
+  V3 f1 = d0 * m0.y + d1 * m1.y + d2 * m2.y;
+  V3 f2 = d0 * m0.z + d1 * m1.z + d2 * m2.z;
+  V3 j0 = cross(f1, f2), j1 = cross(f2, f0), j2 = cross(f0, f1);
+  float J = dot(f0, j0);
+  float s2 = lamV * (J - alphaT);
+  V3 p0 = f0 * muV + j0 * s2, p1 = f1 * muV + j1 * s2, p2 = f2 * muV + j2 * s2;
+  V3 w;
+  if (self == t.ids.y) w = v3(m0.x, m0.y, m0.z);
+  else if (self == t.ids.z) w = v3(m1.x, m1.y, m1.z);
+  else if (self == t.ids.w) w = v3(m2.x, m2.y, m2.z);
+  else w = -(m0 + m1 + m2);
+  acc.gl += p0 * w.x + p1 * w.y + p2 * w.z;
+  V3 g = j0 * w.x + j1 * w.y + j2 * w.z;
+  acc_lin_diag(acc, muV * dot(w, w));
+  acc_lin_outer(acc, g, lamV);
+}
+
+// ---------------------------------------------------------------------------
+// rigid / particle contact manifold -- 40_solver.metal:1155-1235
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V3 contact_force(const ManifoldRec& m, int ci, V3 dqAL, V3 dqAA, V3 dqBL, V3 dqBA, V3 rAW,
+                                            V3 rBW, float alpha, V3 n, V3 t1, V3 t2, V3& Cout, float& fs,
+                                            float& bounds) {
+  const ContactRec& c = m.c[ci];
+  V3 C = xyz(c.C0) * (1.0f - alpha);
+  C += v3(dot(n, dqAL), dot(t1, dqAL), dot(t2, dqAL));
+  C -= v3(dot(n, dqBL), dot(t1, dqBL), dot(t2, dqBL));
+  C += v3(dot(cross(rAW, n), dqAA), dot(cross(rAW, t1), dqAA), dot(cross(rAW, t2), dqAA));
+  C -= v3(dot(cross(rBW, n), dqBA), dot(cross(rBW, t1), dqBA), dot(cross(rBW, t2), dqBA));
+  V3 pen = xyz(c.penalty), lam = xyz(c.lambda);
+  V3 F = pen * C + lam;
+  F.x = fminf(F.x, 0.0f);
+  bounds = fabsf(F.x) * m.basisT1.w;
+  fs = sqrtf(F.y * F.y + F.z * F.z);
+  if (fs > bounds && fs > 0.0f) {
+    float dyn = fabsf(F.x) * m.basisN.w;
+    F.y *= dyn / fs;
+    F.z *= dyn / fs;
+  }
+  Cout = C;
+  return F;
+}
+
+__device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t self, bool rigidSelf, const PoseView& pv,
+                                               const F4* initLin, const F4* initAng, float alpha, Acc& acc) {
+  uint32_t a = m.header.x, b = m.header.y, n = m.header.z;
+  bool isA = self == a;
+  bool rA_round = (m.header.w & 2u) != 0, rB_round = (m.header.w & 4u) != 0;
+  F4 pAL = pv.L(a, self), pBL = pv.L(b, self), qA = pv.A(a, self), qB = pv.A(b, self);
+  V3 dqAL = xyz(pAL) - xyz(initLin[a]);
+  V3 dqAA = q_sub(qA, initAng[a]);
+  V3 dqBL = xyz(pBL) - xyz(initLin[b]);
+  V3 dqBA = q_sub(qB, initAng[b]);
+  V3 nrm = xyz(m.basisN), t1 = xyz(m.basisT1), t2 = cross(nrm, t1);
+  float s = isA ? 1.0f : -1.0f;
+  for (uint32_t i = 0; i < n; i++) {
+    V3 rAW = rA_round ? xyz(m.c[i].rA) : q_rotate(qA, xyz(m.c[i].rA));
+    V3 rBW = rB_round ? xyz(m.c[i].rB) : q_rotate(qB, xyz(m.c[i].rB));
+    V3 C;
+    float fs, bnd;
+    V3 F = contact_force(m, (int)i, dqAL, dqAA, dqBL, dqBA, rAW, rBW, alpha, nrm, t1, t2, C, fs, bnd);
+    V3 K = xyz(m.c[i].penalty);
+    V3 jl0 = nrm * s, jl1 = t1 * s, jl2 = t2 * s;
+    acc_lin_outer(acc, jl0, K.x);
+    acc_lin_outer(acc, jl1, K.y);
+    acc_lin_outer(acc, jl2, K.z);
+    acc.gl += jl0 * F.x + jl1 * F.y + jl2 * F.z;
+    if (rigidSelf) {
+      V3 rW = isA ? rAW : rBW;
+      V3 ja0 = cross(rW, jl0), ja1 = cross(rW, jl1), ja2 = cross(rW, jl2);
+      acc_ang_outer(acc, ja0, K.x);
+      acc_ang_outer(acc, ja1, K.y);
+      acc_ang_outer(acc, ja2, K.z);
+      acc_cross_outer(acc, ja0, jl0, K.x);
+      acc_cross_outer(acc, ja1, jl1, K.y);
+      acc_cross_outer(acc, ja2, jl2, K.z);
+      acc.ga += ja0 * F.x + ja1 * F.y + ja2 * F.z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// joints -- 40_solver.metal:741-908 (ball-socket / weld / hinge with motors,
+// limits and armature)
+// ---------------------------------------------------------------------------
+AVBD_HD M3 geom_stiff_ball_socket(int k, V3 v) {
+  M3 m = m3_diag(v3s(-comp(v, k)));
+  if (k == 0) { m.r0.x += v.x; m.r1.x += v.y; m.r2.x += v.z; }
+  else if (k == 1) { m.r0.y += v.x; m.r1.y += v.y; m.r2.y += v.z; }
+  else { m.r0.z += v.x; m.r1.z += v.y; m.r2.z += v.z; }
+  return m;
+}
+AVBD_HD float wrap_pi(float a) { return a - 6.2831853f * floorf((a + 3.14159265f) / 6.2831853f); }
+AVBD_HD float hinge_twist(F4 qAr, F4 qB, V3 axis) {
+  F4 r = q_mul(q_inv(qAr), qB);
+  if (r.w < 0.0f) { r.x = -r.x; r.y = -r.y; r.z = -r.z; r.w = -r.w; }
+  return 2.0f * atan2f(dot(v3(r.x, r.y, r.z), axis), r.w);
+}
+
+__device__ __forceinline__ void stamp_joint(const JointRec& j, uint32_t self, const F4* posLin, const F4* posAng,
+                                            const F4* initAng, float alpha, float dt, Acc& acc) {
+  uint32_t a = j.header.x, b = j.header.y;
+  bool isA = self == a;
+  float torqueArm = j.C0Lin.w;
+  V3 penLin = xyz(j.penaltyLin);
+  if (dot(penLin, penLin) > 0.0f) {
+    V3 pA = a == kWorld ? xyz(j.rA) : xform(xyz(posLin[a]), posAng[a], xyz(j.rA));
+    V3 pB = xform(xyz(posLin[b]), posAng[b], xyz(j.rB));
+    V3 C = pA - pB;
+    if (j.header.w & 1u) C -= xyz(j.C0Lin) * alpha;
+    V3 F = penLin * C + xyz(j.lambdaLin);
+    float js = isA ? 1.0f : -1.0f;
+    V3 rW = isA ? q_rotate(posAng[a], xyz(j.rA)) : q_rotate(posAng[b], xyz(j.rB));
+    M3 jAng = m3_skew(-(rW * js));
+    M3 jAngT = m3_transpose(jAng);
+    M3 K = m3_diag(penLin);
+    M3 jAngTk = m3_mulm(jAngT, K);
+    acc_lin_m3(acc, K);
+    acc_ang_m3(acc, m3_mulm(jAngTk, jAng));
+    acc.cr = m3_add(acc.cr, m3_scale(jAngTk, js));
+    V3 r = rW * js;
+    M3 H = m3_add(m3_add(m3_scale(geom_stiff_ball_socket(0, r), F.x), m3_scale(geom_stiff_ball_socket(1, r), F.y)),
+                  m3_scale(geom_stiff_ball_socket(2, r), F.z));
+    acc_ang_m3(acc, m3_diagonalize(H));
+    acc.gl += F * js;
+    acc.ga += m3_mul(jAngT, F);
+  }
+  V3 penAng = xyz(j.penaltyAng);
+  if (dot(penAng, penAng) > 0.0f) {
+    F4 qA = a == kWorld ? F4{0, 0, 0, 1} : posAng[a];
+    F4 qAr = q_mul(qA, j.restRel);
+    if (j.hingeAxis.w != 0.0f) {
+      V3 axis = xyz(j.hingeAxis);
+      V3 aB = q_rotate(posAng[b], axis);
+      V3 aA = q_rotate(qAr, axis);
+      V3 C = cross(aA, aB) * torqueArm;
+      if (j.header.w & 2u) C -= xyz(j.C0Ang) * alpha;
+      V3 F = penAng * C + xyz(j.lambdaAng);
+      float alignment = dot(aA, aB);
+      M3 Jm = isA ? m3_add(m3_outer(aA, aB), m3_scale(m3_identity(), -alignment))
+                  : m3_add(m3_scale(m3_identity(), alignment), m3_scale(m3_outer(aB, aA), -1.0f));
+      Jm = m3_scale(Jm, torqueArm);
+      M3 JT = m3_transpose(Jm);
+      acc_ang_m3(acc, m3_mulm(m3_mulm(JT, m3_diag(penAng)), Jm));
+      acc.ga += m3_mul(JT, F);
+      if (j.motor.y > 0.0f || j.limits.x < j.limits.y) {
+        float twist = hinge_twist(qAr, posAng[b], axis);
+        float sm = isA ? -1.0f : 1.0f;
+        if (j.dynamics.x > 0.0f) {
+          float Ca = wrap_pi(twist - j.dynamics.y);
+          float ka = j.dynamics.x / fmaxf(dt * dt, 1e-12f);
+          acc_ang_outer(acc, aB, ka);
+          acc.ga += aB * (ka * Ca * sm);
+        }
+        if (j.limits.x < j.limits.y) {
+          float over = fmaxf(twist - j.limits.y, 0.0f) + fminf(twist - j.limits.x, 0.0f);
+          if (over != 0.0f) {
+            float kL = 4.0e4f;
+            float FL = clampf(kL * over, -3000.0f, 3000.0f);
+            acc_ang_outer(acc, aB, kL);
+            acc.ga += aB * (FL * sm);
+          }
+        }
+        if (j.motor.y > 0.0f) {
+          uint32_t mode = j.header.w & kMotorModeMask;
+          if (mode == kMotorExplicitPD || mode == kMotorVelocity) {
+            acc.ga += aB * (-j.dynamics.z * sm);
+          } else {
+            float Cm = wrap_pi(twist - j.motor.x);
+            F4 qA0 = a == kWorld ? F4{0, 0, 0, 1} : initAng[a];
+            float twist0 = hinge_twist(q_mul(qA0, j.restRel), initAng[b], axis);
+            float dTwist = wrap_pi(twist - twist0);
+            float cD = j.limits.z / fmaxf(dt, 1e-6f);
+            float raw = j.motor.w * Cm + cD * dTwist;
+            float effort = clampf(raw, -j.motor.y, j.motor.y);
+            if (fabsf(raw) < j.motor.y) acc_ang_outer(acc, aB, j.motor.w + cD);
+            acc.ga += aB * (effort * sm);
+          }
+        }
+      }
+    } else {
+      V3 C = q_sub(qAr, posAng[b]) * torqueArm;
+      float s = (isA ? 1.0f : -1.0f) * torqueArm;
+      if (j.header.w & 2u) C -= xyz(j.C0Ang) * alpha;
+      V3 F = penAng * C + xyz(j.lambdaAng);
+      acc_ang_diag3(acc, penAng * (s * s));
+      acc.ga += F * s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// springs / hard rods -- 40_solver.metal:910-970
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stamp_spring(const SpringRec& sp, uint32_t self, bool rigidSelf, const F4* posLin,
+                                             const F4* posAng, float alpha, Acc& acc) {
+  uint32_t a = sp.header.x, b = sp.header.y;
+  float stiffness = sp.rA.w, rest = sp.rB.w;
+  V3 pA = xform(xyz(posLin[a]), posAng[a], xyz(sp.rA));
+  V3 pB = xform(xyz(posLin[b]), posAng[b], xyz(sp.rB));
+  V3 d = pA - pB;
+  float dLen = length(d);
+  if (dLen <= 1.0e-6f) return;
+  V3 n = d / dLen;
+  float f;
+  if (sp.header.z != 0) {
+    float C = dLen - rest - sp.dual.z * alpha;
+    stiffness = sp.dual.y;
+    f = stiffness * C + sp.dual.x;
+    if (f <= 0.0f) return;
+  } else {
+    f = stiffness * (dLen - rest);
+  }
+  bool isA = self == a;
+  V3 jLin = isA ? n : -n;
+  acc_lin_outer(acc, jLin, stiffness);
+  acc.gl += jLin * f;
+  if (rigidSelf) {
+    V3 rW = isA ? q_rotate(posAng[a], xyz(sp.rA)) : q_rotate(posAng[b], xyz(sp.rB));
+    V3 jAng = isA ? cross(rW, n) : -cross(rW, n);
+    acc_ang_outer(acc, jAng, stiffness);
+    acc_cross_outer(acc, jAng, jLin, stiffness);
+    acc.ga += jAng * f;
+  }
+  float ft = fmaxf(f, 0.0f) / dLen;
+  if (ft > 0.0f) {
+    acc_lin_diag(acc, ft);
+    acc_lin_outer(acc, n, -ft);
+  }
+}
+
+// One adjacency entry.
+__device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView& pv, uint32_t entry, uint32_t self,
+                                            bool rigidSelf, float alpha, float dt, Acc& acc) {
+  uint32_t kind = entry >> kAdjShift, idx = entry & kAdjIndexMask;
+  if (kind == FK_TET) {
+    stamp_tet(S.tets[idx], self, S.posLin, acc);
+  } else if (kind == FK_MANIFOLD) {
+    stamp_manifold(S.manifolds[idx], self, rigidSelf, pv, S.initLin, S.initAng, alpha, acc);
+  } else if (kind == FK_JOINT) {
+    const JointRec& j = S.joints[idx];
+    if (j.header.z == 0) stamp_joint(j, self, S.posLin, S.posAng, S.initAng, alpha, dt, acc);  // skip broken
+  } else if (kind == FK_SPRING) {
+    stamp_spring(S.springs[idx], self, rigidSelf, S.posLin, S.posAng, alpha, acc);
+  }
+}
+
+}  // namespace avbd

@@ -1,0 +1,54 @@
+// state.hpp -- the device-side view of one solver: every buffer pointer the
+// kernels touch, passed by value as a kernel argument (scalar kernarg loads).
+#pragma once
+#include "common.hpp"
+
+namespace avbd {
+
+struct alignas(8) U2 {
+  uint32_t x, y;
+};
+
+struct DeviceState {
+  // bodies (one F4 per body, each field its own array)
+  F4 *posLin, *posAng, *initLin, *initAng, *inertLin, *inertAng;
+  F4 *velLin, *velAng, *prevVelLin;
+  F4 *props;          // xyz principal moment, w friction
+  F4 *shape;          // xyz size, w = +radius (rigid) / -radius (particle)
+  float* gravityScale;
+  uint32_t* clothGroup;
+  // colliders
+  uint32_t *colliderOwner, *colliderShapeType, *colliderGroup, *colliderShared;
+  F4 *colliderShape, *colliderLocalPos, *colliderLocalRot, *colliderFriction;
+  // broad phase
+  uint32_t *hashedIdx, *globalIdx;
+  uint32_t *cellCount, *cellStart, *cellBodies, *cellRigid;
+  U2* bodyCellSlot;
+  uint32_t *pairCount, *pairStart;
+  U2* pairs;
+  U2* exclusions;
+  // contacts
+  ManifoldRec *manifolds, *prevManifolds;
+  uint32_t *mapKeyA, *mapKeyB, *mapVal;
+  // elements
+  TetRec* tets;
+  JointRec* joints;
+  SpringRec* springs;
+  // adjacency: static topology CSR (built once) + per-frame contact CSR
+  uint32_t *sAdjStart, *sAdjCount, *sAdjList;
+  uint32_t *dAdjCount, *dAdjStart, *dAdjCursor, *dAdjList;
+  // colors
+  uint32_t *colors, *colorsB, *colorList, *colorStart, *bodySlot;
+  // start-of-iteration pose snapshot: contact partners that share the
+  // body's color are read from here, so a same-color contact is a Jacobi
+  // coupling with a schedule-independent result
+  F4 *posLinSnap, *posAngSnap;
+  // misc
+  uint32_t* colorFlags;       // per-pass changed flags of the dynamic coloring
+  uint32_t* counters;
+  uint32_t* scanScratch;      // block sums for the scan
+  float* residualPartial;     // per-block residual partial sums
+  uint32_t dAdjCapacity;
+};
+
+}  // namespace avbd

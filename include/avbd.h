@@ -1,0 +1,312 @@
+/*
+ * avbd.h -- C ABI of the MI355X (gfx950) AVBD step backend.
+ *
+ * This is the drop-in boundary for the per-step AVBD solve of
+ * LuckyIYI/avbd-metal.  Every entry point below replaces one member of the
+ * reference `GPUSolver` class (Sources/PhysicsAVBD/GPU/GPUSolver.swift); the
+ * citation above each declaration names the member it stands in for.
+ *
+ * Conventions
+ *  - Plain C: pointers + sizes, no C++ or device types cross the boundary.
+ *  - The library owns all device memory.  Callers hand over and receive host
+ *    arrays (copy-in / copy-out), exactly like the reference's shared-buffer
+ *    accessors after `sync()`.
+ *  - Every function returns an `avbd_status` (0 = OK) and never aborts.
+ *  - Runtime failures are latched sticky, earliest first: once a solver
+ *    reports a failure it stays failed and must be destroyed
+ *    (GPUSolver.swift:2221-2223 "Rebuild the solver; never clear and
+ *    continue").
+ *  - A solver handle is not thread-safe; use one handle per thread.
+ *  - Vectors are packed float arrays: 3 floats per F3, 4 per quaternion
+ *    stored (x, y, z, w) with w the real part (GPUSolver.swift:952).
+ */
+#ifndef AVBD_H
+#define AVBD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_ABI_VERSION 1
+#define AVBD_MAX_COLORS 64
+
+/* ---- status codes ------------------------------------------------------ */
+typedef enum avbd_status {
+    AVBD_OK = 0,
+    AVBD_ERR_INVALID_ARGUMENT = 1,  /* precondition violated (Scene.swift preconditions) */
+    AVBD_ERR_NO_DEVICE = 2,         /* AVBDError.noDevice          GPUSolver.swift:641 */
+    AVBD_ERR_ALLOC = 3,             /* AVBDError.allocFailed       GPUSolver.swift:642 */
+    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus, capsule, hull) */
+    AVBD_ERR_RUNTIME_FAILURE = 5,   /* a RuntimeFailure is latched; see avbd_last_failure */
+    AVBD_ERR_HIP = 6                /* HIP API error outside a latched frame */
+} avbd_status;
+
+/* RuntimeFailure kinds, GPUSolver.swift:663-758 */
+typedef enum avbd_failure_kind {
+    AVBD_FAIL_NONE = 0,
+    AVBD_FAIL_COMMAND_EXECUTION = 1,     /* .commandExecution (HIP launch / completion error) */
+    AVBD_FAIL_RIGID_PAIR_CAPACITY = 2,   /* .rigidPairCapacity */
+    AVBD_FAIL_SOFT_CONTACT_CAPACITY = 3, /* .softContactCapacity */
+    AVBD_FAIL_STATIC_COLOR_CAPACITY = 4, /* .staticColorCapacity (thrown by create) */
+    AVBD_FAIL_UNRESOLVED_COLORING = 5,   /* .unresolvedColoring */
+    AVBD_FAIL_STAGE_DENIED = 6,          /* .commandEncoderCreation (test hook) */
+    AVBD_FAIL_BARRIER_TIMEOUT = 7        /* persistent kernel grid barrier exceeded its spin bound */
+} avbd_failure_kind;
+
+typedef struct avbd_failure {
+    int32_t kind;          /* avbd_failure_kind */
+    int64_t frame;         /* frame the failure belongs to (-1 at init) */
+    int64_t required;      /* demand (pairs, colors, conflicting bodies ...) */
+    int64_t capacity;      /* capacity that was exceeded */
+    int32_t hip_error;     /* hipError_t for COMMAND_EXECUTION, else 0 */
+    char message[160];
+} avbd_failure;
+
+/* ---- scene wire format ------------------------------------------------- */
+
+/* SimSettings, Scene.swift:485-545 (defaults: avbd_settings_default) */
+typedef struct avbd_settings {
+    float dt;                     /* 1/60 */
+    float gravity;                /* -10, along z */
+    int32_t iterations;           /* 10 */
+    float collision_margin;       /* 0.01 */
+    uint32_t friction_combine_mode; /* 0 geometric .. 4 average */
+    float alpha;                  /* 0.99 */
+    float beta_lin;               /* 1e4 */
+    float beta_ang;               /* 100 */
+    float gamma;                  /* 0.999 */
+    float lambda_max;             /* 1e6 */
+    float rod_decay_pow;          /* 0 */
+    float particle_damping;       /* 0 */
+    float rigid_linear_damping;   /* 0 */
+    float rigid_angular_damping;  /* 0 */
+    float cloth_viscosity;        /* 0 */
+} avbd_settings;
+
+/* BodyShape, Scene.swift:6-11 */
+enum { AVBD_SHAPE_BOX = 0, AVBD_SHAPE_SPHERE = 1, AVBD_SHAPE_TORUS = 2, AVBD_SHAPE_CAPSULE = 3 };
+
+/*
+ * Packed SoA mirror of PhysicsScene (Scene.swift:547-752).  Optional arrays
+ * may be NULL and then take the Scene.swift default.  All upload-time
+ * precompute (mass/inertia, Dm^-1, topology, coloring) happens inside the
+ * library, as GPUSolver.upload does (GPUSolver.swift:901-2081).
+ */
+typedef struct avbd_scene_desc {
+    uint32_t abi_version;                 /* AVBD_ABI_VERSION */
+    avbd_settings settings;
+
+    /* SceneBody, Scene.swift:34-113 */
+    uint32_t num_bodies;
+    const float* body_size;               /* [n*3] */
+    const float* body_density;            /* [n]   0 = static */
+    const float* body_friction;           /* [n]   static friction */
+    const float* body_dynamic_friction;   /* [n]   NULL -> friction */
+    const float* body_position;           /* [n*3] */
+    const float* body_rotation;           /* [n*4] NULL -> identity */
+    const float* body_velocity;           /* [n*3] NULL -> 0 */
+    const uint32_t* body_shape;           /* [n]   AVBD_SHAPE_* */
+    const uint8_t* body_is_particle;      /* [n]   NULL -> 0 */
+    const float* body_mass;               /* [n]   NULL or NaN -> derive from density */
+    const float* body_inertia;            /* [n*3] used where body_mass is given */
+    const float* body_gravity_scale;      /* [n]   NULL -> 1 */
+
+    /* SceneCollider, Scene.swift:121-184 */
+    uint32_t num_colliders;
+    const uint32_t* collider_body;        /* [m] */
+    const float* collider_size;           /* [m*3] */
+    const float* collider_friction;       /* [m] */
+    const float* collider_dynamic_friction; /* [m] NULL -> friction */
+    const float* collider_local_position; /* [m*3] NULL -> 0 */
+    const float* collider_local_rotation; /* [m*4] NULL -> identity */
+    const uint32_t* collider_shape;       /* [m] AVBD_SHAPE_* */
+    const uint32_t* collider_group;       /* [m] NULL -> 0 */
+    const uint8_t* collider_shared;       /* [m] collidesWithSharedGeometry, NULL -> 1 */
+    const uint8_t* collider_enabled;      /* [m] collisionEnabled, NULL -> 1 */
+    const uint8_t* collider_world_round_anchor; /* [m] usesWorldSpaceRoundAnchor, NULL -> 0 */
+
+    /* SceneJoint, Scene.swift:206-299 */
+    uint32_t num_joints;
+    const int32_t* joint_body_a;          /* [j] -1 = world */
+    const int32_t* joint_body_b;          /* [j] */
+    const float* joint_ra;                /* [j*3] */
+    const float* joint_rb;                /* [j*3] */
+    const float* joint_stiffness_lin;     /* [j] +inf = hard */
+    const float* joint_stiffness_ang;     /* [j] +inf = hard, 0 = free */
+    const float* joint_fracture;          /* [j] +inf = unbreakable */
+    const uint8_t* joint_fracture_linear; /* [j] NULL -> 0 */
+    const float* joint_hinge_axis;        /* [j*3] NaN x -> no hinge */
+    const float* joint_motor;             /* [j*6] target, torque limit, kp, kd, armature, mode(0 implicit PD,1 explicit PD,2 velocity) ; NULL -> off */
+    const float* joint_limits;            /* [j*2] lo, hi ; NULL -> (1,-1) disabled */
+
+    /* SceneSpring, Scene.swift:433-452 */
+    uint32_t num_springs;
+    const uint32_t* spring_body_a;
+    const uint32_t* spring_body_b;
+    const float* spring_ra;               /* [s*3] */
+    const float* spring_rb;               /* [s*3] */
+    const float* spring_stiffness;        /* [s] */
+    const float* spring_rest;             /* [s] <0 -> spawn distance */
+    const uint8_t* spring_hard;           /* [s] */
+
+    /* SceneTet, Scene.swift:337-361 */
+    uint32_t num_tets;
+    const uint32_t* tet_ids;              /* [t*4] */
+    const float* tet_mu;                  /* [t] */
+    const float* tet_lambda;              /* [t] */
+
+    /* SceneTri, Scene.swift:321-334 */
+    uint32_t num_tris;
+    const uint32_t* tri_ids;              /* [r*3] */
+    const float* tri_mu;
+    const float* tri_lambda;
+    const float* tri_bend;
+
+    /* SceneCollisionExclusion, Scene.swift:466-475 */
+    uint32_t num_exclusions;
+    const uint32_t* exclusion_pairs;      /* [e*2] */
+} avbd_scene_desc;
+
+/* Solver creation options (GPUSolver.init maxPairsPerBody, GPUSolver.swift:331) */
+typedef struct avbd_options {
+    int32_t max_pairs_per_body;        /* 16 */
+    int32_t lanes_per_body;            /* 0 = autotune; else 4, 8, 16 or 32 lanes per body in the primal sweep */
+    int32_t sweep_mode;                /* 0 = autotune, 1 = one launch per color, 2 = persistent cooperative sweep */
+    int32_t use_graph;                 /* 1 = capture each step in a hipGraph (default), 0 = eager launches */
+    int32_t pair_capacity_override;    /* test hook (planarDATPairCapacityForTesting analogue): >0 caps rigid pairs */
+} avbd_options;
+
+typedef struct avbd_solver avbd_solver;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+
+/* Fills Scene.swift:485-545 defaults. */
+void avbd_settings_default(avbd_settings* out);
+/* Fills avbd_options defaults. */
+void avbd_options_default(avbd_options* out);
+
+/* GPUSolver.init(scene:device:maxPairsPerBody:) throws -- GPUSolver.swift:331-338.
+ * Builds device buffers, runs the upload precompute and the static coloring.
+ * On AVBD_ERR_RUNTIME_FAILURE (static palette > 64 colors, GPUSolver.swift:2008)
+ * *out is still set so avbd_last_failure can be read; destroy it afterwards. */
+avbd_status avbd_create(const avbd_scene_desc* scene, const avbd_options* opts,
+                        avbd_solver** out);
+/* deinit of GPUSolver */
+void avbd_destroy(avbd_solver* s);
+
+/* ---- stepping ------------------------------------------------------------ */
+
+/* GPUSolver.submitStep() throws -- GPUSolver.swift:3107.  Enqueues one frame
+ * (at most 2 frames in flight, GPUSolver.swift:3059-3074). */
+avbd_status avbd_submit_step(avbd_solver* s);
+/* GPUSolver.step() -- GPUSolver.swift:3095 (submit + no fatal wrapper: returns status). */
+avbd_status avbd_step(avbd_solver* s);
+/* GPUSolver.synchronize() throws -- GPUSolver.swift:2924.  Retires all
+ * frames, publishes stats, surfaces the sticky failure. */
+avbd_status avbd_synchronize(avbd_solver* s);
+
+/* mutable `settings: SimSettings` -- GPUSolver.swift:45 / syncParams :2097 */
+avbd_status avbd_get_settings(const avbd_solver* s, avbd_settings* out);
+avbd_status avbd_set_settings(avbd_solver* s, const avbd_settings* in);
+
+/* ---- state readback / reset ---------------------------------------------- */
+
+/* bodyPosition/Rotation/Velocity/AngularVelocity, GPUSolver.swift:4431-4469,
+ * bodyStates(_:) :4471.  idx NULL -> bodies 0..n-1.  Any output may be NULL. */
+avbd_status avbd_get_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n,
+                                 float* pos3, float* rot4, float* vel3, float* angvel3);
+/* bodyMass / bodyDiagonalInertia, GPUSolver.swift:4443-4455 */
+avbd_status avbd_get_body_mass(avbd_solver* s, uint32_t n, float* mass, float* inertia3);
+/* setBodyStates(_:) -- GPUSolver.swift:2409-2473, including the warm-start
+ * clearing of incident joints, springs and manifolds (:2433-2472). */
+avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n,
+                                 const float* pos3, const float* rot4,
+                                 const float* vel3, const float* angvel3);
+/* setMotorTargets / setMotorTorques -- GPUSolver.swift:2496-2590 */
+avbd_status avbd_set_motor_targets(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                   const float* target);
+avbd_status avbd_set_motor_torques(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                   const float* torque);
+
+/* ---- stats / debug ------------------------------------------------------- */
+
+/* lastNumPairs/lastPairCandidates/lastNumSoft/lastColorCounts/lastMaxColorUsed/
+ * frameIndex -- GPUSolver.swift:314-328, published by retire :2829-2880. */
+typedef struct avbd_stats {
+    int64_t frame_index;
+    int32_t num_pairs;
+    int32_t pair_candidates;
+    int32_t num_soft;
+    int32_t soft_candidates;
+    int32_t num_active_manifolds;
+    int32_t max_color_used;
+    int32_t num_colors;
+    int32_t color_counts[AVBD_MAX_COLORS];
+    int32_t color_conflicts;
+    int32_t uses_dynamic_coloring;
+    int32_t lanes_per_body;        /* primal lane split in use */
+    int32_t sweep_mode;            /* 1 per-color launches, 2 persistent */
+} avbd_stats;
+avbd_status avbd_get_stats(avbd_solver* s, avbd_stats* out);
+
+/* debugColors() -- GPUSolver.swift:2674.  colors[n] */
+avbd_status avbd_get_colors(avbd_solver* s, uint32_t n, uint32_t* colors);
+
+/* Spatial hash of the last frame (bp_count/scan/scatter/sort, 20_broadphase.metal:69-131):
+ * cell_start/cell_count [hash_size], cell_bodies [num_hashed] (collider ids sorted per cell).
+ * Pass NULL arrays to query sizes. */
+avbd_status avbd_get_cell_lists(avbd_solver* s, uint32_t* hash_size, uint32_t* num_hashed,
+                                float* cell_size, uint32_t* cell_start, uint32_t* cell_count,
+                                uint32_t* cell_bodies);
+/* Rigid contact-pair stream of the last frame (bp_emit_pairs_deterministic,
+ * 20_broadphase.metal:457): pairs[2*n] collider ids in emission order. */
+avbd_status avbd_get_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs);
+
+/* Manifolds of the last frame (ManifoldGPU, GPUTypes.swift:164): raw 704-byte
+ * records, `capacity` records at most. */
+avbd_status avbd_get_manifolds(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records);
+
+/* Persistence map lookup (pairMapFind, 20_broadphase.metal:596): for each
+ * (a,b) collider pair returns the manifold index of the map built by the last
+ * frame, or -1. */
+avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out);
+
+/* maxConstraintErrorChecked() -- GPUSolver.swift:4829 (diag_error kernel 40_solver.metal:2189) */
+avbd_status avbd_max_constraint_error(avbd_solver* s, float* out);
+
+/* Block residual r = ||g||_2 / ||m g_grav||_2 at the current iterate, g the
+ * per-body block right-hand side primal_one assembles (40_solver.metal:1384-1425).
+ * Non-mutating.  The reference defines no residual; this is the backend's. */
+avbd_status avbd_residual(avbd_solver* s, float* out);
+
+/* runtimeFailure -- GPUSolver.swift:2224 */
+avbd_status avbd_last_failure(const avbd_solver* s, avbd_failure* out);
+
+/* profiling/profileNS -- GPUSolver.swift:2200-2202.  Per stage GPU ms of the last
+ * retired frame, names comma separated. */
+avbd_status avbd_get_profile(avbd_solver* s, int32_t enable, char* names, uint32_t names_cap,
+                             float* ms, uint32_t* n_stages);
+
+/* Times `reps` full primal color sweeps (every color once) on the solver's stream
+ * with HIP events and reports the average duration of one per-color primal
+ * launch and the algorithmic bytes one sweep touches (per-stamp record model).
+ * Mutates positions (runs extra sweeps); call it after the timed region. */
+avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us,
+                             double* sweep_bytes, int32_t* launches_per_sweep);
+
+/* Test hooks, GPUSolver.swift:2233-2241 (deniedEncoderStageForTesting,
+ * completionFailureForTesting): deny the named stage ("broadphase",
+ * "narrowphase", "warmstart", "adjacency", "coloring", "solve", "finalize")
+ * on frame `frame` (-1 = every frame).  NULL clears. */
+avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t frame);
+
+/* Library / device information string. */
+const char* avbd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AVBD_H */

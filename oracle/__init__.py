@@ -1,0 +1,203 @@
+"""oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of oracle/liboracle.so, the serial CPU restatement of the
+reference AVBD step.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use it, and only as the checker / baseline.
+
+Parity status per oracle function (what pins it; see tests/test_oracle.py):
+  splitmix64            pinned: published SplitMix64 known answers
+  snh_tet               pinned: finite-difference gradient of the SNH energy,
+                        rest-state closed form, Hessian vs FD of the gradient
+  solve6x6              pinned: numpy.linalg.solve on random SPD systems
+  cell hash / lists     pinned: brute-force recompute of floor(p/cell) + hash
+  pair stream           pinned: O(n^2) sphere-bound brute force
+  static coloring       pinned: validity + serial greedy on small graphs
+  dynamic coloring      pinned: validity (no same-color contact edge)
+  predict / finalize    pinned: free fall equals implicit Euler (closed form)
+  contact / dual        pinned: box rest height 0.5 +- 0.03, stack thresholds
+                        (CPUSolverTests.swift:35-98), dual sign / cone invariants
+  joints                pinned: pendulum length invariant (CPUSolverTests.swift:67-77)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    vp, u32p, fp, dp, i32p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_double), \
+        C.POINTER(C.c_int32)
+    sig = {
+        "oracle_create": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "oracle_destroy": (None, [vp]),
+        "oracle_step": (C.c_int, [vp]),
+        "oracle_set_settings": (C.c_int, [vp, vp]),
+        "oracle_get_body_states": (C.c_int, [vp, C.c_uint32, fp, fp, fp, fp]),
+        "oracle_get_body_states_f64": (C.c_int, [vp, C.c_uint32, dp, dp, dp, dp]),
+        "oracle_set_body_states": (C.c_int, [vp, u32p, C.c_uint32, fp, fp, fp, fp]),
+        "oracle_get_colors": (C.c_int, [vp, C.c_uint32, u32p]),
+        "oracle_get_cell_lists": (C.c_int, [vp, u32p, u32p, fp, u32p, u32p, u32p]),
+        "oracle_get_pairs": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
+        "oracle_get_manifolds": (C.c_int, [vp, C.c_uint32, u32p, u32p, fp, fp]),
+        "oracle_residual": (C.c_int, [vp, dp]),
+        "oracle_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p]),
+        "oracle_map_find": (C.c_int, [vp, u32p, C.c_uint32, i32p]),
+        "oracle_splitmix64": (None, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), fp]),
+        "oracle_snh_tet": (None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
+        "oracle_solve6x6": (None, [dp, dp, dp, C.c_int]),
+        "oracle_cell_hash": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleSolver:
+    """Same surface as avbd_amd.GPUSolver, CPU serial, FP32 or FP64."""
+
+    def __init__(self, scene, precision=32, maxPairsPerBody=16):
+        self._lib = load()
+        self.scene = scene
+        desc, self._keep = scene.to_desc()
+        h = C.c_void_p()
+        rc = self._lib.oracle_create(C.byref(desc), int(precision), int(maxPairsPerBody), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"oracle_create failed ({rc})")
+        self._h = h
+        self.bodyCount = len(scene.bodies)
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.oracle_destroy(self._h)
+            self._h = None
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._lib.oracle_step(self._h)
+
+    def set_settings(self, settings):
+        s = settings.to_c()
+        self._lib.oracle_set_settings(self._h, C.byref(s))
+
+    def bodyStates(self):
+        n = self.bodyCount
+        p, r, v, w = (np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32), np.zeros((n, 3), np.float32),
+                      np.zeros((n, 3), np.float32))
+        self._lib.oracle_get_body_states(self._h, n, _p(p, C.c_float), _p(r, C.c_float), _p(v, C.c_float),
+                                         _p(w, C.c_float))
+        return p, r, v, w
+
+    def bodyStates64(self):
+        n = self.bodyCount
+        p, r, v, w = (np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3)))
+        self._lib.oracle_get_body_states_f64(self._h, n, _p(p, C.c_double), _p(r, C.c_double), _p(v, C.c_double),
+                                             _p(w, C.c_double))
+        return p, r, v, w
+
+    def positions(self):
+        return self.bodyStates()[0]
+
+    def setBodyStates(self, idx, pos, rot, vel, angvel):
+        idx = np.ascontiguousarray(idx, np.uint32)
+        a = [np.ascontiguousarray(x, np.float32) for x in (pos, rot, vel, angvel)]
+        self._lib.oracle_set_body_states(self._h, _p(idx, C.c_uint32), len(idx), *[_p(x, C.c_float) for x in a])
+
+    def debugColors(self):
+        c = np.zeros(self.bodyCount, np.uint32)
+        self._lib.oracle_get_colors(self._h, self.bodyCount, _p(c, C.c_uint32))
+        return c
+
+    def cellLists(self):
+        hs, nh, cs = C.c_uint32(), C.c_uint32(), C.c_float()
+        self._lib.oracle_get_cell_lists(self._h, C.byref(hs), C.byref(nh), C.byref(cs), None, None, None)
+        start, count = np.zeros(hs.value, np.uint32), np.zeros(hs.value, np.uint32)
+        bodies = np.zeros(max(1, nh.value), np.uint32)
+        self._lib.oracle_get_cell_lists(self._h, None, None, None, _p(start, C.c_uint32), _p(count, C.c_uint32),
+                                        _p(bodies, C.c_uint32))
+        return dict(hash_size=hs.value, cell_size=cs.value, start=start, count=count, bodies=bodies[:nh.value])
+
+    def pairs(self):
+        n = C.c_uint32()
+        self._lib.oracle_get_pairs(self._h, 0, C.byref(n), None)
+        out = np.zeros((max(1, n.value), 2), np.uint32)
+        self._lib.oracle_get_pairs(self._h, n.value, C.byref(n), _p(out, C.c_uint32))
+        return out[:n.value]
+
+    def manifolds(self):
+        n = C.c_uint32()
+        self._lib.oracle_get_manifolds(self._h, 0, C.byref(n), None, None, None)
+        h = np.zeros((max(1, n.value), 3), np.uint32)
+        nrm = np.zeros((max(1, n.value), 3), np.float32)
+        lp = np.zeros((max(1, n.value), 2), np.float32)
+        self._lib.oracle_get_manifolds(self._h, n.value, C.byref(n), _p(h, C.c_uint32), _p(nrm, C.c_float),
+                                       _p(lp, C.c_float))
+        return h[:n.value], nrm[:n.value], lp[:n.value]
+
+    def mapFind(self, pairs):
+        q = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        out = np.zeros(len(q), np.int32)
+        self._lib.oracle_map_find(self._h, _p(q, C.c_uint32), len(q), _p(out, C.c_int32))
+        return out
+
+    def residual(self):
+        v = C.c_double()
+        self._lib.oracle_residual(self._h, C.byref(v))
+        return v.value
+
+    def stats(self):
+        v = [C.c_int32() for _ in range(5)]
+        self._lib.oracle_stats(self._h, *[C.byref(x) for x in v])
+        return dict(num_pairs=v[0].value, active_manifolds=v[1].value, num_colors=v[2].value,
+                    pair_candidates=v[3].value, color_conflicts=v[4].value)
+
+
+def splitmix64(seed, n):
+    lib = load()
+    u = np.zeros(n, np.uint64)
+    f = np.zeros(n, np.float32)
+    lib.oracle_splitmix64(seed, n, _p(u, C.c_uint64), _p(f, C.c_float))
+    return u, f
+
+
+def snh_tet(x, dminv, mu, lam, vol, slot):
+    lib = load()
+    x = np.ascontiguousarray(x, np.float64).reshape(12)
+    dm = np.ascontiguousarray(dminv, np.float64).reshape(9)
+    e = C.c_double()
+    g = np.zeros(3)
+    h = np.zeros(9)
+    lib.oracle_snh_tet(_p(x, C.c_double), _p(dm, C.c_double), mu, lam, vol, int(slot), C.byref(e),
+                       _p(g, C.c_double), _p(h, C.c_double))
+    return e.value, g, h.reshape(3, 3)
+
+
+def solve6x6(A, b, precision=64):
+    lib = load()
+    A = np.ascontiguousarray(A, np.float64).reshape(36)
+    b = np.ascontiguousarray(b, np.float64).reshape(6)
+    x = np.zeros(6)
+    lib.oracle_solve6x6(_p(A, C.c_double), _p(b, C.c_double), _p(x, C.c_double), int(precision))
+    return x
+
+
+def cell_hash(p, cell_size, group, hash_size):
+    return load().oracle_cell_hash(float(p[0]), float(p[1]), float(p[2]), float(cell_size), int(group),
+                                   int(hash_size))

@@ -1,0 +1,1735 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY (parity checker and CPU baseline).
+//
+// A serial CPU restatement of the per-step AVBD solve of LuckyIYI/avbd-metal,
+// templated on the scalar type so the same code runs in FP32 (parity) and
+// FP64 (error reference).  Each function names the reference source it
+// restates; paths are relative to the reference repository root.
+//
+// Semantics choices (the reference's CPU and GPU paths differ; this follows
+// the GPU kernels, which are the spec for the step path):
+//  * preconditioned, pivot-clamped 6x6 LDL^T (00_common.metal:137-221);
+//  * trust region |dx| <= 0.35 r, |dth| <= 0.5 (40_solver.metal:1469-1489);
+//  * mass-scaled contact dual cap (40_solver.metal:1820);
+//  * sorted per-body force order: joints, springs, manifolds, tets
+//    (40_solver.metal:220-243);
+//  * a contact partner that shares the body's color is read from the
+//    start-of-iteration pose snapshot (the backend closes the reference's
+//    same-color Jacobi race this way; the oracle mirrors the rule).
+// Out of scope here, as in the backend: surface truncation (isotropic OGC /
+// Planar-DAT), cloth elements, torus / capsule / hull colliders.
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <map>
+#include <set>
+#include <vector>
+
+namespace {
+
+template <typename T>
+struct V {
+  T x, y, z;
+  V() : x(0), y(0), z(0) {}
+  V(T a, T b, T c) : x(a), y(b), z(c) {}
+  V operator+(const V& o) const { return V(x + o.x, y + o.y, z + o.z); }
+  V operator-(const V& o) const { return V(x - o.x, y - o.y, z - o.z); }
+  V operator-() const { return V(-x, -y, -z); }
+  V operator*(T s) const { return V(x * s, y * s, z * s); }
+  V operator/(T s) const { return V(x / s, y / s, z / s); }
+  T& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+  T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <typename T> V<T> mulv(const V<T>& a, const V<T>& b) { return V<T>(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <typename T> T dotv(const V<T>& a, const V<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> V<T> crossv(const V<T>& a, const V<T>& b) {
+  return V<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename T> T lenv(const V<T>& a) { return std::sqrt(dotv(a, a)); }
+template <typename T> V<T> normv(const V<T>& a) { return a * (T(1) / std::sqrt(dotv(a, a))); }
+
+template <typename T>
+struct Q {
+  T x, y, z, w;
+};
+template <typename T> Q<T> qmul(const Q<T>& a, const Q<T>& b) {
+  return Q<T>{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+template <typename T> Q<T> qconj(const Q<T>& q) { return Q<T>{-q.x, -q.y, -q.z, q.w}; }
+template <typename T> Q<T> qinv(const Q<T>& q) {
+  T n = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+  return Q<T>{-q.x / n, -q.y / n, -q.z / n, q.w / n};
+}
+// 00_common.metal:98-101
+template <typename T> V<T> qrot(const Q<T>& q, const V<T>& v) {
+  V<T> u(q.x, q.y, q.z);
+  V<T> t = crossv(u, v) * T(2);
+  return v + t * q.w + crossv(u, t);
+}
+// 00_common.metal:113-118 (canonical hemisphere)
+template <typename T> V<T> qsub(const Q<T>& a, const Q<T>& b) {
+  Q<T> r = qmul(a, qinv(b));
+  if (r.w < T(0)) r = Q<T>{-r.x, -r.y, -r.z, -r.w};
+  return V<T>(r.x, r.y, r.z) * T(2);
+}
+// 00_common.metal:119-123
+template <typename T> Q<T> qaddw(const Q<T>& q, const V<T>& w) {
+  Q<T> d = qmul(Q<T>{w.x, w.y, w.z, T(0)}, q);
+  Q<T> r{q.x + T(0.5) * d.x, q.y + T(0.5) * d.y, q.z + T(0.5) * d.z, q.w + T(0.5) * d.w};
+  T s = T(1) / std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w);
+  return Q<T>{r.x * s, r.y * s, r.z * s, r.w * s};
+}
+template <typename T> V<T> xformv(const V<T>& p, const Q<T>& q, const V<T>& v) { return qrot(q, v) + p; }
+
+// row-major 3x3
+template <typename T>
+struct M {
+  T a[3][3];
+  static M zero() { M m; memset(m.a, 0, sizeof(m.a)); return m; }
+  static M diag(const V<T>& d) { M m = zero(); m.a[0][0] = d.x; m.a[1][1] = d.y; m.a[2][2] = d.z; return m; }
+  static M outer(const V<T>& u, const V<T>& v) {
+    M m;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m.a[i][j] = u[i] * v[j];
+    return m;
+  }
+  M operator+(const M& o) const { M m; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m.a[i][j] = a[i][j] + o.a[i][j]; return m; }
+  M operator*(T s) const { M m; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m.a[i][j] = a[i][j] * s; return m; }
+  M mul(const M& o) const {
+    M m = zero();
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { T s = 0; for (int k = 0; k < 3; k++) s += a[i][k] * o.a[k][j]; m.a[i][j] = s; }
+    return m;
+  }
+  V<T> mulv(const V<T>& v) const {
+    return V<T>(a[0][0] * v.x + a[0][1] * v.y + a[0][2] * v.z, a[1][0] * v.x + a[1][1] * v.y + a[1][2] * v.z,
+                a[2][0] * v.x + a[2][1] * v.y + a[2][2] * v.z);
+  }
+  M tr() const { M m; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m.a[i][j] = a[j][i]; return m; }
+};
+template <typename T> M<T> skew(const V<T>& r) {
+  M<T> m = M<T>::zero();
+  m.a[0][1] = -r.z; m.a[0][2] = r.y; m.a[1][0] = r.z; m.a[1][2] = -r.x; m.a[2][0] = -r.y; m.a[2][1] = r.x;
+  return m;
+}
+// R diag(I) R^T (00_common.metal:102-110)
+template <typename T> M<T> worldInertia(const Q<T>& q, const V<T>& I) {
+  V<T> c0 = qrot(q, V<T>(1, 0, 0)), c1 = qrot(q, V<T>(0, 1, 0)), c2 = qrot(q, V<T>(0, 0, 1));
+  return M<T>::outer(c0, c0) * I.x + M<T>::outer(c1, c1) * I.y + M<T>::outer(c2, c2) * I.z;
+}
+
+// 00_common.metal:137-221 -- Jacobi preconditioning, LDL^T, pivot clamp 1e-6.
+template <typename T>
+void solve6(const T A[6][6], const T b[6], T x[6]) {
+  T s[6], P[6][6], rb[6];
+  for (int i = 0; i < 6; i++) s[i] = T(1) / std::sqrt(std::max(A[i][i], T(1e-30)));
+  for (int i = 0; i < 6; i++) {
+    for (int j = 0; j < 6; j++) P[i][j] = (i == j) ? T(1) : A[i][j] * s[i] * s[j];
+    rb[i] = b[i] * s[i];
+  }
+  T L[6][6] = {}, D[6];
+  const T eps = T(1e-6);
+  for (int j = 0; j < 6; j++) {
+    T d = P[j][j];
+    for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k] * D[k];
+    D[j] = std::max(d, eps);
+    for (int i = j + 1; i < 6; i++) {
+      T v = P[i][j];
+      for (int k = 0; k < j; k++) v -= L[j][k] * L[i][k] * D[k];
+      L[i][j] = v / D[j];
+    }
+  }
+  T y[6];
+  for (int i = 0; i < 6; i++) {
+    T v = rb[i];
+    for (int k = 0; k < i; k++) v -= L[i][k] * y[k];
+    y[i] = v;
+  }
+  T z[6];
+  for (int i = 0; i < 6; i++) z[i] = y[i] / D[i];
+  for (int i = 5; i >= 0; i--) {
+    T v = z[i];
+    for (int k = i + 1; k < 6; k++) v -= L[k][i] * x[k];
+    x[i] = v;
+  }
+  for (int i = 0; i < 6; i++) x[i] *= s[i];
+}
+
+template <typename T> T clampT(T v, T lo, T hi) { return std::min(std::max(v, lo), hi); }
+template <typename T> bool finiteT(T v) { return std::isfinite(v); }
+
+const uint32_t WORLD = 0xFFFFFFFFu;
+const uint32_t FKJ = 0, FKS = 1, FKM = 2, FKT = 3;
+
+// ---------------------------------------------------------------------------
+// records (GPUTypes.swift:80-173 field meanings)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Contact {
+  V<T> rA, rB, C0, lambda, penalty;
+  uint32_t feature = 0;
+  T stick = 0;
+};
+template <typename T>
+struct Manifold {
+  uint32_t a = 0, b = 0, n = 0, flags = 0, ca = 0, cb = 0;
+  V<T> nrm, t1;
+  T dynF = 0, statF = 0;
+  Contact<T> c[8];
+};
+template <typename T>
+struct Joint {
+  uint32_t a, b, broken, flags;
+  V<T> rA, rB;
+  T stiffLin, stiffAng, torqueArm, fracture;
+  V<T> C0Lin, C0Ang, lambdaLin, lambdaAng, penaltyLin, penaltyAng;
+  V<T> penLin0, penAng0;
+  Q<T> restRel;
+  V<T> axis;
+  bool hinge;
+  T motorTarget, motorTorque, motorZ, kp, limLo, limHi, kd, armature, predTwist, effort;
+};
+template <typename T>
+struct Spring {
+  uint32_t a, b, hard;
+  V<T> rA, rB;
+  T stiffness, rest, lambda, penalty, C0;
+};
+template <typename T>
+struct Tet {
+  uint32_t ids[4];
+  V<T> r[3];
+  T six, muV, lamV;
+};
+
+uint32_t cellHash32(int cx, int cy, int cz, uint32_t group, uint32_t hashSize) {
+  // 20_broadphase.metal:16-28
+  uint32_t h = ((uint32_t)cx * 92837111u) ^ ((uint32_t)cy * 689287499u) ^ ((uint32_t)cz * 283923481u);
+  h ^= group * 0x9e3779b9u;
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  return h & (hashSize - 1u);
+}
+
+uint32_t nextPow2(uint32_t v) { uint32_t n = 1; while (n < v) n <<= 1; return n; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the solver
+// ---------------------------------------------------------------------------
+struct oracle_base {
+  virtual ~oracle_base() {}
+};
+
+template <typename T>
+struct Oracle : oracle_base {
+  avbd_settings set{};
+  uint32_t nb = 0, nc = 0;
+  // bodies
+  std::vector<V<T>> pos, vel, angVel, prevVel, initLin, inertLin, moment, size;
+  std::vector<Q<T>> rot, initAng, inertAng;
+  std::vector<V<T>> snapLin;
+  std::vector<Q<T>> snapAng;
+  std::vector<T> mass, radiusSigned, gravityScale, friction;
+  std::vector<uint8_t> particle;
+  std::vector<uint32_t> clothGroup;
+  // colliders
+  std::vector<uint32_t> cOwner, cGroup, cShared, cFlags;
+  std::vector<V<T>> cSize, cLocal;
+  std::vector<Q<T>> cLocalRot;
+  std::vector<T> cRad, cStatF, cDynF;
+  std::vector<uint32_t> hashed, globals;
+  uint32_t nHashedRigid = 0;
+  float cellSize = 0.1f;
+  uint32_t hashSize = 64, maxPairs = 4096;
+  std::vector<std::pair<uint32_t, uint32_t>> excl;
+  // elements
+  std::vector<Joint<T>> joints;
+  std::vector<Spring<T>> springs;
+  std::vector<Tet<T>> tets;
+  std::vector<std::vector<uint32_t>> staticAdj;  // per body, sorted entries (joints, springs, tets)
+  // frame state
+  std::vector<Manifold<T>> manifolds, prevManifolds;
+  std::map<std::pair<uint32_t, uint32_t>, int> prevMap;  // collider pair -> index in prevManifolds
+  std::vector<uint32_t> cellStart, cellCount, cellBodies;
+  std::vector<std::pair<uint32_t, uint32_t>> pairs;
+  uint32_t pairCandidates = 0, activeManifolds = 0, colorConflicts = 0;
+  // colors
+  bool dynamicColoring = false;
+  std::vector<uint32_t> colors;
+  std::vector<std::vector<uint32_t>> buckets;
+  int64_t frame = 0;
+
+  bool dyn(uint32_t b) const { return b != WORLD && mass[b] > T(0); }
+
+  // -------------------------------------------------------------------------
+  // upload (GPUSolver.swift:901-2081)
+  // -------------------------------------------------------------------------
+  int init(const avbd_scene_desc* d, int maxPairsPerBody) {
+    set = d->settings;
+    nb = d->num_bodies;
+    nc = d->num_colliders;
+    auto r3 = [](const float* p, uint32_t i) { return V<T>(p[3 * i], p[3 * i + 1], p[3 * i + 2]); };
+    auto r4 = [](const float* p, uint32_t i) { return Q<T>{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; };
+    pos.resize(nb); vel.resize(nb); angVel.resize(nb); prevVel.resize(nb); initLin.resize(nb); inertLin.resize(nb);
+    moment.resize(nb); size.resize(nb); rot.resize(nb); initAng.resize(nb); inertAng.resize(nb);
+    mass.resize(nb); radiusSigned.resize(nb); gravityScale.resize(nb); friction.resize(nb); particle.resize(nb);
+    const float pi = 3.14159265358979f;
+    for (uint32_t i = 0; i < nb; i++) {
+      // mass / inertia per shape: computed in float as the upload does (GPUSolver.swift:914-962)
+      float sx = d->body_size[3 * i], sy = d->body_size[3 * i + 1], sz = d->body_size[3 * i + 2];
+      uint32_t sh = d->body_shape[i];
+      if (sh == AVBD_SHAPE_SPHERE) { sy = sx; sz = sx; }
+      float den = d->body_density[i], m = 0, r = 0, ix = 0, iy = 0, iz = 0;
+      if (sh == AVBD_SHAPE_BOX) {
+        m = den > 0 ? sx * sy * sz * den : 0.0f;
+        ix = (sy * sy + sz * sz) / 12.0f * m; iy = (sx * sx + sz * sz) / 12.0f * m; iz = (sx * sx + sy * sy) / 12.0f * m;
+        float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f;
+        r = std::sqrt(hx * hx + hy * hy + hz * hz);
+      } else if (sh == AVBD_SHAPE_SPHERE) {
+        float rr = sx / 2.0f;
+        m = den > 0 ? 4.0f / 3.0f * pi * rr * rr * rr * den : 0.0f;
+        ix = iy = iz = 0.4f * m * rr * rr;
+        r = rr;
+      } else {
+        return 4;
+      }
+      if (d->body_mass && !std::isnan(d->body_mass[i])) {
+        m = d->body_mass[i];
+        ix = d->body_inertia[3 * i]; iy = d->body_inertia[3 * i + 1]; iz = d->body_inertia[3 * i + 2];
+      }
+      particle[i] = d->body_is_particle ? d->body_is_particle[i] : 0;
+      mass[i] = m;
+      moment[i] = V<T>(ix, iy, iz);
+      size[i] = V<T>(sx, sy, sz);
+      radiusSigned[i] = particle[i] ? -T(r) : T(r);
+      pos[i] = r3(d->body_position, i);
+      rot[i] = d->body_rotation ? r4(d->body_rotation, i) : Q<T>{0, 0, 0, 1};
+      vel[i] = d->body_velocity ? r3(d->body_velocity, i) : V<T>();
+      prevVel[i] = vel[i];
+      angVel[i] = V<T>();
+      gravityScale[i] = d->body_gravity_scale ? d->body_gravity_scale[i] : 1.0f;
+      friction[i] = d->body_friction[i];
+    }
+    // colliders (GPUSolver.swift:971-1090)
+    cOwner.resize(nc); cGroup.resize(nc); cShared.resize(nc); cFlags.resize(nc); cSize.resize(nc); cLocal.resize(nc);
+    cLocalRot.resize(nc); cRad.resize(nc); cStatF.resize(nc); cDynF.resize(nc);
+    std::vector<float> radii;
+    uint32_t enabled = 0;
+    bool isolated = false;
+    std::vector<float> rad32(nc);
+    for (uint32_t i = 0; i < nc; i++) {
+      uint32_t b = d->collider_body[i], sh = d->collider_shape[i];
+      float sx = d->collider_size[3 * i], sy = d->collider_size[3 * i + 1], sz = d->collider_size[3 * i + 2];
+      if (sh == AVBD_SHAPE_SPHERE) { sy = sx; sz = sx; }
+      float r;
+      if (sh == AVBD_SHAPE_SPHERE) r = sx / 2.0f;
+      else { float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f; r = std::sqrt(hx * hx + hy * hy + hz * hz); }
+      rad32[i] = r;
+      bool en = !d->collider_enabled || d->collider_enabled[i];
+      cOwner[i] = b;
+      cGroup[i] = d->collider_group ? d->collider_group[i] : 0;
+      cShared[i] = (!d->collider_shared || d->collider_shared[i]) ? 1 : 0;
+      cSize[i] = V<T>(sx, sy, sz);
+      cRad[i] = particle[b] ? -T(r) : T(r);
+      cLocal[i] = d->collider_local_position ? r3(d->collider_local_position, i) : V<T>();
+      Q<T> lq = d->collider_local_rotation ? r4(d->collider_local_rotation, i) : Q<T>{0, 0, 0, 1};
+      T qn = std::sqrt(lq.x * lq.x + lq.y * lq.y + lq.z * lq.z + lq.w * lq.w);
+      cLocalRot[i] = Q<T>{lq.x / qn, lq.y / qn, lq.z / qn, lq.w / qn};
+      cStatF[i] = d->collider_friction[i];
+      cDynF[i] = d->collider_dynamic_friction ? d->collider_dynamic_friction[i] : d->collider_friction[i];
+      uint32_t f = sh;
+      if (particle[b]) f |= 0x10u;
+      if (d->collider_world_round_anchor && d->collider_world_round_anchor[i]) f |= 0x20u;
+      cFlags[i] = f;
+      if (en) {
+        enabled++;
+        if (cGroup[i] != 0) isolated = true;
+        bool bodyDyn = (d->body_mass && !std::isnan(d->body_mass[b])) ? d->body_mass[b] > 0 : d->body_density[b] > 0;
+        if (bodyDyn) radii.push_back(r);
+      }
+    }
+    std::sort(radii.begin(), radii.end());
+    float median = radii.empty() ? 0.5f : radii[radii.size() / 2];
+    float thr = median * 4.0f;
+    for (uint32_t i = 0; i < nc; i++) {
+      bool en = !d->collider_enabled || d->collider_enabled[i];
+      if (!en) continue;
+      if (rad32[i] > thr || (isolated && cGroup[i] == 0)) globals.push_back(i);
+      else hashed.push_back(i);
+    }
+    float maxR = 0.05f;
+    for (uint32_t i : hashed) maxR = std::max(maxR, rad32[i]);
+    cellSize = maxR * 2.0f;
+    for (uint32_t i : hashed) if (!particle[cOwner[i]]) nHashedRigid++;
+    maxPairs = std::max<uint32_t>(4096u, enabled * (uint32_t)maxPairsPerBody);
+    hashSize = nextPow2(std::max<uint32_t>(64u, 2u * std::max<uint32_t>(1u, enabled)));
+    // joints (GPUSolver.swift:1092-1193)
+    for (uint32_t i = 0; i < d->num_joints; i++) {
+      Joint<T> j{};
+      int32_t a = d->joint_body_a[i], b = d->joint_body_b[i];
+      float kl = d->joint_stiffness_lin[i], ka = d->joint_stiffness_ang[i];
+      float fr = d->joint_fracture ? d->joint_fracture[i] : INFINITY;
+      j.a = a >= 0 ? (uint32_t)a : WORLD;
+      j.b = (uint32_t)b;
+      j.broken = 0;
+      j.flags = (std::isinf(kl) ? 1u : 0u) | (std::isinf(ka) ? 2u : 0u) | (std::isfinite(fr) ? 4u : 0u) |
+                ((d->joint_fracture_linear && d->joint_fracture_linear[i]) ? 8u : 0u);
+      j.rA = r3(d->joint_ra, i);
+      j.rB = r3(d->joint_rb, i);
+      j.stiffLin = std::min(kl, 3.0e10f);
+      j.stiffAng = std::min(ka, 3.0e10f);
+      V<T> sA = a >= 0 ? size[a] : V<T>();
+      V<T> ss = sA + size[b];
+      j.torqueArm = dotv(ss, ss);
+      j.fracture = std::min(fr, 3.0e18f);
+      Q<T> qa = a >= 0 ? rot[a] : Q<T>{0, 0, 0, 1};
+      Q<T> rel = qmul(qinv(qa), rot[b]);
+      T rn = std::sqrt(rel.x * rel.x + rel.y * rel.y + rel.z * rel.z + rel.w * rel.w);
+      j.restRel = Q<T>{rel.x / rn, rel.y / rn, rel.z / rn, rel.w / rn};
+      j.hinge = d->joint_hinge_axis && !std::isnan(d->joint_hinge_axis[3 * i]);
+      const float* mo = d->joint_motor ? d->joint_motor + 6 * i : nullptr;
+      j.motorTorque = 0; j.motorTarget = 0; j.kp = 0; j.kd = 0; j.armature = 0; j.limLo = 1; j.limHi = -1;
+      float lo = d->joint_limits ? d->joint_limits[2 * i] : 1.0f, hi = d->joint_limits ? d->joint_limits[2 * i + 1] : -1.0f;
+      if (j.hinge) {
+        j.axis = normv(r3(d->joint_hinge_axis, i));
+        j.armature = mo ? mo[4] : 0.0f;
+        if (mo && mo[1] > 0) {
+          j.motorTarget = mo[0]; j.motorTorque = mo[1]; j.kp = mo[2]; j.kd = mo[3];
+          j.limLo = lo; j.limHi = hi;
+          int mode = (int)mo[5];
+          j.flags |= (mode == 1 ? (2u << 4) : (mode == 2 ? (3u << 4) : (1u << 4)));
+        }
+        if (lo < hi && !(mo && mo[1] > 0)) { j.limLo = lo; j.limHi = hi; j.kd = 0; }
+      }
+      T mA = a >= 0 ? mass[a] : T(0), mB = mass[b];
+      T eff = 0;
+      bool any = false;
+      if (mA > 0) { eff = mA; any = true; }
+      if (mB > 0) { eff = any ? std::min(eff, mB) : mB; any = true; }
+      T dt = set.dt;
+      T floorK = std::min(T(1.0e9), std::max(T(1), eff / std::max(dt * dt, T(1e-12))));
+      if (std::isinf(kl)) j.penaltyLin = V<T>(floorK, floorK, floorK);
+      if (std::isinf(ka)) j.penaltyAng = V<T>(floorK, floorK, floorK);
+      if (kl > 0 && std::isfinite(kl)) { T k = std::min(kl, 1e9f); j.penaltyLin = V<T>(k, k, k); }
+      if (ka > 0 && std::isfinite(ka)) { T k = std::min(ka, 1e9f); j.penaltyAng = V<T>(k, k, k); }
+      j.penLin0 = j.penaltyLin;
+      j.penAng0 = j.penaltyAng;
+      joints.push_back(j);
+    }
+    // springs (GPUSolver.swift:1195-1212)
+    for (uint32_t i = 0; i < d->num_springs; i++) {
+      Spring<T> s{};
+      s.a = d->spring_body_a[i];
+      s.b = d->spring_body_b[i];
+      s.hard = (d->spring_hard && d->spring_hard[i]) ? 1 : 0;
+      s.rA = r3(d->spring_ra, i);
+      s.rB = r3(d->spring_rb, i);
+      s.stiffness = d->spring_stiffness[i];
+      T rest = d->spring_rest[i];
+      if (rest < 0) rest = lenv((pos[s.a] + qrot(rot[s.a], s.rA)) - (pos[s.b] + qrot(rot[s.b], s.rB)));
+      s.rest = rest;
+      springs.push_back(s);
+    }
+    // tets: Dm from spawn positions, rows of Dm^-1, 6V, mu V, lambda V (GPUSolver.swift:1214-1236)
+    for (uint32_t t = 0; t < d->num_tets; t++) {
+      Tet<T> g{};
+      for (int k = 0; k < 4; k++) g.ids[k] = d->tet_ids[4 * t + k];
+      double D[3][3];
+      for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++)
+          D[k][c] = (double)d->body_position[3 * g.ids[c + 1] + k] - (double)d->body_position[3 * g.ids[0] + k];
+      double det = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) - D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                   D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+      double C[3][3];  // cofactor transpose / det = inverse
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+          C[i][j] = (D[i1][j1] * D[i2][j2] - D[i1][j2] * D[i2][j1]) / det;
+        }
+      for (int r = 0; r < 3; r++) g.r[r] = V<T>((T)(float)C[r][0], (T)(float)C[r][1], (T)(float)C[r][2]);
+      double vol = std::fabs(det) / 6.0;
+      g.six = (T)(float)det;
+      g.muV = (T)(float)(d->tet_mu[t] * vol);
+      g.lamV = (T)(float)(d->tet_lambda[t] * vol);
+      tets.push_back(g);
+    }
+    // exclusions (GPUSolver.swift:1265-1288)
+    std::set<std::pair<uint32_t, uint32_t>> ex;
+    for (auto& j : joints) if (j.a != WORLD) ex.insert({std::min(j.a, j.b), std::max(j.a, j.b)});
+    for (auto& s : springs) ex.insert({std::min(s.a, s.b), std::max(s.a, s.b)});
+    for (uint32_t i = 0; i < d->num_exclusions; i++) {
+      uint32_t a = d->exclusion_pairs[2 * i], b = d->exclusion_pairs[2 * i + 1];
+      ex.insert({std::min(a, b), std::max(a, b)});
+    }
+    excl.assign(ex.begin(), ex.end());
+    // soft collision group of tet boundary particles (GPUSolver.swift:1476-1522)
+    clothGroup.assign(nb, 0);
+    if (!tets.empty()) {
+      std::map<std::array<uint32_t, 3>, int> faces;
+      const int fi[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      for (auto& t : tets)
+        for (int f = 0; f < 4; f++) {
+          std::array<uint32_t, 3> k{t.ids[fi[f][0]], t.ids[fi[f][1]], t.ids[fi[f][2]]};
+          std::sort(k.begin(), k.end());
+          faces[k]++;
+        }
+      std::vector<uint32_t> parent(nb);
+      for (uint32_t i = 0; i < nb; i++) parent[i] = i;
+      std::function<uint32_t(uint32_t)> find = [&](uint32_t x) { return parent[x] == x ? x : (parent[x] = find(parent[x])); };
+      for (auto& t : tets) for (int k = 1; k < 4; k++) parent[find(t.ids[k])] = find(t.ids[0]);
+      std::map<uint32_t, uint32_t> gid;
+      std::vector<uint8_t> surf(nb, 0);
+      for (auto& kv : faces) if (kv.second == 1) for (uint32_t v : kv.first) surf[v] = 1;
+      for (uint32_t i = 0; i < nb; i++) {
+        if (!surf[i] || !particle[i]) continue;
+        uint32_t r = find(i);
+        if (!gid.count(r)) { uint32_t nid = (uint32_t)gid.size() + 1; gid[r] = nid; }
+        clothGroup[i] = gid[r];
+      }
+    }
+    // static force adjacency, sorted: joints, springs, tets
+    staticAdj.assign(nb, {});
+    for (uint32_t i = 0; i < joints.size(); i++) {
+      const Joint<T>& j = joints[i];
+      if (j.stiffLin == 0 && j.stiffAng == 0 && j.motorTorque == 0) continue;
+      if (dyn(j.a)) staticAdj[j.a].push_back((FKJ << 28) | i);
+      if (dyn(j.b)) staticAdj[j.b].push_back((FKJ << 28) | i);
+    }
+    for (uint32_t i = 0; i < springs.size(); i++) {
+      if (dyn(springs[i].a)) staticAdj[springs[i].a].push_back((FKS << 28) | i);
+      if (dyn(springs[i].b)) staticAdj[springs[i].b].push_back((FKS << 28) | i);
+    }
+    for (uint32_t t = 0; t < tets.size(); t++)
+      for (int k = 0; k < 4; k++) if (dyn(tets[t].ids[k])) staticAdj[tets[t].ids[k]].push_back((FKT << 28) | t);
+    dynamicColoring = d->num_tets == 0 && d->num_tris == 0;
+    // static greedy palette in body-index order (GPUSolver.swift:1956-2050)
+    std::vector<std::set<uint32_t>> adj(nb);
+    auto link = [&](uint32_t a, uint32_t b) {
+      if (a == WORLD || b == WORLD) return;
+      if (dyn(a) && dyn(b)) { adj[a].insert(b); adj[b].insert(a); }
+    };
+    for (auto& j : joints) if (j.stiffLin != 0 || j.stiffAng != 0 || j.motorTorque != 0) link(j.a, j.b);
+    for (auto& s : springs) link(s.a, s.b);
+    for (auto& t : tets) for (int i = 0; i < 4; i++) for (int k = i + 1; k < 4; k++) link(t.ids[i], t.ids[k]);
+    colors.assign(nb, 0);
+    for (uint32_t v = 0; v < nb; v++) {
+      if (!dyn(v)) continue;
+      uint64_t used = 0;
+      for (uint32_t n : adj[v]) if (n < v && colors[n] < 64) used |= 1ull << colors[n];
+      if (used == ~0ull) return 5;  // staticColorCapacity
+      uint32_t c = 0;
+      while (used & (1ull << c)) c++;
+      colors[v] = c;
+    }
+    rebuildBuckets();
+    snapLin = pos;
+    snapAng = rot;
+    return 0;
+  }
+
+  void rebuildBuckets() {
+    buckets.assign(64, {});
+    for (uint32_t v = 0; v < nb; v++) if (dyn(v)) buckets[colors[v] & 63u].push_back(v);
+  }
+
+  // -------------------------------------------------------------------------
+  // broad phase (20_broadphase.metal:69-512) -- computed in float like the
+  // kernels so cells and pair sets are comparable bit for bit
+  // -------------------------------------------------------------------------
+  V<T> center(uint32_t c) const { return pos[cOwner[c]] + qrot(rot[cOwner[c]], cLocal[c]); }
+  static bool groupsOk(uint32_t a, uint32_t b) { return a == 0 || b == 0 || a == b; }
+  static bool domainsOk(uint32_t ga, uint32_t gb, uint32_t sa, uint32_t sb) {
+    if (!groupsOk(ga, gb)) return false;
+    if (ga == 0 && gb != 0) return sb != 0;
+    if (gb == 0 && ga != 0) return sa != 0;
+    return true;
+  }
+  void cellOf(const V<T>& p, int& x, int& y, int& z) const {
+    x = (int)std::floor(p.x / (T)cellSize);
+    y = (int)std::floor(p.y / (T)cellSize);
+    z = (int)std::floor(p.z / (T)cellSize);
+  }
+  bool excluded(uint32_t a, uint32_t b) const {
+    return std::binary_search(excl.begin(), excl.end(), std::make_pair(a, b));
+  }
+  void broadphase() {
+    cellCount.assign(hashSize, 0);
+    cellStart.assign(hashSize, 0);
+    std::vector<uint32_t> cellOfCol(hashed.size());
+    std::vector<uint8_t> cellRigid(hashSize, 0);
+    for (size_t k = 0; k < hashed.size(); k++) {
+      uint32_t c = hashed[k];
+      int x, y, z;
+      cellOf(center(c), x, y, z);
+      uint32_t h = cellHash32(x, y, z, cGroup[c], hashSize);
+      cellOfCol[k] = h;
+      cellCount[h]++;
+      if (cRad[c] >= 0) cellRigid[h] = 1;
+    }
+    uint32_t run = 0;
+    for (uint32_t h = 0; h < hashSize; h++) { cellStart[h] = run; run += cellCount[h]; }
+    cellBodies.assign(hashed.size(), 0);
+    std::vector<uint32_t> fill(hashSize, 0);
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    for (size_t k = 0; k < hashed.size(); k++) order.push_back({cellOfCol[k], hashed[k]});
+    std::sort(order.begin(), order.end());  // each bucket sorted by collider id (bp_sort_cells)
+    for (auto& o : order) cellBodies[cellStart[o.first] + fill[o.first]++] = o.second;
+    // pairs, producer order = hashed then globals (bpProcessProducer)
+    pairs.clear();
+    pairCandidates = 0;
+    auto accept = [&](uint32_t a, uint32_t b, bool canonical) {
+      pairCandidates++;
+      if (pairs.size() < maxPairs) pairs.push_back(canonical ? std::make_pair(std::min(a, b), std::max(a, b)) : std::make_pair(a, b));
+    };
+    auto test = [&](uint32_t a, uint32_t b, uint32_t ga, bool cellScan) -> bool {
+      if (!domainsOk(cGroup[a], cGroup[b], cShared[a], cShared[b])) return false;
+      uint32_t oa = cOwner[a], ob = cOwner[b];
+      if (oa == ob) return false;
+      if (!(mass[oa] > 0) && !(mass[ob] > 0)) return false;
+      if (cellScan) { if (ga != 0 && clothGroup[ob] != 0) return false; }
+      else { if (ga != 0 && clothGroup[ob] == ga) return false; }
+      T ra = std::fabs(cRad[a]), rb = std::fabs(cRad[b]);
+      V<T> dp = center(a) - center(b);
+      T r = ra + rb;
+      if (dotv(dp, dp) > r * r) return false;
+      if (excluded(std::min(oa, ob), std::max(oa, ob))) return false;
+      return true;
+    };
+    for (uint32_t a : hashed) {
+      uint32_t ga = clothGroup[cOwner[a]];
+      bool scan = !(ga != 0 && nHashedRigid == 0);
+      if (scan) {
+        int cx, cy, cz;
+        cellOf(center(a), cx, cy, cz);
+        std::vector<uint32_t> seen;
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              uint32_t h = cellHash32(cx + dx, cy + dy, cz + dz, cGroup[a], hashSize);
+              if (std::find(seen.begin(), seen.end(), h) != seen.end()) continue;
+              seen.push_back(h);
+              if (ga != 0 && !cellRigid[h]) continue;
+              for (uint32_t k = cellStart[h]; k < cellStart[h] + cellCount[h]; k++) {
+                uint32_t b = cellBodies[k];
+                if (b <= a) continue;
+                if (test(a, b, ga, true)) accept(a, b, false);
+              }
+            }
+      }
+      for (uint32_t b : globals)
+        if (test(a, b, ga, false)) accept(a, b, true);
+    }
+    for (size_t gi = 0; gi < globals.size(); gi++) {
+      uint32_t a = globals[gi];
+      uint32_t ga = clothGroup[cOwner[a]];
+      for (size_t gj = gi + 1; gj < globals.size(); gj++)
+        if (test(a, globals[gj], ga, false)) accept(a, globals[gj], true);
+    }
+  }
+
+  // -------------------------------------------------------------------------
+  // narrow phase (30_narrowphase.metal:337-1322, sphere and box branches)
+  // -------------------------------------------------------------------------
+  struct Box { V<T> c, h, ax[3]; };
+  struct Best { int type, ia, ib; T sep; V<T> n; bool valid; };
+  struct Pt { V<T> xA, xB; uint32_t f; };
+
+  T specCap(T ra, T rb) const {
+    T r = std::min(std::fabs(ra), std::fabs(rb));
+    return std::min(T(0.25), std::max(T(4) * (T)set.collision_margin, T(3) * r));
+  }
+  T detectMargin(const V<T>& n, const V<T>& rv, T ra, T rb) const {
+    T ap = std::max(T(0), -dotv(rv, n));
+    return (T)set.collision_margin + std::min(ap * (T)set.dt, specCap(ra, rb));
+  }
+  V<T> penFloor(uint32_t a, uint32_t b) const {
+    T big = std::numeric_limits<T>::max();
+    T mA = mass[a], mB = mass[b];
+    T mm = std::min(mA > 0 ? mA : big, mB > 0 ? mB : big);
+    if (mm == big) return V<T>(1, 1, 1);
+    T dt = set.dt;
+    T k = std::min(T(1e10), std::max(T(1), mm / std::max(dt * dt, T(1e-12))));
+    return V<T>(k, std::min(k, T(1e6)), std::min(k, T(1e6)));
+  }
+  static void basis(const V<T>& n, V<T>& t1, V<T>& t2) {
+    t1 = std::fabs(n.x) > std::fabs(n.z) ? V<T>(-n.y, n.x, 0) : V<T>(0, -n.z, n.y);
+    t1 = normv(t1);
+    t2 = crossv(n, t1);
+  }
+  T combine(T a, T b) const {
+    uint32_t m = set.friction_combine_mode;
+    if (m == 1) return a * b;
+    if (m == 2) return std::min(a, b);
+    if (m == 3) return std::max(a, b);
+    if (m == 4) return T(0.5) * (a + b);
+    return std::sqrt(std::max(a * b, T(0)));
+  }
+  bool testAxis(const Box& A, const Box& B, const V<T>& delta, V<T> axis, int type, int ia, int ib, const V<T>& rv,
+                T cap, Best& best) const {
+    T l2 = dotv(axis, axis);
+    if (l2 < T(1e-6)) return true;
+    V<T> n = axis * (T(1) / std::sqrt(l2));
+    if (dotv(n, delta) < 0) n = -n;
+    T dist = std::fabs(dotv(delta, n));
+    T rA = A.h.x * std::fabs(dotv(n, A.ax[0])) + A.h.y * std::fabs(dotv(n, A.ax[1])) + A.h.z * std::fabs(dotv(n, A.ax[2]));
+    T rB = B.h.x * std::fabs(dotv(n, B.ax[0])) + B.h.y * std::fabs(dotv(n, B.ax[1])) + B.h.z * std::fabs(dotv(n, B.ax[2]));
+    T sep = dist - (rA + rB);
+    T allowed = std::min(std::max(T(0), -dotv(rv, n)) * (T)set.dt, cap);
+    if (sep > allowed) return false;
+    if (!best.valid || sep > best.sep) best = Best{type, ia, ib, sep, n, true};
+    return true;
+  }
+  static V<T> support(const Box& b, const V<T>& d) {
+    T sx = dotv(d, b.ax[0]) >= 0 ? T(1) : T(-1), sy = dotv(d, b.ax[1]) >= 0 ? T(1) : T(-1), sz = dotv(d, b.ax[2]) >= 0 ? T(1) : T(-1);
+    return b.c + b.ax[0] * (b.h.x * sx) + b.ax[1] * (b.h.y * sy) + b.ax[2] * (b.h.z * sz);
+  }
+  static int clip(const V<T>* in, int cnt, const V<T>& n, T off, V<T>* out) {
+    if (cnt <= 0) return 0;
+    int oc = 0;
+    V<T> a = in[cnt - 1];
+    T da = dotv(n, a) - off;
+    for (int i = 0; i < cnt; i++) {
+      V<T> b = in[i];
+      T db = dotv(n, b) - off;
+      bool ai = da <= T(1e-5), bi = db <= T(1e-5);
+      if (ai != bi) {
+        T t = 0, den = da - db;
+        if (std::fabs(den) > T(1e-6)) t = clampT(da / den, T(0), T(1));
+        if (oc < 16) out[oc++] = a + (b - a) * t;
+      }
+      if (bi && oc < 16) out[oc++] = b;
+      a = b;
+      da = db;
+    }
+    return oc;
+  }
+  static bool addPt(std::vector<Pt>& cs, std::vector<V<T>>& mids, const V<T>& xA, const V<T>& xB, uint32_t f) {
+    V<T> mid = (xA + xB) * T(0.5);
+    for (auto& m : mids) { V<T> d = mid - m; if (dotv(d, d) < T(1e-6)) return false; }
+    if (cs.size() >= 8) return false;
+    cs.push_back(Pt{xA, xB, f});
+    mids.push_back(mid);
+    return true;
+  }
+  static void segSeg(V<T> p0, V<T> p1, V<T> q0, V<T> q1, V<T>& c0, V<T>& c1) {
+    V<T> d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    T a = dotv(d1, d1), e = dotv(d2, d2), f = dotv(d2, r), s = 0, t = 0;
+    const T eps = T(1e-6);
+    if (a <= eps && e <= eps) { c0 = p0; c1 = q0; return; }
+    if (a <= eps) t = clampT(f / e, T(0), T(1));
+    else {
+      T c = dotv(d1, r);
+      if (e <= eps) s = clampT(-c / a, T(0), T(1));
+      else {
+        T b = dotv(d1, d2), den = a * e - b * b;
+        if (std::fabs(den) > eps) s = clampT((b * f - c * e) / den, T(0), T(1));
+        t = (b * s + f) / e;
+        if (t < 0) { t = 0; s = clampT(-c / a, T(0), T(1)); }
+        else if (t > 1) { t = 1; s = clampT((b - c) / a, T(0), T(1)); }
+      }
+    }
+    c0 = p0 + d1 * s;
+    c1 = q0 + d2 * t;
+  }
+  static T half(const Box& b, int i) { return b.h[i]; }
+  static void faceAxes(const Box& b, int k, V<T>& u, V<T>& v, T& eu, T& ev) {
+    if (k == 0) { u = b.ax[1]; v = b.ax[2]; eu = b.h.y; ev = b.h.z; }
+    else if (k == 1) { u = b.ax[0]; v = b.ax[2]; eu = b.h.x; ev = b.h.z; }
+    else { u = b.ax[0]; v = b.ax[1]; eu = b.h.x; ev = b.h.y; }
+  }
+  static void supportEdge(const Box& b, int k, const V<T>& dir, V<T>& e0, V<T>& e1) {
+    int a1 = (k + 1) % 3, a2 = (k + 2) % 3;
+    T s1 = dotv(dir, b.ax[a1]) >= 0 ? T(1) : T(-1), s2 = dotv(dir, b.ax[a2]) >= 0 ? T(1) : T(-1);
+    V<T> c = b.c + b.ax[a1] * (half(b, a1) * s1) + b.ax[a2] * (half(b, a2) * s2);
+    e0 = c - b.ax[k] * half(b, k);
+    e1 = c + b.ax[k] * half(b, k);
+  }
+
+  void collide(uint32_t pi, Manifold<T>& out) {
+    uint32_t ia = pairs[pi].first, ib = pairs[pi].second;
+    uint32_t ba = cOwner[ia], bb = cOwner[ib];
+    out = Manifold<T>{};
+    out.a = ba; out.b = bb; out.ca = ia; out.cb = ib;
+    V<T> cA = pos[ba] + qrot(rot[ba], cLocal[ia]), cB = pos[bb] + qrot(rot[bb], cLocal[ib]);
+    Q<T> qA = qmul(rot[ba], cLocalRot[ia]), qB = qmul(rot[bb], cLocalRot[ib]);
+    V<T> vA = vel[ba] + crossv(angVel[ba], cA - pos[ba]), vB = vel[bb] + crossv(angVel[bb], cB - pos[bb]);
+    V<T> rv = vB - vA;
+    Box A{cA, cSize[ia] * T(0.5), {qrot(qA, V<T>(1, 0, 0)), qrot(qA, V<T>(0, 1, 0)), qrot(qA, V<T>(0, 0, 1))}};
+    Box B{cB, cSize[ib] * T(0.5), {qrot(qB, V<T>(1, 0, 0)), qrot(qB, V<T>(0, 1, 0)), qrot(qB, V<T>(0, 0, 1))}};
+    V<T> delta = B.c - A.c;
+    bool sphA = (cFlags[ia] & 0xF) == 1, sphB = (cFlags[ib] & 0xF) == 1;
+    T statF = combine(cStatF[ia], cStatF[ib]), dynF = combine(cDynF[ia], cDynF[ib]);
+    T warm = (T)set.alpha * (T)set.gamma;
+    T gam = set.gamma;
+    std::vector<Pt> cs;
+    std::vector<V<T>> mids;
+    V<T> nrm;
+    auto prevIndex = [&]() -> int {
+      auto it = prevMap.find({ia, ib});
+      return it == prevMap.end() ? -1 : it->second;
+    };
+    V<T> fl = penFloor(ba, bb);
+    auto warmPen = [&](V<T> p) {
+      return V<T>(clampT(p.x * gam, fl.x, T(1e10)), clampT(p.y * gam, fl.y, T(1e6)), clampT(p.z * gam, fl.z, T(1e6)));
+    };
+    if (sphA || sphB) {
+      T rA = cSize[ia].x * T(0.5), rB = cSize[ib].x * T(0.5);
+      if (sphA && sphB) {
+        V<T> d = A.c - B.c;
+        T dist = lenv(d);
+        V<T> nT = dist > T(1e-9) ? d / dist : V<T>(0, 0, 1);
+        if (dist > rA + rB + detectMargin(nT, vA - vB, rA, rB)) return;
+        nrm = nT;
+        cs.push_back(Pt{A.c - nrm * rA, B.c + nrm * rB, 0});
+      } else {
+        bool sIsA = sphA;
+        T r = sIsA ? rA : rB;
+        const Box& box = sIsA ? B : A;
+        Q<T> qBox = sIsA ? qB : qA;
+        V<T> sc = sIsA ? A.c : B.c;
+        V<T> local = qrot(qconj(qBox), sc - box.c);
+        V<T> cl(clampT(local.x, -box.h.x, box.h.x), clampT(local.y, -box.h.y, box.h.y), clampT(local.z, -box.h.z, box.h.z));
+        V<T> nL, qL;
+        if (cl.x == local.x && cl.y == local.y && cl.z == local.z) {
+          V<T> pen = box.h - V<T>(std::fabs(local.x), std::fabs(local.y), std::fabs(local.z));
+          int ax = 0;
+          if (pen.y < pen[ax]) ax = 1;
+          if (pen.z < pen[ax]) ax = 2;
+          nL = V<T>();
+          nL[ax] = local[ax] >= 0 ? T(1) : T(-1);
+          qL = local;
+          qL[ax] = nL[ax] * box.h[ax];
+        } else {
+          V<T> d = local - cl;
+          T dist = lenv(d);
+          V<T> nLT = d / std::max(dist, T(1e-9));
+          V<T> nWT = qrot(qBox, nLT);
+          V<T> vS = sIsA ? vA : vB, vX = sIsA ? vB : vA;
+          if (dist > r + detectMargin(nWT, vS - vX, r, sIsA ? cRad[ib] : cRad[ia])) return;
+          nL = nLT;
+          qL = cl;
+        }
+        V<T> nW = qrot(qBox, nL), qW = qrot(qBox, qL) + box.c, xS = sc - nW * r;
+        nrm = sIsA ? nW : -nW;
+        cs.push_back(Pt{sIsA ? xS : qW, sIsA ? qW : xS, 0});
+      }
+      V<T> t1, t2;
+      basis(nrm, t1, t2);
+      int prev = prevIndex();
+      bool roundA = (cFlags[ia] & 0x20) != 0, roundB = (cFlags[ib] & 0x20) != 0;
+      out.n = (uint32_t)cs.size();
+      out.flags = 1u | (roundA ? 2u : 0u) | (roundB ? 4u : 0u);
+      out.nrm = nrm; out.t1 = t1; out.dynF = dynF; out.statF = statF;
+      for (size_t i = 0; i < cs.size(); i++) {
+        V<T> rA_ = roundA ? cs[i].xA - pos[ba] : qrot(qconj(rot[ba]), cs[i].xA - pos[ba]);
+        V<T> rB_ = roundB ? cs[i].xB - pos[bb] : qrot(qconj(rot[bb]), cs[i].xB - pos[bb]);
+        V<T> lam, pen;
+        T stick = 0;
+        if (prev >= 0) {
+          const Manifold<T>& pm = prevManifolds[prev];
+          T bestD = T(0.5) * std::max(rA, rB);
+          int bj = -1;
+          for (uint32_t j = 0; j < pm.n; j++) {
+            T dj = lenv(pm.c[j].rA - rA_);
+            if (dj < bestD) { bestD = dj; bj = (int)j; }
+          }
+          if (bj >= 0) {
+            lam = pm.c[bj].lambda;
+            pen = pm.c[bj].penalty;
+            V<T> t2o = crossv(pm.nrm, pm.t1);
+            V<T> lt = pm.t1 * lam.y + t2o * lam.z;
+            lam.y = dotv(t1, lt);
+            lam.z = dotv(t2, lt);
+            bool pA = cRad[ia] < 0, pB = cRad[ib] < 0;
+            if ((pA || !roundA) && (pB || !roundB)) {
+              stick = pm.c[bj].stick;
+              if (stick != 0) { rA_ = pm.c[bj].rA; rB_ = pm.c[bj].rB; }
+            }
+          }
+        }
+        V<T> xAw = roundA ? pos[ba] + rA_ : xformv(pos[ba], rot[ba], rA_);
+        V<T> xBw = roundB ? pos[bb] + rB_ : xformv(pos[bb], rot[bb], rB_);
+        V<T> d = xAw - xBw;
+        Contact<T>& c = out.c[i];
+        c.rA = rA_; c.rB = rB_; c.feature = cs[i].f; c.stick = stick;
+        c.C0 = V<T>(dotv(nrm, d) + (T)set.collision_margin, dotv(t1, d), dotv(t2, d));
+        c.lambda = lam * warm;
+        c.penalty = warmPen(pen);
+      }
+      return;
+    }
+    // box-box SAT
+    Best bf{0, 0, 0, -std::numeric_limits<T>::max(), V<T>(), false}, be = bf;
+    T cap = specCap(cRad[ia], cRad[ib]);
+    bool sepd = false;
+    for (int i = 0; i < 3 && !sepd; i++) if (!testAxis(A, B, delta, A.ax[i], 0, i, -1, rv, cap, bf)) sepd = true;
+    for (int i = 0; i < 3 && !sepd; i++) if (!testAxis(A, B, delta, B.ax[i], 1, -1, i, rv, cap, bf)) sepd = true;
+    for (int i = 0; i < 3 && !sepd; i++)
+      for (int j = 0; j < 3 && !sepd; j++)
+        if (!testAxis(A, B, delta, crossv(A.ax[i], B.ax[j]), 2, i, j, rv, cap, be)) sepd = true;
+    if (sepd || !bf.valid) return;
+    Best best = bf;
+    if (be.valid && T(0.95) * be.sep > bf.sep + T(0.01)) best = be;
+    if (best.type == 2) {
+      V<T> a0, a1, b0, b1, xA, xB;
+      supportEdge(A, best.ia, best.n, a0, a1);
+      supportEdge(B, best.ib, -best.n, b0, b1);
+      segSeg(a0, a1, b0, b1, xA, xB);
+      uint32_t f = (2u << 24) | ((uint32_t)(best.ia & 0xFF) << 8) | (uint32_t)(best.ib & 0xFF);
+      addPt(cs, mids, xA, xB, f);
+      if (cs.empty()) addPt(cs, mids, support(A, best.n), support(B, -best.n), f);
+    } else {
+      bool refIsA = best.type == 0;
+      int ra = refIsA ? best.ia : best.ib;
+      const Box& ref = refIsA ? A : B;
+      const Box& inc = refIsA ? B : A;
+      V<T> ro = refIsA ? best.n : -best.n;
+      T sR = dotv(ro, ref.ax[ra]) >= 0 ? T(1) : T(-1);
+      V<T> rN = ref.ax[ra] * sR, rC = ref.c + rN * half(ref, ra);
+      V<T> rU, rV;
+      T eU, eV;
+      faceAxes(ref, ra, rU, rV, eU, eV);
+      int ia2 = 0;
+      T bd = -std::numeric_limits<T>::max();
+      for (int i = 0; i < 3; i++) { T dd = std::fabs(dotv(inc.ax[i], rN)); if (dd > bd) { bd = dd; ia2 = i; } }
+      T sI = dotv(inc.ax[ia2], rN) > 0 ? T(-1) : T(1);
+      V<T> iN = inc.ax[ia2] * sI, iC = inc.c + iN * half(inc, ia2);
+      V<T> iU, iV;
+      T fU, fV;
+      faceAxes(inc, ia2, iU, iV, fU, fV);
+      V<T> p0[16], p1[16];
+      p0[0] = iC + iU * fU + iV * fV;
+      p0[1] = iC - iU * fU + iV * fV;
+      p0[2] = iC - iU * fU - iV * fV;
+      p0[3] = iC + iU * fU - iV * fV;
+      int n = 4;
+      n = clip(p0, n, rU, dotv(rU, rC) + eU, p1);
+      n = clip(p1, n, -rU, dotv(-rU, rC) + eU, p0);
+      n = clip(p0, n, rV, dotv(rV, rC) + eV, p1);
+      n = clip(p1, n, -rV, dotv(-rV, rC) + eV, p0);
+      uint32_t prefix = ((refIsA ? 0u : 1u) << 24) | ((uint32_t)(ra & 0xFF) << 16) | ((uint32_t)(ia2 & 0xFF) << 8);
+      T slop = T(1e-5) + std::max(T(0), best.sep);
+      for (int i = 0; i < n && cs.size() < 8; i++) {
+        V<T> pI = p0[i];
+        T dd = dotv(pI - rC, rN);
+        if (dd > slop) continue;
+        V<T> pR = pI - rN * dd;
+        addPt(cs, mids, refIsA ? pR : pI, refIsA ? pI : pR, prefix | (uint32_t)(i & 0xFF));
+      }
+      if (cs.empty()) addPt(cs, mids, support(A, best.n), support(B, -best.n), prefix);
+    }
+    nrm = -best.n;
+    V<T> t1, t2;
+    basis(nrm, t1, t2);
+    int prev = prevIndex();
+    out.n = (uint32_t)cs.size();
+    out.flags = 1u;
+    out.nrm = nrm; out.t1 = t1; out.dynF = dynF; out.statF = statF;
+    for (size_t i = 0; i < cs.size(); i++) {
+      V<T> rA = qrot(qconj(rot[ba]), cs[i].xA - pos[ba]);
+      V<T> rB = qrot(qconj(rot[bb]), cs[i].xB - pos[bb]);
+      V<T> lam, pen;
+      T stick = 0;
+      if (prev >= 0) {
+        const Manifold<T>& pm = prevManifolds[prev];
+        for (uint32_t j = 0; j < pm.n; j++) {
+          if (pm.c[j].feature == cs[i].f) {
+            lam = pm.c[j].lambda; pen = pm.c[j].penalty; stick = pm.c[j].stick;
+            if (stick != 0) { rA = pm.c[j].rA; rB = pm.c[j].rB; }
+            break;
+          }
+        }
+      }
+      V<T> d = xformv(pos[ba], rot[ba], rA) - xformv(pos[bb], rot[bb], rB);
+      Contact<T>& c = out.c[i];
+      c.rA = rA; c.rB = rB; c.feature = cs[i].f; c.stick = stick;
+      c.C0 = V<T>(dotv(nrm, d) + (T)set.collision_margin, dotv(t1, d), dotv(t2, d));
+      c.lambda = lam * warm;
+      c.penalty = warmPen(pen);
+    }
+  }
+
+  // -------------------------------------------------------------------------
+  // warm start (40_solver.metal:546-722)
+  // -------------------------------------------------------------------------
+  static T twistOf(const Q<T>& qAr, const Q<T>& qB, const V<T>& axis) {
+    Q<T> r = qmul(qinv(qAr), qB);
+    if (r.w < 0) r = Q<T>{-r.x, -r.y, -r.z, -r.w};
+    return T(2) * std::atan2(dotv(V<T>(r.x, r.y, r.z), axis), r.w);
+  }
+  static T wrapPi(T a) { return a - T(6.2831853) * std::floor((a + T(3.14159265)) / T(6.2831853)); }
+
+  void warmstartConstraints() {
+    T warm = (T)set.alpha * (T)set.gamma, gam = set.gamma;
+    for (auto& sp : springs) {
+      if (!sp.hard) continue;
+      V<T> pA = xformv(pos[sp.a], rot[sp.a], sp.rA), pB = xformv(pos[sp.b], rot[sp.b], sp.rB);
+      sp.C0 = T(0.5) * std::max(lenv(pA - pB) - sp.rest, T(0));
+      sp.lambda *= warm;
+      sp.penalty = std::min(clampT(sp.penalty * gam, T(1), T(1e10)), sp.stiffness);
+    }
+    for (auto& j : joints) {
+      if (j.broken) continue;
+      V<T> pA = j.a == WORLD ? j.rA : xformv(pos[j.a], rot[j.a], j.rA);
+      V<T> pB = xformv(pos[j.b], rot[j.b], j.rB);
+      Q<T> qA = j.a == WORLD ? Q<T>{0, 0, 0, 1} : rot[j.a];
+      j.C0Lin = pA - pB;
+      if (j.hinge) {
+        V<T> aB = qrot(rot[j.b], j.axis), aA = qrot(qmul(qA, j.restRel), j.axis);
+        j.C0Ang = crossv(aA, aB) * j.torqueArm;
+        if (j.armature > 0) {
+          T tw = twistOf(qmul(qA, j.restRel), rot[j.b], j.axis);
+          V<T> wA = j.a == WORLD ? V<T>() : angVel[j.a];
+          j.predTwist = tw + (T)set.dt * dotv(angVel[j.b] - wA, aB);
+        }
+      } else {
+        j.C0Ang = qsub(qmul(qA, j.restRel), rot[j.b]) * j.torqueArm;
+      }
+      if (j.motorTorque > 0) {
+        j.motorZ = 0;
+        j.effort = 0;
+        uint32_t mode = j.flags & (3u << 4);
+        if (mode == (2u << 4) || mode == (3u << 4)) {
+          Q<T> qAr = j.a == WORLD ? j.restRel : qmul(rot[j.a], j.restRel);
+          T tw = twistOf(qAr, rot[j.b], j.axis);
+          V<T> axW = qrot(rot[j.b], j.axis);
+          V<T> wA = j.a == WORLD ? V<T>() : angVel[j.a];
+          T rate = dotv(angVel[j.b] - wA, axW);
+          if (mode == (3u << 4)) j.effort = clampT(j.kd * (j.motorTarget - rate), -j.motorTorque, j.motorTorque);
+          else j.effort = clampT(j.kp * (j.motorTarget - tw) - j.kd * rate, -j.motorTorque, j.motorTorque);
+        }
+      }
+      j.lambdaLin = j.lambdaLin * warm;
+      j.lambdaAng = j.lambdaAng * warm;
+      for (int k = 0; k < 3; k++) {
+        j.penaltyLin[k] = std::min(clampT(j.penaltyLin[k] * gam, T(1), T(1e10)), j.stiffLin);
+        j.penaltyAng[k] = std::min(clampT(j.penaltyAng[k] * gam, T(1), T(1e10)), j.stiffAng);
+      }
+    }
+  }
+
+  void warmstartBodies() {
+    T dt = set.dt, g0 = set.gravity;
+    for (uint32_t i = 0; i < nb; i++) {
+      T m = mass[i];
+      T g = g0 * gravityScale[i];
+      V<T> inert = pos[i] + vel[i] * dt;
+      if (m > 0) inert = inert + V<T>(0, 0, g) * (dt * dt);
+      inertLin[i] = inert;
+      inertAng[i] = qaddw(rot[i], angVel[i] * dt);
+      V<T> acc = (vel[i] - prevVel[i]) / dt;
+      T gs = g0 < 0 ? T(-1) : (g0 > 0 ? T(1) : T(0));
+      T w = clampT(acc.z * gs / std::fabs(g0), T(0), T(1));
+      if (!std::isfinite(w)) w = 0;
+      initLin[i] = pos[i];
+      initAng[i] = rot[i];
+      if (m > 0) {
+        pos[i] = pos[i] + vel[i] * dt + V<T>(0, 0, g) * (w * dt * dt);
+        rot[i] = inertAng[i];
+      }
+    }
+    snapLin = pos;
+    snapAng = rot;
+  }
+
+  // -------------------------------------------------------------------------
+  // per-body system (40_solver.metal:724-1360, 1384-1495)
+  // -------------------------------------------------------------------------
+  struct Sys {
+    M<T> ll, aa, cr;
+    V<T> gl, ga;
+  };
+  V<T> pL(uint32_t b, uint32_t self) const {
+    return (b != self && b != WORLD && colors[b] == colors[self] && dyn(b)) ? snapLin[b] : pos[b];
+  }
+  Q<T> pA(uint32_t b, uint32_t self) const {
+    return (b != self && b != WORLD && colors[b] == colors[self] && dyn(b)) ? snapAng[b] : rot[b];
+  }
+
+  void stampTet(const Tet<T>& t, uint32_t self, Sys& s) const {
+    V<T> x0 = pos[t.ids[0]], x1 = pos[t.ids[1]], x2 = pos[t.ids[2]], x3 = pos[t.ids[3]];
+    T muV = t.muV, lamV = t.lamV;
+    T aT = T(1) + muV / std::max(lamV, T(1e-9));
+    V<T> d0 = x1 - x0, d1 = x2 - x0, d2 = x3 - x0;
+    V<T> f[3];
+    for (int c = 0; c < 3; c++) f[c] = d0 * t.r[0][c] + d1 * t.r[1][c] + d2 * t.r[2][c];
+    V<T> j0 = crossv(f[1], f[2]), j1 = crossv(f[2], f[0]), j2 = crossv(f[0], f[1]);
+    T J = dotv(f[0], j0);
+    T s2 = lamV * (J - aT);
+    V<T> p0 = f[0] * muV + j0 * s2, p1 = f[1] * muV + j1 * s2, p2 = f[2] * muV + j2 * s2;
+    V<T> w;
+    if (self == t.ids[1]) w = t.r[0];
+    else if (self == t.ids[2]) w = t.r[1];
+    else if (self == t.ids[3]) w = t.r[2];
+    else w = -(t.r[0] + t.r[1] + t.r[2]);
+    s.gl = s.gl + p0 * w.x + p1 * w.y + p2 * w.z;
+    V<T> g = j0 * w.x + j1 * w.y + j2 * w.z;
+    s.ll = s.ll + M<T>::diag(V<T>(1, 1, 1) * (muV * dotv(w, w))) + M<T>::outer(g, g) * lamV;
+  }
+
+  V<T> contactC(const Manifold<T>& m, int i, const V<T>& dqAL, const V<T>& dqAA, const V<T>& dqBL, const V<T>& dqBA,
+                const V<T>& rAW, const V<T>& rBW, T& fs, T& bnd, V<T>& F) const {
+    V<T> n = m.nrm, t1 = m.t1, t2 = crossv(n, t1);
+    const Contact<T>& c = m.c[i];
+    V<T> C = c.C0 * (T(1) - (T)set.alpha);
+    C = C + V<T>(dotv(n, dqAL), dotv(t1, dqAL), dotv(t2, dqAL)) - V<T>(dotv(n, dqBL), dotv(t1, dqBL), dotv(t2, dqBL));
+    C = C + V<T>(dotv(crossv(rAW, n), dqAA), dotv(crossv(rAW, t1), dqAA), dotv(crossv(rAW, t2), dqAA));
+    C = C - V<T>(dotv(crossv(rBW, n), dqBA), dotv(crossv(rBW, t1), dqBA), dotv(crossv(rBW, t2), dqBA));
+    F = mulv(c.penalty, C) + c.lambda;
+    F.x = std::min(F.x, T(0));
+    bnd = std::fabs(F.x) * m.statF;
+    fs = std::sqrt(F.y * F.y + F.z * F.z);
+    if (fs > bnd && fs > 0) {
+      T dynB = std::fabs(F.x) * m.dynF;
+      F.y = F.y * (dynB / fs);
+      F.z = F.z * (dynB / fs);
+    }
+    return C;
+  }
+
+  void stampManifold(const Manifold<T>& m, uint32_t self, Sys& s) const {
+    bool isA = self == m.a;
+    bool rAr = (m.flags & 2u) != 0, rBr = (m.flags & 4u) != 0;
+    V<T> pAL = pL(m.a, self), pBL = pL(m.b, self);
+    Q<T> qA = pA(m.a, self), qB = pA(m.b, self);
+    V<T> dqAL = pAL - initLin[m.a], dqBL = pBL - initLin[m.b];
+    V<T> dqAA = qsub(qA, initAng[m.a]), dqBA = qsub(qB, initAng[m.b]);
+    V<T> n = m.nrm, t1 = m.t1, t2 = crossv(n, t1);
+    T sg = isA ? T(1) : T(-1);
+    for (uint32_t i = 0; i < m.n; i++) {
+      V<T> rAW = rAr ? m.c[i].rA : qrot(qA, m.c[i].rA);
+      V<T> rBW = rBr ? m.c[i].rB : qrot(qB, m.c[i].rB);
+      T fs, bnd;
+      V<T> F;
+      contactC(m, (int)i, dqAL, dqAA, dqBL, dqBA, rAW, rBW, fs, bnd, F);
+      V<T> jl[3] = {n * sg, t1 * sg, t2 * sg};
+      V<T> rW = isA ? rAW : rBW;
+      V<T> K = m.c[i].penalty;
+      for (int k = 0; k < 3; k++) {
+        V<T> ja = crossv(rW, jl[k]);
+        s.ll = s.ll + M<T>::outer(jl[k], jl[k]) * K[k];
+        s.aa = s.aa + M<T>::outer(ja, ja) * K[k];
+        s.cr = s.cr + M<T>::outer(ja, jl[k]) * K[k];
+        s.gl = s.gl + jl[k] * F[k];
+        s.ga = s.ga + ja * F[k];
+      }
+    }
+  }
+
+  static M<T> geomBS(int k, const V<T>& v) {
+    M<T> m = M<T>::diag(V<T>(-v[k], -v[k], -v[k]));
+    for (int r = 0; r < 3; r++) m.a[r][k] += v[r];
+    return m;
+  }
+  void stampJoint(const Joint<T>& j, uint32_t self, Sys& s) const {
+    bool isA = self == j.a;
+    T alpha = set.alpha, dt = set.dt;
+    if (dotv(j.penaltyLin, j.penaltyLin) > 0) {
+      V<T> pAw = j.a == WORLD ? j.rA : xformv(pos[j.a], rot[j.a], j.rA);
+      V<T> pBw = xformv(pos[j.b], rot[j.b], j.rB);
+      V<T> C = pAw - pBw;
+      if (j.flags & 1u) C = C - j.C0Lin * alpha;
+      V<T> F = mulv(j.penaltyLin, C) + j.lambdaLin;
+      T js = isA ? T(1) : T(-1);
+      V<T> rW = isA ? qrot(rot[j.a], j.rA) : qrot(rot[j.b], j.rB);
+      M<T> jAng = skew(-(rW * js)), jAngT = jAng.tr(), K = M<T>::diag(j.penaltyLin), jAngTk = jAngT.mul(K);
+      s.ll = s.ll + K;
+      s.aa = s.aa + jAngTk.mul(jAng);
+      s.cr = s.cr + jAngTk * js;
+      V<T> r = rW * js;
+      M<T> H = geomBS(0, r) * F.x + geomBS(1, r) * F.y + geomBS(2, r) * F.z;
+      V<T> dg;
+      for (int c = 0; c < 3; c++) dg[c] = std::sqrt(H.a[0][c] * H.a[0][c] + H.a[1][c] * H.a[1][c] + H.a[2][c] * H.a[2][c]);
+      s.aa = s.aa + M<T>::diag(dg);
+      s.gl = s.gl + F * js;
+      s.ga = s.ga + jAngT.mulv(F);
+    }
+    if (dotv(j.penaltyAng, j.penaltyAng) > 0) {
+      Q<T> qA = j.a == WORLD ? Q<T>{0, 0, 0, 1} : rot[j.a];
+      Q<T> qAr = qmul(qA, j.restRel);
+      if (j.hinge) {
+        V<T> aB = qrot(rot[j.b], j.axis), aA = qrot(qAr, j.axis);
+        V<T> C = crossv(aA, aB) * j.torqueArm;
+        if (j.flags & 2u) C = C - j.C0Ang * alpha;
+        V<T> F = mulv(j.penaltyAng, C) + j.lambdaAng;
+        T al = dotv(aA, aB);
+        M<T> I3 = M<T>::diag(V<T>(1, 1, 1));
+        M<T> Jm = isA ? M<T>::outer(aA, aB) + I3 * (-al) : I3 * al + M<T>::outer(aB, aA) * T(-1);
+        Jm = Jm * j.torqueArm;
+        M<T> JT = Jm.tr();
+        s.aa = s.aa + JT.mul(M<T>::diag(j.penaltyAng)).mul(Jm);
+        s.ga = s.ga + JT.mulv(F);
+        if (j.motorTorque > 0 || j.limLo < j.limHi) {
+          T tw = twistOf(qAr, rot[j.b], j.axis);
+          T sm = isA ? T(-1) : T(1);
+          if (j.armature > 0) {
+            T Ca = wrapPi(tw - j.predTwist);
+            T ka = j.armature / std::max(dt * dt, T(1e-12));
+            s.aa = s.aa + M<T>::outer(aB, aB) * ka;
+            s.ga = s.ga + aB * (ka * Ca * sm);
+          }
+          if (j.limLo < j.limHi) {
+            T over = std::max(tw - j.limHi, T(0)) + std::min(tw - j.limLo, T(0));
+            if (over != 0) {
+              T kL = 4.0e4;
+              T FL = clampT(kL * over, T(-3000), T(3000));
+              s.aa = s.aa + M<T>::outer(aB, aB) * kL;
+              s.ga = s.ga + aB * (FL * sm);
+            }
+          }
+          if (j.motorTorque > 0) {
+            uint32_t mode = j.flags & (3u << 4);
+            if (mode == (2u << 4) || mode == (3u << 4)) {
+              s.ga = s.ga + aB * (-j.effort * sm);
+            } else {
+              T Cm = wrapPi(tw - j.motorTarget);
+              Q<T> qA0 = j.a == WORLD ? Q<T>{0, 0, 0, 1} : initAng[j.a];
+              T tw0 = twistOf(qmul(qA0, j.restRel), initAng[j.b], j.axis);
+              T dT = wrapPi(tw - tw0);
+              T cD = j.kd / std::max(dt, T(1e-6));
+              T raw = j.kp * Cm + cD * dT;
+              T eff = clampT(raw, -j.motorTorque, j.motorTorque);
+              if (std::fabs(raw) < j.motorTorque) s.aa = s.aa + M<T>::outer(aB, aB) * (j.kp + cD);
+              s.ga = s.ga + aB * (eff * sm);
+            }
+          }
+        }
+      } else {
+        V<T> C = qsub(qAr, rot[j.b]) * j.torqueArm;
+        T sg = (isA ? T(1) : T(-1)) * j.torqueArm;
+        if (j.flags & 2u) C = C - j.C0Ang * alpha;
+        V<T> F = mulv(j.penaltyAng, C) + j.lambdaAng;
+        s.aa = s.aa + M<T>::diag(j.penaltyAng * (sg * sg));
+        s.ga = s.ga + F * sg;
+      }
+    }
+  }
+
+  void stampSpring(const Spring<T>& sp, uint32_t self, Sys& s) const {
+    V<T> pAw = xformv(pos[sp.a], rot[sp.a], sp.rA), pBw = xformv(pos[sp.b], rot[sp.b], sp.rB);
+    V<T> d = pAw - pBw;
+    T L = lenv(d);
+    if (L <= T(1e-6)) return;
+    V<T> n = d / L;
+    T k = sp.stiffness, f;
+    if (sp.hard) {
+      T C = L - sp.rest - sp.C0 * (T)set.alpha;
+      k = sp.penalty;
+      f = k * C + sp.lambda;
+      if (f <= 0) return;
+    } else {
+      f = k * (L - sp.rest);
+    }
+    bool isA = self == sp.a;
+    V<T> rW = isA ? qrot(rot[sp.a], sp.rA) : qrot(rot[sp.b], sp.rB);
+    V<T> jl = isA ? n : -n;
+    V<T> ja = isA ? crossv(rW, n) : -crossv(rW, n);
+    s.ll = s.ll + M<T>::outer(jl, jl) * k;
+    s.aa = s.aa + M<T>::outer(ja, ja) * k;
+    s.cr = s.cr + M<T>::outer(ja, jl) * k;
+    s.gl = s.gl + jl * f;
+    s.ga = s.ga + ja * f;
+    T ft = std::max(f, T(0)) / L;
+    if (ft > 0) s.ll = s.ll + M<T>::diag(V<T>(ft, ft, ft)) + M<T>::outer(n, n) * (-ft);
+  }
+
+  // contact adjacency of this frame: per body, active manifold indices in order
+  std::vector<std::vector<uint32_t>> contactAdj;
+
+  Sys assemble(uint32_t body) const {
+    Sys s{M<T>::zero(), M<T>::zero(), M<T>::zero(), V<T>(), V<T>()};
+    T dt2 = (T)set.dt * (T)set.dt;
+    T m = mass[body];
+    s.ll = M<T>::diag(V<T>(1, 1, 1) * (m / dt2));
+    M<T> In = worldInertia(rot[body], moment[body]) * (T(1) / dt2);
+    s.aa = In;
+    s.gl = (pos[body] - inertLin[body]) * (m / dt2);
+    s.ga = In.mulv(qsub(rot[body], inertAng[body]));
+    // reference order of the sorted CSR: joints, springs, manifolds, tets
+    for (uint32_t e : staticAdj[body]) {
+      uint32_t k = e >> 28, i = e & 0x0FFFFFFFu;
+      if (k == FKJ && !joints[i].broken) stampJoint(joints[i], body, s);
+      else if (k == FKS) stampSpring(springs[i], body, s);
+    }
+    for (uint32_t mi : contactAdj[body]) stampManifold(manifolds[mi], body, s);
+    for (uint32_t e : staticAdj[body])
+      if ((e >> 28) == FKT) stampTet(tets[e & 0x0FFFFFFFu], body, s);
+    return s;
+  }
+
+  void primal(uint32_t body) {
+    Sys s = assemble(body);
+    bool part = radiusSigned[body] < 0;
+    if (part) { s.aa = M<T>::diag(V<T>(1, 1, 1)); s.cr = M<T>::zero(); s.ga = V<T>(); }
+    T A[6][6], b[6], x[6];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        A[i][j] = s.ll.a[i][j];
+        A[3 + i][3 + j] = s.aa.a[i][j];
+        A[3 + i][j] = s.cr.a[i][j];
+        A[j][3 + i] = s.cr.a[i][j];
+      }
+    for (int i = 0; i < 3; i++) { b[i] = -s.gl[i]; b[3 + i] = -s.ga[i]; }
+    solve6(A, b, x);
+    V<T> dl(x[0], x[1], x[2]), da(x[3], x[4], x[5]);
+    for (int i = 0; i < 6; i++) if (!std::isfinite(x[i])) return;
+    T maxLin = T(0.35) * std::fabs(radiusSigned[body]);
+    T l2 = dotv(dl, dl);
+    if (l2 > maxLin * maxLin) dl = dl * (maxLin / std::sqrt(l2));
+    T a2 = dotv(da, da);
+    if (a2 > T(0.25)) da = da * (T(0.5) / std::sqrt(a2));
+    pos[body] = pos[body] + dl;
+    if (!part) rot[body] = qaddw(rot[body], da);
+  }
+
+  // -------------------------------------------------------------------------
+  // dual (40_solver.metal:1689-1910)
+  // -------------------------------------------------------------------------
+  void dual() {
+    T alpha = set.alpha, bl = set.beta_lin, ba = set.beta_ang, lmax = set.lambda_max;
+    for (auto& j : joints) {
+      if (j.broken) continue;
+      if (dotv(j.penaltyLin, j.penaltyLin) > 0) {
+        V<T> pAw = j.a == WORLD ? j.rA : xformv(pos[j.a], rot[j.a], j.rA);
+        V<T> C = pAw - xformv(pos[j.b], rot[j.b], j.rB);
+        if (j.flags & 1u) {
+          C = C - j.C0Lin * alpha;
+          V<T> l = mulv(j.penaltyLin, C) + j.lambdaLin;
+          j.lambdaLin = V<T>(clampT(l.x, -lmax, lmax), clampT(l.y, -lmax, lmax), clampT(l.z, -lmax, lmax));
+        }
+        T cap = std::min(j.stiffLin, T(1e10));
+        for (int k = 0; k < 3; k++) j.penaltyLin[k] = std::min(j.penaltyLin[k] + std::fabs(C[k]) * bl, cap);
+      }
+      if (dotv(j.penaltyAng, j.penaltyAng) > 0) {
+        Q<T> qA = j.a == WORLD ? Q<T>{0, 0, 0, 1} : rot[j.a];
+        V<T> C;
+        if (j.hinge) C = crossv(qrot(qmul(qA, j.restRel), j.axis), qrot(rot[j.b], j.axis)) * j.torqueArm;
+        else C = qsub(qmul(qA, j.restRel), rot[j.b]) * j.torqueArm;
+        if (j.flags & 2u) {
+          C = C - j.C0Ang * alpha;
+          V<T> l = mulv(j.penaltyAng, C) + j.lambdaAng;
+          j.lambdaAng = V<T>(clampT(l.x, -lmax, lmax), clampT(l.y, -lmax, lmax), clampT(l.z, -lmax, lmax));
+        }
+        T cap = std::min(j.stiffAng, T(1e10));
+        for (int k = 0; k < 3; k++) j.penaltyAng[k] = std::min(j.penaltyAng[k] + std::fabs(C[k]) * ba, cap);
+      }
+      T lin2 = (j.flags & 8u) ? dotv(j.lambdaLin, j.lambdaLin) : T(0);
+      if ((j.flags & 4u) && dotv(j.lambdaAng, j.lambdaAng) + lin2 > j.fracture * j.fracture) {
+        j.penaltyLin = j.penaltyAng = j.lambdaLin = j.lambdaAng = V<T>();
+        j.broken = 1;
+      }
+    }
+    for (auto& sp : springs) {
+      if (!sp.hard) continue;
+      T C = lenv(xformv(pos[sp.a], rot[sp.a], sp.rA) - xformv(pos[sp.b], rot[sp.b], sp.rB)) - sp.rest - sp.C0 * alpha;
+      T big = std::numeric_limits<T>::max();
+      T mm = std::min(mass[sp.a] > 0 ? mass[sp.a] : big, mass[sp.b] > 0 ? mass[sp.b] : big);
+      T cap = std::min(lmax, std::max(T(2), T(5e3) * mm));
+      sp.lambda = clampT(T(0.98) * sp.lambda + sp.penalty * C, T(0), cap);
+      if (C > 0) sp.penalty = std::min(sp.penalty + C * bl, std::min(sp.stiffness, T(1e10)));
+    }
+    for (auto& m : manifolds) {
+      if (m.n == 0) continue;
+      bool rAr = (m.flags & 2u) != 0, rBr = (m.flags & 4u) != 0;
+      V<T> dqAL = pos[m.a] - initLin[m.a], dqBL = pos[m.b] - initLin[m.b];
+      V<T> dqAA = qsub(rot[m.a], initAng[m.a]), dqBA = qsub(rot[m.b], initAng[m.b]);
+      T big = std::numeric_limits<T>::max();
+      T mm = std::min(mass[m.a] > 0 ? mass[m.a] : big, mass[m.b] > 0 ? mass[m.b] : big);
+      T cap = std::min(lmax, std::max(T(10), T(1e5) * mm));
+      for (uint32_t i = 0; i < m.n; i++) {
+        V<T> rAW = rAr ? m.c[i].rA : qrot(rot[m.a], m.c[i].rA);
+        V<T> rBW = rBr ? m.c[i].rB : qrot(rot[m.b], m.c[i].rB);
+        T fs, bnd;
+        V<T> F;
+        V<T> C = contactC(m, (int)i, dqAL, dqAA, dqBL, dqBA, rAW, rBW, fs, bnd, F);
+        F.x = std::max(F.x, -cap);
+        m.c[i].lambda = F;
+        V<T> p = m.c[i].penalty;
+        if (F.x < 0) p.x = std::min(p.x + bl * std::fabs(C.x), T(1e10));
+        if (fs <= bnd) {
+          p.y = std::min(p.y + bl * std::fabs(C.y), T(1e6));
+          p.z = std::min(p.z + bl * std::fabs(C.z), T(1e6));
+          m.c[i].stick = std::sqrt(C.y * C.y + C.z * C.z) < T(1e-5) ? T(1) : T(0);
+        }
+        m.c[i].penalty = p;
+      }
+    }
+    snapLin = pos;
+    snapAng = rot;
+  }
+
+  // -------------------------------------------------------------------------
+  // dynamic coloring (40_solver.metal:320-540)
+  // -------------------------------------------------------------------------
+  void neighbors(uint32_t v, std::vector<uint32_t>& out) const {
+    out.clear();
+    for (uint32_t e : staticAdj[v]) {
+      uint32_t k = e >> 28, i = e & 0x0FFFFFFFu;
+      if (k == FKJ) out.push_back(joints[i].a == v ? joints[i].b : joints[i].a);
+      else if (k == FKS) out.push_back(springs[i].a == v ? springs[i].b : springs[i].a);
+      else if (k == FKT) for (int q = 0; q < 4; q++) if (tets[i].ids[q] != v) out.push_back(tets[i].ids[q]);
+    }
+    for (uint32_t mi : contactAdj[v]) out.push_back(manifolds[mi].a == v ? manifolds[mi].b : manifolds[mi].a);
+  }
+  void dynamicColors() {
+    std::vector<uint32_t> src = colors, dst(nb, 0), nbrs;
+    std::vector<uint8_t> changed(20, 0);
+    for (int pass = 0; pass < 20; pass++) {
+      if (pass > 0 && !changed[pass - 1]) { std::swap(src, dst); continue; }
+      for (uint32_t v = 0; v < nb; v++) {
+        if (!(mass[v] > 0)) { dst[v] = 0; continue; }
+        uint32_t my = src[v];
+        uint64_t lower = 0, all = 0;
+        bool conflict = false;
+        neighbors(v, nbrs);
+        for (uint32_t n : nbrs) {
+          if (!dyn(n)) continue;
+          uint64_t bit = src[n] < 64 ? 1ull << src[n] : 0;
+          all |= bit;
+          if (n < v) { lower |= bit; if (src[n] == my) conflict = true; }
+        }
+        if (conflict) {
+          uint64_t fr = ~lower;
+          dst[v] = fr ? std::min<uint32_t>((uint32_t)__builtin_ctzll(fr), 63u) : 63u;
+          changed[pass] = 1;
+        } else {
+          uint64_t fr = ~all;
+          uint32_t best = fr ? (uint32_t)__builtin_ctzll(fr) : my;
+          dst[v] = best < my ? best : my;
+          if (best < my) changed[pass] = 1;
+        }
+      }
+      std::swap(src, dst);
+    }
+    // src holds the final pass output (20 passes: back in the first buffer)
+    if (changed[19]) {
+      for (uint32_t v = 0; v < nb; v++) {
+        if (!(mass[v] > 0)) { src[v] = 0; continue; }
+        uint64_t lower = 0;
+        neighbors(v, nbrs);
+        for (uint32_t n : nbrs) if (dyn(n) && n < v && src[n] < 64) lower |= 1ull << src[n];
+        uint64_t fr = ~lower;
+        src[v] = fr ? (uint32_t)__builtin_ctzll(fr) : 63u;
+      }
+    }
+    colors = src;
+    colorConflicts = 0;
+    for (uint32_t v = 0; v < nb; v++) {
+      if (!(mass[v] > 0)) continue;
+      neighbors(v, nbrs);
+      for (uint32_t n : nbrs) if (dyn(n) && n < v && colors[n] == colors[v]) { colorConflicts++; break; }
+    }
+    rebuildBuckets();
+  }
+
+  // -------------------------------------------------------------------------
+  // the step (GPUSolver.swift:3138-4405 order)
+  // -------------------------------------------------------------------------
+  void step() {
+    frame++;
+    broadphase();
+    manifolds.assign(pairs.size(), Manifold<T>{});
+    for (size_t i = 0; i < pairs.size(); i++) collide((uint32_t)i, manifolds[i]);
+    warmstartConstraints();
+    warmstartBodies();
+    contactAdj.assign(nb, {});
+    activeManifolds = 0;
+    for (uint32_t i = 0; i < manifolds.size(); i++) {
+      const Manifold<T>& m = manifolds[i];
+      if (m.n == 0) continue;
+      activeManifolds++;
+      if (dyn(m.a)) contactAdj[m.a].push_back(i);
+      if (dyn(m.b)) contactAdj[m.b].push_back(i);
+    }
+    if (dynamicColoring) dynamicColors();
+    for (int it = 0; it < set.iterations; it++) {
+      for (int c = 0; c < 64; c++)
+        for (uint32_t b : buckets[c]) primal(b);
+      dual();
+    }
+    // BDF1 finalize (40_solver.metal:2108-2150)
+    T dt = set.dt;
+    T maxSpeed = std::max(T(30), T(1.5) * (T)cellSize / dt * T(0.5));
+    for (uint32_t i = 0; i < nb; i++) {
+      V<T> vOld = vel[i];
+      prevVel[i] = vOld;
+      if (!(mass[i] > 0)) continue;
+      V<T> v = (pos[i] - initLin[i]) / dt;
+      T s2 = dotv(v, v);
+      if (s2 > maxSpeed * maxSpeed) v = v * (maxSpeed / std::sqrt(s2));
+      if (radiusSigned[i] < 0 && set.particle_damping > 0) {
+        V<T> bal = vOld + V<T>(0, 0, (T)set.gravity * gravityScale[i]) * dt;
+        v = bal + (v - bal) / (T(1) + (T)set.particle_damping * dt);
+      }
+      if (radiusSigned[i] >= 0 && set.rigid_linear_damping > 0) v = v * std::exp(-(T)set.rigid_linear_damping * dt);
+      V<T> om = qsub(rot[i], initAng[i]) / dt;
+      if (radiusSigned[i] >= 0 && set.rigid_angular_damping > 0) om = om * std::exp(-(T)set.rigid_angular_damping * dt);
+      vel[i] = v;
+      angVel[i] = om;
+    }
+    // persistence map of this frame for the next one
+    prevManifolds = manifolds;
+    prevMap.clear();
+    for (uint32_t i = 0; i < prevManifolds.size(); i++)
+      if (prevManifolds[i].n) prevMap[{prevManifolds[i].ca, prevManifolds[i].cb}] = (int)i;
+  }
+
+  double residual() const {
+    double g2 = 0, w2 = 0;
+    // evaluated with live poses only (no same-color snapshot)
+    Oracle<T>* self = const_cast<Oracle<T>*>(this);
+    std::vector<V<T>> sl = snapLin;
+    std::vector<Q<T>> sa = snapAng;
+    self->snapLin = pos;
+    self->snapAng = rot;
+    for (uint32_t b = 0; b < nb; b++) {
+      if (!(mass[b] > 0)) continue;
+      Sys s = assemble(b);
+      g2 += (double)dotv(s.gl, s.gl);
+      if (radiusSigned[b] >= 0) g2 += (double)dotv(s.ga, s.ga);
+      double mg = (double)mass[b] * (double)set.gravity * (double)gravityScale[b];
+      w2 += mg * mg;
+    }
+    self->snapLin = sl;
+    self->snapAng = sa;
+    return w2 > 0 ? std::sqrt(g2 / w2) : std::sqrt(g2);
+  }
+
+  void setStates(const uint32_t* idx, uint32_t n, const float* p, const float* r, const float* v, const float* w) {
+    std::vector<uint8_t> flag(nb, 0);
+    for (uint32_t k = 0; k < n; k++) {
+      uint32_t i = idx ? idx[k] : k;
+      pos[i] = V<T>(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+      rot[i] = Q<T>{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
+      vel[i] = V<T>(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+      angVel[i] = V<T>(w[3 * k], w[3 * k + 1], w[3 * k + 2]);
+      prevVel[i] = vel[i];
+      flag[i] = 1;
+    }
+    auto f = [&](uint32_t b) { return b != WORLD && flag[b]; };
+    for (auto& j : joints)
+      if (f(j.a) || f(j.b)) {
+        j.lambdaLin = j.lambdaAng = V<T>();
+        j.motorZ = 0; j.predTwist = 0; j.effort = 0;
+        j.penaltyLin = j.penLin0;
+        j.penaltyAng = j.penAng0;
+      }
+    for (auto& s : springs) if (f(s.a) || f(s.b)) { s.lambda = 0; s.penalty = 0; s.C0 = 0; }
+    // incident manifolds (both frame buffers) become cold
+    for (auto& m : prevManifolds) if (f(m.a) || f(m.b)) m = Manifold<T>{};
+    for (auto& m : manifolds) if (f(m.a) || f(m.b)) m = Manifold<T>{};
+  }
+};
+
+struct oracle_solver {
+  int precision;
+  Oracle<float>* f = nullptr;
+  Oracle<double>* d = nullptr;
+};
+
+extern "C" {
+
+int oracle_create(const avbd_scene_desc* scene, int precision, int maxPairsPerBody, oracle_solver** out) {
+  if (!scene || !out) return 1;
+  if (scene->num_tris) return 4;
+  oracle_solver* o = new oracle_solver();
+  o->precision = precision == 64 ? 64 : 32;
+  int rc;
+  if (o->precision == 64) { o->d = new Oracle<double>(); rc = o->d->init(scene, maxPairsPerBody > 0 ? maxPairsPerBody : 16); }
+  else { o->f = new Oracle<float>(); rc = o->f->init(scene, maxPairsPerBody > 0 ? maxPairsPerBody : 16); }
+  if (rc) { oracle_destroy(o); *out = nullptr; return rc; }
+  *out = o;
+  return 0;
+}
+
+void oracle_destroy(oracle_solver* o) {
+  if (!o) return;
+  delete o->f;
+  delete o->d;
+  delete o;
+}
+
+int oracle_step(oracle_solver* o) {
+  if (o->f) o->f->step(); else o->d->step();
+  return 0;
+}
+
+int oracle_set_settings(oracle_solver* o, const avbd_settings* s) {
+  if (o->f) o->f->set = *s; else o->d->set = *s;
+  return 0;
+}
+
+}  // extern "C"
+
+template <typename T>
+static void getStates(Oracle<T>* s, uint32_t n, float* p, float* r, float* v, float* w) {
+  for (uint32_t i = 0; i < n && i < s->nb; i++) {
+    if (p) { p[3 * i] = (float)s->pos[i].x; p[3 * i + 1] = (float)s->pos[i].y; p[3 * i + 2] = (float)s->pos[i].z; }
+    if (r) { r[4 * i] = (float)s->rot[i].x; r[4 * i + 1] = (float)s->rot[i].y; r[4 * i + 2] = (float)s->rot[i].z; r[4 * i + 3] = (float)s->rot[i].w; }
+    if (v) { v[3 * i] = (float)s->vel[i].x; v[3 * i + 1] = (float)s->vel[i].y; v[3 * i + 2] = (float)s->vel[i].z; }
+    if (w) { w[3 * i] = (float)s->angVel[i].x; w[3 * i + 1] = (float)s->angVel[i].y; w[3 * i + 2] = (float)s->angVel[i].z; }
+  }
+}
+template <typename T>
+static void getStates64(Oracle<T>* s, uint32_t n, double* p, double* r, double* v, double* w) {
+  for (uint32_t i = 0; i < n && i < s->nb; i++) {
+    if (p) { p[3 * i] = s->pos[i].x; p[3 * i + 1] = s->pos[i].y; p[3 * i + 2] = s->pos[i].z; }
+    if (r) { r[4 * i] = s->rot[i].x; r[4 * i + 1] = s->rot[i].y; r[4 * i + 2] = s->rot[i].z; r[4 * i + 3] = s->rot[i].w; }
+    if (v) { v[3 * i] = s->vel[i].x; v[3 * i + 1] = s->vel[i].y; v[3 * i + 2] = s->vel[i].z; }
+    if (w) { w[3 * i] = s->angVel[i].x; w[3 * i + 1] = s->angVel[i].y; w[3 * i + 2] = s->angVel[i].z; }
+  }
+}
+
+extern "C" {
+
+int oracle_get_body_states(oracle_solver* o, uint32_t n, float* p, float* r, float* v, float* w) {
+  if (o->f) getStates(o->f, n, p, r, v, w); else getStates(o->d, n, p, r, v, w);
+  return 0;
+}
+int oracle_get_body_states_f64(oracle_solver* o, uint32_t n, double* p, double* r, double* v, double* w) {
+  if (o->f) getStates64(o->f, n, p, r, v, w); else getStates64(o->d, n, p, r, v, w);
+  return 0;
+}
+int oracle_set_body_states(oracle_solver* o, const uint32_t* idx, uint32_t n, const float* p, const float* r,
+                           const float* v, const float* w) {
+  if (o->f) o->f->setStates(idx, n, p, r, v, w); else o->d->setStates(idx, n, p, r, v, w);
+  return 0;
+}
+int oracle_get_colors(oracle_solver* o, uint32_t n, uint32_t* c) {
+  const std::vector<uint32_t>& col = o->f ? o->f->colors : o->d->colors;
+  for (uint32_t i = 0; i < n && i < col.size(); i++) c[i] = col[i];
+  return 0;
+}
+int oracle_get_cell_lists(oracle_solver* o, uint32_t* hs, uint32_t* nh, float* cs, uint32_t* start, uint32_t* count,
+                          uint32_t* bodies) {
+  auto go = [&](auto* s) {
+    if (hs) *hs = s->hashSize;
+    if (nh) *nh = (uint32_t)s->hashed.size();
+    if (cs) *cs = s->cellSize;
+    if (start) for (size_t i = 0; i < s->cellStart.size(); i++) start[i] = s->cellStart[i];
+    if (count) for (size_t i = 0; i < s->cellCount.size(); i++) count[i] = s->cellCount[i];
+    if (bodies) for (size_t i = 0; i < s->cellBodies.size(); i++) bodies[i] = s->cellBodies[i];
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_get_pairs(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* pairs) {
+  auto go = [&](auto* s) {
+    *n = (uint32_t)s->pairs.size();
+    for (uint32_t i = 0; i < *n && i < cap; i++) { pairs[2 * i] = s->pairs[i].first; pairs[2 * i + 1] = s->pairs[i].second; }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* h3, float* n3, float* lp) {
+  auto go = [&](auto* s) {
+    const auto& ms = s->prevManifolds;
+    *n = (uint32_t)ms.size();
+    for (uint32_t i = 0; i < *n && i < cap; i++) {
+      if (h3) { h3[3 * i] = ms[i].a; h3[3 * i + 1] = ms[i].b; h3[3 * i + 2] = ms[i].n; }
+      if (n3) { n3[3 * i] = (float)ms[i].nrm.x; n3[3 * i + 1] = (float)ms[i].nrm.y; n3[3 * i + 2] = (float)ms[i].nrm.z; }
+      if (lp) { lp[2 * i] = ms[i].n ? (float)ms[i].c[0].lambda.x : 0.0f; lp[2 * i + 1] = ms[i].n ? (float)ms[i].c[0].penalty.x : 0.0f; }
+    }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_residual(oracle_solver* o, double* out) {
+  *out = o->f ? o->f->residual() : o->d->residual();
+  return 0;
+}
+int oracle_stats(oracle_solver* o, int32_t* np, int32_t* am, int32_t* nc, int32_t* pc, int32_t* cc) {
+  auto go = [&](auto* s) {
+    if (np) *np = (int32_t)s->pairs.size();
+    if (am) *am = (int32_t)s->activeManifolds;
+    if (pc) *pc = (int32_t)s->pairCandidates;
+    if (cc) *cc = (int32_t)s->colorConflicts;
+    if (nc) {
+      int used = 0;
+      for (int c = 0; c < 64; c++) if (!s->buckets[c].empty()) used = c + 1;
+      *nc = used;
+    }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_map_find(oracle_solver* o, const uint32_t* pairs, uint32_t n, int32_t* out) {
+  auto go = [&](auto* s) {
+    for (uint32_t i = 0; i < n; i++) {
+      auto it = s->prevMap.find({pairs[2 * i], pairs[2 * i + 1]});
+      out[i] = it == s->prevMap.end() ? -1 : it->second;
+    }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+
+// Random.swift:3-21
+void oracle_splitmix64(uint64_t seed, uint32_t n, uint64_t* out, float* outf) {
+  uint64_t s = seed;
+  for (uint32_t i = 0; i < n; i++) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    if (out) out[i] = z;
+    if (outf) outf[i] = (float)(z >> 40) / (float)(1u << 24);
+  }
+}
+
+// 40_solver.metal:965-1020 in double, plus the SNH energy for gradient checks
+void oracle_snh_tet(const double* x, const double* dm, double mu, double lam, double vol, int self, double* energy,
+                    double* grad3, double* hess9) {
+  V<double> X[4];
+  for (int i = 0; i < 4; i++) X[i] = V<double>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+  V<double> r[3] = {V<double>(dm[0], dm[1], dm[2]), V<double>(dm[3], dm[4], dm[5]), V<double>(dm[6], dm[7], dm[8])};
+  double muV = mu * vol, lamV = lam * vol, aT = 1.0 + muV / std::max(lamV, 1e-9);
+  V<double> d0 = X[1] - X[0], d1 = X[2] - X[0], d2 = X[3] - X[0];
+  V<double> f[3];
+  for (int c = 0; c < 3; c++) f[c] = d0 * r[0][c] + d1 * r[1][c] + d2 * r[2][c];
+  V<double> j0 = crossv(f[1], f[2]), j1 = crossv(f[2], f[0]), j2 = crossv(f[0], f[1]);
+  double J = dotv(f[0], j0);
+  double Ic = dotv(f[0], f[0]) + dotv(f[1], f[1]) + dotv(f[2], f[2]);
+  if (energy) *energy = 0.5 * muV * (Ic - 3.0) + 0.5 * lamV * (J - aT) * (J - aT);
+  double s2 = lamV * (J - aT);
+  V<double> p0 = f[0] * muV + j0 * s2, p1 = f[1] * muV + j1 * s2, p2 = f[2] * muV + j2 * s2;
+  V<double> w = self == 1 ? r[0] : self == 2 ? r[1] : self == 3 ? r[2] : -(r[0] + r[1] + r[2]);
+  V<double> g = p0 * w.x + p1 * w.y + p2 * w.z;
+  V<double> gj = j0 * w.x + j1 * w.y + j2 * w.z;
+  if (grad3) { grad3[0] = g.x; grad3[1] = g.y; grad3[2] = g.z; }
+  if (hess9)
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) hess9[3 * i + k] = (i == k ? muV * dotv(w, w) : 0.0) + lamV * gj[i] * gj[k];
+}
+
+void oracle_solve6x6(const double* A36, const double* b6, double* x6, int precision) {
+  if (precision == 64) {
+    double A[6][6], b[6];
+    for (int i = 0; i < 36; i++) A[i / 6][i % 6] = A36[i];
+    for (int i = 0; i < 6; i++) b[i] = b6[i];
+    solve6(A, b, x6);
+  } else {
+    float A[6][6], b[6], x[6];
+    for (int i = 0; i < 36; i++) A[i / 6][i % 6] = (float)A36[i];
+    for (int i = 0; i < 6; i++) b[i] = (float)b6[i];
+    solve6(A, b, x);
+    for (int i = 0; i < 6; i++) x6[i] = x[i];
+  }
+}
+
+uint32_t oracle_cell_hash(float px, float py, float pz, float cs, uint32_t group, uint32_t hs) {
+  return cellHash32((int)std::floor(px / cs), (int)std::floor(py / cs), (int)std::floor(pz / cs), group, hs);
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.  CPU restatement of the reference's
+ * AVBD step (LuckyIYI/avbd-metal) used as the parity checker for the HIP
+ * backend and as the CPU baseline.  Nothing in the product links this.
+ *
+ * Serial, templated on float / double.  It consumes the same scene wire
+ * format as the backend (include/avbd.h) so both start from identical inputs.
+ */
+#ifndef AVBD_ORACLE_H
+#define AVBD_ORACLE_H
+#include <stdint.h>
+#include "../include/avbd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_solver oracle_solver;
+
+/* precision: 32 or 64 */
+int oracle_create(const avbd_scene_desc* scene, int precision, int max_pairs_per_body, oracle_solver** out);
+void oracle_destroy(oracle_solver* o);
+int oracle_step(oracle_solver* o);
+int oracle_set_settings(oracle_solver* o, const avbd_settings* s);
+int oracle_get_body_states(oracle_solver* o, uint32_t n, float* pos3, float* rot4, float* vel3, float* angvel3);
+int oracle_get_body_states_f64(oracle_solver* o, uint32_t n, double* pos3, double* rot4, double* vel3,
+                               double* angvel3);
+int oracle_set_body_states(oracle_solver* o, const uint32_t* idx, uint32_t n, const float* pos3, const float* rot4,
+                           const float* vel3, const float* angvel3);
+int oracle_get_colors(oracle_solver* o, uint32_t n, uint32_t* colors);
+int oracle_get_cell_lists(oracle_solver* o, uint32_t* hash_size, uint32_t* num_hashed, float* cell_size,
+                          uint32_t* cell_start, uint32_t* cell_count, uint32_t* cell_bodies);
+int oracle_get_pairs(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* pairs);
+/* manifold summary of the last frame: per pair (bodyA, bodyB, numContacts) and contact normals */
+int oracle_get_manifolds(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* header3, float* normal3,
+                         float* lambda_pen);
+int oracle_residual(oracle_solver* o, double* out);
+int oracle_stats(oracle_solver* o, int32_t* num_pairs, int32_t* active_manifolds, int32_t* num_colors,
+                 int32_t* pair_candidates, int32_t* color_conflicts);
+int oracle_map_find(oracle_solver* o, const uint32_t* pairs, uint32_t n, int32_t* out);
+
+/* unit restatements for pinning */
+void oracle_splitmix64(uint64_t seed, uint32_t n, uint64_t* out_u64, float* out_float);
+/* SNH tet: positions x[12], DmInv rows r[9], mu, lambda, vol, self slot (0..3):
+ * energy, gradient on self (3) and the per-vertex Hessian approximation (9) */
+void oracle_snh_tet(const double* x, const double* dminv, double mu, double lam, double vol, int self, double* energy,
+                    double* grad3, double* hess9);
+void oracle_solve6x6(const double* A36, const double* b6, double* x6, int precision);
+uint32_t oracle_cell_hash(float px, float py, float pz, float cell_size, uint32_t group, uint32_t hash_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

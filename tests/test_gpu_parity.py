@@ -1,0 +1,234 @@
+"""GPU parity: the HIP backend (through the C ABI) against the oracle.
+
+Bars: integer outputs (colorings, hash cell lists, pair streams, manifold
+activity) bit-exact; FP32 state after one iteration from an identical state
+within 1e-5 relative (positions are O(1 m), so 1e-5 m absolute); 60-step
+trajectories within an RMS of 2e-3 m; bitwise self-determinism.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from avbd_amd import GPUSolver, RuntimeFailure, demos
+from avbd_amd.scene import SimSettings
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    flush_print(f"[gpu case] {name}")
+
+
+def _block(cells=20, iterations=20):
+    return demos.soft_block_scene(cells=cells, iterations=iterations)
+
+
+def test_static_coloring_bitexact_c2():
+    _case("static coloring, 21^3 soft block")
+    s = _block()
+    g = GPUSolver(s)
+    o = oracle.OracleSolver(s)
+    cg, co = g.debugColors(), o.debugColors()
+    assert np.array_equal(cg, co)
+    st = g.stats()
+    assert st.num_colors == int(co[1:].max()) + 1
+    assert sum(st.color_counts) == s.numDynamic
+
+
+def test_hash_cells_and_pairs_bitexact():
+    for name, s in [("stack 10", demos.stack_scene(10)), ("soft block 9^3", _block(8, 5))]:
+        _case(f"hash + pairs, {name}")
+        g = GPUSolver(s)
+        o = oracle.OracleSolver(s)
+        g.step()
+        o.step()
+        g.synchronize()
+        cg, co = g.cellLists(), o.cellLists()
+        assert cg["hash_size"] == co["hash_size"] and cg["cell_size"] == co["cell_size"]
+        assert np.array_equal(cg["count"], co["count"])
+        assert np.array_equal(cg["start"], co["start"])
+        assert np.array_equal(cg["bodies"], co["bodies"])
+        assert np.array_equal(g.pairs(), o.pairs())
+        # manifold activity of the first frame (identical start state)
+        mg = g.manifolds()
+        ho, _, _ = o.manifolds()
+        assert np.array_equal(mg["header"][:, :3], ho)
+
+
+def _set_same_state(g, o, n):
+    p, r, v, w = g.bodyStates()
+    idx = np.arange(n, dtype=np.uint32)
+    g.setBodyStates(idx, p, r, v, w)
+    o.setBodyStates(idx, p, r, v, w)
+    return p
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32])
+def test_one_iteration_parity_c2(lanes):
+    _case(f"one iteration from identical state, 21^3 block, lanes={lanes}")
+    s = _block()
+    g = GPUSolver(s, lanesPerBody=lanes, sweepMode=1)
+    o = oracle.OracleSolver(s)
+    g.step(12)  # reach contact with the ground
+    p0 = _set_same_state(g, o, len(s.bodies))
+    st = g.settings
+    st.iterations = 1
+    g.settings = st
+    o.set_settings(st)
+    g.step()
+    o.step()
+    pg, po = g.positions(), o.positions()
+    assert np.array_equal(g.pairs(), o.pairs())
+    err = np.abs(pg - po).max()
+    dx = np.abs(po - p0).max()
+    flush_print(f"  max|x_gpu-x_oracle| = {err:.3e} m, max|dx| = {dx:.3e} m")
+    assert err <= 1e-5 * max(1.0, float(np.abs(po).max()))
+
+
+def test_trajectory_rms_c2_60_steps():
+    _case("60-step trajectory, 21^3 block")
+    s = _block()
+    g = GPUSolver(s)
+    o = oracle.OracleSolver(s)
+    o64 = oracle.OracleSolver(s, precision=64)
+    for _ in range(60):
+        g.step()
+        o.step()
+        o64.step()
+    pg, po = g.positions(), o.positions()
+    p64 = o64.bodyStates64()[0]
+    rms = float(np.sqrt(np.mean(np.sum((pg - po) ** 2, axis=1))))
+    e64g = float(np.sqrt(np.mean(np.sum((pg - p64) ** 2, axis=1))))
+    e64o = float(np.sqrt(np.mean(np.sum((po - p64) ** 2, axis=1))))
+    flush_print(f"  RMS gpu-oracle32 {rms:.3e} m, gpu-oracle64 {e64g:.3e} m, oracle32-oracle64 {e64o:.3e} m")
+    assert np.all(np.isfinite(pg))
+    assert rms < 2e-3
+
+
+def test_bitwise_determinism_c2():
+    _case("bitwise self-determinism, 21^3 block, 30 steps")
+    s = _block()
+    a = GPUSolver(s)
+    b = GPUSolver(s)
+    a.step(30)
+    b.step(30)
+    for x, y in zip(a.bodyStates(), b.bodyStates()):
+        assert np.array_equal(x, y)
+
+
+def test_sweep_modes_agree():
+    _case("per-color launches vs persistent cooperative sweep")
+    s = _block(12, 10)
+    a = GPUSolver(s, lanesPerBody=8, sweepMode=1)
+    b = GPUSolver(s, lanesPerBody=8, sweepMode=2)
+    if b.stats().sweep_mode != 2:
+        pytest.skip("cooperative launch unavailable")
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.positions(), b.positions())
+
+
+def test_graph_vs_eager_identical():
+    _case("hipGraph replay vs eager launches")
+    s = _block(8, 10)
+    a = GPUSolver(s, useGraph=True)
+    b = GPUSolver(s, useGraph=False)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.positions(), b.positions())
+
+
+def test_stack_parity_and_thresholds():
+    _case("rigid stack of 10, 240 steps")
+    s = demos.stack_scene(10)
+    g = GPUSolver(s)
+    o = oracle.OracleSolver(s)
+    for _ in range(240):
+        g.step()
+        o.step()
+    pg, po = g.positions(), o.positions()
+    flush_print(f"  heights gpu {np.round(pg[1:, 2], 3)}")
+    # CPU-GPU agreement of the reference's own parity test (GPUSolverTests.swift:831-846)
+    assert np.abs(pg - po).max() < 0.05
+    # each box sinks by about the 1 cm contact margin: z_i = 0.5 + i - 0.01 (i + 1)
+    for i in range(10):
+        assert abs(pg[1 + i, 2] - (0.5 + i - 0.01 * (i + 1))) < 0.03
+        assert np.hypot(pg[1 + i, 0], pg[1 + i, 1]) < 0.15
+    assert g.stats().color_conflicts == 0
+
+
+def test_five_stack_reference_thresholds():
+    _case("5-stack, 240 steps, reference thresholds")
+    # CPUSolverTests.swift:80-98: z = 0.5 + i +- 0.08, lateral drift < 0.15
+    s = demos.stack_scene(5, friction=0.7, jitter=0.0)
+    g = GPUSolver(s)
+    g.step(240)
+    p = g.positions()
+    for i in range(5):
+        assert abs(p[1 + i, 2] - (0.5 + i)) < 0.08
+        assert np.hypot(p[1 + i, 0], p[1 + i, 1]) < 0.15
+
+
+def test_box_rests_on_ground():
+    _case("single box rests")
+    # CPUSolverTests.swift:35-47: z = 0.5 +- 0.03 after 120 steps
+    s = demos.stack_scene(1, jitter=0.0)
+    g = GPUSolver(s)
+    g.step(120)
+    assert abs(g.positions()[1, 2] - 0.5) < 0.03
+
+
+def test_free_fall_implicit_euler():
+    _case("free fall")
+    s = demos.PhysicsScene("fall", SimSettings(iterations=4))
+    s.addBody((1, 1, 1), 1.0, 0.5, (0, 0, 10), velocity=(1.0, 0.0, 2.0))
+    g = GPUSolver(s)
+    dt = s.settings.dt
+    n = 60
+    g.step(n)
+    p = g.positions()[0].astype(np.float64)
+    # implicit Euler: v_k = v0 + k g dt ; x_n = x0 + sum_k v_k dt
+    vz = 2.0 + (-10.0) * dt * np.arange(1, n + 1)
+    z = 10.0 + np.sum(vz) * dt
+    assert abs(p[2] - z) < 1e-3
+    assert abs(p[0] - n * dt) < 1e-4
+
+
+def test_pair_capacity_latches():
+    _case("rigid pair capacity fail-closed")
+    s = demos.stack_scene(10)
+    g = GPUSolver(s, pairCapacityForTesting=4)
+    g.step()
+    with pytest.raises(RuntimeFailure) as e:
+        g.synchronize()
+    assert e.value.kind == "rigidPairCapacity" and e.value.capacity == 4
+    with pytest.raises(RuntimeFailure):
+        g.step()
+
+
+def test_denied_stage_latches():
+    _case("denied stage latches a sticky failure")
+    s = demos.stack_scene(3)
+    g = GPUSolver(s)
+    g.step()
+    g.denyStageForTesting("narrowphase", frame=2)
+    with pytest.raises(RuntimeFailure) as e:
+        g.step()
+    assert e.value.kind == "commandEncoderCreation" and e.value.frame == 2
+    with pytest.raises(RuntimeFailure):
+        g.synchronize()
+
+
+def test_set_body_states_cold_starts_contacts():
+    _case("setBodyStates clears incident warm starts")
+    s = demos.stack_scene(3)
+    g = GPUSolver(s)
+    g.step(30)
+    m = g.manifolds()
+    assert (m["header"][:, 2] > 0).any()
+    p, r, v, w = g.bodyStates()
+    g.setBodyStates(np.arange(len(p)), p, r, v, w)
+    m2 = g.manifolds()
+    assert np.all(m2["c"]["lambda"] == 0)
