@@ -53,7 +53,9 @@ __global__ void __launch_bounds__(256) k_bp_count(DeviceState S, StepParams P) {
   uint32_t h = cell_hash(cx, cy, cz, S.colliderGroup[c], P.gridHashSize);
   uint32_t slot = atomicAdd(&S.cellCount[h], 1u);
   S.bodyCellSlot[gid] = U2{h, slot};
-  if (S.colliderShape[c].w >= 0.0f) S.cellRigid[h] = 1u;
+  float r = S.colliderShape[c].w;
+  S.hashCenter[gid] = f4(p, fabsf(r));
+  if (r >= 0.0f) S.cellRigid[h] = 1u;
 }
 
 __global__ void __launch_bounds__(256) k_bp_scatter(DeviceState S, StepParams P) {
@@ -64,13 +66,33 @@ __global__ void __launch_bounds__(256) k_bp_scatter(DeviceState S, StepParams P)
 }
 
 // Sort each bucket by collider id so pair enumeration order is independent
-// of the atomic scatter schedule.
+// of the atomic scatter schedule.  Buckets of up to 32 (hash aliasing puts
+// two or three lattice cells in one bucket) are sorted in registers by an
+// odd-even transposition network; longer ones in place.
 __global__ void __launch_bounds__(256) k_bp_sort_cells(DeviceState S, StepParams P) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P.gridHashSize) return;
   uint32_t n = S.cellCount[gid];
   if (n < 2) return;
   uint32_t s = S.cellStart[gid];
+  if (n <= 32) {
+    uint32_t v[32];
+#pragma unroll
+    for (int i = 0; i < 32; i++) v[i] = (uint32_t)i < n ? S.cellBodies[s + i] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int pass = 0; pass < 32; pass++) {
+#pragma unroll
+      for (int i = pass & 1; i + 1 < 32; i += 2) {
+        uint32_t a = v[i], b = v[i + 1];
+        v[i] = a < b ? a : b;
+        v[i + 1] = a < b ? b : a;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+      if ((uint32_t)i < n) S.cellBodies[s + i] = v[i];
+    return;
+  }
   for (uint32_t i = 1; i < n; i++) {
     uint32_t v = S.cellBodies[s + i];
     uint32_t j = i;
@@ -80,6 +102,15 @@ __global__ void __launch_bounds__(256) k_bp_sort_cells(DeviceState S, StepParams
     }
     S.cellBodies[s + j] = v;
   }
+}
+
+// cell-ordered copies of the per-collider center / metadata
+__global__ void __launch_bounds__(256) k_bp_gather(DeviceState S, StepParams P) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P.numHashed) return;
+  uint32_t g = S.colliderToHashed[S.cellBodies[k]];
+  S.cellPos[k] = S.hashCenter[g];
+  S.cellMeta[k] = S.hashMeta[g];
 }
 
 __device__ __forceinline__ bool pair_excluded(const DeviceState& S, uint32_t count, uint32_t a, uint32_t b) {
@@ -95,117 +126,141 @@ __device__ __forceinline__ bool pair_excluded(const DeviceState& S, uint32_t cou
   return e.x == a && e.y == b;
 }
 
-// One producer (hashed collider, or global collider) counts -- or emits at
-// its scanned offset -- its accepted candidates.  The same predicate chain
-// runs twice so the output order is a pure function of the state
-// (20_broadphase.metal:287-424).
-__device__ uint32_t bp_producer(const DeviceState& S, const StepParams& P, uint32_t producer, bool emit,
-                                uint32_t base) {
-  uint32_t count = 0;
-  if (producer < P.numHashed) {
-    uint32_t a = S.hashedIdx[producer];
-    uint32_t ownerA = S.colliderOwner[a];
-    uint32_t grpA = S.colliderGroup[a];
-    uint32_t shA = S.colliderShared[a];
-    V3 pa = collider_center(S, a);
-    float ra = fabsf(S.colliderShape[a].w);
-    bool aDyn = S.posLin[ownerA].w > 0.0f;
-    uint32_t ga = S.clothGroup[ownerA];
-    bool scanCells = !(ga != 0 && P.numHashedRigid == 0);
-    int cx, cy, cz;
-    cell_coord(pa, P.cellSize, cx, cy, cz);
-    if (scanCells) {
-      uint32_t visited[27];
-      uint32_t nvis = 0;
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            uint32_t h = cell_hash(cx + dx, cy + dy, cz + dz, grpA, P.gridHashSize);
-            bool dup = false;
-            for (uint32_t v = 0; v < nvis; v++) dup |= (visited[v] == h);
-            if (dup) continue;
-            visited[nvis++] = h;
-            if (ga != 0 && S.cellRigid[h] == 0) continue;
-            uint32_t s = S.cellStart[h], e = s + S.cellCount[h];
-            for (uint32_t k = s; k < e; k++) {
-              uint32_t b = S.cellBodies[k];
-              if (b <= a) continue;
-              if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
-              uint32_t ownerB = S.colliderOwner[b];
-              if (ownerA == ownerB) continue;
-              if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
-              if (ga != 0 && S.clothGroup[ownerB] != 0) continue;
-              float rb = fabsf(S.colliderShape[b].w);
-              V3 d = pa - collider_center(S, b);
-              float r = ra + rb;
-              if (dot(d, d) > r * r) continue;
-              uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
-              if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
-              uint32_t slot = base + count;
-              if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a, b};
-              count++;
-            }
-          }
-    }
-    for (uint32_t g = 0; g < P.numGlobals; g++) {
-      uint32_t b = S.globalIdx[g];
-      if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
-      uint32_t ownerB = S.colliderOwner[b];
-      if (ownerA == ownerB) continue;
-      if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
-      if (ga != 0 && S.clothGroup[ownerB] == ga) continue;
-      float rb = fabsf(S.colliderShape[b].w);
-      V3 d = pa - collider_center(S, b);
-      float r = ra + rb;
-      if (dot(d, d) > r * r) continue;
-      uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
-      if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
-      uint32_t slot = base + count;
-      if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a < b ? a : b, a < b ? b : a};
-      count++;
-    }
-  } else {
-    uint32_t gp = producer - P.numHashed;
-    if (gp >= P.numGlobals) return 0;
-    uint32_t a = S.globalIdx[gp];
-    uint32_t ownerA = S.colliderOwner[a];
-    uint32_t grpA = S.colliderGroup[a];
-    uint32_t shA = S.colliderShared[a];
-    V3 pa = collider_center(S, a);
-    float ra = fabsf(S.colliderShape[a].w);
-    bool aDyn = S.posLin[ownerA].w > 0.0f;
-    uint32_t ga = S.clothGroup[ownerA];
-    for (uint32_t g = gp + 1; g < P.numGlobals; g++) {
-      uint32_t b = S.globalIdx[g];
-      if (!domains_compatible(grpA, S.colliderGroup[b], shA, S.colliderShared[b])) continue;
-      uint32_t ownerB = S.colliderOwner[b];
-      if (ownerA == ownerB) continue;
-      if (!aDyn && S.posLin[ownerB].w <= 0.0f) continue;
-      if (ga != 0 && S.clothGroup[ownerB] == ga) continue;
-      float rb = fabsf(S.colliderShape[b].w);
-      V3 d = pa - collider_center(S, b);
-      float r = ra + rb;
-      if (dot(d, d) > r * r) continue;
-      uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
-      if (pair_excluded(S, P.numExclusions, lo, hi)) continue;
-      uint32_t slot = base + count;
-      if (emit && slot < P.maxPairs) S.pairs[slot] = U2{a < b ? a : b, a < b ? b : a};
-      count++;
-    }
-  }
-  return count;
+// Candidate filters shared by every producer kind (20_broadphase.metal:287-424).
+__device__ __forceinline__ bool pair_accept(const DeviceState& S, const StepParams& P, uint32_t ownerA, uint32_t grpA,
+                                            uint32_t shA, bool aDyn, V3 pa, float ra, uint32_t ownerB,
+                                            uint32_t grpB, uint32_t shB, bool bDyn, V3 pb, float rb) {
+  if (!domains_compatible(grpA, grpB, shA, shB)) return false;
+  if (ownerA == ownerB) return false;
+  if (!aDyn && !bDyn) return false;
+  V3 d = pa - pb;
+  float r = ra + rb;
+  if (dot(d, d) > r * r) return false;
+  uint32_t lo = ownerA < ownerB ? ownerA : ownerB, hi = ownerA < ownerB ? ownerB : ownerA;
+  return !pair_excluded(S, P.numExclusions, lo, hi);
 }
 
-__global__ void __launch_bounds__(256) k_bp_count_pairs(DeviceState S, StepParams P) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= P.numHashed + P.numGlobals) return;
-  S.pairCount[gid] = bp_producer(S, P, gid, false, 0);
+// One wave per producer (a hashed collider scanning its 27 neighbour cells
+// and the global list, or a global collider scanning the later globals).
+// Lanes split the producer's linearised candidate list; a ballot ranks the
+// accepted candidates so the emission order equals the serial order --
+// cells in (dz, dy, dx) order, each bucket in collider order, then globals.
+// Pass 1 (EMIT = false) writes the per-producer count, pass 2 emits at the
+// scanned offset.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t producer = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (producer >= P.numHashed + P.numGlobals) return;  // whole wave
+  const uint64_t ltMask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  const uint32_t base = EMIT ? S.pairStart[producer] : 0u;
+  uint32_t count = 0;
+  auto emitRanked = [&](bool accept, uint32_t a, uint32_t b) {
+    uint64_t mask = __ballot(accept);
+    if (EMIT && accept) {
+      uint32_t slot = base + count + (uint32_t)__builtin_popcountll(mask & ltMask);
+      if (slot < P.maxPairs) S.pairs[slot] = U2{a, b};
+    }
+    count += (uint32_t)__builtin_popcountll(mask);
+  };
+  if (producer < P.numHashed) {
+    const uint32_t a = S.hashedIdx[producer];
+    const U4 ma = S.hashMeta[producer];
+    const F4 ca = S.hashCenter[producer];
+    const uint32_t ownerA = ma.x, grpA = ma.y, shA = ma.z & 1u, ga = ma.w;
+    const bool aDyn = (ma.z & 2u) != 0;
+    const V3 pa = xyz(ca);
+    const float ra = ca.w;
+    if (!(ga != 0 && P.numHashedRigid == 0)) {
+      int cx, cy, cz;
+      cell_coord(pa, P.cellSize, cx, cy, cz);
+      // lanes 0..26 own one neighbour cell each
+      uint32_t h = 0, cs = 0, cc = 0;
+      if (lane < 27) {
+        int q = (int)lane;
+        h = cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, grpA, P.gridHashSize);
+      }
+      bool dup = false;
+      for (uint32_t j = 0; j < 27; j++) {
+        uint32_t hj = __shfl(h, (int)j, 64);
+        dup |= (j < lane) && (hj == h);
+      }
+      if (lane < 27 && !dup && !(ga != 0 && S.cellRigid[h] == 0)) {
+        cs = S.cellStart[h];
+        cc = S.cellCount[h];
+      }
+      uint32_t incl = cc;
+#pragma unroll
+      for (int d = 1; d < 32; d <<= 1) {
+        uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += y;
+      }
+      const uint32_t excl = incl - cc;
+      const uint32_t total = __shfl(incl, 26, 64);
+      for (uint32_t chunk = 0; chunk < total; chunk += 64) {
+        uint32_t j = chunk + lane;
+        uint32_t k = 0;
+        bool valid = false;
+        for (uint32_t t = 0; t < 27; t++) {
+          uint32_t et = __shfl(excl, (int)t, 64), ct = __shfl(cc, (int)t, 64), st = __shfl(cs, (int)t, 64);
+          if (j >= et && j < et + ct) { k = st + (j - et); valid = true; }
+        }
+        bool accept = false;
+        uint32_t b = 0;
+        if (valid) {
+          b = S.cellBodies[k];
+          if (b > a) {
+            const U4 mb = S.cellMeta[k];
+            const F4 cb = S.cellPos[k];
+            accept = !(ga != 0 && mb.w != 0) &&
+                     pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, mb.x, mb.y, mb.z & 1u, (mb.z & 2u) != 0,
+                                 xyz(cb), cb.w);
+          }
+        }
+        emitRanked(accept, a, b);
+      }
+    }
+    for (uint32_t g0 = 0; g0 < P.numGlobals; g0 += 64) {
+      uint32_t g = g0 + lane;
+      bool accept = false;
+      uint32_t b = 0;
+      if (g < P.numGlobals) {
+        b = S.globalIdx[g];
+        uint32_t ownerB = S.colliderOwner[b];
+        accept = !(ga != 0 && S.clothGroup[ownerB] == ga) &&
+                 pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, ownerB, S.colliderGroup[b],
+                             S.colliderShared[b], S.posLin[ownerB].w > 0.0f, collider_center(S, b),
+                             fabsf(S.colliderShape[b].w));
+      }
+      emitRanked(accept, a < b ? a : b, a < b ? b : a);
+    }
+  } else {
+    const uint32_t gp = producer - P.numHashed;
+    const uint32_t a = S.globalIdx[gp];
+    const uint32_t ownerA = S.colliderOwner[a];
+    const uint32_t grpA = S.colliderGroup[a], shA = S.colliderShared[a];
+    const V3 pa = collider_center(S, a);
+    const float ra = fabsf(S.colliderShape[a].w);
+    const bool aDyn = S.posLin[ownerA].w > 0.0f;
+    const uint32_t ga = S.clothGroup[ownerA];
+    for (uint32_t g0 = gp + 1; g0 < P.numGlobals; g0 += 64) {
+      uint32_t g = g0 + lane;
+      bool accept = false;
+      uint32_t b = 0;
+      if (g < P.numGlobals) {
+        b = S.globalIdx[g];
+        uint32_t ownerB = S.colliderOwner[b];
+        accept = !(ga != 0 && S.clothGroup[ownerB] == ga) &&
+                 pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, ownerB, S.colliderGroup[b],
+                             S.colliderShared[b], S.posLin[ownerB].w > 0.0f, collider_center(S, b),
+                             fabsf(S.colliderShape[b].w));
+      }
+      emitRanked(accept, a < b ? a : b, a < b ? b : a);
+    }
+  }
+  if (!EMIT && lane == 0) S.pairCount[producer] = count;
 }
-__global__ void __launch_bounds__(256) k_bp_emit_pairs(DeviceState S, StepParams P) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= P.numHashed + P.numGlobals) return;
-  (void)bp_producer(S, P, gid, true, S.pairStart[gid]);
-}
+
 __global__ void k_bp_finalize(DeviceState S, StepParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t producers = P.numHashed + P.numGlobals;
@@ -698,11 +753,14 @@ void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P
     hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   }
   hipLaunchKernelGGL(k_bp_sort_cells, dim3(blocks_for(P.gridHashSize, 256)), dim3(256), 0, st, S, P);
+  if (P.numHashed > 0) {
+    hipLaunchKernelGGL(k_bp_gather, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
+  }
   uint32_t producers = P.numHashed + P.numGlobals;
   if (producers > 0) {
-    hipLaunchKernelGGL(k_bp_count_pairs, dim3(blocks_for(producers, 256)), dim3(256), 0, st, S, P);
+    hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for(producers, 4)), dim3(256), 0, st, S, P);
     launch_exclusive_scan(st, S.pairCount, S.pairStart, producers, S.scanScratch);
-    hipLaunchKernelGGL(k_bp_emit_pairs, dim3(blocks_for(producers, 256)), dim3(256), 0, st, S, P);
+    hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(producers, 4)), dim3(256), 0, st, S, P);
   }
   hipLaunchKernelGGL(k_bp_finalize, dim3(1), dim3(64), 0, st, S, P);
 }

@@ -266,6 +266,19 @@ struct alignas(16) TetRec {
 };
 static_assert(sizeof(TetRec) == 64, "TetRec");
 
+// One (body, tet) adjacency slot, stored at the slot's position in the
+// static CSR so a body's slots are contiguous (coalesced lane loads).  With
+// the shape-function gradients b_o of the three other vertices,
+//   F = sum_o (x_o - x_self) b_o^T  and  w_self = -(b_1 + b_2 + b_3),
+// so the stamp needs exactly one level of gathers (the three positions).
+struct alignas(16) TetSlot {
+  U4 other;   // other vertex ids; w = tet index
+  F4 b1;      // w = mu V
+  F4 b2;      // w = lambda V
+  F4 b3;      // w = alpha_T = 1 + mu / lambda
+};
+static_assert(sizeof(TetSlot) == 64, "TetSlot");
+
 struct alignas(16) ContactRec {
   F4 rA;       // w = feature key bits
   F4 rB;       // w = stick flag
@@ -314,7 +327,12 @@ struct StepParams {
   uint32_t numBodies, numColliders, numJoints, numSprings, numTets;
   uint32_t mapCapacity, maxPairs, gridHashSize, numHashed, numGlobals, numHashedRigid;
   uint32_t frictionCombineMode, numExclusions, iterations;
+  uint32_t staticPalette;   // colors fixed for the scene's lifetime (tets / cloth present)
 };
+
+// ManifoldRec.header.w flag: both bodies dynamic and of the same static
+// color -- the partner's pose is read from the start-of-iteration snapshot
+constexpr uint32_t kManifoldSameColor = 0x100u;
 
 AVBD_HD float combine_friction(float a, float b, uint32_t mode) {
   if (mode == 1u) return a * b;

@@ -6,6 +6,8 @@ namespace avbd {
 
 // util.hip
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t value);
+// zero up to 6 buffers in one launch
+void launch_zero_multi(hipStream_t st, uint32_t* const* ptrs, const uint32_t* counts, int num);
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch);
 uint32_t scan_scratch_words(uint32_t n);
 
@@ -23,16 +25,16 @@ void launch_contact_adjacency(hipStream_t st, const DeviceState& S, const StepPa
 void launch_static_color_pass(hipStream_t st, const DeviceState& S, const StepParams& P, const uint32_t* in,
                               uint32_t* out);
 void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepParams& P);
+// softOnly: every body of the launch is a particle whose static entries are tets
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
-                         uint32_t colorCount, int lanes);
+                         uint32_t colorCount, int lanes, bool softOnly);
 void launch_dual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 void launch_finalize(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks);
 void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t* out, uint32_t gridCap);
 // persistent cooperative sweep: all iterations x colors x dual in one launch
-hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P,
-                                   const uint32_t* hostColorCounts, uint32_t numColors, int lanes,
-                                   uint32_t gridBlocks, uint32_t barrierBase);
-int persistent_max_blocks(int lanes);
+hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t numColors,
+                                   int lanes, bool softOnly, uint32_t gridBlocks, uint32_t barrierBase);
+int persistent_max_blocks(int lanes, bool softOnly);
 
 }  // namespace avbd

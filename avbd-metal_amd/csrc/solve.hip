@@ -150,8 +150,11 @@ __global__ void __launch_bounds__(256) k_cadj_scatter(DeviceState S, StepParams 
     U4 h = S.manifolds[m].header;
     if (h.z == 0) continue;
     uint32_t entry = (FK_MANIFOLD << kAdjShift) | m;
-    if (body_dynamic(S, h.x)) S.dAdjList[atomicAdd(&S.dAdjCursor[h.x], 1u)] = entry;
-    if (body_dynamic(S, h.y)) S.dAdjList[atomicAdd(&S.dAdjCursor[h.y], 1u)] = entry;
+    bool da = body_dynamic(S, h.x), db = body_dynamic(S, h.y);
+    if (da) S.dAdjList[atomicAdd(&S.dAdjCursor[h.x], 1u)] = entry;
+    if (db) S.dAdjList[atomicAdd(&S.dAdjCursor[h.y], 1u)] = entry;
+    if (P.staticPalette && da && db && S.colors[h.x] == S.colors[h.y])
+      S.manifolds[m].header.w = h.w | kManifoldSameColor;
   }
 }
 // Sorting each (short) segment makes the accumulation order a function of
@@ -171,8 +174,8 @@ __global__ void __launch_bounds__(256) k_cadj_sort(DeviceState S, StepParams P) 
   }
 }
 
+// dAdjCount must be zero on entry (cleared with the frame's counters)
 void launch_contact_adjacency(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  launch_fill_u32(st, S.dAdjCount, P.numBodies, 0u);
   uint32_t blocks = (P.maxPairs + 255) / 256;
   if (blocks > gridCap) blocks = gridCap;
   if (blocks == 0) blocks = 1;
@@ -414,20 +417,52 @@ __device__ __forceinline__ V3 group_sum3(V3 v) {
 // Gathers and stamps one body's entries (lane-strided), reduces, and lane 0
 // solves and writes.  Must be reached by whole waves (EXEC full) because of
 // the ballot and DPP reductions; inactive groups pass active = false.
-template <int L>
+template <int L, bool SOFT>
 __device__ __forceinline__ void primal_group(const DeviceState& S, const StepParams& P, bool active, uint32_t body,
                                              uint32_t lane, uint32_t color) {
-  bool rigid = active && S.shape[body].w >= 0.0f;
-  const bool anyRigid = __ballot(rigid) != 0ull;
+  bool rigid = !SOFT && active && S.shape[body].w >= 0.0f;
+  const bool anyRigid = !SOFT && __ballot(rigid) != 0ull;
   Acc acc;
   acc_zero(acc);
   if (active) {
-    PoseView pv{S.posLin, S.posAng, S.posLinSnap, S.posAngSnap, S.colors, color};
-    uint32_t ss = S.sAdjStart[body], sc = S.sAdjCount[body];
-    uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
-    for (uint32_t k = lane; k < sc + dc; k += L) {
-      uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
-      stamp_entry(S, pv, entry, body, rigid, P.alpha, P.dt, acc);
+    PoseView pv{S.posLin, S.posAng, S.posLinSnap, S.posAngSnap, true};
+    const V3 xs = xyz(S.posLin[body]);
+    const uint32_t ss = S.sAdjStart[body], sc = S.sAdjCount[body];
+    const uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
+    if (SOFT) {
+      // particle colors: static entries are tets, per-frame entries manifolds.
+      // Each lane first issues all of its slot-record loads of a batch, then
+      // all position gathers, then the arithmetic.
+      constexpr int B = L >= 32 ? 1 : (L == 16 ? 2 : 4);
+      for (uint32_t base = lane; base < sc; base += L * B) {
+        TetSlot r[B];
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+          uint32_t k = base + i * L;
+          if (k < sc) r[i] = S.tetSlots[ss + k];
+        }
+        V3 x[B][3];
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+          if (base + i * L < sc) {
+            x[i][0] = xyz(S.posLin[r[i].other.x]);
+            x[i][1] = xyz(S.posLin[r[i].other.y]);
+            x[i][2] = xyz(S.posLin[r[i].other.z]);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < B; i++)
+          if (base + i * L < sc) stamp_tet_slot_x(r[i], xs, x[i][0], x[i][1], x[i][2], acc);
+      }
+      // contact entries continue the lane round-robin after the tets
+      for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L)
+        stamp_manifold(S.manifolds[S.dAdjList[ds + k] & kAdjIndexMask], body, false, pv, S.initLin, S.initAng,
+                       P.alpha, acc);
+    } else {
+      for (uint32_t k = lane; k < sc + dc; k += L) {
+        uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
+        stamp_entry(S, pv, entry, ss + k, body, xs, rigid, P.alpha, P.dt, acc);
+      }
     }
   }
   acc.l00 = group_sum<L>(acc.l00);
@@ -488,7 +523,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
   return r;
 }
 
-template <int L>
+template <int L, bool SOFT>
 __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count) {
   const uint32_t groupsPerBlock = 256 / L;
   uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -498,23 +533,30 @@ __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uin
   if (count == 0xFFFFFFFFu) count = S.colorStart[color + 1] - s;  // device-sized (dynamic palette)
   bool active = slot < count;
   uint32_t body = active ? S.colorList[s + slot] : 0u;
-  primal_group<L>(S, P, active, body, lane, color);
+  primal_group<L, SOFT>(S, P, active, body, lane, color);
+}
+
+template <bool SOFT>
+static void launch_primal_t(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
+                            uint32_t count, int lanes, uint32_t blocks) {
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((k_primal<4, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 8: hipLaunchKernelGGL((k_primal<8, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 16: hipLaunchKernelGGL((k_primal<16, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    default: hipLaunchKernelGGL((k_primal<32, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+  }
 }
 
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t count,
-                         int lanes) {
+                         int lanes, bool softOnly) {
   if (count == 0) return;
   uint32_t groupsPerBlock = 256 / lanes;
   uint32_t n = count == 0xFFFFFFFFu ? P.numBodies : count;
   uint32_t blocks = (n + groupsPerBlock - 1) / groupsPerBlock;
   // the XCD remap needs the padded slot range to cover every body
   blocks = (blocks + 7) / 8 * 8;
-  switch (lanes) {
-    case 4: hipLaunchKernelGGL(k_primal<4>, dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 8: hipLaunchKernelGGL(k_primal<8>, dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 16: hipLaunchKernelGGL(k_primal<16>, dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    default: hipLaunchKernelGGL(k_primal<32>, dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-  }
+  if (softOnly) launch_primal_t<true>(st, S, P, color, count, lanes, blocks);
+  else launch_primal_t<false>(st, S, P, color, count, lanes, blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -697,12 +739,13 @@ __global__ void __launch_bounds__(256) k_residual(DeviceState S, StepParams P) {
     bool rigid = S.shape[body].w >= 0.0f;
     Acc acc;
     acc_zero(acc);
-    PoseView pv{S.posLin, S.posAng, S.posLin, S.posAng, S.colors, 0xFFFFFFFFu};
+    PoseView pv{S.posLin, S.posAng, S.posLin, S.posAng, false};
     uint32_t ss = S.sAdjStart[body], sc = S.sAdjCount[body];
     uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
+    const V3 xs = xyz(S.posLin[body]);
     for (uint32_t k = 0; k < sc + dc; k++) {
       uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
-      stamp_entry(S, pv, entry, body, rigid, P.alpha, P.dt, acc);
+      stamp_entry(S, pv, entry, ss + k, body, xs, rigid, P.alpha, P.dt, acc);
     }
     F4 pl = S.posLin[body];
     float md = pl.w / (P.dt * P.dt);
@@ -800,7 +843,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint
   __syncthreads();
 }
 
-template <int L>
+template <int L, bool SOFT>
 __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P, uint32_t numColors,
                                                     uint32_t barrierBase) {
   const uint32_t groupsPerBlock = 256 / L;
@@ -822,7 +865,7 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
       for (uint32_t r = 0; r < rounds; r++) {
         uint32_t slot = r * ngroups + gslot;
         bool active = slot < count;
-        primal_group<L>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+        primal_group<L, SOFT>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
       }
       epoch += gridDim.x;
       grid_barrier(bar, fail, epoch);
@@ -837,36 +880,39 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
   }
 }
 
-int persistent_max_blocks(int lanes) {
+static const void* persistent_fn(int lanes, bool soft) {
+  if (soft) {
+    switch (lanes) {
+      case 4: return (const void*)k_persistent<4, true>;
+      case 8: return (const void*)k_persistent<8, true>;
+      case 16: return (const void*)k_persistent<16, true>;
+      default: return (const void*)k_persistent<32, true>;
+    }
+  }
+  switch (lanes) {
+    case 4: return (const void*)k_persistent<4, false>;
+    case 8: return (const void*)k_persistent<8, false>;
+    case 16: return (const void*)k_persistent<16, false>;
+    default: return (const void*)k_persistent<32, false>;
+  }
+}
+
+int persistent_max_blocks(int lanes, bool softOnly) {
   int dev = 0, cus = 0, per = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipError_t e = hipSuccess;
-  switch (lanes) {
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_persistent<4>, 256, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_persistent<8>, 256, 0); break;
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_persistent<16>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_persistent<32>, 256, 0); break;
-  }
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, persistent_fn(lanes, softOnly), 256, 0);
   if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
   return per * cus;
 }
 
-hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P,
-                                   const uint32_t* /*hostColorCounts*/, uint32_t numColors, int lanes,
-                                   uint32_t gridBlocks, uint32_t barrierBase) {
+hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t numColors,
+                                   int lanes, bool softOnly, uint32_t gridBlocks, uint32_t barrierBase) {
   DeviceState s = S;
   StepParams p = P;
   uint32_t nc = numColors, bb = barrierBase;
   void* args[] = {&s, &p, &nc, &bb};
-  const void* fn;
-  switch (lanes) {
-    case 4: fn = (const void*)k_persistent<4>; break;
-    case 8: fn = (const void*)k_persistent<8>; break;
-    case 16: fn = (const void*)k_persistent<16>; break;
-    default: fn = (const void*)k_persistent<32>; break;
-  }
-  return hipLaunchCooperativeKernel(fn, dim3(gridBlocks), dim3(256), args, 0, st);
+  return hipLaunchCooperativeKernel(persistent_fn(lanes, softOnly), dim3(gridBlocks), dim3(256), args, 0, st);
 }
 
 }  // namespace avbd

@@ -50,6 +50,8 @@ struct avbd_solver {
   bool dynamicColoring = false;
   uint32_t staticUsedColors = 1;
   std::vector<uint32_t> staticColorCounts;  // per color
+  std::vector<uint8_t> colorSoftOnly;       // per color: particles whose static entries are all tets
+  bool allSoft = false;
   int lanes = 8, sweepMode = 1;
   int persistentBlocks = 0;
   uint32_t gridCap = 1024;
@@ -167,6 +169,7 @@ StepParams make_params(const avbd_solver* s) {
   P.frictionCombineMode = s->settings.friction_combine_mode;
   P.numExclusions = s->nExcl;
   P.iterations = (uint32_t)std::max(0, s->settings.iterations);
+  P.staticPalette = s->dynamicColoring ? 0u : 1u;
   return P;
 }
 
@@ -178,9 +181,9 @@ bool stage_denied(const avbd_solver* s, const char* stage, int64_t frame) {
 hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S) {
   hipStream_t st = s->stream;
   StepParams P = make_params(s);
-  launch_fill_u32(st, S.counters, CTR_TOTAL, 0u);
-  launch_fill_u32(st, S.cellCount, s->gridHashSize, 0u);
-  launch_fill_u32(st, S.cellRigid, s->gridHashSize, 0u);
+  uint32_t* zp[4] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount};
+  uint32_t zn[4] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb};
+  launch_zero_multi(st, zp, zn, 4);
   launch_broadphase(st, S, P);
   launch_narrowphase(st, S, P, s->gridCap);
   launch_map_rebuild(st, S, P, s->gridCap);
@@ -188,17 +191,17 @@ hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S) {
   launch_contact_adjacency(st, S, P, s->gridCap);
   if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
   if (s->sweepMode == 2) {
-    hipError_t e = launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors,
-                                           s->lanes, (uint32_t)s->persistentBlocks, 0u);
+    hipError_t e = launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, s->lanes,
+                                           s->allSoft, (uint32_t)s->persistentBlocks, 0u);
     if (e != hipSuccess) return e;
   } else {
     for (uint32_t it = 0; it < P.iterations; it++) {
       if (s->dynamicColoring) {
         for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
-          launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes);
+          launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
       } else {
         for (uint32_t c = 0; c < s->staticUsedColors; c++)
-          launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes);
+          launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
       }
       launch_dual(st, S, P, s->gridCap);
     }
@@ -780,6 +783,37 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   }
   s->sAdjCount = sCount;
   s->sAdjListHost = sList;
+  // stamp-major tet slots at each tet entry's CSR position: the other three
+  // vertices and their shape-function gradients b (rows of Dm^-1; vertex 0
+  // takes minus their sum)
+  std::vector<TetSlot> slots(std::max<size_t>(1, sList.size()));
+  memset(slots.data(), 0, slots.size() * sizeof(TetSlot));
+  for (uint32_t b = 0; b < nb; b++) {
+    for (uint32_t k = 0; k < sCount[b]; k++) {
+      uint32_t e = sList[sStart[b] + k];
+      if ((e >> kAdjShift) != FK_TET) continue;
+      uint32_t t = e & kAdjIndexMask;
+      const TetRec& g = tr[t];
+      uint32_t ids[4] = {g.ids.x, g.ids.y, g.ids.z, g.ids.w};
+      V3 grad[4];
+      grad[1] = xyz(g.r0);
+      grad[2] = xyz(g.r1);
+      grad[3] = xyz(g.r2);
+      grad[0] = -(grad[1] + grad[2] + grad[3]);
+      TetSlot sl;
+      uint32_t o[3];
+      V3 ob[3];
+      int n = 0;
+      for (int q = 0; q < 4; q++)
+        if (ids[q] != b && n < 3) { o[n] = ids[q]; ob[n] = grad[q]; n++; }
+      float muV = g.r1.w, lamV = g.r2.w;
+      sl.other = U4{o[0], o[1], o[2], t};
+      sl.b1 = f4(ob[0], muV);
+      sl.b2 = f4(ob[1], lamV);
+      sl.b3 = f4(ob[2], 1.0f + muV / fmaxf(lamV, 1e-9f));
+      slots[sStart[b] + k] = sl;
+    }
+  }
 
   // ---- device buffers
   DeviceState& S = s->S;
@@ -796,6 +830,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.hashedIdx, std::max<size_t>(1, hashed.size())) ALLOC(S.globalIdx, std::max<size_t>(1, globals.size()))
   ALLOC(S.cellCount, s->gridHashSize) ALLOC(S.cellStart, s->gridHashSize) ALLOC(S.cellRigid, s->gridHashSize)
   ALLOC(S.cellBodies, std::max<size_t>(1, hashed.size())) ALLOC(S.bodyCellSlot, std::max<size_t>(1, hashed.size()))
+  ALLOC(S.hashCenter, std::max<size_t>(1, hashed.size())) ALLOC(S.hashMeta, std::max<size_t>(1, hashed.size()))
+  ALLOC(S.cellPos, std::max<size_t>(1, hashed.size())) ALLOC(S.cellMeta, std::max<size_t>(1, hashed.size()))
+  ALLOC(S.colliderToHashed, std::max<uint32_t>(1, nc))
   uint32_t producers = s->nHashed + s->nGlobal;
   ALLOC(S.pairCount, std::max<uint32_t>(1, producers)) ALLOC(S.pairStart, std::max<uint32_t>(1, producers))
   ALLOC(S.pairs, s->maxPairs) ALLOC(S.exclusions, std::max<size_t>(1, exr.size()))
@@ -804,6 +841,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.tets, std::max<uint32_t>(1, nt)) ALLOC(S.joints, std::max<uint32_t>(1, nj))
   ALLOC(S.springs, std::max<uint32_t>(1, ns))
   ALLOC(S.sAdjStart, nb) ALLOC(S.sAdjCount, nb) ALLOC(S.sAdjList, std::max<size_t>(1, sList.size()))
+  ALLOC(S.tetSlots, slots.size())
   ALLOC(S.dAdjCount, nb) ALLOC(S.dAdjStart, nb) ALLOC(S.dAdjCursor, nb)
   S.dAdjCapacity = 2 * s->maxPairs;
   ALLOC(S.dAdjList, S.dAdjCapacity)
@@ -819,6 +857,15 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(s->devScratchU32, s->devScratchWords)
 #undef ALLOC
 
+  // static metadata of the hashed colliders and the collider -> hashed index map
+  std::vector<U4> hmeta(std::max<size_t>(1, hashed.size()), U4{0, 0, 0, 0});
+  std::vector<uint32_t> c2h(std::max<uint32_t>(1, nc), 0xFFFFFFFFu);
+  for (size_t k = 0; k < hashed.size(); k++) {
+    uint32_t c = hashed[k], b = co[c];
+    hmeta[k] = U4{b, cg[c], csc[c] | ((pl[b].w > 0.0f) ? 2u : 0u), clothGroup[b]};
+    c2h[c] = (uint32_t)k;
+  }
+  if (upload(S.hashMeta, hmeta) || upload(S.colliderToHashed, c2h)) return AVBD_ERR_HIP;
   if (upload(S.posLin, pl) || upload(S.posAng, pa) || upload(S.velLin, vl) || upload(S.velAng, va) ||
       upload(S.prevVelLin, pv) || upload(S.props, pr) || upload(S.shape, sh) || upload(S.gravityScale, gs) ||
       upload(S.clothGroup, clothGroup) || upload(S.colliderOwner, co) || upload(S.colliderShapeType, ct) ||
@@ -826,7 +873,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.colliderLocalPos, cp) || upload(S.colliderLocalRot, cq) || upload(S.colliderFriction, cf) ||
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
       upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||
-      upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.posLinSnap, pl) ||
+      upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.tetSlots, slots) ||
+      upload(S.posLinSnap, pl) ||
       upload(S.posAngSnap, pa))
     return AVBD_ERR_HIP;
 
@@ -896,9 +944,25 @@ avbd_status static_coloring(avbd_solver* s) {
   cstart.push_back(s->staticUsedColors);
   if (upload(S.colorStart, cstart) || upload(S.colorList, clist)) return AVBD_ERR_HIP;
   s->staticColorCounts.assign(kMaxColors, 0);
+  s->colorSoftOnly.assign(kMaxColors, 0);
+  std::vector<uint32_t> sStart(s->nb);
+  {
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < s->nb; b++) { sStart[b] = run; run += s->sAdjCount[b]; }
+  }
+  s->allSoft = !s->dynamicColoring;
   for (int c = 0; c < kMaxColors; c++) {
     s->staticColorCounts[c] = (uint32_t)buckets[c].size();
     s->stats.color_counts[c] = (int32_t)buckets[c].size();
+    bool soft = true;
+    for (uint32_t v : buckets[c]) {
+      if (!s->isParticle[v]) { soft = false; break; }
+      for (uint32_t k = 0; k < s->sAdjCount[v] && soft; k++)
+        if ((s->sAdjListHost[sStart[v] + k] >> kAdjShift) != FK_TET) soft = false;
+      if (!soft) break;
+    }
+    s->colorSoftOnly[c] = soft ? 1 : 0;
+    if (c < (int)s->staticUsedColors && !soft) s->allSoft = false;
   }
   s->stats.max_color_used = (int32_t)s->staticUsedColors - 1;
   s->stats.num_colors = (int32_t)s->staticUsedColors;
@@ -925,7 +989,7 @@ void autotune(avbd_solver* s) {
   }
   // persistent grid size per lane count
   auto pblocks = [&](int L) {
-    int maxB = persistent_max_blocks(L);
+    int maxB = persistent_max_blocks(L, s->allSoft);
     uint32_t maxCount = 0;
     for (uint32_t c : s->staticColorCounts) maxCount = std::max(maxCount, c);
     if (s->dynamicColoring) maxCount = nb;
@@ -981,15 +1045,16 @@ void autotune(avbd_solver* s) {
       auto run = [&]() -> bool {
         if (M == 2) {
           launch_fill_u32(st, S.counters + CTR_GBAR, 2, 0u);
-          return launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors, L,
+          return launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, L, s->allSoft,
                                          (uint32_t)blocks, 0u) == hipSuccess;
         }
         for (uint32_t it = 0; it < P.iterations; it++) {
           if (s->dynamicColoring) {
-            for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++) launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L);
+            for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+              launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L, false);
           } else {
             for (uint32_t c = 0; c < s->staticUsedColors; c++)
-              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L);
+              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L, s->colorSoftOnly[c] != 0);
           }
           launch_dual(st, S, P, s->gridCap);
         }
@@ -1078,9 +1143,9 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
       if (name) names.push_back(name);
     };
     mark(nullptr);
-    launch_fill_u32(st, S.counters, CTR_TOTAL, 0u);
-    launch_fill_u32(st, S.cellCount, s->gridHashSize, 0u);
-    launch_fill_u32(st, S.cellRigid, s->gridHashSize, 0u);
+    uint32_t* zp[4] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount};
+    uint32_t zn[4] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb};
+    launch_zero_multi(st, zp, zn, 4);
     launch_broadphase(st, S, P);
     mark("broadphase");
     launch_narrowphase(st, S, P, s->gridCap);
@@ -1093,17 +1158,18 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
     if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
     mark("coloring");
     if (s->sweepMode == 2) {
-      HIP_OR_LATCH(s, launch_persistent_solve(st, S, P, nullptr, s->dynamicColoring ? 0u : s->staticUsedColors,
-                                              s->lanes, (uint32_t)s->persistentBlocks, 0u),
+      HIP_OR_LATCH(s, launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, s->lanes,
+                                              s->allSoft, (uint32_t)s->persistentBlocks, 0u),
                    frame);
       mark("solve");
     } else {
       for (uint32_t it = 0; it < P.iterations; it++) {
         if (s->dynamicColoring) {
-          for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++) launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes);
+          for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+            launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
         } else {
           for (uint32_t c = 0; c < s->staticUsedColors; c++)
-            launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes);
+            launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
         }
         launch_dual(st, S, P, s->gridCap);
       }
@@ -1539,7 +1605,7 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   hipEventRecord(e0, s->stream);
   for (int r = 0; r < reps; r++)
     for (uint32_t c = 0; c < s->staticUsedColors; c++)
-      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes);
+      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
   hipEventRecord(e1, s->stream);
   hipError_t he = hipEventSynchronize(e1);
   float ms = 0.0f;

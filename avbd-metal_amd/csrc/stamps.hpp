@@ -49,20 +49,21 @@ AVBD_HD M3 lower_to_m3(float x00, float x10, float x11, float x20, float x21, fl
   return M3{v3(x00, x10, x20), v3(x10, x11, x21), v3(x20, x21, x22)};
 }
 
-// Pose of a contact partner: the start-of-iteration snapshot when the partner
-// shares self's color (Jacobi coupling), else the live pose.
+// Pose of a contact partner: the start-of-iteration snapshot when the
+// manifold links two bodies of the same color (Jacobi coupling, flagged at
+// adjacency build), else the live pose.  useSnap = false disables the rule
+// (residual evaluation at a fixed iterate).
 struct PoseView {
   const F4* lin;
   const F4* ang;
   const F4* linSnap;
   const F4* angSnap;
-  const uint32_t* colors;
-  uint32_t selfColor;
-  __device__ __forceinline__ F4 L(uint32_t b, uint32_t self) const {
-    return (b != self && colors[b] == selfColor) ? linSnap[b] : lin[b];
+  bool useSnap;
+  __device__ __forceinline__ F4 L(uint32_t b, uint32_t self, bool same) const {
+    return (useSnap && same && b != self) ? linSnap[b] : lin[b];
   }
-  __device__ __forceinline__ F4 A(uint32_t b, uint32_t self) const {
-    return (b != self && colors[b] == selfColor) ? angSnap[b] : ang[b];
+  __device__ __forceinline__ F4 A(uint32_t b, uint32_t self, bool same) const {
+    return (useSnap && same && b != self) ? angSnap[b] : ang[b];
   }
 };
 
@@ -93,6 +94,27 @@ __device__ __forceinline__ void stamp_tet(const TetRec& t, uint32_t self, const 
   V3 g = j0 * w.x + j1 * w.y + j2 * w.z;
   acc_lin_diag(acc, muV * dot(w, w));
   acc_lin_outer(acc, g, lamV);
+}
+
+// The same stamp from a stamp-major slot: one gather level.
+__device__ __forceinline__ void stamp_tet_slot_x(const TetSlot& t, V3 xs, V3 x1, V3 x2, V3 x3, Acc& acc) {
+  V3 e1 = x1 - xs, e2 = x2 - xs, e3 = x3 - xs;
+  V3 b1 = xyz(t.b1), b2 = xyz(t.b2), b3 = xyz(t.b3);
+  float muV = t.b1.w, lamV = t.b2.w, alphaT = t.b3.w;
+  V3 f0 = e1 * b1.x + e2 * b2.x + e3 * b3.x;
+  V3 f1 = e1 * b1.y + e2 * b2.y + e3 * b3.y;
+  V3 f2 = e1 * b1.z + e2 * b2.z + e3 * b3.z;
+  V3 j0 = cross(f1, f2), j1 = cross(f2, f0), j2 = cross(f0, f1);
+  float J = dot(f0, j0);
+  float s2 = lamV * (J - alphaT);
+  V3 w = -(b1 + b2 + b3);
+  V3 g = j0 * w.x + j1 * w.y + j2 * w.z;  // dJ/dx_self
+  acc.gl += (f0 * w.x + f1 * w.y + f2 * w.z) * muV + g * s2;
+  acc_lin_diag(acc, muV * dot(w, w));
+  acc_lin_outer(acc, g, lamV);
+}
+__device__ __forceinline__ void stamp_tet_slot(const TetSlot& t, V3 xs, const F4* pos, Acc& acc) {
+  stamp_tet_slot_x(t, xs, xyz(pos[t.other.x]), xyz(pos[t.other.y]), xyz(pos[t.other.z]), acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -126,7 +148,8 @@ __device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t se
   uint32_t a = m.header.x, b = m.header.y, n = m.header.z;
   bool isA = self == a;
   bool rA_round = (m.header.w & 2u) != 0, rB_round = (m.header.w & 4u) != 0;
-  F4 pAL = pv.L(a, self), pBL = pv.L(b, self), qA = pv.A(a, self), qB = pv.A(b, self);
+  bool same = (m.header.w & kManifoldSameColor) != 0;
+  F4 pAL = pv.L(a, self, same), pBL = pv.L(b, self, same), qA = pv.A(a, self, same), qB = pv.A(b, self, same);
   V3 dqAL = xyz(pAL) - xyz(initLin[a]);
   V3 dqAA = q_sub(qA, initAng[a]);
   V3 dqBL = xyz(pBL) - xyz(initLin[b]);
@@ -309,12 +332,13 @@ __device__ __forceinline__ void stamp_spring(const SpringRec& sp, uint32_t self,
   }
 }
 
-// One adjacency entry.
-__device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView& pv, uint32_t entry, uint32_t self,
-                                            bool rigidSelf, float alpha, float dt, Acc& acc) {
+// One adjacency entry; `slot` is the entry's static-CSR position (tets
+// read their stamp-major record there).
+__device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView& pv, uint32_t entry, uint32_t slot,
+                                            uint32_t self, V3 xs, bool rigidSelf, float alpha, float dt, Acc& acc) {
   uint32_t kind = entry >> kAdjShift, idx = entry & kAdjIndexMask;
   if (kind == FK_TET) {
-    stamp_tet(S.tets[idx], self, S.posLin, acc);
+    stamp_tet_slot(S.tetSlots[slot], xs, S.posLin, acc);
   } else if (kind == FK_MANIFOLD) {
     stamp_manifold(S.manifolds[idx], self, rigidSelf, pv, S.initLin, S.initAng, alpha, acc);
   } else if (kind == FK_JOINT) {

@@ -23,6 +23,15 @@ struct DeviceState {
   // broad phase
   uint32_t *hashedIdx, *globalIdx;
   uint32_t *cellCount, *cellStart, *cellBodies, *cellRigid;
+  // per hashed collider (index = position in hashedIdx): this frame's center
+  // and |radius|, and static metadata (owner, group, shared | dynamic << 1,
+  // soft group of the owner); both re-gathered into cell order each frame so
+  // a candidate test is one level of contiguous loads
+  F4* hashCenter;
+  U4* hashMeta;
+  uint32_t* colliderToHashed;
+  F4* cellPos;
+  U4* cellMeta;
   U2* bodyCellSlot;
   uint32_t *pairCount, *pairStart;
   U2* pairs;
@@ -36,6 +45,7 @@ struct DeviceState {
   SpringRec* springs;
   // adjacency: static topology CSR (built once) + per-frame contact CSR
   uint32_t *sAdjStart, *sAdjCount, *sAdjList;
+  TetSlot* tetSlots;          // parallel to sAdjList (tet entries populated)
   uint32_t *dAdjCount, *dAdjStart, *dAdjCursor, *dAdjList;
   // colors
   uint32_t *colors, *colorsB, *colorList, *colorStart, *bodySlot;

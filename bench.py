@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cells", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
+    ap.add_argument("--sweep", type=int, default=0, help="1 per-color launches, 2 persistent (0 = autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -75,7 +77,7 @@ def main():
     from avbd_amd import GPUSolver, demos
 
     scene = demos.soft_block_scene(cells=args.cells)
-    solver = GPUSolver(scene)
+    solver = GPUSolver(scene, lanesPerBody=args.lanes, sweepMode=args.sweep)
     st = solver.stats()
     for _ in range(args.warmup):
         solver.submitStep()

@@ -1,0 +1,272 @@
+"""Pins of the oracle (CPU only).  Each test checks the oracle against
+something other than its own formula: published known answers, finite
+differences, library solves, brute force, or closed forms from the
+reference's own tests."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from avbd_amd import demos
+from avbd_amd.scene import PhysicsScene, SimSettings
+
+
+# -- SplitMix64 (Random.swift:3-21): published reference outputs for seed 0 --
+def test_splitmix64_known_answers():
+    u, f = oracle.splitmix64(0, 4)
+    assert [int(x) for x in u] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    # nextFloat = top 24 bits / 2^24
+    assert f[0] == np.float32((0xE220A8397B1DCDAF >> 40) / float(1 << 24))
+    rng = demos.SplitMix64(0)
+    assert [rng.next() for _ in range(4)] == [int(x) for x in u]
+
+
+# -- Stable Neo-Hookean stamp (40_solver.metal:965-1020) ---------------------
+def _tet():
+    rest = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.05
+    Dm = (rest[1:] - rest[0]).T
+    return rest, np.linalg.inv(Dm)
+
+
+@pytest.mark.parametrize("slot", [0, 1, 2, 3])
+def test_snh_gradient_is_energy_derivative(slot):
+    rest, dminv = _tet()
+    rng = np.random.default_rng(slot)
+    x = rest + rng.normal(0, 0.004, rest.shape)
+    mu, lam, vol = 35714.3, 142857.1, 0.05 ** 3 / 6
+    e0, g, _ = oracle.snh_tet(x, dminv, mu, lam, vol, slot)
+    h = 1e-7
+    fd = np.zeros(3)
+    for k in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[slot, k] += h
+        xm[slot, k] -= h
+        fd[k] = (oracle.snh_tet(xp, dminv, mu, lam, vol, slot)[0] - oracle.snh_tet(xm, dminv, mu, lam, vol, slot)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+
+
+def test_snh_rest_state_is_force_free():
+    # alpha_T = 1 + mu/lambda makes F = I, J = 1 an equilibrium
+    rest, dminv = _tet()
+    for slot in range(4):
+        _, g, _ = oracle.snh_tet(rest, dminv, 1000.0, 5000.0, 1.0, slot)
+        assert np.abs(g).max() < 1e-9
+
+
+def test_snh_hessian_is_vbd_approximation():
+    # H = V (mu |w|^2 I + lambda g g^T), g = dJ/dx_self checked by finite differences of det(F)
+    rest, dminv = _tet()
+    rng = np.random.default_rng(7)
+    x = rest + rng.normal(0, 0.003, rest.shape)
+    mu, lam, vol, slot = 2.0, 3.0, 0.5, 2
+    _, _, H = oracle.snh_tet(x, dminv, mu, lam, vol, slot)
+
+    def J(xx):
+        return np.linalg.det((xx[1:] - xx[0]).T @ dminv)
+
+    gJ = np.zeros(3)
+    for k in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[slot, k] += 1e-7
+        xm[slot, k] -= 1e-7
+        gJ[k] = (J(xp) - J(xm)) / 2e-7
+    w = dminv[slot - 1] if slot else -dminv.sum(axis=0)
+    ref = vol * (mu * (w @ w) * np.eye(3) + lam * np.outer(gJ, gJ))
+    np.testing.assert_allclose(H, ref, rtol=1e-6, atol=1e-9)
+    assert np.all(np.linalg.eigvalsh(H) > 0)
+
+
+# -- 6x6 LDL^T (00_common.metal:137-221) --------------------------------------
+def test_solve6x6_matches_library_solve():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        M = rng.normal(size=(6, 6))
+        A = M @ M.T + 6 * np.eye(6)
+        A[:3, :3] *= 1e4  # penalty-scale linear block next to small angular terms
+        b = rng.normal(size=6)
+        x = oracle.solve6x6(A, b, 64)
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9)
+        x32 = oracle.solve6x6(A, b, 32)
+        np.testing.assert_allclose(A @ x32, b, rtol=1e-4, atol=1e-4 * np.abs(b).max())
+
+
+# -- spatial hash (20_broadphase.metal:12-131) --------------------------------
+def _hash_py(c, group, size):
+    m = 0xFFFFFFFF
+    h = ((c[0] * 92837111) & m) ^ ((c[1] * 689287499) & m) ^ ((c[2] * 283923481) & m)
+    h ^= (group * 0x9E3779B9) & m
+    h ^= h >> 16
+    h = (h * 0x7FEB352D) & m
+    h ^= h >> 15
+    return h & (size - 1)
+
+
+def test_cell_lists_brute_force():
+    s = demos.soft_block_scene(cells=6, iterations=2)
+    o = oracle.OracleSolver(s)
+    o.step()
+    cl = o.cellLists()
+    cs = np.float32(cl["cell_size"])
+    hs = cl["hash_size"]
+    # initial positions are the ones hashed in frame 1; hashed colliders = particles
+    pos = np.array([b["position"] for b in s.bodies], np.float32)
+    expect = {}
+    for c in range(1, len(s.bodies)):
+        cell = [int(math.floor(np.float32(p) / cs)) for p in pos[c]]
+        h = _hash_py(cell, 0, hs)
+        assert h == oracle.cell_hash(pos[c], cs, 0, hs)
+        expect.setdefault(h, []).append(c)
+    for h in range(hs):
+        got = cl["bodies"][cl["start"][h]:cl["start"][h] + cl["count"][h]].tolist()
+        assert got == sorted(expect.get(h, []))
+    assert cl["start"][0] == 0 and np.all(np.diff(cl["start"].astype(np.int64)) == cl["count"][:-1])
+
+
+def _brute_pairs(scene):
+    """O(n^2) sphere-bound candidate set with the reference's filters."""
+    pos = np.array([b["position"] for b in scene.bodies], np.float32)
+    C = scene.colliders
+    out = set()
+    for a in range(len(C)):
+        for b in range(a + 1, len(C)):
+            ba, bb = C[a]["body"], C[b]["body"]
+            da = scene.bodies[ba]["density"] > 0
+            db = scene.bodies[bb]["density"] > 0
+            if ba == bb or not (da or db):
+                continue
+
+            def rad(c):
+                sz = np.array(c["size"], np.float32)
+                return sz[0] / 2 if c["shape"] == 1 else np.float32(np.sqrt(np.sum((sz * np.float32(0.5)) ** 2)))
+
+            d = pos[ba] - pos[bb]
+            r = rad(C[a]) + rad(C[b])
+            if np.dot(d, d) <= r * r:
+                out.add((a, b))
+    return out
+
+
+def test_pair_set_brute_force_rigid_cloud():
+    rng = demos.SplitMix64(11)
+    s = PhysicsScene("cloud", SimSettings(iterations=2))
+    demos.add_ground(s)
+    for i in range(60):
+        p = (rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.2, 3))
+        if i % 3 == 0:
+            s.addSphere(rng.uniform(0.2, 0.6), 1.0, 0.5, p)
+        else:
+            s.addBody((rng.uniform(0.2, 0.8),) * 3, 1.0, 0.5, p)
+    o = oracle.OracleSolver(s)
+    o.step()
+    got = o.pairs()
+    assert len(set(map(tuple, got.tolist()))) == len(got)  # no duplicates
+    assert set((int(min(a, b)), int(max(a, b))) for a, b in got) == _brute_pairs(s)
+
+
+# -- static greedy coloring (GPUSolver.swift:1956-2050) -----------------------
+def _greedy(n_bodies, edges, dynamic):
+    adj = [set() for _ in range(n_bodies)]
+    for a, b in edges:
+        if dynamic[a] and dynamic[b]:
+            adj[a].add(b)
+            adj[b].add(a)
+    col = [0] * n_bodies
+    for v in range(n_bodies):
+        if not dynamic[v]:
+            continue
+        used = {col[u] for u in adj[v] if u < v}
+        c = 0
+        while c in used:
+            c += 1
+        col[v] = c
+    return col
+
+
+def test_static_coloring_equals_serial_greedy_and_is_valid():
+    s = demos.soft_block_scene(cells=5, iterations=2)
+    o = oracle.OracleSolver(s)
+    col = o.debugColors()
+    edges = []
+    for ids, _, _ in s.tets:
+        for i in range(4):
+            for k in range(i + 1, 4):
+                edges.append((ids[i], ids[k]))
+    dyn = [b["density"] > 0 for b in s.bodies]
+    assert col.tolist() == _greedy(len(s.bodies), edges, dyn)
+    for a, b in edges:
+        assert col[a] != col[b]
+
+
+# -- closed forms ----------------------------------------------------------------
+def test_free_fall_equals_implicit_euler():
+    # CPUSolverTests.swift:50-64
+    s = PhysicsScene("fall", SimSettings(iterations=3))
+    s.addBody((1, 1, 1), 1.0, 0.5, (0, 0, 10), velocity=(0.5, 0.0, 3.0))
+    o = oracle.OracleSolver(s, precision=64)
+    n = 90
+    o.step(n)
+    p = o.bodyStates64()[0][0]
+    dt = float(np.float32(1.0 / 60.0))  # the settings wire format carries dt in float32
+    vz = 3.0 - 10.0 * dt * np.arange(1, n + 1)
+    assert abs(p[2] - (10.0 + vz.sum() * dt)) < 1e-6
+    assert abs(p[0] - 0.5 * n * dt) < 1e-9
+
+
+def test_box_rests_on_ground():
+    # CPUSolverTests.swift:35-47
+    o = oracle.OracleSolver(demos.stack_scene(1, jitter=0.0))
+    o.step(120)
+    assert abs(o.positions()[1, 2] - 0.5) < 0.03
+
+
+def test_five_stack_thresholds():
+    # CPUSolverTests.swift:80-98
+    o = oracle.OracleSolver(demos.stack_scene(5, friction=0.7, jitter=0.0))
+    o.step(240)
+    p = o.positions()
+    for i in range(5):
+        assert abs(p[1 + i, 2] - (0.5 + i)) < 0.08
+        assert np.hypot(p[1 + i, 0], p[1 + i, 1]) < 0.15
+    assert o.stats()["color_conflicts"] == 0
+
+
+def test_pendulum_keeps_length():
+    # CPUSolverTests.swift:67-77: ball-socket pendulum, hard joint holds the arm
+    s = PhysicsScene("pendulum", SimSettings(iterations=10))
+    b = s.addBody((0.2, 0.2, 0.2), 1.0, 0.5, (1.0, 0.0, 5.0), collisionEnabled=False)
+    s.addJoint(demos.SceneJoint(-1, b, (0.0, 0.0, 5.0), (-1.0, 0.0, 0.0)))
+    o = oracle.OracleSolver(s, precision=64)
+    for _ in range(120):
+        o.step()
+        p = o.bodyStates64()[0][0]
+        assert abs(np.linalg.norm(p - np.array([0, 0, 5.0])) - 1.0) < 0.01
+
+
+def test_contact_dual_invariants():
+    o = oracle.OracleSolver(demos.stack_scene(4))
+    o.step(60)
+    h, nrm, lp = o.manifolds()
+    act = h[:, 2] > 0
+    assert act.any()
+    assert np.all(lp[act, 0] <= 0.0)          # normal lambda is compressive
+    assert np.all(lp[act, 1] >= 1.0)          # penalty stays at or above its floor
+    assert np.allclose(np.linalg.norm(nrm[act], axis=1), 1.0, atol=1e-5)
+
+
+def test_soft_block_keeps_volume():
+    # SoftBodyTests.swift:53-69: Neo-Hookean volume ratio 1 +- 0.05
+    s = demos.soft_block_scene(cells=4, iterations=20, vel_sigma=0.0)
+    o = oracle.OracleSolver(s)
+    o.step(90)
+    p = o.positions().astype(np.float64)
+
+    def vol(P):
+        tot = 0.0
+        for ids, _, _ in s.tets:
+            a, b, c, d = (P[i] for i in ids)
+            tot += abs(np.dot(b - a, np.cross(c - a, d - a))) / 6
+        return tot
+
+    p0 = np.array([b["position"] for b in s.bodies], np.float64)
+    assert abs(vol(p) / vol(p0) - 1.0) < 0.05
