@@ -137,3 +137,57 @@ def chain_scene(links=8, seed=5, iterations=10, torque_sigma=5.0):
         prev = b
         top = (c[0], c[1], c[2] - L / 2)
     return s
+
+
+def batched_scene(num_scenes=4096, pitch=3.0, iterations=10, stack_range=(4, 64), chain_range=(6, 19),
+                  torque_sigma=5.0, seed_base=4 << 32):
+    """Batched uneven rigid scenes in one solver (replica layout of
+    Sources/RL/PushTEnv.swift:115-180): a grid of independent scenes, scene
+    s in collision group s + 1 over one shared group-0 ground sized to the
+    grid.  Scene s draws from SplitMix64(seed_base + s) either
+      * a stack of n ~ U{4..64} cubes of one edge U(0.2, 1) m, density 1, or
+      * a chain of n ~ U{6..19} hinge-jointed boxes (link length
+        U(0.1, 0.4) m), the first hinged to the world, random unit axes,
+        each hinge driven by a constant torque N(0, 5) N m: a velocity motor
+        whose effort saturates at |tau| (target +-1000 rad/s, kd 1e6).
+    Returns (scene, per-scene body ranges)."""
+    cols = int(math.ceil(math.sqrt(num_scenes)))
+    rows = int(math.ceil(num_scenes / cols))
+    hx = 0.5 * (cols - 1) * pitch
+    hy = 0.5 * (rows - 1) * pitch
+    s = PhysicsScene("batched", SimSettings(iterations=iterations))
+    s.addBody((2 * (hx + 3.5), 2 * (hy + 3.5), 2), 0.0, 0.7, (0, 0, -1))
+    ranges = []
+    for e in range(num_scenes):
+        rng = SplitMix64(seed_base + e)
+        cx = (e % cols) * pitch - hx
+        cy = (e // cols) * pitch - hy
+        grp = e + 1
+        first = len(s.bodies)
+        if rng.nextFloat() < 0.5:
+            n = stack_range[0] + int(rng.next() % (stack_range[1] - stack_range[0] + 1))
+            edge = rng.uniform(0.2, 1.0)
+            for i in range(n):
+                jx, jy = rng.uniform(-0.01, 0.01), rng.uniform(-0.01, 0.01)
+                s.addBody((edge, edge, edge), 1.0, 0.5, (cx + jx, cy + jy, edge * (0.5 + i)), collisionGroup=grp)
+        else:
+            n = chain_range[0] + int(rng.next() % (chain_range[1] - chain_range[0] + 1))
+            top = (cx, cy, 0.3 + 0.4 * n)
+            prev = -1
+            prevL = 0.0
+            for i in range(n):
+                L = rng.uniform(0.1, 0.4)
+                c = (top[0], top[1], top[2] - L / 2)
+                b = s.addBody((0.08, 0.08, L), 1.0, 0.5, c, collisionGroup=grp)
+                ax = np.array([rng.gauss(), rng.gauss(), rng.gauss()])
+                ax /= max(1e-9, float(np.linalg.norm(ax)))
+                tau = rng.gauss(torque_sigma)
+                rA = (0.0, 0.0, -prevL / 2) if prev >= 0 else top
+                s.addJoint(SceneJoint(prev, b, rA, (0.0, 0.0, L / 2), stiffnessLin=math.inf,
+                                      stiffnessAng=math.inf, hingeAxis=tuple(float(a) for a in ax),
+                                      motorTarget=1000.0 if tau >= 0 else -1000.0, motorTorque=abs(tau),
+                                      motorStiffness=0.0, motorDamping=1.0e6, motorMode=2))
+                prev, prevL = b, L
+                top = (c[0], c[1], c[2] - L / 2)
+        ranges.append((first, len(s.bodies)))
+    return s, ranges

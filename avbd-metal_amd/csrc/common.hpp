@@ -53,6 +53,8 @@ enum Counter : uint32_t {
   CTR_ACTIVE_MANIFOLDS = 7,
   CTR_COLOR_CHANGED = 8,   // static/dynamic coloring fixed-point flag
   CTR_DYN_ADJ_TOTAL = 9,
+  CTR_REPAIR_BAR = 10,     // grid barrier of the cooperative color repair
+  CTR_REPAIR_PASSES = 11,  // passes the repair needed (diagnostic)
   CTR_COLOR_BASE = 16,     // kMaxColors entries
   CTR_TOTAL = 16 + 2 * kMaxColors
 };

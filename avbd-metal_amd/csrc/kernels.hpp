@@ -28,6 +28,9 @@ void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepPar
 // softOnly: every body of the launch is a particle whose static entries are tets
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
                          uint32_t colorCount, int lanes, bool softOnly);
+// colors >= first of a device-sized palette, one workgroup, in order
+void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t first, int lanes);
+constexpr uint32_t kDynamicColorLaunches = 8;  // dynamic palette: colors launched one kernel each
 void launch_dual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 void launch_finalize(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks);

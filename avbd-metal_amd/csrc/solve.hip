@@ -11,6 +11,29 @@ __device__ __forceinline__ bool body_dynamic(const DeviceState& S, uint32_t b) {
   return b != kWorld && S.posLin[b].w > 0.0f;
 }
 
+// Grid barrier for cooperative launches (every workgroup co-resident).
+// Every wait is bounded: past the spin bound a block records CTR_GBAR_FAIL
+// and proceeds, and once any block has given up later barriers do not wait,
+// so the grid always drains.
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint32_t target) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(bar, 1u);
+    uint32_t spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
+           __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1u << 20)) {
+        atomicAdd(fail, 1u);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // warm start (40_solver.metal:546-722)
 // ---------------------------------------------------------------------------
@@ -291,6 +314,48 @@ __global__ void __launch_bounds__(256) k_color_iterate(DeviceState S, StepParams
   }
 }
 
+// Repair of an unfinished parallel coloring (40_solver.metal:387-436).  The
+// reference walks all bodies serially, giving each the lowest color unused
+// by its lower-index neighbours.  That serial result is the unique fixed
+// point of the same rule applied in parallel: a body's value is final once
+// all lower-index neighbours are final.  So every grid pass re-applies the
+// rule in place, and a pass that changes nothing has reached that fixed
+// point.  Cooperative launch; flags F[21 + p % 3] count changes of pass p.
+__global__ void __launch_bounds__(256) k_color_repair_coop(DeviceState S, StepParams P, uint32_t* colors,
+                                                           uint32_t finalPass) {
+  if (S.colorFlags[finalPass] == 0u) return;  // grid-uniform: the 20 Jacobi passes converged
+  uint32_t* bar = &S.counters[CTR_REPAIR_BAR];
+  uint32_t* fail = &S.counters[CTR_GBAR_FAIL];
+  uint32_t epoch = 0;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  for (uint32_t pass = 0; pass <= P.numBodies; pass++) {
+    if (tid == 0) S.colorFlags[21 + (pass + 1) % 3] = 0u;
+    bool changed = false;
+    for (uint32_t v = tid; v < P.numBodies; v += nth) {
+      uint32_t cur = __hip_atomic_load(&colors[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t nc = 0;
+      if (S.posLin[v].w > 0.0f) {
+        ColorMasks m = neighbor_masks(S, v, colors, cur);
+        uint64_t freeL = ~m.lower;
+        nc = freeL ? (uint32_t)__builtin_ctzll(freeL) : 63u;
+      }
+      if (nc != cur) {
+        __hip_atomic_store(&colors[v], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        changed = true;
+      }
+    }
+    if (__syncthreads_or(changed) && threadIdx.x == 0) atomicAdd(&S.colorFlags[21 + pass % 3], 1u);
+    epoch += gridDim.x;
+    grid_barrier(bar, fail, epoch);
+    uint32_t any = __hip_atomic_load(&S.colorFlags[21 + pass % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (any == 0u || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+      if (tid == 0) S.counters[CTR_REPAIR_PASSES] = pass + 1;
+      return;
+    }
+  }
+}
+
+// Serial form, used only when cooperative launch is unavailable.
 __global__ void k_color_repair(DeviceState S, StepParams P, uint32_t* colors, uint32_t finalPass) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (S.colorFlags[finalPass] == 0u) return;
@@ -300,6 +365,19 @@ __global__ void k_color_repair(DeviceState S, StepParams P, uint32_t* colors, ui
     uint64_t freeL = ~m.lower;
     colors[v] = freeL ? (uint32_t)__builtin_ctzll(freeL) : 63u;
   }
+}
+
+static int repair_coop_blocks(uint32_t numBodies) {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, cus = 0, per = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_color_repair_coop, 256, 0) != hipSuccess) per = 0;
+    cached = per * cus;
+  }
+  int want = (int)((numBodies + 255) / 256);
+  return want < cached ? want : cached;
 }
 
 __global__ void __launch_bounds__(256) k_color_validate(DeviceState S, StepParams P, const uint32_t* colors) {
@@ -368,7 +446,7 @@ __global__ void k_color_starts(DeviceState S, StepParams P) {
 void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepParams& P) {
   const uint32_t passes = 20;
   uint32_t bb = (P.numBodies + 255) / 256;
-  launch_fill_u32(st, S.colorFlags, passes + 1, 0u);
+  launch_fill_u32(st, S.colorFlags, passes + 4, 0u);
   uint32_t* src = S.colors;
   uint32_t* dst = S.colorsB;
   for (uint32_t p = 0; p < passes; p++) {
@@ -376,7 +454,19 @@ void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepPar
     uint32_t* t = src; src = dst; dst = t;
   }
   // after an even number of passes the final colors are back in S.colors
-  hipLaunchKernelGGL(k_color_repair, dim3(1), dim3(64), 0, st, S, P, src, passes - 1);
+  int rb = repair_coop_blocks(P.numBodies);
+  bool coop = false;
+  if (rb > 0) {
+    DeviceState s2 = S;
+    StepParams p2 = P;
+    uint32_t* c2 = src;
+    uint32_t fp = passes - 1;
+    void* args[] = {&s2, &p2, &c2, &fp};
+    coop = hipLaunchCooperativeKernel((const void*)k_color_repair_coop, dim3(rb), dim3(256), args, 0, st) ==
+           hipSuccess;
+    if (!coop) (void)hipGetLastError();
+  }
+  if (!coop) hipLaunchKernelGGL(k_color_repair, dim3(1), dim3(64), 0, st, S, P, src, passes - 1);
   hipLaunchKernelGGL(k_color_validate, dim3(bb), dim3(256), 0, st, S, P, src);
   // buckets: hist lives in the scan scratch tail of dAdjCursor-sized scratch
   uint32_t* hist = S.bodySlot;  // needs 64 * bb words; sized at create
@@ -534,6 +624,37 @@ __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uin
   bool active = slot < count;
   uint32_t body = active ? S.colorList[s + slot] : 0u;
   primal_group<L, SOFT>(S, P, active, body, lane, color);
+}
+
+// Colors [first, 64) of a device-sized palette in one workgroup, in order,
+// with a workgroup barrier between colors.  Used for the (normally empty)
+// tail of a dynamic palette so each iteration launches a fixed, small
+// number of kernels; same per-body arithmetic as k_primal, so the result
+// does not depend on which kernel processed a color.
+template <int L>
+__global__ void __launch_bounds__(256) k_primal_tail(DeviceState S, StepParams P, uint32_t first) {
+  const uint32_t groupsPerBlock = 256 / L;
+  const uint32_t lane = threadIdx.x & (L - 1), grp = threadIdx.x / L;
+  const uint32_t used = S.colorStart[kMaxColors + 1];
+  for (uint32_t c = first; c < used; c++) {
+    uint32_t s = S.colorStart[c], count = S.colorStart[c + 1] - s;
+    for (uint32_t base = 0; base < count; base += groupsPerBlock) {
+      uint32_t slot = base + grp;
+      bool active = slot < count;
+      primal_group<L, false>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t first, int lanes) {
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL(k_primal_tail<4>, dim3(1), dim3(256), 0, st, S, P, first); break;
+    case 8: hipLaunchKernelGGL(k_primal_tail<8>, dim3(1), dim3(256), 0, st, S, P, first); break;
+    case 16: hipLaunchKernelGGL(k_primal_tail<16>, dim3(1), dim3(256), 0, st, S, P, first); break;
+    default: hipLaunchKernelGGL(k_primal_tail<32>, dim3(1), dim3(256), 0, st, S, P, first); break;
+  }
 }
 
 template <bool SOFT>
@@ -819,30 +940,8 @@ void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P
 
 // ---------------------------------------------------------------------------
 // persistent cooperative sweep: iterations x colors x dual in one launch,
-// separated by grid barriers.  Every workgroup is co-resident (cooperative
-// launch), and every barrier wait is bounded: past the spin bound a block
-// records CTR_GBAR_FAIL and proceeds, so the grid always drains.
+// separated by grid barriers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint32_t target) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(bar, 1u);
-    uint32_t spins = 0;
-    // once any block has given up, later barriers do not wait at all
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
-           __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > (1u << 20)) {
-        atomicAdd(fail, 1u);
-        break;
-      }
-    }
-    __threadfence();
-  }
-  __syncthreads();
-}
-
 template <int L, bool SOFT>
 __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P, uint32_t numColors,
                                                     uint32_t barrierBase) {

@@ -197,8 +197,9 @@ hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S) {
   } else {
     for (uint32_t it = 0; it < P.iterations; it++) {
       if (s->dynamicColoring) {
-        for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+        for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
           launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
+        launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
       } else {
         for (uint32_t c = 0; c < s->staticUsedColors; c++)
           launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
@@ -1050,8 +1051,9 @@ void autotune(avbd_solver* s) {
         }
         for (uint32_t it = 0; it < P.iterations; it++) {
           if (s->dynamicColoring) {
-            for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+            for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
               launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L, false);
+            launch_primal_tail(st, S, P, kDynamicColorLaunches, L);
           } else {
             for (uint32_t c = 0; c < s->staticUsedColors; c++)
               launch_primal_color(st, S, P, c, s->staticColorCounts[c], L, s->colorSoftOnly[c] != 0);
@@ -1165,8 +1167,9 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
     } else {
       for (uint32_t it = 0; it < P.iterations; it++) {
         if (s->dynamicColoring) {
-          for (uint32_t c = 0; c < (uint32_t)kMaxColors; c++)
+          for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
             launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
+          launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
         } else {
           for (uint32_t c = 0; c < s->staticUsedColors; c++)
             launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
