@@ -27,7 +27,8 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "NO_DEVICE", 3: "ALLOC", 4: "
 # avbd_failure_kind
 FAILURE_NAMES = {0: "none", 1: "commandExecution", 2: "rigidPairCapacity", 3: "softContactCapacity",
                  4: "staticColorCapacity", 5: "unresolvedColoring", 6: "commandEncoderCreation",
-                 7: "barrierTimeout"}
+                 7: "barrierTimeout", 8: "planarDATPairCapacity", 9: "planarDATElementGridSpan",
+                 10: "planarDATInvalidAnchor"}
 
 FP = C.POINTER(C.c_float)
 U32P = C.POINTER(C.c_uint32)
@@ -72,7 +73,8 @@ class SceneDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("max_pairs_per_body", C.c_int32), ("lanes_per_body", C.c_int32),
                 ("sweep_mode", C.c_int32), ("use_graph", C.c_int32),
-                ("pair_capacity_override", C.c_int32)]
+                ("pair_capacity_override", C.c_int32), ("dat_pair_capacity_override", C.c_int32),
+                ("soft_capacity_override", C.c_int32)]
 
 
 class Failure(C.Structure):
@@ -86,7 +88,17 @@ class Stats(C.Structure):
                 ("num_active_manifolds", C.c_int32), ("max_color_used", C.c_int32),
                 ("num_colors", C.c_int32), ("color_counts", C.c_int32 * MAX_COLORS),
                 ("color_conflicts", C.c_int32), ("uses_dynamic_coloring", C.c_int32),
-                ("lanes_per_body", C.c_int32), ("sweep_mode", C.c_int32)]
+                ("lanes_per_body", C.c_int32), ("sweep_mode", C.c_int32),
+                ("dat_pairs", C.c_int32), ("dat_vt_pairs", C.c_int32), ("dat_ee_pairs", C.c_int32),
+                ("num_soft_rt", C.c_int32)]
+
+
+class ClothInfo(C.Structure):
+    _fields_ = [("num_tris", C.c_uint32), ("num_edges", C.c_uint32), ("num_particles", C.c_uint32),
+                ("num_membranes", C.c_uint32), ("num_bends", C.c_uint32), ("elem_hash_size", C.c_uint32),
+                ("max_soft", C.c_uint32), ("soft_map_capacity", C.c_uint32), ("max_dat_pairs", C.c_uint32),
+                ("elem_cell_size", C.c_float), ("elem_margin", C.c_float), ("dat_query_radius", C.c_float),
+                ("surface_contact_cell_size", C.c_float)]
 
 
 # name -> (restype, argtypes); the complete exported surface of avbd.h
@@ -110,6 +122,10 @@ SIGNATURES = {
     "avbd_get_cell_lists": (C.c_int, [C.c_void_p, U32P, U32P, FP, U32P, U32P, U32P]),
     "avbd_get_pairs": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
     "avbd_get_manifolds": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
+    "avbd_step_to_residual": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.c_int32, I32P, FP]),
+    "avbd_get_cloth_info": (C.c_int, [C.c_void_p, C.POINTER(ClothInfo), U32P, U32P]),
+    "avbd_get_dat_pairs": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
+    "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),
     "avbd_residual": (C.c_int, [C.c_void_p, FP]),

@@ -191,3 +191,143 @@ def batched_scene(num_scenes=4096, pitch=3.0, iterations=10, stack_range=(4, 64)
                 top = (c[0], c[1], c[2] - L / 2)
         ranges.append((first, len(s.bodies)))
     return s, ranges
+
+
+def cloth_scene(n=256, size=2.0, height=1.0, jitter=1.0e-3, seed=3, radius=2.5e-3, areal_density=0.3,
+                stretch_k=1.0e4, membrane_mu=300.0, bend=1.0e-3, sphere_radius=0.5, sphere_z=0.0, iterations=30,
+                friction=0.6, particle_damping=1.5, viscosity=0.25, ground=True):
+    """Cloth sheet dropped onto a fixed sphere (grid layout of
+    Demos7.swift:16-80 addClothGrid, drape settings of Demos7.swift:265-300):
+    an n x n particle grid of side `size` at z = height + U(-jitter, jitter)
+    (SplitMix64(seed), row-major), soft structural springs of stiffness
+    `stretch_k` on the grid edges, two triangles per quad with alternating
+    diagonals carrying StVK membranes (mu = lambda = membrane_mu) and Bergou
+    bending `bend`, per-node mass areal_density x spacing^2, particle radius
+    `radius` (contact radius 2 r).  The static sphere is centred at
+    (0, 0, sphere_z); the static ground's top touches its bottom."""
+    s = PhysicsScene("cloth", SimSettings(iterations=iterations, particleDamping=particle_damping,
+                                          clothViscosity=viscosity))
+    if ground:
+        s.addBody((200, 200, 2), 0.0, friction, (0, 0, sphere_z - sphere_radius - 1.0))
+    s.addSphere(2.0 * sphere_radius, 0.0, friction, (0.0, 0.0, sphere_z))
+    rng = SplitMix64(seed)
+    h = size / (n - 1)
+    mass = areal_density * h * h
+    grid = np.zeros((n, n), np.int64)
+    for i in range(n):
+        for j in range(n):
+            z = height + rng.uniform(-jitter, jitter)
+            grid[i, j] = s.addParticle(radius, mass, (i * h - size / 2, j * h - size / 2, z), friction=friction)
+    for i in range(n):
+        for j in range(n):
+            if i + 1 < n:
+                s.addSpring(int(grid[i, j]), int(grid[i + 1, j]), (0, 0, 0), (0, 0, 0), stretch_k)
+            if j + 1 < n:
+                s.addSpring(int(grid[i, j]), int(grid[i, j + 1]), (0, 0, 0), (0, 0, 0), stretch_k)
+    for i in range(n - 1):
+        for j in range(n - 1):
+            v00, v10, v01, v11 = (int(grid[i, j]), int(grid[i + 1, j]), int(grid[i, j + 1]), int(grid[i + 1, j + 1]))
+            if (i + j) % 2 == 0:
+                s.addTri((v00, v10, v11), membrane_mu, membrane_mu, bend)
+                s.addTri((v00, v11, v01), membrane_mu, membrane_mu, bend)
+            else:
+                s.addTri((v00, v10, v01), membrane_mu, membrane_mu, bend)
+                s.addTri((v10, v11, v01), membrane_mu, membrane_mu, bend)
+    return s
+
+
+def drape_scene(res=24, friction=0.6, membrane_mu=300.0, bend=5.0e-4, structural_k=5000.0, hard_rods=True):
+    """Demos.drape (Demos7.swift:265-300) built with addClothGrid defaults
+    (Demos7.swift:16-80): an n x n sheet (n = max(20, res)) of side 2.8 m at
+    z = 1.75 over a static sphere of diameter 1.1 at z = 1, structural edges as
+    hard AL rods capped at `structural_k`, StVK membranes and Bergou bending on
+    alternating-diagonal triangles, 8 g nodes, r = min(0.05, 0.3 spacing),
+    10 iterations, betaLin 2e4, particle damping 1.5, cloth viscosity 0.25."""
+    s = PhysicsScene("drape", SimSettings(iterations=10, betaLin=20000.0, particleDamping=1.5, clothViscosity=0.25))
+    add_ground(s, friction)
+    s.addSphere(1.1, 0.0, friction, (0.0, 0.0, 1.0))
+    n = max(20, res)
+    size = 2.8
+    h = size / (n - 1)
+    r = min(0.05, 0.3 * h)
+    grid = [[s.addParticle(r, 0.008, (i * h - size / 2, j * h - size / 2, 1.75), friction=friction)
+             for j in range(n)] for i in range(n)]
+    for i in range(n):
+        for j in range(n):
+            if i + 1 < n:
+                s.addSpring(grid[i][j], grid[i + 1][j], (0, 0, 0), (0, 0, 0), structural_k, hard=hard_rods)
+            if j + 1 < n:
+                s.addSpring(grid[i][j], grid[i][j + 1], (0, 0, 0), (0, 0, 0), structural_k, hard=hard_rods)
+    for i in range(n - 1):
+        for j in range(n - 1):
+            v00, v10, v01, v11 = grid[i][j], grid[i + 1][j], grid[i][j + 1], grid[i + 1][j + 1]
+            if (i + j) % 2 == 0:
+                s.addTri((v00, v10, v11), membrane_mu, membrane_mu, bend)
+                s.addTri((v00, v11, v01), membrane_mu, membrane_mu, bend)
+            else:
+                s.addTri((v00, v10, v01), membrane_mu, membrane_mu, bend)
+                s.addTri((v10, v11, v01), membrane_mu, membrane_mu, bend)
+    return s
+
+
+def cloth_metrics(scene, positions):
+    """debugClothMetrics (GPUSolver.swift:2687-2768): the minimum surface gap
+    |p - closest(p, T)| - (r_p + r_T) over particles and non-adjacent
+    triangles, and the largest relative elongation of stiff springs
+    (stiffness >= 1000 or hard)."""
+    x = np.asarray(positions, np.float64)
+    rad = np.array([b["size"][0] / 2 for b in scene.bodies])
+    tris = np.array([t[0] for t in scene.tris], np.int64)
+    nbr = {}
+    for t in tris:
+        for a in t:
+            for b in t:
+                if a != b:
+                    nbr.setdefault(int(a), set()).add(int(b))
+    parts = sorted({int(v) for t in tris for v in t if scene.bodies[int(v)]["particle"]})
+    A, B, Cc = x[tris[:, 0]], x[tris[:, 1]], x[tris[:, 2]]
+    rT = rad[tris].max(axis=1)
+    gap = np.inf
+    for v in parts:
+        p = x[v]
+        skip = np.isin(tris, list(nbr.get(v, set()) | {v})).any(axis=1)
+        q = _closest_points(p, A, B, Cc)
+        d = np.linalg.norm(p - q, axis=1) - (rad[v] + rT)
+        d[skip] = np.inf
+        gap = min(gap, float(d.min()))
+    stretch = 0.0
+    for sp in scene.springs:
+        if not (sp["k"] >= 1000 or sp["hard"]):
+            continue
+        a, b = sp["a"], sp["b"]
+        rest = sp["rest"] if sp["rest"] >= 0 else np.linalg.norm(np.subtract(scene.bodies[a]["position"],
+                                                                             scene.bodies[b]["position"]))
+        if rest > 1e-6:
+            stretch = max(stretch, max(np.linalg.norm(x[a] - x[b]) / rest - 1.0, 0.0))
+    return (0.0 if gap == np.inf else gap), stretch
+
+
+def _closest_points(p, a, b, c):
+    """Closest point of p on each triangle (rows of a, b, c), Ericson 5.1.5, vectorised."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = np.einsum("ij,ij->i", ab, ap), np.einsum("ij,ij->i", ac, ap)
+    bp = p - b
+    d3, d4 = np.einsum("ij,ij->i", ab, bp), np.einsum("ij,ij->i", ac, bp)
+    cp = p - c
+    d5, d6 = np.einsum("ij,ij->i", ab, cp), np.einsum("ij,ij->i", ac, cp)
+    vc, vb, va = d1 * d4 - d3 * d2, d5 * d2 - d1 * d6, d3 * d6 - d5 * d4
+    den = va + vb + vc
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v = vb / den
+        w = vc / den
+        out = a + ab * v[:, None] + ac * w[:, None]
+        e_bc = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+        out = np.where(((va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0))[:, None], b + (c - b) * e_bc[:, None], out)
+        e_ac = d2 / (d2 - d6)
+        out = np.where(((vb <= 0) & (d2 >= 0) & (d6 <= 0))[:, None], a + ac * e_ac[:, None], out)
+        out = np.where(((d6 >= 0) & (d5 <= d6))[:, None], c, out)
+        e_ab = d1 / (d1 - d3)
+        out = np.where(((vc <= 0) & (d1 >= 0) & (d3 <= 0))[:, None], a + ab * e_ab[:, None], out)
+        out = np.where(((d3 >= 0) & (d4 <= d3))[:, None], b, out)
+        out = np.where(((d1 <= 0) & (d2 <= 0))[:, None], a, out)
+    return out

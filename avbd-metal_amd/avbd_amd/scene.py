@@ -159,6 +159,12 @@ class PhysicsScene:
     def addTet(self, ids, mu, lam):
         self.tets.append((tuple(int(i) for i in ids), float(mu), float(lam)))
 
+    def addTri(self, ids, mu=0.0, lam=0.0, bend=0.0):
+        """SceneTri, Scene.swift:321-334: a collision triangle; mu > 0 adds an
+        StVK membrane, bend > 0 Bergou hinges on its shared edges."""
+        self.tris.append((tuple(int(i) for i in ids), float(mu), float(lam), float(bend)))
+        return len(self.tris) - 1
+
     def addCollisionExclusion(self, bodyA, bodyB):
         assert bodyA != bodyB
         self.exclusions.append((bodyA, bodyB))
@@ -242,7 +248,13 @@ class PhysicsScene:
             d.tet_ids = ptr(arr([t[0] for t in T], np.uint32), C.c_uint32)
             d.tet_mu = ptr(arr([t[1] for t in T], np.float32), C.c_float)
             d.tet_lambda = ptr(arr([t[2] for t in T], np.float32), C.c_float)
-        d.num_tris = len(self.tris)
+        R = self.tris
+        d.num_tris = len(R)
+        if R:
+            d.tri_ids = ptr(arr([t[0] for t in R], np.uint32), C.c_uint32)
+            d.tri_mu = ptr(arr([t[1] for t in R], np.float32), C.c_float)
+            d.tri_lambda = ptr(arr([t[2] for t in R], np.float32), C.c_float)
+            d.tri_bend = ptr(arr([t[3] for t in R], np.float32), C.c_float)
         E = self.exclusions
         d.num_exclusions = len(E)
         if E:

@@ -22,7 +22,7 @@ def _u32p(a):
 
 class GPUSolver:
     def __init__(self, scene, maxPairsPerBody=16, lanesPerBody=0, sweepMode=0, useGraph=True,
-                 pairCapacityForTesting=0):
+                 pairCapacityForTesting=0, planarDATPairCapacityForTesting=0, planarDATSoftCapacityForTesting=0):
         self._lib = _capi.load()
         self.scene = scene
         desc, keep = scene.to_desc()
@@ -33,6 +33,8 @@ class GPUSolver:
         opts.sweep_mode = int(sweepMode)
         opts.use_graph = 1 if useGraph else 0
         opts.pair_capacity_override = int(pairCapacityForTesting)
+        opts.dat_pair_capacity_override = int(planarDATPairCapacityForTesting)
+        opts.soft_capacity_override = int(planarDATSoftCapacityForTesting)
         h = C.c_void_p()
         st = self._lib.avbd_create(C.byref(desc), C.byref(opts), C.byref(h))
         self._h = h
@@ -80,6 +82,15 @@ class GPUSolver:
     def step(self, n=1):
         for _ in range(n):
             self._check(self._lib.avbd_step(self._h), "step")
+
+    def stepToResidual(self, tol=1e-4, maxIterations=200, checkEvery=5):
+        """One frame iterated until residual <= tol (at most maxIterations
+        sweeps, checked every checkEvery); returns (iterations, residual)."""
+        it = C.c_int32()
+        r = C.c_float()
+        self._check(self._lib.avbd_step_to_residual(self._h, float(tol), int(maxIterations), int(checkEvery),
+                                                     C.byref(it), C.byref(r)), "stepToResidual")
+        return it.value, r.value
 
     def synchronize(self):
         self._check(self._lib.avbd_synchronize(self._h), "synchronize")
@@ -212,6 +223,39 @@ class GPUSolver:
         out = np.zeros(max(1, n.value), self.MANIFOLD_DTYPE)
         self._check(self._lib.avbd_get_manifolds(self._h, n.value, C.byref(n), out.ctypes.data_as(C.c_void_p)),
                     "manifolds")
+        return out[:n.value]
+
+    SOFT_DTYPE = np.dtype([("ids", np.uint32, 4), ("normal", np.float32, 4), ("anchorA", np.float32, 4),
+                           ("weights", np.float32, 4), ("C0", np.float32, 4), ("lambda", np.float32, 4),
+                           ("penalty", np.float32, 4)])
+
+    def clothInfo(self):
+        """Surface topology and parameters derived at create (GPUSolver.swift:1289-1690)."""
+        info = _capi.ClothInfo()
+        self._check(self._lib.avbd_get_cloth_info(self._h, C.byref(info), None, None), "cloth info")
+        edges = np.zeros((max(1, info.num_edges), 2), np.uint32)
+        parts = np.zeros(max(1, info.num_particles), np.uint32)
+        self._check(self._lib.avbd_get_cloth_info(self._h, None, _u32p(edges), _u32p(parts)), "cloth info")
+        out = {f: getattr(info, f) for f, _ in _capi.ClothInfo._fields_}
+        out["edges"] = edges[:info.num_edges]
+        out["particles"] = parts[:info.num_particles]
+        return out
+
+    def datPairs(self):
+        """Planar-DAT pair stream of the last frame: rows (kind 0 V-T / 1 E-E, owner, other)."""
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_dat_pairs(self._h, 0, C.byref(n), None), "dat pairs")
+        out = np.zeros((max(1, n.value), 3), np.uint32)
+        self._check(self._lib.avbd_get_dat_pairs(self._h, n.value, C.byref(n), _u32p(out)), "dat pairs")
+        return out[:n.value]
+
+    def softContacts(self):
+        """Element contacts (V-T, rigid-T, E-E) of the last frame, SoftContactGPU records."""
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_soft_contacts(self._h, 0, C.byref(n), None), "soft contacts")
+        out = np.zeros(max(1, n.value), self.SOFT_DTYPE)
+        self._check(self._lib.avbd_get_soft_contacts(self._h, n.value, C.byref(n), out.ctypes.data_as(C.c_void_p)),
+                    "soft contacts")
         return out[:n.value]
 
     def mapFind(self, pairs):

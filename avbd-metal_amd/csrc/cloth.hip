@@ -1,0 +1,858 @@
+// cloth.hip -- cloth-surface contact pipeline for gfx950: the element grid,
+// the Planar-DAT vertex-triangle / edge-edge pair stream, OGC element-contact
+// emission (V-T, E-E and rigid feature vs triangle) with the soft
+// persistence map, and the 1-ring velocity viscosity.
+//
+// Follows 25_cloth.metal:1-333, 990-1163, 27_planar_dat.metal:1-510 and
+// 40_solver.metal:2154-2177.  The reference appends pairs and contacts with
+// atomics, so its record order depends on the schedule; here every stream is
+// built count / scan / emit (pairs additionally sorted per producer), so
+// record order -- and with it every downstream sum -- is a function of the
+// inputs only.
+#include "kernels.hpp"
+#include "grid.hpp"
+
+namespace avbd {
+
+namespace {
+
+constexpr float kFltMax = 3.402823466e38f;
+constexpr uint32_t kRtMaxPerTri = 4u;
+constexpr float kEeInteriorEps = 0.02f;
+constexpr float kDatGeometryEps = 1.0e-8f;
+constexpr uint32_t kDatLdsCap = 512u;  // per-producer sorted-emission buffer (LDS)
+
+// Ericson, Real-Time Collision Detection 5.1.5 (25_cloth.metal:26-61)
+__device__ V3 closest_pt_triangle(V3 p, V3 a, V3 b, V3 c, V3& bary) {
+  V3 ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) { bary = v3(1, 0, 0); return a; }
+  V3 bp = p - b;
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) { bary = v3(0, 1, 0); return b; }
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float v = d1 / fmaxf(d1 - d3, 1e-12f);
+    bary = v3(1 - v, v, 0);
+    return a + ab * v;
+  }
+  V3 cp = p - c;
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) { bary = v3(0, 0, 1); return c; }
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / fmaxf(d2 - d6, 1e-12f);
+    bary = v3(1 - w, 0, w);
+    return a + ac * w;
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / fmaxf((d4 - d3) + (d5 - d6), 1e-12f);
+    bary = v3(0, 1 - w, w);
+    return b + (c - b) * w;
+  }
+  float denom = va + vb + vc;
+  if (fabsf(denom) < 1e-20f) { bary = v3(1, 0, 0); return a; }
+  float v = vb / denom, w = vc / denom;
+  bary = v3(1 - v - w, v, w);
+  return a + ab * v + ac * w;
+}
+
+// closest points of two segments, parameters s on p, t on q (25_cloth.metal:632-648)
+__device__ void closest_seg_seg(V3 p0, V3 p1, V3 q0, V3 q1, float& s, float& t) {
+  V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+  float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  s = 0.0f;
+  t = 0.0f;
+  if (a <= 1e-12f && e <= 1e-12f) return;
+  if (a <= 1e-12f) { t = clampf(f / e, 0.0f, 1.0f); return; }
+  float c = dot(d1, r);
+  if (e <= 1e-12f) { s = clampf(-c / a, 0.0f, 1.0f); return; }
+  float b = dot(d1, d2);
+  float denom = a * e - b * b;
+  if (fabsf(denom) > 1e-12f) s = clampf((b * f - c * e) / denom, 0.0f, 1.0f);
+  t = (b * s + f) / e;
+  if (t < 0.0f) { t = 0.0f; s = clampf(-c / a, 0.0f, 1.0f); }
+  else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / a, 0.0f, 1.0f); }
+}
+
+__device__ __forceinline__ bool nbr_contains(const uint32_t* list, uint32_t s, uint32_t e, uint32_t x) {
+  uint32_t lo = s, hi = e;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (list[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < e && list[lo] == x;
+}
+
+__device__ __forceinline__ V3 P3(const DeviceState& S, uint32_t b) { return xyz(S.posLin[b]); }
+
+// Element AABB padded by the pair-query radius (25_cloth.metal:147-181,
+// Planar-DAT branch): triangles first, then edges.
+__device__ __forceinline__ void elem_bounds(const DeviceState& S, const StepParams& P, uint32_t gid, V3& lo, V3& hi) {
+  V3 pad = v3s(P.datQueryRadius);
+  if (gid < P.numTris) {
+    U4 t = S.tris[gid];
+    V3 a = P3(S, t.x), b = P3(S, t.y), c = P3(S, t.z);
+    lo = vmin(vmin(a, b), c) - pad;
+    hi = vmax(vmax(a, b), c) + pad;
+  } else {
+    U2 e = S.edges[gid - P.numTris];
+    V3 a = P3(S, e.x), b = P3(S, e.y);
+    lo = vmin(a, b) - pad;
+    hi = vmax(a, b) + pad;
+  }
+}
+
+struct Cell3 {
+  int x, y, z;
+};
+__device__ __forceinline__ Cell3 cell_of(V3 p, float cs) {
+  Cell3 c;
+  cell_coord(p, cs, c.x, c.y, c.z);
+  return c;
+}
+// Cells c0 .. c0 + span of an AABB, the span clamped to 3 per axis (a cover
+// of at most 4 cells); computed in 64 bits so saturated coordinates of a
+// diverged state cannot wrap into unbounded loops.
+struct CellRange {
+  Cell3 c0;
+  int sx, sy, sz;
+  bool over;  // the unclamped cover exceeded 4 cells on an axis
+};
+__device__ __forceinline__ int clamp_span(long long d) { return d < 0 ? 0 : (d > 3 ? 3 : (int)d); }
+__device__ __forceinline__ CellRange range_of(V3 lo, V3 hi, float cs) {
+  CellRange r;
+  r.c0 = cell_of(lo, cs);
+  Cell3 raw = cell_of(hi, cs);
+  long long dx = (long long)raw.x - r.c0.x, dy = (long long)raw.y - r.c0.y, dz = (long long)raw.z - r.c0.z;
+  r.over = dx > 3 || dy > 3 || dz > 3;
+  r.sx = clamp_span(dx);
+  r.sy = clamp_span(dy);
+  r.sz = clamp_span(dz);
+  return r;
+}
+__device__ __forceinline__ int cell_add(int c, int i) { return (int)((uint32_t)c + (uint32_t)i); }
+__device__ __forceinline__ bool axis_in(int c, int c0, int s) {
+  return (long long)c >= c0 && (long long)c <= (long long)c0 + s;
+}
+__device__ __forceinline__ bool in_range(const CellRange& r, Cell3 c) {
+  return axis_in(c.x, r.c0.x, r.sx) && axis_in(c.y, r.c0.y, r.sy) && axis_in(c.z, r.c0.z, r.sz);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// element grid: every triangle / edge registers in each distinct hash bucket
+// of its padded AABB cover, at most 4 cells per axis (25_cloth.metal:182-267)
+// ---------------------------------------------------------------------------
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numTris + P.numEdges) return;
+  V3 lo, hi;
+  elem_bounds(S, P, gid, lo, hi);
+  if (!finite3(lo) || !finite3(hi)) {
+    if (!SCATTER) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    return;
+  }
+  CellRange r = range_of(lo, hi, P.elemCellSize);
+  if (!SCATTER && r.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+  const Cell3 c0 = r.c0;
+  const int nx = r.sx + 1, ny = r.sy + 1, nz = r.sz + 1;
+  const int n = nx * ny * nz;  // <= 64
+  uint32_t entry = gid < P.numTris ? gid : (kElemEdgeBit | (gid - P.numTris));
+  for (int i = 0; i < n; i++) {
+    int x = cell_add(c0.x, i % nx), y = cell_add(c0.y, (i / nx) % ny), z = cell_add(c0.z, i / (nx * ny));
+    uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
+    bool dup = false;
+    for (int j = 0; j < i && !dup; j++) {
+      int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
+      dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
+    }
+    if (dup) continue;
+    if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[h], 1u)] = entry;
+    else atomicAdd(&S.elemCellCount[h], 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_copy_u32(uint32_t* dst, const uint32_t* src, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Planar-DAT pair stream (27_planar_dat.metal:35-229): producer p is surface
+// particle p (V-T query over its cell's bucket) or, past the particles, edge
+// p - numParticles (E-E over the buckets of its tight AABB, owned by A < B
+// and by the minimum cell of tight(A) x padded(B)).
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell, uint32_t ns,
+                          uint32_t ne, uint32_t entry, float q2) {
+  if ((entry & kElemEdgeBit) != 0u || entry >= P.numTris) return false;
+  U4 t = S.tris[entry];
+  if (t.x == v || t.y == v || t.z == v) return false;
+  uint32_t g = S.clothGroup[v];
+  bool same = g != 0u && S.clothGroup[t.x] == g && S.clothGroup[t.y] == g && S.clothGroup[t.z] == g;
+  if (same && S.selfCollision[v] == 0u) return false;
+  if (same && (nbr_contains(S.nbrList, ns, ne, t.x) || nbr_contains(S.nbrList, ns, ne, t.y) ||
+               nbr_contains(S.nbrList, ns, ne, t.z)))
+    return false;
+  V3 lo, hi;
+  elem_bounds(S, P, entry, lo, hi);
+  if (!in_range(range_of(lo, hi, P.elemCellSize), cell)) return false;
+  V3 bary;
+  V3 q = closest_pt_triangle(p, P3(S, t.x), P3(S, t.y), P3(S, t.z), bary);
+  V3 d = p - q;
+  return !(dot(d, d) > q2);
+}
+
+__device__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
+                          const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1, Cell3 cell,
+                          uint32_t entry, float q2) {
+  if ((entry & kElemEdgeBit) == 0u) return false;
+  uint32_t other = entry & ~kElemEdgeBit;
+  if (other <= gid || other >= P.numEdges) return false;
+  U2 b = S.edges[other];
+  if (b.x == a.x || b.x == a.y || b.y == a.x || b.y == a.y) return false;
+  uint32_t g = S.clothGroup[a.x];
+  bool same = g != 0u && S.clothGroup[a.y] == g && S.clothGroup[b.x] == g && S.clothGroup[b.y] == g;
+  if (same && S.selfCollision[a.x] == 0u) return false;
+  if (same && (nbr_contains(S.nbrList, ns0, ne0, b.x) || nbr_contains(S.nbrList, ns0, ne0, b.y) ||
+               nbr_contains(S.nbrList, ns1, ne1, b.x) || nbr_contains(S.nbrList, ns1, ne1, b.y)))
+    return false;
+  V3 blo, bhi;
+  elem_bounds(S, P, P.numTris + other, blo, bhi);
+  CellRange br = range_of(blo, bhi, P.elemCellSize);
+  // owned by the minimum cell of tight(A) x padded(B)
+  Cell3 o0{max(ar.c0.x, br.c0.x), max(ar.c0.y, br.c0.y), max(ar.c0.z, br.c0.z)};
+  if (!in_range(ar, o0) || !in_range(br, o0)) return false;
+  if (cell.x != o0.x || cell.y != o0.y || cell.z != o0.z) return false;
+  float s, t;
+  V3 b0 = P3(S, b.x), b1 = P3(S, b.y);
+  closest_seg_seg(a0, a1, b0, b1, s, t);
+  V3 ca = a0 + (a1 - a0) * s, cb = b0 + (b1 - b0) * t;
+  V3 d = ca - cb;
+  return !(dot(d, d) > q2);
+}
+
+// Visits producer `prod`'s candidates in chunks of `width` positions starting
+// at `lane`; f(accepted, other) runs once per chunk on every lane (uniform
+// trip counts, so a wave may ballot inside f).  Counters only when `diag`.
+template <typename F>
+__device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t prod, uint32_t lane, uint32_t width,
+                          bool diag, F&& f) {
+  const float q2 = P.datQueryRadius * P.datQueryRadius;
+  if (prod < P.numParticles) {
+    uint32_t v = S.particleIdx[prod];
+    V3 p = P3(S, v);
+    if (!finite3(p)) {
+      if (diag && lane == 0) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      return;
+    }
+    Cell3 cell = cell_of(p, P.elemCellSize);
+    uint32_t h = cell_hash(cell.x, cell.y, cell.z, 0u, P.elemHashSize);
+    uint32_t s = S.elemCellStart[h], e = s + S.elemCellCount[h];
+    uint32_t ns = S.nbrStart[v], ne = ns + S.nbrCount[v];
+    for (uint32_t base = s; base < e; base += width) {
+      uint32_t k = base + lane;
+      uint32_t entry = k < e ? S.elemCells[k] : 0u;
+      bool acc = k < e && vt_accept(S, P, v, p, cell, ns, ne, entry, q2);
+      f(acc, entry);
+    }
+    return;
+  }
+  uint32_t gid = prod - P.numParticles;
+  U2 a = S.edges[gid];
+  V3 a0 = P3(S, a.x), a1 = P3(S, a.y);
+  if (!finite3(a0) || !finite3(a1)) {
+    if (diag && lane == 0) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    return;
+  }
+  CellRange ar = range_of(vmin(a0, a1), vmax(a0, a1), P.elemCellSize);
+  if (diag && lane == 0 && ar.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+  uint32_t ns0 = S.nbrStart[a.x], ne0 = ns0 + S.nbrCount[a.x];
+  uint32_t ns1 = S.nbrStart[a.y], ne1 = ns1 + S.nbrCount[a.y];
+  for (int iz = 0; iz <= ar.sz; iz++)
+    for (int iy = 0; iy <= ar.sy; iy++)
+      for (int ix = 0; ix <= ar.sx; ix++) {
+        int x = cell_add(ar.c0.x, ix), y = cell_add(ar.c0.y, iy), z = cell_add(ar.c0.z, iz);
+        Cell3 cell{x, y, z};
+        uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
+        uint32_t s = S.elemCellStart[h], e = s + S.elemCellCount[h];
+        for (uint32_t base = s; base < e; base += width) {
+          uint32_t k = base + lane;
+          uint32_t entry = k < e ? S.elemCells[k] : 0u;
+          bool acc = k < e && ee_accept(S, P, gid, a, a0, a1, ar, ns0, ne0, ns1, ne1, cell, entry, q2);
+          f(acc, entry & ~kElemEdgeBit);
+        }
+      }
+}
+
+__device__ __forceinline__ U4 dat_record(const StepParams& P, uint32_t prod, uint32_t other) {
+  return prod < P.numParticles ? U4{kDatVT, 0u, other, 0u} : U4{kDatEE, prod - P.numParticles, other, 0u};
+}
+
+}  // namespace
+
+// One wave (64 lanes) per producer, four producers per block.  The count
+// pass writes datCount[prod]; the emit pass gathers the accepted partners in
+// LDS and writes them ranked by partner index at datStart[prod].
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_dat_pairs(DeviceState S, StepParams P) {
+  __shared__ uint32_t buf[4][kDatLdsCap];
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t prod = blockIdx.x * 4u + wv;
+  const uint32_t nProd = P.numParticles + P.numEdges;
+  const bool valid = prod < nProd;
+  uint32_t start = 0, want = 0;
+  if (EMIT && valid) {
+    start = S.datStart[prod];
+    want = S.datStart[prod + 1] - start;
+  }
+  const bool big = want > kDatLdsCap;
+  uint32_t n = 0;
+  if (valid && !(EMIT && (want == 0u || big))) {
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    dat_visit(S, P, prod, lane, 64u, !EMIT, [&](bool acc, uint32_t other) {
+      uint64_t m = __ballot(acc);
+      if (EMIT && acc) {
+        uint32_t r = n + (uint32_t)__popcll(m & below);
+        if (r < kDatLdsCap) buf[wv][r] = other;
+      }
+      n += (uint32_t)__popcll(m);
+    });
+  }
+  if (!EMIT) {
+    if (valid && lane == 0) S.datCount[prod] = n;
+    return;
+  }
+  __syncthreads();
+  if (!valid || want == 0u) return;
+  const uint32_t wo = prod < P.numParticles ? S.particleIdx[prod] : 0u;
+  if (!big) {
+    uint32_t cnt = n < want ? n : want;
+    for (uint32_t i = lane; i < cnt; i += 64u) {
+      uint32_t x = buf[wv][i], rank = 0;
+      for (uint32_t j = 0; j < cnt; j++) rank += buf[wv][j] < x ? 1u : 0u;
+      uint32_t slot = start + rank;
+      if (slot < P.maxDatPairs) {
+        U4 r = dat_record(P, prod, x);
+        if (r.x == kDatVT) r.y = wo;
+        S.datPairs[slot] = r;
+      }
+    }
+    return;
+  }
+  // more partners than the LDS buffer holds: one lane inserts in order
+  if (lane != 0) return;
+  uint32_t cnt = 0;
+  dat_visit(S, P, prod, 0u, 1u, false, [&](bool acc, uint32_t other) {
+    if (!acc || cnt >= want) return;
+    uint32_t j = cnt;
+    while (j > 0 && start + j - 1 < P.maxDatPairs && S.datPairs[start + j - 1].z > other) {
+      if (start + j < P.maxDatPairs) S.datPairs[start + j] = S.datPairs[start + j - 1];
+      j--;
+    }
+    if (start + j < P.maxDatPairs) {
+      U4 r = dat_record(P, prod, other);
+      if (r.x == kDatVT) r.y = wo;
+      S.datPairs[start + j] = r;
+    }
+    cnt++;
+  });
+}
+
+__global__ void k_dat_finalize(DeviceState S, StepParams P) {
+  uint32_t nProd = P.numParticles + P.numEdges;
+  uint32_t total = S.datStart[nProd];
+  uint32_t vt = S.datStart[P.numParticles];
+  S.counters[CTR_DAT_PAIRS] = total;
+  S.counters[CTR_DAT_VT] = vt;
+  S.counters[CTR_DAT_EE] = total - vt;
+}
+
+// ---------------------------------------------------------------------------
+// soft persistence map (25_cloth.metal:79-137)
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ uint32_t soft_key_hash(uint32_t a, uint32_t b, uint32_t cap) {
+  uint32_t h = (a * 0x9E3779B1u) ^ (b * 0x85EBCA77u);
+  h ^= h >> 16;
+  h *= 0xC2B2AE3Du;
+  h ^= h >> 13;
+  return h & (cap - 1u);
+}
+
+__device__ int soft_map_find(const DeviceState& S, const StepParams& P, uint32_t keyA, uint32_t keyB) {
+  uint32_t h = soft_key_hash(keyA, keyB, P.softMapCapacity);
+  for (uint32_t probe = 0; probe < 64u; probe++) {
+    uint32_t ka = __hip_atomic_load(&S.softMapKeyA[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ka == 0u) return -1;
+    if (ka == keyA + 1u && S.softMapKeyB[h] == keyB) return (int)S.softMapVal[h];
+    h = (h + 1u) & (P.softMapCapacity - 1u);
+  }
+  return -1;
+}
+
+// previous side (+1 / -1) of a persisting contact, else the proposal
+__device__ float soft_prev_side(const DeviceState& S, const StepParams& P, uint32_t keyA, uint32_t keyB,
+                                float proposed) {
+  int prev = soft_map_find(S, P, keyA, keyB);
+  if (prev < 0) return proposed;
+  return (as_uint(S.prevSoft[prev].anchorA.w) & kSoftSideNeg) ? -1.0f : 1.0f;
+}
+
+__device__ __forceinline__ float soft_lambda_cap(const StepParams& P, float minMass) {
+  return fminf(P.lambdaMax, fmaxf(10.0f, 1.0e5f * minMass));
+}
+
+// Warm start from the persistence map (tangential dual transported into the
+// new basis), mass-aware penalty floor, record fill (25_cloth.metal:273-333).
+__device__ SoftRec soft_record(const DeviceState& S, const StepParams& P, U4 ids, F4 weights, V3 n, float friction,
+                               V3 anchorA, float C0n, float restT, float lamCap, float minMass, uint32_t kind,
+                               bool rigidA, bool roundA, bool sideNeg, uint32_t keyA, uint32_t keyB) {
+  V3 lam = v3s(0), pen = v3s(0);
+  int prev = soft_map_find(S, P, keyA, keyB);
+  if (prev >= 0) {
+    const SoftRec& ps = S.prevSoft[prev];
+    lam = xyz(ps.lambda) * (P.alpha * P.gamma);
+    pen = xyz(ps.penalty);
+    V3 t1o, t2o, t1n, t2n;
+    orthonormal(xyz(ps.normal), t1o, t2o);
+    orthonormal(n, t1n, t2n);
+    V3 lt = t1o * lam.y + t2o * lam.z;
+    lam.y = dot(t1n, lt);
+    lam.z = dot(t2n, lt);
+  }
+  float penFloor = fmaxf(kPenaltyMin, minMass / (P.dt * P.dt));
+  pen = v3(clampf(pen.x * P.gamma, penFloor, kPenaltyMax), clampf(pen.y * P.gamma, penFloor, kPenaltyMax),
+           clampf(pen.z * P.gamma, penFloor, kPenaltyMax));
+  uint32_t flags = (rigidA ? kSoftRigidA : 0u) | (sideNeg ? kSoftSideNeg : 0u) | (kind << kSoftKindShift) |
+                   (roundA ? kSoftRoundA : 0u);
+  SoftRec r;
+  r.ids = ids;
+  r.normal = f4(n, friction);
+  r.anchorA = f4(rigidA ? anchorA : v3(restT, 0, 0), as_float(flags));
+  r.weights = weights;
+  r.C0 = F4{C0n, 0.0f, 0.0f, lamCap};
+  r.lambda = f4(lam, as_float(keyA));
+  r.penalty = f4(pen, as_float(keyB));
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// rigid feature points vs triangles at the step-start pose
+// (25_cloth.metal:928-1142): box corners / sphere centres of the globals and
+// hashed rigid colliders against every triangle, at most 4 per triangle
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RtFeature {
+  V3 world;
+  float radius;
+  uint32_t id;
+};
+
+__device__ int rt_features(const DeviceState& S, uint32_t c, V3 center, F4 rot, V3 m, float rT, float margin,
+                           RtFeature* out) {
+  uint32_t kind = S.colliderShapeType[c] & kShapeKindMask;
+  V3 size = xyz(S.colliderShape[c]);
+  if (kind == 1u) {
+    out[0] = RtFeature{center, size.x * 0.5f, 8u};
+    return 1;
+  }
+  if (kind != 0u) return 0;
+  V3 h3 = size * 0.5f;
+  int n = 0;
+  for (uint32_t i = 0; i < 8u; i++) {
+    V3 local = v3((i & 1u) ? h3.x : -h3.x, (i & 2u) ? h3.y : -h3.y, (i & 4u) ? h3.z : -h3.z);
+    V3 w = xform(center, rot, local);
+    if (length(w - m) > rT + 4.0f * margin) continue;
+    if (n < 4) out[n++] = RtFeature{w, 0.0f, i};
+  }
+  return n;
+}
+
+// One (collider, triangle) test; emits through f(record) up to the cap.
+template <typename F>
+__device__ void rt_test(const DeviceState& S, const StepParams& P, uint32_t tri, U4 tid, V3 a, V3 b, V3 c, V3 m,
+                        float rT, float rt_, V3 triN, V3 velT, uint32_t col, uint32_t& emitted, F&& f) {
+  uint32_t o = S.colliderOwner[col];
+  F4 bq = S.posAng[o];
+  V3 bp = P3(S, o);
+  V3 center = bp + q_rotate(bq, xyz(S.colliderLocalPos[col]));
+  F4 rot = q_mul(bq, S.colliderLocalRot[col]);
+  float rb = fabsf(S.colliderShape[col].w);
+  bool nearO = length(center - m) <= rT + rb + P.collisionMargin;
+  uint32_t kind = S.colliderShapeType[col] & kShapeKindMask;
+  if (nearO && kind == 0u) {
+    V3 lm = q_rotate(q_conj(rot), m - center);
+    V3 dq2 = vmax(vabs(lm) - xyz(S.colliderShape[col]) * 0.5f, v3s(0.0f));
+    nearO = length(dq2) <= rT + P.collisionMargin + 0.05f;
+  }
+  if (!nearO) return;
+  if (pair_excluded(S, P.numExclusions, min(o, tid.x), max(o, tid.x)) ||
+      pair_excluded(S, P.numExclusions, min(o, tid.y), max(o, tid.y)) ||
+      pair_excluded(S, P.numExclusions, min(o, tid.z), max(o, tid.z)))
+    return;
+  RtFeature feats[4];
+  int nf = rt_features(S, col, center, rot, m, rT, P.collisionMargin, feats);
+  for (int fi = 0; fi < nf && emitted < kRtMaxPerTri; fi++) {
+    V3 bary;
+    V3 q = closest_pt_triangle(feats[fi].world, a, b, c, bary);
+    V3 d = feats[fi].world - q;
+    float dist = length(d);
+    float hSum = rt_ + feats[fi].radius;
+    float inflate = fminf(length(xyz(S.velLin[o]) - velT) * P.dt, 0.5f * rT);
+    if (dist - hSum > P.elemMargin + inflate) continue;
+    float side = dot(d, triN) >= 0.0f ? 1.0f : -1.0f;
+    uint32_t keyA = (kSoftRT << 28) | col;
+    uint32_t keyB = tri * 16u + feats[fi].id;
+    float proposed = dot(bp - q, triN) >= 0.0f ? 1.0f : -1.0f;
+    bool interior = bary.x > 0.02f && bary.y > 0.02f && bary.z > 0.02f;
+    float sMem = interior ? soft_prev_side(S, P, keyA, keyB, proposed) : proposed;
+    V3 n;
+    float g;
+    if (dist > 1e-7f && side == sMem) { n = d / dist; g = dist; }
+    else { n = triN * sMem; g = -dist; }
+    float fricT = (S.props[tid.x].w + S.props[tid.y].w + S.props[tid.z].w) / 3.0f;
+    float friction = sqrtf(fmaxf(S.props[o].w * fricT, 0.0f));
+    float mo = S.posLin[o].w;
+    float minMass = mo > 0.0f ? mo : kFltMax;
+    float mA = S.posLin[tid.x].w, mB = S.posLin[tid.y].w, mC = S.posLin[tid.z].w;
+    if (mA > 0.0f) minMass = fminf(minMass, mA);
+    if (mB > 0.0f) minMass = fminf(minMass, mB);
+    if (mC > 0.0f) minMass = fminf(minMass, mC);
+    if (minMass == kFltMax) continue;
+    bool roundA = kind != 0u;
+    V3 anchor = roundA ? feats[fi].world - bp : q_rotate(q_conj(bq), feats[fi].world - bp);
+    f(o, tid, bary, n, friction, anchor, g - hSum + P.elemMargin, hSum - P.elemMargin, minMass, roundA,
+      sMem < 0.0f, keyA, keyB);
+    emitted++;
+  }
+}
+
+}  // namespace
+
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_rt_emit(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!EMIT && gid == 0) S.softFlag[P.numTris] = 0u;
+  if (gid >= P.numTris) return;
+  U4 tid = S.tris[gid];
+  V3 a = P3(S, tid.x), b = P3(S, tid.y), c = P3(S, tid.z);
+  V3 m = (a + b + c) / 3.0f;
+  float rt_ = fmaxf(fabsf(S.shape[tid.x].w), fmaxf(fabsf(S.shape[tid.y].w), fabsf(S.shape[tid.z].w)));
+  float rT = fmaxf(length(m - a), fmaxf(length(m - b), length(m - c))) + rt_;
+  V3 triN = cross(b - a, c - a);
+  float tnLen = length(triN);
+  uint32_t emitted = 0;
+  uint32_t base = EMIT ? S.softStart[gid] : 0u;
+  if (tnLen >= 1e-12f) {
+    triN = triN / tnLen;
+    V3 velT = (xyz(S.velLin[tid.x]) + xyz(S.velLin[tid.y]) + xyz(S.velLin[tid.z])) / 3.0f;
+    auto emit = [&](uint32_t o, U4 t, V3 bary, V3 n, float friction, V3 anchor, float C0n, float restT,
+                    float minMass, bool roundA, bool sideNeg, uint32_t keyA, uint32_t keyB) {
+      if (!EMIT) return;
+      uint32_t slot = base + emitted;
+      if (slot >= P.maxSoft) return;
+      S.soft[slot] = soft_record(S, P, U4{o, t.x, t.y, t.z}, F4{1.0f, -bary.x, -bary.y, -bary.z}, n, friction,
+                                 anchor, C0n, restT, soft_lambda_cap(P, minMass), minMass, kSoftRT, true, roundA,
+                                 sideNeg, keyA, keyB);
+    };
+    if (P.numHashedRigid > 0u && P.numHashedRigid <= 8u) {
+      for (uint32_t ri = 0; ri < P.numHashedRigid && emitted < kRtMaxPerTri; ri++)
+        rt_test(S, P, gid, tid, a, b, c, m, rT, rt_, triN, velT, S.hashedRigid[ri], emitted, emit);
+    } else if (P.numHashedRigid > 0u) {
+      // widen the scan so centres further than one cell are still reached
+      int R = (int)ceilf((rT + 0.5f * P.cellSize) / P.cellSize);
+      R = R < 1 ? 1 : (R > 4 ? 4 : R);
+      int cx, cy, cz;
+      cell_coord(m, P.cellSize, cx, cy, cz);
+      for (int dz = -R; dz <= R && emitted < kRtMaxPerTri; dz++)
+        for (int dy = -R; dy <= R && emitted < kRtMaxPerTri; dy++)
+          for (int dx = -R; dx <= R && emitted < kRtMaxPerTri; dx++) {
+            uint32_t h = cell_hash(cx + dx, cy + dy, cz + dz, 0u, P.gridHashSize);
+            uint32_t s = S.cellStart[h], e = s + S.cellCount[h];
+            for (uint32_t k = s; k < e && emitted < kRtMaxPerTri; k++) {
+              uint32_t col = S.cellBodies[k];
+              if (S.colliderShape[col].w < 0.0f) continue;  // particles: V-T covers them
+              rt_test(S, P, gid, tid, a, b, c, m, rT, rt_, triN, velT, col, emitted, emit);
+            }
+          }
+    }
+    for (uint32_t gI = 0; gI < P.numGlobals && emitted < kRtMaxPerTri; gI++) {
+      uint32_t col = S.globalIdx[gI];
+      if (S.colliderShape[col].w < 0.0f) continue;
+      rt_test(S, P, gid, tid, a, b, c, m, rT, rt_, triN, velT, col, emitted, emit);
+    }
+  }
+  if (!EMIT) S.softFlag[gid] = emitted;
+  else if (gid == P.numTris - 1u) S.counters[CTR_SOFT_RT] = S.softStart[P.numTris];
+}
+
+// ---------------------------------------------------------------------------
+// OGC contacts from the accepted pair stream at the predictor pose, C0
+// back-evaluated to the step-start reference (27_planar_dat.metal:344-510)
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ bool dat_contact(const DeviceState& S, const StepParams& P, U4 pair, SoftRec& out) {
+  const F4* ref = S.initLin;
+  if (pair.x == kDatVT) {
+    uint32_t v = pair.y, ti = pair.z;
+    if (ti >= P.numTris) return false;
+    U4 t = S.tris[ti];
+    V3 p = P3(S, v), a = P3(S, t.x), b = P3(S, t.y), c = P3(S, t.z);
+    V3 bary;
+    V3 q = closest_pt_triangle(p, a, b, c, bary);
+    V3 d = p - q;
+    float dist2 = dot(d, d);
+    float rv = fabsf(S.shape[v].w);
+    float rt = fmaxf(fabsf(S.shape[t.x].w), fmaxf(fabsf(S.shape[t.y].w), fabsf(S.shape[t.z].w)));
+    float hSum = rv + rt;
+    V3 velT = (xyz(S.velLin[t.x]) + xyz(S.velLin[t.y]) + xyz(S.velLin[t.z])) / 3.0f;
+    float inflate = fminf(length(xyz(S.velLin[v]) - velT) * P.dt, 0.3f * P.surfaceContactCellSize);
+    float detect = fminf(hSum + P.elemMargin + inflate, P.datQueryRadius);
+    if (dist2 > detect * detect) return false;
+    float dist = sqrtf(dist2);
+    V3 triN = cross(b - a, c - a);
+    float nl = length(triN);
+    if (nl < kDatGeometryEps) return false;
+    triN = triN / nl;
+    float side = dot(d, triN) >= 0.0f ? 1.0f : -1.0f;
+    uint32_t keyA = (kSoftVT << 28) | v, keyB = ti;
+    bool interior = bary.x > 0.02f && bary.y > 0.02f && bary.z > 0.02f;
+    float remembered = interior ? soft_prev_side(S, P, keyA, keyB, side) : side;
+    V3 n;
+    float gap;
+    if (dist > 1.0e-7f && side == remembered) { n = d / dist; gap = dist; }
+    else { n = triN * remembered; gap = -dist; }
+    float fricT = S.props[t.x].w * bary.x + S.props[t.y].w * bary.y + S.props[t.z].w * bary.z;
+    float friction = sqrtf(fmaxf(S.props[v].w * fricT, 0.0f));
+    float mv = S.posLin[v].w;
+    float minMass = mv > 0.0f ? mv : kFltMax;
+    uint32_t ids[3] = {t.x, t.y, t.z};
+    for (int j = 0; j < 3; j++) {
+      float mj = S.posLin[ids[j]].w;
+      if (mj > 0.0f) minMass = fminf(minMass, mj);
+    }
+    if (minMass == kFltMax) return false;
+    V3 dq = (p - xyz(ref[v])) - (a - xyz(ref[t.x])) * bary.x - (b - xyz(ref[t.y])) * bary.y -
+            (c - xyz(ref[t.z])) * bary.z;
+    float Cdetect = gap - hSum + P.elemMargin;
+    float C0n = Cdetect - dot(n, dq);
+    out = soft_record(S, P, U4{v, t.x, t.y, t.z}, F4{1.0f, -bary.x, -bary.y, -bary.z}, n, friction, v3s(0), C0n,
+                      hSum - P.elemMargin, soft_lambda_cap(P, minMass), minMass, kSoftVT, false, false,
+                      remembered < 0.0f, keyA, keyB);
+    return true;
+  }
+  uint32_t ea = pair.y, eb = pair.z;
+  if (ea >= P.numEdges || eb >= P.numEdges) return false;
+  U2 ai = S.edges[ea], bi = S.edges[eb];
+  V3 a0 = P3(S, ai.x), a1 = P3(S, ai.y), b0 = P3(S, bi.x), b1 = P3(S, bi.y);
+  float s, t;
+  closest_seg_seg(a0, a1, b0, b1, s, t);
+  if (s < kEeInteriorEps || s > 1.0f - kEeInteriorEps || t < kEeInteriorEps || t > 1.0f - kEeInteriorEps)
+    return false;
+  V3 ca = a0 + (a1 - a0) * s, cb = b0 + (b1 - b0) * t;
+  V3 d = ca - cb;
+  float dist = length(d);
+  float rA = fmaxf(fabsf(S.shape[ai.x].w), fabsf(S.shape[ai.y].w));
+  float rB = fmaxf(fabsf(S.shape[bi.x].w), fabsf(S.shape[bi.y].w));
+  float hSum = rA + rB;
+  V3 va = (xyz(S.velLin[ai.x]) + xyz(S.velLin[ai.y])) * 0.5f;
+  V3 vb = (xyz(S.velLin[bi.x]) + xyz(S.velLin[bi.y])) * 0.5f;
+  float inflate = fminf(length(va - vb) * P.dt, 0.3f * P.surfaceContactCellSize);
+  float detect = fminf(hSum + P.elemMargin + inflate, P.datQueryRadius);
+  if (dist > detect) return false;
+  uint32_t keyA = (kSoftEE << 28) | ea, keyB = eb;
+  int previous = soft_map_find(S, P, keyA, keyB);
+  V3 n;
+  float gap = dist;
+  if (finite_bits(dist) && dist >= 1.0e-7f) {
+    V3 nGeo = d / dist;
+    n = nGeo;
+    if (previous >= 0 && dot(nGeo, xyz(S.prevSoft[previous].normal)) < 0.0f) {
+      n = -nGeo;
+      gap = -dist;
+    }
+  } else if (previous >= 0 && finite3(xyz(S.prevSoft[previous].normal))) {
+    n = xyz(S.prevSoft[previous].normal);
+    gap = 0.0f;
+  } else {
+    // fallback normal from the edge directions / relative motion
+    V3 ua = a1 - a0, ub = b1 - b0, rel = va - vb;
+    V3 raw = cross(ua, ub);
+    float mag = length(raw);
+    if (finite_bits(mag) && mag >= kDatGeometryEps) {
+      n = raw / mag;
+    } else {
+      float la = length(ua), lb = length(ub);
+      V3 u = la >= lb ? ua : ub;
+      float ul = fmaxf(la, lb);
+      if (!finite_bits(ul) || ul < kDatGeometryEps) return false;
+      u = u / ul;
+      V3 perp = rel - u * dot(u, rel);
+      float pl = length(perp);
+      if (!finite_bits(pl) || pl < kDatGeometryEps) {
+        V3 axis = fabsf(u.x) < 0.7f ? v3(1, 0, 0) : v3(0, 1, 0);
+        perp = cross(u, axis);
+        pl = length(perp);
+      }
+      if (!finite_bits(pl) || pl < kDatGeometryEps) return false;
+      n = perp / pl;
+    }
+    if (dot(n, rel) > 0.0f) n = -n;
+    if (!finite3(n)) return false;
+    gap = 0.0f;
+  }
+  float friction = sqrtf(fmaxf((S.props[ai.x].w + S.props[ai.y].w) * 0.5f * (S.props[bi.x].w + S.props[bi.y].w) * 0.5f,
+                               0.0f));
+  float minMass = kFltMax;
+  uint32_t ids[4] = {ai.x, ai.y, bi.x, bi.y};
+  for (int j = 0; j < 4; j++) {
+    float mj = S.posLin[ids[j]].w;
+    if (mj > 0.0f) minMass = fminf(minMass, mj);
+  }
+  if (minMass == kFltMax) return false;
+  V3 dq = (a0 - xyz(ref[ai.x])) * (1.0f - s) + (a1 - xyz(ref[ai.y])) * s - (b0 - xyz(ref[bi.x])) * (1.0f - t) -
+          (b1 - xyz(ref[bi.y])) * t;
+  float Cdetect = gap - hSum + P.elemMargin;
+  float C0n = Cdetect - dot(n, dq);
+  out = soft_record(S, P, U4{ai.x, ai.y, bi.x, bi.y}, F4{1.0f - s, s, -(1.0f - t), -t}, n, friction, v3s(0), C0n,
+                    hSum - P.elemMargin, soft_lambda_cap(P, minMass), minMass, kSoftEE, false, false, false, keyA,
+                    keyB);
+  return true;
+}
+
+}  // namespace
+
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_dat_contacts(DeviceState S, StepParams P) {
+  uint32_t total = S.counters[CTR_DAT_PAIRS];
+  uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
+  uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!EMIT && i0 == 0) S.softFlag[n] = 0u;
+  uint32_t rt = EMIT ? S.counters[CTR_SOFT_RT] : 0u;
+  for (uint32_t i = i0; i < n; i += gridDim.x * blockDim.x) {
+    SoftRec r;
+    bool ok = dat_contact(S, P, S.datPairs[i], r);
+    if (!EMIT) {
+      S.softFlag[i] = ok ? 1u : 0u;
+    } else if (ok) {
+      uint32_t slot = rt + S.softStart[i];
+      if (slot < P.maxSoft) S.soft[slot] = r;
+    }
+  }
+}
+
+// total soft demand; clamp the working count (25_cloth.metal:1149-1163)
+__global__ void k_soft_finalize(DeviceState S, StepParams P) {
+  uint32_t total = S.datStart[P.numParticles + P.numEdges];
+  uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
+  uint32_t demand = S.counters[CTR_SOFT_RT] + S.softStart[n];
+  S.counters[CTR_SOFT_CANDIDATES] = demand;
+  S.counters[CTR_SOFT] = demand < P.maxSoft ? demand : P.maxSoft;
+}
+
+__global__ void __launch_bounds__(256) k_softmap_clear(DeviceState S, StepParams P) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.softMapCapacity; i += gridDim.x * blockDim.x)
+    S.softMapKeyA[i] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_softmap_insert(DeviceState S, StepParams P) {
+  uint32_t n = S.counters[CTR_SOFT];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    uint32_t keyA = as_uint(S.soft[g].lambda.w), keyB = as_uint(S.soft[g].penalty.w);
+    uint32_t h = soft_key_hash(keyA, keyB, P.softMapCapacity);
+    for (uint32_t probe = 0; probe < P.softMapCapacity; probe++) {
+      if (atomicCAS(&S.softMapKeyA[h], 0u, keyA + 1u) == 0u) {
+        S.softMapKeyB[h] = keyB;
+        S.softMapVal[h] = g;
+        break;
+      }
+      h = (h + 1u) & (P.softMapCapacity - 1u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1-ring velocity viscosity of surface particles (40_solver.metal:2154-2177)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_visc_copy(DeviceState S, StepParams P) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.numBodies) S.velScratch[i] = S.velLin[i];
+}
+__global__ void __launch_bounds__(256) k_viscosity(DeviceState S, StepParams P) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.numParticles) return;
+  uint32_t v = S.particleIdx[i];
+  uint32_t s = S.nbrStart[v], n = S.nbrCount[v];
+  if (n == 0) return;
+  V3 avg = v3s(0);
+  for (uint32_t k = s; k < s + n; k++) avg += xyz(S.velScratch[S.nbrList[k]]);
+  avg = avg / (float)n;
+  V3 vel = xyz(S.velScratch[v]);
+  S.velLin[v] = f4(vel + (avg - vel) * P.clothViscosity, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static uint32_t cap_blocks(uint32_t n, uint32_t cap) {
+  uint32_t b = (n + 255) / 256;
+  if (b > cap) b = cap;
+  return b ? b : 1;
+}
+
+void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (P.numTris == 0) return;
+  uint32_t blocks = (P.numTris + 255) / 256;
+  hipLaunchKernelGGL(k_rt_emit<false>, dim3(blocks), dim3(256), 0, st, S, P);
+  launch_exclusive_scan(st, S.softFlag, S.softStart, P.numTris + 1, S.scanScratch);
+  hipLaunchKernelGGL(k_rt_emit<true>, dim3(blocks), dim3(256), 0, st, S, P);
+}
+
+void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  if (P.numTris == 0) return;
+  uint32_t nElem = P.numTris + P.numEdges;
+  uint32_t eb = (nElem + 255) / 256;
+  hipLaunchKernelGGL(k_el_bin<false>, dim3(eb), dim3(256), 0, st, S, P);
+  launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, P.elemHashSize, S.scanScratch);
+  hipLaunchKernelGGL(k_copy_u32, dim3(cap_blocks(P.elemHashSize, gridCap)), dim3(256), 0, st, S.elemCellCursor,
+                     S.elemCellStart, P.elemHashSize);
+  hipLaunchKernelGGL(k_el_bin<true>, dim3(eb), dim3(256), 0, st, S, P);
+  uint32_t nProd = P.numParticles + P.numEdges;
+  uint32_t pb = (nProd + 3) / 4;
+  hipLaunchKernelGGL(k_dat_pairs<false>, dim3(pb), dim3(256), 0, st, S, P);
+  launch_exclusive_scan(st, S.datCount, S.datStart, nProd + 1, S.scanScratch);
+  hipLaunchKernelGGL(k_dat_pairs<true>, dim3(pb), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_dat_finalize, dim3(1), dim3(1), 0, st, S, P);
+}
+
+void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  if (P.numTris == 0) return;
+  uint32_t blocks = cap_blocks(P.maxDatPairs, gridCap);
+  hipLaunchKernelGGL(k_dat_contacts<false>, dim3(blocks), dim3(256), 0, st, S, P);
+  launch_exclusive_scan_dev(st, S.softFlag, S.softStart, P.maxDatPairs + 1, &S.counters[CTR_DAT_PAIRS], P.maxDatPairs,
+                            S.scanScratch);
+  hipLaunchKernelGGL(k_dat_contacts<true>, dim3(blocks), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_soft_finalize, dim3(1), dim3(1), 0, st, S, P);
+  hipLaunchKernelGGL(k_softmap_clear, dim3(cap_blocks(P.softMapCapacity, gridCap)), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_softmap_insert, dim3(cap_blocks(P.maxSoft, gridCap)), dim3(256), 0, st, S, P);
+}
+
+void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (P.numTris == 0 || P.numParticles == 0 || !(P.clothViscosity > 0.0f)) return;
+  hipLaunchKernelGGL(k_visc_copy, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_viscosity, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
+}
+
+}  // namespace avbd

@@ -5,6 +5,7 @@
 // contents) must be bit-reproducible, so floating-point contraction is off
 // here: every product/sum rounds exactly as written.
 #include "kernels.hpp"
+#include "grid.hpp"
 
 #pragma clang fp contract(off)
 
@@ -13,21 +14,6 @@ namespace avbd {
 // ---------------------------------------------------------------------------
 // spatial hash (20_broadphase.metal:12-32)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void cell_coord(V3 p, float cellSize, int& cx, int& cy, int& cz) {
-  cx = (int)floorf(p.x / cellSize);
-  cy = (int)floorf(p.y / cellSize);
-  cz = (int)floorf(p.z / cellSize);
-}
-// Multiply-xor of three primes, salted by collision group, then an
-// avalanche finish (xorshift-multiply) so lattice-aligned cells spread.
-__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t group, uint32_t hashSize) {
-  uint32_t h = ((uint32_t)cx * 92837111u) ^ ((uint32_t)cy * 689287499u) ^ ((uint32_t)cz * 283923481u);
-  h ^= group * 0x9e3779b9u;
-  h ^= h >> 16;
-  h *= 0x7feb352du;
-  h ^= h >> 15;
-  return h & (hashSize - 1u);
-}
 __device__ __forceinline__ V3 collider_center(const DeviceState& S, uint32_t c) {
   uint32_t b = S.colliderOwner[c];
   return xyz(S.posLin[b]) + q_rotate(S.posAng[b], xyz(S.colliderLocalPos[c]));
@@ -111,19 +97,6 @@ __global__ void __launch_bounds__(256) k_bp_gather(DeviceState S, StepParams P) 
   uint32_t g = S.colliderToHashed[S.cellBodies[k]];
   S.cellPos[k] = S.hashCenter[g];
   S.cellMeta[k] = S.hashMeta[g];
-}
-
-__device__ __forceinline__ bool pair_excluded(const DeviceState& S, uint32_t count, uint32_t a, uint32_t b) {
-  uint32_t lo = 0, hi = count;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    U2 e = S.exclusions[mid];
-    if (e.x < a || (e.x == a && e.y < b)) lo = mid + 1;
-    else hi = mid;
-  }
-  if (lo >= count) return false;
-  U2 e = S.exclusions[lo];
-  return e.x == a && e.y == b;
 }
 
 // Candidate filters shared by every producer kind (20_broadphase.metal:287-424).

@@ -26,9 +26,17 @@ constexpr uint32_t kWorld = 0xFFFFFFFFu;
 
 // force kinds packed in the top 4 bits of an adjacency entry
 constexpr uint32_t FK_JOINT = 0, FK_SPRING = 1, FK_MANIFOLD = 2, FK_TET = 3,
-                   FK_SOFT = 4, FK_MEMBRANE = 5, FK_BEND = 6;
+                   FK_SOFT = 4, FK_MEMBRANE = 5, FK_BEND = 6,
+                   FK_PSPRING = 7;  // particle-particle spring with a stamp-major slot
 constexpr uint32_t kAdjShift = 28;
 constexpr uint32_t kAdjIndexMask = 0x0FFFFFFFu;
+// soft-contact entries: index in bits 0-23, bits 24-27 flag the participant
+// slots that share the body's static color (read from the snapshot)
+constexpr uint32_t kSoftIndexMask = 0x00FFFFFFu;
+constexpr uint32_t kSoftSameShift = 24;
+constexpr uint32_t kMaxSoftRecords = 1u << 24;
+// partner id in a cloth slot: bit 31 = partner shares the body's static color
+constexpr uint32_t kSnapBit = 0x80000000u;
 
 // collider shape flags: low nibble kind, 0x10 particle, 0x20 world-round anchor
 constexpr uint32_t kShapeKindMask = 0xFu;
@@ -55,8 +63,14 @@ enum Counter : uint32_t {
   CTR_DYN_ADJ_TOTAL = 9,
   CTR_REPAIR_BAR = 10,     // grid barrier of the cooperative color repair
   CTR_REPAIR_PASSES = 11,  // passes the repair needed (diagnostic)
+  CTR_DAT_PAIRS = 12,      // surface pair stream demand of this frame (V-T + E-E)
+  CTR_DAT_OVERFLOW = 13,   // elements whose cover exceeded 4 cells on an axis
+  CTR_DAT_NONFINITE = 14,  // non-finite element / query anchors
+  CTR_SOFT_RT = 15,        // rigid-feature vs triangle records (prefix of the soft array)
   CTR_COLOR_BASE = 16,     // kMaxColors entries
-  CTR_TOTAL = 16 + 2 * kMaxColors
+  CTR_DAT_VT = 16 + 2 * kMaxColors,
+  CTR_DAT_EE,
+  CTR_TOTAL = 16 + 2 * kMaxColors + 8
 };
 
 // ---------------------------------------------------------------------------
@@ -319,6 +333,40 @@ struct alignas(16) SpringRec {
   F4 dual;        // lambda, penalty, C0
 };
 
+// One (particle, element) slot of the cloth topology, stored at the slot's
+// position in the static CSR (parallel to the entry list).  Partner ids carry
+// kSnapBit when the partner shares the particle's static color.
+//   membrane (StVK, 3 vertices):  u = {o1, o2, bits(mu A), bits(lambda A)},
+//       f = {w1.x, w1.y, w2.x, w2.y}: shape-function gradients of the two
+//       partners, F = sum_o (x_o - x_self) w_o^T, w_self = -(w1 + w2)
+//   bend (Bergou hinge, 4 vertices): u = {o1, o2, o3, bits(s)},
+//       f = {K1, K2, K3, K_self}: m = sum_o K_o (x_o - x_self)
+//   particle spring: u = {other, spring index, hard, 0}, f = {k, rest, 0, 0}
+struct alignas(16) ClothSlot {
+  U4 u;
+  F4 f;
+};
+static_assert(sizeof(ClothSlot) == 32, "ClothSlot");
+
+// Unified element contact (V-T, rigid feature - T, E-E): 4-slot weighted
+// stencil, C = C0 (1 - alpha) + sum_k w_k [n|t1|t2] . dq_k
+struct alignas(16) SoftRec {
+  U4 ids;        // participants; kWorld = unused slot
+  F4 normal;     // xyz normal (pushes slot 0 along +n), w friction
+  F4 anchorA;    // xyz rigid slot-0 anchor (rigid) or (restT, 0, 0); w = flag bits
+  F4 weights;    // signed weight per slot
+  F4 C0;         // xyz constraint at the step-start reference; w = lambda cap
+  F4 lambda;     // xyz duals; w = persistence key A (bits)
+  F4 penalty;    // xyz penalties; w = persistence key B (bits)
+};
+static_assert(sizeof(SoftRec) == 112, "SoftRec");
+constexpr uint32_t kSoftRigidA = 1u, kSoftSideNeg = 2u, kSoftKindShift = 2, kSoftRoundA = 32u;
+constexpr uint32_t kSoftVT = 1u, kSoftRT = 2u, kSoftEE = 3u;
+
+// surface pair stream record: kind (0 V-T, 1 E-E), owner, other
+constexpr uint32_t kDatVT = 0u, kDatEE = 1u;
+constexpr uint32_t kElemEdgeBit = 0x80000000u;
+
 // ---------------------------------------------------------------------------
 // per-step parameters (passed by value as a kernel argument)
 // ---------------------------------------------------------------------------
@@ -330,6 +378,9 @@ struct StepParams {
   uint32_t mapCapacity, maxPairs, gridHashSize, numHashed, numGlobals, numHashedRigid;
   uint32_t frictionCombineMode, numExclusions, iterations;
   uint32_t staticPalette;   // colors fixed for the scene's lifetime (tets / cloth present)
+  // cloth surfaces (Planar-DAT pair stream + OGC element contacts)
+  uint32_t numTris, numEdges, numParticles, elemHashSize, maxSoft, softMapCapacity, maxDatPairs;
+  float elemCellSize, elemMargin, datQueryRadius, surfaceContactCellSize, clothViscosity;
 };
 
 // ManifoldRec.header.w flag: both bodies dynamic and of the same static

@@ -10,6 +10,9 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t value);
 void launch_zero_multi(hipStream_t st, uint32_t* const* ptrs, const uint32_t* counts, int num);
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch);
 uint32_t scan_scratch_words(uint32_t n);
+// exclusive scan of min(*nDev, cap) + 1 elements (length read on the device; nMax bounds it)
+void launch_exclusive_scan_dev(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t nMax, const uint32_t* nDev,
+                               uint32_t cap, uint32_t* scratch);
 
 // collide.hip
 void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P);
@@ -18,6 +21,15 @@ void launch_map_rebuild(hipStream_t st, const DeviceState& S, const StepParams& 
 void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, const U2* q, int32_t* out,
                      uint32_t n);
 
+// cloth.hip (no-ops without cloth triangles)
+// rigid feature vs triangle contacts at the step-start pose: the prefix of this frame's soft records
+void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
+// element grid + V-T / E-E pair stream at the current (predictor) pose
+void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+// OGC records from the pair stream, soft counts, soft persistence map rebuild
+void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P);
+
 // solve.hip
 void launch_warmstart(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_contact_adjacency(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
@@ -25,9 +37,12 @@ void launch_contact_adjacency(hipStream_t st, const DeviceState& S, const StepPa
 void launch_static_color_pass(hipStream_t st, const DeviceState& S, const StepParams& P, const uint32_t* in,
                               uint32_t* out);
 void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepParams& P);
-// softOnly: every body of the launch is a particle whose static entries are tets
+// primal specialisations of a color's bodies
+constexpr int kModeGeneral = 0;  // rigid bodies and particles, any entries
+constexpr int kModeTet = 1;      // particles whose static entries are all tets
+constexpr int kModeParticle = 2; // particles only (cloth slots, springs, contacts)
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
-                         uint32_t colorCount, int lanes, bool softOnly);
+                         uint32_t colorCount, int lanes, int mode);
 // colors >= first of a device-sized palette, one workgroup, in order
 void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t first, int lanes);
 constexpr uint32_t kDynamicColorLaunches = 8;  // dynamic palette: colors launched one kernel each
@@ -37,7 +52,7 @@ void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, 
 void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t* out, uint32_t gridCap);
 // persistent cooperative sweep: all iterations x colors x dual in one launch
 hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t numColors,
-                                   int lanes, bool softOnly, uint32_t gridBlocks, uint32_t barrierBase);
-int persistent_max_blocks(int lanes, bool softOnly);
+                                   int lanes, int mode, uint32_t gridBlocks, uint32_t barrierBase);
+int persistent_max_blocks(int lanes, int mode);
 
 }  // namespace avbd

@@ -35,125 +35,22 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint
 }
 
 // ---------------------------------------------------------------------------
-// warm start (40_solver.metal:546-722)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_warmstart_constraints(DeviceState S, StepParams P) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= P.numJoints) {
-    uint32_t si = gid - P.numJoints;
-    if (si >= P.numSprings) return;
-    SpringRec& sp = S.springs[si];
-    if (sp.header.z == 0) return;
-    uint32_t a = sp.header.x, b = sp.header.y;
-    V3 pA = xform(xyz(S.posLin[a]), S.posAng[a], xyz(sp.rA));
-    V3 pB = xform(xyz(S.posLin[b]), S.posAng[b], xyz(sp.rB));
-    sp.dual.z = 0.5f * fmaxf(length(pA - pB) - sp.rB.w, 0.0f);
-    float warm = P.alpha * P.gamma;
-    sp.dual.x *= warm;
-    sp.dual.y = fminf(clampf(sp.dual.y * P.gamma, kPenaltyMin, kPenaltyMax), sp.rA.w);
-    return;
-  }
-  JointRec& j = S.joints[gid];
-  if (j.header.z != 0) return;
-  uint32_t a = j.header.x, b = j.header.y;
-  V3 pA = a == kWorld ? xyz(j.rA) : xform(xyz(S.posLin[a]), S.posAng[a], xyz(j.rA));
-  V3 pB = xform(xyz(S.posLin[b]), S.posAng[b], xyz(j.rB));
-  F4 qA = a == kWorld ? F4{0, 0, 0, 1} : S.posAng[a];
-  float torqueArm = j.C0Lin.w;
-  j.C0Lin = f4(pA - pB, torqueArm);
-  V3 c0a;
-  V3 axis = xyz(j.hingeAxis);
-  if (j.hingeAxis.w != 0.0f) {
-    V3 aB = q_rotate(S.posAng[b], axis);
-    V3 aA = q_rotate(q_mul(qA, j.restRel), axis);
-    c0a = cross(aA, aB) * torqueArm;
-    if (j.dynamics.x > 0.0f) {
-      float twist = hinge_twist(q_mul(qA, j.restRel), S.posAng[b], axis);
-      V3 omegaA = a == kWorld ? v3s(0) : xyz(S.velAng[a]);
-      float rate = dot(xyz(S.velAng[b]) - omegaA, aB);
-      j.dynamics.y = twist + P.dt * rate;
-    }
-  } else {
-    c0a = q_sub(q_mul(qA, j.restRel), S.posAng[b]) * torqueArm;
-  }
-  j.C0Ang = f4(c0a, j.C0Ang.w);
-  if (j.motor.y > 0.0f) {
-    j.motor.z = 0.0f;
-    j.dynamics.z = 0.0f;
-    uint32_t mode = j.header.w & kMotorModeMask;
-    if (mode == kMotorExplicitPD || mode == kMotorVelocity) {
-      F4 qAr = a == kWorld ? j.restRel : q_mul(S.posAng[a], j.restRel);
-      float twist = hinge_twist(qAr, S.posAng[b], axis);
-      V3 axW = q_rotate(S.posAng[b], axis);
-      V3 omegaA = a == kWorld ? v3s(0) : xyz(S.velAng[a]);
-      float rate = dot(xyz(S.velAng[b]) - omegaA, axW);
-      if (mode == kMotorVelocity)
-        j.dynamics.z = clampf(j.limits.z * (j.motor.x - rate), -j.motor.y, j.motor.y);
-      else
-        j.dynamics.z = clampf(j.motor.w * (j.motor.x - twist) - j.limits.z * rate, -j.motor.y, j.motor.y);
-    }
-  }
-  float warm = P.alpha * P.gamma;
-  float stiffLin = j.rA.w, stiffAng = j.rB.w;
-  j.lambdaLin.x *= warm; j.lambdaLin.y *= warm; j.lambdaLin.z *= warm;
-  j.lambdaAng.x *= warm; j.lambdaAng.y *= warm; j.lambdaAng.z *= warm;
-  j.penaltyLin.x = fminf(clampf(j.penaltyLin.x * P.gamma, kPenaltyMin, kPenaltyMax), stiffLin);
-  j.penaltyLin.y = fminf(clampf(j.penaltyLin.y * P.gamma, kPenaltyMin, kPenaltyMax), stiffLin);
-  j.penaltyLin.z = fminf(clampf(j.penaltyLin.z * P.gamma, kPenaltyMin, kPenaltyMax), stiffLin);
-  j.penaltyAng.x = fminf(clampf(j.penaltyAng.x * P.gamma, kPenaltyMin, kPenaltyMax), stiffAng);
-  j.penaltyAng.y = fminf(clampf(j.penaltyAng.y * P.gamma, kPenaltyMin, kPenaltyMax), stiffAng);
-  j.penaltyAng.z = fminf(clampf(j.penaltyAng.z * P.gamma, kPenaltyMin, kPenaltyMax), stiffAng);
-}
-
-// Inertial predict x~ = x + v dt + g s dt^2, q~ = q (+) w dt; adaptive
-// warm-start guess with the gravity weight clamp(a_z sgn(g)/|g|, 0, 1);
-// start-of-step pose saved; also seeds the first iteration's snapshot.
-__global__ void __launch_bounds__(256) k_warmstart_bodies(DeviceState S, StepParams P) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= P.numBodies) return;
-  F4 pl = S.posLin[gid], pa = S.posAng[gid];
-  float mass = pl.w, dt = P.dt;
-  V3 vl = xyz(S.velLin[gid]), va = xyz(S.velAng[gid]);
-  V3 inertial = xyz(pl) + vl * dt;
-  float g = P.gravity * S.gravityScale[gid];
-  if (mass > 0.0f) inertial += v3(0, 0, g) * (dt * dt);
-  S.inertLin[gid] = f4(inertial, 0.0f);
-  F4 qi = q_addw(pa, va * dt);
-  S.inertAng[gid] = qi;
-  V3 accel = (vl - xyz(S.prevVelLin[gid])) / dt;
-  float gsign = P.gravity < 0.0f ? -1.0f : (P.gravity > 0.0f ? 1.0f : 0.0f);
-  float w = clampf(accel.z * gsign / fabsf(P.gravity), 0.0f, 1.0f);
-  if (!finite_bits(w)) w = 0.0f;
-  S.initLin[gid] = f4(xyz(pl), 0.0f);
-  S.initAng[gid] = pa;
-  F4 newL = pl, newA = pa;
-  if (mass > 0.0f) {
-    V3 guess = xyz(pl) + vl * dt + v3(0, 0, g) * (w * dt * dt);
-    newL = f4(guess, mass);
-    newA = qi;
-    S.posLin[gid] = newL;
-    S.posAng[gid] = newA;
-  }
-  S.posLinSnap[gid] = newL;
-  S.posAngSnap[gid] = newA;
-}
-
-void launch_warmstart(hipStream_t st, const DeviceState& S, const StepParams& P) {
-  uint32_t nc = P.numJoints + P.numSprings;
-  if (nc > 0) hipLaunchKernelGGL(k_warmstart_constraints, dim3((nc + 255) / 256), dim3(256), 0, st, S, P);
-  if (P.numBodies > 0)
-    hipLaunchKernelGGL(k_warmstart_bodies, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
-}
-
-// ---------------------------------------------------------------------------
 // per-frame contact adjacency: count / scan / scatter / sort of active
 // manifolds into a second CSR beside the static topology CSR
 // (40_solver.metal:18-243 restricted to the per-frame forces)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_cadj_count(DeviceState S, StepParams P) {
-  uint32_t n = S.counters[CTR_PAIRS];
+  uint32_t n = S.counters[CTR_PAIRS], ns = S.counters[CTR_SOFT];
   uint32_t active = 0;
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n + ns; m += gridDim.x * blockDim.x) {
+    if (m >= n) {
+      // element contact: one entry per dynamic participant
+      U4 ids = S.soft[m - n].ids;
+      uint32_t v[4] = {ids.x, ids.y, ids.z, ids.w};
+      for (int k = 0; k < 4; k++)
+        if (body_dynamic(S, v[k])) atomicAdd(&S.dAdjCount[v[k]], 1u);
+      continue;
+    }
     U4 h = S.manifolds[m].header;
     if (h.z == 0) continue;
     active++;
@@ -168,8 +65,28 @@ __global__ void __launch_bounds__(256) k_cadj_cursor(DeviceState S, StepParams P
   S.dAdjCursor[gid] = S.dAdjStart[gid];
 }
 __global__ void __launch_bounds__(256) k_cadj_scatter(DeviceState S, StepParams P) {
-  uint32_t n = S.counters[CTR_PAIRS];
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+  uint32_t n = S.counters[CTR_PAIRS], ns = S.counters[CTR_SOFT];
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n + ns; m += gridDim.x * blockDim.x) {
+    if (m >= n) {
+      // same-color participants (static palette) are read from the snapshot
+      uint32_t si = m - n;
+      U4 ids = S.soft[si].ids;
+      uint32_t v[4] = {ids.x, ids.y, ids.z, ids.w};
+      bool dv[4];
+      for (int k = 0; k < 4; k++) dv[k] = body_dynamic(S, v[k]);
+      for (int k = 0; k < 4; k++) {
+        if (!dv[k]) continue;
+        uint32_t same = 0;
+        if (P.staticPalette) {
+          uint32_t ck = S.colors[v[k]];
+          for (int j = 0; j < 4; j++)
+            if (j != k && dv[j] && v[j] != v[k] && S.colors[v[j]] == ck) same |= 1u << j;
+        }
+        uint32_t entry = (FK_SOFT << kAdjShift) | (same << kSoftSameShift) | si;
+        S.dAdjList[atomicAdd(&S.dAdjCursor[v[k]], 1u)] = entry;
+      }
+      continue;
+    }
     U4 h = S.manifolds[m].header;
     if (h.z == 0) continue;
     uint32_t entry = (FK_MANIFOLD << kAdjShift) | m;
@@ -199,7 +116,7 @@ __global__ void __launch_bounds__(256) k_cadj_sort(DeviceState S, StepParams P) 
 
 // dAdjCount must be zero on entry (cleared with the frame's counters)
 void launch_contact_adjacency(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  uint32_t blocks = (P.maxPairs + 255) / 256;
+  uint32_t blocks = (P.maxPairs + P.maxSoft + 255) / 256;
   if (blocks > gridCap) blocks = gridCap;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_cadj_count, dim3(blocks), dim3(256), 0, st, S, P);
@@ -226,7 +143,7 @@ __device__ __forceinline__ void for_each_neighbor(const DeviceState& S, uint32_t
   } else if (kind == FK_JOINT) {
     U4 h = S.joints[idx].header;
     f(h.x == self ? h.y : h.x);
-  } else if (kind == FK_SPRING) {
+  } else if (kind == FK_SPRING || kind == FK_PSPRING) {
     U4 h = S.springs[idx].header;
     f(h.x == self ? h.y : h.x);
   } else if (kind == FK_MANIFOLD && !staticOnly) {
@@ -507,11 +424,12 @@ __device__ __forceinline__ V3 group_sum3(V3 v) {
 // Gathers and stamps one body's entries (lane-strided), reduces, and lane 0
 // solves and writes.  Must be reached by whole waves (EXEC full) because of
 // the ballot and DPP reductions; inactive groups pass active = false.
-template <int L, bool SOFT>
+template <int L, int MODE>
 __device__ __forceinline__ void primal_group(const DeviceState& S, const StepParams& P, bool active, uint32_t body,
                                              uint32_t lane, uint32_t color) {
-  bool rigid = !SOFT && active && S.shape[body].w >= 0.0f;
-  const bool anyRigid = !SOFT && __ballot(rigid) != 0ull;
+  constexpr bool SOFT = MODE == kModeTet;
+  bool rigid = MODE == kModeGeneral && active && S.shape[body].w >= 0.0f;
+  const bool anyRigid = MODE == kModeGeneral && __ballot(rigid) != 0ull;
   Acc acc;
   acc_zero(acc);
   if (active) {
@@ -545,9 +463,30 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
           if (base + i * L < sc) stamp_tet_slot_x(r[i], xs, x[i][0], x[i][1], x[i][2], acc);
       }
       // contact entries continue the lane round-robin after the tets
-      for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L)
-        stamp_manifold(S.manifolds[S.dAdjList[ds + k] & kAdjIndexMask], body, false, pv, S.initLin, S.initAng,
-                       P.alpha, acc);
+      for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L) {
+        uint32_t entry = S.dAdjList[ds + k];
+        if ((entry >> kAdjShift) == FK_SOFT)
+          stamp_soft(S.soft[entry & kSoftIndexMask], body, (entry >> kSoftSameShift) & 15u, pv, S.initLin, S.initAng,
+                     P.alpha, acc);
+        else
+          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc);
+      }
+    } else if (MODE == kModeParticle) {
+      // particles: cloth slots inline, any other static entry generically
+      for (uint32_t k = lane; k < sc; k += L) {
+        uint32_t entry = S.sAdjList[ss + k];
+        uint32_t kind = entry >> kAdjShift;
+        if (kind >= FK_MEMBRANE) stamp_cloth_slot(kind, S.clothSlots[ss + k], pv, xs, S.springs, P.alpha, acc);
+        else stamp_entry(S, pv, entry, ss + k, body, xs, false, P.alpha, P.dt, acc);
+      }
+      for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L) {
+        uint32_t entry = S.dAdjList[ds + k];
+        if ((entry >> kAdjShift) == FK_SOFT)
+          stamp_soft(S.soft[entry & kSoftIndexMask], body, (entry >> kSoftSameShift) & 15u, pv, S.initLin, S.initAng,
+                     P.alpha, acc);
+        else
+          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc);
+      }
     } else {
       for (uint32_t k = lane; k < sc + dc; k += L) {
         uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
@@ -613,7 +552,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
   return r;
 }
 
-template <int L, bool SOFT>
+template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count) {
   const uint32_t groupsPerBlock = 256 / L;
   uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -623,7 +562,7 @@ __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uin
   if (count == 0xFFFFFFFFu) count = S.colorStart[color + 1] - s;  // device-sized (dynamic palette)
   bool active = slot < count;
   uint32_t body = active ? S.colorList[s + slot] : 0u;
-  primal_group<L, SOFT>(S, P, active, body, lane, color);
+  primal_group<L, MODE>(S, P, active, body, lane, color);
 }
 
 // Colors [first, 64) of a device-sized palette in one workgroup, in order,
@@ -641,7 +580,7 @@ __global__ void __launch_bounds__(256) k_primal_tail(DeviceState S, StepParams P
     for (uint32_t base = 0; base < count; base += groupsPerBlock) {
       uint32_t slot = base + grp;
       bool active = slot < count;
-      primal_group<L, false>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+      primal_group<L, kModeGeneral>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
     }
     __threadfence_block();
     __syncthreads();
@@ -657,27 +596,28 @@ void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& 
   }
 }
 
-template <bool SOFT>
+template <int MODE>
 static void launch_primal_t(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
                             uint32_t count, int lanes, uint32_t blocks) {
   switch (lanes) {
-    case 4: hipLaunchKernelGGL((k_primal<4, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 8: hipLaunchKernelGGL((k_primal<8, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 16: hipLaunchKernelGGL((k_primal<16, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    default: hipLaunchKernelGGL((k_primal<32, SOFT>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 4: hipLaunchKernelGGL((k_primal<4, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 8: hipLaunchKernelGGL((k_primal<8, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 16: hipLaunchKernelGGL((k_primal<16, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    default: hipLaunchKernelGGL((k_primal<32, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
   }
 }
 
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t count,
-                         int lanes, bool softOnly) {
+                         int lanes, int mode) {
   if (count == 0) return;
   uint32_t groupsPerBlock = 256 / lanes;
   uint32_t n = count == 0xFFFFFFFFu ? P.numBodies : count;
   uint32_t blocks = (n + groupsPerBlock - 1) / groupsPerBlock;
   // the XCD remap needs the padded slot range to cover every body
   blocks = (blocks + 7) / 8 * 8;
-  if (softOnly) launch_primal_t<true>(st, S, P, color, count, lanes, blocks);
-  else launch_primal_t<false>(st, S, P, color, count, lanes, blocks);
+  if (mode == kModeTet) launch_primal_t<kModeTet>(st, S, P, color, count, lanes, blocks);
+  else if (mode == kModeParticle) launch_primal_t<kModeParticle>(st, S, P, color, count, lanes, blocks);
+  else launch_primal_t<kModeGeneral>(st, S, P, color, count, lanes, blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -787,19 +727,47 @@ __device__ __forceinline__ void dual_manifold(const DeviceState& S, const StepPa
   }
 }
 
-// one item of the fused dual pass over [joints | springs | manifolds], plus
-// the next iteration's pose snapshot over bodies
+// element contact dual: the quadratic stage only (the log-barrier stage is
+// dual-free), mass-scaled normal cap, penalties capped at 1e6
+// (40_solver.metal:1689-1727)
+__device__ __forceinline__ void dual_soft(const DeviceState& S, const StepParams& P, uint32_t gid) {
+  SoftRec& sc = S.soft[gid];
+  PoseView pv{S.posLin, S.posAng, S.posLin, S.posAng, false};
+  V3 n, t1, t2, rAW;
+  V3 C = soft_contact_C(sc, pv, 0u, S.initLin, S.initAng, P.alpha, n, t1, t2, rAW);
+  V3 F = xyz(sc.penalty) * C + xyz(sc.lambda);
+  F.x = fminf(F.x, 0.0f);
+  float bnd = fabsf(F.x) * sc.normal.w;
+  float fs = sqrtf(F.y * F.y + F.z * F.z);
+  if (fs > bnd && fs > 0.0f) {
+    F.y *= bnd / fs;
+    F.z *= bnd / fs;
+  }
+  F.x = fmaxf(F.x, -sc.C0.w);
+  sc.lambda = f4(F, sc.lambda.w);
+  V3 pen = xyz(sc.penalty);
+  if (F.x < 0.0f) pen.x = fminf(pen.x + P.betaLin * fabsf(C.x), kPenaltyMaxT);
+  if (fs <= bnd) {
+    pen.y = fminf(pen.y + P.betaLin * fabsf(C.y), kPenaltyMaxT);
+    pen.z = fminf(pen.z + P.betaLin * fabsf(C.z), kPenaltyMaxT);
+  }
+  sc.penalty = f4(pen, sc.penalty.w);
+}
+
+// one item of the fused dual pass over [joints | springs | manifolds | soft],
+// plus the next iteration's pose snapshot over bodies
 __device__ __forceinline__ void dual_item(const DeviceState& S, const StepParams& P, uint32_t gid, uint32_t numPairs) {
   if (gid < P.numJoints) { dual_joint(S, P, gid); return; }
   uint32_t si = gid - P.numJoints;
   if (si < P.numSprings) { dual_spring(S, P, si); return; }
   uint32_t mi = si - P.numSprings;
-  if (mi < numPairs) dual_manifold(S, P, mi);
+  if (mi < numPairs) { dual_manifold(S, P, mi); return; }
+  dual_soft(S, P, mi - numPairs);
 }
 
 __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
   uint32_t numPairs = S.counters[CTR_PAIRS];
-  uint32_t total = P.numJoints + P.numSprings + numPairs;
+  uint32_t total = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += stride) dual_item(S, P, gid, numPairs);
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < P.numBodies; b += stride) {
@@ -809,7 +777,7 @@ __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
 }
 
 void launch_dual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  uint32_t n = P.numJoints + P.numSprings + P.maxPairs;
+  uint32_t n = P.numJoints + P.numSprings + P.maxPairs + (P.numTris ? P.maxSoft : 0u);
   if (P.numBodies > n) n = P.numBodies;
   uint32_t blocks = (n + 255) / 256;
   if (blocks > gridCap) blocks = gridCap;
@@ -942,7 +910,7 @@ void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P
 // persistent cooperative sweep: iterations x colors x dual in one launch,
 // separated by grid barriers
 // ---------------------------------------------------------------------------
-template <int L, bool SOFT>
+template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P, uint32_t numColors,
                                                     uint32_t barrierBase) {
   const uint32_t groupsPerBlock = 256 / L;
@@ -954,7 +922,7 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
   uint32_t epoch = barrierBase;
   const uint32_t colorsUsed = numColors ? numColors : S.colorStart[kMaxColors + 1];
   const uint32_t numPairs = S.counters[CTR_PAIRS];
-  const uint32_t dualTotal = P.numJoints + P.numSprings + numPairs;
+  const uint32_t dualTotal = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
   for (uint32_t it = 0; it < P.iterations; it++) {
     for (uint32_t c = 0; c < colorsUsed; c++) {
@@ -964,7 +932,7 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
       for (uint32_t r = 0; r < rounds; r++) {
         uint32_t slot = r * ngroups + gslot;
         bool active = slot < count;
-        primal_group<L, SOFT>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+        primal_group<L, MODE>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
       }
       epoch += gridDim.x;
       grid_barrier(bar, fail, epoch);
@@ -979,39 +947,37 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
   }
 }
 
-static const void* persistent_fn(int lanes, bool soft) {
-  if (soft) {
-    switch (lanes) {
-      case 4: return (const void*)k_persistent<4, true>;
-      case 8: return (const void*)k_persistent<8, true>;
-      case 16: return (const void*)k_persistent<16, true>;
-      default: return (const void*)k_persistent<32, true>;
-    }
-  }
+template <int MODE>
+static const void* persistent_fn_t(int lanes) {
   switch (lanes) {
-    case 4: return (const void*)k_persistent<4, false>;
-    case 8: return (const void*)k_persistent<8, false>;
-    case 16: return (const void*)k_persistent<16, false>;
-    default: return (const void*)k_persistent<32, false>;
+    case 4: return (const void*)k_persistent<4, MODE>;
+    case 8: return (const void*)k_persistent<8, MODE>;
+    case 16: return (const void*)k_persistent<16, MODE>;
+    default: return (const void*)k_persistent<32, MODE>;
   }
 }
+static const void* persistent_fn(int lanes, int mode) {
+  if (mode == kModeTet) return persistent_fn_t<kModeTet>(lanes);
+  if (mode == kModeParticle) return persistent_fn_t<kModeParticle>(lanes);
+  return persistent_fn_t<kModeGeneral>(lanes);
+}
 
-int persistent_max_blocks(int lanes, bool softOnly) {
+int persistent_max_blocks(int lanes, int mode) {
   int dev = 0, cus = 0, per = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, persistent_fn(lanes, softOnly), 256, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, persistent_fn(lanes, mode), 256, 0);
   if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
   return per * cus;
 }
 
 hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t numColors,
-                                   int lanes, bool softOnly, uint32_t gridBlocks, uint32_t barrierBase) {
+                                   int lanes, int mode, uint32_t gridBlocks, uint32_t barrierBase) {
   DeviceState s = S;
   StepParams p = P;
   uint32_t nc = numColors, bb = barrierBase;
   void* args[] = {&s, &p, &nc, &bb};
-  return hipLaunchCooperativeKernel(persistent_fn(lanes, softOnly), dim3(gridBlocks), dim3(256), args, 0, st);
+  return hipLaunchCooperativeKernel(persistent_fn(lanes, mode), dim3(gridBlocks), dim3(256), args, 0, st);
 }
 
 }  // namespace avbd

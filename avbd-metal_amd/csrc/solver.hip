@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <algorithm>
 #include <array>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
@@ -50,8 +51,15 @@ struct avbd_solver {
   bool dynamicColoring = false;
   uint32_t staticUsedColors = 1;
   std::vector<uint32_t> staticColorCounts;  // per color
-  std::vector<uint8_t> colorSoftOnly;       // per color: particles whose static entries are all tets
-  bool allSoft = false;
+  std::vector<uint8_t> colorMode;           // per color: primal specialisation (kMode*)
+  std::vector<ClothSlot> clothSlotsHost;    // static cloth slots without same-color bits
+  bool hasClothSlots = false;               // membrane / bend / particle-spring entries exist
+  std::vector<uint32_t> sAdjStartHost;
+  int allMode = kModeGeneral;               // common mode of every used color, else general
+  // cloth surfaces
+  uint32_t nTris = 0, nEdges = 0, nParticles = 0, nMembranes = 0, nBends = 0;
+  uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
+  float elemCellSize = 0.0f, elemMargin = 0.0f, datQueryRadius = 0.0f, surfaceContactCellSize = 0.0f;
   int lanes = 8, sweepMode = 1;
   int persistentBlocks = 0;
   uint32_t gridCap = 1024;
@@ -170,6 +178,18 @@ StepParams make_params(const avbd_solver* s) {
   P.numExclusions = s->nExcl;
   P.iterations = (uint32_t)std::max(0, s->settings.iterations);
   P.staticPalette = s->dynamicColoring ? 0u : 1u;
+  P.numTris = s->nTris;
+  P.numEdges = s->nEdges;
+  P.numParticles = s->nParticles;
+  P.elemHashSize = s->elemHashSize;
+  P.maxSoft = s->maxSoft;
+  P.softMapCapacity = s->softMapCapacity;
+  P.maxDatPairs = s->maxDatPairs;
+  P.elemCellSize = s->elemCellSize;
+  P.elemMargin = s->elemMargin;
+  P.datQueryRadius = s->datQueryRadius;
+  P.surfaceContactCellSize = s->surfaceContactCellSize;
+  P.clothViscosity = s->settings.cloth_viscosity;
   return P;
 }
 
@@ -177,37 +197,84 @@ bool stage_denied(const avbd_solver* s, const char* stage, int64_t frame) {
   return !s->deniedStage.empty() && s->deniedStage == stage && (s->deniedFrame < 0 || s->deniedFrame == frame);
 }
 
-// Enqueue one frame's kernels on the stream (eagerly or under capture).
-hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S) {
+// One frame is pre-solve (collision, warm start, adjacency, coloring), the
+// iterations, and post-solve (finalize, viscosity); `stage` (profiling) is
+// called after each stage with its name.
+using StageFn = std::function<void(const char*)>;
+
+void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
   hipStream_t st = s->stream;
-  StepParams P = make_params(s);
-  uint32_t* zp[4] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount};
-  uint32_t zn[4] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb};
-  launch_zero_multi(st, zp, zn, 4);
+  uint32_t* zp[5] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.elemCellCount};
+  uint32_t zn[5] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->nTris ? s->elemHashSize : 0u};
+  launch_zero_multi(st, zp, zn, 5);
   launch_broadphase(st, S, P);
+  stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
   launch_map_rebuild(st, S, P, s->gridCap);
+  stage("narrowphase");
+  // cloth: rigid-feature contacts at the step-start pose; after the
+  // predictor, the V-T / E-E stream and its OGC contacts at the accepted
+  // predictor pose (GPUSolver.swift:3685-3870)
+  launch_rt_emit(st, S, P);
+  if (s->nTris) stage("rt-emit");
   launch_warmstart(st, S, P);
-  launch_contact_adjacency(st, S, P, s->gridCap);
-  if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
-  if (s->sweepMode == 2) {
-    hipError_t e = launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, s->lanes,
-                                           s->allSoft, (uint32_t)s->persistentBlocks, 0u);
-    if (e != hipSuccess) return e;
-  } else {
-    for (uint32_t it = 0; it < P.iterations; it++) {
-      if (s->dynamicColoring) {
-        for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
-          launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
-        launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
-      } else {
-        for (uint32_t c = 0; c < s->staticUsedColors; c++)
-          launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
-      }
-      launch_dual(st, S, P, s->gridCap);
-    }
+  stage("warmstart");
+  if (s->nTris) {
+    launch_dat_pairs(st, S, P, s->gridCap);
+    stage("dat-pairs");
+    launch_dat_contacts(st, S, P, s->gridCap);
+    stage("vt-ee-emit");
   }
-  launch_finalize(st, S, P);
+  launch_contact_adjacency(st, S, P, s->gridCap);
+  stage("adjacency");
+  if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
+  stage("coloring");
+}
+
+// `iterations` sweeps (every color, then the dual); the persistent mode runs
+// them in one cooperative launch whose barrier counter starts at zero
+hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepParams& P0, uint32_t iterations,
+                              bool clearBarrier) {
+  hipStream_t st = s->stream;
+  StepParams P = P0;
+  P.iterations = iterations;
+  if (iterations == 0) return hipSuccess;
+  if (s->sweepMode == 2) {
+    if (clearBarrier) launch_fill_u32(st, S.counters + CTR_GBAR, 1, 0u);
+    return launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, s->lanes, s->allMode,
+                                   (uint32_t)s->persistentBlocks, 0u);
+  }
+  for (uint32_t it = 0; it < iterations; it++) {
+    if (s->dynamicColoring) {
+      for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
+        launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, kModeGeneral);
+      launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
+    } else {
+      for (uint32_t c = 0; c < s->staticUsedColors; c++)
+        launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorMode[c]);
+    }
+    launch_dual(st, S, P, s->gridCap);
+  }
+  return hipSuccess;
+}
+
+void enqueue_post_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
+  launch_finalize(s->stream, S, P);
+  launch_viscosity(s->stream, S, P);
+  stage("finalize");
+}
+
+// Enqueue one frame's kernels on the stream (eagerly or under capture).
+hipError_t enqueue_frame(avbd_solver* s, const DeviceState& S, const StageFn& mark = nullptr) {
+  auto stage = [&](const char* name) {
+    if (mark) mark(name);
+  };
+  StepParams P = make_params(s);
+  enqueue_pre_solve(s, S, P, stage);
+  hipError_t e = enqueue_iterations(s, S, P, P.iterations, false);
+  if (e != hipSuccess) return e;
+  stage("solve");
+  enqueue_post_solve(s, S, P, stage);
   return hipGetLastError();
 }
 
@@ -234,6 +301,7 @@ hipError_t build_graphs(avbd_solver* s) {
     hipGraphDestroy(g);
     if (e != hipSuccess) return e;
     std::swap(S.manifolds, S.prevManifolds);
+    std::swap(S.soft, S.prevSoft);
   }
   s->graphsValid = true;
   s->graphBaseFrame = s->frameIndex + 1;
@@ -257,6 +325,10 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
   s->stats.num_soft = (int32_t)c[CTR_SOFT];
   s->stats.soft_candidates = (int32_t)c[CTR_SOFT_CANDIDATES];
   s->stats.num_active_manifolds = (int32_t)c[CTR_ACTIVE_MANIFOLDS];
+  s->stats.dat_pairs = (int32_t)c[CTR_DAT_PAIRS];
+  s->stats.dat_vt_pairs = (int32_t)c[CTR_DAT_VT];
+  s->stats.dat_ee_pairs = (int32_t)c[CTR_DAT_EE];
+  s->stats.num_soft_rt = (int32_t)c[CTR_SOFT_RT];
   s->stats.color_conflicts = (int32_t)c[CTR_COLOR_CONFLICTS];
   if (s->dynamicColoring) {
     int maxUsed = -1;
@@ -276,6 +348,30 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
     snprintf(m, sizeof(m), "physics frame %lld: rigid-pair demand %u exceeds capacity %u", (long long)frame,
              c[CTR_PAIR_CANDIDATES], s->maxPairs);
     return latch(s, AVBD_FAIL_RIGID_PAIR_CAPACITY, frame, c[CTR_PAIR_CANDIDATES], s->maxPairs, 0, m);
+  }
+  if (s->nTris && (int64_t)c[CTR_SOFT_CANDIDATES] > (int64_t)s->maxSoft) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: soft-contact demand %u exceeds capacity %u", (long long)frame,
+             c[CTR_SOFT_CANDIDATES], s->maxSoft);
+    return latch(s, AVBD_FAIL_SOFT_CONTACT_CAPACITY, frame, c[CTR_SOFT_CANDIDATES], s->maxSoft, 0, m);
+  }
+  if (s->nTris && (int64_t)c[CTR_DAT_PAIRS] > (int64_t)s->maxDatPairs) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: Planar-DAT pair demand %u exceeds capacity %u", (long long)frame,
+             c[CTR_DAT_PAIRS], s->maxDatPairs);
+    return latch(s, AVBD_FAIL_DAT_PAIR_CAPACITY, frame, c[CTR_DAT_PAIRS], s->maxDatPairs, 0, m);
+  }
+  if (s->nTris && c[CTR_DAT_OVERFLOW] > 0) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: %u surface elements span more than 4 grid cells", (long long)frame,
+             c[CTR_DAT_OVERFLOW]);
+    return latch(s, AVBD_FAIL_DAT_GRID_SPAN, frame, c[CTR_DAT_OVERFLOW], 0, 0, m);
+  }
+  if (s->nTris && c[CTR_DAT_NONFINITE] > 0) {
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: %u non-finite surface anchors", (long long)frame,
+             c[CTR_DAT_NONFINITE]);
+    return latch(s, AVBD_FAIL_DAT_INVALID_ANCHOR, frame, c[CTR_DAT_NONFINITE], 0, 0, m);
   }
   if (c[CTR_COLOR_CONFLICTS] > 0) {
     char m[160];
@@ -337,7 +433,13 @@ avbd_status validate(const avbd_scene_desc* d) {
     if (d->spring_body_a[i] >= d->num_bodies || d->spring_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < d->num_tets * 4; i++)
     if (d->tet_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
-  if (d->num_tris > 0) return AVBD_ERR_UNSUPPORTED;  // cloth elements are not in this backend yet
+  if (d->num_tris > 0 && !d->tri_ids) return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < d->num_tris * 3; i++)
+    if (d->tri_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  // cloth surfaces run the Planar-DAT path, which the reference selects for
+  // authored triangles without tets; tet surfaces with element contacts
+  // (isotropic-DAT trackers) are not in this backend
+  if (d->num_tris > 0 && d->num_tets > 0) return AVBD_ERR_UNSUPPORTED;
   for (uint32_t i = 0; i < d->num_exclusions * 2; i++)
     if (d->exclusion_pairs[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
   return AVBD_OK;
@@ -361,6 +463,223 @@ std::vector<uint8_t> tet_surface_flags(const avbd_scene_desc* d) {
     if (kv.second == 1)
       for (uint32_t v : kv.first) flag[v] = 1;
   return flag;
+}
+
+// ---------------------------------------------------------------------------
+// cloth surface topology and elements (GPUSolver.swift:391-443, 1289-1690)
+// ---------------------------------------------------------------------------
+struct MembraneHost {
+  uint32_t ids[3];
+  float dm[4];  // inverse rest shape, column major (a, b, c, d)
+  float muA, lamA;
+};
+struct BendHost {
+  uint32_t ids[4];  // edge ends, then the two opposite vertices
+  float K[4];
+  float s;
+};
+struct ClothHost {
+  std::vector<U4> tris;
+  std::vector<U2> edges;
+  std::vector<uint32_t> nbrStart, nbrCount, nbrList, particles, group, selfCollision;
+  std::vector<MembraneHost> membranes;
+  std::vector<BendHost> bends;
+  uint32_t elemHashSize = 64, maxDatPairs = 1, maxSoft = 1, softMapCapacity = 64;
+  float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
+};
+
+float body_pos_dist(const avbd_scene_desc* d, uint32_t a, uint32_t b) {
+  V3 d3 = rd3(d->body_position, a) - rd3(d->body_position, b);
+  return sqrtf(d3.x * d3.x + d3.y * d3.y + d3.z * d3.z);
+}
+
+void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
+  const uint32_t nb = d->num_bodies, nr = d->num_tris;
+  auto isParticle = [&](uint32_t i) { return d->body_is_particle && d->body_is_particle[i]; };
+  auto sizeX = [&](uint32_t i) { return d->body_size[3 * i]; };
+  c.tris.resize(nr);
+  std::set<uint64_t> seen;
+  std::vector<uint64_t> keys;  // unique edges in first-appearance order
+  float maxElemR = 0.0f, maxPartR = 0.0f;
+  for (uint32_t t = 0; t < nr; t++) {
+    uint32_t a = d->tri_ids[3 * t], b = d->tri_ids[3 * t + 1], cc = d->tri_ids[3 * t + 2];
+    c.tris[t] = U4{a, b, cc, 0};
+    uint32_t e[3][2] = {{a, b}, {b, cc}, {a, cc}};
+    for (auto& uv : e) {
+      uint64_t k = (uint64_t)std::min(uv[0], uv[1]) << 32 | std::max(uv[0], uv[1]);
+      if (seen.insert(k).second) keys.push_back(k);
+    }
+    V3 pa = rd3(d->body_position, a), pb = rd3(d->body_position, b), pc = rd3(d->body_position, cc);
+    V3 m = (pa + pb + pc) / 3.0f;
+    float thick = fmaxf(sizeX(a), fmaxf(sizeX(b), sizeX(cc))) / 2.0f;
+    maxPartR = fmaxf(maxPartR, thick);
+    maxElemR = fmaxf(maxElemR, fmaxf(length(m - pa), fmaxf(length(m - pb), length(m - pc))) + thick);
+  }
+  c.edges.resize(keys.size());
+  std::vector<std::vector<uint32_t>> nbr(nb);
+  float edgeLenSum = 0.0f;
+  for (size_t i = 0; i < keys.size(); i++) {
+    uint32_t a = (uint32_t)(keys[i] >> 32), b = (uint32_t)(keys[i] & 0xFFFFFFFFu);
+    c.edges[i] = U2{a, b};
+    nbr[a].push_back(b);
+    nbr[b].push_back(a);
+    maxElemR = fmaxf(maxElemR, body_pos_dist(d, a, b) / 2.0f + fmaxf(sizeX(a), sizeX(b)) / 2.0f);
+    maxPartR = fmaxf(maxPartR, fmaxf(sizeX(a), sizeX(b)) / 2.0f);
+    edgeLenSum += body_pos_dist(d, a, b);
+  }
+  c.nbrStart.assign(nb, 0);
+  c.nbrCount.assign(nb, 0);
+  for (uint32_t i = 0; i < nb; i++) {
+    std::sort(nbr[i].begin(), nbr[i].end());
+    c.nbrStart[i] = (uint32_t)c.nbrList.size();
+    c.nbrCount[i] = (uint32_t)nbr[i].size();
+    c.nbrList.insert(c.nbrList.end(), nbr[i].begin(), nbr[i].end());
+  }
+  // connected surfaces (union-find over triangles), numbered by first
+  // triangle; authored sheets self-collide (GPUSolver.swift:1476-1530)
+  std::vector<uint32_t> parent(nb);
+  for (uint32_t i = 0; i < nb; i++) parent[i] = i;
+  std::function<uint32_t(uint32_t)> find = [&](uint32_t x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (const U4& t : c.tris) {
+    uint32_t r = find(t.x);
+    parent[find(t.y)] = r;
+    parent[find(t.z)] = r;
+  }
+  c.group.assign(nb, 0);
+  c.selfCollision.assign(nb, 0);
+  std::map<uint32_t, uint32_t> gidx;
+  std::vector<uint8_t> surface(nb, 0);
+  for (const U4& t : c.tris) {
+    uint32_t root = find(t.x);
+    auto it = gidx.find(root);
+    if (it == gidx.end()) it = gidx.emplace(root, (uint32_t)gidx.size() + 1).first;
+    for (uint32_t v : {t.x, t.y, t.z})
+      if (isParticle(v)) {
+        c.group[v] = it->second;
+        c.selfCollision[v] = 1;
+        surface[v] = 1;
+      }
+  }
+  for (uint32_t i = 0; i < nb; i++)
+    if (surface[i]) c.particles.push_back(i);
+  // membranes: rest 2x2 in the triangle's own frame, inverse column major
+  std::map<uint64_t, std::vector<std::pair<uint32_t, float>>> edgeTris;  // edge -> (opposite, bend)
+  for (uint32_t t = 0; t < nr; t++) {
+    float mu = d->tri_mu ? d->tri_mu[t] : 0.0f;
+    if (!(mu > 0.0f)) continue;
+    float lam = d->tri_lambda ? d->tri_lambda[t] : mu;
+    float bend = d->tri_bend ? d->tri_bend[t] : 0.0f;
+    uint32_t i0 = d->tri_ids[3 * t], i1 = d->tri_ids[3 * t + 1], i2 = d->tri_ids[3 * t + 2];
+    double x0[3], d1[3], d2[3];
+    for (int k = 0; k < 3; k++) {
+      x0[k] = d->body_position[3 * i0 + k];
+      d1[k] = (double)d->body_position[3 * i1 + k] - x0[k];
+      d2[k] = (double)d->body_position[3 * i2 + k] - x0[k];
+    }
+    double n[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
+    double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    double area = nl / 2.0;
+    if (!(area > 1e-10)) continue;
+    double l1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+    double e1[3] = {d1[0] / l1, d1[1] / l1, d1[2] / l1};
+    double nn[3] = {n[0] / nl, n[1] / nl, n[2] / nl};
+    double e2[3] = {nn[1] * e1[2] - nn[2] * e1[1], nn[2] * e1[0] - nn[0] * e1[2], nn[0] * e1[1] - nn[1] * e1[0]};
+    double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    for (int k = 0; k < 3; k++) e2[k] /= l2;
+    auto dotd = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double a = dotd(d1, e1), b = dotd(d1, e2), cc = dotd(d2, e1), dd = dotd(d2, e2);
+    double det = a * dd - b * cc;
+    if (!(fabs(det) > 1e-12)) continue;
+    MembraneHost mh;
+    mh.ids[0] = i0; mh.ids[1] = i1; mh.ids[2] = i2;
+    mh.dm[0] = (float)(dd / det);
+    mh.dm[1] = (float)(-b / det);
+    mh.dm[2] = (float)(-cc / det);
+    mh.dm[3] = (float)(a / det);
+    mh.muA = (float)(mu * area);
+    mh.lamA = (float)(lam * area);
+    c.membranes.push_back(mh);
+    if (bend > 0.0f) {
+      uint32_t ev[3][3] = {{i0, i1, i2}, {i1, i2, i0}, {i0, i2, i1}};
+      for (auto& e : ev) {
+        uint64_t k = (uint64_t)std::min(e[0], e[1]) << 32 | std::max(e[0], e[1]);
+        edgeTris[k].push_back({e[2], bend});
+      }
+    }
+  }
+  // hinges over edges shared by exactly two bending triangles, in edge-key
+  // order: K spans the null space of {1, x, y} of the unfolded rest flaps,
+  // scaled to the cotangent convention; s = 3 (k1 + k2) / 2 / (A1 + A2)
+  for (auto& kv : edgeTris) {
+    if (kv.second.size() != 2) continue;
+    uint32_t v0 = (uint32_t)(kv.first >> 32), v1 = (uint32_t)(kv.first & 0xFFFFFFFFu);
+    uint32_t opp2 = kv.second[0].first, opp3 = kv.second[1].first;
+    float k1 = kv.second[0].second, k2 = kv.second[1].second;
+    double p0[3], p1[3];
+    for (int k = 0; k < 3; k++) { p0[k] = d->body_position[3 * v0 + k]; p1[k] = d->body_position[3 * v1 + k]; }
+    auto dist = [&](uint32_t o, const double* p) {
+      double s2 = 0;
+      for (int k = 0; k < 3; k++) { double q = (double)d->body_position[3 * o + k] - p[k]; s2 += q * q; }
+      return sqrt(s2);
+    };
+    double L = sqrt((p0[0] - p1[0]) * (p0[0] - p1[0]) + (p0[1] - p1[1]) * (p0[1] - p1[1]) +
+                    (p0[2] - p1[2]) * (p0[2] - p1[2]));
+    if (!(L > 1e-9)) continue;
+    auto unfold = [&](uint32_t o, double side, double& x, double& y) {
+      double l0 = dist(o, p0), l1 = dist(o, p1);
+      x = (L * L + l0 * l0 - l1 * l1) / (2 * L);
+      double h2 = l0 * l0 - x * x;
+      if (!(h2 > 1e-12)) return false;
+      y = side * sqrt(h2);
+      return true;
+    };
+    double q[4][2] = {{0, 0}, {L, 0}, {0, 0}, {0, 0}};
+    if (!unfold(opp2, 1.0, q[2][0], q[2][1]) || !unfold(opp3, -1.0, q[3][0], q[3][1])) continue;
+    auto minor = [&](int c0, int c1, int c2) {
+      int cs[3] = {c0, c1, c2};
+      double m[3][3];
+      for (int j = 0; j < 3; j++) { m[0][j] = 1.0; m[1][j] = q[cs[j]][0]; m[2][j] = q[cs[j]][1]; }
+      return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+             m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    };
+    double K[4] = {minor(1, 2, 3), -minor(0, 2, 3), minor(0, 1, 3), -minor(0, 1, 2)};
+    auto cot = [&](int apex, int ia, int ib) {
+      double ux = q[ia][0] - q[apex][0], uy = q[ia][1] - q[apex][1];
+      double vx = q[ib][0] - q[apex][0], vy = q[ib][1] - q[apex][1];
+      double cr = ux * vy - uy * vx;
+      return fabs(cr) > 1e-12 ? (ux * vx + uy * vy) / fabs(cr) : 0.0;
+    };
+    double want = cot(0, 1, 2) + cot(1, 0, 2);
+    if (!(fabs(K[2]) > 1e-12)) continue;
+    double scale = want / K[2];
+    double a1 = fabs((q[1][0] - q[0][0]) * q[2][1]) / 2, a2 = fabs((q[1][0] - q[0][0]) * q[3][1]) / 2;
+    BendHost bh;
+    bh.ids[0] = v0; bh.ids[1] = v1; bh.ids[2] = opp2; bh.ids[3] = opp3;
+    for (int k = 0; k < 4; k++) bh.K[k] = (float)(K[k] * scale);
+    bh.s = (float)(3.0 * ((double)k1 + (double)k2) / 2.0 / std::max(a1 + a2, 1e-9));
+    c.bends.push_back(bh);
+  }
+  // grids, query radius and capacities (GPUSolver.swift:391-443, 1643-1682)
+  float meanEdge = keys.empty() ? 0.2f : edgeLenSum / (float)keys.size();
+  c.elemMargin = fminf(0.01f, fmaxf(0.002f, 0.5f * maxPartR));
+  float forceReach = 2.0f * maxPartR + c.elemMargin;
+  c.datQueryRadius = fmaxf(1.5f * forceReach, 0.75f * meanEdge);
+  float reach = (2.0f * maxPartR + c.elemMargin) / 0.7f;
+  c.surfaceContactCellSize = fmaxf(0.02f, fmaxf(fmaxf(reach, meanEdge * 1.05f), maxElemR + maxPartR + c.elemMargin));
+  c.elemCellSize = fmaxf(0.02f, fmaxf(fmaxf(reach, meanEdge * 1.05f), 2.0f * maxElemR + c.datQueryRadius));
+  c.elemHashSize = next_pow2(std::max<uint32_t>(64u, 8u * nr));
+  uint64_t np = c.particles.size(), ne = keys.size();
+  uint64_t pairCap = std::max<uint64_t>(1, 64 * np + 128 * ne);
+  uint64_t budget = std::min<uint64_t>(pairCap, 8 * np + 8 * ne);
+  c.maxDatPairs = (uint32_t)std::min<uint64_t>(pairCap, 0xFFFFFFF0ull);
+  c.maxSoft = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, 4ull * nr + budget), kMaxSoftRecords - 1);
+  c.softMapCapacity = next_pow2(std::max<uint32_t>(64u, 2u * c.maxSoft));
 }
 
 }  // namespace
@@ -730,7 +1049,28 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   // surface share a group (GPUSolver.swift:1476-1522); one union-find over
   // boundary faces
   std::vector<uint32_t> clothGroup(nb, 0);
-  if (nt > 0) {
+  ClothHost ch;
+  if (d->num_tris > 0) {
+    build_cloth(d, ch);
+    clothGroup = ch.group;
+    s->nTris = d->num_tris;
+    s->nEdges = (uint32_t)ch.edges.size();
+    s->nParticles = (uint32_t)ch.particles.size();
+    s->nMembranes = (uint32_t)ch.membranes.size();
+    s->nBends = (uint32_t)ch.bends.size();
+    s->elemHashSize = ch.elemHashSize;
+    s->maxDatPairs = ch.maxDatPairs;
+    if (s->opts.dat_pair_capacity_override > 0)
+      s->maxDatPairs = std::min<uint32_t>(s->maxDatPairs, (uint32_t)s->opts.dat_pair_capacity_override);
+    s->maxSoft = ch.maxSoft;
+    if (s->opts.soft_capacity_override > 0)
+      s->maxSoft = std::min<uint32_t>(s->maxSoft, (uint32_t)s->opts.soft_capacity_override);
+    s->softMapCapacity = ch.softMapCapacity;
+    s->elemCellSize = ch.elemCellSize;
+    s->elemMargin = ch.elemMargin;
+    s->datQueryRadius = ch.datQueryRadius;
+    s->surfaceContactCellSize = ch.surfaceContactCellSize;
+  } else if (nt > 0) {
     std::vector<uint8_t> surf = tet_surface_flags(d);
     std::vector<uint32_t> parent(nb);
     for (uint32_t i = 0; i < nb; i++) parent[i] = i;
@@ -765,8 +1105,11 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     if (dyn(g.header.x)) adj[g.header.x].push_back(e);
     if (dyn(g.header.y)) adj[g.header.y].push_back(e);
   }
+  // particle-particle springs without offsets take a stamp-major slot
+  auto zero3 = [](F4 v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; };
   for (uint32_t i = 0; i < ns; i++) {
-    uint32_t e = (FK_SPRING << kAdjShift) | i;
+    bool slot = s->isParticle[sr[i].header.x] && s->isParticle[sr[i].header.y] && zero3(sr[i].rA) && zero3(sr[i].rB);
+    uint32_t e = ((slot ? FK_PSPRING : FK_SPRING) << kAdjShift) | i;
     if (dyn(sr[i].header.x)) adj[sr[i].header.x].push_back(e);
     if (dyn(sr[i].header.y)) adj[sr[i].header.y].push_back(e);
   }
@@ -776,6 +1119,12 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     for (int k = 0; k < 4; k++)
       if (dyn(id[k])) adj[id[k]].push_back(e);
   }
+  for (uint32_t m = 0; m < ch.membranes.size(); m++)
+    for (int k = 0; k < 3; k++)
+      if (dyn(ch.membranes[m].ids[k])) adj[ch.membranes[m].ids[k]].push_back((FK_MEMBRANE << kAdjShift) | m);
+  for (uint32_t m = 0; m < ch.bends.size(); m++)
+    for (int k = 0; k < 4; k++)
+      if (dyn(ch.bends[m].ids[k])) adj[ch.bends[m].ids[k]].push_back((FK_BEND << kAdjShift) | m);
   std::vector<uint32_t> sStart(nb), sCount(nb), sList;
   for (uint32_t b = 0; b < nb; b++) {
     sStart[b] = (uint32_t)sList.size();
@@ -816,6 +1165,55 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     }
   }
 
+  // cloth slots at each membrane / bend / particle-spring entry's CSR
+  // position (same-color partner bits are added once the palette is known)
+  bool anySlot = false;
+  for (uint32_t e : sList) {
+    uint32_t kind = e >> kAdjShift;
+    if (kind == FK_MEMBRANE || kind == FK_BEND || kind == FK_PSPRING) anySlot = true;
+  }
+  s->hasClothSlots = anySlot;
+  std::vector<ClothSlot> cslots(std::max<size_t>(1, anySlot ? sList.size() : 1));
+  memset(cslots.data(), 0, cslots.size() * sizeof(ClothSlot));
+  for (uint32_t b = 0; b < nb && anySlot; b++) {
+    for (uint32_t k = 0; k < sCount[b]; k++) {
+      uint32_t e = sList[sStart[b] + k], kind = e >> kAdjShift, i = e & kAdjIndexMask;
+      ClothSlot cs{};
+      if (kind == FK_MEMBRANE) {
+        const MembraneHost& m = ch.membranes[i];
+        // shape-function gradients w_1 = (a, c), w_2 = (b, d), w_0 = -(w_1 + w_2)
+        float w[3][2] = {{-(m.dm[0] + m.dm[1]), -(m.dm[2] + m.dm[3])}, {m.dm[0], m.dm[2]}, {m.dm[1], m.dm[3]}};
+        uint32_t o[2];
+        float ow[2][2];
+        int n = 0;
+        for (int q = 0; q < 3; q++)
+          if (m.ids[q] != b && n < 2) { o[n] = m.ids[q]; ow[n][0] = w[q][0]; ow[n][1] = w[q][1]; n++; }
+        cs.u = U4{o[0], o[1], as_uint(m.muA), as_uint(m.lamA)};
+        cs.f = F4{ow[0][0], ow[0][1], ow[1][0], ow[1][1]};
+      } else if (kind == FK_BEND) {
+        const BendHost& h = ch.bends[i];
+        uint32_t o[3];
+        float ok[3], kself = 0.0f;
+        int n = 0;
+        for (int q = 0; q < 4; q++) {
+          if (h.ids[q] == b) kself = h.K[q];
+          else if (n < 3) { o[n] = h.ids[q]; ok[n] = h.K[q]; n++; }
+        }
+        cs.u = U4{o[0], o[1], o[2], as_uint(h.s)};
+        cs.f = F4{ok[0], ok[1], ok[2], kself};
+      } else if (kind == FK_PSPRING) {
+        const SpringRec& r = sr[i];
+        cs.u = U4{r.header.x == b ? r.header.y : r.header.x, i, r.header.z, 0u};
+        cs.f = F4{r.rA.w, r.rB.w, 0.0f, 0.0f};
+      } else {
+        continue;
+      }
+      cslots[sStart[b] + k] = cs;
+    }
+  }
+  s->clothSlotsHost = cslots;
+  s->sAdjStartHost = sStart;
+
   // ---- device buffers
   DeviceState& S = s->S;
   hipError_t e = hipSuccess;
@@ -844,13 +1242,35 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.sAdjStart, nb) ALLOC(S.sAdjCount, nb) ALLOC(S.sAdjList, std::max<size_t>(1, sList.size()))
   ALLOC(S.tetSlots, slots.size())
   ALLOC(S.dAdjCount, nb) ALLOC(S.dAdjStart, nb) ALLOC(S.dAdjCursor, nb)
-  S.dAdjCapacity = 2 * s->maxPairs;
+  S.dAdjCapacity = 2 * s->maxPairs + 4 * s->maxSoft;
   ALLOC(S.dAdjList, S.dAdjCapacity)
   ALLOC(S.colors, nb) ALLOC(S.colorsB, nb) ALLOC(S.colorList, nb) ALLOC(S.colorStart, kMaxColors + 2)
   uint32_t bb = (nb + 255) / 256;
   ALLOC(S.bodySlot, std::max<uint32_t>(nb, 2 * kMaxColors * bb))
   ALLOC(S.colorFlags, 32) ALLOC(S.counters, CTR_TOTAL)
+  ALLOC(S.hashedRigid, std::max<size_t>(1, hashedRigid.size()))
+  ALLOC(S.clothSlots, cslots.size())
+  {
+    const bool cl = d->num_tris > 0;
+    const uint32_t nElem = cl ? s->nTris + s->nEdges : 0u, nProd = cl ? s->nParticles + s->nEdges : 0u;
+    ALLOC(S.tris, std::max<uint32_t>(1, s->nTris)) ALLOC(S.edges, std::max<uint32_t>(1, s->nEdges))
+    ALLOC(S.nbrStart, cl ? nb : 1u) ALLOC(S.nbrCount, cl ? nb : 1u)
+    ALLOC(S.nbrList, std::max<size_t>(1, ch.nbrList.size())) ALLOC(S.particleIdx, std::max<uint32_t>(1, s->nParticles))
+    ALLOC(S.selfCollision, cl ? nb : 1u)
+    ALLOC(S.elemCellCount, cl ? s->elemHashSize : 1u) ALLOC(S.elemCellStart, cl ? s->elemHashSize : 1u)
+    ALLOC(S.elemCellCursor, cl ? s->elemHashSize : 1u) ALLOC(S.elemCells, std::max<uint64_t>(1, 64ull * nElem))
+    ALLOC(S.datCount, nProd + 1) ALLOC(S.datStart, nProd + 1)
+    ALLOC(S.datPairs, cl ? s->maxDatPairs : 1u)
+    uint32_t flagN = cl ? std::max(s->nTris, s->maxDatPairs) + 1 : 1u;
+    ALLOC(S.softFlag, flagN) ALLOC(S.softStart, flagN)
+    ALLOC(S.soft, cl ? s->maxSoft : 1u) ALLOC(S.prevSoft, cl ? s->maxSoft : 1u)
+    ALLOC(S.softMapKeyA, cl ? s->softMapCapacity : 1u) ALLOC(S.softMapKeyB, cl ? s->softMapCapacity : 1u)
+    ALLOC(S.softMapVal, cl ? s->softMapCapacity : 1u)
+    ALLOC(S.velScratch, cl ? nb : 1u)
+  }
   uint32_t scanMax = std::max<uint32_t>(std::max<uint32_t>(s->gridHashSize, producers), std::max(nb, kMaxColors * bb));
+  scanMax = std::max<uint32_t>(scanMax, std::max(s->elemHashSize, s->nParticles + s->nEdges + 1));
+  scanMax = std::max<uint32_t>(scanMax, std::max(s->nTris, s->maxDatPairs) + 1);
   ALLOC(S.scanScratch, scan_scratch_words(scanMax) + 16)
   s->residualBlocks = bb;
   ALLOC(S.residualPartial, 2 * bb)
@@ -875,6 +1295,10 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
       upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||
       upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.tetSlots, slots) ||
+      upload(S.hashedRigid, hashedRigid) || upload(S.clothSlots, cslots) || upload(S.tris, ch.tris) ||
+      upload(S.edges, ch.edges) || upload(S.nbrStart, ch.nbrStart) || upload(S.nbrCount, ch.nbrCount) ||
+      upload(S.nbrList, ch.nbrList) || upload(S.particleIdx, ch.particles) ||
+      upload(S.selfCollision, ch.selfCollision) ||
       upload(S.posLinSnap, pl) ||
       upload(S.posAngSnap, pa))
     return AVBD_ERR_HIP;
@@ -945,25 +1369,48 @@ avbd_status static_coloring(avbd_solver* s) {
   cstart.push_back(s->staticUsedColors);
   if (upload(S.colorStart, cstart) || upload(S.colorList, clist)) return AVBD_ERR_HIP;
   s->staticColorCounts.assign(kMaxColors, 0);
-  s->colorSoftOnly.assign(kMaxColors, 0);
+  s->colorMode.assign(kMaxColors, kModeGeneral);
   std::vector<uint32_t> sStart(s->nb);
   {
     uint32_t run = 0;
     for (uint32_t b = 0; b < s->nb; b++) { sStart[b] = run; run += s->sAdjCount[b]; }
   }
-  s->allSoft = !s->dynamicColoring;
+  bool first = true;
+  s->allMode = kModeGeneral;
+  bool uniform = !s->dynamicColoring;
   for (int c = 0; c < kMaxColors; c++) {
     s->staticColorCounts[c] = (uint32_t)buckets[c].size();
     s->stats.color_counts[c] = (int32_t)buckets[c].size();
-    bool soft = true;
+    bool particles = true, tetOnly = true;
     for (uint32_t v : buckets[c]) {
-      if (!s->isParticle[v]) { soft = false; break; }
-      for (uint32_t k = 0; k < s->sAdjCount[v] && soft; k++)
-        if ((s->sAdjListHost[sStart[v] + k] >> kAdjShift) != FK_TET) soft = false;
-      if (!soft) break;
+      if (!s->isParticle[v]) { particles = false; break; }
+      for (uint32_t k = 0; k < s->sAdjCount[v] && tetOnly; k++)
+        if ((s->sAdjListHost[sStart[v] + k] >> kAdjShift) != FK_TET) tetOnly = false;
     }
-    s->colorSoftOnly[c] = soft ? 1 : 0;
-    if (c < (int)s->staticUsedColors && !soft) s->allSoft = false;
+    int mode = !particles ? kModeGeneral : (tetOnly ? kModeTet : kModeParticle);
+    s->colorMode[c] = (uint8_t)mode;
+    if (c < (int)s->staticUsedColors) {
+      if (first) { s->allMode = mode; first = false; }
+      else if (mode != s->allMode) uniform = false;
+    }
+  }
+  if (!uniform) s->allMode = kModeGeneral;
+  // cloth slots: flag partners of the same static color (snapshot reads)
+  if (s->hasClothSlots) {
+    std::vector<ClothSlot> cs = s->clothSlotsHost;
+    auto mark = [&](uint32_t& id, uint32_t self) {
+      if (s->mass[id] > 0.0f && colors[id] == colors[self]) id |= kSnapBit;
+    };
+    for (uint32_t b = 0; b < s->nb; b++) {
+      for (uint32_t k = 0; k < s->sAdjCount[b]; k++) {
+        uint32_t kind = s->sAdjListHost[sStart[b] + k] >> kAdjShift;
+        ClothSlot& q = cs[sStart[b] + k];
+        if (kind == FK_MEMBRANE) { mark(q.u.x, b); mark(q.u.y, b); }
+        else if (kind == FK_BEND) { mark(q.u.x, b); mark(q.u.y, b); mark(q.u.z, b); }
+        else if (kind == FK_PSPRING) mark(q.u.x, b);
+      }
+    }
+    if (upload(S.clothSlots, cs)) return AVBD_ERR_HIP;
   }
   s->stats.max_color_used = (int32_t)s->staticUsedColors - 1;
   s->stats.num_colors = (int32_t)s->staticUsedColors;
@@ -990,7 +1437,7 @@ void autotune(avbd_solver* s) {
   }
   // persistent grid size per lane count
   auto pblocks = [&](int L) {
-    int maxB = persistent_max_blocks(L, s->allSoft);
+    int maxB = persistent_max_blocks(L, s->allMode);
     uint32_t maxCount = 0;
     for (uint32_t c : s->staticColorCounts) maxCount = std::max(maxCount, c);
     if (s->dynamicColoring) maxCount = nb;
@@ -1046,17 +1493,17 @@ void autotune(avbd_solver* s) {
       auto run = [&]() -> bool {
         if (M == 2) {
           launch_fill_u32(st, S.counters + CTR_GBAR, 2, 0u);
-          return launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, L, s->allSoft,
+          return launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, L, s->allMode,
                                          (uint32_t)blocks, 0u) == hipSuccess;
         }
         for (uint32_t it = 0; it < P.iterations; it++) {
           if (s->dynamicColoring) {
             for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
-              launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L, false);
+              launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L, kModeGeneral);
             launch_primal_tail(st, S, P, kDynamicColorLaunches, L);
           } else {
             for (uint32_t c = 0; c < s->staticUsedColors; c++)
-              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L, s->colorSoftOnly[c] != 0);
+              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L, s->colorMode[c]);
           }
           launch_dual(st, S, P, s->gridCap);
         }
@@ -1132,8 +1579,6 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
     HIP_OR_LATCH(s, hipGraphLaunch(s->graphs[(frame - s->graphBaseFrame) & 1], s->stream), frame);
   } else if (s->profiling) {
     // eager, with per-stage events
-    DeviceState S = s->S;
-    StepParams P = make_params(s);
     hipStream_t st = s->stream;
     std::vector<hipEvent_t> ev;
     std::vector<std::string> names;
@@ -1145,51 +1590,17 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
       if (name) names.push_back(name);
     };
     mark(nullptr);
-    uint32_t* zp[4] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount};
-    uint32_t zn[4] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb};
-    launch_zero_multi(st, zp, zn, 4);
-    launch_broadphase(st, S, P);
-    mark("broadphase");
-    launch_narrowphase(st, S, P, s->gridCap);
-    launch_map_rebuild(st, S, P, s->gridCap);
-    mark("narrowphase");
-    launch_warmstart(st, S, P);
-    mark("warmstart");
-    launch_contact_adjacency(st, S, P, s->gridCap);
-    mark("adjacency");
-    if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
-    mark("coloring");
-    if (s->sweepMode == 2) {
-      HIP_OR_LATCH(s, launch_persistent_solve(st, S, P, s->dynamicColoring ? 0u : s->staticUsedColors, s->lanes,
-                                              s->allSoft, (uint32_t)s->persistentBlocks, 0u),
-                   frame);
-      mark("solve");
-    } else {
-      for (uint32_t it = 0; it < P.iterations; it++) {
-        if (s->dynamicColoring) {
-          for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
-            launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, false);
-          launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
-        } else {
-          for (uint32_t c = 0; c < s->staticUsedColors; c++)
-            launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
-        }
-        launch_dual(st, S, P, s->gridCap);
-      }
-      mark("solve");
-    }
-    launch_finalize(st, S, P);
-    mark("finalize");
+    HIP_OR_LATCH(s, enqueue_frame(s, s->S, mark), frame);
     HIP_OR_LATCH(s, hipStreamSynchronize(st), frame);
     s->profNames = names;
     s->profMs.assign(names.size(), 0.0f);
     for (size_t i = 0; i + 1 < ev.size(); i++) hipEventElapsedTime(&s->profMs[i], ev[i], ev[i + 1]);
     for (auto e : ev) hipEventDestroy(e);
-    std::swap(s->S.manifolds, s->S.prevManifolds);
   } else {
     HIP_OR_LATCH(s, enqueue_frame(s, s->S), frame);
   }
-  if (!s->profiling) std::swap(s->S.manifolds, s->S.prevManifolds);
+  std::swap(s->S.manifolds, s->S.prevManifolds);
+  std::swap(s->S.soft, s->S.prevSoft);
   HIP_OR_LATCH(s,
                hipMemcpyAsync(s->pinnedCounters + slot * CTR_TOTAL, s->S.counters, CTR_TOTAL * 4,
                               hipMemcpyDeviceToHost, s->stream),
@@ -1202,6 +1613,59 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
 }
 
 extern "C" avbd_status avbd_step(avbd_solver* s) { return avbd_submit_step(s); }
+
+// Iterate one frame until the residual reaches `tol` (or max_iterations):
+// chunks of `check_every` sweeps, each followed by the residual kernel and a
+// host readback of its block sums.  Eager launches; the frame is complete
+// (and retired) on return.
+extern "C" avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t max_iterations, int32_t check_every,
+                                             int32_t* iterations_used, float* residual) {
+  if (!s || !(tol > 0.0f) || max_iterations <= 0 || check_every <= 0) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st0 = sync_all(s);
+  if (st0 != AVBD_OK) return st0;
+  int64_t frame = s->frameIndex + 1;
+  int slot = (int)(frame & 1);
+  hipStream_t st = s->stream;
+  DeviceState S = s->S;
+  StepParams P = make_params(s);
+  auto none = [](const char*) {};
+  enqueue_pre_solve(s, S, P, none);
+  int32_t it = 0;
+  double r = INFINITY;
+  while (it < max_iterations) {
+    int32_t n = std::min(check_every, max_iterations - it);
+    HIP_OR_LATCH(s, enqueue_iterations(s, S, P, (uint32_t)n, true), frame);
+    it += n;
+    launch_residual(st, S, P, s->residualBlocks);
+    HIP_OR_LATCH(s,
+                 hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 8 * s->residualBlocks,
+                                hipMemcpyDeviceToHost, st),
+                 frame);
+    HIP_OR_LATCH(s, hipStreamSynchronize(st), frame);
+    double g2 = 0.0, w2 = 0.0;
+    for (uint32_t b = 0; b < s->residualBlocks; b++) {
+      g2 += s->residualPartialHost[2 * b];
+      w2 += s->residualPartialHost[2 * b + 1];
+    }
+    r = w2 > 0.0 ? sqrt(g2 / w2) : sqrt(g2);
+    if (r <= tol) break;
+  }
+  enqueue_post_solve(s, S, P, none);
+  HIP_OR_LATCH(s, hipGetLastError(), frame);
+  std::swap(s->S.manifolds, s->S.prevManifolds);
+  std::swap(s->S.soft, s->S.prevSoft);
+  HIP_OR_LATCH(s,
+               hipMemcpyAsync(s->pinnedCounters + slot * CTR_TOTAL, s->S.counters, CTR_TOTAL * 4,
+                              hipMemcpyDeviceToHost, st),
+               frame);
+  HIP_OR_LATCH(s, hipEventRecord(s->frameDone[slot], st), frame);
+  s->slotBusy[slot] = true;
+  s->slotFrame[slot] = frame;
+  s->frameIndex = frame;
+  if (iterations_used) *iterations_used = it;
+  if (residual) *residual = (float)r;
+  return sync_all(s);
+}
 
 extern "C" avbd_status avbd_synchronize(avbd_solver* s) {
   if (!s) return AVBD_ERR_INVALID_ARGUMENT;
@@ -1478,6 +1942,69 @@ extern "C" avbd_status avbd_get_manifolds(avbd_solver* s, uint32_t capacity, uin
   return AVBD_OK;
 }
 
+extern "C" avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info, uint32_t* edges,
+                                           uint32_t* particles) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  if (info) {
+    info->num_tris = s->nTris;
+    info->num_edges = s->nEdges;
+    info->num_particles = s->nParticles;
+    info->num_membranes = s->nMembranes;
+    info->num_bends = s->nBends;
+    info->elem_hash_size = s->elemHashSize;
+    info->max_soft = s->maxSoft;
+    info->soft_map_capacity = s->softMapCapacity;
+    info->max_dat_pairs = s->maxDatPairs;
+    info->elem_cell_size = s->elemCellSize;
+    info->elem_margin = s->elemMargin;
+    info->dat_query_radius = s->datQueryRadius;
+    info->surface_contact_cell_size = s->surfaceContactCellSize;
+  }
+  if (edges && s->nEdges && hipMemcpy(edges, s->S.edges, 8 * (size_t)s->nEdges, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  if (particles && s->nParticles &&
+      hipMemcpy(particles, s->S.particleIdx, 4 * (size_t)s->nParticles, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  *n = 0;
+  if (!s->nTris) return AVBD_OK;
+  uint32_t cnt = 0;
+  if (hipMemcpy(&cnt, s->S.counters + CTR_DAT_PAIRS, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  cnt = std::min(cnt, s->maxDatPairs);
+  *n = cnt;
+  uint32_t m = std::min(cnt, capacity);
+  if (pairs && m) {
+    std::vector<U4> tmp(m);
+    if (hipMemcpy(tmp.data(), s->S.datPairs, sizeof(U4) * m, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+    for (uint32_t i = 0; i < m; i++) {
+      pairs[3 * i] = tmp[i].x;
+      pairs[3 * i + 1] = tmp[i].y;
+      pairs[3 * i + 2] = tmp[i].z;
+    }
+  }
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  *n = 0;
+  if (!s->nTris) return AVBD_OK;
+  uint32_t cnt = 0;
+  if (hipMemcpy(&cnt, s->S.counters + CTR_SOFT, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  *n = cnt;
+  uint32_t m = std::min(cnt, capacity);
+  // the frame just retired wrote into what is now prevSoft
+  if (records && m && hipMemcpy(records, s->S.prevSoft, sizeof(SoftRec) * m, hipMemcpyDeviceToHost))
+    return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
 extern "C" avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out) {
   if (!s || (n && (!pairs || !out))) return AVBD_ERR_INVALID_ARGUMENT;
   if (n == 0) return AVBD_OK;
@@ -1502,6 +2029,7 @@ extern "C" avbd_status avbd_max_constraint_error(avbd_solver* s, float* out) {
   // the last frame's manifolds are in prevManifolds after the swap
   DeviceState S = s->S;
   std::swap(S.manifolds, S.prevManifolds);
+  std::swap(S.soft, S.prevSoft);
   StepParams P = make_params(s);
   launch_fill_u32(s->stream, s->devScratchU32, 1, 0u);
   launch_diag_error(s->stream, S, P, s->devScratchU32, s->gridCap);
@@ -1519,6 +2047,7 @@ extern "C" avbd_status avbd_residual(avbd_solver* s, float* out) {
   if (st != AVBD_OK) return st;
   DeviceState S = s->S;
   std::swap(S.manifolds, S.prevManifolds);
+  std::swap(S.soft, S.prevSoft);
   StepParams P = make_params(s);
   launch_residual(s->stream, S, P, s->residualBlocks);
   if (hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 8 * s->residualBlocks, hipMemcpyDeviceToHost,
@@ -1568,6 +2097,7 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   if (s->dynamicColoring) return AVBD_ERR_UNSUPPORTED;
   DeviceState S = s->S;
   std::swap(S.manifolds, S.prevManifolds);  // the last frame's contacts
+  std::swap(S.soft, S.prevSoft);
   StepParams P = make_params(s);
   // algorithmic bytes of one sweep: per body fixed + per stamp record model
   std::vector<uint32_t> dc, dstart, dlist;
@@ -1594,10 +2124,17 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
       uint32_t e = s->sAdjListHost[sStart[b] + k], kind = e >> kAdjShift;
       if (kind == FK_TET) bytes += 4 + 64 + 4 * 16;
       else if (kind == FK_JOINT) bytes += 4 + 224 + 2 * 32 + 2 * 16;
-      else if (kind == FK_SPRING) bytes += 4 + 64 + 2 * 32;
+      else if (kind == FK_SPRING || kind == FK_PSPRING) bytes += 4 + 64 + 2 * 32;
+      else if (kind == FK_MEMBRANE) bytes += 4 + 48 + 3 * 16;
+      else if (kind == FK_BEND) bytes += 4 + 48 + 4 * 16;
     }
     for (uint32_t k = 0; k < dc[b]; k++) {
-      uint32_t m = dlist[dstart[b] + k] & kAdjIndexMask;
+      uint32_t e = dlist[dstart[b] + k];
+      if ((e >> kAdjShift) == FK_SOFT) {
+        bytes += 4 + 112 + 4 * 16;  // record + the four participant positions
+        continue;
+      }
+      uint32_t m = e & kAdjIndexMask;
       uint32_t nct = m < mf.size() ? mf[m].header.z : 0;
       bytes += 4 + 64 + 80.0 * nct + 2 * 32 + 2 * 32;
     }
@@ -1608,7 +2145,7 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   hipEventRecord(e0, s->stream);
   for (int r = 0; r < reps; r++)
     for (uint32_t c = 0; c < s->staticUsedColors; c++)
-      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes, s->colorSoftOnly[c] != 0);
+      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes, s->colorMode[c]);
   hipEventRecord(e1, s->stream);
   hipError_t he = hipEventSynchronize(e1);
   float ms = 0.0f;

@@ -332,8 +332,189 @@ __device__ __forceinline__ void stamp_spring(const SpringRec& sp, uint32_t self,
   }
 }
 
-// One adjacency entry; `slot` is the entry's static-CSR position (tets
-// read their stamp-major record there).
+// ---------------------------------------------------------------------------
+// cloth elements from stamp-major slots (40_solver.metal:910-970, 1088-1150)
+// ---------------------------------------------------------------------------
+// Partner position of a cloth slot: kSnapBit marks a partner of the same
+// static color, read from the start-of-iteration snapshot.
+__device__ __forceinline__ V3 slot_pos(const PoseView& pv, uint32_t id) {
+  uint32_t b = id & ~kSnapBit;
+  return xyz((pv.useSnap && (id & kSnapBit)) ? pv.linSnap[b] : pv.lin[b]);
+}
+
+// StVK membrane: Psi = mu A |E|^2 + lambda A / 2 tr(E)^2, E = (F^T F - I)/2,
+// exact force P = F S; Gauss-Newton Hessian from the strain gradients
+// (drops the indefinite geometric term).
+__device__ __forceinline__ void stamp_membrane_slot(const ClothSlot& c, V3 xs, V3 x1, V3 x2, Acc& acc) {
+  float muA = as_float(c.u.z), lamA = as_float(c.u.w);
+  V3 e1 = x1 - xs, e2 = x2 - xs;
+  V3 f1 = e1 * c.f.x + e2 * c.f.z;
+  V3 f2 = e1 * c.f.y + e2 * c.f.w;
+  float E11 = 0.5f * (dot(f1, f1) - 1.0f);
+  float E22 = 0.5f * (dot(f2, f2) - 1.0f);
+  float E12 = 0.5f * dot(f1, f2);
+  float trE = E11 + E22;
+  float S11 = 2.0f * muA * E11 + lamA * trE;
+  float S22 = 2.0f * muA * E22 + lamA * trE;
+  float S12 = 2.0f * muA * E12;
+  V3 p1 = f1 * S11 + f2 * S12;
+  V3 p2 = f1 * S12 + f2 * S22;
+  float wx = -(c.f.x + c.f.z), wy = -(c.f.y + c.f.w);
+  acc.gl += p1 * wx + p2 * wy;
+  V3 g11 = f1 * wx, g22 = f2 * wy;
+  V3 g12 = (f2 * wx + f1 * wy) * 0.5f;
+  V3 gtr = g11 + g22;
+  acc_lin_outer(acc, g11, 2.0f * muA);
+  acc_lin_outer(acc, g22, 2.0f * muA);
+  acc_lin_outer(acc, g12, 4.0f * muA);
+  acc_lin_outer(acc, gtr, lamA);
+}
+
+// Quadratic (Bergou) bending: E = s/2 |sum_i K_i x_i|^2, evaluated
+// translation-free as m = sum_o K_o (x_o - x_self) (K annihilates constants);
+// grad_self = s K_self m, H_self = s K_self^2 I.
+__device__ __forceinline__ void stamp_bend_slot(const ClothSlot& c, V3 xs, V3 x1, V3 x2, V3 x3, Acc& acc) {
+  float s = as_float(c.u.w), ks = c.f.w;
+  V3 m = (x1 - xs) * c.f.x + (x2 - xs) * c.f.y + (x3 - xs) * c.f.z;
+  acc.gl += m * (s * ks);
+  acc_lin_diag(acc, s * ks * ks);
+}
+
+// Particle-particle spring without offsets: soft k (|d| - L) or a
+// tension-only augmented-Lagrangian rod, plus the geometric stiffness
+// f/|d| (I - n n^T) while in tension.
+__device__ __forceinline__ void stamp_pspring_slot(const ClothSlot& c, V3 xs, V3 xo, const SpringRec* springs,
+                                                   float alpha, Acc& acc) {
+  V3 d = xs - xo;
+  float L = length(d);
+  if (L <= 1.0e-6f) return;
+  V3 n = d / L;
+  float k = c.f.x, f;
+  if (c.u.z != 0u) {
+    F4 dual = springs[c.u.y].dual;
+    float C = L - c.f.y - dual.z * alpha;
+    k = dual.y;
+    f = k * C + dual.x;
+    if (f <= 0.0f) return;
+  } else {
+    f = k * (L - c.f.y);
+  }
+  acc_lin_outer(acc, n, k);
+  acc.gl += n * f;
+  float ft = fmaxf(f, 0.0f) / L;
+  if (ft > 0.0f) {
+    acc_lin_diag(acc, ft);
+    acc_lin_outer(acc, n, -ft);
+  }
+}
+
+__device__ __forceinline__ void stamp_cloth_slot(uint32_t kind, const ClothSlot& c, const PoseView& pv, V3 xs,
+                                                 const SpringRec* springs, float alpha, Acc& acc) {
+  if (kind == FK_BEND) {
+    stamp_bend_slot(c, xs, slot_pos(pv, c.u.x), slot_pos(pv, c.u.y), slot_pos(pv, c.u.z), acc);
+  } else if (kind == FK_MEMBRANE) {
+    stamp_membrane_slot(c, xs, slot_pos(pv, c.u.x), slot_pos(pv, c.u.y), acc);
+  } else {
+    stamp_pspring_slot(c, xs, slot_pos(pv, c.u.x), springs, alpha, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// element contacts (OGC, 40_solver.metal:1237-1335): 4-slot weighted stencil
+// ---------------------------------------------------------------------------
+// `same` bit k: participant k shares the body's static color (snapshot pose).
+__device__ __forceinline__ V3 soft_contact_C(const SoftRec& sc, const PoseView& pv, uint32_t same, const F4* initLin,
+                                             const F4* initAng, float alpha, V3& n, V3& t1, V3& t2, V3& rAW) {
+  uint32_t flags = as_uint(sc.anchorA.w);
+  bool rigidA = (flags & kSoftRigidA) != 0, roundA = (flags & kSoftRoundA) != 0;
+  n = xyz(sc.normal);
+  orthonormal(n, t1, t2);
+  rAW = v3s(0);
+  V3 C = xyz(sc.C0) * (1.0f - alpha);
+  auto add = [&](uint32_t b, float w, uint32_t k) {
+    if (b == kWorld) return;
+    bool snap = pv.useSnap && ((same >> k) & 1u);
+    V3 dq = xyz(snap ? pv.linSnap[b] : pv.lin[b]) - xyz(initLin[b]);
+    C += v3(dot(n, dq), dot(t1, dq), dot(t2, dq)) * w;
+    if (k == 0 && rigidA) {
+      F4 q = snap ? pv.angSnap[b] : pv.ang[b];
+      rAW = roundA ? xyz(sc.anchorA) : q_rotate(q, xyz(sc.anchorA));
+      V3 dqa = q_sub(q, initAng[b]);
+      C += v3(dot(cross(rAW, n), dqa), dot(cross(rAW, t1), dqa), dot(cross(rAW, t2), dqa)) * w;
+    }
+  };
+  add(sc.ids.x, sc.weights.x, 0);
+  add(sc.ids.y, sc.weights.y, 1);
+  add(sc.ids.z, sc.weights.z, 2);
+  add(sc.ids.w, sc.weights.w, 3);
+  return C;
+}
+
+// F = K C + lambda; below tau = restT / 2 the V-T / E-E normal switches to
+// the log-barrier stage k' / d (k' = tau kc (r - tau), stitched C1);
+// normal <= 0 and the Coulomb cone on the tangential pair.
+__device__ __forceinline__ V3 soft_contact_force(const SoftRec& sc, V3 C, float& kNormal) {
+  V3 F = xyz(sc.penalty) * C + xyz(sc.lambda);
+  kNormal = sc.penalty.x;
+  float rT = sc.anchorA.x;
+  uint32_t kind = (as_uint(sc.anchorA.w) >> kSoftKindShift) & 3u;
+  if (rT > 0.0f && kind != kSoftRT && C.x < -0.5f * rT) {
+    float dsep = fmaxf(C.x + rT, 0.02f * rT);
+    float kp = 0.25f * rT * rT * sc.penalty.x;
+    F.x = -kp / dsep + sc.lambda.x;
+    kNormal = kp / (dsep * dsep);
+  }
+  F.x = fminf(F.x, 0.0f);
+  float bounds = fabsf(F.x) * sc.normal.w;
+  float fs = sqrtf(F.y * F.y + F.z * F.z);
+  if (fs > bounds && fs > 0.0f) {
+    F.y *= bounds / fs;
+    F.z *= bounds / fs;
+  }
+  return F;
+}
+
+__device__ __forceinline__ void stamp_soft(const SoftRec& sc, uint32_t self, uint32_t same, const PoseView& pv,
+                                           const F4* initLin, const F4* initAng, float alpha, Acc& acc) {
+  V3 n, t1, t2, rAW;
+  V3 C = soft_contact_C(sc, pv, same, initLin, initAng, alpha, n, t1, t2, rAW);
+  float kN;
+  V3 F = soft_contact_force(sc, C, kN);
+  float w;
+  int slot;
+  if (sc.ids.x == self) { slot = 0; w = sc.weights.x; }
+  else if (sc.ids.y == self) { slot = 1; w = sc.weights.y; }
+  else if (sc.ids.z == self) { slot = 2; w = sc.weights.z; }
+  else if (sc.ids.w == self) { slot = 3; w = sc.weights.w; }
+  else return;
+  if (slot == 0 && (as_uint(sc.anchorA.w) & kSoftRigidA)) {
+    V3 l0 = n * w, l1 = t1 * w, l2 = t2 * w;
+    V3 a0 = cross(rAW, l0), a1 = cross(rAW, l1), a2 = cross(rAW, l2);
+    float k1 = sc.penalty.y, k2 = sc.penalty.z;
+    acc_lin_outer(acc, l0, kN);
+    acc_lin_outer(acc, l1, k1);
+    acc_lin_outer(acc, l2, k2);
+    acc_ang_outer(acc, a0, kN);
+    acc_ang_outer(acc, a1, k1);
+    acc_ang_outer(acc, a2, k2);
+    acc_cross_outer(acc, a0, l0, kN);
+    acc_cross_outer(acc, a1, l1, k1);
+    acc_cross_outer(acc, a2, l2, k2);
+    // (angular first: both branches then end on the same gl update, so
+    // the compiler cannot merge them into an address select on acc)
+    acc.ga += a0 * F.x + a1 * F.y + a2 * F.z;
+    acc.gl += l0 * F.x + l1 * F.y + l2 * F.z;
+  } else {
+    float w2 = w * w;
+    acc_lin_outer(acc, n, kN * w2);
+    acc_lin_outer(acc, t1, sc.penalty.y * w2);
+    acc_lin_outer(acc, t2, sc.penalty.z * w2);
+    acc.gl += (n * F.x + t1 * F.y + t2 * F.z) * w;
+  }
+}
+
+// One adjacency entry; `slot` is the entry's static-CSR position (tets and
+// cloth elements read their stamp-major record there).
 __device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView& pv, uint32_t entry, uint32_t slot,
                                             uint32_t self, V3 xs, bool rigidSelf, float alpha, float dt, Acc& acc) {
   uint32_t kind = entry >> kAdjShift, idx = entry & kAdjIndexMask;
@@ -346,6 +527,11 @@ __device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView
     if (j.header.z == 0) stamp_joint(j, self, S.posLin, S.posAng, S.initAng, alpha, dt, acc);  // skip broken
   } else if (kind == FK_SPRING) {
     stamp_spring(S.springs[idx], self, rigidSelf, S.posLin, S.posAng, alpha, acc);
+  } else if (kind == FK_SOFT) {
+    stamp_soft(S.soft[entry & kSoftIndexMask], self, (entry >> kSoftSameShift) & 15u, pv, S.initLin, S.initAng, alpha,
+               acc);
+  } else if (kind == FK_MEMBRANE || kind == FK_BEND || kind == FK_PSPRING) {
+    stamp_cloth_slot(kind, S.clothSlots[slot], pv, xs, S.springs, alpha, acc);
   }
 }
 

@@ -22,6 +22,7 @@ struct DeviceState {
   F4 *colliderShape, *colliderLocalPos, *colliderLocalRot, *colliderFriction;
   // broad phase
   uint32_t *hashedIdx, *globalIdx;
+  uint32_t* hashedRigid;      // hashed colliders whose owner is not a particle
   uint32_t *cellCount, *cellStart, *cellBodies, *cellRigid;
   // per hashed collider (index = position in hashedIdx): this frame's center
   // and |radius|, and static metadata (owner, group, shared | dynamic << 1,
@@ -53,6 +54,20 @@ struct DeviceState {
   // body's color are read from here, so a same-color contact is a Jacobi
   // coupling with a schedule-independent result
   F4 *posLinSnap, *posAngSnap;
+  // cloth surfaces
+  ClothSlot* clothSlots;      // parallel to sAdjList (membrane / bend / particle-spring entries)
+  U4* tris;                   // collision triangles (vertex ids in xyz)
+  U2* edges;                  // unique surface edges (ids ascending)
+  uint32_t *nbrStart, *nbrCount, *nbrList;  // sorted topological 1-ring of every body
+  uint32_t* particleIdx;      // surface particles, ascending
+  uint32_t* selfCollision;    // per body: its surface self-collides
+  uint32_t *elemCellCount, *elemCellStart, *elemCellCursor, *elemCells;
+  uint32_t *datCount, *datStart;  // per producer: surface particles, then edges (+1 total)
+  U4* datPairs;               // (kind, owner, other, 0), producer-major, other ascending
+  uint32_t *softFlag, *softStart;  // emission counts / offsets (RT per triangle, DAT per pair)
+  SoftRec *soft, *prevSoft;
+  uint32_t *softMapKeyA, *softMapKeyB, *softMapVal;
+  F4* velScratch;             // unsmoothed velocities for the 1-ring viscosity gather
   // misc
   uint32_t* colorFlags;       // per-pass changed flags of the dynamic coloring
   uint32_t* counters;

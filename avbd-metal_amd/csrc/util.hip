@@ -142,6 +142,66 @@ __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t* in, uint32
   }
 }
 
+// Device-sized variants: the scanned length is min(*nDev, cap) + 1 (the
+// extra element is a zero the producer wrote, so out[n - 1] is the total);
+// tiles past it exit at once.
+__device__ __forceinline__ uint32_t dev_len(const uint32_t* nDev, uint32_t cap) {
+  uint32_t n = *nDev;
+  return (n < cap ? n : cap) + 1u;
+}
+__global__ void __launch_bounds__(256) k_scan_tiles_dev(const uint32_t* in, uint32_t* out, const uint32_t* nDev,
+                                                       uint32_t cap, uint32_t* blockSums) {
+  __shared__ uint32_t lds[4];
+  const uint32_t n = dev_len(nDev, cap);
+  if (blockIdx.x * kScanTile >= n) return;
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
+  uint32_t a[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) a[i] = (base + i < n) ? in[base + i] : 0u;
+  uint32_t local = a[0] + a[1] + a[2] + a[3];
+  uint32_t total;
+  uint32_t off = block_exclusive_scan_256(local, lds, total);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (base + i < n) out[base + i] = off;
+    off += a[i];
+  }
+  if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(256) k_scan_tile_sums_dev(uint32_t* sums, const uint32_t* nDev, uint32_t cap) {
+  __shared__ uint32_t lds[4];
+  const uint32_t n = dev_len(nDev, cap);
+  const uint32_t numTiles = (n + kScanTile - 1) / kScanTile;
+  uint32_t carry = 0;
+  for (uint32_t chunk = 0; chunk < numTiles; chunk += 256) {
+    uint32_t i = chunk + threadIdx.x;
+    uint32_t v = i < numTiles ? sums[i] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan_256(v, lds, total);
+    if (i < numTiles) sums[i] = carry + ex;
+    carry += total;
+  }
+}
+__global__ void __launch_bounds__(256) k_scan_add_dev(uint32_t* out, const uint32_t* nDev, uint32_t cap,
+                                                     const uint32_t* sums) {
+  const uint32_t n = dev_len(nDev, cap);
+  if (blockIdx.x * kScanTile >= n) return;
+  uint32_t add = sums[blockIdx.x];
+  if (add == 0) return;
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (base + i < n) out[base + i] += add;
+}
+void launch_exclusive_scan_dev(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t nMax, const uint32_t* nDev,
+                               uint32_t cap, uint32_t* scratch) {
+  if (nMax == 0) return;
+  uint32_t tiles = (nMax + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles_dev, dim3(tiles), dim3(256), 0, st, in, out, nDev, cap, scratch);
+  hipLaunchKernelGGL(k_scan_tile_sums_dev, dim3(1), dim3(256), 0, st, scratch, nDev, cap);
+  hipLaunchKernelGGL(k_scan_add_dev, dim3(tiles), dim3(256), 0, st, out, nDev, cap, scratch);
+}
+
 uint32_t scan_scratch_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch) {

@@ -9,8 +9,12 @@ onto the ground, dt = 1/60.  Inputs are resident on the GPU; the timed region
 is K steps bracketed by device synchronisation.
 
 Prints one JSON line: value = vertex-iters/s (dynamic vertices x iterations
-x steps / wall time), the primal color-sweep roofline measured live with HIP
-events, and the oracle CPU baseline on a bounded sample.
+x steps / wall time), the step rate when every frame iterates to residual
+1e-4 (cap 200 sweeps), the primal color-sweep roofline measured live with
+HIP events, and the oracle CPU baseline on a bounded sample.
+
+--workload cloth / batched run the 256^2 cloth drape or the 4096 batched
+rigid scenes instead (reported the same way; the default is the soft block).
 """
 import argparse
 import json
@@ -27,10 +31,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "vertex-iters/s + ms/step to residual 1e-4; batched+learned vs plain AVBD"
 
 
-def traffic_from_profiles():
-    """HBM bytes per primal launch from the committed rocprofv3 PMC summary
-    (profiles/primal_soft_block_21_pmc.json), when present."""
-    path = os.path.join(ROOT, "profiles", "primal_soft_block_21_pmc.json")
+def traffic_from_profiles(name):
+    """HBM bytes per primal launch from a committed rocprofv3 PMC summary, when present."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
@@ -40,7 +43,7 @@ def traffic_from_profiles():
         return None
 
 
-def cpu_baseline(scene, target_s=10.0):
+def cpu_baseline(scene, unit, target_s=10.0):
     """Oracle (serial C++ FP32 restatement) on the host, one core, bounded sample."""
     import oracle
 
@@ -54,9 +57,27 @@ def cpu_baseline(scene, target_s=10.0):
     o.step(n)
     dt = time.perf_counter() - t0
     iters = scene.settings.iterations
-    return {"value": scene.numDynamic * iters * n / dt, "unit": "vertex-iters/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same soft-block workload after 2 warm-up steps, serial oracle FP32, "
-                      f"{dt:.2f} s"}
+    return {"value": scene.numDynamic * iters * n / dt, "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the same workload after 2 warm-up steps, serial oracle FP32, {dt:.2f} s"}
+
+
+def build(workload, cells):
+    from avbd_amd import demos
+
+    if workload == "cloth":
+        s = demos.cloth_scene()
+        desc = (f"cloth drape 256^2 ({s.numDynamic} vertices, {len(s.tris)} triangles, {len(s.springs)} "
+                f"structural springs) onto a fixed r = 0.5 m sphere, {s.settings.iterations} iterations, dt 1/60")
+        return s, desc, "vertex-iters/s", None
+    if workload == "batched":
+        s, ranges = demos.batched_scene()
+        desc = (f"4096 batched uneven rigid scenes ({s.numDynamic} bodies, {len(s.joints)} hinges) in one solver, "
+                f"{s.settings.iterations} iterations, dt 1/60, contact-aware dynamic coloring")
+        return s, desc, "body-iters/s", None
+    s = demos.soft_block_scene(cells=cells)
+    desc = (f"soft block {cells}^3 cells, {s.numDynamic} vertices, {len(s.tets)} SNH tets, "
+            f"{s.settings.iterations} iterations, dt 1/60, static coloring")
+    return s, desc, "vertex-iters/s", "primal_soft_block_21_pmc.json"
 
 
 def main():
@@ -65,6 +86,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cells", type=int, default=20)
+    ap.add_argument("--workload", choices=["soft_block", "cloth", "batched"], default="soft_block")
+    ap.add_argument("--residual-steps", type=int, default=30, help="frames iterated to residual 1e-4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
     ap.add_argument("--sweep", type=int, default=0, help="1 per-color launches, 2 persistent (0 = autotune)")
@@ -74,9 +97,9 @@ def main():
               f"(independent scenes would shard per GPU with no per-step exchange)", file=sys.stderr)
         sys.exit(2)
 
-    from avbd_amd import GPUSolver, demos
+    from avbd_amd import GPUSolver
 
-    scene = demos.soft_block_scene(cells=args.cells)
+    scene, desc, unit, pmc = build(args.workload, args.cells)
     solver = GPUSolver(scene, lanesPerBody=args.lanes, sweepMode=args.sweep)
     st = solver.stats()
     for _ in range(args.warmup):
@@ -92,29 +115,45 @@ def main():
     value = nv * iters * args.steps / wall
     ms_step = wall * 1000.0 / args.steps
     residual = solver.residual()
+    st = solver.stats()
 
-    # roofline of the dominant kernel (per-color primal sweep), HIP events on the solver stream
-    tp = solver.timePrimal(reps=50)
-    bytes_per_launch = tp["sweep_bytes"] / tp["launches_per_sweep"]
-    achieved = bytes_per_launch / (tp["avg_launch_us"] * 1e-6) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-            "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
-            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+    # the second half of the metric: frames iterated until residual <= 1e-4 (cap 200)
+    res = None
+    if args.residual_steps > 0:
+        its, reached = [], 0
+        t0 = time.perf_counter()
+        for _ in range(args.residual_steps):
+            n, r = solver.stepToResidual(1e-4, 200, 5)
+            its.append(n)
+            reached += r <= 1e-4
+        rwall = time.perf_counter() - t0
+        res = {"tolerance": 1e-4, "max_iterations": 200, "check_every": 5, "steps": args.residual_steps,
+               "ms_per_step": round(rwall * 1000.0 / args.residual_steps, 4),
+               "iterations_mean": round(sum(its) / len(its), 2), "reached_fraction": reached / len(its)}
 
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "vertex-iters/s", "n_gpus": 1,
+    out = {"metric": METRIC, "value": round(value, 1), "unit": unit, "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded SplitMix64 scene)",
-           "config": {"workload": f"soft block {args.cells}^3 cells, {nv} vertices, {len(scene.tets)} SNH tets, "
-                                  f"{iters} iterations, dt 1/60, static coloring",
-                      "vertices": nv, "tets": len(scene.tets), "iterations": iters,
+           "config": {"workload": desc, "dynamic_bodies": nv, "iterations": iters,
                       "colors": st.num_colors, "lanes_per_body": st.lanes_per_body,
                       "sweep_mode": "persistent" if st.sweep_mode == 2 else "per-color launches",
-                      "end_residual": residual},
-           "roofline": roof}
+                      "end_residual": residual, "residual_1e-4": res}}
+    if args.workload == "cloth":
+        out["config"].update({"soft_contacts_last_frame": st.num_soft, "dat_pairs_last_frame": st.dat_pairs})
+
+    # roofline of the dominant kernel (per-color primal sweep), HIP events on the solver stream
+    if args.workload != "batched":
+        tp = solver.timePrimal(reps=50)
+        bytes_per_launch = tp["sweep_bytes"] / tp["launches_per_sweep"]
+        achieved = bytes_per_launch / (tp["avg_launch_us"] * 1e-6) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": traffic_from_profiles(pmc) if pmc else None,
+                           "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
+                           "algorithmic_bytes_per_launch": round(bytes_per_launch)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene)
+        out["cpu_baseline"] = cpu_baseline(scene, unit)
     print(json.dumps(out), flush=True)
 
 

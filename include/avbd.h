@@ -53,7 +53,10 @@ typedef enum avbd_failure_kind {
     AVBD_FAIL_STATIC_COLOR_CAPACITY = 4, /* .staticColorCapacity (thrown by create) */
     AVBD_FAIL_UNRESOLVED_COLORING = 5,   /* .unresolvedColoring */
     AVBD_FAIL_STAGE_DENIED = 6,          /* .commandEncoderCreation (test hook) */
-    AVBD_FAIL_BARRIER_TIMEOUT = 7        /* persistent kernel grid barrier exceeded its spin bound */
+    AVBD_FAIL_BARRIER_TIMEOUT = 7,       /* persistent kernel grid barrier exceeded its spin bound */
+    AVBD_FAIL_DAT_PAIR_CAPACITY = 8,     /* .planarDATPairCapacity (commandExecution, PlanarDAT code 1) */
+    AVBD_FAIL_DAT_GRID_SPAN = 9,         /* .planarDATElementGridSpan (PlanarDAT code 2) */
+    AVBD_FAIL_DAT_INVALID_ANCHOR = 10    /* .planarDATInvalidAnchor, non-finite anchors (PlanarDAT code 3) */
 } avbd_failure_kind;
 
 typedef struct avbd_failure {
@@ -177,6 +180,8 @@ typedef struct avbd_options {
     int32_t sweep_mode;                /* 0 = autotune, 1 = one launch per color, 2 = persistent cooperative sweep */
     int32_t use_graph;                 /* 1 = capture each step in a hipGraph (default), 0 = eager launches */
     int32_t pair_capacity_override;    /* test hook (planarDATPairCapacityForTesting analogue): >0 caps rigid pairs */
+    int32_t dat_pair_capacity_override;  /* test hook, planarDATPairCapacityForTesting (GPUSolver.swift:2119): >0 caps the V-T/E-E stream */
+    int32_t soft_capacity_override;      /* test hook, planarDATSoftCapacityForTesting (GPUSolver.swift:2128): >0 caps soft contacts */
 } avbd_options;
 
 typedef struct avbd_solver avbd_solver;
@@ -249,6 +254,10 @@ typedef struct avbd_stats {
     int32_t uses_dynamic_coloring;
     int32_t lanes_per_body;        /* primal lane split in use */
     int32_t sweep_mode;            /* 1 per-color launches, 2 persistent */
+    int32_t dat_pairs;             /* lastPlanarDATPairs: V-T + E-E pair demand */
+    int32_t dat_vt_pairs;          /* lastPlanarDATVertexTrianglePairs */
+    int32_t dat_ee_pairs;          /* lastPlanarDATEdgeEdgePairs */
+    int32_t num_soft_rt;           /* rigid-feature vs triangle records (prefix of the soft array) */
 } avbd_stats;
 avbd_status avbd_get_stats(avbd_solver* s, avbd_stats* out);
 
@@ -268,6 +277,39 @@ avbd_status avbd_get_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint3
 /* Manifolds of the last frame (ManifoldGPU, GPUTypes.swift:164): raw 704-byte
  * records, `capacity` records at most. */
 avbd_status avbd_get_manifolds(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records);
+
+/* Residual-driven step: runs one frame whose iteration loop stops once the
+ * residual r = |g| / |m g_grav| over dynamic bodies (the block right-hand
+ * side of primal_one, 40_solver.metal:1384-1425, at the current iterate)
+ * reaches `tol`, checking every `check_every` sweeps, at most
+ * `max_iterations`.  No reference counterpart (the reference always runs
+ * settings.iterations sweeps, GPUSolver.swift:3138-4405); it measures the
+ * "step to residual 1e-4" rate.  Completes and retires the frame. */
+avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t max_iterations, int32_t check_every,
+                                  int32_t* iterations_used, float* residual);
+
+/* Cloth surface topology derived at create (GPUSolver.swift:1289-1640): counts
+ * of collision triangles, unique edges, surface particles, membrane and bend
+ * elements, and the grid / query parameters.  Any output pointer may be NULL;
+ * edges[2*num_edges] (ascending vertex ids, edge order of the pair stream),
+ * particles[num_particles]. */
+typedef struct avbd_cloth_info {
+    uint32_t num_tris, num_edges, num_particles, num_membranes, num_bends;
+    uint32_t elem_hash_size, max_soft, soft_map_capacity, max_dat_pairs;
+    float elem_cell_size, elem_margin, dat_query_radius, surface_contact_cell_size;
+} avbd_cloth_info;
+avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
+
+/* Planar-DAT pair stream of the last frame (dat_build_vt_pairs /
+ * dat_build_ee_pairs, 27_planar_dat.metal:35-229): pairs[3*n] = (kind 0 V-T
+ * | 1 E-E, owner vertex / edge, triangle / edge), producer-major with the
+ * partner ascending. */
+avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs);
+
+/* Element contacts of the last frame (SoftContactGPU, GPUTypes.swift:127-136):
+ * raw 112-byte records -- ids u32[4], normal+friction, anchorA+flags, weights,
+ * C0+lambdaCap, lambda+keyA, penalty+keyB (float4 each). */
+avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records);
 
 /* Persistence map lookup (pairMapFind, 20_broadphase.metal:596): for each
  * (a,b) collider pair returns the manifold index of the map built by the last

@@ -42,6 +42,7 @@ def load():
         "oracle_destroy": (None, [vp]),
         "oracle_step": (C.c_int, [vp]),
         "oracle_set_settings": (C.c_int, [vp, vp]),
+        "oracle_step_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]),
         "oracle_get_body_states": (C.c_int, [vp, C.c_uint32, fp, fp, fp, fp]),
         "oracle_get_body_states_f64": (C.c_int, [vp, C.c_uint32, dp, dp, dp, dp]),
         "oracle_set_body_states": (C.c_int, [vp, u32p, C.c_uint32, fp, fp, fp, fp]),
@@ -56,6 +57,13 @@ def load():
         "oracle_snh_tet": (None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_solve6x6": (None, [dp, dp, dp, C.c_int]),
         "oracle_cell_hash": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32]),
+        "oracle_get_cloth_info": (C.c_int, [vp, vp, u32p, u32p]),
+        "oracle_get_dat_pairs": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
+        "oracle_get_soft_contacts": (C.c_int, [vp, C.c_uint32, u32p, vp]),
+        "oracle_cloth_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p]),
+        "oracle_dat_pairs_at": (C.c_int, [vp, fp, C.c_uint32, u32p, u32p]),
+        "oracle_membrane": (None, [dp, dp, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
+        "oracle_hinge": (C.c_int, [fp, C.c_float, C.c_float, fp, fp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -92,6 +100,13 @@ class OracleSolver:
     def step(self, n=1):
         for _ in range(n):
             self._lib.oracle_step(self._h)
+
+    def stepToResidual(self, tol=1e-4, maxIterations=200, checkEvery=5):
+        it = C.c_int()
+        r = C.c_double()
+        self._lib.oracle_step_to_residual(self._h, float(tol), int(maxIterations), int(checkEvery), C.byref(it),
+                                          C.byref(r))
+        return it.value, r.value
 
     def set_settings(self, settings):
         s = settings.to_c()
@@ -157,6 +172,53 @@ class OracleSolver:
         self._lib.oracle_map_find(self._h, _p(q, C.c_uint32), len(q), _p(out, C.c_int32))
         return out
 
+    def clothInfo(self):
+        import sys
+        sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "avbd-metal_amd"))
+        from avbd_amd import _capi
+        info = _capi.ClothInfo()
+        self._lib.oracle_get_cloth_info(self._h, C.byref(info), None, None)
+        edges = np.zeros((max(1, info.num_edges), 2), np.uint32)
+        parts = np.zeros(max(1, info.num_particles), np.uint32)
+        self._lib.oracle_get_cloth_info(self._h, None, _p(edges, C.c_uint32), _p(parts, C.c_uint32))
+        out = {f: getattr(info, f) for f, _ in _capi.ClothInfo._fields_}
+        out["edges"] = edges[:info.num_edges]
+        out["particles"] = parts[:info.num_particles]
+        return out
+
+    def datPairs(self):
+        n = C.c_uint32()
+        self._lib.oracle_get_dat_pairs(self._h, 0, C.byref(n), None)
+        out = np.zeros((max(1, n.value), 3), np.uint32)
+        self._lib.oracle_get_dat_pairs(self._h, n.value, C.byref(n), _p(out, C.c_uint32))
+        return out[:n.value]
+
+    SOFT_DTYPE = np.dtype([("ids", np.uint32, 4), ("normal", np.float32, 4), ("anchorA", np.float32, 4),
+                           ("weights", np.float32, 4), ("C0", np.float32, 4), ("lambda", np.float32, 4),
+                           ("penalty", np.float32, 4)])
+
+    def softContacts(self):
+        n = C.c_uint32()
+        self._lib.oracle_get_soft_contacts(self._h, 0, C.byref(n), None)
+        out = np.zeros(max(1, n.value), self.SOFT_DTYPE)
+        self._lib.oracle_get_soft_contacts(self._h, n.value, C.byref(n), out.ctypes.data_as(C.c_void_p))
+        return out[:n.value]
+
+    def datPairsAt(self, positions):
+        """Pair stream the element grid yields at the given positions (state unchanged)."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = C.c_uint32()
+        self._lib.oracle_dat_pairs_at(self._h, _p(p, C.c_float), 0, C.byref(n), None)
+        out = np.zeros((max(1, n.value), 3), np.uint32)
+        self._lib.oracle_dat_pairs_at(self._h, _p(p, C.c_float), n.value, C.byref(n), _p(out, C.c_uint32))
+        return out[:n.value]
+
+    def clothStats(self):
+        v = [C.c_int32() for _ in range(6)]
+        self._lib.oracle_cloth_stats(self._h, *[C.byref(x) for x in v])
+        return dict(zip(["dat_pairs", "dat_vt_pairs", "soft_candidates", "num_soft_rt", "overflow", "nonfinite"],
+                        [x.value for x in v]))
+
     def residual(self):
         v = C.c_double()
         self._lib.oracle_residual(self._h, C.byref(v))
@@ -201,3 +263,27 @@ def solve6x6(A, b, precision=64):
 def cell_hash(p, cell_size, group, hash_size):
     return load().oracle_cell_hash(float(p[0]), float(p[1]), float(p[2]), float(cell_size), int(group),
                                    int(hash_size))
+
+
+def membrane(x, dm, muA, lamA, slot):
+    """StVK membrane energy, gradient on `slot` and its Gauss-Newton block (double)."""
+    lib = load()
+    x = np.ascontiguousarray(x, np.float64).reshape(9)
+    dm = np.ascontiguousarray(dm, np.float64).reshape(4)
+    e = C.c_double()
+    g = np.zeros(3)
+    h = np.zeros(9)
+    lib.oracle_membrane(_p(x, C.c_double), _p(dm, C.c_double), float(muA), float(lamA), int(slot), C.byref(e),
+                        _p(g, C.c_double), _p(h, C.c_double))
+    return e.value, g, h.reshape(3, 3)
+
+
+def hinge(x, k1, k2):
+    """Hinge constants (K[4], s) the upload derives for two triangles sharing
+    the edge x[0]-x[1], opposite vertices x[2], x[3]; None when skipped."""
+    lib = load()
+    x = np.ascontiguousarray(x, np.float32).reshape(12)
+    K = np.zeros(4, np.float32)
+    s = C.c_float()
+    ok = lib.oracle_hinge(_p(x, C.c_float), float(k1), float(k2), _p(K, C.c_float), C.byref(s))
+    return (K, s.value) if ok else None

@@ -15,8 +15,8 @@
 //  * a contact partner that shares the body's color is read from the
 //    start-of-iteration pose snapshot (the backend closes the reference's
 //    same-color Jacobi race this way; the oracle mirrors the rule).
-// Out of scope here, as in the backend: surface truncation (isotropic OGC /
-// Planar-DAT), cloth elements, torus / capsule / hull colliders.
+// Out of scope here, as in the backend: surface truncation passes (isotropic
+// OGC bounds, Planar-DAT reduce/apply), torus / capsule / hull colliders.
 #include "oracle.h"
 
 #include <math.h>
@@ -166,7 +166,7 @@ template <typename T> T clampT(T v, T lo, T hi) { return std::min(std::max(v, lo
 template <typename T> bool finiteT(T v) { return std::isfinite(v); }
 
 const uint32_t WORLD = 0xFFFFFFFFu;
-const uint32_t FKJ = 0, FKS = 1, FKM = 2, FKT = 3;
+const uint32_t FKJ = 0, FKS = 1, FKM = 2, FKT = 3, FKMEM = 5, FKBEND = 6;
 
 // ---------------------------------------------------------------------------
 // records (GPUTypes.swift:80-173 field meanings)
@@ -207,6 +207,20 @@ struct Tet {
   uint32_t ids[4];
   V<T> r[3];
   T six, muV, lamV;
+};
+
+// cloth: StVK membrane (2x2 inverse rest shape, column major), Bergou hinge,
+// unified element contact (GPUTypes.swift:127-148)
+template <typename T>
+struct MembraneT { uint32_t ids[3]; T dm[4]; T muA, lamA; };
+template <typename T>
+struct BendT { uint32_t ids[4]; T K[4]; T s; };
+template <typename T>
+struct SoftT {
+  uint32_t ids[4];
+  V<T> n, anchor, lambda, penalty;
+  T friction, C0, lamCap, w[4];
+  uint32_t flags, keyA, keyB;
 };
 
 uint32_t cellHash32(int cx, int cy, int cz, uint32_t group, uint32_t hashSize) {
@@ -257,6 +271,23 @@ struct Oracle : oracle_base {
   std::vector<Spring<T>> springs;
   std::vector<Tet<T>> tets;
   std::vector<std::vector<uint32_t>> staticAdj;  // per body, sorted entries (joints, springs, tets)
+  // cloth surfaces (GPUSolver.swift:1289-1690, 25_cloth.metal, 27_planar_dat.metal)
+  using Membrane = MembraneT<T>;
+  using Bend = BendT<T>;
+  using Soft = SoftT<T>;
+  uint32_t nTris = 0;
+  std::vector<std::array<uint32_t, 3>> tris;
+  std::vector<std::pair<uint32_t, uint32_t>> edges;
+  std::vector<uint32_t> nbrStart, nbrCount, nbrList, particles, selfColl, hashedRigidList;
+  std::vector<Membrane> membranes;
+  std::vector<Bend> bends;
+  float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
+  uint32_t maxDatPairs = 1, maxSoft = 1;
+  std::vector<std::array<uint32_t, 3>> datPairs;
+  uint32_t datTotal = 0, datVT = 0, datOverflow = 0, datNonfinite = 0, softCandidates = 0, softRT = 0;
+  std::vector<Soft> soft, prevSoft;
+  std::map<std::pair<uint32_t, uint32_t>, int> softMap;  // keys of prevSoft -> index
+  std::vector<std::vector<uint32_t>> softAdj;
   // frame state
   std::vector<Manifold<T>> manifolds, prevManifolds;
   std::map<std::pair<uint32_t, uint32_t>, int> prevMap;  // collider pair -> index in prevManifolds
@@ -370,7 +401,8 @@ struct Oracle : oracle_base {
     float maxR = 0.05f;
     for (uint32_t i : hashed) maxR = std::max(maxR, rad32[i]);
     cellSize = maxR * 2.0f;
-    for (uint32_t i : hashed) if (!particle[cOwner[i]]) nHashedRigid++;
+    for (uint32_t i : hashed)
+      if (!particle[cOwner[i]]) { nHashedRigid++; hashedRigidList.push_back(i); }
     maxPairs = std::max<uint32_t>(4096u, enabled * (uint32_t)maxPairsPerBody);
     hashSize = nextPow2(std::max<uint32_t>(64u, 2u * std::max<uint32_t>(1u, enabled)));
     // joints (GPUSolver.swift:1092-1193)
@@ -474,7 +506,10 @@ struct Oracle : oracle_base {
     excl.assign(ex.begin(), ex.end());
     // soft collision group of tet boundary particles (GPUSolver.swift:1476-1522)
     clothGroup.assign(nb, 0);
-    if (!tets.empty()) {
+    if (d->num_tris > 0) {
+      if (d->num_tets > 0) return 4;  // tet surfaces with element contacts: not restated
+      clothInit(d);
+    } else if (!tets.empty()) {
       std::map<std::array<uint32_t, 3>, int> faces;
       const int fi[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
       for (auto& t : tets)
@@ -511,6 +546,10 @@ struct Oracle : oracle_base {
     }
     for (uint32_t t = 0; t < tets.size(); t++)
       for (int k = 0; k < 4; k++) if (dyn(tets[t].ids[k])) staticAdj[tets[t].ids[k]].push_back((FKT << 28) | t);
+    for (uint32_t m = 0; m < membranes.size(); m++)
+      for (int k = 0; k < 3; k++) if (dyn(membranes[m].ids[k])) staticAdj[membranes[m].ids[k]].push_back((FKMEM << 28) | m);
+    for (uint32_t m = 0; m < bends.size(); m++)
+      for (int k = 0; k < 4; k++) if (dyn(bends[m].ids[k])) staticAdj[bends[m].ids[k]].push_back((FKBEND << 28) | m);
     dynamicColoring = d->num_tets == 0 && d->num_tris == 0;
     // static greedy palette in body-index order (GPUSolver.swift:1956-2050)
     std::vector<std::set<uint32_t>> adj(nb);
@@ -535,6 +574,153 @@ struct Oracle : oracle_base {
     snapLin = pos;
     snapAng = rot;
     return 0;
+  }
+
+  // Surface topology, membranes, hinges, grid / query parameters and
+  // capacities (GPUSolver.swift:391-443, 1289-1690).  Element constants are
+  // derived in double and rounded to float, as the records store them.
+  void clothInit(const avbd_scene_desc* d) {
+    nTris = d->num_tris;
+    auto sizeX = [&](uint32_t i) { return d->body_size[3 * i]; };
+    auto bp = [&](uint32_t i, int k) { return d->body_position[3 * i + k]; };
+    auto fdist = [&](uint32_t a, uint32_t b) {
+      float dx = bp(a, 0) - bp(b, 0), dy = bp(a, 1) - bp(b, 1), dz = bp(a, 2) - bp(b, 2);
+      return std::sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    std::set<uint64_t> seen;
+    float maxElemR = 0.0f, maxPartR = 0.0f;
+    for (uint32_t t = 0; t < nTris; t++) {
+      std::array<uint32_t, 3> id{d->tri_ids[3 * t], d->tri_ids[3 * t + 1], d->tri_ids[3 * t + 2]};
+      tris.push_back(id);
+      uint32_t ev[3][2] = {{id[0], id[1]}, {id[1], id[2]}, {id[0], id[2]}};
+      for (auto& e : ev) {
+        uint64_t k = (uint64_t)std::min(e[0], e[1]) << 32 | std::max(e[0], e[1]);
+        if (seen.insert(k).second) edges.push_back({std::min(e[0], e[1]), std::max(e[0], e[1])});
+      }
+      float m[3];
+      for (int k = 0; k < 3; k++) m[k] = (bp(id[0], k) + bp(id[1], k) + bp(id[2], k)) / 3.0f;
+      auto dm = [&](uint32_t v) {
+        float dx = m[0] - bp(v, 0), dy = m[1] - bp(v, 1), dz = m[2] - bp(v, 2);
+        return std::sqrt(dx * dx + dy * dy + dz * dz);
+      };
+      float thick = std::max(sizeX(id[0]), std::max(sizeX(id[1]), sizeX(id[2]))) / 2.0f;
+      maxPartR = std::max(maxPartR, thick);
+      maxElemR = std::max(maxElemR, std::max(dm(id[0]), std::max(dm(id[1]), dm(id[2]))) + thick);
+    }
+    std::vector<std::vector<uint32_t>> nb1(nb);
+    float edgeLenSum = 0.0f;
+    for (auto& e : edges) {
+      nb1[e.first].push_back(e.second);
+      nb1[e.second].push_back(e.first);
+      maxElemR = std::max(maxElemR, fdist(e.first, e.second) / 2.0f + std::max(sizeX(e.first), sizeX(e.second)) / 2.0f);
+      maxPartR = std::max(maxPartR, std::max(sizeX(e.first), sizeX(e.second)) / 2.0f);
+      edgeLenSum += fdist(e.first, e.second);
+    }
+    nbrStart.assign(nb, 0);
+    nbrCount.assign(nb, 0);
+    for (uint32_t i = 0; i < nb; i++) {
+      std::sort(nb1[i].begin(), nb1[i].end());
+      nbrStart[i] = (uint32_t)nbrList.size();
+      nbrCount[i] = (uint32_t)nb1[i].size();
+      nbrList.insert(nbrList.end(), nb1[i].begin(), nb1[i].end());
+    }
+    // connected sheets, numbered by their first triangle; authored sheets self-collide
+    std::vector<uint32_t> parent(nb);
+    for (uint32_t i = 0; i < nb; i++) parent[i] = i;
+    std::function<uint32_t(uint32_t)> find = [&](uint32_t x) { return parent[x] == x ? x : (parent[x] = find(parent[x])); };
+    for (auto& t : tris) { uint32_t r = find(t[0]); parent[find(t[1])] = r; parent[find(t[2])] = r; }
+    selfColl.assign(nb, 0);
+    std::map<uint32_t, uint32_t> gid;
+    std::vector<uint8_t> surf(nb, 0);
+    for (auto& t : tris) {
+      uint32_t r = find(t[0]);
+      if (!gid.count(r)) { uint32_t nid = (uint32_t)gid.size() + 1; gid[r] = nid; }
+      for (uint32_t v : t)
+        if (particle[v]) { clothGroup[v] = gid[r]; selfColl[v] = 1; surf[v] = 1; }
+    }
+    for (uint32_t i = 0; i < nb; i++) if (surf[i]) particles.push_back(i);
+    // StVK membranes: rest 2x2 in the triangle frame, inverse column major
+    std::map<uint64_t, std::vector<std::pair<uint32_t, float>>> edgeTris;
+    for (uint32_t t = 0; t < nTris; t++) {
+      float mu = d->tri_mu ? d->tri_mu[t] : 0.0f;
+      if (!(mu > 0.0f)) continue;
+      float lam = d->tri_lambda ? d->tri_lambda[t] : mu, bend = d->tri_bend ? d->tri_bend[t] : 0.0f;
+      uint32_t i0 = tris[t][0], i1 = tris[t][1], i2 = tris[t][2];
+      V<double> x0(bp(i0, 0), bp(i0, 1), bp(i0, 2));
+      V<double> d1 = V<double>(bp(i1, 0), bp(i1, 1), bp(i1, 2)) - x0, d2 = V<double>(bp(i2, 0), bp(i2, 1), bp(i2, 2)) - x0;
+      V<double> n = crossv(d1, d2);
+      double area = lenv(n) / 2.0;
+      if (!(area > 1e-10)) continue;
+      V<double> e1 = normv(d1), e2 = normv(crossv(normv(n), e1));
+      double a = dotv(d1, e1), b = dotv(d1, e2), c = dotv(d2, e1), dd = dotv(d2, e2);
+      double det = a * dd - b * c;
+      if (!(std::fabs(det) > 1e-12)) continue;
+      Membrane mb;
+      mb.ids[0] = i0; mb.ids[1] = i1; mb.ids[2] = i2;
+      mb.dm[0] = (T)(float)(dd / det); mb.dm[1] = (T)(float)(-b / det);
+      mb.dm[2] = (T)(float)(-c / det); mb.dm[3] = (T)(float)(a / det);
+      mb.muA = (T)(float)(mu * area);
+      mb.lamA = (T)(float)(lam * area);
+      membranes.push_back(mb);
+      if (bend > 0.0f) {
+        uint32_t ev[3][3] = {{i0, i1, i2}, {i1, i2, i0}, {i0, i2, i1}};
+        for (auto& e : ev) edgeTris[(uint64_t)std::min(e[0], e[1]) << 32 | std::max(e[0], e[1])].push_back({e[2], bend});
+      }
+    }
+    // Bergou hinges on edges shared by two bending triangles (edge-key order)
+    for (auto& kv : edgeTris) {
+      if (kv.second.size() != 2) continue;
+      uint32_t v0 = (uint32_t)(kv.first >> 32), v1 = (uint32_t)(kv.first & 0xFFFFFFFFu);
+      uint32_t o2 = kv.second[0].first, o3 = kv.second[1].first;
+      V<double> p0(bp(v0, 0), bp(v0, 1), bp(v0, 2)), p1(bp(v1, 0), bp(v1, 1), bp(v1, 2));
+      double L = lenv(p0 - p1);
+      if (!(L > 1e-9)) continue;
+      double q[4][2] = {{0, 0}, {L, 0}, {0, 0}, {0, 0}};
+      bool ok = true;
+      for (int f = 0; f < 2 && ok; f++) {
+        uint32_t o = f == 0 ? o2 : o3;
+        V<double> po(bp(o, 0), bp(o, 1), bp(o, 2));
+        double l0 = lenv(po - p0), l1 = lenv(po - p1);
+        double x = (L * L + l0 * l0 - l1 * l1) / (2 * L), h2 = l0 * l0 - x * x;
+        if (!(h2 > 1e-12)) { ok = false; break; }
+        q[2 + f][0] = x;
+        q[2 + f][1] = (f == 0 ? 1.0 : -1.0) * std::sqrt(h2);
+      }
+      if (!ok) continue;
+      // K = generalized 4-D cross product of the rows {1}, {x}, {y}
+      auto minor = [&](int c0, int c1, int c2) {
+        double m00 = 1, m01 = 1, m02 = 1;
+        double m10 = q[c0][0], m11 = q[c1][0], m12 = q[c2][0];
+        double m20 = q[c0][1], m21 = q[c1][1], m22 = q[c2][1];
+        return m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+      };
+      double K[4] = {minor(1, 2, 3), -minor(0, 2, 3), minor(0, 1, 3), -minor(0, 1, 2)};
+      auto cot = [&](int ap, int ia, int ib) {
+        double ux = q[ia][0] - q[ap][0], uy = q[ia][1] - q[ap][1], vx = q[ib][0] - q[ap][0], vy = q[ib][1] - q[ap][1];
+        double cr = ux * vy - uy * vx;
+        return std::fabs(cr) > 1e-12 ? (ux * vx + uy * vy) / std::fabs(cr) : 0.0;
+      };
+      double want = cot(0, 1, 2) + cot(1, 0, 2);
+      if (!(std::fabs(K[2]) > 1e-12)) continue;
+      double a1 = std::fabs((q[1][0] - q[0][0]) * q[2][1]) / 2, a2 = std::fabs((q[1][0] - q[0][0]) * q[3][1]) / 2;
+      Bend bd;
+      bd.ids[0] = v0; bd.ids[1] = v1; bd.ids[2] = o2; bd.ids[3] = o3;
+      for (int k = 0; k < 4; k++) bd.K[k] = (T)(float)(K[k] * (want / K[2]));
+      bd.s = (T)(float)(3.0 * ((double)kv.second[0].second + (double)kv.second[1].second) / 2.0 / std::max(a1 + a2, 1e-9));
+      bends.push_back(bd);
+    }
+    float meanEdge = edges.empty() ? 0.2f : edgeLenSum / (float)edges.size();
+    elemMargin = std::min(0.01f, std::max(0.002f, 0.5f * maxPartR));
+    float forceReach = 2.0f * maxPartR + elemMargin;
+    datQueryRadius = std::max(1.5f * forceReach, 0.75f * meanEdge);
+    float reach = (2.0f * maxPartR + elemMargin) / 0.7f;
+    surfaceContactCellSize = std::max(0.02f, std::max(std::max(reach, meanEdge * 1.05f), maxElemR + maxPartR + elemMargin));
+    elemCellSize = std::max(0.02f, std::max(std::max(reach, meanEdge * 1.05f), 2.0f * maxElemR + datQueryRadius));
+    uint64_t np = particles.size(), ne = edges.size();
+    uint64_t cap = std::max<uint64_t>(1, 64 * np + 128 * ne);
+    maxDatPairs = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFF0ull);
+    maxSoft = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, 4ull * nTris + std::min<uint64_t>(cap, 8 * np + 8 * ne)),
+                                           (1u << 24) - 1);
   }
 
   void rebuildBuckets() {
@@ -1243,6 +1429,568 @@ struct Oracle : oracle_base {
     if (ft > 0) s.ll = s.ll + M<T>::diag(V<T>(ft, ft, ft)) + M<T>::outer(n, n) * (-ft);
   }
 
+
+  // -------------------------------------------------------------------------
+  // cloth contacts: rigid feature vs triangle (25_cloth.metal:928-1142), the
+  // Planar-DAT pair stream (27_planar_dat.metal:35-229) and OGC emission
+  // (27_planar_dat.metal:344-510, 25_cloth.metal:273-347).  The pair set is
+  // computed on exact cells (the kernels' hash buckets only add candidates
+  // that the exact cell-range / ownership tests reject), each producer's
+  // partners ascending.
+  // -------------------------------------------------------------------------
+  static V<T> closestTri(const V<T>& p, const V<T>& a, const V<T>& b, const V<T>& c, V<T>& bary) {
+    V<T> ab = b - a, ac = c - a, ap = p - a;
+    T d1 = dotv(ab, ap), d2 = dotv(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { bary = V<T>(1, 0, 0); return a; }
+    V<T> bpv = p - b;
+    T d3 = dotv(ab, bpv), d4 = dotv(ac, bpv);
+    if (d3 >= 0 && d4 <= d3) { bary = V<T>(0, 1, 0); return b; }
+    T vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+      T v = d1 / std::max(d1 - d3, T(1e-12));
+      bary = V<T>(1 - v, v, 0);
+      return a + ab * v;
+    }
+    V<T> cp = p - c;
+    T d5 = dotv(ab, cp), d6 = dotv(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { bary = V<T>(0, 0, 1); return c; }
+    T vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+      T w = d2 / std::max(d2 - d6, T(1e-12));
+      bary = V<T>(1 - w, 0, w);
+      return a + ac * w;
+    }
+    T va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+      T w = (d4 - d3) / std::max((d4 - d3) + (d5 - d6), T(1e-12));
+      bary = V<T>(0, 1 - w, w);
+      return b + (c - b) * w;
+    }
+    T den = va + vb + vc;
+    if (std::fabs(den) < T(1e-20)) { bary = V<T>(1, 0, 0); return a; }
+    T v = vb / den, w = vc / den;
+    bary = V<T>(1 - v - w, v, w);
+    return a + ab * v + ac * w;
+  }
+  static void segSeg(const V<T>& p0, const V<T>& p1, const V<T>& q0, const V<T>& q1, T& s_, T& t_) {
+    V<T> d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    T a = dotv(d1, d1), e = dotv(d2, d2), f = dotv(d2, r);
+    s_ = 0; t_ = 0;
+    if (a <= T(1e-12) && e <= T(1e-12)) return;
+    if (a <= T(1e-12)) { t_ = clampT(f / e, T(0), T(1)); return; }
+    T c = dotv(d1, r);
+    if (e <= T(1e-12)) { s_ = clampT(-c / a, T(0), T(1)); return; }
+    T b = dotv(d1, d2), den = a * e - b * b;
+    if (std::fabs(den) > T(1e-12)) s_ = clampT((b * f - c * e) / den, T(0), T(1));
+    t_ = (b * s_ + f) / e;
+    if (t_ < 0) { t_ = 0; s_ = clampT(-c / a, T(0), T(1)); }
+    else if (t_ > 1) { t_ = 1; s_ = clampT((b - c) / a, T(0), T(1)); }
+  }
+  static void orthoBasis(const V<T>& n, V<T>& t1, V<T>& t2) {
+    // 00_common.metal:224-228
+    t1 = std::fabs(n.x) > std::fabs(n.z) ? V<T>(-n.y, n.x, 0) : V<T>(0, -n.z, n.y);
+    t1 = normv(t1);
+    t2 = crossv(n, t1);
+  }
+  bool nbrHas(uint32_t v, uint32_t x) const {
+    auto b = nbrList.begin() + nbrStart[v], e = b + nbrCount[v];
+    return std::binary_search(b, e, x);
+  }
+  struct Cell { int x, y, z; bool operator<(const Cell& o) const { return std::tie(x, y, z) < std::tie(o.x, o.y, o.z); } };
+  Cell elemCell(const V<T>& p) const {
+    T cs = (T)elemCellSize;
+    return Cell{(int)std::floor(p.x / cs), (int)std::floor(p.y / cs), (int)std::floor(p.z / cs)};
+  }
+  static Cell clampCell(Cell raw, Cell c0) {
+    return Cell{std::min(raw.x, c0.x + 3), std::min(raw.y, c0.y + 3), std::min(raw.z, c0.z + 3)};
+  }
+  static bool finite3(const V<T>& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+  void elemBounds(uint32_t g, V<T>& lo, V<T>& hi) const {
+    T pad = (T)datQueryRadius;
+    auto vmin = [](const V<T>& a, const V<T>& b) { return V<T>(std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)); };
+    auto vmax = [](const V<T>& a, const V<T>& b) { return V<T>(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)); };
+    if (g < nTris) {
+      const auto& t = tris[g];
+      lo = vmin(vmin(pos[t[0]], pos[t[1]]), pos[t[2]]) - V<T>(pad, pad, pad);
+      hi = vmax(vmax(pos[t[0]], pos[t[1]]), pos[t[2]]) + V<T>(pad, pad, pad);
+    } else {
+      const auto& e = edges[g - nTris];
+      lo = vmin(pos[e.first], pos[e.second]) - V<T>(pad, pad, pad);
+      hi = vmax(pos[e.first], pos[e.second]) + V<T>(pad, pad, pad);
+    }
+  }
+  int softFind(uint32_t keyA, uint32_t keyB) const {
+    auto it = softMap.find({keyA, keyB});
+    return it == softMap.end() ? -1 : it->second;
+  }
+  Soft makeSoft(const uint32_t ids[4], const T w[4], const V<T>& n, T friction, const V<T>& anchor, T C0n, T restT,
+                T minMass, uint32_t kind, bool rigidA, bool roundA, bool sideNeg, uint32_t keyA, uint32_t keyB) const {
+    Soft r;
+    V<T> lam, pen;
+    int prev = softFind(keyA, keyB);
+    if (prev >= 0) {
+      const Soft& ps = prevSoft[prev];
+      lam = ps.lambda * ((T)set.alpha * (T)set.gamma);
+      pen = ps.penalty;
+      V<T> t1o, t2o, t1n, t2n;
+      orthoBasis(ps.n, t1o, t2o);
+      orthoBasis(n, t1n, t2n);
+      V<T> lt = t1o * lam.y + t2o * lam.z;
+      lam.y = dotv(t1n, lt);
+      lam.z = dotv(t2n, lt);
+    }
+    T dt = set.dt;
+    T floorK = std::max(T(1), minMass / (dt * dt));
+    for (int k = 0; k < 3; k++) pen[k] = clampT(pen[k] * (T)set.gamma, floorK, T(1e10));
+    for (int k = 0; k < 4; k++) { r.ids[k] = ids[k]; r.w[k] = w[k]; }
+    r.n = n;
+    r.friction = friction;
+    r.anchor = rigidA ? anchor : V<T>(restT, 0, 0);
+    r.flags = (rigidA ? 1u : 0u) | (sideNeg ? 2u : 0u) | (kind << 2) | (roundA ? 32u : 0u);
+    r.C0 = C0n;
+    r.lamCap = std::min((T)set.lambda_max, std::max(T(10), T(1e5) * minMass));
+    r.lambda = lam;
+    r.penalty = pen;
+    r.keyA = keyA;
+    r.keyB = keyB;
+    return r;
+  }
+  T prevSide(uint32_t keyA, uint32_t keyB, T proposed) const {
+    int prev = softFind(keyA, keyB);
+    if (prev < 0) return proposed;
+    return (prevSoft[prev].flags & 2u) ? T(-1) : T(1);
+  }
+
+  void rtEmit() {
+    const T big = std::numeric_limits<T>::max();
+    T margin = (T)set.collision_margin;
+    for (uint32_t gid = 0; gid < nTris; gid++) {
+      const auto& tid = tris[gid];
+      V<T> a = pos[tid[0]], b = pos[tid[1]], c = pos[tid[2]];
+      V<T> m = (a + b + c) / T(3);
+      T rt_ = std::max(std::fabs(radiusSigned[tid[0]]), std::max(std::fabs(radiusSigned[tid[1]]), std::fabs(radiusSigned[tid[2]])));
+      T rT = std::max(lenv(m - a), std::max(lenv(m - b), lenv(m - c))) + rt_;
+      V<T> triN = crossv(b - a, c - a);
+      T tl = lenv(triN);
+      if (tl < T(1e-12)) continue;
+      triN = triN / tl;
+      V<T> velT = (vel[tid[0]] + vel[tid[1]] + vel[tid[2]]) / T(3);
+      uint32_t emitted = 0;
+      auto test = [&](uint32_t col) {
+        uint32_t o = cOwner[col];
+        V<T> bp = pos[o];
+        Q<T> bq = rot[o];
+        V<T> cen = bp + qrot(bq, cLocal[col]);
+        Q<T> cr = qmul(bq, cLocalRot[col]);
+        uint32_t kind = cFlags[col] & 0xFu;
+        bool nearO = lenv(cen - m) <= rT + std::fabs(cRad[col]) + margin;
+        if (nearO && kind == 0) {
+          V<T> lm = qrot(qconj(cr), m - cen);
+          V<T> dq2(std::max(std::fabs(lm.x) - cSize[col].x * T(0.5), T(0)),
+                   std::max(std::fabs(lm.y) - cSize[col].y * T(0.5), T(0)),
+                   std::max(std::fabs(lm.z) - cSize[col].z * T(0.5), T(0)));
+          nearO = lenv(dq2) <= rT + margin + T(0.05);
+        }
+        if (!nearO) return;
+        for (int k = 0; k < 3; k++)
+          if (excluded(std::min(o, tid[k]), std::max(o, tid[k]))) return;
+        V<T> fw[4];
+        T fr[4];
+        uint32_t fid[4];
+        int nf = 0;
+        if (kind == 1) { fw[0] = cen; fr[0] = cSize[col].x * T(0.5); fid[0] = 8; nf = 1; }
+        else if (kind == 0) {
+          V<T> h3 = cSize[col] * T(0.5);
+          for (uint32_t i = 0; i < 8; i++) {
+            V<T> loc((i & 1) ? h3.x : -h3.x, (i & 2) ? h3.y : -h3.y, (i & 4) ? h3.z : -h3.z);
+            V<T> w = xformv(cen, cr, loc);
+            if (lenv(w - m) > rT + T(4) * margin) continue;
+            if (nf < 4) { fw[nf] = w; fr[nf] = 0; fid[nf] = i; nf++; }
+          }
+        }
+        for (int fi = 0; fi < nf && emitted < 4; fi++) {
+          V<T> bary;
+          V<T> q = closestTri(fw[fi], a, b, c, bary);
+          V<T> dv = fw[fi] - q;
+          T dist = lenv(dv), hSum = rt_ + fr[fi];
+          T inflate = std::min(lenv(vel[o] - velT) * (T)set.dt, T(0.5) * rT);
+          if (dist - hSum > (T)elemMargin + inflate) continue;
+          T side = dotv(dv, triN) >= 0 ? T(1) : T(-1);
+          uint32_t keyA = (2u << 28) | col, keyB = gid * 16u + fid[fi];
+          T proposed = dotv(bp - q, triN) >= 0 ? T(1) : T(-1);
+          bool interior = bary.x > T(0.02) && bary.y > T(0.02) && bary.z > T(0.02);
+          T sMem = interior ? prevSide(keyA, keyB, proposed) : proposed;
+          V<T> n;
+          T g;
+          if (dist > T(1e-7) && side == sMem) { n = dv / dist; g = dist; }
+          else { n = triN * sMem; g = -dist; }
+          T fricT = (friction[tid[0]] + friction[tid[1]] + friction[tid[2]]) / T(3);
+          T fric = std::sqrt(std::max(friction[o] * fricT, T(0)));
+          T minMass = mass[o] > 0 ? mass[o] : big;
+          for (int k = 0; k < 3; k++) if (mass[tid[k]] > 0) minMass = std::min(minMass, mass[tid[k]]);
+          if (minMass == big) continue;
+          bool roundA = kind != 0;
+          V<T> anchor = roundA ? fw[fi] - bp : qrot(qconj(bq), fw[fi] - bp);
+          uint32_t ids[4] = {o, tid[0], tid[1], tid[2]};
+          T w[4] = {T(1), -bary.x, -bary.y, -bary.z};
+          Soft r = makeSoft(ids, w, n, fric, anchor, g - hSum + (T)elemMargin, hSum - (T)elemMargin, minMass, 2u, true,
+                            roundA, sMem < 0, keyA, keyB);
+          softCandidates++;
+          if (soft.size() < maxSoft) soft.push_back(r);
+          emitted++;
+        }
+      };
+      if (nHashedRigid > 0 && nHashedRigid <= 8) {
+        for (uint32_t ri = 0; ri < nHashedRigid && emitted < 4; ri++) test(hashedRigidList[ri]);
+      } else if (nHashedRigid > 0) {
+        int R = (int)std::ceil((rT + T(0.5) * (T)cellSize) / (T)cellSize);
+        R = std::max(1, std::min(4, R));
+        int cx, cy, cz;
+        cellOf(m, cx, cy, cz);
+        for (int dz = -R; dz <= R && emitted < 4; dz++)
+          for (int dy = -R; dy <= R && emitted < 4; dy++)
+            for (int dx = -R; dx <= R && emitted < 4; dx++) {
+              uint32_t h = cellHash32(cx + dx, cy + dy, cz + dz, 0u, hashSize);
+              for (uint32_t k = cellStart[h]; k < cellStart[h] + cellCount[h] && emitted < 4; k++) {
+                uint32_t col = cellBodies[k];
+                if (cRad[col] < 0) continue;
+                test(col);
+              }
+            }
+      }
+      for (uint32_t gI = 0; gI < globals.size() && emitted < 4; gI++) {
+        uint32_t col = globals[gI];
+        if (cRad[col] < 0) continue;
+        test(col);
+      }
+    }
+    softRT = softCandidates;
+  }
+
+  void datBuild() {
+    datPairs.clear();
+    datTotal = datVT = 0;
+    std::map<Cell, std::vector<uint32_t>> grid;
+    for (uint32_t g = 0; g < nTris + (uint32_t)edges.size(); g++) {
+      V<T> lo, hi;
+      elemBounds(g, lo, hi);
+      if (!finite3(lo) || !finite3(hi)) { datNonfinite++; continue; }
+      Cell c0 = elemCell(lo), raw = elemCell(hi);
+      if (raw.x - c0.x > 3 || raw.y - c0.y > 3 || raw.z - c0.z > 3) datOverflow++;
+      Cell c1 = clampCell(raw, c0);
+      for (int z = c0.z; z <= c1.z; z++)
+        for (int y = c0.y; y <= c1.y; y++)
+          for (int x = c0.x; x <= c1.x; x++) grid[Cell{x, y, z}].push_back(g);
+    }
+    T q2 = (T)datQueryRadius * (T)datQueryRadius;
+    auto emit = [&](uint32_t kind, uint32_t owner, std::vector<uint32_t>& others) {
+      std::sort(others.begin(), others.end());
+      for (uint32_t o : others) {
+        datTotal++;
+        if (datPairs.size() < maxDatPairs) datPairs.push_back({kind, owner, o});
+      }
+    };
+    for (uint32_t v : particles) {
+      V<T> p = pos[v];
+      if (!finite3(p)) { datNonfinite++; continue; }
+      Cell cell = elemCell(p);
+      std::vector<uint32_t> acc;
+      auto it = grid.find(cell);
+      if (it != grid.end()) {
+        for (uint32_t e : it->second) {
+          if (e >= nTris) continue;
+          const auto& t = tris[e];
+          if (t[0] == v || t[1] == v || t[2] == v) continue;
+          uint32_t g = clothGroup[v];
+          bool same = g != 0 && clothGroup[t[0]] == g && clothGroup[t[1]] == g && clothGroup[t[2]] == g;
+          if (same && !selfColl[v]) continue;
+          if (same && (nbrHas(v, t[0]) || nbrHas(v, t[1]) || nbrHas(v, t[2]))) continue;
+          V<T> bary;
+          V<T> q = closestTri(p, pos[t[0]], pos[t[1]], pos[t[2]], bary);
+          V<T> dv = p - q;
+          if (dotv(dv, dv) > q2) continue;
+          acc.push_back(e);
+        }
+      }
+      emit(0u, v, acc);
+    }
+    datVT = datTotal;
+    for (uint32_t gid = 0; gid < edges.size(); gid++) {
+      auto a = edges[gid];
+      V<T> a0 = pos[a.first], a1 = pos[a.second];
+      if (!finite3(a0) || !finite3(a1)) { datNonfinite++; continue; }
+      auto vmin = V<T>(std::min(a0.x, a1.x), std::min(a0.y, a1.y), std::min(a0.z, a1.z));
+      auto vmax = V<T>(std::max(a0.x, a1.x), std::max(a0.y, a1.y), std::max(a0.z, a1.z));
+      Cell ac0 = elemCell(vmin), raw = elemCell(vmax);
+      if (raw.x - ac0.x > 3 || raw.y - ac0.y > 3 || raw.z - ac0.z > 3) datOverflow++;
+      Cell ac1 = clampCell(raw, ac0);
+      std::vector<uint32_t> acc;
+      for (int z = ac0.z; z <= ac1.z; z++)
+        for (int y = ac0.y; y <= ac1.y; y++)
+          for (int x = ac0.x; x <= ac1.x; x++) {
+            auto it = grid.find(Cell{x, y, z});
+            if (it == grid.end()) continue;
+            for (uint32_t e : it->second) {
+              if (e < nTris) continue;
+              uint32_t other = e - nTris;
+              if (other <= gid) continue;
+              auto b = edges[other];
+              if (b.first == a.first || b.first == a.second || b.second == a.first || b.second == a.second) continue;
+              uint32_t g = clothGroup[a.first];
+              bool same = g != 0 && clothGroup[a.second] == g && clothGroup[b.first] == g && clothGroup[b.second] == g;
+              if (same && !selfColl[a.first]) continue;
+              if (same && (nbrHas(a.first, b.first) || nbrHas(a.first, b.second) || nbrHas(a.second, b.first) ||
+                           nbrHas(a.second, b.second)))
+                continue;
+              V<T> blo, bhi;
+              elemBounds(nTris + other, blo, bhi);
+              Cell bc0 = elemCell(blo), bc1 = clampCell(elemCell(bhi), bc0);
+              Cell o0{std::max(ac0.x, bc0.x), std::max(ac0.y, bc0.y), std::max(ac0.z, bc0.z)};
+              Cell o1{std::min(ac1.x, bc1.x), std::min(ac1.y, bc1.y), std::min(ac1.z, bc1.z)};
+              if (o0.x > o1.x || o0.y > o1.y || o0.z > o1.z) continue;
+              if (x != o0.x || y != o0.y || z != o0.z) continue;
+              T s_, t_;
+              V<T> b0 = pos[b.first], b1 = pos[b.second];
+              segSeg(a0, a1, b0, b1, s_, t_);
+              V<T> dv = (a0 + (a1 - a0) * s_) - (b0 + (b1 - b0) * t_);
+              if (dotv(dv, dv) > q2) continue;
+              acc.push_back(other);
+            }
+          }
+      emit(1u, gid, acc);
+    }
+  }
+
+  bool datContact(const std::array<uint32_t, 3>& pr, Soft& out) const {
+    const T big = std::numeric_limits<T>::max();
+    const std::vector<V<T>>& ref = initLin;
+    T rq = (T)datQueryRadius, em = (T)elemMargin;
+    if (pr[0] == 0) {
+      uint32_t v = pr[1], ti = pr[2];
+      const auto& t = tris[ti];
+      V<T> p = pos[v], a = pos[t[0]], b = pos[t[1]], c = pos[t[2]];
+      V<T> bary;
+      V<T> q = closestTri(p, a, b, c, bary);
+      V<T> dv = p - q;
+      T dist2 = dotv(dv, dv);
+      T rv = std::fabs(radiusSigned[v]);
+      T rt = std::max(std::fabs(radiusSigned[t[0]]), std::max(std::fabs(radiusSigned[t[1]]), std::fabs(radiusSigned[t[2]])));
+      T hSum = rv + rt;
+      V<T> velT = (vel[t[0]] + vel[t[1]] + vel[t[2]]) / T(3);
+      T inflate = std::min(lenv(vel[v] - velT) * (T)set.dt, T(0.3) * (T)surfaceContactCellSize);
+      T detect = std::min(hSum + em + inflate, rq);
+      if (dist2 > detect * detect) return false;
+      T dist = std::sqrt(dist2);
+      V<T> triN = crossv(b - a, c - a);
+      T nl = lenv(triN);
+      if (nl < T(1e-8)) return false;
+      triN = triN / nl;
+      T side = dotv(dv, triN) >= 0 ? T(1) : T(-1);
+      uint32_t keyA = (1u << 28) | v, keyB = ti;
+      bool interior = bary.x > T(0.02) && bary.y > T(0.02) && bary.z > T(0.02);
+      T rem = interior ? prevSide(keyA, keyB, side) : side;
+      V<T> n;
+      T gap;
+      if (dist > T(1e-7) && side == rem) { n = dv / dist; gap = dist; }
+      else { n = triN * rem; gap = -dist; }
+      T fricT = friction[t[0]] * bary.x + friction[t[1]] * bary.y + friction[t[2]] * bary.z;
+      T fric = std::sqrt(std::max(friction[v] * fricT, T(0)));
+      T minMass = mass[v] > 0 ? mass[v] : big;
+      for (int j = 0; j < 3; j++) if (mass[t[j]] > 0) minMass = std::min(minMass, mass[t[j]]);
+      if (minMass == big) return false;
+      V<T> dq = (p - ref[v]) - (a - ref[t[0]]) * bary.x - (b - ref[t[1]]) * bary.y - (c - ref[t[2]]) * bary.z;
+      T C0n = (gap - hSum + em) - dotv(n, dq);
+      uint32_t ids[4] = {v, t[0], t[1], t[2]};
+      T w[4] = {T(1), -bary.x, -bary.y, -bary.z};
+      out = makeSoft(ids, w, n, fric, V<T>(), C0n, hSum - em, minMass, 1u, false, false, rem < 0, keyA, keyB);
+      return true;
+    }
+    uint32_t ea = pr[1], eb = pr[2];
+    auto ai = edges[ea], bi = edges[eb];
+    V<T> a0 = pos[ai.first], a1 = pos[ai.second], b0 = pos[bi.first], b1 = pos[bi.second];
+    T s_, t_;
+    segSeg(a0, a1, b0, b1, s_, t_);
+    const T eps = T(0.02);
+    if (s_ < eps || s_ > T(1) - eps || t_ < eps || t_ > T(1) - eps) return false;
+    V<T> dv = (a0 + (a1 - a0) * s_) - (b0 + (b1 - b0) * t_);
+    T dist = lenv(dv);
+    T rA = std::max(std::fabs(radiusSigned[ai.first]), std::fabs(radiusSigned[ai.second]));
+    T rB = std::max(std::fabs(radiusSigned[bi.first]), std::fabs(radiusSigned[bi.second]));
+    T hSum = rA + rB;
+    V<T> va = (vel[ai.first] + vel[ai.second]) * T(0.5), vb = (vel[bi.first] + vel[bi.second]) * T(0.5);
+    T inflate = std::min(lenv(va - vb) * (T)set.dt, T(0.3) * (T)surfaceContactCellSize);
+    T detect = std::min(hSum + em + inflate, rq);
+    if (dist > detect) return false;
+    uint32_t keyA = (3u << 28) | ea, keyB = eb;
+    int prev = softFind(keyA, keyB);
+    V<T> n;
+    T gap = dist;
+    if (std::isfinite(dist) && dist >= T(1e-7)) {
+      V<T> nGeo = dv / dist;
+      n = nGeo;
+      if (prev >= 0 && dotv(nGeo, prevSoft[prev].n) < 0) { n = -nGeo; gap = -dist; }
+    } else if (prev >= 0 && finite3(prevSoft[prev].n)) {
+      n = prevSoft[prev].n;
+      gap = 0;
+    } else {
+      V<T> ua = a1 - a0, ub = b1 - b0, rel = va - vb;
+      V<T> raw = crossv(ua, ub);
+      T mag = lenv(raw);
+      if (std::isfinite(mag) && mag >= T(1e-8)) {
+        n = raw / mag;
+      } else {
+        T la = lenv(ua), lb = lenv(ub);
+        V<T> u = la >= lb ? ua : ub;
+        T ul = std::max(la, lb);
+        if (!std::isfinite(ul) || ul < T(1e-8)) return false;
+        u = u / ul;
+        V<T> perp = rel - u * dotv(u, rel);
+        T pl = lenv(perp);
+        if (!std::isfinite(pl) || pl < T(1e-8)) {
+          V<T> axis = std::fabs(u.x) < T(0.7) ? V<T>(1, 0, 0) : V<T>(0, 1, 0);
+          perp = crossv(u, axis);
+          pl = lenv(perp);
+        }
+        if (!std::isfinite(pl) || pl < T(1e-8)) return false;
+        n = perp / pl;
+      }
+      if (dotv(n, rel) > 0) n = -n;
+      if (!finite3(n)) return false;
+      gap = 0;
+    }
+    T fric = std::sqrt(std::max((friction[ai.first] + friction[ai.second]) * T(0.5) *
+                                (friction[bi.first] + friction[bi.second]) * T(0.5), T(0)));
+    T minMass = big;
+    uint32_t ids[4] = {ai.first, ai.second, bi.first, bi.second};
+    for (int j = 0; j < 4; j++) if (mass[ids[j]] > 0) minMass = std::min(minMass, mass[ids[j]]);
+    if (minMass == big) return false;
+    V<T> dq = (a0 - ref[ai.first]) * (T(1) - s_) + (a1 - ref[ai.second]) * s_ - (b0 - ref[bi.first]) * (T(1) - t_) -
+              (b1 - ref[bi.second]) * t_;
+    T C0n = (gap - hSum + em) - dotv(n, dq);
+    T w[4] = {T(1) - s_, s_, -(T(1) - t_), -t_};
+    out = makeSoft(ids, w, n, fric, V<T>(), C0n, hSum - em, minMass, 3u, false, false, false, keyA, keyB);
+    return true;
+  }
+
+  void datContacts() {
+    for (const auto& pr : datPairs) {
+      Soft r;
+      if (!datContact(pr, r)) continue;
+      softCandidates++;
+      if (soft.size() < maxSoft) soft.push_back(r);
+    }
+  }
+
+  // OGC contact row (40_solver.metal:1237-1300); `snap` reads same-color partners from the snapshot
+  V<T> softC(const Soft& sc, uint32_t self, bool snap, V<T>& n, V<T>& t1, V<T>& t2, V<T>& rAW) const {
+    bool rigidA = sc.flags & 1u, roundA = sc.flags & 32u;
+    n = sc.n;
+    orthoBasis(n, t1, t2);
+    rAW = V<T>();
+    V<T> C = V<T>(sc.C0, 0, 0) * (T(1) - (T)set.alpha);
+    for (int k = 0; k < 4; k++) {
+      uint32_t b = sc.ids[k];
+      if (b == WORLD) continue;
+      V<T> p = snap ? pL(b, self) : pos[b];
+      V<T> dq = p - initLin[b];
+      C = C + V<T>(dotv(n, dq), dotv(t1, dq), dotv(t2, dq)) * sc.w[k];
+      if (k == 0 && rigidA) {
+        Q<T> q = snap ? pA(b, self) : rot[b];
+        rAW = roundA ? sc.anchor : qrot(q, sc.anchor);
+        V<T> dqa = qsub(q, initAng[b]);
+        C = C + V<T>(dotv(crossv(rAW, n), dqa), dotv(crossv(rAW, t1), dqa), dotv(crossv(rAW, t2), dqa)) * sc.w[k];
+      }
+    }
+    return C;
+  }
+  V<T> softF(const Soft& sc, const V<T>& C, T& kN) const {
+    V<T> F = mulv(sc.penalty, C) + sc.lambda;
+    kN = sc.penalty.x;
+    T rT = sc.anchor.x;
+    uint32_t kind = (sc.flags >> 2) & 3u;
+    if (rT > 0 && kind != 2u && C.x < T(-0.5) * rT) {
+      T dsep = std::max(C.x + rT, T(0.02) * rT);
+      T kp = T(0.25) * rT * rT * sc.penalty.x;
+      F.x = -kp / dsep + sc.lambda.x;
+      kN = kp / (dsep * dsep);
+    }
+    F.x = std::min(F.x, T(0));
+    T bnd = std::fabs(F.x) * sc.friction, fs = std::sqrt(F.y * F.y + F.z * F.z);
+    if (fs > bnd && fs > 0) { F.y = F.y * (bnd / fs); F.z = F.z * (bnd / fs); }
+    return F;
+  }
+  void stampSoft(const Soft& sc, uint32_t self, Sys& s) const {
+    V<T> n, t1, t2, rAW;
+    V<T> C = softC(sc, self, true, n, t1, t2, rAW);
+    T kN;
+    V<T> F = softF(sc, C, kN);
+    int slot = -1;
+    for (int k = 0; k < 4; k++) if (sc.ids[k] == self) { slot = k; break; }
+    if (slot < 0) return;
+    T w = sc.w[slot];
+    T K[3] = {kN, sc.penalty.y, sc.penalty.z};
+    V<T> rows[3] = {n * w, t1 * w, t2 * w};
+    for (int r = 0; r < 3; r++) {
+      s.ll = s.ll + M<T>::outer(rows[r], rows[r]) * K[r];
+      s.gl = s.gl + rows[r] * F[r];
+      if (slot == 0 && (sc.flags & 1u)) {
+        V<T> ja = crossv(rAW, rows[r]);
+        s.aa = s.aa + M<T>::outer(ja, ja) * K[r];
+        s.cr = s.cr + M<T>::outer(ja, rows[r]) * K[r];
+        s.ga = s.ga + ja * F[r];
+      }
+    }
+  }
+  // StVK membrane (40_solver.metal:1088-1130)
+  void stampMembrane(const Membrane& mb, uint32_t self, Sys& s) const {
+    V<T> x0 = pL(mb.ids[0], self), x1 = pL(mb.ids[1], self), x2 = pL(mb.ids[2], self);
+    T a = mb.dm[0], b = mb.dm[1], c = mb.dm[2], d = mb.dm[3];
+    V<T> d0 = x1 - x0, d1 = x2 - x0;
+    V<T> f1 = d0 * a + d1 * b, f2 = d0 * c + d1 * d;
+    T E11 = T(0.5) * (dotv(f1, f1) - T(1)), E22 = T(0.5) * (dotv(f2, f2) - T(1)), E12 = T(0.5) * dotv(f1, f2);
+    T trE = E11 + E22;
+    T S11 = T(2) * mb.muA * E11 + mb.lamA * trE, S22 = T(2) * mb.muA * E22 + mb.lamA * trE, S12 = T(2) * mb.muA * E12;
+    V<T> p1 = f1 * S11 + f2 * S12, p2 = f1 * S12 + f2 * S22;
+    T wx, wy;
+    if (self == mb.ids[1]) { wx = a; wy = c; }
+    else if (self == mb.ids[2]) { wx = b; wy = d; }
+    else { wx = -a - b; wy = -c - d; }
+    s.gl = s.gl + p1 * wx + p2 * wy;
+    V<T> g11 = f1 * wx, g22 = f2 * wy, g12 = (f2 * wx + f1 * wy) * T(0.5), gtr = g11 + g22;
+    s.ll = s.ll + M<T>::outer(g11, g11) * (T(2) * mb.muA) + M<T>::outer(g22, g22) * (T(2) * mb.muA) +
+           M<T>::outer(g12, g12) * (T(4) * mb.muA) + M<T>::outer(gtr, gtr) * mb.lamA;
+  }
+  // Bergou hinge (40_solver.metal:1136-1150), evaluated translation-free:
+  // m = sum_{o != self} K_o (x_o - x_self), equal to sum_i K_i x_i since K is
+  // orthogonal to the constants
+  void stampBend(const Bend& bd, uint32_t self, Sys& s) const {
+    V<T> xs = pos[self], m;
+    T k = 0;
+    for (int i = 0; i < 4; i++) {
+      if (bd.ids[i] == self) { k = bd.K[i]; continue; }
+      m = m + (pL(bd.ids[i], self) - xs) * bd.K[i];
+    }
+    s.gl = s.gl + m * (bd.s * k);
+    s.ll = s.ll + M<T>::diag(V<T>(1, 1, 1) * (bd.s * k * k));
+  }
+  // element contact dual: quadratic stage, mass-scaled cap, penalties <= 1e6 (40_solver.metal:1689-1727)
+  void dualSoft(Soft& sc) {
+    V<T> n, t1, t2, rAW;
+    V<T> C = softC(sc, 0, false, n, t1, t2, rAW);
+    V<T> F = mulv(sc.penalty, C) + sc.lambda;
+    F.x = std::min(F.x, T(0));
+    T bnd = std::fabs(F.x) * sc.friction, fs = std::sqrt(F.y * F.y + F.z * F.z);
+    if (fs > bnd && fs > 0) { F.y = F.y * (bnd / fs); F.z = F.z * (bnd / fs); }
+    F.x = std::max(F.x, -sc.lamCap);
+    sc.lambda = F;
+    T bl = set.beta_lin;
+    if (F.x < 0) sc.penalty.x = std::min(sc.penalty.x + bl * std::fabs(C.x), T(1e6));
+    if (fs <= bnd) {
+      sc.penalty.y = std::min(sc.penalty.y + bl * std::fabs(C.y), T(1e6));
+      sc.penalty.z = std::min(sc.penalty.z + bl * std::fabs(C.z), T(1e6));
+    }
+  }
+
   // contact adjacency of this frame: per body, active manifold indices in order
   std::vector<std::vector<uint32_t>> contactAdj;
 
@@ -1262,8 +2010,14 @@ struct Oracle : oracle_base {
       else if (k == FKS) stampSpring(springs[i], body, s);
     }
     for (uint32_t mi : contactAdj[body]) stampManifold(manifolds[mi], body, s);
-    for (uint32_t e : staticAdj[body])
-      if ((e >> 28) == FKT) stampTet(tets[e & 0x0FFFFFFFu], body, s);
+    for (uint32_t e : staticAdj[body]) {
+      uint32_t k = e >> 28, i = e & 0x0FFFFFFFu;
+      if (k == FKT) stampTet(tets[i], body, s);
+      else if (k == FKMEM) stampMembrane(membranes[i], body, s);
+      else if (k == FKBEND) stampBend(bends[i], body, s);
+    }
+    if (!softAdj.empty())
+      for (uint32_t si : softAdj[body]) stampSoft(soft[si], body, s);
     return s;
   }
 
@@ -1364,6 +2118,7 @@ struct Oracle : oracle_base {
         m.c[i].penalty = p;
       }
     }
+    for (auto& sc : soft) dualSoft(sc);
     snapLin = pos;
     snapAng = rot;
   }
@@ -1435,13 +2190,41 @@ struct Oracle : oracle_base {
   // -------------------------------------------------------------------------
   // the step (GPUSolver.swift:3138-4405 order)
   // -------------------------------------------------------------------------
+  // residual-driven frames (tolerance > 0): see stepToResidual
+  double tolerance = 0, lastResidual = 0;
+  int maxIterations = 200, checkEvery = 5, itersUsed = 0;
+  void sweep() {
+    for (int c = 0; c < 64; c++)
+      for (uint32_t b : buckets[c]) primal(b);
+    dual();
+  }
+  void stepToResidual(double tol, int maxIt, int every) {
+    tolerance = tol;
+    maxIterations = maxIt;
+    checkEvery = every;
+    step();
+    tolerance = 0;
+  }
+
   void step() {
     frame++;
     broadphase();
     manifolds.assign(pairs.size(), Manifold<T>{});
     for (size_t i = 0; i < pairs.size(); i++) collide((uint32_t)i, manifolds[i]);
+    soft.clear();
+    softCandidates = softRT = 0;
+    datOverflow = datNonfinite = 0;
+    if (nTris) rtEmit();
     warmstartConstraints();
     warmstartBodies();
+    if (nTris) {
+      datBuild();
+      datContacts();
+    }
+    softAdj.assign(nb, {});
+    for (uint32_t i = 0; i < soft.size(); i++)
+      for (int k = 0; k < 4; k++)
+        if (dyn(soft[i].ids[k])) softAdj[soft[i].ids[k]].push_back(i);
     contactAdj.assign(nb, {});
     activeManifolds = 0;
     for (uint32_t i = 0; i < manifolds.size(); i++) {
@@ -1452,10 +2235,18 @@ struct Oracle : oracle_base {
       if (dyn(m.b)) contactAdj[m.b].push_back(i);
     }
     if (dynamicColoring) dynamicColors();
-    for (int it = 0; it < set.iterations; it++) {
-      for (int c = 0; c < 64; c++)
-        for (uint32_t b : buckets[c]) primal(b);
-      dual();
+    if (tolerance > 0) {
+      // residual-driven: chunks of checkEvery sweeps until r <= tolerance
+      itersUsed = 0;
+      while (itersUsed < maxIterations) {
+        int n = std::min(checkEvery, maxIterations - itersUsed);
+        for (int it = 0; it < n; it++) sweep();
+        itersUsed += n;
+        lastResidual = residual();
+        if (lastResidual <= tolerance) break;
+      }
+    } else {
+      for (int it = 0; it < set.iterations; it++) sweep();
     }
     // BDF1 finalize (40_solver.metal:2108-2150)
     T dt = set.dt;
@@ -1477,7 +2268,23 @@ struct Oracle : oracle_base {
       vel[i] = v;
       angVel[i] = om;
     }
-    // persistence map of this frame for the next one
+    // 1-ring velocity viscosity of surface particles (40_solver.metal:2154-2177)
+    if (nTris && set.cloth_viscosity > 0) {
+      std::vector<V<T>> v0 = vel;
+      T wv = set.cloth_viscosity;
+      for (uint32_t v : particles) {
+        uint32_t n = nbrCount[v];
+        if (n == 0) continue;
+        V<T> avg;
+        for (uint32_t k = nbrStart[v]; k < nbrStart[v] + n; k++) avg = avg + v0[nbrList[k]];
+        avg = avg / (T)n;
+        vel[v] = v0[v] + (avg - v0[v]) * wv;
+      }
+    }
+    // persistence maps of this frame for the next one
+    prevSoft = soft;
+    softMap.clear();
+    for (uint32_t i = 0; i < prevSoft.size(); i++) softMap.emplace(std::make_pair(prevSoft[i].keyA, prevSoft[i].keyB), (int)i);
     prevManifolds = manifolds;
     prevMap.clear();
     for (uint32_t i = 0; i < prevManifolds.size(); i++)
@@ -1541,7 +2348,6 @@ extern "C" {
 
 int oracle_create(const avbd_scene_desc* scene, int precision, int maxPairsPerBody, oracle_solver** out) {
   if (!scene || !out) return 1;
-  if (scene->num_tris) return 4;
   oracle_solver* o = new oracle_solver();
   o->precision = precision == 64 ? 64 : 32;
   int rc;
@@ -1564,6 +2370,16 @@ int oracle_step(oracle_solver* o) {
   return 0;
 }
 
+int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, int check_every, int* iterations,
+                            double* residual) {
+  auto go = [&](auto* s) {
+    s->stepToResidual(tol, max_iterations, check_every);
+    if (iterations) *iterations = s->itersUsed;
+    if (residual) *residual = s->lastResidual;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
 int oracle_set_settings(oracle_solver* o, const avbd_settings* s) {
   if (o->f) o->f->set = *s; else o->d->set = *s;
   return 0;
@@ -1640,6 +2456,98 @@ int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* 
       if (n3) { n3[3 * i] = (float)ms[i].nrm.x; n3[3 * i + 1] = (float)ms[i].nrm.y; n3[3 * i + 2] = (float)ms[i].nrm.z; }
       if (lp) { lp[2 * i] = ms[i].n ? (float)ms[i].c[0].lambda.x : 0.0f; lp[2 * i + 1] = ms[i].n ? (float)ms[i].c[0].penalty.x : 0.0f; }
     }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles) {
+  auto go = [&](auto* s) {
+    if (info) {
+      info->num_tris = s->nTris;
+      info->num_edges = (uint32_t)s->edges.size();
+      info->num_particles = (uint32_t)s->particles.size();
+      info->num_membranes = (uint32_t)s->membranes.size();
+      info->num_bends = (uint32_t)s->bends.size();
+      info->elem_hash_size = 0;
+      info->max_soft = s->maxSoft;
+      info->soft_map_capacity = 0;
+      info->max_dat_pairs = s->maxDatPairs;
+      info->elem_cell_size = s->elemCellSize;
+      info->elem_margin = s->elemMargin;
+      info->dat_query_radius = s->datQueryRadius;
+      info->surface_contact_cell_size = s->surfaceContactCellSize;
+    }
+    if (edges)
+      for (size_t i = 0; i < s->edges.size(); i++) { edges[2 * i] = s->edges[i].first; edges[2 * i + 1] = s->edges[i].second; }
+    if (particles)
+      for (size_t i = 0; i < s->particles.size(); i++) particles[i] = s->particles[i];
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_get_dat_pairs(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* pairs) {
+  auto go = [&](auto* s) {
+    *n = (uint32_t)s->datPairs.size();
+    for (uint32_t i = 0; i < *n && i < cap && pairs; i++)
+      for (int k = 0; k < 3; k++) pairs[3 * i + k] = s->datPairs[i][k];
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+// records in the backend's 112-byte SoftContactGPU layout (floats, ids and keys as bits)
+int oracle_get_soft_contacts(oracle_solver* o, uint32_t cap, uint32_t* n, void* recs) {
+  auto go = [&](auto* s) {
+    *n = (uint32_t)s->prevSoft.size();
+    if (!recs) return;
+    for (uint32_t i = 0; i < *n && i < cap; i++) {
+      const auto& r = s->prevSoft[i];
+      uint32_t* u = (uint32_t*)recs + 28 * i;
+      float* f = (float*)u;
+      for (int k = 0; k < 4; k++) u[k] = r.ids[k];
+      f[4] = (float)r.n.x; f[5] = (float)r.n.y; f[6] = (float)r.n.z; f[7] = (float)r.friction;
+      f[8] = (float)r.anchor.x; f[9] = (float)r.anchor.y; f[10] = (float)r.anchor.z;
+      memcpy(&f[11], &r.flags, 4);
+      for (int k = 0; k < 4; k++) f[12 + k] = (float)r.w[k];
+      f[16] = (float)r.C0; f[17] = 0.0f; f[18] = 0.0f; f[19] = (float)r.lamCap;
+      f[20] = (float)r.lambda.x; f[21] = (float)r.lambda.y; f[22] = (float)r.lambda.z;
+      memcpy(&f[23], &r.keyA, 4);
+      f[24] = (float)r.penalty.x; f[25] = (float)r.penalty.y; f[26] = (float)r.penalty.z;
+      memcpy(&f[27], &r.keyB, 4);
+    }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+// the pair stream the element grid yields for the given positions (the
+// solver state is left unchanged)
+int oracle_dat_pairs_at(oracle_solver* o, const float* pos3, uint32_t cap, uint32_t* n, uint32_t* pairs) {
+  auto go = [&](auto* s) {
+    auto saved = s->pos;
+    auto savedPairs = s->datPairs;
+    for (uint32_t i = 0; i < s->nb; i++) {
+      s->pos[i].x = pos3[3 * i];
+      s->pos[i].y = pos3[3 * i + 1];
+      s->pos[i].z = pos3[3 * i + 2];
+    }
+    s->datBuild();
+    *n = (uint32_t)s->datPairs.size();
+    for (uint32_t i = 0; i < *n && i < cap && pairs; i++)
+      for (int k = 0; k < 3; k++) pairs[3 * i + k] = s->datPairs[i][k];
+    s->pos = saved;
+    s->datPairs = savedPairs;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_cloth_stats(oracle_solver* o, int32_t* datPairs, int32_t* datVT, int32_t* softCandidates, int32_t* softRT,
+                       int32_t* overflow, int32_t* nonfinite) {
+  auto go = [&](auto* s) {
+    if (datPairs) *datPairs = (int32_t)s->datTotal;
+    if (datVT) *datVT = (int32_t)s->datVT;
+    if (softCandidates) *softCandidates = (int32_t)s->softCandidates;
+    if (softRT) *softRT = (int32_t)s->softRT;
+    if (overflow) *overflow = (int32_t)s->datOverflow;
+    if (nonfinite) *nonfinite = (int32_t)s->datNonfinite;
   };
   if (o->f) go(o->f); else go(o->d);
   return 0;
@@ -1726,6 +2634,58 @@ void oracle_solve6x6(const double* A36, const double* b6, double* x6, int precis
     solve6(A, b, x);
     for (int i = 0; i < 6; i++) x6[i] = x[i];
   }
+}
+
+// StVK membrane in double (40_solver.metal:1088-1130): positions x[9]
+// (vertices 0..2), inverse rest shape dm[4] column major, mu A, lambda A,
+// self slot; energy mu A |E|^2 + lambda A / 2 tr(E)^2, gradient and the
+// Gauss-Newton block on self
+void oracle_membrane(const double* x, const double* dm, double muA, double lamA, int self, double* energy,
+                     double* grad3, double* hess9) {
+  V<double> x0(x[0], x[1], x[2]), x1(x[3], x[4], x[5]), x2(x[6], x[7], x[8]);
+  double a = dm[0], b = dm[1], c = dm[2], d = dm[3];
+  V<double> d0 = x1 - x0, d1 = x2 - x0, f1 = d0 * a + d1 * b, f2 = d0 * c + d1 * d;
+  double E11 = 0.5 * (dotv(f1, f1) - 1), E22 = 0.5 * (dotv(f2, f2) - 1), E12 = 0.5 * dotv(f1, f2), tr = E11 + E22;
+  if (energy) *energy = muA * (E11 * E11 + E22 * E22 + 2 * E12 * E12) + 0.5 * lamA * tr * tr;
+  double S11 = 2 * muA * E11 + lamA * tr, S22 = 2 * muA * E22 + lamA * tr, S12 = 2 * muA * E12;
+  V<double> p1 = f1 * S11 + f2 * S12, p2 = f1 * S12 + f2 * S22;
+  double wx = self == 1 ? a : self == 2 ? b : -a - b, wy = self == 1 ? c : self == 2 ? d : -c - d;
+  V<double> g = p1 * wx + p2 * wy;
+  if (grad3) { grad3[0] = g.x; grad3[1] = g.y; grad3[2] = g.z; }
+  V<double> g11 = f1 * wx, g22 = f2 * wy, g12 = (f2 * wx + f1 * wy) * 0.5, gt = g11 + g22;
+  M<double> H = M<double>::outer(g11, g11) * (2 * muA) + M<double>::outer(g22, g22) * (2 * muA) +
+                M<double>::outer(g12, g12) * (4 * muA) + M<double>::outer(gt, gt) * lamA;
+  if (hess9) for (int i = 0; i < 9; i++) hess9[i] = H.a[i / 3][i % 3];
+}
+
+// Hinge constants exactly as the upload derives them: a two-triangle scene
+// (x[12] = edge ends v0, v1 then the opposite vertices), bends k1, k2;
+// returns K[4] (scaled to the cotangent convention) and the stiffness s,
+// or 0 when the upload would skip the hinge
+int oracle_hinge(const float* x, float k1, float k2, float* K4, float* s_out) {
+  avbd_scene_desc d;
+  memset(&d, 0, sizeof(d));
+  d.abi_version = AVBD_ABI_VERSION;
+  d.settings.dt = 1.0f / 60.0f;
+  float size[12], dens[4] = {1, 1, 1, 1}, fr[4] = {0.5f, 0.5f, 0.5f, 0.5f};
+  uint32_t shape[4] = {AVBD_SHAPE_SPHERE, AVBD_SHAPE_SPHERE, AVBD_SHAPE_SPHERE, AVBD_SHAPE_SPHERE};
+  uint8_t part[4] = {1, 1, 1, 1};
+  for (int i = 0; i < 12; i++) size[i] = 0.002f;
+  uint32_t tri[6] = {0, 1, 2, 0, 1, 3};
+  float mu[2] = {1, 1}, bend[2] = {k1, k2};
+  d.num_bodies = 4; d.body_size = size; d.body_density = dens; d.body_friction = fr; d.body_position = x;
+  d.body_shape = shape; d.body_is_particle = part;
+  d.num_tris = 2; d.tri_ids = tri; d.tri_mu = mu; d.tri_lambda = mu; d.tri_bend = bend;
+  Oracle<float> o;
+  o.set = d.settings;
+  o.nb = 4;
+  o.particle.assign(4, 1);
+  o.clothGroup.assign(4, 0);
+  o.clothInit(&d);
+  if (o.bends.empty()) return 0;
+  for (int k = 0; k < 4; k++) K4[k] = o.bends[0].K[k];
+  *s_out = o.bends[0].s;
+  return 1;
 }
 
 uint32_t oracle_cell_hash(float px, float py, float pz, float cs, uint32_t group, uint32_t hs) {

@@ -21,6 +21,9 @@ typedef struct oracle_solver oracle_solver;
 int oracle_create(const avbd_scene_desc* scene, int precision, int max_pairs_per_body, oracle_solver** out);
 void oracle_destroy(oracle_solver* o);
 int oracle_step(oracle_solver* o);
+/* one frame iterated until the residual <= tol (chunks of check_every sweeps, at most max_iterations) */
+int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, int check_every, int* iterations,
+                            double* residual);
 int oracle_set_settings(oracle_solver* o, const avbd_settings* s);
 int oracle_get_body_states(oracle_solver* o, uint32_t n, float* pos3, float* rot4, float* vel3, float* angvel3);
 int oracle_get_body_states_f64(oracle_solver* o, uint32_t n, double* pos3, double* rot4, double* vel3,
@@ -38,6 +41,14 @@ int oracle_residual(oracle_solver* o, double* out);
 int oracle_stats(oracle_solver* o, int32_t* num_pairs, int32_t* active_manifolds, int32_t* num_colors,
                  int32_t* pair_candidates, int32_t* color_conflicts);
 int oracle_map_find(oracle_solver* o, const uint32_t* pairs, uint32_t n, int32_t* out);
+/* cloth: derived topology / parameters, the V-T / E-E pair stream (kind, owner, other),
+ * element contacts of the last frame (112-byte records as the backend's) and counts */
+int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
+int oracle_get_dat_pairs(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* pairs3);
+int oracle_get_soft_contacts(oracle_solver* o, uint32_t capacity, uint32_t* n, void* records);
+int oracle_dat_pairs_at(oracle_solver* o, const float* pos3, uint32_t capacity, uint32_t* n, uint32_t* pairs3);
+int oracle_cloth_stats(oracle_solver* o, int32_t* dat_pairs, int32_t* dat_vt, int32_t* soft_candidates,
+                       int32_t* soft_rt, int32_t* overflow, int32_t* nonfinite);
 
 /* unit restatements for pinning */
 void oracle_splitmix64(uint64_t seed, uint32_t n, uint64_t* out_u64, float* out_float);
@@ -46,6 +57,9 @@ void oracle_splitmix64(uint64_t seed, uint32_t n, uint64_t* out_u64, float* out_
 void oracle_snh_tet(const double* x, const double* dminv, double mu, double lam, double vol, int self, double* energy,
                     double* grad3, double* hess9);
 void oracle_solve6x6(const double* A36, const double* b6, double* x6, int precision);
+void oracle_membrane(const double* x9, const double* dm4, double muA, double lamA, int self, double* energy,
+                     double* grad3, double* hess9);
+int oracle_hinge(const float* x12, float k1, float k2, float* K4, float* s);
 uint32_t oracle_cell_hash(float px, float py, float pz, float cell_size, uint32_t group, uint32_t hash_size);
 
 #ifdef __cplusplus
