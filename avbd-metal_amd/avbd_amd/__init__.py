@@ -7,5 +7,6 @@ GPUSolver requires it.
 """
 from .scene import BOX, CAPSULE, SPHERE, TORUS, PhysicsScene, SceneJoint, SimSettings  # noqa: F401
 from .solver import GPUSolver, version  # noqa: F401
+from .learned import WarmStartMLP  # noqa: F401
 from . import demos  # noqa: F401
 from ._capi import AVBDError, RuntimeFailure, LIB_PATH  # noqa: F401

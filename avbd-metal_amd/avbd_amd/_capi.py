@@ -90,7 +90,8 @@ class Stats(C.Structure):
                 ("color_conflicts", C.c_int32), ("uses_dynamic_coloring", C.c_int32),
                 ("lanes_per_body", C.c_int32), ("sweep_mode", C.c_int32),
                 ("dat_pairs", C.c_int32), ("dat_vt_pairs", C.c_int32), ("dat_ee_pairs", C.c_int32),
-                ("num_soft_rt", C.c_int32)]
+                ("num_soft_rt", C.c_int32), ("ws_accepted", C.c_int32),
+                ("ws_residual_plain", C.c_float), ("ws_residual_learned", C.c_float)]
 
 
 class ClothInfo(C.Structure):
@@ -133,8 +134,27 @@ SIGNATURES = {
     "avbd_get_profile": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_uint32, FP, U32P]),
     "avbd_time_primal": (C.c_int, [C.c_void_p, C.c_int32, FP, C.POINTER(C.c_double), I32P]),
     "avbd_test_deny_stage": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "avbd_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "avbd_mlp_destroy": (None, [C.c_void_p]),
+    "avbd_mlp_get_params": (C.c_int, [C.c_void_p, FP]),
+    "avbd_mlp_set_params": (C.c_int, [C.c_void_p, FP]),
+    "avbd_mlp_forward": (C.c_int, [C.c_void_p, FP, C.c_uint32, FP]),
+    "avbd_mlp_loss_grad": (C.c_int, [C.c_void_p, FP, FP, C.c_uint32, FP, FP]),
+    "avbd_mlp_train": (C.c_int, [C.c_void_p, FP, FP, C.c_uint32, C.c_int32, C.c_uint32, C.c_float, C.c_uint64,
+                                 FP]),
+    "avbd_mlp_time_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, FP]),
+    "avbd_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "avbd_get_warm_start_samples": (C.c_int, [C.c_void_p, C.c_uint32, U32P, FP, FP, U32P]),
     "avbd_version": (C.c_char_p, []),
 }
+
+# learned warm start (avbd.h)
+MLP_INPUTS = 24
+MLP_OUTPUTS = 3
+MLP_PARAMS = 20099
+WS_LEARNED = 1
+WS_LOG = 2
+WS_FORCE = 4
 
 _lib = None
 

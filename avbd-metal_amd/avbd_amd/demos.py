@@ -193,18 +193,20 @@ def batched_scene(num_scenes=4096, pitch=3.0, iterations=10, stack_range=(4, 64)
     return s, ranges
 
 
-def cloth_scene(n=256, size=2.0, height=1.0, jitter=1.0e-3, seed=3, radius=2.5e-3, areal_density=0.3,
-                stretch_k=1.0e4, membrane_mu=300.0, bend=1.0e-3, sphere_radius=0.5, sphere_z=0.0, iterations=30,
-                friction=0.6, particle_damping=1.5, viscosity=0.25, ground=True):
+def cloth_scene(n=256, size=2.0, height=1.0, jitter=1.0e-3, seed=3, radius=2.5e-3, areal_density=0.5,
+                stretch_k=1.0e4, hard_rods=True, membrane_mu=300.0, bend=1.0e-3, sphere_radius=0.5, sphere_z=0.45,
+                iterations=30, friction=0.6, particle_damping=1.5, viscosity=0.25, ground=True):
     """Cloth sheet dropped onto a fixed sphere (grid layout of
     Demos7.swift:16-80 addClothGrid, drape settings of Demos7.swift:265-300):
     an n x n particle grid of side `size` at z = height + U(-jitter, jitter)
-    (SplitMix64(seed), row-major), soft structural springs of stiffness
-    `stretch_k` on the grid edges, two triangles per quad with alternating
-    diagonals carrying StVK membranes (mu = lambda = membrane_mu) and Bergou
-    bending `bend`, per-node mass areal_density x spacing^2, particle radius
-    `radius` (contact radius 2 r).  The static sphere is centred at
-    (0, 0, sphere_z); the static ground's top touches its bottom."""
+    (SplitMix64(seed), row-major); structural grid edges as addClothGrid
+    builds them -- hard augmented-Lagrangian rods whose penalty is capped at
+    `stretch_k` (soft springs of stiffness `stretch_k` with hard_rods=False);
+    two triangles per quad with alternating diagonals carrying StVK
+    membranes (mu = lambda = membrane_mu) and Bergou bending `bend`; per-node
+    mass areal_density x spacing^2; particle radius `radius` (contact radius
+    2 r).  The static sphere is centred at (0, 0, sphere_z) just below the
+    sheet; the static ground's top touches the sphere's bottom."""
     s = PhysicsScene("cloth", SimSettings(iterations=iterations, particleDamping=particle_damping,
                                           clothViscosity=viscosity))
     if ground:
@@ -221,9 +223,9 @@ def cloth_scene(n=256, size=2.0, height=1.0, jitter=1.0e-3, seed=3, radius=2.5e-
     for i in range(n):
         for j in range(n):
             if i + 1 < n:
-                s.addSpring(int(grid[i, j]), int(grid[i + 1, j]), (0, 0, 0), (0, 0, 0), stretch_k)
+                s.addSpring(int(grid[i, j]), int(grid[i + 1, j]), (0, 0, 0), (0, 0, 0), stretch_k, hard=hard_rods)
             if j + 1 < n:
-                s.addSpring(int(grid[i, j]), int(grid[i, j + 1]), (0, 0, 0), (0, 0, 0), stretch_k)
+                s.addSpring(int(grid[i, j]), int(grid[i, j + 1]), (0, 0, 0), (0, 0, 0), stretch_k, hard=hard_rods)
     for i in range(n - 1):
         for j in range(n - 1):
             v00, v10, v01, v11 = (int(grid[i, j]), int(grid[i + 1, j]), int(grid[i, j + 1]), int(grid[i + 1, j + 1]))

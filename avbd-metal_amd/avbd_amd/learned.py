@@ -1,0 +1,107 @@
+"""WarmStartMLP: the learned warm start's network over the C ABI (avbd.h).
+
+A per-particle MLP 24 -> 128 -> 128 -> 3 (ReLU, BF16 weights and
+activations, FP32 accumulation) whose forward, backward and Adam steps run as
+hand-written gfx950 kernels.  It predicts a step's converged correction
+relative to the adaptive warm-start guess in units of |g| dt^2; attach it with
+GPUSolver.setWarmStart.  The reference has no learned warm start.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+
+NUM_PARAMS = _capi.MLP_PARAMS
+INPUTS = _capi.MLP_INPUTS
+OUTPUTS = _capi.MLP_OUTPUTS
+# offsets of the flat parameter array: W1[24][128], b1, W2[128][128], b2, W3[128][3], b3 (W[in][out])
+LAYOUT = (("W1", (INPUTS, 128)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)), ("W3", (128, OUTPUTS)),
+          ("b3", (OUTPUTS,)))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _rows(a, width):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != width:
+        raise ValueError(f"expected an [n, {width}] array, got {a.shape}")
+    return a
+
+
+def split_params(p):
+    """Flat parameter array -> dict of named arrays (views)."""
+    out, o = {}, 0
+    for name, shape in LAYOUT:
+        n = int(np.prod(shape))
+        out[name] = p[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+class WarmStartMLP:
+    def __init__(self, seed=0):
+        self._lib = _capi.load()
+        h = C.c_void_p()
+        st = self._lib.avbd_mlp_create(C.c_uint64(seed), C.byref(h))
+        if st != _capi.OK:
+            raise _capi.AVBDError(st, "avbd_mlp_create")
+        self._h = h
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.avbd_mlp_destroy(self._h)
+            self._h = None
+
+    def _check(self, st, what):
+        if st != _capi.OK:
+            raise _capi.AVBDError(st, what)
+
+    @property
+    def params(self):
+        p = np.zeros(NUM_PARAMS, np.float32)
+        self._check(self._lib.avbd_mlp_get_params(self._h, _fp(p)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (NUM_PARAMS,):
+            raise ValueError(f"expected {NUM_PARAMS} parameters")
+        self._check(self._lib.avbd_mlp_set_params(self._h, _fp(p)), "set_params")
+
+    def forward(self, x):
+        x = _rows(x, INPUTS)
+        y = np.zeros((len(x), OUTPUTS), np.float32)
+        self._check(self._lib.avbd_mlp_forward(self._h, _fp(x), len(x), _fp(y)), "forward")
+        return y
+
+    def lossGrad(self, x, t):
+        """(loss, grad) of sum |y - t|^2 / (3 n) over one batch of rows."""
+        x, t = _rows(x, INPUTS), _rows(t, OUTPUTS)
+        loss = C.c_float()
+        g = np.zeros(NUM_PARAMS, np.float32)
+        self._check(self._lib.avbd_mlp_loss_grad(self._h, _fp(x), _fp(t), len(x), C.byref(loss), _fp(g)),
+                    "loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=0):
+        """Adam over shuffled batches on the GPU; returns the mean loss per epoch."""
+        x, t = _rows(x, INPUTS), _rows(t, OUTPUTS)
+        if len(x) != len(t):
+            raise ValueError("features and targets differ in length")
+        losses = np.zeros(int(epochs), np.float32)
+        self._check(self._lib.avbd_mlp_train(self._h, _fp(x), _fp(t), len(x), int(epochs), int(batch), float(lr),
+                                             C.c_uint64(seed), _fp(losses)), "train")
+        return losses
+
+    def timeForward(self, n, reps=20):
+        """Average microseconds of one forward launch over n rows."""
+        us = C.c_float()
+        self._check(self._lib.avbd_mlp_time_forward(self._h, int(n), int(reps), C.byref(us)), "time_forward")
+        return us.value

@@ -281,6 +281,33 @@ class GPUSolver:
                     "timePrimal")
         return dict(avg_launch_us=us.value, sweep_bytes=by.value, launches_per_sweep=la.value)
 
+    # -- learned warm start (no reference counterpart) ------------------------
+    def setWarmStart(self, mlp=None, learned=True, log=False, force=False):
+        """Attach a WarmStartMLP: learned=True replaces the adaptive guess of
+        the dynamic particles with guess + |g| dt^2 * MLP(features), keeping
+        the plain guess whenever the learned start's residual is higher
+        (force=True keeps it unconditionally); log=True records each step's
+        features and converged corrections (warmStartSamples).  No arguments
+        detach."""
+        mode = (_capi.WS_LEARNED if (learned and mlp is not None) else 0) | (_capi.WS_LOG if log else 0)
+        if mode & _capi.WS_LEARNED and force:
+            mode |= _capi.WS_FORCE
+        h = mlp._h if (mlp is not None and mode & _capi.WS_LEARNED) else None
+        self._check(self._lib.avbd_set_warm_start(self._h, h, mode), "setWarmStart")
+        self._warm_mlp = mlp if h else None   # keeps the network alive while attached
+
+    def warmStartSamples(self):
+        """(features [n,24], corrections [n,3], body ids [n]) of the last frame."""
+        n = C.c_uint32()
+        self._check(self._lib.avbd_get_warm_start_samples(self._h, 0, C.byref(n), None, None, None),
+                    "warmStartSamples")
+        x = np.zeros((n.value, _capi.MLP_INPUTS), np.float32)
+        t = np.zeros((n.value, _capi.MLP_OUTPUTS), np.float32)
+        idx = np.zeros(n.value, np.uint32)
+        self._check(self._lib.avbd_get_warm_start_samples(self._h, n.value, C.byref(n), _fp(x), _fp(t), _u32p(idx)),
+                    "warmStartSamples")
+        return x, t, idx
+
     def setProfiling(self, on=True):
         self._check(self._lib.avbd_get_profile(self._h, 1 if on else 0, None, 0, None, None), "profile")
 

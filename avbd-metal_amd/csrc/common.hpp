@@ -70,6 +70,9 @@ enum Counter : uint32_t {
   CTR_COLOR_BASE = 16,     // kMaxColors entries
   CTR_DAT_VT = 16 + 2 * kMaxColors,
   CTR_DAT_EE,
+  CTR_WS_ACCEPT,           // learned warm start kept this frame (1) or fell back (0)
+  CTR_WS_RES_PLAIN,        // residual at the plain start (f32 bits)
+  CTR_WS_RES_LEARNED,      // residual at the learned start (f32 bits)
   CTR_TOTAL = 16 + 2 * kMaxColors + 8
 };
 

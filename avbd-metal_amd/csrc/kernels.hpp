@@ -41,8 +41,10 @@ void launch_dynamic_coloring(hipStream_t st, const DeviceState& S, const StepPar
 constexpr int kModeGeneral = 0;  // rigid bodies and particles, any entries
 constexpr int kModeTet = 1;      // particles whose static entries are all tets
 constexpr int kModeParticle = 2; // particles only (cloth slots, springs, contacts)
+// colorCount 0xFFFFFFFF: device-sized color of the dynamic palette; otherwise
+// the color's `colorCount` work records start at colorWork[base]
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
-                         uint32_t colorCount, int lanes, int mode);
+                         uint32_t colorCount, uint32_t base, int lanes, int mode);
 // colors >= first of a device-sized palette, one workgroup, in order
 void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t first, int lanes);
 constexpr uint32_t kDynamicColorLaunches = 8;  // dynamic palette: colors launched one kernel each

@@ -1,0 +1,145 @@
+// learned.hip -- the learned warm-start stage of a step.  After the
+// adaptive guess (predict.hip) and the contact adjacency, each dynamic
+// particle's features go through the MLP (mlp.hip), whose output replaces
+// the guess: x_start = x_guess + L y with L = |g| dt^2.  The residual
+// r = |g| / |m g_grav| is evaluated at both starts and the plain guess is
+// kept whenever the learned start's residual is higher (or not finite).
+// After the iterations, (x - x_guess) / L is the step's converged
+// correction: the next step's feature and, when logging, the training target.
+//
+// Features (24, f32, in units of L; v_n, v_{n-1} velocities, x_n the
+// step-start position, x~ the inertial target, c the previous correction,
+// <.> the mean over the topological 1-ring):
+//   0-2  v_n dt / L               3-5   (v_n - v_{n-1}) dt / L
+//   6-8  (x~ - x_n) / L           9-11  (x_guess - x_n) / L
+//   12-14 c                       15-17 (<v_j> - v_n) dt / L
+//   18-20 <c_j> - c               21    min(contact entries, 8) / 8
+//   22   1-ring size / 16         23    |c|
+// The reference has no learned warm start; this stage is the backend's own.
+#include "kernels.hpp"
+#include "mlp.hpp"
+
+namespace avbd {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__global__ void __launch_bounds__(256) k_ws_features(DeviceState S, StepParams P, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  const uint32_t b = W.idx[i];
+  const float invL = 1.0f / W.L, vs = P.dt * invL;
+  const F4 g4 = S.posLin[b];
+  const V3 x = xyz(S.initLin[b]), v = xyz(S.velLin[b]), c = xyz(W.corr[b]);
+  V3 vSum = v3s(0.0f), cSum = v3s(0.0f);
+  const uint32_t rs = W.ringStart[i], rc = W.ringCount[i];
+  for (uint32_t k = 0; k < rc; k++) {
+    const uint32_t j = W.ringList[rs + k];
+    vSum += xyz(S.velLin[j]);
+    cSum += xyz(W.corr[j]);
+  }
+  const float inv = rc ? 1.0f / (float)rc : 0.0f;
+  const V3 f0 = v * vs;
+  const V3 f1 = (v - xyz(S.prevVelLin[b])) * vs;
+  const V3 f2 = (xyz(S.inertLin[b]) - x) * invL;
+  const V3 f3 = (xyz(g4) - x) * invL;
+  const V3 f5 = rc ? (vSum * inv - v) * vs : v3s(0.0f);
+  const V3 f6 = rc ? cSum * inv - c : v3s(0.0f);
+  const float f[kMlpIn] = {f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, f2.x, f2.y, f2.z, f3.x, f3.y, f3.z,
+                           c.x,  c.y,  c.z,  f5.x, f5.y, f5.z, f6.x, f6.y, f6.z,
+                           fminf((float)S.dAdjCount[b], 8.0f) * 0.125f, (float)rc * 0.0625f, length(c)};
+  bf16x8 row[4];
+#pragma unroll
+  for (int k = 0; k < 32; k++) row[k / 8][k % 8] = (bf16)(k < kMlpIn ? f[k] : 0.0f);
+#pragma unroll
+  for (int q = 0; q < 4; q++) *(bf16x8*)(W.x + (size_t)i * kMlpInPad + 8 * q) = row[q];
+  if (W.log) {
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) W.logX[(size_t)i * kMlpIn + k] = f[k];
+  }
+  W.guess[i] = g4;
+}
+
+// sums of the residual partials: |g|^2 into res[slot], |m g_grav|^2 into res[2]
+__global__ void __launch_bounds__(256) k_ws_reduce(const float* partial, uint32_t blocks, float* res, int slot) {
+  __shared__ float red[2][256];
+  float g2 = 0.0f, w2 = 0.0f;
+  for (uint32_t b = threadIdx.x; b < blocks; b += 256) {
+    g2 += partial[2 * b];
+    w2 += partial[2 * b + 1];
+  }
+  red[0][threadIdx.x] = g2;
+  red[1][threadIdx.x] = w2;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[slot] = red[0][0];
+    res[2] = red[1][0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ws_apply(DeviceState S, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  const uint32_t b = W.idx[i];
+  const F4 g = W.guess[i];
+  const V3 y = v3(W.y[3 * i], W.y[3 * i + 1], W.y[3 * i + 2]);
+  const F4 p = f4(xyz(g) + y * W.L, g.w);
+  S.posLin[b] = p;
+  S.posLinSnap[b] = p;
+}
+
+__global__ void __launch_bounds__(256) k_ws_select(DeviceState S, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float plain = W.res[0], learned = W.res[1];
+  const bool keep = W.force || learned <= plain;  // false for a non-finite learned residual
+  if (i == 0) {
+    const float w2 = W.res[2] > 0.0f ? W.res[2] : 1.0f;
+    S.counters[CTR_WS_ACCEPT] = keep ? 1u : 0u;
+    S.counters[CTR_WS_RES_PLAIN] = as_uint(sqrtf(plain / w2));
+    S.counters[CTR_WS_RES_LEARNED] = as_uint(sqrtf(learned / w2));
+  }
+  if (i >= W.n || keep) return;
+  const uint32_t b = W.idx[i];
+  S.posLin[b] = W.guess[i];
+  S.posLinSnap[b] = W.guess[i];
+}
+
+__global__ void __launch_bounds__(256) k_ws_target(DeviceState S, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  const uint32_t b = W.idx[i];
+  const V3 t = (xyz(S.posLin[b]) - xyz(W.guess[i])) * (1.0f / W.L);
+  W.corr[b] = f4(t, 0.0f);
+  if (W.log) {
+    W.logT[3 * i] = t.x;
+    W.logT[3 * i + 1] = t.y;
+    W.logT[3 * i + 2] = t.z;
+  }
+}
+
+static dim3 grid_n(uint32_t n) { return dim3(n ? (n + 255) / 256 : 1); }
+
+void launch_ws_features(hipStream_t st, const DeviceState& S, const StepParams& P, const WarmState& W) {
+  if (W.n) hipLaunchKernelGGL(k_ws_features, grid_n(W.n), dim3(256), 0, st, S, P, W);
+}
+void launch_ws_reduce(hipStream_t st, const DeviceState& S, uint32_t blocks, const WarmState& W, int slot) {
+  hipLaunchKernelGGL(k_ws_reduce, dim3(1), dim3(256), 0, st, (const float*)S.residualPartial, blocks, W.res, slot);
+}
+void launch_ws_apply(hipStream_t st, const DeviceState& S, const WarmState& W) {
+  if (W.n) hipLaunchKernelGGL(k_ws_apply, grid_n(W.n), dim3(256), 0, st, S, W);
+}
+void launch_ws_select(hipStream_t st, const DeviceState& S, const WarmState& W) {
+  hipLaunchKernelGGL(k_ws_select, grid_n(W.n), dim3(256), 0, st, S, W);
+}
+void launch_ws_target(hipStream_t st, const DeviceState& S, const WarmState& W) {
+  if (W.n) hipLaunchKernelGGL(k_ws_target, grid_n(W.n), dim3(256), 0, st, S, W);
+}
+
+}  // namespace avbd

@@ -425,9 +425,10 @@ __device__ __forceinline__ V3 group_sum3(V3 v) {
 // solves and writes.  Must be reached by whole waves (EXEC full) because of
 // the ballot and DPP reductions; inactive groups pass active = false.
 template <int L, int MODE>
-__device__ __forceinline__ void primal_group(const DeviceState& S, const StepParams& P, bool active, uint32_t body,
+__device__ __forceinline__ void primal_group(const DeviceState& S, const StepParams& P, bool active, U4 work,
                                              uint32_t lane, uint32_t color) {
   constexpr bool SOFT = MODE == kModeTet;
+  const uint32_t body = work.x;
   bool rigid = MODE == kModeGeneral && active && S.shape[body].w >= 0.0f;
   const bool anyRigid = MODE == kModeGeneral && __ballot(rigid) != 0ull;
   Acc acc;
@@ -435,7 +436,7 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   if (active) {
     PoseView pv{S.posLin, S.posAng, S.posLinSnap, S.posAngSnap, true};
     const V3 xs = xyz(S.posLin[body]);
-    const uint32_t ss = S.sAdjStart[body], sc = S.sAdjCount[body];
+    const uint32_t ss = work.y, sc = work.z;
     const uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
     if (SOFT) {
       // particle colors: static entries are tets, per-frame entries manifolds.
@@ -542,6 +543,11 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   if (rigid) S.posAng[body] = q_addw(S.posAng[body], dxA);
 }
 
+// (body, static CSR start, static CSR count) of a body of a device-built palette
+__device__ __forceinline__ U4 body_work(const DeviceState& S, bool active, uint32_t body) {
+  return active ? U4{body, S.sAdjStart[body], S.sAdjCount[body], 0u} : U4{0u, 0u, 0u, 0u};
+}
+
 // XCD-aware block order: the hardware deals consecutive workgroups to the
 // 8 XCDs round-robin; remapping gives each XCD one contiguous slice of the
 // color list so neighbour gathers of adjacent bodies share that XCD's L2.
@@ -553,16 +559,26 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
 }
 
 template <int L, int MODE>
-__global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count) {
+__global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count,
+                                                uint32_t base) {
   const uint32_t groupsPerBlock = 256 / L;
   uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   uint32_t slot = blk * groupsPerBlock + threadIdx.x / L;
   uint32_t lane = threadIdx.x & (L - 1);
-  uint32_t s = S.colorStart[color];
-  if (count == 0xFFFFFFFFu) count = S.colorStart[color + 1] - s;  // device-sized (dynamic palette)
-  bool active = slot < count;
-  uint32_t body = active ? S.colorList[s + slot] : 0u;
-  primal_group<L, MODE>(S, P, active, body, lane, color);
+  U4 work;
+  bool active;
+  if (count == 0xFFFFFFFFu) {
+    // device-sized (dynamic palette): color list, then the body's CSR range
+    uint32_t s = S.colorStart[color];
+    count = S.colorStart[color + 1] - s;
+    active = slot < count;
+    work = body_work(S, active, active ? S.colorList[s + slot] : 0u);
+  } else {
+    // static palette: one load of the precomputed (body, CSR start, count)
+    active = slot < count;
+    work = active ? S.colorWork[base + slot] : U4{0u, 0u, 0u, 0u};
+  }
+  primal_group<L, MODE>(S, P, active, work, lane, color);
 }
 
 // Colors [first, 64) of a device-sized palette in one workgroup, in order,
@@ -580,7 +596,7 @@ __global__ void __launch_bounds__(256) k_primal_tail(DeviceState S, StepParams P
     for (uint32_t base = 0; base < count; base += groupsPerBlock) {
       uint32_t slot = base + grp;
       bool active = slot < count;
-      primal_group<L, kModeGeneral>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+      primal_group<L, kModeGeneral>(S, P, active, body_work(S, active, active ? S.colorList[s + slot] : 0u), lane, c);
     }
     __threadfence_block();
     __syncthreads();
@@ -598,26 +614,30 @@ void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& 
 
 template <int MODE>
 static void launch_primal_t(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color,
-                            uint32_t count, int lanes, uint32_t blocks) {
+                            uint32_t count, uint32_t base, int lanes, uint32_t blocks) {
   switch (lanes) {
-    case 4: hipLaunchKernelGGL((k_primal<4, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 8: hipLaunchKernelGGL((k_primal<8, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    case 16: hipLaunchKernelGGL((k_primal<16, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
-    default: hipLaunchKernelGGL((k_primal<32, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count); break;
+    case 4: hipLaunchKernelGGL((k_primal<4, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count, base); break;
+    case 8: hipLaunchKernelGGL((k_primal<8, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count, base); break;
+    case 16:
+      hipLaunchKernelGGL((k_primal<16, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count, base);
+      break;
+    default:
+      hipLaunchKernelGGL((k_primal<32, MODE>), dim3(blocks), dim3(256), 0, st, S, P, color, count, base);
+      break;
   }
 }
 
 void launch_primal_color(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t count,
-                         int lanes, int mode) {
+                         uint32_t base, int lanes, int mode) {
   if (count == 0) return;
   uint32_t groupsPerBlock = 256 / lanes;
   uint32_t n = count == 0xFFFFFFFFu ? P.numBodies : count;
   uint32_t blocks = (n + groupsPerBlock - 1) / groupsPerBlock;
   // the XCD remap needs the padded slot range to cover every body
   blocks = (blocks + 7) / 8 * 8;
-  if (mode == kModeTet) launch_primal_t<kModeTet>(st, S, P, color, count, lanes, blocks);
-  else if (mode == kModeParticle) launch_primal_t<kModeParticle>(st, S, P, color, count, lanes, blocks);
-  else launch_primal_t<kModeGeneral>(st, S, P, color, count, lanes, blocks);
+  if (mode == kModeTet) launch_primal_t<kModeTet>(st, S, P, color, count, base, lanes, blocks);
+  else if (mode == kModeParticle) launch_primal_t<kModeParticle>(st, S, P, color, count, base, lanes, blocks);
+  else launch_primal_t<kModeGeneral>(st, S, P, color, count, base, lanes, blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -932,7 +952,10 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
       for (uint32_t r = 0; r < rounds; r++) {
         uint32_t slot = r * ngroups + gslot;
         bool active = slot < count;
-        primal_group<L, MODE>(S, P, active, active ? S.colorList[s + slot] : 0u, lane, c);
+        primal_group<L, MODE>(S, P, active,
+                              numColors ? (active ? S.colorWork[s + slot] : U4{0u, 0u, 0u, 0u})
+                                        : body_work(S, active, active ? S.colorList[s + slot] : 0u),
+                              lane, c);
       }
       epoch += gridDim.x;
       grid_barrier(bar, fail, epoch);

@@ -18,6 +18,7 @@
 
 #include "../../include/avbd.h"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 // upload-time geometry must round exactly as written (radii feed the
 // hashed/global partition and the cell size)
@@ -51,6 +52,7 @@ struct avbd_solver {
   bool dynamicColoring = false;
   uint32_t staticUsedColors = 1;
   std::vector<uint32_t> staticColorCounts;  // per color
+  std::vector<uint32_t> staticColorStarts;  // per color: first position in the color order
   std::vector<uint8_t> colorMode;           // per color: primal specialisation (kMode*)
   std::vector<ClothSlot> clothSlotsHost;    // static cloth slots without same-color bits
   bool hasClothSlots = false;               // membrane / bend / particle-spring entries exist
@@ -101,6 +103,14 @@ struct avbd_solver {
   uint32_t devScratchWords = 0;
   float* residualPartialHost = nullptr;
   uint32_t residualBlocks = 0;
+  uint32_t cus = 256;
+  // learned warm start
+  avbd_mlp* mlp = nullptr;
+  int wsMode = 0;
+  bool wsBuilt = false;
+  WarmState W{};
+  std::vector<U4> tetIdsHost;
+  std::vector<U2> springEndsHost;
 };
 
 namespace {
@@ -193,6 +203,16 @@ StepParams make_params(const avbd_solver* s) {
   return P;
 }
 
+// the learned stage's per-frame parameters (L follows the current settings)
+WarmState ws_params(const avbd_solver* s) {
+  WarmState W = s->W;
+  const float dt = s->settings.dt;
+  W.L = fmaxf(fabsf(s->settings.gravity) * dt * dt, 1.0e-6f);
+  W.force = (s->wsMode & AVBD_WS_FORCE) ? 1 : 0;
+  W.log = (s->wsMode & AVBD_WS_LOG) ? 1 : 0;
+  return W;
+}
+
 bool stage_denied(const avbd_solver* s, const char* stage, int64_t frame) {
   return !s->deniedStage.empty() && s->deniedStage == stage && (s->deniedFrame < 0 || s->deniedFrame == frame);
 }
@@ -229,6 +249,22 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   stage("adjacency");
   if (s->dynamicColoring) launch_dynamic_coloring(st, S, P);
   stage("coloring");
+  // learned warm start: features, then (with a network) the learned start
+  // and the residual comparison against the plain guess
+  if (s->wsMode && s->W.n) {
+    const WarmState W = ws_params(s);
+    launch_ws_features(st, S, P, W);
+    if ((s->wsMode & AVBD_WS_LEARNED) && s->mlp) {
+      launch_residual(st, S, P, s->residualBlocks);
+      launch_ws_reduce(st, S, s->residualBlocks, W, 0);
+      launch_mlp_forward(st, s->mlp->M, W.x, W.n, W.y, s->cus);
+      launch_ws_apply(st, S, W);
+      launch_residual(st, S, P, s->residualBlocks);
+      launch_ws_reduce(st, S, s->residualBlocks, W, 1);
+      launch_ws_select(st, S, W);
+    }
+    stage("learned-start");
+  }
 }
 
 // `iterations` sweeps (every color, then the dual); the persistent mode runs
@@ -247,11 +283,11 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
   for (uint32_t it = 0; it < iterations; it++) {
     if (s->dynamicColoring) {
       for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
-        launch_primal_color(st, S, P, c, 0xFFFFFFFFu, s->lanes, kModeGeneral);
+        launch_primal_color(st, S, P, c, 0xFFFFFFFFu, 0u, s->lanes, kModeGeneral);
       launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
     } else {
       for (uint32_t c = 0; c < s->staticUsedColors; c++)
-        launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->lanes, s->colorMode[c]);
+        launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes, s->colorMode[c]);
     }
     launch_dual(st, S, P, s->gridCap);
   }
@@ -259,6 +295,7 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
 }
 
 void enqueue_post_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
+  if (s->wsMode && s->W.n) launch_ws_target(s->stream, S, ws_params(s));
   launch_finalize(s->stream, S, P);
   launch_viscosity(s->stream, S, P);
   stage("finalize");
@@ -327,6 +364,9 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
   s->stats.num_active_manifolds = (int32_t)c[CTR_ACTIVE_MANIFOLDS];
   s->stats.dat_pairs = (int32_t)c[CTR_DAT_PAIRS];
   s->stats.dat_vt_pairs = (int32_t)c[CTR_DAT_VT];
+  s->stats.ws_accepted = (int32_t)c[CTR_WS_ACCEPT];
+  s->stats.ws_residual_plain = as_float(c[CTR_WS_RES_PLAIN]);
+  s->stats.ws_residual_learned = as_float(c[CTR_WS_RES_LEARNED]);
   s->stats.dat_ee_pairs = (int32_t)c[CTR_DAT_EE];
   s->stats.num_soft_rt = (int32_t)c[CTR_SOFT_RT];
   s->stats.color_conflicts = (int32_t)c[CTR_COLOR_CONFLICTS];
@@ -990,6 +1030,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     g.rA = f4(rA, d->spring_stiffness[i]);
     g.rB = f4(rB, rest);
     sr[i] = g;
+    s->springEndsHost.push_back(U2{a, b});
   }
 
   // ---- tets (GPUSolver.swift:1214-1236): Dm^-1 rows, 6V, mu V, lambda V
@@ -1016,6 +1057,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     double vol = fabs(det) / 6.0;
     TetRec g;
     g.ids = U4{id[0], id[1], id[2], id[3]};
+    s->tetIdsHost.push_back(g.ids);
     g.r0 = F4{(float)inv[0][0], (float)inv[0][1], (float)inv[0][2], (float)det};
     g.r1 = F4{(float)inv[1][0], (float)inv[1][1], (float)inv[1][2], (float)(d->tet_mu[t] * vol)};
     g.r2 = F4{(float)inv[2][0], (float)inv[2][1], (float)inv[2][2], (float)(d->tet_lambda[t] * vol)};
@@ -1245,6 +1287,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   S.dAdjCapacity = 2 * s->maxPairs + 4 * s->maxSoft;
   ALLOC(S.dAdjList, S.dAdjCapacity)
   ALLOC(S.colors, nb) ALLOC(S.colorsB, nb) ALLOC(S.colorList, nb) ALLOC(S.colorStart, kMaxColors + 2)
+  ALLOC(S.colorWork, nb)
   uint32_t bb = (nb + 255) / 256;
   ALLOC(S.bodySlot, std::max<uint32_t>(nb, 2 * kMaxColors * bb))
   ALLOC(S.colorFlags, 32) ALLOC(S.counters, CTR_TOTAL)
@@ -1315,6 +1358,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   s->gridCap = (uint32_t)std::max(256, 8 * cus);
+  s->cus = (uint32_t)std::max(1, cus);
   return AVBD_OK;
 }
 
@@ -1368,6 +1412,16 @@ avbd_status static_coloring(avbd_solver* s) {
   cstart.push_back((uint32_t)clist.size());
   cstart.push_back(s->staticUsedColors);
   if (upload(S.colorStart, cstart) || upload(S.colorList, clist)) return AVBD_ERR_HIP;
+  // per-color work records: one load gives the body and its static CSR range
+  {
+    std::vector<uint32_t> st(s->nb);
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < s->nb; b++) { st[b] = run; run += s->sAdjCount[b]; }
+    std::vector<U4> work(clist.size());
+    for (size_t i = 0; i < clist.size(); i++) work[i] = U4{clist[i], st[clist[i]], s->sAdjCount[clist[i]], 0u};
+    if (upload(S.colorWork, work)) return AVBD_ERR_HIP;
+  }
+  s->staticColorStarts.assign(cstart.begin(), cstart.begin() + kMaxColors);
   s->staticColorCounts.assign(kMaxColors, 0);
   s->colorMode.assign(kMaxColors, kModeGeneral);
   std::vector<uint32_t> sStart(s->nb);
@@ -1499,11 +1553,11 @@ void autotune(avbd_solver* s) {
         for (uint32_t it = 0; it < P.iterations; it++) {
           if (s->dynamicColoring) {
             for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
-              launch_primal_color(st, S, P, c, 0xFFFFFFFFu, L, kModeGeneral);
+              launch_primal_color(st, S, P, c, 0xFFFFFFFFu, 0u, L, kModeGeneral);
             launch_primal_tail(st, S, P, kDynamicColorLaunches, L);
           } else {
             for (uint32_t c = 0; c < s->staticUsedColors; c++)
-              launch_primal_color(st, S, P, c, s->staticColorCounts[c], L, s->colorMode[c]);
+              launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], L, s->colorMode[c]);
           }
           launch_dual(st, S, P, s->gridCap);
         }
@@ -1839,6 +1893,8 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
   uint32_t total = std::max(std::max(s->nj, s->ns), s->maxPairs);
   hipLaunchKernelGGL(k_reset_incident, dim3((total + 255) / 256), dim3(256), 0, s->stream, s->S, P, dFlag, dPen,
                      dPen + s->nj, s->S.prevManifolds, s->maxPairs);
+  // written states invalidate the learned stage's correction history
+  if (s->wsBuilt) hipMemsetAsync(s->W.corr, 0, 16 * (size_t)s->nb, s->stream);
   hipError_t e = hipStreamSynchronize(s->stream);
   hipFree(dIdx);
   hipFree(dFlag);
@@ -2145,7 +2201,8 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   hipEventRecord(e0, s->stream);
   for (int r = 0; r < reps; r++)
     for (uint32_t c = 0; c < s->staticUsedColors; c++)
-      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->lanes, s->colorMode[c]);
+      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes,
+                          s->colorMode[c]);
   hipEventRecord(e1, s->stream);
   hipError_t he = hipEventSynchronize(e1);
   float ms = 0.0f;
@@ -2157,6 +2214,103 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   if (avg_launch_us) *avg_launch_us = ms * 1000.0f / (float)(reps * launches);
   if (sweep_bytes) *sweep_bytes = bytes;
   if (launches_per_sweep) *launches_per_sweep = launches;
+  return AVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned warm start (no reference counterpart)
+// ---------------------------------------------------------------------------
+namespace {
+// dynamic particles in index order and their topological 1-ring: every body
+// sharing a tet, membrane, bend or spring with the particle
+avbd_status build_warm_state(avbd_solver* s) {
+  std::vector<uint32_t> idx;
+  for (uint32_t i = 0; i < s->nb; i++)
+    if (s->isParticle[i] && s->mass[i] > 0.0f) idx.push_back(i);
+  const uint32_t n = (uint32_t)idx.size();
+  std::vector<uint32_t> rStart(n), rCount(n), rList;
+  std::vector<uint32_t> nb;
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t b = idx[k];
+    nb.clear();
+    const uint32_t start = s->sAdjStartHost[b], cnt = s->sAdjCount[b];
+    for (uint32_t q = 0; q < cnt; q++) {
+      const uint32_t e = s->sAdjListHost[start + q], kind = e >> kAdjShift, id = e & kAdjIndexMask;
+      if (kind == FK_TET) {
+        const U4 t = s->tetIdsHost[id];
+        nb.insert(nb.end(), {t.x, t.y, t.z, t.w});
+      } else if (kind == FK_SPRING || kind == FK_PSPRING) {
+        nb.push_back(s->springEndsHost[id].x);
+        nb.push_back(s->springEndsHost[id].y);
+      } else if (kind == FK_MEMBRANE) {
+        const ClothSlot& c = s->clothSlotsHost[start + q];
+        nb.push_back(c.u.x & ~kSnapBit);
+        nb.push_back(c.u.y & ~kSnapBit);
+      } else if (kind == FK_BEND) {
+        const ClothSlot& c = s->clothSlotsHost[start + q];
+        nb.push_back(c.u.x & ~kSnapBit);
+        nb.push_back(c.u.y & ~kSnapBit);
+        nb.push_back(c.u.z & ~kSnapBit);
+      }
+    }
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    rStart[k] = (uint32_t)rList.size();
+    for (uint32_t j : nb)
+      if (j != b && j < s->nb) rList.push_back(j);
+    rCount[k] = (uint32_t)rList.size() - rStart[k];
+  }
+  WarmState W{};
+  W.n = n;
+  hipError_t e;
+  __bf16* xb = nullptr;
+  const size_t m = std::max<uint32_t>(n, 1);
+  if ((e = dev_alloc(s, &W.idx, m)) || (e = dev_alloc(s, &W.ringStart, m)) || (e = dev_alloc(s, &W.ringCount, m)) ||
+      (e = dev_alloc(s, &W.ringList, std::max<size_t>(rList.size(), 1))) || (e = dev_alloc(s, &W.guess, m)) ||
+      (e = dev_alloc(s, &W.corr, s->nb)) || (e = dev_alloc(s, &xb, m * kMlpInPad)) ||
+      (e = dev_alloc(s, &W.y, m * kMlpOut)) || (e = dev_alloc(s, &W.logX, m * kMlpIn)) ||
+      (e = dev_alloc(s, &W.logT, m * kMlpOut)) || (e = dev_alloc(s, &W.res, 4)))
+    return AVBD_ERR_ALLOC;
+  W.x = xb;
+  if (upload(W.idx, idx) || upload(W.ringStart, rStart) || upload(W.ringCount, rCount) || upload(W.ringList, rList))
+    return AVBD_ERR_HIP;
+  s->W = W;
+  s->wsBuilt = true;
+  return AVBD_OK;
+}
+}  // namespace
+
+extern "C" avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t mode) {
+  if (!s || mode < 0 || mode > (AVBD_WS_LEARNED | AVBD_WS_LOG | AVBD_WS_FORCE)) return AVBD_ERR_INVALID_ARGUMENT;
+  if ((mode & AVBD_WS_LEARNED) && !m) return AVBD_ERR_INVALID_ARGUMENT;
+  if (mode == AVBD_WS_FORCE) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  if (mode && !s->wsBuilt) {
+    st = build_warm_state(s);
+    if (st != AVBD_OK) return st;
+  }
+  // a fresh attachment starts without correction history
+  if (mode && !s->wsMode && hipMemset(s->W.corr, 0, 16 * (size_t)s->nb) != hipSuccess) return AVBD_ERR_HIP;
+  s->wsMode = mode;
+  s->mlp = mode ? m : nullptr;
+  invalidate_graphs(s);
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_warm_start_samples(avbd_solver* s, uint32_t capacity, uint32_t* n, float* x, float* t,
+                                                   uint32_t* idx) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!(s->wsMode & AVBD_WS_LOG)) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  *n = s->W.n;
+  const size_t c = std::min(capacity, s->W.n);
+  if (c == 0) return AVBD_OK;
+  if ((x && hipMemcpy(x, s->W.logX, c * kMlpIn * 4, hipMemcpyDeviceToHost)) ||
+      (t && hipMemcpy(t, s->W.logT, c * kMlpOut * 4, hipMemcpyDeviceToHost)) ||
+      (idx && hipMemcpy(idx, s->W.idx, c * 4, hipMemcpyDeviceToHost)))
+    return AVBD_ERR_HIP;
   return AVBD_OK;
 }
 

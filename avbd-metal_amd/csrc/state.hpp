@@ -50,6 +50,7 @@ struct DeviceState {
   uint32_t *dAdjCount, *dAdjStart, *dAdjCursor, *dAdjList;
   // colors
   uint32_t *colors, *colorsB, *colorList, *colorStart, *bodySlot;
+  U4* colorWork;              // static palette, color order: (body, static CSR start, count, 0)
   // start-of-iteration pose snapshot: contact partners that share the
   // body's color are read from here, so a same-color contact is a Jacobi
   // coupling with a schedule-independent result

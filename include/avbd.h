@@ -258,6 +258,9 @@ typedef struct avbd_stats {
     int32_t dat_vt_pairs;          /* lastPlanarDATVertexTrianglePairs */
     int32_t dat_ee_pairs;          /* lastPlanarDATEdgeEdgePairs */
     int32_t num_soft_rt;           /* rigid-feature vs triangle records (prefix of the soft array) */
+    int32_t ws_accepted;           /* learned warm start kept (1) or fell back to the plain guess (0) */
+    float ws_residual_plain;       /* residual at the plain adaptive guess (learned stage active) */
+    float ws_residual_learned;     /* residual at the learned start */
 } avbd_stats;
 avbd_status avbd_get_stats(avbd_solver* s, avbd_stats* out);
 
@@ -344,6 +347,57 @@ avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us,
  * "narrowphase", "warmstart", "adjacency", "coloring", "solve", "finalize")
  * on frame `frame` (-1 = every frame).  NULL clears. */
 avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t frame);
+
+/* ---- Learned warm start (no counterpart in the reference) ----------------
+ * A per-particle MLP 24 -> 128 -> 128 -> 3 (ReLU; BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) predicts the step's
+ * converged correction relative to the adaptive warm-start guess of
+ * warmstart_bodies (40_solver.metal:657-722), in units of L = |g| dt^2.
+ * Features (24 per particle, units of L): v dt, (v - v_prev) dt,
+ * x~ - x, x_guess - x, the previous step's correction c, the 1-ring means
+ * of v dt and c relative to the particle's own, the contact-entry count / 8,
+ * the 1-ring size / 16 and |c|.  The parameters are a flat f32 array:
+ * W1[24][128], b1[128], W2[128][128], b2[128], W3[128][3], b3[3] (W[in][out]). */
+#define AVBD_MLP_INPUTS 24
+#define AVBD_MLP_OUTPUTS 3
+#define AVBD_MLP_PARAMS 20099
+typedef struct avbd_mlp avbd_mlp;
+/* He-uniform initialisation from `seed` (SplitMix64), zero biases */
+avbd_status avbd_mlp_create(uint64_t seed, avbd_mlp** out);
+void avbd_mlp_destroy(avbd_mlp* m);
+avbd_status avbd_mlp_get_params(avbd_mlp* m, float* params);
+/* also resets the Adam moments */
+avbd_status avbd_mlp_set_params(avbd_mlp* m, const float* params);
+/* y[n][3] = MLP(x[n][24]) */
+avbd_status avbd_mlp_forward(avbd_mlp* m, const float* x, uint32_t n, float* y);
+/* loss = sum |y - t|^2 / (3 n) and its gradient grad[AVBD_MLP_PARAMS] over
+ * the n rows (one batch, the training kernels) */
+avbd_status avbd_mlp_loss_grad(avbd_mlp* m, const float* x, const float* t, uint32_t n, float* loss,
+                               float* grad);
+/* Adam (lr, beta 0.9 / 0.999) over `epochs` passes of shuffled batches of
+ * `batch` rows (the last partial batch of an epoch is dropped unless it is the
+ * only one); epoch_loss[epochs] = mean batch loss per epoch (may be NULL). */
+avbd_status avbd_mlp_train(avbd_mlp* m, const float* x, const float* t, uint32_t n, int32_t epochs,
+                           uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
+/* average duration of one forward launch over n rows (HIP events on the
+ * network's stream, `reps` launches) */
+avbd_status avbd_mlp_time_forward(avbd_mlp* m, uint32_t n, int32_t reps, float* avg_us);
+
+/* Attach a network to a solver.  mode bits: AVBD_WS_LEARNED use the learned
+ * start (the plain guess is kept whenever the learned start's residual is
+ * higher), AVBD_WS_LOG record every step's features and converged
+ * corrections, AVBD_WS_FORCE keep the learned start unconditionally
+ * (testing).  m may be NULL for AVBD_WS_LOG alone; the network must outlive
+ * the attachment.  mode 0 detaches.  Rigid bodies keep the plain guess. */
+#define AVBD_WS_LEARNED 1
+#define AVBD_WS_LOG 2
+#define AVBD_WS_FORCE 4
+avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t mode);
+/* Samples of the last retired frame (AVBD_WS_LOG): x[n][24] features and
+ * t[n][3] converged corrections of the dynamic particles in index order;
+ * idx[n] their body ids (any output may be NULL). */
+avbd_status avbd_get_warm_start_samples(avbd_solver* s, uint32_t capacity, uint32_t* n, float* x, float* t,
+                                        uint32_t* idx);
 
 /* Library / device information string. */
 const char* avbd_version(void);

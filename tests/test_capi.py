@@ -33,7 +33,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_capi.Options) == 7 * 4
     assert C.sizeof(_capi.ClothInfo) == 9 * 4 + 4 * 4
     # int64 frame + 7 counters + 64 color counts + 4 + 4 cloth counters, 8-byte aligned
-    assert C.sizeof(_capi.Stats) == 8 + 4 * (7 + 64 + 4 + 4) + 4
+    assert C.sizeof(_capi.Stats) == 8 + 4 * (7 + 64 + 4 + 4 + 1 + 2)
     assert C.sizeof(_capi.Failure) == 4 + 4 + 8 * 3 + 4 + 160 + 4
 
 

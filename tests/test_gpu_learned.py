@@ -1,0 +1,200 @@
+"""GPU tests of the learned warm start: the BF16 MFMA network kernels against
+the CPU restatement (oracle/mlp.py), GPU training, the feature kernel against
+its restatement, the residual fallback, and graph capture of the stage.
+
+Bars: forward within 5e-3 (relative to max |y|) of the bf16-emulating oracle
+and 5e-2 of FP64; per-layer gradient relative error <= 2e-2 against the
+emulation and <= 6e-2 against FP64; features within 5e-4 (units of
+|g| dt^2: a last-bit difference of the positions); a rejected learned start
+leaves the trajectory bitwise equal to the plain solver's.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+from avbd_amd import GPUSolver, WarmStartMLP, demos
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    flush_print(f"[gpu case] {name}")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal((n, 24)) * 2.0).astype(np.float32)
+    t = (rng.standard_normal((n, 3))).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("n", [1, 15, 1000, 4133])
+def test_mlp_forward_matches_oracle(n):
+    _case(f"MLP forward vs oracle, n={n}")
+    m = WarmStartMLP(seed=3)
+    p = m.params
+    p[omlp.NUM_PARAMS - 3:] = [0.1, -0.2, 0.3]           # nonzero output biases
+    p[24 * 128:24 * 128 + 128] = np.linspace(-0.5, 0.5, 128)
+    m.params = p
+    x, _ = _data(n, 1)
+    y = m.forward(x)
+    ye, _ = omlp.forward(p, x, emulate=True)
+    y64, _ = omlp.forward(p, x, emulate=False)
+    scale = max(np.abs(y64).max(), 1e-3)
+    e_em = np.abs(y - ye).max() / scale
+    e_64 = np.abs(y - y64).max() / scale
+    flush_print(f"  max|y|={scale:.3f}  err vs bf16 emulation {e_em:.2e}, vs FP64 {e_64:.2e}")
+    assert e_em <= 5e-3
+    assert e_64 <= 5e-2
+
+
+@pytest.mark.parametrize("n", [37, 2048, 5000])
+def test_mlp_gradient_matches_oracle(n):
+    _case(f"MLP loss gradient vs oracle, n={n}")
+    m = WarmStartMLP(seed=5)
+    p = m.params
+    p[24 * 128:24 * 128 + 128] = 0.05                    # most units active
+    m.params = p
+    x, t = _data(n, 2)
+    loss, g = m.lossGrad(x, t)
+    le, ge = omlp.loss_grad(p, x, t, emulate=True)
+    l64, g64 = omlp.loss_grad(p, x, t, emulate=False)
+    flush_print(f"  loss gpu {loss:.6f} emulated {le:.6f} fp64 {l64:.6f}")
+    assert abs(loss - le) <= 1e-3 * abs(le)
+    off = 0
+    for name, shape in omlp.LAYOUT:
+        k = int(np.prod(shape))
+        a, b, c = g[off:off + k], ge[off:off + k], g64[off:off + k]
+        off += k
+        r_em, r_64 = _rel(a, b), _rel(a, c)
+        flush_print(f"  {name}: rel err vs emulation {r_em:.2e}, vs FP64 {r_64:.2e}")
+        assert r_em <= 2e-2, name
+        assert r_64 <= 6e-2, name
+
+
+def test_mlp_training_reduces_loss():
+    _case("MLP training (Adam, shuffled batches) on a smooth regression")
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((20000, 24)).astype(np.float32)
+    t = np.stack([np.tanh(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2], 0.3 * x[:, 3] - x[:, 4] ** 2 * 0.2,
+                  np.sin(x[:, 5])], 1).astype(np.float32)
+    m = WarmStartMLP(seed=1)
+    losses = m.train(x, t, epochs=25, batch=1024, lr=2e-3, seed=3)
+    flush_print(f"  epoch losses {losses[0]:.4f} -> {losses[-1]:.4f} (var t {t.var():.4f})")
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.15 * losses[0]
+    # the trained forward agrees with the oracle at the trained parameters
+    y = m.forward(x[:500])
+    ye, _ = omlp.forward(m.params, x[:500], emulate=True)
+    assert np.abs(y - ye).max() <= 5e-3 * max(np.abs(ye).max(), 1e-3)
+
+
+def _free_block(cells=3, seed=4):
+    s = demos.soft_block_scene(cells=cells, seed=seed, bottom=6.0, vel_sigma=0.5)
+    return s
+
+
+def test_features_match_restatement():
+    _case("warm-start features + converged corrections vs restatement, two steps of a free block")
+    s = _free_block()
+    g = GPUSolver(s, useGraph=False)
+    g.setWarmStart(None, log=True)
+    nb = len(s.bodies)
+    rng = np.random.default_rng(3)
+    p0, r0, _, w0 = g.bodyStates()
+    v0 = rng.normal(0, 0.4, (nb, 3)).astype(np.float32)
+    v0[0] = 0
+    g.setBodyStates(np.arange(nb, dtype=np.uint32), p0, r0, v0, w0)
+    st = g.settings
+    dt, grav = np.float32(st.dt), np.float32(st.gravity)
+    L = np.float32(max(np.float32(abs(grav)) * dt * dt, np.float32(1e-6)))
+    mass = g.bodyMass()
+    ids = np.array([i for i, b in enumerate(s.bodies) if b["particle"] and mass[i] > 0])
+    ring = omlp.ring_of(s, ids)
+    gz = np.array([0, 0, grav], np.float32)
+    corr = np.zeros((nb, 3), np.float32)
+    x_n, v_n, v_prev = p0.copy(), v0.copy(), v0.copy()
+    for step in range(2):
+        g.step()
+        X, T, idx = g.warmStartSamples()
+        assert np.array_equal(idx, ids)
+        x_tilde = x_n + v_n * dt + gz * (dt * dt)
+        acc = (v_n - v_prev) / dt
+        w = np.clip(acc[:, 2] * np.float32(np.sign(grav)) / np.float32(abs(grav)), 0, 1).astype(np.float32)
+        x_guess = x_n + v_n * dt + gz * ((w[:, None] * dt) * dt)
+        want = omlp.features_of(ids, x_n, v_n, v_prev, x_tilde, x_guess, corr, ring, np.zeros(len(ids)), L, dt)
+        err = np.abs(X - want).max()
+        x1 = g.positions()
+        tw = (x1[ids] - x_guess[ids]) / L
+        terr = np.abs(T - tw).max()
+        flush_print(f"  step {step}: features max err {err:.2e}, targets max err {terr:.2e}")
+        # a last-bit difference of a ~6 m position is 2e-4 in units of L
+        assert err <= 5e-4
+        assert terr <= 1e-3
+        if step == 0:
+            # free flight closed forms: x~ - x = v dt + g dt^2, guess offset v dt
+            np.testing.assert_allclose(X[:, 6:9], (v0[ids] * dt + gz * dt * dt) / L, rtol=1e-5, atol=1e-5)
+            np.testing.assert_allclose(X[:, 9:12], v0[ids] * dt / L, rtol=1e-5, atol=1e-5)
+            assert np.all(X[:, 12:15] == 0)
+        v_prev = v_n
+        _, _, v_n, _ = g.bodyStates()
+        x_n = x1
+        corr = np.zeros((nb, 3), np.float32)
+        corr[ids] = T
+
+
+def _bad_net():
+    m = WarmStartMLP(seed=2)
+    p = m.params
+    p[omlp.NUM_PARAMS - 3:] = [40.0, -30.0, 25.0]   # a start tens of |g| dt^2 off
+    m.params = p
+    return m
+
+
+def test_fallback_keeps_plain_trajectory():
+    _case("learned start with a higher residual falls back: bitwise the plain trajectory")
+    s = demos.soft_block_scene(cells=4, seed=6)
+    a = GPUSolver(s)
+    b = GPUSolver(s)
+    m = _bad_net()
+    b.setWarmStart(m, learned=True)
+    for _ in range(12):
+        a.step()
+        b.step()
+        st = b.stats()
+        assert st.ws_accepted == 0
+        assert st.ws_residual_learned > st.ws_residual_plain
+    for x, y in zip(a.bodyStates(), b.bodyStates()):
+        assert np.array_equal(x, y)
+    c = GPUSolver(s)
+    c.setWarmStart(m, learned=True, force=True)
+    d = GPUSolver(s)
+    c.step(3)
+    d.step(3)
+    st = c.stats()
+    assert st.ws_accepted == 1
+    assert not np.array_equal(c.positions(), d.positions())
+
+
+def test_learned_stage_graph_matches_eager():
+    _case("learned stage under hipGraph replay vs eager launches")
+    s = demos.soft_block_scene(cells=4, seed=8)
+    m = WarmStartMLP(seed=4)
+    a = GPUSolver(s, useGraph=True, lanesPerBody=8, sweepMode=1)
+    b = GPUSolver(s, useGraph=False, lanesPerBody=8, sweepMode=1)
+    for g in (a, b):
+        g.setWarmStart(m, learned=True, force=True, log=True)
+    for k in range(8):
+        a.step()
+        b.step()
+        a.synchronize()
+        assert np.array_equal(a.positions(), b.positions()), k
+    xa, ta, _ = a.warmStartSamples()
+    xb, tb, _ = b.warmStartSamples()
+    assert np.array_equal(xa, xb) and np.array_equal(ta, tb)
