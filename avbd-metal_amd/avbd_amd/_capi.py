@@ -155,6 +155,7 @@ MLP_PARAMS = 20099
 WS_LEARNED = 1
 WS_LOG = 2
 WS_FORCE = 4
+WS_SMOOTH_SHIFT = 8
 
 _lib = None
 

@@ -282,16 +282,19 @@ class GPUSolver:
         return dict(avg_launch_us=us.value, sweep_bytes=by.value, launches_per_sweep=la.value)
 
     # -- learned warm start (no reference counterpart) ------------------------
-    def setWarmStart(self, mlp=None, learned=True, log=False, force=False):
+    def setWarmStart(self, mlp=None, learned=True, log=False, force=False, smooth=2):
         """Attach a WarmStartMLP: learned=True replaces the adaptive guess of
-        the dynamic particles with guess + |g| dt^2 * MLP(features), keeping
-        the plain guess whenever the learned start's residual is higher
+        the dynamic particles with guess + |g| dt^2 * MLP(features) (after
+        `smooth` passes of 1-ring smoothing of the prediction), keeping the
+        plain guess whenever the learned start's residual is higher
         (force=True keeps it unconditionally); log=True records each step's
         features and converged corrections (warmStartSamples).  No arguments
         detach."""
         mode = (_capi.WS_LEARNED if (learned and mlp is not None) else 0) | (_capi.WS_LOG if log else 0)
         if mode & _capi.WS_LEARNED and force:
             mode |= _capi.WS_FORCE
+        if mode & _capi.WS_LEARNED:
+            mode |= (int(smooth) & 0xF) << _capi.WS_SMOOTH_SHIFT
         h = mlp._h if (mlp is not None and mode & _capi.WS_LEARNED) else None
         self._check(self._lib.avbd_set_warm_start(self._h, h, mode), "setWarmStart")
         self._warm_mlp = mlp if h else None   # keeps the network alive while attached

@@ -1,7 +1,8 @@
 // learned.hip -- the learned warm-start stage of a step.  After the
 // adaptive guess (predict.hip) and the contact adjacency, each dynamic
 // particle's features go through the MLP (mlp.hip), whose output replaces
-// the guess: x_start = x_guess + L y with L = |g| dt^2.  The residual
+// the guess: x_start = x_guess + L c with L = |g| dt^2 and c the network
+// output after optional 1-ring smoothing passes.  The residual
 // r = |g| / |m g_grav| is evaluated at both starts and the plain guess is
 // kept whenever the learned start's residual is higher (or not finite).
 // After the iterations, (x - x_guess) / L is the step's converged
@@ -16,6 +17,8 @@
 //   18-20 <c_j> - c               21    min(contact entries, 8) / 8
 //   22   1-ring size / 16         23    |c|
 // The reference has no learned warm start; this stage is the backend's own.
+#include <utility>
+
 #include "kernels.hpp"
 #include "mlp.hpp"
 
@@ -84,13 +87,34 @@ __global__ void __launch_bounds__(256) k_ws_reduce(const float* partial, uint32_
   }
 }
 
-__global__ void __launch_bounds__(256) k_ws_apply(DeviceState S, WarmState W) {
+__global__ void __launch_bounds__(256) k_ws_pred(WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  W.pred[W.idx[i]] = F4{W.y[3 * i], W.y[3 * i + 1], W.y[3 * i + 2], 0.0f};
+}
+
+// one Jacobi pass c_i <- (c_i + <c_j>) / 2 over the 1-ring (bodies off the
+// list keep a zero correction)
+__global__ void __launch_bounds__(256) k_ws_smooth(WarmState W, const F4* in, F4* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  const uint32_t b = W.idx[i], rs = W.ringStart[i], rc = W.ringCount[i];
+  const V3 c = xyz(in[b]);
+  if (rc == 0) {
+    out[b] = in[b];
+    return;
+  }
+  V3 sum = v3s(0.0f);
+  for (uint32_t k = 0; k < rc; k++) sum += xyz(in[W.ringList[rs + k]]);
+  out[b] = f4((c + sum * (1.0f / (float)rc)) * 0.5f, 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_ws_apply(DeviceState S, WarmState W, const F4* pred) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W.n) return;
   const uint32_t b = W.idx[i];
   const F4 g = W.guess[i];
-  const V3 y = v3(W.y[3 * i], W.y[3 * i + 1], W.y[3 * i + 2]);
-  const F4 p = f4(xyz(g) + y * W.L, g.w);
+  const F4 p = f4(xyz(g) + xyz(pred[b]) * W.L, g.w);
   S.posLin[b] = p;
   S.posLinSnap[b] = p;
 }
@@ -133,7 +157,14 @@ void launch_ws_reduce(hipStream_t st, const DeviceState& S, uint32_t blocks, con
   hipLaunchKernelGGL(k_ws_reduce, dim3(1), dim3(256), 0, st, (const float*)S.residualPartial, blocks, W.res, slot);
 }
 void launch_ws_apply(hipStream_t st, const DeviceState& S, const WarmState& W) {
-  if (W.n) hipLaunchKernelGGL(k_ws_apply, grid_n(W.n), dim3(256), 0, st, S, W);
+  if (!W.n) return;
+  hipLaunchKernelGGL(k_ws_pred, grid_n(W.n), dim3(256), 0, st, W);
+  F4 *a = W.pred, *b = W.pred2;
+  for (uint32_t k = 0; k < W.smoothPasses; k++) {
+    hipLaunchKernelGGL(k_ws_smooth, grid_n(W.n), dim3(256), 0, st, W, (const F4*)a, b);
+    std::swap(a, b);
+  }
+  hipLaunchKernelGGL(k_ws_apply, grid_n(W.n), dim3(256), 0, st, S, W, (const F4*)a);
 }
 void launch_ws_select(hipStream_t st, const DeviceState& S, const WarmState& W) {
   hipLaunchKernelGGL(k_ws_select, grid_n(W.n), dim3(256), 0, st, S, W);

@@ -47,8 +47,10 @@ struct WarmState {
   float* y;               // [n][3] network output
   float* logX;            // [n][24] features (logging)
   float* logT;            // [n][3] targets (logging)
-  float* res;             // [2] |g|^2 at the plain and the learned start
+  float* res;             // [3] |g|^2 at the plain and the learned start, |m g_grav|^2
+  F4 *pred, *pred2;       // [numBodies] predicted correction / L (ping-pong for the 1-ring smoothing)
   uint32_t n;
+  uint32_t smoothPasses;  // Jacobi smoothing passes of the predicted correction over the 1-ring
   float L;                // |g| dt^2 (feature and target unit)
   int force;              // 1: keep the learned start even when its residual is higher
   int log;                // 1: write logX / logT
@@ -71,6 +73,8 @@ void launch_mlp_gather_batch(hipStream_t st, const void* Xall, const float* Tall
 void launch_ws_features(hipStream_t st, const DeviceState& S, const StepParams& P, const WarmState& W);
 // |g|^2 of the residual partials into W.res[slot]
 void launch_ws_reduce(hipStream_t st, const DeviceState& S, uint32_t blocks, const WarmState& W, int slot);
+// network output -> per-body correction, `smoothPasses` passes of
+// c <- (c + <c_j>) / 2 over the 1-ring, then x_start = x_guess + L c
 void launch_ws_apply(hipStream_t st, const DeviceState& S, const WarmState& W);
 void launch_ws_select(hipStream_t st, const DeviceState& S, const WarmState& W);
 // converged correction of this step (and the logged target)

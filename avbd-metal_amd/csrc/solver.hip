@@ -2269,7 +2269,8 @@ avbd_status build_warm_state(avbd_solver* s) {
       (e = dev_alloc(s, &W.ringList, std::max<size_t>(rList.size(), 1))) || (e = dev_alloc(s, &W.guess, m)) ||
       (e = dev_alloc(s, &W.corr, s->nb)) || (e = dev_alloc(s, &xb, m * kMlpInPad)) ||
       (e = dev_alloc(s, &W.y, m * kMlpOut)) || (e = dev_alloc(s, &W.logX, m * kMlpIn)) ||
-      (e = dev_alloc(s, &W.logT, m * kMlpOut)) || (e = dev_alloc(s, &W.res, 4)))
+      (e = dev_alloc(s, &W.logT, m * kMlpOut)) || (e = dev_alloc(s, &W.res, 4)) ||
+      (e = dev_alloc(s, &W.pred, s->nb)) || (e = dev_alloc(s, &W.pred2, s->nb)))
     return AVBD_ERR_ALLOC;
   W.x = xb;
   if (upload(W.idx, idx) || upload(W.ringStart, rStart) || upload(W.ringCount, rCount) || upload(W.ringList, rList))
@@ -2281,9 +2282,13 @@ avbd_status build_warm_state(avbd_solver* s) {
 }  // namespace
 
 extern "C" avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t mode) {
-  if (!s || mode < 0 || mode > (AVBD_WS_LEARNED | AVBD_WS_LOG | AVBD_WS_FORCE)) return AVBD_ERR_INVALID_ARGUMENT;
-  if ((mode & AVBD_WS_LEARNED) && !m) return AVBD_ERR_INVALID_ARGUMENT;
-  if (mode == AVBD_WS_FORCE) return AVBD_ERR_INVALID_ARGUMENT;
+  const int32_t flags = mode & 0xFF, passes = (mode >> AVBD_WS_SMOOTH_SHIFT) & 0xF;
+  if (!s || mode < 0 || (mode >> (AVBD_WS_SMOOTH_SHIFT + 4)) != 0 ||
+      flags > (AVBD_WS_LEARNED | AVBD_WS_LOG | AVBD_WS_FORCE))
+    return AVBD_ERR_INVALID_ARGUMENT;
+  if ((flags & AVBD_WS_LEARNED) && !m) return AVBD_ERR_INVALID_ARGUMENT;
+  if ((flags & AVBD_WS_FORCE) && !(flags & AVBD_WS_LEARNED)) return AVBD_ERR_INVALID_ARGUMENT;
+  if (flags == 0) mode = 0;
   avbd_status st = sync_all(s);
   if (st != AVBD_OK) return st;
   if (mode && !s->wsBuilt) {
@@ -2293,6 +2298,7 @@ extern "C" avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t 
   // a fresh attachment starts without correction history
   if (mode && !s->wsMode && hipMemset(s->W.corr, 0, 16 * (size_t)s->nb) != hipSuccess) return AVBD_ERR_HIP;
   s->wsMode = mode;
+  s->W.smoothPasses = (uint32_t)passes;
   s->mlp = mode ? m : nullptr;
   invalidate_graphs(s);
   return AVBD_OK;

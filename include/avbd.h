@@ -392,6 +392,9 @@ avbd_status avbd_mlp_time_forward(avbd_mlp* m, uint32_t n, int32_t reps, float* 
 #define AVBD_WS_LEARNED 1
 #define AVBD_WS_LOG 2
 #define AVBD_WS_FORCE 4
+/* bits 8-11: passes of c <- (c + <c_j>) / 2 smoothing the predicted
+ * correction over the 1-ring before it is applied (0-15) */
+#define AVBD_WS_SMOOTH_SHIFT 8
 avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t mode);
 /* Samples of the last retired frame (AVBD_WS_LOG): x[n][24] features and
  * t[n][3] converged corrections of the dynamic particles in index order;

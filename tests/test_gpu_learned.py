@@ -139,8 +139,9 @@ def test_features_match_restatement():
         assert terr <= 1e-3
         if step == 0:
             # free flight closed forms: x~ - x = v dt + g dt^2, guess offset v dt
-            np.testing.assert_allclose(X[:, 6:9], (v0[ids] * dt + gz * dt * dt) / L, rtol=1e-5, atol=1e-5)
-            np.testing.assert_allclose(X[:, 9:12], v0[ids] * dt / L, rtol=1e-5, atol=1e-5)
+            # (within the rounding of a ~6 m position, 2e-4 in units of L)
+            np.testing.assert_allclose(X[:, 6:9], (v0[ids] * dt + gz * dt * dt) / L, rtol=0, atol=5e-4)
+            np.testing.assert_allclose(X[:, 9:12], v0[ids] * dt / L, rtol=0, atol=5e-4)
             assert np.all(X[:, 12:15] == 0)
         v_prev = v_n
         _, _, v_n, _ = g.bodyStates()
@@ -160,8 +161,9 @@ def _bad_net():
 def test_fallback_keeps_plain_trajectory():
     _case("learned start with a higher residual falls back: bitwise the plain trajectory")
     s = demos.soft_block_scene(cells=4, seed=6)
-    a = GPUSolver(s)
-    b = GPUSolver(s)
+    kw = dict(lanesPerBody=8, sweepMode=1)   # one kernel configuration for every solver compared
+    a = GPUSolver(s, **kw)
+    b = GPUSolver(s, **kw)
     m = _bad_net()
     b.setWarmStart(m, learned=True)
     for _ in range(12):
@@ -172,9 +174,9 @@ def test_fallback_keeps_plain_trajectory():
         assert st.ws_residual_learned > st.ws_residual_plain
     for x, y in zip(a.bodyStates(), b.bodyStates()):
         assert np.array_equal(x, y)
-    c = GPUSolver(s)
+    c = GPUSolver(s, **kw)
     c.setWarmStart(m, learned=True, force=True)
-    d = GPUSolver(s)
+    d = GPUSolver(s, **kw)
     c.step(3)
     d.step(3)
     st = c.stats()
@@ -189,7 +191,7 @@ def test_learned_stage_graph_matches_eager():
     a = GPUSolver(s, useGraph=True, lanesPerBody=8, sweepMode=1)
     b = GPUSolver(s, useGraph=False, lanesPerBody=8, sweepMode=1)
     for g in (a, b):
-        g.setWarmStart(m, learned=True, force=True, log=True)
+        g.setWarmStart(m, learned=True, force=True, log=True, smooth=2)
     for k in range(8):
         a.step()
         b.step()
