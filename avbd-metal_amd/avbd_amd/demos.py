@@ -84,12 +84,20 @@ def soft_block_scene(cells=20, edge=1.0, E=1.0e5, nu=0.4, density=1000.0, bottom
     lumped mass rho V / 4 per tet vertex, seeded initial velocity N(0, sigma)
     per component, dropped onto the ground.  Particles are ordered as
     addSoftBlock does (Demos5.swift:79-107): index = (i*ny + j)*nz + k."""
+    s = PhysicsScene("soft_block", SimSettings(iterations=iterations))
+    add_ground(s, ground_friction)
+    add_soft_block(s, cells, edge, E, nu, density, bottom, seed, vel_sigma, particle_friction, radius_factor, origin)
+    return s
+
+
+def add_soft_block(s, cells, edge=1.0, E=1.0e5, nu=0.4, density=1000.0, bottom=0.5, seed=2, vel_sigma=0.1,
+                   particle_friction=0.5, radius_factor=0.32, origin=(0.0, 0.0)):
+    """Adds one soft block (see soft_block_scene) centred at `origin` in xy;
+    returns its body range."""
     n = cells + 1
     h = edge / cells
     mu = E / (2.0 * (1.0 + nu))
     lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
-    s = PhysicsScene("soft_block", SimSettings(iterations=iterations))
-    add_ground(s, ground_friction)
     base = len(s.bodies)
 
     def index(i, j, k):
@@ -113,6 +121,50 @@ def soft_block_scene(cells=20, edge=1.0, E=1.0e5, nu=0.4, density=1000.0, bottom
                               velocity=vel)
     for t in tets:
         s.addTet(t, mu, lam)
+    return base, len(s.bodies)
+
+
+def batched_soft_specs(num_scenes=64, cells_range=(3, 8), seed_base=5 << 32):
+    """Per-scene draws of the batched soft blocks: scene e takes from
+    SplitMix64(seed_base + e) cells ~ U{lo..hi}, edge U(0.3, 1) m, Young's
+    modulus 10^U(4, 5) Pa, drop height U(0.1, 0.6) m and its velocity seed."""
+    specs = []
+    for e in range(num_scenes):
+        rng = SplitMix64(seed_base + e)
+        cells = cells_range[0] + int(rng.next() % (cells_range[1] - cells_range[0] + 1))
+        edge = rng.uniform(0.3, 1.0)
+        E = 10.0 ** rng.uniform(4.0, 5.0)
+        bottom = rng.uniform(0.1, 0.6)
+        specs.append(dict(cells=cells, edge=edge, E=E, bottom=bottom, seed=int(rng.next() & 0xFFFFFFFF)))
+    return specs
+
+
+def batched_soft_scene(num_scenes=64, cells_range=(3, 8), pitch=2.0, iterations=20, seed_base=5 << 32,
+                       separate=False):
+    """Uneven soft blocks (batched_soft_specs) on a grid of `pitch` m over
+    one shared ground: one scene holding every block, or with separate=True
+    one scene per block (each on its own ground) with the same bodies --
+    the same scenes stepped one solver at a time.  Returns the scene (or the
+    list of scenes)."""
+    specs = batched_soft_specs(num_scenes, cells_range, seed_base)
+    cols = int(math.ceil(math.sqrt(num_scenes)))
+    rows = int(math.ceil(num_scenes / cols))
+    hx = 0.5 * (cols - 1) * pitch
+    hy = 0.5 * (rows - 1) * pitch
+    if separate:
+        out = []
+        for e, sp in enumerate(specs):
+            s = PhysicsScene(f"soft_{e}", SimSettings(iterations=iterations))
+            cx, cy = (e % cols) * pitch - hx, (e // cols) * pitch - hy
+            s.addBody((pitch, pitch, 2), 0.0, 0.7, (cx, cy, -1))
+            add_soft_block(s, sp["cells"], sp["edge"], sp["E"], bottom=sp["bottom"], seed=sp["seed"], origin=(cx, cy))
+            out.append(s)
+        return out
+    s = PhysicsScene("batched_soft", SimSettings(iterations=iterations))
+    s.addBody((2 * hx + pitch, 2 * hy + pitch, 2), 0.0, 0.7, (0, 0, -1))
+    for e, sp in enumerate(specs):
+        cx, cy = (e % cols) * pitch - hx, (e // cols) * pitch - hy
+        add_soft_block(s, sp["cells"], sp["edge"], sp["E"], bottom=sp["bottom"], seed=sp["seed"], origin=(cx, cy))
     return s
 
 

@@ -105,3 +105,24 @@ class WarmStartMLP:
         us = C.c_float()
         self._check(self._lib.avbd_mlp_time_forward(self._h, int(n), int(reps), C.byref(us)), "time_forward")
         return us.value
+
+
+def collect_samples(scenes, frames, tol=2.0e-4, max_iterations=200, check_every=10, **solver_kw):
+    """Training pairs from the plain solver: each scene is stepped `frames`
+    times, every frame iterated until residual <= tol (at most
+    max_iterations sweeps); returns (features [n,24], corrections [n,3]) of
+    every dynamic particle and frame, and the mean sweeps per frame."""
+    from .solver import GPUSolver
+
+    xs, ts, its = [], [], []
+    for scene in scenes:
+        g = GPUSolver(scene, **solver_kw)
+        g.setWarmStart(None, log=True)
+        for _ in range(frames):
+            n, _ = g.stepToResidual(tol, max_iterations, check_every)
+            its.append(n)
+            x, t, _ = g.warmStartSamples()
+            xs.append(x)
+            ts.append(t)
+        g.close()
+    return np.concatenate(xs), np.concatenate(ts), float(np.mean(its))

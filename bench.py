@@ -11,7 +11,10 @@ is K steps bracketed by device synchronisation.
 Prints one JSON line: value = vertex-iters/s (dynamic vertices x iterations
 x steps / wall time), the step rate when every frame iterates to residual
 1e-4 (cap 200 sweeps), the primal color-sweep roofline measured live with
-HIP events, and the oracle CPU baseline on a bounded sample.
+HIP events, the oracle CPU baseline on a bounded sample, and the learned
+warm start against the plain one (frames iterated to residual 1e-3 -- the
+FP32 iterate of this block floors near 5e-4 -- and the residual after the
+fixed 20 sweeps), with the MLP forward's matrix-core roofline.
 
 --workload cloth / batched run the 256^2 cloth drape or the 4096 batched
 rigid scenes instead (reported the same way; the default is the soft block).
@@ -61,6 +64,115 @@ def cpu_baseline(scene, unit, target_s=10.0):
             "sample": f"{n} steps of the same workload after 2 warm-up steps, serial oracle FP32, {dt:.2f} s"}
 
 
+def learned_comparison(args, nv):
+    """Learned vs plain warm start on the metric's scene: training pairs from
+    the plain solver on other seeds of the same block (every frame iterated to
+    residual 2e-4 or 200 sweeps), GPU training, then frames iterated to
+    `tol` with and without the learned start (plain fallback active), the
+    residual after the fixed iteration count, and the MLP forward roofline."""
+    import numpy as np
+    from avbd_amd import GPUSolver, WarmStartMLP, demos
+    from avbd_amd.learned import collect_samples
+
+    t0 = time.perf_counter()
+    train = [demos.soft_block_scene(cells=args.cells, seed=10 + k) for k in range(args.train_seeds)]
+    X, T, label_its = collect_samples(train, args.train_frames)
+    t_collect = time.perf_counter() - t0
+    m = WarmStartMLP(seed=1)
+    t0 = time.perf_counter()
+    losses = m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
+    t_train = time.perf_counter() - t0
+
+    def run(learned):
+        s = demos.soft_block_scene(cells=args.cells)
+        g = GPUSolver(s)
+        if learned:
+            g.setWarmStart(m, smooth=2)
+        its, acc = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.learned_frames):
+            n, _ = g.stepToResidual(args.learned_tol, 200, 5)
+            its.append(n)
+            acc.append(g.stats().ws_accepted)
+        wall = time.perf_counter() - t0
+        g2 = GPUSolver(s)
+        if learned:
+            g2.setWarmStart(m, smooth=2)
+        res = []
+        for _ in range(args.learned_frames):
+            g2.step()
+            res.append(g2.residual())
+        return {"iterations_mean": round(float(np.mean(its)), 2), "ms_per_step": round(wall * 1000 / len(its), 4),
+                "learned_start_kept": round(float(np.mean(acc)), 3) if learned else None,
+                "residual_after_fixed_iterations_mean": float(np.mean(res)),
+                "residual_after_fixed_iterations_median": float(np.median(res))}
+
+    plain, learned = run(False), run(True)
+    us = m.timeForward(nv, reps=50)
+    flops = 2.0 * nv * (24 * 128 + 128 * 128 + 128 * 3)
+    tflops = flops / (us * 1e-6) / 1e12
+    out = {"tolerance": args.learned_tol, "frames": args.learned_frames, "max_iterations": 200,
+           "training": {"samples": int(len(X)), "scenes": f"{args.train_seeds} seeds x {args.train_frames} frames",
+                        "label_sweeps_mean": round(label_its, 1), "collect_s": round(t_collect, 2),
+                        "epochs": args.epochs, "train_s": round(t_train, 2),
+                        "loss_first_last": [float(losses[0]), float(losses[-1])], "target_variance": float(T.var())},
+           "plain": plain, "learned": learned,
+           "step_rate_ratio": round(plain["ms_per_step"] / learned["ms_per_step"], 3),
+           "iteration_ratio": round(plain["iterations_mean"] / learned["iterations_mean"], 3)}
+    roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_mlp_forward",
+            "avg_launch_us": round(us, 3), "rows": nv, "flops_per_row": 2 * (24 * 128 + 128 * 128 + 128 * 3)}
+    return out, roof
+
+
+def batched_learned_comparison(args):
+    """Batched + learned vs plain AVBD on the same uneven soft blocks: the
+    plain path steps each block with its own solver (plain warm start), the
+    batched paths step all blocks in one solver, without and with the learned
+    start (trained on other draws of the same generator).  Frames iterate to
+    residual `tol` (per solver: each block for the plain path, all blocks
+    together for the batched paths), cap 200 sweeps."""
+    import numpy as np
+    from avbd_amd import GPUSolver, WarmStartMLP, demos
+    from avbd_amd.learned import collect_samples
+
+    n, frames, tol = args.batch_scenes, args.batch_frames, args.learned_tol
+    t0 = time.perf_counter()
+    train = [demos.batched_soft_scene(n, seed_base=(6 << 32) + 1000 * k) for k in range(2)]
+    X, T, _ = collect_samples(train, args.train_frames)
+    m = WarmStartMLP(seed=1)
+    m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
+    t_train = time.perf_counter() - t0
+
+    def frames_to_tol(solvers):
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            for g in solvers:
+                k, _ = g.stepToResidual(tol, 200, 5)
+                its.append(k)
+        return (time.perf_counter() - t0) * 1000.0 / frames, float(np.mean(its))
+
+    sep = [GPUSolver(sc) for sc in demos.batched_soft_scene(n, separate=True)]
+    ms_plain, it_plain = frames_to_tol(sep)
+    for g in sep:
+        g.close()
+    scene = demos.batched_soft_scene(n)
+    gb = GPUSolver(scene)
+    ms_batched, it_batched = frames_to_tol([gb])
+    gl = GPUSolver(scene)
+    gl.setWarmStart(m, smooth=2)
+    ms_learned, it_learned = frames_to_tol([gl])
+    return {"scenes": n, "vertices": scene.numDynamic, "tets": len(scene.tets), "frames": frames,
+            "tolerance": tol, "train_s": round(t_train, 2), "train_samples": int(len(X)),
+            "plain_per_scene_solvers": {"ms_per_step": round(ms_plain, 3), "iterations_mean": round(it_plain, 2)},
+            "batched_plain": {"ms_per_step": round(ms_batched, 3), "iterations_mean": round(it_batched, 2)},
+            "batched_learned": {"ms_per_step": round(ms_learned, 3), "iterations_mean": round(it_learned, 2)},
+            "step_rate_ratio_batched_learned_vs_plain": round(ms_plain / ms_learned, 2),
+            "step_rate_ratio_batching_only": round(ms_plain / ms_batched, 2),
+            "step_rate_ratio_learned_only": round(ms_batched / ms_learned, 3)}
+
+
 def build(workload, cells):
     from avbd_amd import demos
 
@@ -89,6 +201,14 @@ def main():
     ap.add_argument("--workload", choices=["soft_block", "cloth", "batched"], default="soft_block")
     ap.add_argument("--residual-steps", type=int, default=30, help="frames iterated to residual 1e-4")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain warm start comparison")
+    ap.add_argument("--learned-tol", type=float, default=1e-3)
+    ap.add_argument("--learned-frames", type=int, default=60)
+    ap.add_argument("--train-seeds", type=int, default=3)
+    ap.add_argument("--train-frames", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch-scenes", type=int, default=64, help="uneven soft blocks of the batched comparison")
+    ap.add_argument("--batch-frames", type=int, default=30)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
     ap.add_argument("--sweep", type=int, default=0, help="1 per-color launches, 2 persistent (0 = autotune)")
     args = ap.parse_args()
@@ -152,6 +272,11 @@ def main():
                            "traffic": traffic_from_profiles(pmc) if pmc else None,
                            "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
                            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+    if args.workload == "soft_block" and not args.no_learned:
+        cmp, roof = learned_comparison(args, nv)
+        out["config"]["learned_vs_plain"] = cmp
+        out["roofline_mlp"] = roof
+        out["config"]["batched_learned_vs_plain"] = batched_learned_comparison(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, unit)
     print(json.dumps(out), flush=True)
