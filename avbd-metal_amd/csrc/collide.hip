@@ -79,15 +79,77 @@ __global__ void __launch_bounds__(256) k_bp_sort_cells(DeviceState S, StepParams
       if ((uint32_t)i < n) S.cellBodies[s + i] = v[i];
     return;
   }
-  for (uint32_t i = 1; i < n; i++) {
-    uint32_t v = S.cellBodies[s + i];
-    uint32_t j = i;
-    while (j > 0 && S.cellBodies[s + j - 1] > v) {
-      S.cellBodies[s + j] = S.cellBodies[s + j - 1];
-      j--;
+  // longer buckets (dense particle packings under the 0.05 m radius floor
+  // of the cell size): one workgroup each, k_bp_sort_large
+  S.largeCells[atomicAdd(&S.counters[CTR_LARGE_CELLS], 1u)] = gid;
+}
+
+// Buckets of more than 32 colliders, one workgroup per bucket: bitonic sort
+// in LDS (up to kSortLds entries; a larger bucket is insertion-sorted by one
+// thread).
+constexpr uint32_t kSortLds = 4096;
+__global__ void __launch_bounds__(256) k_bp_sort_large(DeviceState S, StepParams P) {
+  __shared__ uint32_t v[kSortLds];
+  const uint32_t count = S.counters[CTR_LARGE_CELLS];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const uint32_t h = S.largeCells[q];
+    const uint32_t n = S.cellCount[h], s0 = S.cellStart[h];
+    if (n > kSortLds) {
+      if (tid == 0) {
+        for (uint32_t i = 1; i < n; i++) {
+          uint32_t x = S.cellBodies[s0 + i];
+          uint32_t j = i;
+          while (j > 0 && S.cellBodies[s0 + j - 1] > x) {
+            S.cellBodies[s0 + j] = S.cellBodies[s0 + j - 1];
+            j--;
+          }
+          S.cellBodies[s0 + j] = x;
+        }
+      }
+      __syncthreads();
+      continue;
     }
-    S.cellBodies[s + j] = v;
+    uint32_t m = 64;
+    while (m < n) m <<= 1;
+    for (uint32_t i = tid; i < m; i += blockDim.x) v[i] = i < n ? S.cellBodies[s0 + i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = tid; i < m; i += blockDim.x) {
+          const uint32_t l = i ^ j;
+          if (l > i) {
+            const uint32_t x = v[i], y = v[l];
+            if ((x > y) == ((i & k) == 0u)) {
+              v[i] = y;
+              v[l] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = tid; i < n; i += blockDim.x) S.cellBodies[s0 + i] = v[i];
+    __syncthreads();
   }
+}
+
+// Candidate search grid: cells of 2 x the largest hashed radius, keyed by the
+// same hash and collision group; holds hashed-collider indices, unordered.
+__global__ void __launch_bounds__(256) k_bp_fine_count(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed) return;
+  int fx, fy, fz;
+  cell_coord(xyz(S.hashCenter[gid]), P.fineCellSize, fx, fy, fz);
+  uint32_t h = cell_hash(fx, fy, fz, S.colliderGroup[S.hashedIdx[gid]], P.gridHashSize);
+  S.fineSlot[gid] = U2{h, atomicAdd(&S.fineCount[h], 1u)};
+}
+
+__global__ void __launch_bounds__(256) k_bp_fine_scatter(DeviceState S, StepParams P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numHashed) return;
+  U2 fs = S.fineSlot[gid];
+  S.fineBodies[S.fineStart[fs.x] + fs.y] = gid;
 }
 
 // cell-ordered copies of the per-collider center / metadata
@@ -113,6 +175,15 @@ __device__ __forceinline__ bool pair_accept(const DeviceState& S, const StepPara
   return !pair_excluded(S, P.numExclusions, lo, hi);
 }
 
+// LDS writes of one lane visible to the other lanes of its wave
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kFineCap = 128;  // accepted candidates one wave sorts in LDS
+
 // One wave per producer (a hashed collider scanning its 27 neighbour cells
 // and the global list, or a global collider scanning the later globals).
 // Lanes split the producer's linearised candidate list; a ballot ranks the
@@ -120,8 +191,16 @@ __device__ __forceinline__ bool pair_accept(const DeviceState& S, const StepPara
 // cells in (dz, dy, dx) order, each bucket in collider order, then globals.
 // Pass 1 (EMIT = false) writes the per-producer count, pass 2 emits at the
 // scanned offset.
+// With the candidate search grid (P.fineCellSize > 0: hash cells much larger
+// than the colliders), a hashed producer instead scans its 27 search cells,
+// applies the same filters, keys every accepted candidate by (position of
+// its hash bucket in the 27-cell visit order, collider id) and emits the
+// keys sorted -- the same set in the same order as the bucket walk, since an
+// accepted partner lies within the radius sum, hence in an adjacent hash
+// cell.  More than kFineCap accepted candidates fall back to the walk.
 template <bool EMIT>
 __global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
+  __shared__ uint64_t fineKeys[4][2][kFineCap];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t producer = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (producer >= P.numHashed + P.numGlobals) return;  // whole wave
@@ -158,7 +237,96 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
         uint32_t hj = __shfl(h, (int)j, 64);
         dup |= (j < lane) && (hj == h);
       }
-      if (lane < 27 && !dup && !(ga != 0 && S.cellRigid[h] == 0)) {
+      bool walk = true;
+      if (P.fineCellSize > 0.0f) {
+        uint64_t* keys = fineKeys[threadIdx.x >> 6][0];
+        uint64_t* sorted = fineKeys[threadIdx.x >> 6][1];
+        int fx, fy, fz;
+        cell_coord(pa, P.fineCellSize, fx, fy, fz);
+        uint32_t hf = 0, fs = 0, fc = 0;
+        if (lane < 27) {
+          int q = (int)lane;
+          hf = cell_hash(fx + q % 3 - 1, fy + (q / 3) % 3 - 1, fz + q / 9 - 1, grpA, P.gridHashSize);
+        }
+        bool fdup = false;
+        for (uint32_t j = 0; j < 27; j++) {
+          uint32_t hj = __shfl(hf, (int)j, 64);
+          fdup |= (j < lane) && (hj == hf);
+        }
+        if (lane < 27 && !fdup) {
+          fs = S.fineStart[hf];
+          fc = S.fineCount[hf];
+        }
+        uint32_t fincl = fc;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+          uint32_t y = __shfl_up(fincl, d, 64);
+          if (lane >= (uint32_t)d) fincl += y;
+        }
+        const uint32_t fexcl = fincl - fc;
+        const uint32_t ftotal = __shfl(fincl, 26, 64);
+        uint32_t nAcc = 0;
+        for (uint32_t chunk = 0; chunk < ftotal; chunk += 64) {
+          const uint32_t j = chunk + lane;
+          uint32_t k = 0;
+          bool valid = false;
+          for (uint32_t t = 0; t < 27; t++) {
+            uint32_t et = __shfl(fexcl, (int)t, 64), ct = __shfl(fc, (int)t, 64), st = __shfl(fs, (int)t, 64);
+            if (j >= et && j < et + ct) { k = st + (j - et); valid = true; }
+          }
+          uint32_t b = 0, hb = 0xFFFFFFFFu;
+          U4 mb = U4{0, 0, 0, 0};
+          F4 cb = F4{0, 0, 0, 0};
+          bool cand = false;
+          if (valid) {
+            const uint32_t g = S.fineBodies[k];
+            b = S.hashedIdx[g];
+            cand = b > a;
+            if (cand) {
+              mb = S.hashMeta[g];
+              cb = S.hashCenter[g];
+              int bx, by, bz;
+              cell_coord(xyz(cb), P.cellSize, bx, by, bz);
+              hb = cell_hash(bx, by, bz, S.colliderGroup[b], P.gridHashSize);
+            }
+          }
+          uint32_t pos = 27;  // first position of the partner's bucket in the visit order
+          for (uint32_t q = 0; q < 27; q++) {
+            uint32_t hq = __shfl(h, (int)q, 64);
+            if (pos == 27 && hq == hb) pos = q;
+          }
+          bool accept = false;
+          if (cand && pos < 27) {
+            accept = !(ga != 0 && S.cellRigid[hb] == 0) && !(ga != 0 && mb.w != 0) &&
+                     pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, mb.x, mb.y, mb.z & 1u, (mb.z & 2u) != 0,
+                                 xyz(cb), cb.w);
+          }
+          const uint64_t mask = __ballot(accept);
+          if (accept) {
+            const uint32_t idx = nAcc + (uint32_t)__builtin_popcountll(mask & ltMask);
+            if (idx < kFineCap) keys[idx] = ((uint64_t)pos << 32) | b;
+          }
+          nAcc += (uint32_t)__builtin_popcountll(mask);
+        }
+        if (nAcc <= kFineCap) {
+          walk = false;
+          wave_lds_sync();
+          for (uint32_t i = lane; i < nAcc; i += 64) {
+            const uint64_t key = keys[i];
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < nAcc; j++) rank += keys[j] < key ? 1u : 0u;
+            sorted[rank] = key;
+          }
+          wave_lds_sync();
+          for (uint32_t i0 = 0; i0 < nAcc; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const bool ok = i < nAcc;
+            emitRanked(ok, a, ok ? (uint32_t)(sorted[i] & 0xFFFFFFFFull) : 0u);
+          }
+          wave_lds_sync();  // the next producer of this wave rewrites the keys
+        }
+      }
+      if (walk && lane < 27 && !dup && !(ga != 0 && S.cellRigid[h] == 0)) {
         cs = S.cellStart[h];
         cc = S.cellCount[h];
       }
@@ -169,7 +337,7 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
         if (lane >= (uint32_t)d) incl += y;
       }
       const uint32_t excl = incl - cc;
-      const uint32_t total = __shfl(incl, 26, 64);
+      const uint32_t total = walk ? __shfl(incl, 26, 64) : 0u;
       for (uint32_t chunk = 0; chunk < total; chunk += 64) {
         uint32_t j = chunk + lane;
         uint32_t k = 0;
@@ -718,14 +886,22 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
 static inline uint32_t blocks_for(uint32_t n, uint32_t tpb) { return (n + tpb - 1) / tpb; }
 
 void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  const bool fine = P.fineCellSize > 0.0f && P.numHashed > 0;
   if (P.numHashed > 0) {
     hipLaunchKernelGGL(k_bp_count, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   }
+  if (fine) hipLaunchKernelGGL(k_bp_fine_count, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   launch_exclusive_scan(st, S.cellCount, S.cellStart, P.gridHashSize, S.scanScratch);
+  if (fine) launch_exclusive_scan(st, S.fineCount, S.fineStart, P.gridHashSize, S.scanScratch);
   if (P.numHashed > 0) {
     hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   }
+  if (fine) hipLaunchKernelGGL(k_bp_fine_scatter, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_bp_sort_cells, dim3(blocks_for(P.gridHashSize, 256)), dim3(256), 0, st, S, P);
+  {
+    uint32_t large = P.numHashed / 33u + 1u;
+    hipLaunchKernelGGL(k_bp_sort_large, dim3(large < 1024u ? large : 1024u), dim3(256), 0, st, S, P);
+  }
   if (P.numHashed > 0) {
     hipLaunchKernelGGL(k_bp_gather, dim3(blocks_for(P.numHashed, 256)), dim3(256), 0, st, S, P);
   }

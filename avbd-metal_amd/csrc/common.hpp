@@ -73,6 +73,7 @@ enum Counter : uint32_t {
   CTR_WS_ACCEPT,           // learned warm start kept this frame (1) or fell back (0)
   CTR_WS_RES_PLAIN,        // residual at the plain start (f32 bits)
   CTR_WS_RES_LEARNED,      // residual at the learned start (f32 bits)
+  CTR_LARGE_CELLS,         // hash buckets of more than 32 colliders this frame
   CTR_TOTAL = 16 + 2 * kMaxColors + 8
 };
 
@@ -384,6 +385,7 @@ struct StepParams {
   // cloth surfaces (Planar-DAT pair stream + OGC element contacts)
   uint32_t numTris, numEdges, numParticles, elemHashSize, maxSoft, softMapCapacity, maxDatPairs;
   float elemCellSize, elemMargin, datQueryRadius, surfaceContactCellSize, clothViscosity;
+  float fineCellSize;       // > 0: candidate search grid of 2 x the largest hashed radius
 };
 
 // ManifoldRec.header.w flag: both bodies dynamic and of the same static

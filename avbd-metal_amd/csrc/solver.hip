@@ -49,6 +49,7 @@ struct avbd_solver {
   uint32_t nHashed = 0, nGlobal = 0, nHashedRigid = 0, nExcl = 0;
   uint32_t maxPairs = 0, mapCapacity = 0, gridHashSize = 0;
   float cellSize = 0.1f;
+  float fineCellSize = 0.0f;  // candidate search grid (0 = off)
   bool dynamicColoring = false;
   uint32_t staticUsedColors = 1;
   std::vector<uint32_t> staticColorCounts;  // per color
@@ -168,6 +169,7 @@ StepParams make_params(const avbd_solver* s) {
   P.gamma = s->settings.gamma;
   P.lambdaMax = s->settings.lambda_max;
   P.cellSize = s->cellSize;
+  P.fineCellSize = s->fineCellSize;
   P.maxSpeed = fmaxf(30.0f, 1.5f * s->cellSize / s->settings.dt * 0.5f);
   P.collisionMargin = fmaxf(s->settings.collision_margin, 0.0f);
   P.particleDamping = s->settings.particle_damping;
@@ -224,9 +226,10 @@ using StageFn = std::function<void(const char*)>;
 
 void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
   hipStream_t st = s->stream;
-  uint32_t* zp[5] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.elemCellCount};
-  uint32_t zn[5] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->nTris ? s->elemHashSize : 0u};
-  launch_zero_multi(st, zp, zn, 5);
+  uint32_t* zp[6] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.elemCellCount, S.fineCount};
+  uint32_t zn[6] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->nTris ? s->elemHashSize : 0u,
+                    s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
+  launch_zero_multi(st, zp, zn, 6);
   launch_broadphase(st, S, P);
   stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
@@ -937,9 +940,17 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     if (crad[i] > threshold || (hasIsolated && cg[i] == 0)) globals.push_back(i);
     else hashed.push_back(i);
   }
-  float maxHashedR = 0.05f;
-  for (uint32_t i : hashed) maxHashedR = fmaxf(maxHashedR, fabsf(cs[i].w));
+  float maxHashedR = 0.05f, maxR = 0.0f;
+  for (uint32_t i : hashed) {
+    maxHashedR = fmaxf(maxHashedR, fabsf(cs[i].w));
+    maxR = fmaxf(maxR, fabsf(cs[i].w));
+  }
   s->cellSize = maxHashedR * 2.0f;
+  // hash cells at least twice the largest hashed diameter (the 0.05 m radius
+  // floor under dense particles): candidates come from a search grid of
+  // cells just over that diameter instead of the bucket walk
+  const float fine = 2.0f * maxR * 1.001f;
+  s->fineCellSize = (maxR > 0.0f && 2.0f * fine <= s->cellSize) ? fine : 0.0f;
   s->nHashed = (uint32_t)hashed.size();
   s->nGlobal = (uint32_t)globals.size();
   std::vector<uint32_t> hashedRigid;
@@ -1274,6 +1285,13 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.hashCenter, std::max<size_t>(1, hashed.size())) ALLOC(S.hashMeta, std::max<size_t>(1, hashed.size()))
   ALLOC(S.cellPos, std::max<size_t>(1, hashed.size())) ALLOC(S.cellMeta, std::max<size_t>(1, hashed.size()))
   ALLOC(S.colliderToHashed, std::max<uint32_t>(1, nc))
+  {
+    const bool fineOn = s->fineCellSize > 0.0f;
+    ALLOC(S.fineCount, fineOn ? s->gridHashSize : 1u) ALLOC(S.fineStart, fineOn ? s->gridHashSize : 1u)
+    ALLOC(S.fineBodies, fineOn ? std::max<size_t>(1, hashed.size()) : 1u)
+    ALLOC(S.fineSlot, fineOn ? std::max<size_t>(1, hashed.size()) : 1u)
+    ALLOC(S.largeCells, hashed.size() / 33 + 1)
+  }
   uint32_t producers = s->nHashed + s->nGlobal;
   ALLOC(S.pairCount, std::max<uint32_t>(1, producers)) ALLOC(S.pairStart, std::max<uint32_t>(1, producers))
   ALLOC(S.pairs, s->maxPairs) ALLOC(S.exclusions, std::max<size_t>(1, exr.size()))
@@ -2195,20 +2213,41 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
       bytes += 4 + 64 + 80.0 * nct + 2 * 32 + 2 * 32;
     }
   }
+  // the `reps` sweeps are captured in one hipGraph and replayed (warm-up
+  // replay, then one timed replay), so the events bracket device-side
+  // dispatch only, as the frame's own graph does
+  auto sweeps = [&]() {
+    for (int r = 0; r < reps; r++)
+      for (uint32_t c = 0; c < s->staticUsedColors; c++)
+        launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes,
+                            s->colorMode[c]);
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t he = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+  if (he == hipSuccess) {
+    sweeps();
+    he = hipStreamEndCapture(s->stream, &graph);
+  }
+  if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (he != hipSuccess) {
+    hipGetLastError();
+    return AVBD_ERR_HIP;
+  }
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  hipGraphLaunch(exec, s->stream);
   hipEventRecord(e0, s->stream);
-  for (int r = 0; r < reps; r++)
-    for (uint32_t c = 0; c < s->staticUsedColors; c++)
-      launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes,
-                          s->colorMode[c]);
+  hipGraphLaunch(exec, s->stream);
   hipEventRecord(e1, s->stream);
-  hipError_t he = hipEventSynchronize(e1);
+  he = hipEventSynchronize(e1);
   float ms = 0.0f;
   hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  hipGraphExecDestroy(exec);
   if (he != hipSuccess) return AVBD_ERR_HIP;
   int32_t launches = (int32_t)s->staticUsedColors;
   if (avg_launch_us) *avg_launch_us = ms * 1000.0f / (float)(reps * launches);

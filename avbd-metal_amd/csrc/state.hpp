@@ -34,6 +34,11 @@ struct DeviceState {
   F4* cellPos;
   U4* cellMeta;
   U2* bodyCellSlot;
+  // candidate search grid (cells of 2 x the largest hashed radius, same
+  // hash): hashed-collider indices per bucket, unordered
+  uint32_t *fineCount, *fineStart, *fineBodies;
+  U2* fineSlot;
+  uint32_t* largeCells;       // buckets of more than 32 colliders (sorted by a workgroup each)
   uint32_t *pairCount, *pairStart;
   U2* pairs;
   U2* exclusions;

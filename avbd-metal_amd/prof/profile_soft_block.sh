@@ -1,5 +1,6 @@
 #!/bin/bash
-# Profiling recipe behind profiles/*soft_block_21*: kernel trace of the
+# Profiling recipe behind profiles/*soft_block_21* (CELLS=20, default) and
+# profiles/*soft_block_56* (CELLS=55, the 1M-tet block): kernel trace of the
 # benchmark, then one rocprofv3 PMC pass per counter (FETCH_SIZE, WRITE_SIZE)
 # on the benchmark and on the copy calibration kernel.  Run from the repo
 # root on the GPU host; writes raw output under gpurun_out/prof_* and the
@@ -8,7 +9,8 @@ set -u
 cd "$(dirname "$0")/../.."
 export TMPDIR=/tmp
 OUT=gpurun_out
-BENCH="bench.py --steps 4 --warmup 2 --no-cpu --lanes ${LANES:-32} --sweep ${SWEEP:-1}"
+CELLS=${CELLS:-20}
+BENCH="bench.py --cells $CELLS --steps 4 --warmup 2 --no-cpu --no-learned --residual-steps 0 --lanes ${LANES:-32} --sweep ${SWEEP:-1}"
 /opt/rocm/bin/hipcc -O2 --offload-arch=gfx950 -o avbd-metal_amd/lib/copy_calib avbd-metal_amd/prof/copy_calib.hip || exit 1
 echo "pass 1: kernel trace"
 timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/prof_trace -o trace -- python3 $BENCH > $OUT/prof_trace.log 2>&1
@@ -21,4 +23,4 @@ for C in FETCH_SIZE WRITE_SIZE; do
   timeout -s KILL 60 rocprofv3 --pmc $C --output-format csv -d $OUT/prof_$C -o copy -- ./avbd-metal_amd/lib/copy_calib > $OUT/prof_${C}_copy.log 2>&1
   rc=$?; tail -1 $OUT/prof_${C}_copy.log; [ -f $OUT/prof_$C/copy_counter_collection.csv ] || exit $rc
 done
-python3 avbd-metal_amd/prof/summarize.py "$OUT" "$BENCH"
+python3 avbd-metal_amd/prof/summarize.py "$OUT" "$BENCH" $((CELLS + 1))

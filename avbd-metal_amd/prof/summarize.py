@@ -1,7 +1,8 @@
-"""Summaries of the profiling recipe (profile_soft_block.sh): per-kernel
+"""Summaries of the profiling recipes (profile_soft_block.sh,
+profile_learned.sh): per-kernel
 durations of the benchmark and HBM bytes per launch of the primal kernel
 from the FETCH_SIZE / WRITE_SIZE passes, calibrated on a plain copy of known
-size.  Writes kernels_soft_block_21.txt and primal_soft_block_21_pmc.json
+size.  Writes kernels_soft_block_<side>.txt and primal_soft_block_<side>_pmc.json
 into <out>/summary (copied into profiles/ afterwards)."""
 import csv
 import json
@@ -29,14 +30,33 @@ def counter_avg(path, counter, name_filter):
     return (sum(allv) / len(allv) if allv else None), len(allv)
 
 
+def kernels_only(stats_csv, dest, title, cmd):
+    """Per-kernel table of one kernel trace (no counters)."""
+    stats = kernel_stats(stats_csv)
+    lines = [f"# {title}", f"# command: {cmd}", "#",
+             f"# {'kernel':70s} {'calls':>6s} {'avg_us':>9s} {'total_ms':>9s} {'pct':>6s}"]
+    for r in stats[:40]:
+        lines.append(f"{r['Name'][:72]:72s} {r['Calls']:>6s} {float(r['AverageNs']) / 1e3:9.3f} "
+                     f"{float(r['TotalDurationNs']) / 1e6:9.3f} {float(r['Percentage']):6.2f}")
+    os.makedirs(os.path.dirname(dest), exist_ok=True)
+    with open(dest, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines[:12]))
+
+
 def main():
+    if sys.argv[1] == "--kernels-only":
+        kernels_only(*sys.argv[2:6])
+        return
     out, bench_cmd = sys.argv[1], sys.argv[2]
+    side = int(sys.argv[3]) if len(sys.argv) > 3 else 21
     prof = os.path.join(out, "summary")
     os.makedirs(prof, exist_ok=True)
     stats = kernel_stats(os.path.join(out, "prof_trace", "trace_kernel_stats.csv"))
-    cmd = ("avbd-metal_amd/prof/profile_soft_block.sh  (rocprofv3 --kernel-trace --stats --output-format csv "
+    cmd = (f"CELLS={side - 1} avbd-metal_amd/prof/profile_soft_block.sh  (rocprofv3 --kernel-trace --stats --output-format csv "
            f"-- python3 {bench_cmd})")
-    lines = [f"# rocprofv3 kernel trace, soft block 21^3 (9261 vertices, 48000 tets, 20 iterations)",
+    cells = side - 1
+    lines = [f"# rocprofv3 kernel trace, soft block {side}^3 ({side ** 3} vertices, {6 * cells ** 3} tets, 20 iterations)",
              f"# command: {cmd}", "#",
              f"# {'kernel':70s} {'calls':>6s} {'avg_us':>9s} {'total_ms':>9s} {'pct':>6s}"]
     for r in stats[:30]:
@@ -59,9 +79,9 @@ def main():
     if res["FETCH_SIZE"]["primal_bytes"] is not None and res["WRITE_SIZE"]["primal_bytes"] is not None:
         res["hbm_bytes_per_launch"] = res["FETCH_SIZE"]["primal_bytes"] + res["WRITE_SIZE"]["primal_bytes"]
         lines.append(f"# HBM bytes per k_primal launch (calibrated): {res['hbm_bytes_per_launch']:.0f}")
-    with open(os.path.join(prof, "kernels_soft_block_21.txt"), "w") as f:
+    with open(os.path.join(prof, f"kernels_soft_block_{side}.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
-    with open(os.path.join(prof, "primal_soft_block_21_pmc.json"), "w") as f:
+    with open(os.path.join(prof, f"primal_soft_block_{side}_pmc.json"), "w") as f:
         json.dump(res, f, indent=1)
     print("\n".join(lines[-4:]))
 

@@ -189,7 +189,7 @@ def build(workload, cells):
     s = demos.soft_block_scene(cells=cells)
     desc = (f"soft block {cells}^3 cells, {s.numDynamic} vertices, {len(s.tets)} SNH tets, "
             f"{s.settings.iterations} iterations, dt 1/60, static coloring")
-    return s, desc, "vertex-iters/s", "primal_soft_block_21_pmc.json"
+    return s, desc, "vertex-iters/s", f"primal_soft_block_{cells + 1}_pmc.json"
 
 
 def main():

@@ -335,10 +335,11 @@ avbd_status avbd_last_failure(const avbd_solver* s, avbd_failure* out);
 avbd_status avbd_get_profile(avbd_solver* s, int32_t enable, char* names, uint32_t names_cap,
                              float* ms, uint32_t* n_stages);
 
-/* Times `reps` full primal color sweeps (every color once) on the solver's stream
- * with HIP events and reports the average duration of one per-color primal
- * launch and the algorithmic bytes one sweep touches (per-stamp record model).
- * Mutates positions (runs extra sweeps); call it after the timed region. */
+/* Times `reps` full primal color sweeps (every color once) on the solver's stream:
+ * the sweeps are captured in a hipGraph and one replay is bracketed by HIP
+ * events; reports the average duration of one per-color primal launch and the
+ * algorithmic bytes one sweep touches (per-stamp record model).  Mutates
+ * positions (runs extra sweeps); call it after the timed region. */
 avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us,
                              double* sweep_bytes, int32_t* launches_per_sweep);
 

@@ -50,9 +50,30 @@ def _cloud(n=300, seed=11):
     return s
 
 
+def _granular(n=1500, seed=12):
+    """Overlapping free particles (r = 1 cm) with small boxes and spheres:
+    hash cells (0.1 m, the radius floor) hold dozens of colliders, so pairs
+    come from the candidate search grid."""
+    rng = demos.SplitMix64(seed)
+    s = demos.PhysicsScene("granular", SimSettings(iterations=2))
+    demos.add_ground(s)
+    for i in range(n):
+        p = (rng.uniform(-0.12, 0.12), rng.uniform(-0.12, 0.12), rng.uniform(0.0, 0.4))
+        s.addParticle(0.01, 0.001, p)
+    for i in range(50):
+        p = (rng.uniform(-0.12, 0.12), rng.uniform(-0.12, 0.12), rng.uniform(0.0, 0.4))
+        if i % 2:
+            s.addSphere(0.03, 500.0, 0.5, p)
+        else:
+            s.addBody((0.02, 0.02, 0.02), 500.0, 0.5, p)
+    return s
+
+
 def test_hash_cells_and_pairs_bitexact():
     for name, s in [("stack 10", demos.stack_scene(10)), ("soft block 9^3", _block(8, 5)),
-                    ("soft block 21^3", _block(20, 2)), ("rigid cloud 300", _cloud())]:
+                    ("soft block 21^3", _block(20, 2)), ("rigid cloud 300", _cloud()),
+                    ("granular 1500 + 50 small rigid (search grid)", _granular()),
+                    ("soft block 41^3 (search grid, 64-collider buckets)", _block(40, 1))]:
         _case(f"hash + pairs, {name}")
         g = GPUSolver(s)
         o = oracle.OracleSolver(s)
