@@ -2,7 +2,8 @@
 # Profiling recipe behind profiles/*soft_block_21* (CELLS=20, default) and
 # profiles/*soft_block_56* (CELLS=55, the 1M-tet block): kernel trace of the
 # benchmark, then one rocprofv3 PMC pass per counter (FETCH_SIZE, WRITE_SIZE)
-# on the benchmark and on the copy calibration kernel.  Run from the repo
+# on the benchmark and on the copy calibration kernel, then one pass of wave,
+# occupancy and LDS counters.  Run from the repo
 # root on the GPU host; writes raw output under gpurun_out/prof_* and the
 # summaries via summarize.py.
 set -u
@@ -23,4 +24,8 @@ for C in FETCH_SIZE WRITE_SIZE; do
   timeout -s KILL 60 rocprofv3 --pmc $C --output-format csv -d $OUT/prof_$C -o copy -- ./avbd-metal_amd/lib/copy_calib > $OUT/prof_${C}_copy.log 2>&1
   rc=$?; tail -1 $OUT/prof_${C}_copy.log; [ -f $OUT/prof_$C/copy_counter_collection.csv ] || exit $rc
 done
+echo "pass: wave / LDS counters"
+timeout -s KILL 120 rocprofv3 --pmc SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT GRBM_GUI_ACTIVE \
+  --output-format csv -d $OUT/prof_SQ -o bench -- python3 $BENCH > $OUT/prof_SQ.log 2>&1
+rc=$?; tail -1 $OUT/prof_SQ.log; [ -f $OUT/prof_SQ/bench_counter_collection.csv ] || exit $rc
 python3 avbd-metal_amd/prof/summarize.py "$OUT" "$BENCH" $((CELLS + 1))

@@ -76,6 +76,18 @@ def main():
                   "primal_bytes": b * 1024.0 * factor if (b is not None and factor) else None}
         lines.append(f"# {c}: k_primal avg {b} KiB over {nb} dispatches; copy kernel {k} KiB for {COPY_BYTES} "
                      f"bytes -> factor {factor}")
+    sq_path = os.path.join(out, "prof_SQ", "bench_counter_collection.csv")
+    if os.path.exists(sq_path):
+        sq = {}
+        for c in ("SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+                  "GRBM_GUI_ACTIVE"):
+            v, nv = counter_avg(sq_path, c, "k_primal")
+            sq[c] = v
+        res["primal_wave_counters_avg_per_dispatch"] = sq
+        if sq.get("SQ_WAVE_CYCLES") and sq.get("SQ_BUSY_CYCLES"):
+            # waves resident per busy SQ cycle, summed over the SQs that count them
+            res["primal_waves_per_busy_cycle"] = sq["SQ_WAVE_CYCLES"] / sq["SQ_BUSY_CYCLES"]
+        lines.append("# k_primal per dispatch: " + ", ".join(f"{k} {v:.0f}" for k, v in sq.items() if v is not None))
     if res["FETCH_SIZE"]["primal_bytes"] is not None and res["WRITE_SIZE"]["primal_bytes"] is not None:
         res["hbm_bytes_per_launch"] = res["FETCH_SIZE"]["primal_bytes"] + res["WRITE_SIZE"]["primal_bytes"]
         lines.append(f"# HBM bytes per k_primal launch (calibrated): {res['hbm_bytes_per_launch']:.0f}")

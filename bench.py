@@ -64,18 +64,19 @@ def cpu_baseline(scene, unit, target_s=10.0):
             "sample": f"{n} steps of the same workload after 2 warm-up steps, serial oracle FP32, {dt:.2f} s"}
 
 
-def learned_comparison(args, nv):
-    """Learned vs plain warm start on the metric's scene: training pairs from
-    the plain solver on other seeds of the same block (every frame iterated to
-    residual 2e-4 or 200 sweeps), GPU training, then frames iterated to
-    `tol` with and without the learned start (plain fallback active), the
-    residual after the fixed iteration count, and the MLP forward roofline."""
+def learned_comparison(args):
+    """Learned vs plain warm start: training pairs from the plain solver on
+    other seeds of the --train-cells block (every frame iterated to residual
+    2e-4 or 200 sweeps), GPU training, then frames of the --eval-cells block
+    (default: the metric's scene) iterated to `tol` with and without the
+    learned start (plain fallback active), the residual after the fixed
+    iteration count, and the MLP forward roofline on the evaluated rows."""
     import numpy as np
     from avbd_amd import GPUSolver, WarmStartMLP, demos
     from avbd_amd.learned import collect_samples
 
     t0 = time.perf_counter()
-    train = [demos.soft_block_scene(cells=args.cells, seed=10 + k) for k in range(args.train_seeds)]
+    train = [demos.soft_block_scene(cells=args.train_cells, seed=10 + k) for k in range(args.train_seeds)]
     X, T, label_its = collect_samples(train, args.train_frames)
     t_collect = time.perf_counter() - t0
     m = WarmStartMLP(seed=1)
@@ -83,16 +84,21 @@ def learned_comparison(args, nv):
     losses = m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
     t_train = time.perf_counter() - t0
 
+    eval_cells = args.eval_cells or args.cells
+    eval_scene = demos.soft_block_scene(cells=eval_cells)
+    nv = eval_scene.numDynamic
+
     def run(learned):
-        s = demos.soft_block_scene(cells=args.cells)
+        s = eval_scene
         g = GPUSolver(s)
         if learned:
             g.setWarmStart(m, smooth=2)
-        its, acc = [], []
+        its, acc, reached = [], [], 0
         t0 = time.perf_counter()
         for _ in range(args.learned_frames):
-            n, _ = g.stepToResidual(args.learned_tol, 200, 5)
+            n, r = g.stepToResidual(args.learned_tol, 200, 5)
             its.append(n)
+            reached += r <= args.learned_tol
             acc.append(g.stats().ws_accepted)
         wall = time.perf_counter() - t0
         g2 = GPUSolver(s)
@@ -103,6 +109,7 @@ def learned_comparison(args, nv):
             g2.step()
             res.append(g2.residual())
         return {"iterations_mean": round(float(np.mean(its)), 2), "ms_per_step": round(wall * 1000 / len(its), 4),
+                "reached_fraction": reached / len(its),
                 "learned_start_kept": round(float(np.mean(acc)), 3) if learned else None,
                 "residual_after_fixed_iterations_mean": float(np.mean(res)),
                 "residual_after_fixed_iterations_median": float(np.median(res))}
@@ -111,7 +118,9 @@ def learned_comparison(args, nv):
     us = m.timeForward(nv, reps=50)
     flops = 2.0 * nv * (24 * 128 + 128 * 128 + 128 * 3)
     tflops = flops / (us * 1e-6) / 1e12
-    out = {"tolerance": args.learned_tol, "frames": args.learned_frames, "max_iterations": 200,
+    out = {"evaluated_on": f"soft block {eval_cells}^3 cells ({nv} vertices, {len(eval_scene.tets)} tets)",
+           "trained_on": f"soft block {args.train_cells}^3 cells, seeds 10..{9 + args.train_seeds}",
+           "tolerance": args.learned_tol, "frames": args.learned_frames, "max_iterations": 200,
            "training": {"samples": int(len(X)), "scenes": f"{args.train_seeds} seeds x {args.train_frames} frames",
                         "label_sweeps_mean": round(label_its, 1), "collect_s": round(t_collect, 2),
                         "epochs": args.epochs, "train_s": round(t_train, 2),
@@ -207,6 +216,8 @@ def main():
     ap.add_argument("--train-seeds", type=int, default=3)
     ap.add_argument("--train-frames", type=int, default=60)
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--train-cells", type=int, default=20, help="block of the learned start's training data")
+    ap.add_argument("--eval-cells", type=int, default=0, help="block it is evaluated on (0 = --cells)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="uneven soft blocks of the batched comparison")
     ap.add_argument("--batch-frames", type=int, default=30)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
@@ -273,7 +284,7 @@ def main():
                            "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
                            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
     if args.workload == "soft_block" and not args.no_learned:
-        cmp, roof = learned_comparison(args, nv)
+        cmp, roof = learned_comparison(args)
         out["config"]["learned_vs_plain"] = cmp
         out["roofline_mlp"] = roof
         out["config"]["batched_learned_vs_plain"] = batched_learned_comparison(args)

@@ -200,3 +200,39 @@ def test_learned_stage_graph_matches_eager():
     xa, ta, _ = a.warmStartSamples()
     xb, tb, _ = b.warmStartSamples()
     assert np.array_equal(xa, xb) and np.array_equal(ta, tb)
+
+
+def test_trained_network_gains_over_plain_warm_start():
+    _case("learned warm start trained on the GPU vs the plain guess, 9^3 block")
+    from avbd_amd.learned import collect_samples
+    cells = 8
+    X, T, label_its = collect_samples([demos.soft_block_scene(cells=cells, seed=20 + k) for k in range(3)], 60)
+    m = WarmStartMLP(seed=1)
+    losses = m.train(X, T, epochs=40, batch=4096, lr=1e-3, seed=2)
+    flush_print(f"  {len(X)} pairs (label sweeps {label_its:.0f}), loss {losses[0]:.3f} -> {losses[-1]:.3f}, "
+                f"target variance {T.var():.3f}")
+    assert losses[-1] < 0.3 * T.var()
+    s = demos.soft_block_scene(cells=cells, seed=99)     # a scene the network never saw
+
+    def residuals(learned):
+        g = GPUSolver(s)
+        if learned:
+            g.setWarmStart(m)
+        out = []
+        for _ in range(60):
+            g.step()
+            out.append(g.residual())
+        return np.array(out)
+
+    def iterations(learned):
+        g = GPUSolver(s)
+        if learned:
+            g.setWarmStart(m)
+        return np.array([g.stepToResidual(1e-3, 200, 5)[0] for _ in range(60)])
+
+    rp, rl = residuals(False), residuals(True)
+    ip, il = iterations(False), iterations(True)
+    flush_print(f"  residual after 20 sweeps: median plain {np.median(rp):.3e} learned {np.median(rl):.3e}; "
+                f"sweeps to 1e-3: plain {ip.mean():.1f} learned {il.mean():.1f}")
+    assert np.median(rl) < 0.6 * np.median(rp)
+    assert il.mean() < ip.mean()
