@@ -134,7 +134,8 @@ SIGNATURES = {
     "avbd_get_profile": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_uint32, FP, U32P]),
     "avbd_time_primal": (C.c_int, [C.c_void_p, C.c_int32, FP, C.POINTER(C.c_double), I32P]),
     "avbd_test_deny_stage": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
-    "avbd_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "avbd_mlp_create": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "avbd_mlp_info": (C.c_int, [C.c_void_p, I32P, I32P]),
     "avbd_mlp_destroy": (None, [C.c_void_p]),
     "avbd_mlp_get_params": (C.c_int, [C.c_void_p, FP]),
     "avbd_mlp_set_params": (C.c_int, [C.c_void_p, FP]),
@@ -145,13 +146,16 @@ SIGNATURES = {
     "avbd_mlp_time_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, FP]),
     "avbd_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "avbd_get_warm_start_samples": (C.c_int, [C.c_void_p, C.c_uint32, U32P, FP, FP, U32P]),
+    "avbd_get_warm_start_samples_rigid": (C.c_int, [C.c_void_p, C.c_uint32, U32P, FP, FP, U32P]),
     "avbd_version": (C.c_char_p, []),
 }
 
 # learned warm start (avbd.h)
 MLP_INPUTS = 24
 MLP_OUTPUTS = 3
+MLP_OUTPUTS_RIGID = 6
 MLP_PARAMS = 20099
+MLP_PARAMS_RIGID = 20486
 WS_LEARNED = 1
 WS_LOG = 2
 WS_FORCE = 4

@@ -282,33 +282,39 @@ class GPUSolver:
         return dict(avg_launch_us=us.value, sweep_bytes=by.value, launches_per_sweep=la.value)
 
     # -- learned warm start (no reference counterpart) ------------------------
-    def setWarmStart(self, mlp=None, learned=True, log=False, force=False, smooth=2):
-        """Attach a WarmStartMLP: learned=True replaces the adaptive guess of
-        the dynamic particles with guess + |g| dt^2 * MLP(features) (after
-        `smooth` passes of 1-ring smoothing of the prediction), keeping the
+    def setWarmStart(self, mlp=None, learned=True, log=False, force=False, smooth=2, rigid=None):
+        """Attach WarmStartMLP networks: `mlp` (3 outputs) serves the dynamic
+        particles, `rigid` (6 outputs) the dynamic rigid bodies.  learned=True
+        replaces their adaptive guess with guess + |g| dt^2 * MLP(features)
+        (particle predictions smoothed by `smooth` 1-ring passes), keeping the
         plain guess whenever the learned start's residual is higher
         (force=True keeps it unconditionally); log=True records each step's
         features and converged corrections (warmStartSamples).  No arguments
         detach."""
-        mode = (_capi.WS_LEARNED if (learned and mlp is not None) else 0) | (_capi.WS_LOG if log else 0)
-        if mode & _capi.WS_LEARNED and force:
-            mode |= _capi.WS_FORCE
-        if mode & _capi.WS_LEARNED:
-            mode |= (int(smooth) & 0xF) << _capi.WS_SMOOTH_SHIFT
-        h = mlp._h if (mlp is not None and mode & _capi.WS_LEARNED) else None
-        self._check(self._lib.avbd_set_warm_start(self._h, h, mode), "setWarmStart")
-        self._warm_mlp = mlp if h else None   # keeps the network alive while attached
+        nets = [n for n in (mlp, rigid) if n is not None]
+        flags = (_capi.WS_LEARNED if (learned and nets) else 0) | (_capi.WS_LOG if log else 0)
+        if flags & _capi.WS_LEARNED and force:
+            flags |= _capi.WS_FORCE
+        if flags & _capi.WS_LEARNED:
+            flags |= (int(smooth) & 0xF) << _capi.WS_SMOOTH_SHIFT
+        self._check(self._lib.avbd_set_warm_start(self._h, None, 0), "setWarmStart")   # detach previous
+        if flags & _capi.WS_LEARNED:
+            for n in nets:
+                self._check(self._lib.avbd_set_warm_start(self._h, n._h, flags), "setWarmStart")
+        elif flags:
+            self._check(self._lib.avbd_set_warm_start(self._h, None, flags), "setWarmStart")
+        self._warm_nets = nets if flags & _capi.WS_LEARNED else []   # keeps the networks alive while attached
 
-    def warmStartSamples(self):
-        """(features [n,24], corrections [n,3], body ids [n]) of the last frame."""
+    def warmStartSamples(self, rigid=False):
+        """(features [n,24], corrections [n,3] or [n,6] for rigid=True, body ids [n]) of the last frame."""
+        fn = self._lib.avbd_get_warm_start_samples_rigid if rigid else self._lib.avbd_get_warm_start_samples
+        w = _capi.MLP_OUTPUTS_RIGID if rigid else _capi.MLP_OUTPUTS
         n = C.c_uint32()
-        self._check(self._lib.avbd_get_warm_start_samples(self._h, 0, C.byref(n), None, None, None),
-                    "warmStartSamples")
+        self._check(fn(self._h, 0, C.byref(n), None, None, None), "warmStartSamples")
         x = np.zeros((n.value, _capi.MLP_INPUTS), np.float32)
-        t = np.zeros((n.value, _capi.MLP_OUTPUTS), np.float32)
+        t = np.zeros((n.value, w), np.float32)
         idx = np.zeros(n.value, np.uint32)
-        self._check(self._lib.avbd_get_warm_start_samples(self._h, n.value, C.byref(n), _fp(x), _fp(t), _u32p(idx)),
-                    "warmStartSamples")
+        self._check(fn(self._h, n.value, C.byref(n), _fp(x), _fp(t), _u32p(idx)), "warmStartSamples")
         return x, t, idx
 
     def setProfiling(self, on=True):

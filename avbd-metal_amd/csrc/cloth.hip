@@ -194,19 +194,20 @@ __device__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v,
   if ((entry & kElemEdgeBit) != 0u || entry >= P.numTris) return false;
   U4 t = S.tris[entry];
   if (t.x == v || t.y == v || t.z == v) return false;
-  uint32_t g = S.clothGroup[v];
-  bool same = g != 0u && S.clothGroup[t.x] == g && S.clothGroup[t.y] == g && S.clothGroup[t.z] == g;
-  if (same && S.selfCollision[v] == 0u) return false;
-  if (same && (nbr_contains(S.nbrList, ns, ne, t.x) || nbr_contains(S.nbrList, ns, ne, t.y) ||
-               nbr_contains(S.nbrList, ns, ne, t.z)))
-    return false;
   V3 lo, hi;
   elem_bounds(S, P, entry, lo, hi);
   if (!in_range(range_of(lo, hi, P.elemCellSize), cell)) return false;
   V3 bary;
   V3 q = closest_pt_triangle(p, P3(S, t.x), P3(S, t.y), P3(S, t.z), bary);
   V3 d = p - q;
-  return !(dot(d, d) > q2);
+  if (dot(d, d) > q2) return false;
+  // topology filters last: most candidates of a bucket are out of reach, and
+  // the 1-ring searches are the longest dependent-load chains
+  uint32_t g = S.clothGroup[v];
+  bool same = g != 0u && S.clothGroup[t.x] == g && S.clothGroup[t.y] == g && S.clothGroup[t.z] == g;
+  if (same && S.selfCollision[v] == 0u) return false;
+  return !(same && (nbr_contains(S.nbrList, ns, ne, t.x) || nbr_contains(S.nbrList, ns, ne, t.y) ||
+                    nbr_contains(S.nbrList, ns, ne, t.z)));
 }
 
 __device__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
@@ -217,12 +218,6 @@ __device__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gi
   if (other <= gid || other >= P.numEdges) return false;
   U2 b = S.edges[other];
   if (b.x == a.x || b.x == a.y || b.y == a.x || b.y == a.y) return false;
-  uint32_t g = S.clothGroup[a.x];
-  bool same = g != 0u && S.clothGroup[a.y] == g && S.clothGroup[b.x] == g && S.clothGroup[b.y] == g;
-  if (same && S.selfCollision[a.x] == 0u) return false;
-  if (same && (nbr_contains(S.nbrList, ns0, ne0, b.x) || nbr_contains(S.nbrList, ns0, ne0, b.y) ||
-               nbr_contains(S.nbrList, ns1, ne1, b.x) || nbr_contains(S.nbrList, ns1, ne1, b.y)))
-    return false;
   V3 blo, bhi;
   elem_bounds(S, P, P.numTris + other, blo, bhi);
   CellRange br = range_of(blo, bhi, P.elemCellSize);
@@ -235,7 +230,13 @@ __device__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gi
   closest_seg_seg(a0, a1, b0, b1, s, t);
   V3 ca = a0 + (a1 - a0) * s, cb = b0 + (b1 - b0) * t;
   V3 d = ca - cb;
-  return !(dot(d, d) > q2);
+  if (dot(d, d) > q2) return false;
+  // topology filters last (see vt_accept)
+  uint32_t g = S.clothGroup[a.x];
+  bool same = g != 0u && S.clothGroup[a.y] == g && S.clothGroup[b.x] == g && S.clothGroup[b.y] == g;
+  if (same && S.selfCollision[a.x] == 0u) return false;
+  return !(same && (nbr_contains(S.nbrList, ns0, ne0, b.x) || nbr_contains(S.nbrList, ns0, ne0, b.y) ||
+                    nbr_contains(S.nbrList, ns1, ne1, b.x) || nbr_contains(S.nbrList, ns1, ne1, b.y)));
 }
 
 // Visits producer `prod`'s candidates in chunks of `width` positions starting

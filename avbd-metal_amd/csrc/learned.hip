@@ -16,6 +16,15 @@
 //   12-14 c                       15-17 (<v_j> - v_n) dt / L
 //   18-20 <c_j> - c               21    min(contact entries, 8) / 8
 //   22   1-ring size / 16         23    |c|
+// Rigid bodies (a 6-output network; r the body's bounding radius, angles as
+// the displacement at r, c_lin / c_ang the previous corrections, <.> the
+// mean over joint partners):
+//   0-2  v_n dt / L               3-5   w_n dt r / L
+//   6-8  (v_n - v_{n-1}) dt / L   9-11  (x_guess - x_n) / L
+//   12-14 c_lin                   15-17 c_ang
+//   18-20 <c_lin_j> - c_lin       21    min(contact entries, 8) / 8
+//   22   min(joints, 4) / 4       23    |c_lin|
+// x_start = x_guess + L y_lin, q_start = q_guess (+) y_ang L / r.
 // The reference has no learned warm start; this stage is the backend's own.
 #include <utility>
 
@@ -119,6 +128,7 @@ __global__ void __launch_bounds__(256) k_ws_apply(DeviceState S, WarmState W, co
   S.posLinSnap[b] = p;
 }
 
+// thread i restores particle i and rigid body i when the plain guess wins
 __global__ void __launch_bounds__(256) k_ws_select(DeviceState S, WarmState W) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const float plain = W.res[0], learned = W.res[1];
@@ -129,10 +139,19 @@ __global__ void __launch_bounds__(256) k_ws_select(DeviceState S, WarmState W) {
     S.counters[CTR_WS_RES_PLAIN] = as_uint(sqrtf(plain / w2));
     S.counters[CTR_WS_RES_LEARNED] = as_uint(sqrtf(learned / w2));
   }
-  if (i >= W.n || keep) return;
-  const uint32_t b = W.idx[i];
-  S.posLin[b] = W.guess[i];
-  S.posLinSnap[b] = W.guess[i];
+  if (keep) return;
+  if (i < W.n) {
+    const uint32_t b = W.idx[i];
+    S.posLin[b] = W.guess[i];
+    S.posLinSnap[b] = W.guess[i];
+  }
+  if (i < W.nR) {
+    const uint32_t b = W.ridx[i];
+    S.posLin[b] = W.rGuessLin[i];
+    S.posLinSnap[b] = W.rGuessLin[i];
+    S.posAng[b] = W.rGuessAng[i];
+    S.posAngSnap[b] = W.rGuessAng[i];
+  }
 }
 
 __global__ void __launch_bounds__(256) k_ws_target(DeviceState S, WarmState W) {
@@ -145,6 +164,79 @@ __global__ void __launch_bounds__(256) k_ws_target(DeviceState S, WarmState W) {
     W.logT[3 * i] = t.x;
     W.logT[3 * i + 1] = t.y;
     W.logT[3 * i + 2] = t.z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rigid bodies
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rigid_radius(const DeviceState& S, uint32_t b) {
+  return fmaxf(S.shape[b].w, 1.0e-3f);
+}
+
+__global__ void __launch_bounds__(256) k_ws_features_rigid(DeviceState S, StepParams P, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.nR) return;
+  const uint32_t b = W.ridx[i];
+  const float invL = 1.0f / W.L, vs = P.dt * invL, ws = P.dt * rigid_radius(S, b) * invL;
+  const F4 g4 = S.posLin[b];
+  const V3 x = xyz(S.initLin[b]), v = xyz(S.velLin[b]), w = xyz(S.velAng[b]);
+  const V3 cl = xyz(W.corr[b]), ca = xyz(W.corrAng[b]);
+  V3 cSum = v3s(0.0f);
+  const uint32_t rs = W.rRingStart[i], rc = W.rRingCount[i];
+  for (uint32_t k = 0; k < rc; k++) cSum += xyz(W.corr[W.rRingList[rs + k]]);
+  const V3 f0 = v * vs;
+  const V3 f1 = w * ws;
+  const V3 f2 = (v - xyz(S.prevVelLin[b])) * vs;
+  const V3 f3 = (xyz(g4) - x) * invL;
+  const V3 f6 = rc ? cSum * (1.0f / (float)rc) - cl : v3s(0.0f);
+  const float f[kMlpIn] = {f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, f2.x, f2.y, f2.z, f3.x, f3.y, f3.z,
+                           cl.x, cl.y, cl.z, ca.x, ca.y, ca.z, f6.x, f6.y, f6.z,
+                           fminf((float)S.dAdjCount[b], 8.0f) * 0.125f, fminf((float)rc, 4.0f) * 0.25f, length(cl)};
+  bf16x8 row[4];
+#pragma unroll
+  for (int k = 0; k < 32; k++) row[k / 8][k % 8] = (bf16)(k < kMlpIn ? f[k] : 0.0f);
+#pragma unroll
+  for (int q = 0; q < 4; q++) *(bf16x8*)(W.xR + (size_t)i * kMlpInPad + 8 * q) = row[q];
+  if (W.log) {
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) W.logXR[(size_t)i * kMlpIn + k] = f[k];
+  }
+  W.rGuessLin[i] = g4;
+  W.rGuessAng[i] = S.posAng[b];
+}
+
+__global__ void __launch_bounds__(256) k_ws_apply_rigid(DeviceState S, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.nR) return;
+  const uint32_t b = W.ridx[i];
+  const float* y = W.yR + (size_t)i * kMlpOutRigid;
+  const F4 g = W.rGuessLin[i];
+  const F4 p = f4(xyz(g) + v3(y[0], y[1], y[2]) * W.L, g.w);
+  const F4 q = q_addw(W.rGuessAng[i], v3(y[3], y[4], y[5]) * (W.L / rigid_radius(S, b)));
+  S.posLin[b] = p;
+  S.posLinSnap[b] = p;
+  S.posAng[b] = q;
+  S.posAngSnap[b] = q;
+}
+
+__global__ void __launch_bounds__(256) k_ws_target_rigid(DeviceState S, WarmState W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.nR) return;
+  const uint32_t b = W.ridx[i];
+  const float invL = 1.0f / W.L;
+  const V3 tl = (xyz(S.posLin[b]) - xyz(W.rGuessLin[i])) * invL;
+  const V3 ta = q_sub(S.posAng[b], W.rGuessAng[i]) * (rigid_radius(S, b) * invL);
+  W.corr[b] = f4(tl, 0.0f);
+  W.corrAng[b] = f4(ta, 0.0f);
+  if (W.log) {
+    float* t = W.logTR + (size_t)i * kMlpOutRigid;
+    t[0] = tl.x;
+    t[1] = tl.y;
+    t[2] = tl.z;
+    t[3] = ta.x;
+    t[4] = ta.y;
+    t[5] = ta.z;
   }
 }
 
@@ -167,7 +259,16 @@ void launch_ws_apply(hipStream_t st, const DeviceState& S, const WarmState& W) {
   hipLaunchKernelGGL(k_ws_apply, grid_n(W.n), dim3(256), 0, st, S, W, (const F4*)a);
 }
 void launch_ws_select(hipStream_t st, const DeviceState& S, const WarmState& W) {
-  hipLaunchKernelGGL(k_ws_select, grid_n(W.n), dim3(256), 0, st, S, W);
+  hipLaunchKernelGGL(k_ws_select, grid_n(W.n > W.nR ? W.n : W.nR), dim3(256), 0, st, S, W);
+}
+void launch_ws_features_rigid(hipStream_t st, const DeviceState& S, const StepParams& P, const WarmState& W) {
+  if (W.nR) hipLaunchKernelGGL(k_ws_features_rigid, grid_n(W.nR), dim3(256), 0, st, S, P, W);
+}
+void launch_ws_apply_rigid(hipStream_t st, const DeviceState& S, const WarmState& W) {
+  if (W.nR) hipLaunchKernelGGL(k_ws_apply_rigid, grid_n(W.nR), dim3(256), 0, st, S, W);
+}
+void launch_ws_target_rigid(hipStream_t st, const DeviceState& S, const WarmState& W) {
+  if (W.nR) hipLaunchKernelGGL(k_ws_target_rigid, grid_n(W.nR), dim3(256), 0, st, S, W);
 }
 void launch_ws_target(hipStream_t st, const DeviceState& S, const WarmState& W) {
   if (W.n) hipLaunchKernelGGL(k_ws_target, grid_n(W.n), dim3(256), 0, st, S, W);

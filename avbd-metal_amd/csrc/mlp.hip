@@ -1,5 +1,5 @@
 // mlp.hip -- the learned warm-start network on gfx950: per-row MLP
-// 24 -> 128 -> 128 -> 3 (ReLU) with BF16 weights and activations and FP32
+// 24 -> 128 -> 128 -> 3 or 6 (ReLU) with BF16 weights and activations and FP32
 // accumulation on the CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16):
 // hand-written forward, activation-gradient and split-K weight-gradient
 // kernels, and Adam on the f32 master copy.
@@ -27,7 +27,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int kIn = kMlpIn, kInPad = kMlpInPad, kHid = kMlpHidden, kOut = kMlpOut, kOutPad = kMlpOutPad;
+constexpr int kIn = kMlpIn, kInPad = kMlpInPad, kHid = kMlpHidden, kOutPad = kMlpOutPad;
 // LDS row strides (bf16 elements): 80 B / 272 B / 48 B rows put the 16 rows
 // of one operand fetch on 16 distinct 4-bank groups
 constexpr int kS1 = 40, kSH = 136, kS3 = 24, kSD = 40;
@@ -142,7 +142,7 @@ __device__ void store_transposed(const bf16* h, int stride, int features, bf16* 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// inference: Y[r][0..2] = MLP(X[r]), X bf16 [rows][32], Y f32 [rows][3];
+// inference: Y[r][0..nOut) = MLP(X[r]), X bf16 [rows][32], Y f32 [rows][nOut];
 // 8 waves share one LDS copy of the weights
 // ---------------------------------------------------------------------------
 constexpr int kFwdWaves = 8;
@@ -157,11 +157,11 @@ __global__ void __launch_bounds__(64 * kFwdWaves) k_mlp_forward(MlpDevice M, con
     const uint32_t r0 = t * 16;
     f32x4 y = tile_forward(M, W, X, rows, r0, hbuf[wv][0], hbuf[wv][1], lane);
     const uint32_t col = lane & 15u;
-    if (col < (uint32_t)kOut) {
+    if (col < M.nOut) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         uint32_t r = r0 + 4 * (lane >> 4) + i;
-        if (r < rows) Y[(size_t)r * kOut + col] = y[i];
+        if (r < rows) Y[(size_t)r * M.nOut + col] = y[i];
       }
     }
     wave_sync();  // the next tile overwrites h1 / h2
@@ -195,13 +195,13 @@ __global__ void __launch_bounds__(256) k_mlp_act_grad(MlpDevice M, MlpTrain T, c
   for (uint32_t t = blockIdx.x * 4 + wv; t < tiles; t += gridDim.x * 4) {
     const uint32_t r0 = t * 16;
     f32x4 y = tile_forward(M, W, X, rows, r0, h1, h2, lane);
-    // dY = 2 (y - t) lossScale (D layout; columns >= 3 and padded rows are zero)
+    // dY = 2 (y - t) lossScale (D layout; columns >= nOut and padded rows are zero)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       uint32_t r = r0 + dr + i;
       float g = 0.0f;
-      if (r < rows && ar < (uint32_t)kOut) {
-        float e = y[i] - Tgt[(size_t)r * kOut + ar];
+      if (r < rows && ar < M.nOut) {
+        float e = y[i] - Tgt[(size_t)r * M.nOut + ar];
         loss += e * e;
         g = 2.0f * e * lossScale;
       }
@@ -309,18 +309,19 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(MlpTrain T, uint32_t samplesP
 }
 
 // gradients in the master layout from the GEMM outputs (padding dropped)
-__global__ void __launch_bounds__(256) k_mlp_gather_grad(MlpTrain T) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)kMlpParams; i += gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(256) k_mlp_gather_grad(MlpTrain T, uint32_t nOut) {
+  const uint32_t offB3 = (uint32_t)mlp_off_b3((int)nOut), n = (uint32_t)mlp_params((int)nOut);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float g;
     if (i < (uint32_t)kMlpOffB1) g = T.gw1[i];  // rows 0..23 of gw1 [32][128]
     else if (i < (uint32_t)kMlpOffW2) g = T.gb1[i - kMlpOffB1];
     else if (i < (uint32_t)kMlpOffB2) g = T.gw2[i - kMlpOffW2];
     else if (i < (uint32_t)kMlpOffW3) g = T.gb2[i - kMlpOffB2];
-    else if (i < (uint32_t)kMlpOffB3) {
+    else if (i < offB3) {
       uint32_t q = i - kMlpOffW3;
-      g = T.gw3[(q / kOut) * kOutPad + q % kOut];
+      g = T.gw3[(q / nOut) * kOutPad + q % nOut];
     } else {
-      g = T.gb3[i - kMlpOffB3];
+      g = T.gb3[i - offB3];
     }
     T.grad[i] = g;
   }
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(256) k_mlp_zero(MlpTrain T) {
 // Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the f32 master parameters
 __global__ void __launch_bounds__(256) k_mlp_adam(MlpDevice M, MlpTrain T, float lr, float b1c, float b2c) {
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)kMlpParams; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M.nParams; i += gridDim.x * blockDim.x) {
     float g = T.grad[i];
     float m = beta1 * T.m[i] + (1.0f - beta1) * g;
     float v = beta2 * T.v[i] + (1.0f - beta2) * g * g;
@@ -366,14 +367,14 @@ __global__ void __launch_bounds__(256) k_mlp_pack(MlpDevice M) {
       M.w1t[b * kInPad + a] = (bf16)w1;
     }
     if (b < (uint32_t)kOutPad) {
-      const float w3 = b < (uint32_t)kOut ? p[kMlpOffW3 + a * kOut + b] : 0.0f;  // W3[in a][out b]
+      const float w3 = b < M.nOut ? p[kMlpOffW3 + a * M.nOut + b] : 0.0f;  // W3[in a][out b]
       M.w3[a * kOutPad + b] = (bf16)w3;
       M.w3t[b * kHid + a] = (bf16)w3;
     }
     if (a == 0) {
       M.b1[b] = p[kMlpOffB1 + b];
       M.b2[b] = p[kMlpOffB2 + b];
-      if (b < (uint32_t)kOutPad) M.b3[b] = b < (uint32_t)kOut ? p[kMlpOffB3 + b] : 0.0f;
+      if (b < (uint32_t)kOutPad) M.b3[b] = b < M.nOut ? p[mlp_off_b3((int)M.nOut) + b] : 0.0f;
     }
   }
 }
@@ -386,13 +387,13 @@ __global__ void __launch_bounds__(256) k_mlp_pack_input(const float* Xf, uint32_
   }
 }
 
-// batch rows by a permutation: four 16-byte chunks of X and three floats of T per row
+// batch rows by a permutation: four 16-byte chunks of X and the nOut targets per row
 __global__ void __launch_bounds__(256) k_mlp_gather_batch(const bf16* Xall, const float* Tall, const uint32_t* perm,
-                                                          uint32_t n, bf16* X, float* T) {
+                                                          uint32_t n, uint32_t nOut, bf16* X, float* T) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 4 * n; i += gridDim.x * blockDim.x) {
     const uint32_t r = i >> 2, c = i & 3u, src = perm[r];
     *(bf16x8*)(X + (size_t)r * kInPad + 8 * c) = *(const bf16x8*)(Xall + (size_t)src * kInPad + 8 * c);
-    if (c < (uint32_t)kOut) T[(size_t)r * kOut + c] = Tall[(size_t)src * kOut + c];
+    for (uint32_t k = c; k < nOut; k += 4) T[(size_t)r * nOut + k] = Tall[(size_t)src * nOut + k];
   }
 }
 
@@ -417,10 +418,10 @@ void launch_mlp_pack_input(hipStream_t st, const float* Xf, uint32_t n, void* X)
   hipLaunchKernelGGL(k_mlp_pack_input, dim3(grid_for(n * kInPad, 2048)), dim3(256), 0, st, Xf, n, (bf16*)X);
 }
 void launch_mlp_gather_batch(hipStream_t st, const void* Xall, const float* Tall, const uint32_t* perm, uint32_t n,
-                             void* X, float* T) {
+                             uint32_t nOut, void* X, float* T) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_mlp_gather_batch, dim3(grid_for(4 * n, 2048)), dim3(256), 0, st, (const bf16*)Xall, Tall,
-                     perm, n, (bf16*)X, T);
+                     perm, n, nOut, (bf16*)X, T);
 }
 void launch_mlp_forward(hipStream_t st, const MlpDevice& M, const void* X, uint32_t rows, float* Y, uint32_t cus) {
   if (rows == 0) return;
@@ -435,7 +436,7 @@ void launch_mlp_grad(hipStream_t st, const MlpDevice& M, const MlpTrain& T, cons
   const uint32_t tiles = T.rowsPad / 16;
   uint32_t blocks = (tiles + 3) / 4;
   if (blocks > cus) blocks = cus;
-  const float scale = 1.0f / (3.0f * (float)(rows > 0 ? rows : 1));
+  const float scale = 1.0f / ((float)M.nOut * (float)(rows > 0 ? rows : 1));
   hipLaunchKernelGGL(k_mlp_act_grad, dim3(blocks), dim3(256), 0, st, M, T, (const bf16*)X, Tgt, rows, scale);
   // split-K: at least 1024 samples per block, about one block per CU and tile group
   const uint32_t P = T.rowsPad;
@@ -444,14 +445,14 @@ void launch_mlp_grad(hipStream_t st, const MlpDevice& M, const MlpTrain& T, cons
   if (spb < 1024) spb = 1024;
   const uint32_t kb = (P + spb - 1) / spb;
   hipLaunchKernelGGL(k_mlp_wgrad, dim3(kb, (kGradTiles + 3) / 4), dim3(256), 0, st, T, spb);
-  hipLaunchKernelGGL(k_mlp_gather_grad, dim3(grid_for(kMlpParams, 256)), dim3(256), 0, st, T);
+  hipLaunchKernelGGL(k_mlp_gather_grad, dim3(grid_for(M.nParams, 256)), dim3(256), 0, st, T, M.nOut);
 }
 void launch_mlp_accum_loss(hipStream_t st, const float* loss, float* dst, float scale) {
   hipLaunchKernelGGL(k_mlp_accum_loss, dim3(1), dim3(64), 0, st, loss, dst, scale);
 }
 void launch_mlp_adam(hipStream_t st, const MlpDevice& M, const MlpTrain& T, float lr, int step) {
   const float b1c = 1.0f - powf(0.9f, (float)step), b2c = 1.0f - powf(0.999f, (float)step);
-  hipLaunchKernelGGL(k_mlp_adam, dim3(grid_for(kMlpParams, 256)), dim3(256), 0, st, M, T, lr, b1c, b2c);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(grid_for(M.nParams, 256)), dim3(256), 0, st, M, T, lr, b1c, b2c);
   launch_mlp_pack(st, M);
 }
 

@@ -71,7 +71,7 @@ hipError_t ensure_stage(avbd_mlp* m, size_t rows) {
   hipError_t e;
   __bf16* xb = nullptr;
   if ((e = grab(m->stageAllocs, &m->xf, rows * kMlpIn)) || (e = grab(m->stageAllocs, &xb, rows * kMlpInPad)) ||
-      (e = grab(m->stageAllocs, &m->yf, rows * kMlpOut)) || (e = grab(m->stageAllocs, &m->tf, rows * kMlpOut)))
+      (e = grab(m->stageAllocs, &m->yf, rows * m->M.nOut)) || (e = grab(m->stageAllocs, &m->tf, rows * m->M.nOut)))
     return e;
   m->xb = xb;
   m->stageRows = rows;
@@ -88,8 +88,8 @@ avbd_status upload_rows(avbd_mlp* m, const float* x, uint32_t n) {
 
 }  // namespace
 
-extern "C" avbd_status avbd_mlp_create(uint64_t seed, avbd_mlp** out) {
-  if (!out) return AVBD_ERR_INVALID_ARGUMENT;
+extern "C" avbd_status avbd_mlp_create(uint64_t seed, int32_t outputs, avbd_mlp** out) {
+  if (!out || (outputs != kMlpOut && outputs != kMlpOutRigid)) return AVBD_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return AVBD_ERR_NO_DEVICE;
@@ -105,16 +105,19 @@ extern "C" avbd_status avbd_mlp_create(uint64_t seed, avbd_mlp** out) {
   m->cus = (uint32_t)std::max(cus, 1);
   MlpDevice& M = m->M;
   MlpTrain& T = m->T;
+  M.nOut = (uint32_t)outputs;
+  M.nParams = (uint32_t)mlp_params(outputs);
+  const size_t np = M.nParams;
   auto& L = m->allocs;
-  if (grab(L, &M.params, kMlpParams) || grab(L, &M.w1t, kMlpHidden * kMlpInPad) ||
+  if (grab(L, &M.params, np) || grab(L, &M.w1t, kMlpHidden * kMlpInPad) ||
       grab(L, &M.w2t, kMlpHidden * kMlpHidden) || grab(L, &M.w3t, kMlpOutPad * kMlpHidden) ||
       grab(L, &M.w2, kMlpHidden * kMlpHidden) || grab(L, &M.w3, kMlpHidden * kMlpOutPad) ||
       grab(L, &M.b1, kMlpHidden) || grab(L, &M.b2, kMlpHidden) || grab(L, &M.b3, kMlpOutPad) ||
-      grab(L, &T.grad, kMlpParams) || grab(L, &T.m, kMlpParams) || grab(L, &T.v, kMlpParams))
+      grab(L, &T.grad, np) || grab(L, &T.m, np) || grab(L, &T.v, np))
     return fail(AVBD_ERR_ALLOC);
   // He-uniform weights (bound sqrt(6 / fan_in)); the output layer scaled by
   // 0.1 so an untrained network starts near the plain guess; zero biases
-  std::vector<float> p(kMlpParams, 0.0f);
+  std::vector<float> p(np, 0.0f);
   uint64_t s = seed;
   auto fill = [&](int off, int fanIn, int count, float scale) {
     const float a = scale * std::sqrt(6.0f / (float)fanIn);
@@ -122,7 +125,7 @@ extern "C" avbd_status avbd_mlp_create(uint64_t seed, avbd_mlp** out) {
   };
   fill(kMlpOffW1, kMlpIn, kMlpIn * kMlpHidden, 1.0f);
   fill(kMlpOffW2, kMlpHidden, kMlpHidden * kMlpHidden, 1.0f);
-  fill(kMlpOffW3, kMlpHidden, kMlpHidden * kMlpOut, 0.1f);
+  fill(kMlpOffW3, kMlpHidden, kMlpHidden * outputs, 0.1f);
   avbd_status st = avbd_mlp_set_params(m, p.data());
   if (st != AVBD_OK) return fail(st);
   *out = m;
@@ -139,20 +142,29 @@ extern "C" void avbd_mlp_destroy(avbd_mlp* m) {
   delete m;
 }
 
+extern "C" avbd_status avbd_mlp_info(const avbd_mlp* m, int32_t* outputs, int32_t* num_params) {
+  if (!m) return AVBD_ERR_INVALID_ARGUMENT;
+  if (outputs) *outputs = (int32_t)m->M.nOut;
+  if (num_params) *num_params = (int32_t)m->M.nParams;
+  return AVBD_OK;
+}
+
 extern "C" avbd_status avbd_mlp_get_params(avbd_mlp* m, float* params) {
   if (!m || !params) return AVBD_ERR_INVALID_ARGUMENT;
   if (hipStreamSynchronize(m->stream) != hipSuccess) return AVBD_ERR_HIP;
-  if (hipMemcpy(params, m->M.params, kMlpParams * 4, hipMemcpyDeviceToHost) != hipSuccess) return AVBD_ERR_HIP;
+  if (hipMemcpy(params, m->M.params, (size_t)m->M.nParams * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return AVBD_ERR_HIP;
   return AVBD_OK;
 }
 
 extern "C" avbd_status avbd_mlp_set_params(avbd_mlp* m, const float* params) {
   if (!m || !params) return AVBD_ERR_INVALID_ARGUMENT;
-  for (int i = 0; i < kMlpParams; i++)
+  const size_t np = m->M.nParams;
+  for (size_t i = 0; i < np; i++)
     if (!std::isfinite(params[i])) return AVBD_ERR_INVALID_ARGUMENT;
   if (hipStreamSynchronize(m->stream) != hipSuccess) return AVBD_ERR_HIP;
-  if (hipMemcpy(m->M.params, params, kMlpParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(m->T.m, 0, kMlpParams * 4) != hipSuccess || hipMemset(m->T.v, 0, kMlpParams * 4) != hipSuccess)
+  if (hipMemcpy(m->M.params, params, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->T.m, 0, np * 4) != hipSuccess || hipMemset(m->T.v, 0, np * 4) != hipSuccess)
     return AVBD_ERR_HIP;
   m->adamStep = 0;
   launch_mlp_pack(m->stream, m->M);
@@ -166,7 +178,7 @@ extern "C" avbd_status avbd_mlp_forward(avbd_mlp* m, const float* x, uint32_t n,
   avbd_status st = upload_rows(m, x, n);
   if (st != AVBD_OK) return st;
   launch_mlp_forward(m->stream, m->M, m->xb, n, m->yf, m->cus);
-  if (hipMemcpyAsync(y, m->yf, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+  if (hipMemcpyAsync(y, m->yf, (size_t)n * m->M.nOut * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
       hipStreamSynchronize(m->stream) != hipSuccess)
     return AVBD_ERR_HIP;
   return AVBD_OK;
@@ -179,17 +191,17 @@ extern "C" avbd_status avbd_mlp_loss_grad(avbd_mlp* m, const float* x, const flo
   if (ensure_train(m, P) != hipSuccess) return AVBD_ERR_ALLOC;
   avbd_status st = upload_rows(m, x, n);
   if (st != AVBD_OK) return st;
-  if (hipMemcpyAsync(m->tf, t, (size_t)n * kMlpOut * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+  if (hipMemcpyAsync(m->tf, t, (size_t)n * m->M.nOut * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess)
     return AVBD_ERR_HIP;
   MlpTrain T = m->T;
   T.rowsPad = P;
   launch_mlp_grad(m->stream, m->M, T, m->xb, m->tf, n, m->cus);
   float l = 0.0f;
   if (hipMemcpyAsync(&l, T.loss, 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
-      (grad && hipMemcpyAsync(grad, T.grad, kMlpParams * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess) ||
+      (grad && hipMemcpyAsync(grad, T.grad, (size_t)m->M.nParams * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess) ||
       hipStreamSynchronize(m->stream) != hipSuccess)
     return AVBD_ERR_HIP;
-  if (loss) *loss = l / (3.0f * (float)n);
+  if (loss) *loss = l / ((float)m->M.nOut * (float)n);
   return AVBD_OK;
 }
 
@@ -210,9 +222,10 @@ extern "C" avbd_status avbd_mlp_train(avbd_mlp* m, const float* x, const float* 
     release(local);
     return st;
   };
+  const uint32_t no = m->M.nOut;
   if (grab(local, &xf, (size_t)n * kMlpIn) || grab(local, &xAll, (size_t)n * kMlpInPad) ||
-      grab(local, &tAll, (size_t)n * kMlpOut) || grab(local, &xbat, (size_t)P * kMlpInPad) ||
-      grab(local, &tb, (size_t)P * kMlpOut) || grab(local, &losses, (size_t)epochs) ||
+      grab(local, &tAll, (size_t)n * no) || grab(local, &xbat, (size_t)P * kMlpInPad) ||
+      grab(local, &tb, (size_t)P * no) || grab(local, &losses, (size_t)epochs) ||
       grab(local, &perm, (size_t)epochs * n))
     return done(AVBD_ERR_ALLOC);
   // one shuffle per epoch (Fisher-Yates on SplitMix64), uploaded at once
@@ -227,16 +240,16 @@ extern "C" avbd_status avbd_mlp_train(avbd_mlp* m, const float* x, const float* 
     }
   }
   if (hipMemcpyAsync(xf, x, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
-      hipMemcpyAsync(tAll, t, (size_t)n * kMlpOut * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+      hipMemcpyAsync(tAll, t, (size_t)n * no * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
       hipMemcpyAsync(perm, order.data(), order.size() * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess)
     return done(AVBD_ERR_HIP);
   launch_mlp_pack_input(m->stream, xf, n, xAll);
   MlpTrain T = m->T;
   T.rowsPad = P;
-  const float meanScale = 1.0f / (3.0f * (float)B * (float)batches);
+  const float meanScale = 1.0f / ((float)no * (float)B * (float)batches);
   for (int32_t e = 0; e < epochs; e++) {
     for (uint32_t k = 0; k < batches; k++) {
-      launch_mlp_gather_batch(m->stream, xAll, tAll, perm + (size_t)e * n + (size_t)k * B, B, xbat, tb);
+      launch_mlp_gather_batch(m->stream, xAll, tAll, perm + (size_t)e * n + (size_t)k * B, B, no, xbat, tb);
       launch_mlp_grad(m->stream, m->M, T, xbat, tb, B, m->cus);
       launch_mlp_accum_loss(m->stream, T.loss, losses + e, meanScale);
       launch_mlp_adam(m->stream, m->M, T, lr, ++m->adamStep);

@@ -106,7 +106,8 @@ struct avbd_solver {
   uint32_t residualBlocks = 0;
   uint32_t cus = 256;
   // learned warm start
-  avbd_mlp* mlp = nullptr;
+  avbd_mlp* mlp = nullptr;       // 3 outputs: dynamic particles
+  avbd_mlp* mlpRigid = nullptr;  // 6 outputs: dynamic rigid bodies
   int wsMode = 0;
   bool wsBuilt = false;
   WarmState W{};
@@ -254,14 +255,22 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   stage("coloring");
   // learned warm start: features, then (with a network) the learned start
   // and the residual comparison against the plain guess
-  if (s->wsMode && s->W.n) {
+  if (s->wsMode && (s->W.n || s->W.nR)) {
     const WarmState W = ws_params(s);
     launch_ws_features(st, S, P, W);
-    if ((s->wsMode & AVBD_WS_LEARNED) && s->mlp) {
+    launch_ws_features_rigid(st, S, P, W);
+    const bool lp = s->mlp && W.n, lr = s->mlpRigid && W.nR;
+    if ((s->wsMode & AVBD_WS_LEARNED) && (lp || lr)) {
       launch_residual(st, S, P, s->residualBlocks);
       launch_ws_reduce(st, S, s->residualBlocks, W, 0);
-      launch_mlp_forward(st, s->mlp->M, W.x, W.n, W.y, s->cus);
-      launch_ws_apply(st, S, W);
+      if (lp) {
+        launch_mlp_forward(st, s->mlp->M, W.x, W.n, W.y, s->cus);
+        launch_ws_apply(st, S, W);
+      }
+      if (lr) {
+        launch_mlp_forward(st, s->mlpRigid->M, W.xR, W.nR, W.yR, s->cus);
+        launch_ws_apply_rigid(st, S, W);
+      }
       launch_residual(st, S, P, s->residualBlocks);
       launch_ws_reduce(st, S, s->residualBlocks, W, 1);
       launch_ws_select(st, S, W);
@@ -298,7 +307,11 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
 }
 
 void enqueue_post_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
-  if (s->wsMode && s->W.n) launch_ws_target(s->stream, S, ws_params(s));
+  if (s->wsMode && (s->W.n || s->W.nR)) {
+    const WarmState W = ws_params(s);
+    launch_ws_target(s->stream, S, W);
+    launch_ws_target_rigid(s->stream, S, W);
+  }
   launch_finalize(s->stream, S, P);
   launch_viscosity(s->stream, S, P);
   stage("finalize");
@@ -1912,7 +1925,10 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
   hipLaunchKernelGGL(k_reset_incident, dim3((total + 255) / 256), dim3(256), 0, s->stream, s->S, P, dFlag, dPen,
                      dPen + s->nj, s->S.prevManifolds, s->maxPairs);
   // written states invalidate the learned stage's correction history
-  if (s->wsBuilt) hipMemsetAsync(s->W.corr, 0, 16 * (size_t)s->nb, s->stream);
+  if (s->wsBuilt) {
+    hipMemsetAsync(s->W.corr, 0, 16 * (size_t)s->nb, s->stream);
+    hipMemsetAsync(s->W.corrAng, 0, 16 * (size_t)s->nb, s->stream);
+  }
   hipError_t e = hipStreamSynchronize(s->stream);
   hipFree(dIdx);
   hipFree(dFlag);
@@ -2299,10 +2315,35 @@ avbd_status build_warm_state(avbd_solver* s) {
       if (j != b && j < s->nb) rList.push_back(j);
     rCount[k] = (uint32_t)rList.size() - rStart[k];
   }
+  // dynamic rigid bodies and their joint partners
+  std::vector<uint32_t> ridx;
+  for (uint32_t i = 0; i < s->nb; i++)
+    if (!s->isParticle[i] && s->mass[i] > 0.0f) ridx.push_back(i);
+  const uint32_t nR = (uint32_t)ridx.size();
+  std::vector<uint32_t> qStart(nR), qCount(nR), qList;
+  for (uint32_t k = 0; k < nR; k++) {
+    const uint32_t b = ridx[k];
+    nb.clear();
+    const uint32_t start = s->sAdjStartHost[b], cnt = s->sAdjCount[b];
+    for (uint32_t q = 0; q < cnt; q++) {
+      const uint32_t e = s->sAdjListHost[start + q];
+      if ((e >> kAdjShift) != FK_JOINT) continue;
+      const U4 h = s->jointHeaders[e & kAdjIndexMask];
+      const uint32_t o = h.x == b ? h.y : h.x;
+      if (o != kWorld && o != b && o < s->nb) nb.push_back(o);
+    }
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    qStart[k] = (uint32_t)qList.size();
+    qList.insert(qList.end(), nb.begin(), nb.end());
+    qCount[k] = (uint32_t)qList.size() - qStart[k];
+  }
   WarmState W{};
   W.n = n;
+  W.nR = nR;
   hipError_t e;
   __bf16* xb = nullptr;
+  __bf16* xr = nullptr;
   const size_t m = std::max<uint32_t>(n, 1);
   if ((e = dev_alloc(s, &W.idx, m)) || (e = dev_alloc(s, &W.ringStart, m)) || (e = dev_alloc(s, &W.ringCount, m)) ||
       (e = dev_alloc(s, &W.ringList, std::max<size_t>(rList.size(), 1))) || (e = dev_alloc(s, &W.guess, m)) ||
@@ -2311,7 +2352,19 @@ avbd_status build_warm_state(avbd_solver* s) {
       (e = dev_alloc(s, &W.logT, m * kMlpOut)) || (e = dev_alloc(s, &W.res, 4)) ||
       (e = dev_alloc(s, &W.pred, s->nb)) || (e = dev_alloc(s, &W.pred2, s->nb)))
     return AVBD_ERR_ALLOC;
+  const size_t mr = std::max<uint32_t>(nR, 1);
+  if ((e = dev_alloc(s, &W.ridx, mr)) || (e = dev_alloc(s, &W.rRingStart, mr)) ||
+      (e = dev_alloc(s, &W.rRingCount, mr)) || (e = dev_alloc(s, &W.rRingList, std::max<size_t>(qList.size(), 1))) ||
+      (e = dev_alloc(s, &W.rGuessLin, mr)) || (e = dev_alloc(s, &W.rGuessAng, mr)) ||
+      (e = dev_alloc(s, &W.corrAng, s->nb)) || (e = dev_alloc(s, &xr, mr * kMlpInPad)) ||
+      (e = dev_alloc(s, &W.yR, mr * kMlpOutRigid)) || (e = dev_alloc(s, &W.logXR, mr * kMlpIn)) ||
+      (e = dev_alloc(s, &W.logTR, mr * kMlpOutRigid)))
+    return AVBD_ERR_ALLOC;
   W.x = xb;
+  W.xR = xr;
+  if (upload(W.ridx, ridx) || upload(W.rRingStart, qStart) || upload(W.rRingCount, qCount) ||
+      upload(W.rRingList, qList))
+    return AVBD_ERR_HIP;
   if (upload(W.idx, idx) || upload(W.ringStart, rStart) || upload(W.ringCount, rCount) || upload(W.ringList, rList))
     return AVBD_ERR_HIP;
   s->W = W;
@@ -2335,11 +2388,35 @@ extern "C" avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t 
     if (st != AVBD_OK) return st;
   }
   // a fresh attachment starts without correction history
-  if (mode && !s->wsMode && hipMemset(s->W.corr, 0, 16 * (size_t)s->nb) != hipSuccess) return AVBD_ERR_HIP;
+  if (mode && !s->wsMode &&
+      (hipMemset(s->W.corr, 0, 16 * (size_t)s->nb) != hipSuccess ||
+       hipMemset(s->W.corrAng, 0, 16 * (size_t)s->nb) != hipSuccess))
+    return AVBD_ERR_HIP;
   s->wsMode = mode;
-  s->W.smoothPasses = (uint32_t)passes;
-  s->mlp = mode ? m : nullptr;
+  if (!mode) {
+    s->mlp = s->mlpRigid = nullptr;
+  } else if (m) {
+    if (m->M.nOut == (uint32_t)kMlpOutRigid) s->mlpRigid = m;
+    else s->mlp = m;
+  }
+  if (!m || m->M.nOut != (uint32_t)kMlpOutRigid) s->W.smoothPasses = (uint32_t)passes;
   invalidate_graphs(s);
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_warm_start_samples_rigid(avbd_solver* s, uint32_t capacity, uint32_t* n, float* x,
+                                                         float* t, uint32_t* idx) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!(s->wsMode & AVBD_WS_LOG)) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  *n = s->W.nR;
+  const size_t c = std::min(capacity, s->W.nR);
+  if (c == 0) return AVBD_OK;
+  if ((x && hipMemcpy(x, s->W.logXR, c * kMlpIn * 4, hipMemcpyDeviceToHost)) ||
+      (t && hipMemcpy(t, s->W.logTR, c * kMlpOutRigid * 4, hipMemcpyDeviceToHost)) ||
+      (idx && hipMemcpy(idx, s->W.ridx, c * 4, hipMemcpyDeviceToHost)))
+    return AVBD_ERR_HIP;
   return AVBD_OK;
 }
 

@@ -182,6 +182,47 @@ def batched_learned_comparison(args):
             "step_rate_ratio_learned_only": round(ms_batched / ms_learned, 3)}
 
 
+def rigid_learned_comparison(args, scene):
+    """Batched rigid scenes with the learned start of the rigid bodies (a
+    6-output network trained on other draws of the generator, labels from the
+    plain solver at 100 sweeps) against the plain guess: the residual after
+    the fixed iteration count over `frames` frames."""
+    import numpy as np
+    from avbd_amd import GPUSolver, WarmStartMLP, demos
+    from avbd_amd.learned import collect_samples
+
+    t0 = time.perf_counter()
+    train = [demos.batched_scene(64, seed_base=(7 << 32) + 1000 * k)[0] for k in range(3)]
+    X, T, _ = collect_samples(train, 60, tol=1e-4, max_iterations=100, check_every=10, rigid=True)
+    m = WarmStartMLP(seed=1, outputs=6)
+    losses = m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
+    t_train = time.perf_counter() - t0
+
+    def run(learned):
+        g = GPUSolver(scene)
+        if learned:
+            g.setWarmStart(rigid=m)
+        res, kept = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.learned_frames):
+            g.step()
+            res.append(g.residual())
+            kept.append(g.stats().ws_accepted)
+        wall = time.perf_counter() - t0
+        return {"residual_after_fixed_iterations_median": float(np.median(res)),
+                "residual_after_fixed_iterations_mean": float(np.mean(res)),
+                "ms_per_step_with_residual_readback": round(wall * 1000 / len(res), 4),
+                "learned_start_kept": round(float(np.mean(kept)), 3) if learned else None}
+
+    plain, learned = run(False), run(True)
+    return {"trained_on": "3 x 64 batched rigid scenes (other seeds), 60 frames, labels at 100 sweeps",
+            "train_samples": int(len(X)), "train_s": round(t_train, 2),
+            "loss_first_last": [float(losses[0]), float(losses[-1])], "frames": args.learned_frames,
+            "plain": plain, "learned": learned,
+            "residual_ratio_median": round(plain["residual_after_fixed_iterations_median"] /
+                                           max(learned["residual_after_fixed_iterations_median"], 1e-30), 3)}
+
+
 def build(workload, cells):
     from avbd_amd import demos
 
@@ -288,6 +329,8 @@ def main():
         out["config"]["learned_vs_plain"] = cmp
         out["roofline_mlp"] = roof
         out["config"]["batched_learned_vs_plain"] = batched_learned_comparison(args)
+    if args.workload == "batched" and not args.no_learned:
+        out["config"]["rigid_learned_vs_plain"] = rigid_learned_comparison(args, scene)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, unit)
     print(json.dumps(out), flush=True)

@@ -350,28 +350,39 @@ avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us,
 avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t frame);
 
 /* ---- Learned warm start (no counterpart in the reference) ----------------
- * A per-particle MLP 24 -> 128 -> 128 -> 3 (ReLU; BF16 weights and
- * activations, FP32 accumulation on the matrix cores) predicts the step's
- * converged correction relative to the adaptive warm-start guess of
- * warmstart_bodies (40_solver.metal:657-722), in units of L = |g| dt^2.
- * Features (24 per particle, units of L): v dt, (v - v_prev) dt,
- * x~ - x, x_guess - x, the previous step's correction c, the 1-ring means
- * of v dt and c relative to the particle's own, the contact-entry count / 8,
- * the 1-ring size / 16 and |c|.  The parameters are a flat f32 array:
- * W1[24][128], b1[128], W2[128][128], b2[128], W3[128][3], b3[3] (W[in][out]). */
+ * A per-body MLP 24 -> 128 -> 128 -> 3 (particles) or 6 (rigid bodies:
+ * linear, angular), ReLU, BF16 weights and activations, FP32 accumulation on
+ * the matrix cores, predicts the step's converged correction relative to the
+ * adaptive warm-start guess of warmstart_bodies (40_solver.metal:657-722), in
+ * units of L = |g| dt^2 (rotations as the displacement at the body's bounding
+ * radius r: rotation vector x r / L).
+ * Particle features (24, units of L): v dt, (v - v_prev) dt, x~ - x,
+ * x_guess - x, the previous step's correction c, the 1-ring means of v dt
+ * and c relative to the particle's own, the contact-entry count / 8, the
+ * 1-ring size / 16 and |c|.  Rigid-body features: v dt, w dt r,
+ * (v - v_prev) dt, x_guess - x, the previous linear and angular corrections,
+ * the joint partners' mean linear correction relative to the body's own, the
+ * contact-entry count / 8, the joint count / 4 and |c_lin|.
+ * The parameters are a flat f32 array: W1[24][128], b1[128], W2[128][128],
+ * b2[128], W3[128][outputs], b3[outputs] (W[in][out]). */
 #define AVBD_MLP_INPUTS 24
 #define AVBD_MLP_OUTPUTS 3
-#define AVBD_MLP_PARAMS 20099
+#define AVBD_MLP_OUTPUTS_RIGID 6
+#define AVBD_MLP_PARAMS 20099        /* 3 outputs */
+#define AVBD_MLP_PARAMS_RIGID 20486  /* 6 outputs */
 typedef struct avbd_mlp avbd_mlp;
-/* He-uniform initialisation from `seed` (SplitMix64), zero biases */
-avbd_status avbd_mlp_create(uint64_t seed, avbd_mlp** out);
+/* outputs: AVBD_MLP_OUTPUTS or AVBD_MLP_OUTPUTS_RIGID.  He-uniform
+ * initialisation from `seed` (SplitMix64), output layer scaled by 0.1, zero
+ * biases. */
+avbd_status avbd_mlp_create(uint64_t seed, int32_t outputs, avbd_mlp** out);
 void avbd_mlp_destroy(avbd_mlp* m);
+avbd_status avbd_mlp_info(const avbd_mlp* m, int32_t* outputs, int32_t* num_params);
 avbd_status avbd_mlp_get_params(avbd_mlp* m, float* params);
 /* also resets the Adam moments */
 avbd_status avbd_mlp_set_params(avbd_mlp* m, const float* params);
-/* y[n][3] = MLP(x[n][24]) */
+/* y[n][outputs] = MLP(x[n][24]); params arrays hold num_params floats */
 avbd_status avbd_mlp_forward(avbd_mlp* m, const float* x, uint32_t n, float* y);
-/* loss = sum |y - t|^2 / (3 n) and its gradient grad[AVBD_MLP_PARAMS] over
+/* loss = sum |y - t|^2 / (outputs n) and its gradient grad[num_params] over
  * the n rows (one batch, the training kernels) */
 avbd_status avbd_mlp_loss_grad(avbd_mlp* m, const float* x, const float* t, uint32_t n, float* loss,
                                float* grad);
@@ -384,12 +395,15 @@ avbd_status avbd_mlp_train(avbd_mlp* m, const float* x, const float* t, uint32_t
  * network's stream, `reps` launches) */
 avbd_status avbd_mlp_time_forward(avbd_mlp* m, uint32_t n, int32_t reps, float* avg_us);
 
-/* Attach a network to a solver.  mode bits: AVBD_WS_LEARNED use the learned
+/* Attach a network to a solver: a 3-output network serves the dynamic
+ * particles, a 6-output network the dynamic rigid bodies (call once per
+ * network; the mode is shared).  mode bits: AVBD_WS_LEARNED use the learned
  * start (the plain guess is kept whenever the learned start's residual is
  * higher), AVBD_WS_LOG record every step's features and converged
  * corrections, AVBD_WS_FORCE keep the learned start unconditionally
  * (testing).  m may be NULL for AVBD_WS_LOG alone; the network must outlive
- * the attachment.  mode 0 detaches.  Rigid bodies keep the plain guess. */
+ * the attachment.  mode 0 detaches every network.  The learned start of
+ * either kind is kept or rejected as a whole (one residual comparison). */
 #define AVBD_WS_LEARNED 1
 #define AVBD_WS_LOG 2
 #define AVBD_WS_FORCE 4
@@ -402,6 +416,9 @@ avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t mode);
  * idx[n] their body ids (any output may be NULL). */
 avbd_status avbd_get_warm_start_samples(avbd_solver* s, uint32_t capacity, uint32_t* n, float* x, float* t,
                                         uint32_t* idx);
+/* The same for the dynamic rigid bodies: x[n][24], t[n][6], idx[n]. */
+avbd_status avbd_get_warm_start_samples_rigid(avbd_solver* s, uint32_t capacity, uint32_t* n, float* x, float* t,
+                                              uint32_t* idx);
 
 /* Library / device information string. */
 const char* avbd_version(void);

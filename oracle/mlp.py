@@ -20,14 +20,31 @@ Parity status:
 import numpy as np
 
 INPUTS, HIDDEN, OUTPUTS = 24, 128, 3
-LAYOUT = (("W1", (INPUTS, HIDDEN)), ("b1", (HIDDEN,)), ("W2", (HIDDEN, HIDDEN)), ("b2", (HIDDEN,)),
-          ("W3", (HIDDEN, OUTPUTS)), ("b3", (OUTPUTS,)))
-NUM_PARAMS = sum(int(np.prod(s)) for _, s in LAYOUT)
+
+
+def layout(nout=OUTPUTS):
+    return (("W1", (INPUTS, HIDDEN)), ("b1", (HIDDEN,)), ("W2", (HIDDEN, HIDDEN)), ("b2", (HIDDEN,)),
+            ("W3", (HIDDEN, nout)), ("b3", (nout,)))
+
+
+def num_params(nout=OUTPUTS):
+    return sum(int(np.prod(s)) for _, s in layout(nout))
+
+
+LAYOUT = layout()
+NUM_PARAMS = num_params()
+
+
+def outputs_of(p):
+    """Output count of a flat parameter array (its length is 19712 + 129 nout)."""
+    n = (len(p) - num_params(0)) // (HIDDEN + 1)
+    assert num_params(n) == len(p), len(p)
+    return n
 
 
 def split(p):
     out, o = {}, 0
-    for name, shape in LAYOUT:
+    for name, shape in layout(outputs_of(p)):
         n = int(np.prod(shape))
         out[name] = np.asarray(p[o:o + n], dtype=np.float64).reshape(shape)
         o += n
@@ -35,7 +52,8 @@ def split(p):
 
 
 def join(d):
-    return np.concatenate([np.asarray(d[name], dtype=np.float64).ravel() for name, _ in LAYOUT])
+    nout = np.asarray(d["b3"]).size
+    return np.concatenate([np.asarray(d[name], dtype=np.float64).ravel() for name, _ in layout(nout)])
 
 
 def bf16(a):
@@ -66,16 +84,17 @@ def forward(p, x, emulate=False):
 
 
 def loss_grad(p, x, t, emulate=False):
-    """loss = sum |y - t|^2 / (3 n) and its gradient (flat, the parameter layout)."""
+    """loss = sum |y - t|^2 / (nout n) and its gradient (flat, the parameter layout)."""
     n = len(x)
     r = bf16 if emulate else _id
     y, (xr, h1, h2, W1, W2, W3) = forward(p, x, emulate)
     e = y - np.asarray(t, dtype=np.float64)
-    loss = float(np.sum(e * e) / (3.0 * n))
+    k = float(y.shape[1] * n)
+    loss = float(np.sum(e * e) / k)
     if emulate:  # the GPU forms dY in f32
-        dy = (2.0 * e.astype(np.float32) * np.float32(1.0 / (3.0 * n))).astype(np.float64)
+        dy = (2.0 * e.astype(np.float32) * np.float32(1.0 / k)).astype(np.float64)
     else:
-        dy = 2.0 * e / (3.0 * n)
+        dy = 2.0 * e / k
     dyr = r(dy)
     g = {"W3": h2.T @ dyr, "b3": dy.sum(0)}
     dz2 = (dyr @ W3.T) * (h2 > 0)
@@ -122,6 +141,58 @@ def features_of(ids, pos_start, vel, vel_prev, x_tilde, x_guess, corr, ring, con
         f[23] = np.sqrt(np.float32(cv[0] * cv[0] + cv[1] * cv[1] + cv[2] * cv[2]))
         out[k] = f
     return out
+
+
+def quat_mul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def features_rigid_of(ids, pos_start, vel, vel_prev, ang_vel, x_guess, corr, corr_ang, radius, ring, contacts, L,
+                      dt):
+    """Rigid-body features of bodies `ids` (avbd.h): per-body [num_bodies, 3]
+    float32 arrays, radius [num_bodies] (bounding radius, floored at 1 mm),
+    ring[k] the ascending joint partners of ids[k]."""
+    f32 = np.float32
+    invL = f32(1.0) / f32(L)
+    vs = f32(dt) * invL
+    out = np.zeros((len(ids), INPUTS), np.float32)
+    for k, b in enumerate(ids):
+        ws = f32(dt) * f32(max(radius[b], f32(1e-3))) * invL
+        cl, ca = corr[b], corr_ang[b]
+        rc = len(ring[k])
+        csum = np.zeros(3, np.float32)
+        for j in ring[k]:
+            csum = csum + corr[j]
+        f = np.zeros(INPUTS, np.float32)
+        f[0:3] = vel[b] * vs
+        f[3:6] = ang_vel[b] * ws
+        f[6:9] = (vel[b] - vel_prev[b]) * vs
+        f[9:12] = (x_guess[b] - pos_start[b]) * invL
+        f[12:15] = cl
+        f[15:18] = ca
+        if rc:
+            f[18:21] = csum * (f32(1.0) / f32(rc)) - cl
+        f[21] = f32(min(float(contacts[k]), 8.0)) * f32(0.125)
+        f[22] = f32(min(float(rc), 4.0)) * f32(0.25)
+        f[23] = np.sqrt(np.float32(cl[0] * cl[0] + cl[1] * cl[1] + cl[2] * cl[2]))
+        out[k] = f
+    return out
+
+
+def joint_ring_of(scene, ids):
+    """Joint partners of each rigid body (bodies of active joints), ascending."""
+    nb = [set() for _ in range(len(scene.bodies))]
+    for j in scene.joints:
+        a, b = j.bodyA, j.bodyB
+        inert = j.stiffnessLin == 0 and j.stiffnessAng == 0 and j.motorTorque == 0
+        if a < 0 or inert:
+            continue
+        nb[a].add(b)
+        nb[b].add(a)
+    return [np.array(sorted(nb[b]), dtype=np.int64) for b in ids]
 
 
 def ring_of(scene, ids):

@@ -14,9 +14,25 @@ def _params(seed, scale=0.3):
 
 
 def test_layout_size_matches_abi():
-    assert o.NUM_PARAMS == 20099      # AVBD_MLP_PARAMS in avbd.h
-    p = np.arange(o.NUM_PARAMS, dtype=np.float64)
-    assert np.array_equal(o.join(o.split(p)), p)
+    assert o.NUM_PARAMS == 20099 == o.num_params(3)      # AVBD_MLP_PARAMS in avbd.h
+    assert o.num_params(6) == 20486                       # AVBD_MLP_PARAMS_RIGID
+    for nout in (3, 6):
+        p = np.arange(o.num_params(nout), dtype=np.float64)
+        assert o.outputs_of(p) == nout
+        assert np.array_equal(o.join(o.split(p)), p)
+
+
+def test_six_output_gradient_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    p = np.concatenate([rng.standard_normal(int(np.prod(s))) * 0.3 for _, s in o.layout(6)])
+    x, t = rng.standard_normal((19, 24)), rng.standard_normal((19, 6))
+    _, g = o.loss_grad(p, x, t)
+    for i in rng.choice(len(p), 40, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += 1e-6
+        pm[i] -= 1e-6
+        fd = (o.loss_grad(pp, x, t)[0] - o.loss_grad(pm, x, t)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), i
 
 
 def test_fp64_gradient_matches_finite_differences():
@@ -40,7 +56,7 @@ def test_linear_regime_closed_form():
     # biases large enough that no ReLU clips: the network is the affine map
     # y = x A + c, and dL/db3 = 2 sum(y - t) / (3 n), dL/dW1 = x^T (dY W3^T W2^T) / ...
     rng = np.random.default_rng(2)
-    q = o.split(_params(3, 0.05))
+    q = o.split(_params(3, 0.05))  # 3 outputs
     q["b1"][:] = 50.0
     q["b2"][:] = 1000.0
     p = o.join(q)
@@ -95,6 +111,22 @@ def test_features_hand_case():
     np.testing.assert_allclose(f[15:18], [0, 0, 20], rtol=1e-6, atol=1e-5)   # mean v (1,0,0) minus own
     np.testing.assert_allclose(f[18:21], [1.5, 0, 2], rtol=1e-6)
     assert f[21] == np.float32(0.375) and f[22] == np.float32(0.125) and f[23] == 0
+
+
+def test_rigid_features_hand_case():
+    # body 0 of radius 0.5 spinning about z at 2 rad/s, joint partner 1 with correction (2, 0, 0)
+    dt, L = np.float32(0.01), np.float32(1e-3)
+    z3 = np.zeros((2, 3), np.float32)
+    vel = np.array([[0, 1, 0], [0, 0, 0]], np.float32)
+    w = np.array([[0, 0, 2], [0, 0, 0]], np.float32)
+    corr = np.array([[0, 0, 0], [2, 0, 0]], np.float32)
+    f = o.features_rigid_of([0], z3, vel, vel, w, z3 + vel * dt, corr, z3, np.array([0.5, 0.5], np.float32),
+                            [np.array([1])], [2], L, dt)[0]
+    np.testing.assert_allclose(f[0:3], [0, 10, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[3:6], [0, 0, 10], rtol=1e-6)        # w dt r / L
+    np.testing.assert_allclose(f[9:12], [0, 10, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[18:21], [2, 0, 0], rtol=1e-6)
+    assert f[21] == np.float32(0.25) and f[22] == np.float32(0.25) and f[23] == 0
 
 
 def test_ring_of_tet_and_hinge():
