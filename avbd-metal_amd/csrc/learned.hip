@@ -9,10 +9,10 @@
 // correction: the next step's feature and, when logging, the training target.
 //
 // Features (24, f32, in units of L; v_n, v_{n-1} velocities, x_n the
-// step-start position, x~ the inertial target, c the previous correction,
+// step-start position, c and c' the previous two steps' corrections,
 // <.> the mean over the topological 1-ring):
 //   0-2  v_n dt / L               3-5   (v_n - v_{n-1}) dt / L
-//   6-8  (x~ - x_n) / L           9-11  (x_guess - x_n) / L
+//   6-8  c'                       9-11  (x_guess - x_n) / L
 //   12-14 c                       15-17 (<v_j> - v_n) dt / L
 //   18-20 <c_j> - c               21    min(contact entries, 8) / 8
 //   22   1-ring size / 16         23    |c|
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_ws_features(DeviceState S, StepParams P
   const float inv = rc ? 1.0f / (float)rc : 0.0f;
   const V3 f0 = v * vs;
   const V3 f1 = (v - xyz(S.prevVelLin[b])) * vs;
-  const V3 f2 = (xyz(S.inertLin[b]) - x) * invL;
+  const V3 f2 = xyz(W.corrPrev[b]);
   const V3 f3 = (xyz(g4) - x) * invL;
   const V3 f5 = rc ? (vSum * inv - v) * vs : v3s(0.0f);
   const V3 f6 = rc ? cSum * inv - c : v3s(0.0f);
@@ -159,6 +159,7 @@ __global__ void __launch_bounds__(256) k_ws_target(DeviceState S, WarmState W) {
   if (i >= W.n) return;
   const uint32_t b = W.idx[i];
   const V3 t = (xyz(S.posLin[b]) - xyz(W.guess[i])) * (1.0f / W.L);
+  W.corrPrev[b] = W.corr[b];
   W.corr[b] = f4(t, 0.0f);
   if (W.log) {
     W.logT[3 * i] = t.x;

@@ -47,6 +47,7 @@ struct WarmState {
   uint32_t *ringStart, *ringCount, *ringList;
   F4* guess;              // [n] plain adaptive guess (w = mass)
   F4* corr;               // [numBodies] last converged correction / L (xyz)
+  F4* corrPrev;           // [numBodies] the one before (particles)
   __bf16* x;              // [n][32] network input rows
   float* y;               // [n][3] network output
   float* logX;            // [n][24] features (logging)

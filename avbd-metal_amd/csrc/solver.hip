@@ -1927,6 +1927,7 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
   // written states invalidate the learned stage's correction history
   if (s->wsBuilt) {
     hipMemsetAsync(s->W.corr, 0, 16 * (size_t)s->nb, s->stream);
+    hipMemsetAsync(s->W.corrPrev, 0, 16 * (size_t)s->nb, s->stream);
     hipMemsetAsync(s->W.corrAng, 0, 16 * (size_t)s->nb, s->stream);
   }
   hipError_t e = hipStreamSynchronize(s->stream);
@@ -2347,7 +2348,7 @@ avbd_status build_warm_state(avbd_solver* s) {
   const size_t m = std::max<uint32_t>(n, 1);
   if ((e = dev_alloc(s, &W.idx, m)) || (e = dev_alloc(s, &W.ringStart, m)) || (e = dev_alloc(s, &W.ringCount, m)) ||
       (e = dev_alloc(s, &W.ringList, std::max<size_t>(rList.size(), 1))) || (e = dev_alloc(s, &W.guess, m)) ||
-      (e = dev_alloc(s, &W.corr, s->nb)) || (e = dev_alloc(s, &xb, m * kMlpInPad)) ||
+      (e = dev_alloc(s, &W.corr, s->nb)) || (e = dev_alloc(s, &W.corrPrev, s->nb)) || (e = dev_alloc(s, &xb, m * kMlpInPad)) ||
       (e = dev_alloc(s, &W.y, m * kMlpOut)) || (e = dev_alloc(s, &W.logX, m * kMlpIn)) ||
       (e = dev_alloc(s, &W.logT, m * kMlpOut)) || (e = dev_alloc(s, &W.res, 4)) ||
       (e = dev_alloc(s, &W.pred, s->nb)) || (e = dev_alloc(s, &W.pred2, s->nb)))
@@ -2390,6 +2391,7 @@ extern "C" avbd_status avbd_set_warm_start(avbd_solver* s, avbd_mlp* m, int32_t 
   // a fresh attachment starts without correction history
   if (mode && !s->wsMode &&
       (hipMemset(s->W.corr, 0, 16 * (size_t)s->nb) != hipSuccess ||
+       hipMemset(s->W.corrPrev, 0, 16 * (size_t)s->nb) != hipSuccess ||
        hipMemset(s->W.corrAng, 0, 16 * (size_t)s->nb) != hipSuccess))
     return AVBD_ERR_HIP;
   s->wsMode = mode;

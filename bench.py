@@ -65,6 +65,19 @@ def cpu_baseline(scene, unit, target_s=10.0):
                       f"serial oracle FP32, {dt:.2f} s"}
 
 
+def primal_roofline(solver, pmc):
+    """Algorithmic bytes of one primal color launch over its graph-replayed
+    duration (HIP events on the solver stream) against the HBM peak."""
+    tp = solver.timePrimal(reps=50)
+    st = solver.stats()
+    bytes_per_launch = tp["sweep_bytes"] / tp["launches_per_sweep"]
+    achieved = bytes_per_launch / (tp["avg_launch_us"] * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(pmc) if pmc else None,
+            "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+
+
 def learned_comparison(args):
     """Learned vs plain warm start: training pairs from the plain solver on
     other seeds of the --train-cells block (every frame iterated to residual
@@ -262,6 +275,7 @@ def main():
     ap.add_argument("--eval-cells", type=int, default=0, help="block it is evaluated on (0 = --cells)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="uneven soft blocks of the batched comparison")
     ap.add_argument("--batch-frames", type=int, default=30)
+    ap.add_argument("--scale-cells", type=int, default=55, help="block of the at-scale primal roofline (0 = skip)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
     ap.add_argument("--sweep", type=int, default=0, help="1 per-color launches, 2 persistent (0 = autotune)")
     args = ap.parse_args()
@@ -317,14 +331,19 @@ def main():
 
     # roofline of the dominant kernel (per-color primal sweep), HIP events on the solver stream
     if args.workload != "batched":
-        tp = solver.timePrimal(reps=50)
-        bytes_per_launch = tp["sweep_bytes"] / tp["launches_per_sweep"]
-        achieved = bytes_per_launch / (tp["avg_launch_us"] * 1e-6) / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": traffic_from_profiles(pmc) if pmc else None,
-                           "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
-                           "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+        out["roofline"] = primal_roofline(solver, pmc)
+    # the same kernel where a color fills the chip: the 1M-tet block (56^3 vertices)
+    if args.workload == "soft_block" and args.scale_cells > 0 and args.scale_cells != args.cells:
+        from avbd_amd import demos as _demos
+        big = _demos.soft_block_scene(cells=args.scale_cells)
+        gb = GPUSolver(big, lanesPerBody=args.lanes, sweepMode=args.sweep)
+        gb.step(3)
+        gb.synchronize()
+        roof = primal_roofline(gb, f"primal_soft_block_{args.scale_cells + 1}_pmc.json")
+        roof["workload"] = (f"soft block {args.scale_cells}^3 cells ({big.numDynamic} vertices, {len(big.tets)} tets), "
+                            f"the same kernel at the 1M-tet size")
+        out["roofline_at_scale"] = roof
+        gb.close()
     if args.workload == "soft_block" and not args.no_learned:
         cmp, roof = learned_comparison(args)
         out["config"]["learned_vs_plain"] = cmp

@@ -356,8 +356,9 @@ avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t fram
  * adaptive warm-start guess of warmstart_bodies (40_solver.metal:657-722), in
  * units of L = |g| dt^2 (rotations as the displacement at the body's bounding
  * radius r: rotation vector x r / L).
- * Particle features (24, units of L): v dt, (v - v_prev) dt, x~ - x,
- * x_guess - x, the previous step's correction c, the 1-ring means of v dt
+ * Particle features (24, units of L): v dt, (v - v_prev) dt, the
+ * correction of two steps back c', x_guess - x, the previous step's
+ * correction c, the 1-ring means of v dt
  * and c relative to the particle's own, the contact-entry count / 8, the
  * 1-ring size / 16 and |c|.  Rigid-body features: v dt, w dt r,
  * (v - v_prev) dt, x_guess - x, the previous linear and angular corrections,

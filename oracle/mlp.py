@@ -14,8 +14,8 @@ Parity status:
                                the FP64 restatement; checked against it within
                                the bf16 bound
   features                     restated from the definition in avbd.h; pinned
-                               by closed forms (free flight: x~ - x = v dt + g
-                               dt^2, guess offset v dt) in tests
+                               by a hand-computed case and closed forms (free
+                               flight: guess offset v dt) in tests
 """
 import numpy as np
 
@@ -108,12 +108,12 @@ def loss_grad(p, x, t, emulate=False):
     return loss, join(g)
 
 
-def features_of(ids, pos_start, vel, vel_prev, x_tilde, x_guess, corr, ring, contacts, L, dt):
+def features_of(ids, pos_start, vel, vel_prev, corr_prev, x_guess, corr, ring, contacts, L, dt):
     """Features of particles `ids` (body ids).  Every per-body array is
     [num_bodies, 3] float32: pos_start x_n, vel v_n, vel_prev v_{n-1},
-    x_tilde the inertial target, x_guess the adaptive guess, corr the previous
-    step's corrections; ring[k] the ascending 1-ring body ids of ids[k];
-    contacts[k] the particle's contact-entry count."""
+    corr_prev the corrections of two steps back, x_guess the adaptive guess,
+    corr the previous step's corrections; ring[k] the ascending 1-ring body
+    ids of ids[k]; contacts[k] the particle's contact-entry count."""
     f32 = np.float32
     invL = f32(1.0) / f32(L)
     vs = f32(dt) * invL
@@ -129,7 +129,7 @@ def features_of(ids, pos_start, vel, vel_prev, x_tilde, x_guess, corr, ring, con
         f = np.zeros(INPUTS, np.float32)
         f[0:3] = vv * vs
         f[3:6] = (vv - vel_prev[b]) * vs
-        f[6:9] = (x_tilde[b] - xv) * invL
+        f[6:9] = corr_prev[b]
         f[9:12] = (x_guess[b] - xv) * invL
         f[12:15] = cv
         if rc:

@@ -122,16 +122,16 @@ def test_features_match_restatement():
     ring = omlp.ring_of(s, ids)
     gz = np.array([0, 0, grav], np.float32)
     corr = np.zeros((nb, 3), np.float32)
+    corr_prev = np.zeros((nb, 3), np.float32)
     x_n, v_n, v_prev = p0.copy(), v0.copy(), v0.copy()
     for step in range(2):
         g.step()
         X, T, idx = g.warmStartSamples()
         assert np.array_equal(idx, ids)
-        x_tilde = x_n + v_n * dt + gz * (dt * dt)
         acc = (v_n - v_prev) / dt
         w = np.clip(acc[:, 2] * np.float32(np.sign(grav)) / np.float32(abs(grav)), 0, 1).astype(np.float32)
         x_guess = x_n + v_n * dt + gz * ((w[:, None] * dt) * dt)
-        want = omlp.features_of(ids, x_n, v_n, v_prev, x_tilde, x_guess, corr, ring, np.zeros(len(ids)), L, dt)
+        want = omlp.features_of(ids, x_n, v_n, v_prev, corr_prev, x_guess, corr, ring, np.zeros(len(ids)), L, dt)
         err = np.abs(X - want).max()
         x1 = g.positions()
         tw = (x1[ids] - x_guess[ids]) / L
@@ -141,14 +141,14 @@ def test_features_match_restatement():
         assert err <= 5e-4
         assert terr <= 1e-3
         if step == 0:
-            # free flight closed forms: x~ - x = v dt + g dt^2, guess offset v dt
-            # (within the rounding of a ~6 m position, 2e-4 in units of L)
-            np.testing.assert_allclose(X[:, 6:9], (v0[ids] * dt + gz * dt * dt) / L, rtol=0, atol=5e-4)
+            # free flight closed form: guess offset v dt (within the rounding of
+            # a ~6 m position, 2e-4 in units of L); no correction history yet
             np.testing.assert_allclose(X[:, 9:12], v0[ids] * dt / L, rtol=0, atol=5e-4)
-            assert np.all(X[:, 12:15] == 0)
+            assert np.all(X[:, 6:9] == 0) and np.all(X[:, 12:15] == 0)
         v_prev = v_n
         _, _, v_n, _ = g.bodyStates()
         x_n = x1
+        corr_prev = corr
         corr = np.zeros((nb, 3), np.float32)
         corr[ids] = T
 

@@ -93,19 +93,20 @@ def test_emulated_rounding_within_bf16_bound():
 
 def test_features_hand_case():
     # one particle at x = (0, 0, 1), v = (1, 0, -2), v_prev = (1, 0, 0),
-    # ring {1, 2} with velocities (0, 0, 0) and (2, 0, 0), no history
+    # ring {1, 2} with velocities (0, 0, 0) and (2, 0, 0), correction two steps
+    # back (0.5, -1, 0)
     dt, L = np.float32(0.01), np.float32(1e-3)
     pos = np.array([[0, 0, 1], [0, 0, 0], [0, 0, 0]], np.float32)
     vel = np.array([[1, 0, -2], [0, 0, 0], [2, 0, 0]], np.float32)
     vp = np.array([[1, 0, 0], [0, 0, 0], [0, 0, 0]], np.float32)
-    xt = pos + vel * dt + np.array([0, 0, -10], np.float32) * dt * dt
     xg = pos + vel * dt
     corr = np.zeros((3, 3), np.float32)
     corr[2] = [3, 0, 4]
-    f = o.features_of([0], pos, vel, vp, xt, xg, corr, [np.array([1, 2])], [3], L, dt)[0]
+    cprev = np.array([[0.5, -1, 0], [0, 0, 0], [0, 0, 0]], np.float32)
+    f = o.features_of([0], pos, vel, vp, cprev, xg, corr, [np.array([1, 2])], [3], L, dt)[0]
     np.testing.assert_allclose(f[0:3], [10, 0, -20], rtol=1e-6)
     np.testing.assert_allclose(f[3:6], [0, 0, -20], rtol=1e-6)
-    np.testing.assert_allclose(f[6:9], [10, 0, -21], rtol=1e-5)
+    np.testing.assert_allclose(f[6:9], [0.5, -1, 0])
     np.testing.assert_allclose(f[9:12], [10, 0, -20], rtol=1e-5)
     np.testing.assert_allclose(f[12:15], 0)
     np.testing.assert_allclose(f[15:18], [0, 0, 20], rtol=1e-6, atol=1e-5)   # mean v (1,0,0) minus own
