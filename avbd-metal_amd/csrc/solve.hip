@@ -15,13 +15,16 @@ __device__ __forceinline__ bool body_dynamic(const DeviceState& S, uint32_t b) {
 // Every wait is bounded: past the spin bound a block records CTR_GBAR_FAIL
 // and proceeds, and once any block has given up later barriers do not wait,
 // so the grid always drains.
+// One release (cache write-back) before arriving, relaxed polling of the
+// arrival counter, and one acquire (cache invalidate) once it is reached:
+// an acquire load inside the spin would invalidate the L2 on every poll.
 __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint32_t target) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(bar, 1u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
            __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > (1u << 20)) {
@@ -29,7 +32,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t* fail, uint
         break;
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
