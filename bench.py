@@ -47,22 +47,36 @@ def traffic_from_profiles(name):
 
 
 def cpu_baseline(scene, unit, target_s=10.0):
-    """Oracle (serial C++ FP32 restatement) on the host, one core, bounded sample."""
+    """Oracle (C++ FP32 restatement of the reference CPU path) on the host:
+    one core (the serial Gauss-Seidel, the reported value), then every color
+    split over the host's cores (identical results), each on a bounded
+    sample of about target_s seconds."""
     import oracle
 
-    o = oracle.OracleSolver(scene)
-    o.step()  # warm-up step (first contact detection, caches)
-    t0 = time.perf_counter()
-    o.step()
-    one = time.perf_counter() - t0
-    n = max(1, min(500, int(target_s / max(one, 1e-3))))
-    t0 = time.perf_counter()
-    o.step(n)
-    dt = time.perf_counter() - t0
-    iters = scene.settings.iterations
-    return {"value": scene.numDynamic * iters * n / dt, "unit": unit, "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same workload after 2 warm-up steps (about {target_s:.0f} s of work), "
-                      f"serial oracle FP32, {dt:.2f} s"}
+    def rate(threads, budget):
+        o = oracle.OracleSolver(scene)
+        o.set_threads(threads)
+        o.step()  # warm-up step (first contact detection, caches)
+        t0 = time.perf_counter()
+        o.step()
+        one = time.perf_counter() - t0
+        n = max(1, min(500, int(budget / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        o.step(n)
+        dt = time.perf_counter() - t0
+        return scene.numDynamic * scene.settings.iterations * n / dt, n, dt
+
+    v1, n1, dt1 = rate(1, target_s)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    out = {"value": v1, "unit": unit, "cores": 1, "kind": "port",
+           "sample": f"{n1} steps of the same workload after 2 warm-up steps (about {target_s:.0f} s of work), "
+                     f"serial oracle FP32, {dt1:.2f} s"}
+    if cores > 1:
+        vn, nn, dtn = rate(cores, target_s / 2)
+        out["all_cores"] = {"value": vn, "cores": cores,
+                            "sample": f"{nn} steps, each color's bodies split over {cores} OpenMP threads, "
+                                      f"{dtn:.2f} s"}
+    return out
 
 
 def primal_roofline(solver, pmc):

@@ -42,6 +42,7 @@ def load():
         "oracle_destroy": (None, [vp]),
         "oracle_step": (C.c_int, [vp]),
         "oracle_set_settings": (C.c_int, [vp, vp]),
+        "oracle_set_threads": (C.c_int, [vp, C.c_int]),
         "oracle_step_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]),
         "oracle_get_body_states": (C.c_int, [vp, C.c_uint32, fp, fp, fp, fp]),
         "oracle_get_body_states_f64": (C.c_int, [vp, C.c_uint32, dp, dp, dp, dp]),
@@ -107,6 +108,10 @@ class OracleSolver:
         self._lib.oracle_step_to_residual(self._h, float(tol), int(maxIterations), int(checkEvery), C.byref(it),
                                           C.byref(r))
         return it.value, r.value
+
+    def set_threads(self, n):
+        """Split each color's primal over n threads (results identical to n = 1)."""
+        self._lib.oracle_set_threads(self._h, int(n))
 
     def set_settings(self, settings):
         s = settings.to_c()

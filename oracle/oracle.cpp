@@ -2193,9 +2193,17 @@ struct Oracle : oracle_base {
   // residual-driven frames (tolerance > 0): see stepToResidual
   double tolerance = 0, lastResidual = 0;
   int maxIterations = 200, checkEvery = 5, itersUsed = 0;
+  // The bodies of one color are independent (same-color contact partners
+  // come from the snapshot), so a color may be split over threads with
+  // results identical to the serial order; threads = 1 is the serial sweep.
+  int threads = 1;
   void sweep() {
-    for (int c = 0; c < 64; c++)
-      for (uint32_t b : buckets[c]) primal(b);
+    for (int c = 0; c < 64; c++) {
+      const std::vector<uint32_t>& bk = buckets[c];
+      const long n = (long)bk.size();
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1 && n > 256)
+      for (long i = 0; i < n; i++) primal(bk[(size_t)i]);
+    }
     dual();
   }
   void stepToResidual(double tol, int maxIt, int every) {
@@ -2378,6 +2386,11 @@ int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, in
     if (residual) *residual = s->lastResidual;
   };
   if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_set_threads(oracle_solver* o, int threads) {
+  if (threads < 1) threads = 1;
+  if (o->f) o->f->threads = threads; else o->d->threads = threads;
   return 0;
 }
 int oracle_set_settings(oracle_solver* o, const avbd_settings* s) {

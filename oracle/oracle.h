@@ -20,6 +20,8 @@ typedef struct oracle_solver oracle_solver;
 /* precision: 32 or 64 */
 int oracle_create(const avbd_scene_desc* scene, int precision, int max_pairs_per_body, oracle_solver** out);
 void oracle_destroy(oracle_solver* o);
+/* split each color's primal over `threads` threads (identical results) */
+int oracle_set_threads(oracle_solver* o, int threads);
 int oracle_step(oracle_solver* o);
 /* one frame iterated until the residual <= tol (chunks of check_every sweeps, at most max_iterations) */
 int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, int check_every, int* iterations,

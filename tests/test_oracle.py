@@ -270,3 +270,15 @@ def test_soft_block_keeps_volume():
 
     p0 = np.array([b["position"] for b in s.bodies], np.float64)
     assert abs(vol(p) / vol(p0) - 1.0) < 0.05
+
+
+def test_threaded_sweep_is_identical_to_serial():
+    # the bodies of one color are independent, so splitting a color over
+    # threads must not change a single bit
+    s = demos.soft_block_scene(cells=8, iterations=5)
+    a, b = oracle.OracleSolver(s), oracle.OracleSolver(s)
+    b.set_threads(4)
+    a.step(4)
+    b.step(4)
+    for x, y in zip(a.bodyStates(), b.bodyStates()):
+        assert np.array_equal(x, y)
