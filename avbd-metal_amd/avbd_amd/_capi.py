@@ -118,6 +118,11 @@ SIGNATURES = {
     "avbd_set_body_states": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP, FP, FP]),
     "avbd_set_motor_targets": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
     "avbd_set_motor_torques": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
+    "avbd_set_motor_velocities": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
+    "avbd_set_joint_world_anchors": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
+    "avbd_set_body_poses": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP]),
+    "avbd_get_motor_states": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP]),
+    "avbd_get_joint_diagnostics": (C.c_int, [C.c_void_p, I32P, FP, FP]),
     "avbd_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "avbd_get_colors": (C.c_int, [C.c_void_p, C.c_uint32, U32P]),
     "avbd_get_cell_lists": (C.c_int, [C.c_void_p, U32P, U32P, FP, U32P, U32P, U32P]),

@@ -158,6 +158,43 @@ class GPUSolver:
         t = np.ascontiguousarray(targets, dtype=np.float32)
         self._check(self._lib.avbd_set_motor_targets(self._h, _u32p(j), len(j), _fp(t)), "setMotorTargets")
 
+    def setMotorVelocities(self, joints, radiansPerSecond):
+        j = np.ascontiguousarray(joints, dtype=np.uint32)
+        v = np.ascontiguousarray(radiansPerSecond, dtype=np.float32)
+        self._check(self._lib.avbd_set_motor_velocities(self._h, _u32p(j), len(j), _fp(v)), "setMotorVelocities")
+
+    def setJointWorldAnchors(self, joints, points):
+        j = np.ascontiguousarray(joints, dtype=np.uint32)
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        self._check(self._lib.avbd_set_joint_world_anchors(self._h, _u32p(j), len(j), _fp(p)), "setJointWorldAnchors")
+
+    def setBodyPoses(self, idx, pos, rot):
+        i = np.ascontiguousarray(idx, dtype=np.uint32)
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        r = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, 4)
+        self._check(self._lib.avbd_set_body_poses(self._h, _u32p(i), len(i), _fp(p), _fp(r)), "setBodyPoses")
+
+    def motorStates(self, joints):
+        """[(angle, velocity)] of hinge joints (GPUSolver.motorStates)."""
+        j = np.ascontiguousarray(joints, dtype=np.uint32)
+        a = np.zeros(len(j), np.float32)
+        v = np.zeros(len(j), np.float32)
+        self._check(self._lib.avbd_get_motor_states(self._h, _u32p(j), len(j), _fp(a), _fp(v)), "motorStates")
+        return list(zip(a.tolist(), v.tolist()))
+
+    def motorAngles(self, joints):
+        return [a for a, _ in self.motorStates(joints)]
+
+    def debugBrokenJoints(self):
+        n = C.c_int32()
+        self._check(self._lib.avbd_get_joint_diagnostics(self._h, C.byref(n), None, None), "debugBrokenJoints")
+        return n.value
+
+    def debugMaxLambda(self):
+        a, b = C.c_float(), C.c_float()
+        self._check(self._lib.avbd_get_joint_diagnostics(self._h, None, C.byref(a), C.byref(b)), "debugMaxLambda")
+        return a.value, b.value
+
     def setMotorTorques(self, joints, torques):
         j = np.ascontiguousarray(joints, dtype=np.uint32)
         t = np.ascontiguousarray(torques, dtype=np.float32)

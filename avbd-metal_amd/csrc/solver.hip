@@ -1940,37 +1940,139 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
 }
 
 namespace {
-__global__ void k_set_motor(DeviceState S, const uint32_t* j, const float* v, uint32_t n, int field) {
+// field 0: motor.x (target angle / velocity), 1: motor.y (effort limit), 2: rA.xyz (world anchor)
+__global__ void k_set_joint_field(DeviceState S, const uint32_t* j, const float* v, uint32_t n, int field) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  if (field == 0) S.joints[j[k]].motor.x = v[k];
-  else S.joints[j[k]].motor.y = v[k];
+  JointRec& r = S.joints[j[k]];
+  if (field == 0) r.motor.x = v[k];
+  else if (field == 1) r.motor.y = v[k];
+  else r.rA = F4{v[3 * k], v[3 * k + 1], v[3 * k + 2], r.rA.w};
 }
-avbd_status set_motor_field(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int field) {
-  if (!s || (n && (!joints || !v))) return AVBD_ERR_INVALID_ARGUMENT;
-  if (n == 0) return AVBD_OK;
-  for (uint32_t k = 0; k < n; k++)
-    if (joints[k] >= s->nj) return AVBD_ERR_INVALID_ARGUMENT;
-  avbd_status st = sync_all(s);
-  if (st != AVBD_OK) return st;
+avbd_status set_joint_field(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int field) {
+  const uint32_t width = field == 2 ? 3u : 1u;
   uint32_t* dj = nullptr;
   float* dv = nullptr;
-  if (hipMalloc(&dj, 4 * n) || hipMalloc(&dv, 4 * n)) return AVBD_ERR_ALLOC;
+  if (hipMalloc(&dj, 4 * n) || hipMalloc(&dv, 4 * width * n)) return AVBD_ERR_ALLOC;
   hipMemcpy(dj, joints, 4 * n, hipMemcpyHostToDevice);
-  hipMemcpy(dv, v, 4 * n, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_set_motor, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dj, dv, n, field);
+  hipMemcpy(dv, v, 4 * width * n, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_set_joint_field, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dj, dv, n, field);
   hipError_t e = hipStreamSynchronize(s->stream);
   hipFree(dj);
   hipFree(dv);
   return e == hipSuccess ? AVBD_OK : AVBD_ERR_HIP;
 }
+// the preconditions of GPUSolver.swift:2496-2590 as argument errors; kept
+// joints (mode 0 targets are no-ops) compacted into idx / val
+avbd_status motor_updates(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int kind,
+                          std::vector<uint32_t>& idx, std::vector<float>& val) {
+  if (!s || (n && (!joints || !v))) return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t k = 0; k < n; k++) {
+    if (joints[k] >= s->nj || !isfinite(v[k])) return AVBD_ERR_INVALID_ARGUMENT;
+    const uint32_t mode = s->jointHeaders[joints[k]].w & kMotorModeMask;
+    if (kind == 0) {         // position targets: position-PD motors; unpowered joints ignore them
+      if (mode == 0) continue;
+      if (mode != kMotorImplicitPD && mode != kMotorExplicitPD) return AVBD_ERR_INVALID_ARGUMENT;
+    } else if (kind == 1) {  // velocity commands: velocity motors only
+      if (mode != kMotorVelocity) return AVBD_ERR_INVALID_ARGUMENT;
+    } else {                 // effort limits: authored motors only, finite and non-negative
+      if (mode == 0 || v[k] < 0.0f) return AVBD_ERR_INVALID_ARGUMENT;
+    }
+    idx.push_back(joints[k]);
+    val.push_back(v[k]);
+  }
+  return AVBD_OK;
+}
+avbd_status set_motor(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int kind) {
+  std::vector<uint32_t> idx;
+  std::vector<float> val;
+  avbd_status st = motor_updates(s, joints, n, v, kind, idx, val);
+  if (st != AVBD_OK) return st;
+  if (idx.empty()) return AVBD_OK;
+  st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  return set_joint_field(s, idx.data(), (uint32_t)idx.size(), val.data(), kind == 2 ? 1 : 0);
+}
 }  // namespace
 
 extern "C" avbd_status avbd_set_motor_targets(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* t) {
-  return set_motor_field(s, joints, n, t, 0);
+  return set_motor(s, joints, n, t, 0);
+}
+extern "C" avbd_status avbd_set_motor_velocities(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                                 const float* rad_per_s) {
+  return set_motor(s, joints, n, rad_per_s, 1);
 }
 extern "C" avbd_status avbd_set_motor_torques(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* t) {
-  return set_motor_field(s, joints, n, t, 1);
+  return set_motor(s, joints, n, t, 2);
+}
+
+extern "C" avbd_status avbd_set_joint_world_anchors(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                                    const float* point3) {
+  if (!s || (n && (!joints || !point3))) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  for (uint32_t k = 0; k < n; k++)
+    if (joints[k] >= s->nj) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  return set_joint_field(s, joints, n, point3, 2);
+}
+
+extern "C" avbd_status avbd_set_body_poses(avbd_solver* s, const uint32_t* idx, uint32_t n, const float* pos3,
+                                           const float* rot4) {
+  if (!s || (n && (!pos3 || !rot4))) return AVBD_ERR_INVALID_ARGUMENT;
+  std::vector<float> zero(3 * (size_t)n, 0.0f);
+  return avbd_set_body_states(s, idx, n, pos3, rot4, zero.data(), zero.data());
+}
+
+extern "C" avbd_status avbd_get_motor_states(avbd_solver* s, const uint32_t* joints, uint32_t n, float* angle,
+                                             float* velocity) {
+  if (!s || (n && !joints)) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  for (uint32_t k = 0; k < n; k++)
+    if (joints[k] >= s->nj) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  (void)st;  // states stay readable after a failure
+  std::vector<JointRec> jr;
+  std::vector<F4> pa, va;
+  if (read_dev(s, jr, s->S.joints, s->nj) || read_dev(s, pa, s->S.posAng, s->nb) ||
+      read_dev(s, va, s->S.velAng, s->nb))
+    return AVBD_ERR_HIP;
+  for (uint32_t k = 0; k < n; k++) {
+    const JointRec& j = jr[joints[k]];
+    const uint32_t a = j.header.x, b = j.header.y;
+    const F4 qA = a == kWorld ? F4{0, 0, 0, 1} : pa[a];
+    const F4 qB = pa[b];
+    // relative rotation of the child from its rest pose, shortest arc
+    F4 r = q_mul(q_inv(q_mul(qA, j.restRel)), qB);
+    if (r.w < 0.0f) r = F4{-r.x, -r.y, -r.z, -r.w};
+    const V3 axis = xyz(j.hingeAxis);
+    if (angle) angle[k] = 2.0f * atan2f(dot(xyz(r), axis), r.w);
+    const V3 wB = xyz(va[b]), wA = a == kWorld ? v3s(0.0f) : xyz(va[a]);
+    if (velocity) velocity[k] = dot(wB - wA, q_rotate(qB, axis));
+  }
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_joint_diagnostics(avbd_solver* s, int32_t* broken, float* max_lambda_lin,
+                                                  float* max_lambda_ang) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  std::vector<JointRec> jr;
+  if (read_dev(s, jr, s->S.joints, s->nj)) return AVBD_ERR_HIP;
+  int32_t nb = 0;
+  float ml = 0.0f, ma = 0.0f;
+  for (const JointRec& j : jr) {
+    if (j.header.z != 0) {
+      nb++;
+      continue;
+    }
+    ml = fmaxf(ml, length(xyz(j.lambdaLin)));
+    ma = fmaxf(ma, length(xyz(j.lambdaAng)));
+  }
+  if (broken) *broken = nb;
+  if (max_lambda_lin) *max_lambda_lin = ml;
+  if (max_lambda_ang) *max_lambda_ang = ma;
+  return AVBD_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -230,11 +230,32 @@ avbd_status avbd_get_body_mass(avbd_solver* s, uint32_t n, float* mass, float* i
 avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n,
                                  const float* pos3, const float* rot4,
                                  const float* vel3, const float* angvel3);
-/* setMotorTargets / setMotorTorques -- GPUSolver.swift:2496-2590 */
+/* setBodyPoses(_:) -- GPUSolver.swift:2399-2404: setBodyStates with zero velocities. */
+avbd_status avbd_set_body_poses(avbd_solver* s, const uint32_t* idx, uint32_t n, const float* pos3,
+                                const float* rot4);
+/* setMotorTargets / setMotorVelocities / setMotorTorques -- GPUSolver.swift:2496-2590.
+ * The reference's preconditions return AVBD_ERR_INVALID_ARGUMENT (nothing is
+ * written): a joint index out of range or a non-finite value; targets need a
+ * position-PD motor (joints without a motor ignore them); velocities need a
+ * velocity motor; effort limits need an authored motor and a value >= 0. */
 avbd_status avbd_set_motor_targets(avbd_solver* s, const uint32_t* joints, uint32_t n,
                                    const float* target);
+avbd_status avbd_set_motor_velocities(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                      const float* rad_per_s);
 avbd_status avbd_set_motor_torques(avbd_solver* s, const uint32_t* joints, uint32_t n,
                                    const float* torque);
+/* setJointWorldAnchors(_:) -- GPUSolver.swift:2481-2490: moves the world point
+ * of world-anchored joints (point3[3*n]). */
+avbd_status avbd_set_joint_world_anchors(avbd_solver* s, const uint32_t* joints, uint32_t n,
+                                         const float* point3);
+/* motorStates(_:) -- GPUSolver.swift:2605-2637: hinge twist angle (2 atan2 of
+ * the rest-relative rotation about the axis) and the relative angular
+ * velocity projected on the child's world hinge axis. */
+avbd_status avbd_get_motor_states(avbd_solver* s, const uint32_t* joints, uint32_t n, float* angle,
+                                  float* velocity);
+/* debugBrokenJoints / debugMaxLambda -- GPUSolver.swift:2652-2672 */
+avbd_status avbd_get_joint_diagnostics(avbd_solver* s, int32_t* broken, float* max_lambda_lin,
+                                       float* max_lambda_ang);
 
 /* ---- stats / debug ------------------------------------------------------- */
 

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import oracle
-from avbd_amd import GPUSolver, RuntimeFailure, demos
+from oracle import mlp as omlp
+from avbd_amd import AVBDError, GPUSolver, RuntimeFailure, demos
 from avbd_amd.scene import SimSettings
 
 from conftest import flush_print
@@ -367,3 +368,85 @@ def test_batched_4096_runs_and_isolates():
     assert np.all(np.isfinite(p))
     assert st.color_conflicts == 0
     assert p[1:, 2].min() > -0.05
+
+
+def _motor_chain(gravity=-10.0):
+    """Three hinged links: joint 0 unpowered (world hinge), joint 1 a velocity
+    motor, joint 2 an implicit position PD motor; a fourth body on a fragile
+    ball joint (fracture 1 N, linear multiplier counted)."""
+    from avbd_amd.scene import SceneJoint
+    s = demos.PhysicsScene("motors", SimSettings(iterations=10, gravity=gravity))
+    demos.add_ground(s)
+    a = s.addBody((0.1, 0.1, 0.3), 500.0, 0.5, (0, 0, 2.85))
+    b = s.addBody((0.1, 0.1, 0.3), 500.0, 0.5, (0, 0, 2.55))
+    c = s.addBody((0.1, 0.1, 0.3), 500.0, 0.5, (0, 0, 2.25))
+    d = s.addBody((0.1, 0.1, 0.1), 500.0, 0.5, (0, 0, 2.05))
+    inf = float("inf")
+    s.addJoint(SceneJoint(-1, a, (0, 0, 3.0), (0, 0, 0.15), stiffnessLin=inf, stiffnessAng=inf, hingeAxis=(1, 0, 0)))
+    s.addJoint(SceneJoint(a, b, (0, 0, -0.15), (0, 0, 0.15), stiffnessLin=inf, stiffnessAng=inf, hingeAxis=(1, 0, 0),
+                          motorTarget=2.0, motorTorque=5.0, motorDamping=0.5, motorMode=2))
+    s.addJoint(SceneJoint(b, c, (0, 0, -0.15), (0, 0, 0.15), stiffnessLin=inf, stiffnessAng=inf, hingeAxis=(0, 1, 0),
+                          motorTarget=0.5, motorTorque=50.0, motorStiffness=500.0, motorDamping=20.0, motorMode=0))
+    s.addJoint(SceneJoint(c, d, (0, 0, -0.15), (0, 0, 0.05), stiffnessLin=inf, fracture=1.0, fractureLinear=True))
+    return s
+
+
+def _twist(qA, rest, qB, axis):
+    r = omlp.quat_mul(omlp.quat_mul(qA, rest) * np.array([-1, -1, -1, 1.0]), qB)
+    if r[3] < 0:
+        r = -r
+    return 2 * np.arctan2(np.dot(r[:3], axis), r[3])
+
+
+def test_motor_setters_states_and_preconditions():
+    _case("motor velocities / targets / torques, motor states, world anchors, poses, joint diagnostics")
+    # without gravity the velocity motor alone sets joint 1's relative rate
+    g = GPUSolver(_motor_chain(gravity=0.0))
+    g.step(90)
+    _, q, _, w = g.bodyStates()
+    st = g.motorStates([0, 1, 2])
+    # motor states vs a numpy evaluation from the body states (rest rotation = identity here)
+    axes = [np.array([1.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0, 1.0, 0])]
+    pairs = [(None, 1), (1, 2), (2, 3)]
+    ident = np.array([0, 0, 0, 1.0])
+    for k, (pa, pb) in enumerate(pairs):
+        qA = ident if pa is None else q[pa].astype(np.float64)
+        qB = q[pb].astype(np.float64)
+        ang = _twist(qA, ident, qB, axes[k])
+        wA = np.zeros(3) if pa is None else w[pa]
+        axw = np.array(omlp.quat_mul(omlp.quat_mul(qB, np.array([*axes[k], 0.0])), qB * np.array([-1, -1, -1, 1.0])))[:3]
+        vel = float(np.dot(w[pb] - wA, axw))
+        assert abs(st[k][0] - ang) < 1e-4, (k, st[k], ang)
+        assert abs(st[k][1] - vel) < 1e-3 * max(1.0, abs(vel)), (k, st[k], vel)
+    flush_print(f"  joint 1 rate after 90 frames at target 2 rad/s: {st[1][1]:.3f}")
+    # the damped motor (kd 0.5 N m s against the chain's inertia) approaches its command
+    assert 1.0 < st[1][1] < 2.3, st[1]
+    g.setMotorVelocities([1], [-1.0])
+    g.step(90)
+    r = g.motorStates([1])[0][1]
+    flush_print(f"  joint 1 rate after 90 more frames at target -1 rad/s: {r:.3f}")
+    assert -1.3 < r < -0.4
+    # preconditions (argument errors, nothing written)
+    for call in (lambda: g.setMotorVelocities([2], [1.0]),       # not a velocity motor
+                 lambda: g.setMotorTargets([1], [0.3]),          # not a position motor
+                 lambda: g.setMotorTorques([1], [-1.0]),         # negative effort
+                 lambda: g.setMotorTorques([0], [1.0]),          # no authored motor
+                 lambda: g.setMotorTargets([9], [0.0])):         # out of range
+        with pytest.raises(AVBDError):
+            call()
+    g.setMotorTargets([0, 2], [0.7, -0.4])                       # joint 0 has no motor: ignored
+    g.setMotorTorques([1, 2], [10.0, 60.0])
+    # with gravity the fragile joint breaks under the body's weight
+    h = GPUSolver(_motor_chain())
+    h.step(30)
+    assert h.debugBrokenJoints() == 1
+    ml, ma = h.debugMaxLambda()
+    assert ml > 0 and np.isfinite(ma)
+    # world anchors move the chain; poses zero the velocities
+    h.setJointWorldAnchors([0], [[0.5, 0.0, 3.0]])
+    h.step(120)
+    p, _, v, _ = h.bodyStates()
+    assert p[1][0] > 0.3
+    h.setBodyPoses([4], [[2.0, 2.0, 1.0]], [[0, 0, 0, 1]])
+    p, _, v, _ = h.bodyStates()
+    assert np.allclose(p[4], [2, 2, 1]) and np.all(v[4] == 0)
