@@ -139,6 +139,7 @@ SIGNATURES = {
     "avbd_get_profile": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_uint32, FP, U32P]),
     "avbd_time_primal": (C.c_int, [C.c_void_p, C.c_int32, FP, C.POINTER(C.c_double), I32P]),
     "avbd_test_deny_stage": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "avbd_test_fail_completion": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "avbd_mlp_create": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "avbd_mlp_info": (C.c_int, [C.c_void_p, I32P, I32P]),
     "avbd_mlp_destroy": (None, [C.c_void_p]),

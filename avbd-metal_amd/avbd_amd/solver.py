@@ -365,6 +365,12 @@ class GPUSolver:
         keys = [k for k in names.value.decode().split(",") if k]
         return dict(zip(keys, ms[:n.value].tolist()))
 
+    def failCompletionForTesting(self, kind, frame=-1):
+        """completionFailureForTesting: frame `frame` completes with failure
+        `kind` (a RuntimeFailure kind name such as "commandExecution")."""
+        codes = {v: k for k, v in _capi.FAILURE_NAMES.items()}
+        self._check(self._lib.avbd_test_fail_completion(self._h, codes[kind], int(frame)), "failCompletionForTesting")
+
     def denyStageForTesting(self, stage, frame=-1):
         self._lib.avbd_test_deny_stage(self._h, stage.encode() if stage else None, int(frame))
 

@@ -94,6 +94,8 @@ struct avbd_solver {
   avbd_stats stats{};
   // test hook
   std::string deniedStage;
+  int completionFailKind = 0;        // completion failure injected at retire (0 = none)
+  int64_t completionFailFrame = -1;
   int64_t deniedFrame = -1;
   // profiling
   bool profiling = false;
@@ -370,6 +372,13 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
     char m[160];
     snprintf(m, sizeof(m), "frame %lld did not complete: %s", (long long)frame, hipGetErrorString(e));
     return latch(s, AVBD_FAIL_COMMAND_EXECUTION, frame, 0, 0, (int)e, m);
+  }
+  if (s->completionFailKind != 0 && (s->completionFailFrame < 0 || s->completionFailFrame == frame)) {
+    const int kind = s->completionFailKind;
+    s->completionFailKind = 0;  // one shot
+    char m[160];
+    snprintf(m, sizeof(m), "physics frame %lld: completion failure injected for testing", (long long)frame);
+    return latch(s, kind, frame, 0, 0, 0, m);
   }
   const uint32_t* c = s->pinnedCounters + slot * CTR_TOTAL;
   s->stats.frame_index = frame;
@@ -2537,6 +2546,13 @@ extern "C" avbd_status avbd_get_warm_start_samples(avbd_solver* s, uint32_t capa
       (t && hipMemcpy(t, s->W.logT, c * kMlpOut * 4, hipMemcpyDeviceToHost)) ||
       (idx && hipMemcpy(idx, s->W.idx, c * 4, hipMemcpyDeviceToHost)))
     return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, int64_t frame) {
+  if (!s || kind < 0 || kind > AVBD_FAIL_DAT_INVALID_ANCHOR) return AVBD_ERR_INVALID_ARGUMENT;
+  s->completionFailKind = kind;
+  s->completionFailFrame = frame;
   return AVBD_OK;
 }
 

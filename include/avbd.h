@@ -369,6 +369,10 @@ avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg_launch_us,
  * "narrowphase", "warmstart", "adjacency", "coloring", "solve", "finalize")
  * on frame `frame` (-1 = every frame).  NULL clears. */
 avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t frame);
+/* completionFailureForTesting (GPUSolver.swift:2239): the frame `frame`
+ * (-1 = the next one retired) completes with failure `kind` (an
+ * avbd_failure_kind), latched like a real one; one shot, 0 clears. */
+avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, int64_t frame);
 
 /* ---- Learned warm start (no counterpart in the reference) ----------------
  * A per-body MLP 24 -> 128 -> 128 -> 3 (particles) or 6 (rigid bodies:

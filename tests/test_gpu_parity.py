@@ -258,6 +258,22 @@ def test_denied_stage_latches():
         g.synchronize()
 
 
+def test_injected_completion_failure_latches():
+    _case("completion failure injected for testing: latched, sticky, earliest wins")
+    s = demos.stack_scene(3)
+    g = GPUSolver(s)
+    g.step(2)
+    g.failCompletionForTesting("commandExecution", frame=4)
+    with pytest.raises(RuntimeFailure) as e:
+        for _ in range(6):
+            g.step()
+        g.synchronize()
+    assert e.value.kind == "commandExecution" and e.value.frame == 4
+    with pytest.raises(RuntimeFailure) as e2:
+        g.step()
+    assert e2.value.frame == 4          # the earliest failure stays
+
+
 def test_set_body_states_cold_starts_contacts():
     _case("setBodyStates clears incident warm starts")
     s = demos.stack_scene(3)
