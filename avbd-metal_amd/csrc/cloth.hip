@@ -23,7 +23,7 @@ constexpr float kDatGeometryEps = 1.0e-8f;
 constexpr uint32_t kDatLdsCap = 512u;  // per-producer sorted-emission buffer (LDS)
 
 // Ericson, Real-Time Collision Detection 5.1.5 (25_cloth.metal:26-61)
-__device__ V3 closest_pt_triangle(V3 p, V3 a, V3 b, V3 c, V3& bary) {
+__device__ __forceinline__ V3 closest_pt_triangle(V3 p, V3 a, V3 b, V3 c, V3& bary) {
   V3 ab = b - a, ac = c - a, ap = p - a;
   float d1 = dot(ab, ap), d2 = dot(ac, ap);
   if (d1 <= 0.0f && d2 <= 0.0f) { bary = v3(1, 0, 0); return a; }
@@ -59,7 +59,7 @@ __device__ V3 closest_pt_triangle(V3 p, V3 a, V3 b, V3 c, V3& bary) {
 }
 
 // closest points of two segments, parameters s on p, t on q (25_cloth.metal:632-648)
-__device__ void closest_seg_seg(V3 p0, V3 p1, V3 q0, V3 q1, float& s, float& t) {
+__device__ __forceinline__ void closest_seg_seg(V3 p0, V3 p1, V3 q0, V3 q1, float& s, float& t) {
   V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
   float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
   s = 0.0f;
@@ -153,6 +153,10 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
   if (gid >= P.numTris + P.numEdges) return;
   V3 lo, hi;
   elem_bounds(S, P, gid, lo, hi);
+  if (!SCATTER) {
+    S.elemBox[2 * gid] = f4(lo, 0.0f);
+    S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
+  }
   if (!finite3(lo) || !finite3(hi)) {
     if (!SCATTER) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
     return;
@@ -189,14 +193,28 @@ __global__ void __launch_bounds__(256) k_copy_u32(uint32_t* dst, const uint32_t*
 // ---------------------------------------------------------------------------
 namespace {
 
-__device__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell, uint32_t ns,
+// Early rejection by the element's padded box (its tight box grown by the
+// query radius, stored by k_el_bin): a partner within the radius lies inside
+// it.  The test box is grown by a further kBoxSlack so rounding of the stored
+// box can never reject a pair the exact test accepts.
+constexpr float kBoxSlack = 1.0e-5f;
+__device__ __forceinline__ bool outside_box(V3 p, F4 lo, F4 hi, float slack) {
+  return p.x < lo.x - slack || p.y < lo.y - slack || p.z < lo.z - slack || p.x > hi.x + slack ||
+         p.y > hi.y + slack || p.z > hi.z + slack;
+}
+__device__ __forceinline__ bool boxes_apart(V3 alo, V3 ahi, F4 lo, F4 hi, float slack) {
+  return ahi.x < lo.x - slack || ahi.y < lo.y - slack || ahi.z < lo.z - slack || alo.x > hi.x + slack ||
+         alo.y > hi.y + slack || alo.z > hi.z + slack;
+}
+
+__device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell, uint32_t ns,
                           uint32_t ne, uint32_t entry, float q2) {
   if ((entry & kElemEdgeBit) != 0u || entry >= P.numTris) return false;
+  const F4 blo = S.elemBox[2 * entry], bhi = S.elemBox[2 * entry + 1];
+  if (outside_box(p, blo, bhi, kBoxSlack)) return false;
   U4 t = S.tris[entry];
   if (t.x == v || t.y == v || t.z == v) return false;
-  V3 lo, hi;
-  elem_bounds(S, P, entry, lo, hi);
-  if (!in_range(range_of(lo, hi, P.elemCellSize), cell)) return false;
+  if (!in_range(range_of(xyz(blo), xyz(bhi), P.elemCellSize), cell)) return false;
   V3 bary;
   V3 q = closest_pt_triangle(p, P3(S, t.x), P3(S, t.y), P3(S, t.z), bary);
   V3 d = p - q;
@@ -210,17 +228,17 @@ __device__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v,
                     nbr_contains(S.nbrList, ns, ne, t.z)));
 }
 
-__device__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
+__device__ __forceinline__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
                           const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1, Cell3 cell,
                           uint32_t entry, float q2) {
   if ((entry & kElemEdgeBit) == 0u) return false;
   uint32_t other = entry & ~kElemEdgeBit;
   if (other <= gid || other >= P.numEdges) return false;
+  const F4 blo = S.elemBox[2 * (P.numTris + other)], bhi = S.elemBox[2 * (P.numTris + other) + 1];
+  if (boxes_apart(vmin(a0, a1), vmax(a0, a1), blo, bhi, kBoxSlack)) return false;
   U2 b = S.edges[other];
   if (b.x == a.x || b.x == a.y || b.y == a.x || b.y == a.y) return false;
-  V3 blo, bhi;
-  elem_bounds(S, P, P.numTris + other, blo, bhi);
-  CellRange br = range_of(blo, bhi, P.elemCellSize);
+  CellRange br = range_of(xyz(blo), xyz(bhi), P.elemCellSize);
   // owned by the minimum cell of tight(A) x padded(B)
   Cell3 o0{max(ar.c0.x, br.c0.x), max(ar.c0.y, br.c0.y), max(ar.c0.z, br.c0.z)};
   if (!in_range(ar, o0) || !in_range(br, o0)) return false;
@@ -298,10 +316,9 @@ __device__ __forceinline__ U4 dat_record(const StepParams& P, uint32_t prod, uin
 
 }  // namespace
 
-// One wave (64 lanes) per producer, four producers per block.  The count
-// pass writes datCount[prod]; the emit pass gathers the accepted partners in
-// LDS and writes them ranked by partner index at datStart[prod].
-template <bool EMIT>
+// Emit pass: one wave (64 lanes) per producer with pairs, four producers
+// per block; the accepted partners are gathered in LDS and written ranked by
+// partner index at datStart[prod] (k_dat_count below is the count pass).
 __global__ void __launch_bounds__(256) k_dat_pairs(DeviceState S, StepParams P) {
   __shared__ uint32_t buf[4][kDatLdsCap];
   const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -309,26 +326,22 @@ __global__ void __launch_bounds__(256) k_dat_pairs(DeviceState S, StepParams P) 
   const uint32_t nProd = P.numParticles + P.numEdges;
   const bool valid = prod < nProd;
   uint32_t start = 0, want = 0;
-  if (EMIT && valid) {
+  if (valid) {
     start = S.datStart[prod];
     want = S.datStart[prod + 1] - start;
   }
   const bool big = want > kDatLdsCap;
   uint32_t n = 0;
-  if (valid && !(EMIT && (want == 0u || big))) {
+  if (valid && want != 0u && !big) {
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    dat_visit(S, P, prod, lane, 64u, !EMIT, [&](bool acc, uint32_t other) {
+    dat_visit(S, P, prod, lane, 64u, false, [&](bool acc, uint32_t other) {
       uint64_t m = __ballot(acc);
-      if (EMIT && acc) {
+      if (acc) {
         uint32_t r = n + (uint32_t)__popcll(m & below);
         if (r < kDatLdsCap) buf[wv][r] = other;
       }
       n += (uint32_t)__popcll(m);
     });
-  }
-  if (!EMIT) {
-    if (valid && lane == 0) S.datCount[prod] = n;
-    return;
   }
   __syncthreads();
   if (!valid || want == 0u) return;
@@ -364,6 +377,18 @@ __global__ void __launch_bounds__(256) k_dat_pairs(DeviceState S, StepParams P) 
     }
     cnt++;
   });
+}
+
+// Count pass: 16 lanes per producer, four producers per wave (a producer's
+// bucket rarely fills 64 lanes), per-lane counts summed within the group.
+__global__ void __launch_bounds__(256) k_dat_count(DeviceState S, StepParams P) {
+  const uint32_t sub = threadIdx.x & 15u;
+  const uint32_t prod = blockIdx.x * 16u + (threadIdx.x >> 4);
+  const uint32_t nProd = P.numParticles + P.numEdges;
+  uint32_t n = 0;
+  if (prod < nProd) dat_visit(S, P, prod, sub, 16u, true, [&](bool acc, uint32_t) { n += acc ? 1u : 0u; });
+  for (int o = 8; o > 0; o >>= 1) n += __shfl_xor(n, o, 16);
+  if (prod < nProd && sub == 0) S.datCount[prod] = n;
 }
 
 __global__ void k_dat_finalize(DeviceState S, StepParams P) {
@@ -832,9 +857,9 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   hipLaunchKernelGGL(k_el_bin<true>, dim3(eb), dim3(256), 0, st, S, P);
   uint32_t nProd = P.numParticles + P.numEdges;
   uint32_t pb = (nProd + 3) / 4;
-  hipLaunchKernelGGL(k_dat_pairs<false>, dim3(pb), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_dat_count, dim3((nProd + 15) / 16), dim3(256), 0, st, S, P);
   launch_exclusive_scan(st, S.datCount, S.datStart, nProd + 1, S.scanScratch);
-  hipLaunchKernelGGL(k_dat_pairs<true>, dim3(pb), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_dat_pairs, dim3(pb), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_dat_finalize, dim3(1), dim3(1), 0, st, S, P);
 }
 

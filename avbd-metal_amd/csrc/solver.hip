@@ -1342,6 +1342,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.selfCollision, cl ? nb : 1u)
     ALLOC(S.elemCellCount, cl ? s->elemHashSize : 1u) ALLOC(S.elemCellStart, cl ? s->elemHashSize : 1u)
     ALLOC(S.elemCellCursor, cl ? s->elemHashSize : 1u) ALLOC(S.elemCells, std::max<uint64_t>(1, 64ull * nElem))
+    ALLOC(S.elemBox, std::max<uint32_t>(2, 2 * nElem))
     ALLOC(S.datCount, nProd + 1) ALLOC(S.datStart, nProd + 1)
     ALLOC(S.datPairs, cl ? s->maxDatPairs : 1u)
     uint32_t flagN = cl ? std::max(s->nTris, s->maxDatPairs) + 1 : 1u;

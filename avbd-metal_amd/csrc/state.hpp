@@ -68,6 +68,7 @@ struct DeviceState {
   uint32_t* particleIdx;      // surface particles, ascending
   uint32_t* selfCollision;    // per body: its surface self-collides
   uint32_t *elemCellCount, *elemCellStart, *elemCellCursor, *elemCells;
+  F4* elemBox;                // per element (triangles, then edges): padded box lo, hi
   uint32_t *datCount, *datStart;  // per producer: surface particles, then edges (+1 total)
   U4* datPairs;               // (kind, owner, other, 0), producer-major, other ascending
   uint32_t *softFlag, *softStart;  // emission counts / offsets (RT per triangle, DAT per pair)
