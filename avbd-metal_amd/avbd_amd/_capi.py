@@ -91,7 +91,10 @@ class Stats(C.Structure):
                 ("lanes_per_body", C.c_int32), ("sweep_mode", C.c_int32),
                 ("dat_pairs", C.c_int32), ("dat_vt_pairs", C.c_int32), ("dat_ee_pairs", C.c_int32),
                 ("num_soft_rt", C.c_int32), ("ws_accepted", C.c_int32),
-                ("ws_residual_plain", C.c_float), ("ws_residual_learned", C.c_float)]
+                ("ws_residual_plain", C.c_float), ("ws_residual_learned", C.c_float),
+                ("dat_truncations", C.c_int32), ("dat_invalid_anchors", C.c_int32),
+                ("dat_vt_degeneracies", C.c_int32), ("dat_ee_degeneracies", C.c_int32),
+                ("dat_nonfinite", C.c_int32)]
 
 
 class ClothInfo(C.Structure):
@@ -140,6 +143,7 @@ SIGNATURES = {
     "avbd_time_primal": (C.c_int, [C.c_void_p, C.c_int32, FP, C.POINTER(C.c_double), I32P]),
     "avbd_test_deny_stage": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "avbd_test_fail_completion": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "avbd_test_set_dat_query_radius": (C.c_int, [C.c_void_p, C.c_float]),
     "avbd_mlp_create": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "avbd_mlp_info": (C.c_int, [C.c_void_p, I32P, I32P]),
     "avbd_mlp_destroy": (None, [C.c_void_p]),

@@ -371,6 +371,10 @@ class GPUSolver:
         codes = {v: k for k, v in _capi.FAILURE_NAMES.items()}
         self._check(self._lib.avbd_test_fail_completion(self._h, codes[kind], int(frame)), "failCompletionForTesting")
 
+    def setDATQueryRadiusForTesting(self, r):
+        """params.planarDATQueryRadius = r (PlanarDATTests.swift:930); r <= 0 restores."""
+        self._check(self._lib.avbd_test_set_dat_query_radius(self._h, float(r)), "setDATQueryRadiusForTesting")
+
     def denyStageForTesting(self, stage, frame=-1):
         self._lib.avbd_test_deny_stage(self._h, stage.encode() if stage else None, int(frame))
 

@@ -158,7 +158,10 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
     S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
   }
   if (!finite3(lo) || !finite3(hi)) {
-    if (!SCATTER) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    if (!SCATTER) {
+      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+      atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    }
     return;
   }
   CellRange r = range_of(lo, hi, P.elemCellSize);
@@ -268,7 +271,10 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
     uint32_t v = S.particleIdx[prod];
     V3 p = P3(S, v);
     if (!finite3(p)) {
-      if (diag && lane == 0) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      if (diag && lane == 0) {
+        atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+        atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      }
       return;
     }
     Cell3 cell = cell_of(p, P.elemCellSize);
@@ -287,7 +293,10 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
   U2 a = S.edges[gid];
   V3 a0 = P3(S, a.x), a1 = P3(S, a.y);
   if (!finite3(a0) || !finite3(a1)) {
-    if (diag && lane == 0) atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    if (diag && lane == 0) {
+      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+      atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    }
     return;
   }
   CellRange ar = range_of(vmin(a0, a1), vmax(a0, a1), P.elemCellSize);
@@ -391,13 +400,269 @@ __global__ void __launch_bounds__(256) k_dat_count(DeviceState S, StepParams P) 
   if (prod < nProd && sub == 0) S.datCount[prod] = n;
 }
 
+// The frame's demand counters keep the larger of its two queries (step
+// start and accepted predictor pose); the working count of the stream in
+// use is datStart[numParticles + numEdges].
 __global__ void k_dat_finalize(DeviceState S, StepParams P) {
   uint32_t nProd = P.numParticles + P.numEdges;
   uint32_t total = S.datStart[nProd];
   uint32_t vt = S.datStart[P.numParticles];
-  S.counters[CTR_DAT_PAIRS] = total;
-  S.counters[CTR_DAT_VT] = vt;
-  S.counters[CTR_DAT_EE] = total - vt;
+  atomicMax(&S.counters[CTR_DAT_PAIRS], total);
+  atomicMax(&S.counters[CTR_DAT_VT], vt);
+  atomicMax(&S.counters[CTR_DAT_EE], total - vt);
+}
+
+// ---------------------------------------------------------------------------
+// Planar-DAT truncation (27_planar_dat.metal:536-560, 291-337, 756-906,
+// 1176-1261, 1289-1310).  Each pair of the stream places a division plane
+// between its sides at the anchor pose; a vertex approaching it keeps the
+// fraction of its displacement that stops short of it (relaxation 0.85,
+// time margin 1e-3).  The reduction takes the minimum over a vertex's pairs
+// with an unsigned atomic minimum on the f32 bits (fractions are in
+// [0, 1], where bit order is value order), so the result does not depend on
+// the schedule; the apply pass consumes the fraction, restores 1.0 and caps
+// the displacement at 0.5 * 0.85 * rq.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr float kDatRelax = 0.85f, kDatTimeMargin = 1.0e-3f;
+constexpr uint32_t kDatFullPass = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void dat_lower(const DeviceState& S, uint32_t v, float t) {
+  if (!(t > 0.0f)) t = 0.0f;  // +0 for -0 (unsigned order)
+  atomicMin(&S.datT[v], as_uint(t));
+}
+
+__device__ __forceinline__ float dat_plane_fraction(const DeviceState& S, V3 x, V3 dx, V3 n, V3 plane, float side,
+                                                    bool report) {
+  float s0 = side * dot(n, x - plane);
+  float q = side * dot(n, dx);
+  if (!finite_bits(s0) || !finite_bits(q)) {
+    if (report) {
+      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+      atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    }
+    return 0.0f;
+  }
+  if (s0 < -kDatGeometryEps) return 0.0f;
+  // an observed sign change is constrained however small the motion
+  if (q >= 0.0f || s0 + q > 0.0f) return 1.0f;
+  float hit = fmaxf(s0, 0.0f) / (-q);
+  if (hit > 1.0f + kDatGeometryEps) return 1.0f;
+  return clampf(fminf(kDatRelax * hit, hit - kDatTimeMargin), 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ void dat_vertex(const DeviceState& S, uint32_t v, V3 x, V3 dx, V3 n, V3 plane,
+                                           float side) {
+  float t = dat_plane_fraction(S, x, dx, n, plane, side, true);
+  if (t < 1.0f) dat_lower(S, v, t);
+}
+
+__device__ __forceinline__ bool dat_vt_fallback(V3 a, V3 b, V3 c, V3 rel, V3& n) {
+  V3 raw = cross(b - a, c - a);
+  float mag = sqrtf(dot(raw, raw));
+  if (!(finite_bits(mag) && mag >= kDatGeometryEps)) return false;
+  n = raw / mag;
+  if (dot(n, rel) > 0.0f) n = -n;
+  return true;
+}
+
+__device__ __forceinline__ bool dat_ee_fallback(V3 a0, V3 a1, V3 b0, V3 b1, V3 rel, V3& n) {
+  V3 ua = a1 - a0, ub = b1 - b0;
+  V3 raw = cross(ua, ub);
+  float mag = sqrtf(dot(raw, raw));
+  if (finite_bits(mag) && mag >= kDatGeometryEps) {
+    n = raw / mag;
+  } else {
+    float la = sqrtf(dot(ua, ua)), lb = sqrtf(dot(ub, ub));
+    V3 u = la >= lb ? ua : ub;
+    float ul = fmaxf(la, lb);
+    if (!finite_bits(ul) || ul < kDatGeometryEps) return false;
+    u = u / ul;
+    V3 perp = rel - u * dot(u, rel);
+    float pl = sqrtf(dot(perp, perp));
+    if (!finite_bits(pl) || pl < kDatGeometryEps) {
+      V3 axis = fabsf(u.x) < 0.7f ? v3(1, 0, 0) : v3(0, 1, 0);
+      perp = cross(u, axis);
+      pl = sqrtf(dot(perp, perp));
+    }
+    if (!finite_bits(pl) || pl < kDatGeometryEps) return false;
+    n = perp / pl;
+  }
+  if (dot(n, rel) > 0.0f) n = -n;
+  return finite3(n);
+}
+
+__device__ __forceinline__ bool dat_active(const DeviceState& S, uint32_t b, uint32_t color) {
+  return color == kDatFullPass || (S.posLin[b].w > 0.0f && S.colors[b] == color);
+}
+
+__device__ __forceinline__ void dat_freeze4(const DeviceState& S, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  dat_lower(S, a, 0.0f);
+  dat_lower(S, b, 0.0f);
+  dat_lower(S, c, 0.0f);
+  dat_lower(S, d, 0.0f);
+}
+
+__device__ void dat_reduce_pair(const DeviceState& S, const StepParams& P, const F4* anchor, U4 pr, uint32_t color) {
+  const float rq = P.datQueryRadius;
+  if (pr.x == kDatVT) {
+    const uint32_t v = pr.y;
+    const U4 t = S.tris[pr.z];
+    const bool aV = dat_active(S, v, color), aA = dat_active(S, t.x, color), aB = dat_active(S, t.y, color),
+               aC = dat_active(S, t.z, color);
+    if (!aV && !aA && !aB && !aC) return;
+    V3 xv = xyz(anchor[v]), a = xyz(anchor[t.x]), b = xyz(anchor[t.y]), c = xyz(anchor[t.z]);
+    V3 bary;
+    V3 cp = closest_pt_triangle(xv, a, b, c, bary);
+    V3 nh = xv - cp;
+    float gap = sqrtf(dot(nh, nh));
+    if (!finite_bits(gap)) {
+      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+      atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      dat_freeze4(S, v, t.x, t.y, t.z);
+      return;
+    }
+    if (gap > rq) return;
+    V3 dv = P3(S, v) - xv, da = P3(S, t.x) - a, db = P3(S, t.y) - b, dc = P3(S, t.z) - c;
+    V3 n;
+    if (gap >= kDatGeometryEps) {
+      n = nh / gap;
+    } else {
+      if (!dat_vt_fallback(a, b, c, dv - (da + db + dc) / 3.0f, n)) {
+        atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+        atomicAdd(&S.counters[CTR_DAT_VT_DEGEN], 1u);
+        dat_freeze4(S, v, t.x, t.y, t.z);
+        return;
+      }
+      atomicAdd(&S.counters[CTR_DAT_VT_DEGEN], 1u);
+    }
+    float apV = fmaxf(-dot(n, dv), 0.0f);
+    float apT = fmaxf(fmaxf(dot(n, da), dot(n, db)), fmaxf(dot(n, dc), 0.0f));
+    float lambda = apV + apT == 0.0f ? 0.5f : clampf(apT / (apV + apT), 0.05f, 0.95f);
+    V3 plane = gap >= kDatGeometryEps ? cp + nh * lambda : cp;
+    if (apV > 0.0f && aV) dat_vertex(S, v, xv, dv, n, plane, 1.0f);
+    if (apT > 0.0f) {
+      if (aA) dat_vertex(S, t.x, a, da, n, plane, -1.0f);
+      if (aB) dat_vertex(S, t.y, b, db, n, plane, -1.0f);
+      if (aC) dat_vertex(S, t.z, c, dc, n, plane, -1.0f);
+    }
+    return;
+  }
+  const U2 ea = S.edges[pr.y], eb = S.edges[pr.z];
+  const bool aA0 = dat_active(S, ea.x, color), aA1 = dat_active(S, ea.y, color), aB0 = dat_active(S, eb.x, color),
+             aB1 = dat_active(S, eb.y, color);
+  if (!aA0 && !aA1 && !aB0 && !aB1) return;
+  V3 a0 = xyz(anchor[ea.x]), a1 = xyz(anchor[ea.y]), b0 = xyz(anchor[eb.x]), b1 = xyz(anchor[eb.y]);
+  float s, t;
+  closest_seg_seg(a0, a1, b0, b1, s, t);
+  V3 ca = a0 + (a1 - a0) * s, cb = b0 + (b1 - b0) * t;
+  V3 nh = ca - cb;
+  float gap = sqrtf(dot(nh, nh));
+  if (!finite_bits(gap)) {
+    atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+    atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    dat_freeze4(S, ea.x, ea.y, eb.x, eb.y);
+    return;
+  }
+  if (gap > rq) return;
+  V3 da0 = P3(S, ea.x) - a0, da1 = P3(S, ea.y) - a1, db0 = P3(S, eb.x) - b0, db1 = P3(S, eb.y) - b1;
+  V3 n;
+  if (gap > kDatGeometryEps) {
+    n = nh / gap;
+  } else {
+    if (!dat_ee_fallback(a0, a1, b0, b1, (da0 + da1 - db0 - db1) * 0.5f, n)) {
+      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+      atomicAdd(&S.counters[CTR_DAT_EE_DEGEN], 1u);
+      dat_freeze4(S, ea.x, ea.y, eb.x, eb.y);
+      return;
+    }
+    atomicAdd(&S.counters[CTR_DAT_EE_DEGEN], 1u);
+  }
+  float apA = fmaxf(fmaxf(-dot(n, da0), -dot(n, da1)), 0.0f);
+  float apB = fmaxf(fmaxf(dot(n, db0), dot(n, db1)), 0.0f);
+  float lambda = apA + apB == 0.0f ? 0.5f : clampf(apB / (apA + apB), 0.05f, 0.95f);
+  V3 plane = gap > kDatGeometryEps ? cb + nh * lambda : cb;
+  if (apA > 0.0f) {
+    if (aA0) dat_vertex(S, ea.x, a0, da0, n, plane, 1.0f);
+    if (aA1) dat_vertex(S, ea.y, a1, da1, n, plane, 1.0f);
+  }
+  if (apB > 0.0f) {
+    if (aB0) dat_vertex(S, eb.x, b0, db0, n, plane, -1.0f);
+    if (aB1) dat_vertex(S, eb.y, b1, db1, n, plane, -1.0f);
+  }
+}
+
+}  // namespace
+
+// one thread per pair of the stream in use; `color` = kDatFullPass after
+// the predictor, else the static color whose primal just ran
+__global__ void __launch_bounds__(256) k_dat_reduce(DeviceState S, StepParams P, const F4* anchor, uint32_t color) {
+  const uint32_t total = S.datStart[P.numParticles + P.numEdges];
+  const uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dat_reduce_pair(S, P, anchor, S.datPairs[i], color);
+}
+
+// full pass: every surface particle (also refreshing the start-of-iteration
+// snapshot); color pass: the color's bodies from colorWork[start, start+count)
+__global__ void __launch_bounds__(256) k_dat_apply(DeviceState S, StepParams P, const F4* anchor, uint32_t full,
+                                                   uint32_t start, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t v;
+  if (full) {
+    if (i >= P.numParticles) return;
+    v = S.particleIdx[i];
+  } else {
+    if (i >= count) return;
+    v = S.colorWork[start + i].x;
+    if (S.clothGroup[v] == 0u) return;
+  }
+  const F4 cur = S.posLin[v];
+  const V3 x0 = xyz(anchor[v]);
+  V3 d = xyz(cur) - x0;
+  const bool unsafeCap = S.datStart[P.numParticles + P.numEdges] > P.maxDatPairs ||
+                         S.counters[CTR_DAT_PAIRS] > P.maxDatPairs || S.counters[CTR_DAT_OVERFLOW] > 0u ||
+                         __hip_atomic_load(&S.counters[CTR_DAT_INVALID], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0u;
+  float t = as_float(S.datT[v]);
+  S.datT[v] = as_uint(1.0f);
+  if (unsafeCap) t = 0.0f;
+  if (!finite3(d) || !finite_bits(t)) {
+    atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+    atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+    d = v3s(0.0f);
+    t = 0.0f;
+  }
+  bool truncated = t < 1.0f;
+  d = d * clampf(t, 0.0f, 1.0f);
+  const float maxD = 0.5f * kDatRelax * P.datQueryRadius;
+  const float d2 = dot(d, d);
+  if (d2 > maxD * maxD) {
+    d = d * (maxD / sqrtf(d2));
+    truncated = true;
+  }
+  if (truncated) {
+    const F4 np = f4(x0 + d, cur.w);
+    S.posLin[v] = np;
+    if (full) S.posLinSnap[v] = np;
+    atomicAdd(&S.counters[CTR_DAT_TRUNC], 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dat_reanchor(DeviceState S, StepParams P) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.numParticles) return;
+  const uint32_t v = S.particleIdx[i];
+  S.datAnchor[v] = f4(xyz(S.posLin[v]), 0.0f);
+}
+
+// a failed surface query restores every particle to its step-start pose
+__global__ void __launch_bounds__(256) k_dat_restore(DeviceState S, StepParams P) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.numBodies || S.shape[b].w >= 0.0f) return;
+  const bool failed = S.counters[CTR_SOFT_CANDIDATES] > P.maxSoft || S.counters[CTR_DAT_PAIRS] > P.maxDatPairs ||
+                      S.counters[CTR_DAT_OVERFLOW] > 0u || S.counters[CTR_DAT_INVALID] > 0u;
+  if (failed) S.posLin[b] = f4(xyz(S.initLin[b]), S.posLin[b].w);
 }
 
 // ---------------------------------------------------------------------------
@@ -762,7 +1027,7 @@ __device__ bool dat_contact(const DeviceState& S, const StepParams& P, U4 pair, 
 
 template <bool EMIT>
 __global__ void __launch_bounds__(256) k_dat_contacts(DeviceState S, StepParams P) {
-  uint32_t total = S.counters[CTR_DAT_PAIRS];
+  uint32_t total = S.datStart[P.numParticles + P.numEdges];
   uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
   uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (!EMIT && i0 == 0) S.softFlag[n] = 0u;
@@ -867,12 +1132,40 @@ void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams&
   if (P.numTris == 0) return;
   uint32_t blocks = cap_blocks(P.maxDatPairs, gridCap);
   hipLaunchKernelGGL(k_dat_contacts<false>, dim3(blocks), dim3(256), 0, st, S, P);
-  launch_exclusive_scan_dev(st, S.softFlag, S.softStart, P.maxDatPairs + 1, &S.counters[CTR_DAT_PAIRS], P.maxDatPairs,
-                            S.scanScratch);
+  launch_exclusive_scan_dev(st, S.softFlag, S.softStart, P.maxDatPairs + 1, S.datStart + P.numParticles + P.numEdges,
+                            P.maxDatPairs, S.scanScratch);
   hipLaunchKernelGGL(k_dat_contacts<true>, dim3(blocks), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_soft_finalize, dim3(1), dim3(1), 0, st, S, P);
   hipLaunchKernelGGL(k_softmap_clear, dim3(cap_blocks(P.softMapCapacity, gridCap)), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_softmap_insert, dim3(cap_blocks(P.maxSoft, gridCap)), dim3(256), 0, st, S, P);
+}
+
+void launch_dat_predictor_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  if (P.numTris == 0) return;
+  hipLaunchKernelGGL(k_dat_reduce, dim3(cap_blocks(P.maxDatPairs, gridCap)), dim3(256), 0, st, S, P,
+                     (const F4*)S.initLin, kDatFullPass);
+  if (P.numParticles)
+    hipLaunchKernelGGL(k_dat_apply, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P,
+                       (const F4*)S.initLin, 1u, 0u, 0u);
+}
+
+void launch_dat_reanchor(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (P.numTris == 0 || P.numParticles == 0) return;
+  hipLaunchKernelGGL(k_dat_reanchor, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
+}
+
+void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t start,
+                           uint32_t count, uint32_t gridCap) {
+  if (P.numTris == 0 || count == 0) return;
+  hipLaunchKernelGGL(k_dat_reduce, dim3(cap_blocks(P.maxDatPairs, gridCap)), dim3(256), 0, st, S, P,
+                     (const F4*)S.datAnchor, color);
+  hipLaunchKernelGGL(k_dat_apply, dim3((count + 255) / 256), dim3(256), 0, st, S, P, (const F4*)S.datAnchor, 0u,
+                     start, count);
+}
+
+void launch_dat_restore(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (P.numTris == 0) return;
+  hipLaunchKernelGGL(k_dat_restore, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
 }
 
 void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P) {

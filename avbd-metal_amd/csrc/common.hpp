@@ -74,7 +74,11 @@ enum Counter : uint32_t {
   CTR_WS_RES_PLAIN,        // residual at the plain start (f32 bits)
   CTR_WS_RES_LEARNED,      // residual at the learned start (f32 bits)
   CTR_LARGE_CELLS,         // hash buckets of more than 32 colliders this frame
-  CTR_TOTAL = 16 + 2 * kMaxColors + 8
+  CTR_DAT_INVALID,         // invalid truncation anchors (non-finite or unresolvable planes)
+  CTR_DAT_TRUNC,           // surface particles truncated by a Planar-DAT pass
+  CTR_DAT_VT_DEGEN,        // coincident V-T anchors (fallback normal)
+  CTR_DAT_EE_DEGEN,        // coincident / parallel E-E anchors (fallback normal)
+  CTR_TOTAL = 16 + 2 * kMaxColors + 12
 };
 
 // ---------------------------------------------------------------------------

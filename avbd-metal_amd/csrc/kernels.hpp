@@ -28,6 +28,14 @@ void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 // OGC records from the pair stream, soft counts, soft persistence map rebuild
 void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+// Planar-DAT truncation: the full pass after the predictor (anchor = step
+// start), the anchor at the accepted predictor pose, the pass after each
+// static color, and the failed-surface restore before finalize
+void launch_dat_predictor_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+void launch_dat_reanchor(hipStream_t st, const DeviceState& S, const StepParams& P);
+void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t start,
+                           uint32_t count, uint32_t gridCap);
+void launch_dat_restore(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P);
 
 // solve.hip

@@ -63,6 +63,7 @@ struct avbd_solver {
   uint32_t nTris = 0, nEdges = 0, nParticles = 0, nMembranes = 0, nBends = 0;
   uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
   float elemCellSize = 0.0f, elemMargin = 0.0f, datQueryRadius = 0.0f, surfaceContactCellSize = 0.0f;
+  float datQueryRadiusDerived = 0.0f;  // rq derived at create (a test hook may replace datQueryRadius)
   int lanes = 8, sweepMode = 1;
   int persistentBlocks = 0;
   uint32_t gridCap = 1024;
@@ -233,23 +234,34 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   uint32_t zn[6] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->nTris ? s->elemHashSize : 0u,
                     s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
   launch_zero_multi(st, zp, zn, 6);
+  if (s->nTris) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
   launch_broadphase(st, S, P);
   stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
   launch_map_rebuild(st, S, P, s->gridCap);
   stage("narrowphase");
-  // cloth: rigid-feature contacts at the step-start pose; after the
-  // predictor, the V-T / E-E stream and its OGC contacts at the accepted
-  // predictor pose (GPUSolver.swift:3685-3870)
-  launch_rt_emit(st, S, P);
-  if (s->nTris) stage("rt-emit");
-  launch_warmstart(st, S, P);
-  stage("warmstart");
+  // cloth (GPUSolver.swift:3685-3870): the V-T / E-E query and the
+  // rigid-feature contacts at the step-start pose; the predictor and its
+  // full Planar-DAT pass; the query again at the accepted predictor pose,
+  // which feeds the OGC contacts and every per-color pass, anchored there
   if (s->nTris) {
     launch_dat_pairs(st, S, P, s->gridCap);
     stage("dat-pairs");
+    launch_rt_emit(st, S, P);
+    stage("rt-emit");
+  }
+  launch_warmstart(st, S, P);
+  stage("warmstart");
+  if (s->nTris) {
+    launch_dat_predictor_pass(st, S, P, s->gridCap);
+    stage("dat-predict");
+    launch_fill_u32(st, S.elemCellCount, s->elemHashSize, 0u);
+    launch_dat_pairs(st, S, P, s->gridCap);
+    stage("dat-accepted-pairs");
     launch_dat_contacts(st, S, P, s->gridCap);
     stage("vt-ee-emit");
+    launch_dat_reanchor(st, S, P);
+    stage("dat-reanchor");
   }
   launch_contact_adjacency(st, S, P, s->gridCap);
   stage("adjacency");
@@ -300,8 +312,12 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
         launch_primal_color(st, S, P, c, 0xFFFFFFFFu, 0u, s->lanes, kModeGeneral);
       launch_primal_tail(st, S, P, kDynamicColorLaunches, s->lanes);
     } else {
-      for (uint32_t c = 0; c < s->staticUsedColors; c++)
+      for (uint32_t c = 0; c < s->staticUsedColors; c++) {
         launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes, s->colorMode[c]);
+        // Planar-DAT after every color (GPUSolver.swift:4196-4229)
+        if (s->nTris)
+          launch_dat_color_pass(st, S, P, c, s->staticColorStarts[c], s->staticColorCounts[c], s->gridCap);
+      }
     }
     launch_dual(st, S, P, s->gridCap);
   }
@@ -314,6 +330,7 @@ void enqueue_post_solve(avbd_solver* s, const DeviceState& S, const StepParams& 
     launch_ws_target(s->stream, S, W);
     launch_ws_target_rigid(s->stream, S, W);
   }
+  launch_dat_restore(s->stream, S, P);
   launch_finalize(s->stream, S, P);
   launch_viscosity(s->stream, S, P);
   stage("finalize");
@@ -394,6 +411,11 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
   s->stats.ws_residual_learned = as_float(c[CTR_WS_RES_LEARNED]);
   s->stats.dat_ee_pairs = (int32_t)c[CTR_DAT_EE];
   s->stats.num_soft_rt = (int32_t)c[CTR_SOFT_RT];
+  s->stats.dat_truncations = (int32_t)c[CTR_DAT_TRUNC];
+  s->stats.dat_invalid_anchors = (int32_t)c[CTR_DAT_INVALID];
+  s->stats.dat_vt_degeneracies = (int32_t)c[CTR_DAT_VT_DEGEN];
+  s->stats.dat_ee_degeneracies = (int32_t)c[CTR_DAT_EE_DEGEN];
+  s->stats.dat_nonfinite = (int32_t)c[CTR_DAT_NONFINITE];
   s->stats.color_conflicts = (int32_t)c[CTR_COLOR_CONFLICTS];
   if (s->dynamicColoring) {
     int maxUsed = -1;
@@ -422,21 +444,23 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
   }
   if (s->nTris && (int64_t)c[CTR_DAT_PAIRS] > (int64_t)s->maxDatPairs) {
     char m[160];
-    snprintf(m, sizeof(m), "physics frame %lld: Planar-DAT pair demand %u exceeds capacity %u", (long long)frame,
+    snprintf(m, sizeof(m), "physics frame %lld: Planar-DAT safety-pair demand %u exceeds capacity %u", (long long)frame,
              c[CTR_DAT_PAIRS], s->maxDatPairs);
     return latch(s, AVBD_FAIL_DAT_PAIR_CAPACITY, frame, c[CTR_DAT_PAIRS], s->maxDatPairs, 0, m);
   }
   if (s->nTris && c[CTR_DAT_OVERFLOW] > 0) {
     char m[160];
-    snprintf(m, sizeof(m), "physics frame %lld: %u surface elements span more than 4 grid cells", (long long)frame,
-             c[CTR_DAT_OVERFLOW]);
+    snprintf(m, sizeof(m), "physics frame %lld: %u Planar-DAT element AABBs exceeded the supported 4x4x4 grid span",
+             (long long)frame, c[CTR_DAT_OVERFLOW]);
     return latch(s, AVBD_FAIL_DAT_GRID_SPAN, frame, c[CTR_DAT_OVERFLOW], 0, 0, m);
   }
-  if (s->nTris && c[CTR_DAT_NONFINITE] > 0) {
-    char m[160];
-    snprintf(m, sizeof(m), "physics frame %lld: %u non-finite surface anchors", (long long)frame,
-             c[CTR_DAT_NONFINITE]);
-    return latch(s, AVBD_FAIL_DAT_INVALID_ANCHOR, frame, c[CTR_DAT_NONFINITE], 0, 0, m);
+  if (s->nTris && c[CTR_DAT_INVALID] > 0) {
+    char m[200];
+    snprintf(m, sizeof(m),
+             "physics frame %lld: %u Planar-DAT anchors were invalid (V-T coincident %u, E-E degenerate %u, "
+             "non-finite %u)",
+             (long long)frame, c[CTR_DAT_INVALID], c[CTR_DAT_VT_DEGEN], c[CTR_DAT_EE_DEGEN], c[CTR_DAT_NONFINITE]);
+    return latch(s, AVBD_FAIL_DAT_INVALID_ANCHOR, frame, c[CTR_DAT_INVALID], 0, 0, m);
   }
   if (c[CTR_COLOR_CONFLICTS] > 0) {
     char m[160];
@@ -1144,6 +1168,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->elemCellSize = ch.elemCellSize;
     s->elemMargin = ch.elemMargin;
     s->datQueryRadius = ch.datQueryRadius;
+    s->datQueryRadiusDerived = ch.datQueryRadius;
     s->surfaceContactCellSize = ch.surfaceContactCellSize;
   } else if (nt > 0) {
     std::vector<uint8_t> surf = tet_surface_flags(d);
@@ -1343,6 +1368,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.elemCellCount, cl ? s->elemHashSize : 1u) ALLOC(S.elemCellStart, cl ? s->elemHashSize : 1u)
     ALLOC(S.elemCellCursor, cl ? s->elemHashSize : 1u) ALLOC(S.elemCells, std::max<uint64_t>(1, 64ull * nElem))
     ALLOC(S.elemBox, std::max<uint32_t>(2, 2 * nElem))
+    ALLOC(S.datAnchor, cl ? nb : 1u) ALLOC(S.datT, cl ? nb : 1u)
     ALLOC(S.datCount, nProd + 1) ALLOC(S.datStart, nProd + 1)
     ALLOC(S.datPairs, cl ? s->maxDatPairs : 1u)
     uint32_t flagN = cl ? std::max(s->nTris, s->maxDatPairs) + 1 : 1u;
@@ -1526,6 +1552,8 @@ void autotune(avbd_solver* s) {
   if (wantMode == 1 || wantMode == 2) modeCands = {wantMode};
   // dynamic coloring with per-color launches would dispatch all 64 colors
   if (s->dynamicColoring && wantMode == 0) modeCands = {2, 1};
+  // cloth: the Planar-DAT passes run between the colors' launches
+  if (s->nTris) modeCands = {1};
   if (laneCands.size() == 1 && modeCands.size() == 1) {
     s->lanes = laneCands[0];
     s->sweepMode = modeCands[0];
@@ -2177,7 +2205,8 @@ extern "C" avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uin
   *n = 0;
   if (!s->nTris) return AVBD_OK;
   uint32_t cnt = 0;
-  if (hipMemcpy(&cnt, s->S.counters + CTR_DAT_PAIRS, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  // the stream of the accepted predictor pose (its demand is the working count)
+  if (hipMemcpy(&cnt, s->S.datStart + s->nParticles + s->nEdges, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
   cnt = std::min(cnt, s->maxDatPairs);
   *n = cnt;
   uint32_t m = std::min(cnt, capacity);
@@ -2554,6 +2583,13 @@ extern "C" avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, i
   if (!s || kind < 0 || kind > AVBD_FAIL_DAT_INVALID_ANCHOR) return AVBD_ERR_INVALID_ARGUMENT;
   s->completionFailKind = kind;
   s->completionFailFrame = frame;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_test_set_dat_query_radius(avbd_solver* s, float r) {
+  if (!s || !(r == r)) return AVBD_ERR_INVALID_ARGUMENT;
+  s->datQueryRadius = r > 0.0f ? r : s->datQueryRadiusDerived;
+  invalidate_graphs(s);  // captured launches hold the old parameters
   return AVBD_OK;
 }
 

@@ -69,6 +69,8 @@ struct DeviceState {
   uint32_t* selfCollision;    // per body: its surface self-collides
   uint32_t *elemCellCount, *elemCellStart, *elemCellCursor, *elemCells;
   F4* elemBox;                // per element (triangles, then edges): padded box lo, hi
+  F4* datAnchor;              // Planar-DAT anchor pose of the color passes (accepted predictor pose)
+  uint32_t* datT;             // per body: step fraction of the running pass (f32 bits, 1.0 between passes)
   uint32_t *datCount, *datStart;  // per producer: surface particles, then edges (+1 total)
   U4* datPairs;               // (kind, owner, other, 0), producer-major, other ascending
   uint32_t *softFlag, *softStart;  // emission counts / offsets (RT per triangle, DAT per pair)

@@ -282,6 +282,11 @@ typedef struct avbd_stats {
     int32_t ws_accepted;           /* learned warm start kept (1) or fell back to the plain guess (0) */
     float ws_residual_plain;       /* residual at the plain adaptive guess (learned stage active) */
     float ws_residual_learned;     /* residual at the learned start */
+    int32_t dat_truncations;       /* lastPlanarDATTruncations: particles truncated by the Planar-DAT passes */
+    int32_t dat_invalid_anchors;   /* invalid Planar-DAT anchors (any > 0 latches AVBD_FAIL_DAT_INVALID_ANCHOR) */
+    int32_t dat_vt_degeneracies;   /* lastPlanarDATVertexTriangleDegeneracies */
+    int32_t dat_ee_degeneracies;   /* lastPlanarDATEdgeEdgeDegeneracies */
+    int32_t dat_nonfinite;         /* lastPlanarDATNonfiniteValues */
 } avbd_stats;
 avbd_status avbd_get_stats(avbd_solver* s, avbd_stats* out);
 
@@ -325,9 +330,11 @@ typedef struct avbd_cloth_info {
 avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
 
 /* Planar-DAT pair stream of the last frame (dat_build_vt_pairs /
- * dat_build_ee_pairs, 27_planar_dat.metal:35-229): pairs[3*n] = (kind 0 V-T
- * | 1 E-E, owner vertex / edge, triangle / edge), producer-major with the
- * partner ascending. */
+ * dat_build_ee_pairs, 27_planar_dat.metal:35-229), the query at the accepted
+ * predictor pose that feeds the contacts and the per-color truncation
+ * passes: pairs[3*n] = (kind 0 V-T | 1 E-E, owner vertex / edge, triangle /
+ * edge), producer-major with the partner ascending.  avbd_stats.dat_pairs
+ * is the larger demand of the frame's two queries. */
 avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs);
 
 /* Element contacts of the last frame (SoftContactGPU, GPUTypes.swift:127-136):
@@ -373,6 +380,11 @@ avbd_status avbd_test_deny_stage(avbd_solver* s, const char* stage, int64_t fram
  * (-1 = the next one retired) completes with failure `kind` (an
  * avbd_failure_kind), latched like a real one; one shot, 0 clears. */
 avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, int64_t frame);
+/* params.planarDATQueryRadius assignment of the Planar-DAT tests
+ * (PlanarDATTests.swift:930, 960, 990): replaces the query radius rq of
+ * later frames (pair query, element padding, truncation cap); the element
+ * grid keeps the cell size derived at create.  r <= 0 restores the derived rq. */
+avbd_status avbd_test_set_dat_query_radius(avbd_solver* s, float r);
 
 /* ---- Learned warm start (no counterpart in the reference) ----------------
  * A per-body MLP 24 -> 128 -> 128 -> 3 (particles) or 6 (rigid bodies:

@@ -62,6 +62,8 @@ def load():
         "oracle_get_dat_pairs": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
         "oracle_get_soft_contacts": (C.c_int, [vp, C.c_uint32, u32p, vp]),
         "oracle_cloth_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p]),
+        "oracle_dat_stats": (C.c_int, [vp, i32p]),
+        "oracle_set_dat_test_hooks": (C.c_int, [vp, C.c_float, C.c_int32, C.c_int32]),
         "oracle_dat_pairs_at": (C.c_int, [vp, fp, C.c_uint32, u32p, u32p]),
         "oracle_membrane": (None, [dp, dp, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_hinge": (C.c_int, [fp, C.c_float, C.c_float, fp, fp]),
@@ -223,6 +225,16 @@ class OracleSolver:
         self._lib.oracle_cloth_stats(self._h, *[C.byref(x) for x in v])
         return dict(zip(["dat_pairs", "dat_vt_pairs", "soft_candidates", "num_soft_rt", "overflow", "nonfinite"],
                         [x.value for x in v]))
+
+    def setDatTestHooks(self, queryRadius=0.0, pairCapacity=0, softCapacity=0):
+        """Planar-DAT query radius override and pair / soft capacity caps (0 keeps)."""
+        self._lib.oracle_set_dat_test_hooks(self._h, float(queryRadius), int(pairCapacity), int(softCapacity))
+
+    def datStats(self):
+        v = (C.c_int32 * 6)()
+        self._lib.oracle_dat_stats(self._h, v)
+        return dict(zip(["dat_truncations", "dat_invalid_anchors", "dat_vt_degeneracies", "dat_ee_degeneracies",
+                         "dat_nonfinite", "overflow"], list(v)))
 
     def residual(self):
         v = C.c_double()

@@ -285,6 +285,11 @@ struct Oracle : oracle_base {
   uint32_t maxDatPairs = 1, maxSoft = 1;
   std::vector<std::array<uint32_t, 3>> datPairs;
   uint32_t datTotal = 0, datVT = 0, datOverflow = 0, datNonfinite = 0, softCandidates = 0, softRT = 0;
+  // Planar-DAT truncation state: fixed-plane anchors, per-vertex step
+  // fractions, and the frame's counters (peak pair demand over both queries)
+  std::vector<V<T>> datAnchor;
+  std::vector<T> datT;
+  uint32_t datPeak = 0, datPeakVT = 0, datInvalid = 0, datTrunc = 0, datVTDeg = 0, datEEDeg = 0;
   std::vector<Soft> soft, prevSoft;
   std::map<std::pair<uint32_t, uint32_t>, int> softMap;  // keys of prevSoft -> index
   std::vector<std::vector<uint32_t>> softAdj;
@@ -639,6 +644,8 @@ struct Oracle : oracle_base {
         if (particle[v]) { clothGroup[v] = gid[r]; selfColl[v] = 1; surf[v] = 1; }
     }
     for (uint32_t i = 0; i < nb; i++) if (surf[i]) particles.push_back(i);
+    datAnchor.assign(nb, V<T>());
+    datT.assign(nb, T(1));
     // StVK membranes: rest 2x2 in the triangle frame, inverse column major
     std::map<uint64_t, std::vector<std::pair<uint32_t, float>>> edgeTris;
     for (uint32_t t = 0; t < nTris; t++) {
@@ -1759,6 +1766,10 @@ struct Oracle : oracle_base {
           }
       emit(1u, gid, acc);
     }
+    // the frame reports the larger demand of its two queries (dat_finalize_pairs,
+    // 27_planar_dat.metal:208-229, atomic max)
+    datPeak = std::max(datPeak, datTotal);
+    datPeakVT = std::max(datPeakVT, datVT);
   }
 
   bool datContact(const std::array<uint32_t, 3>& pr, Soft& out) const {
@@ -1879,6 +1890,200 @@ struct Oracle : oracle_base {
       softCandidates++;
       if (soft.size() < maxSoft) soft.push_back(r);
     }
+  }
+
+  // -------------------------------------------------------------------------
+  // Planar-DAT truncation (27_planar_dat.metal:536-560 plane fraction,
+  // 291-337 fallback normals, 756-906 pair reduction, 1176-1261 apply,
+  // 1289-1310 failed-surface restore; order GPUSolver.swift:3824-3871 after
+  // the predictor and 4196-4229 after every color).  Each pair places a
+  // division plane between its two sides at the anchor pose; every vertex
+  // approaching the plane keeps the fraction of its displacement that stops
+  // it short of the plane (relaxation 0.85, time margin 1e-3), the minimum
+  // over its pairs, and its displacement is capped at 0.5*0.85*rq.
+  // -------------------------------------------------------------------------
+  static constexpr double kDatGeomEps = 1.0e-8, kDatTimeMargin = 1.0e-3, kDatRelax = 0.85;
+  T planeFraction(const V<T>& x, const V<T>& dx, const V<T>& n, const V<T>& plane, T side, bool report) {
+    T s0 = side * dotv(n, x - plane);
+    T q = side * dotv(n, dx);
+    if (!std::isfinite(s0) || !std::isfinite(q)) {
+      if (report) { datInvalid++; datNonfinite++; }
+      return T(0);
+    }
+    if (s0 < -(T)kDatGeomEps) return T(0);
+    if (q >= T(0) || s0 + q > T(0)) return T(1);
+    T hit = std::max(s0, T(0)) / (-q);
+    if (hit > T(1) + (T)kDatGeomEps) return T(1);
+    return clampT(std::min((T)kDatRelax * hit, hit - (T)kDatTimeMargin), T(0), T(1));
+  }
+  void lowerT(uint32_t v, T t) {
+    if (t < datT[v]) datT[v] = t;
+  }
+  void atomicT(uint32_t v, const V<T>& x, const V<T>& dx, const V<T>& n, const V<T>& plane, T side) {
+    T t = planeFraction(x, dx, n, plane, side, true);
+    if (t < T(1)) lowerT(v, t);
+  }
+  static bool vtFallbackNormal(const V<T>& a, const V<T>& b, const V<T>& c, const V<T>& rel, V<T>& n) {
+    V<T> raw = crossv(b - a, c - a);
+    T mag = lenv(raw);
+    if (!(std::isfinite(mag) && mag >= (T)kDatGeomEps)) return false;
+    n = raw / mag;
+    if (dotv(n, rel) > T(0)) n = -n;
+    return true;
+  }
+  static bool eeFallbackNormal(const V<T>& a0, const V<T>& a1, const V<T>& b0, const V<T>& b1, const V<T>& rel,
+                               V<T>& n) {
+    V<T> ua = a1 - a0, ub = b1 - b0;
+    V<T> raw = crossv(ua, ub);
+    T mag = lenv(raw);
+    if (std::isfinite(mag) && mag >= (T)kDatGeomEps) {
+      n = raw / mag;
+    } else {
+      T la = lenv(ua), lb = lenv(ub);
+      V<T> u = la >= lb ? ua : ub;
+      T ul = std::max(la, lb);
+      if (!std::isfinite(ul) || ul < (T)kDatGeomEps) return false;
+      u = u / ul;
+      V<T> perp = rel - u * dotv(u, rel);
+      T pl = lenv(perp);
+      if (!std::isfinite(pl) || pl < (T)kDatGeomEps) {
+        V<T> axis = std::fabs(u.x) < T(0.7) ? V<T>(1, 0, 0) : V<T>(0, 1, 0);
+        perp = crossv(u, axis);
+        pl = lenv(perp);
+      }
+      if (!std::isfinite(pl) || pl < (T)kDatGeomEps) return false;
+      n = perp / pl;
+    }
+    if (dotv(n, rel) > T(0)) n = -n;
+    return finite3(n);
+  }
+  // one pair's plane fractions (dat_reduce); color < 0 is the full pass
+  void datReducePair(const std::array<uint32_t, 3>& pr, const std::vector<V<T>>& anchor, int color) {
+    auto active = [&](uint32_t b) { return color < 0 || (mass[b] > T(0) && (int)colors[b] == color); };
+    const T rq = (T)datQueryRadius, eps = (T)kDatGeomEps;
+    if (pr[0] == 0) {
+      uint32_t v = pr[1];
+      const auto& t = tris[pr[2]];
+      bool aV = active(v), aA = active(t[0]), aB = active(t[1]), aC = active(t[2]);
+      if (!aV && !aA && !aB && !aC) return;
+      V<T> xv = anchor[v], a = anchor[t[0]], b = anchor[t[1]], c = anchor[t[2]];
+      V<T> bary;
+      V<T> cp = closestTri(xv, a, b, c, bary);
+      V<T> nh = xv - cp;
+      T gap = lenv(nh);
+      if (!std::isfinite(gap)) {
+        datInvalid++;
+        datNonfinite++;
+        lowerT(v, T(0)); lowerT(t[0], T(0)); lowerT(t[1], T(0)); lowerT(t[2], T(0));
+        return;
+      }
+      if (gap > rq) return;
+      V<T> dv = pos[v] - xv, da = pos[t[0]] - a, db = pos[t[1]] - b, dc = pos[t[2]] - c;
+      V<T> n;
+      if (gap >= eps) {
+        n = nh / gap;
+      } else {
+        if (!vtFallbackNormal(a, b, c, dv - (da + db + dc) / T(3), n)) {
+          datInvalid++;
+          datVTDeg++;
+          lowerT(v, T(0)); lowerT(t[0], T(0)); lowerT(t[1], T(0)); lowerT(t[2], T(0));
+          return;
+        }
+        datVTDeg++;
+      }
+      T apV = std::max(-dotv(n, dv), T(0));
+      T apT = std::max(std::max(dotv(n, da), dotv(n, db)), std::max(dotv(n, dc), T(0)));
+      T lambda = apV + apT == T(0) ? T(0.5) : clampT(apT / (apV + apT), T(0.05), T(0.95));
+      V<T> plane = gap >= eps ? cp + nh * lambda : cp;
+      if (apV > T(0) && aV) atomicT(v, xv, dv, n, plane, T(1));
+      if (apT > T(0)) {
+        if (aA) atomicT(t[0], a, da, n, plane, T(-1));
+        if (aB) atomicT(t[1], b, db, n, plane, T(-1));
+        if (aC) atomicT(t[2], c, dc, n, plane, T(-1));
+      }
+      return;
+    }
+    auto ea = edges[pr[1]], eb = edges[pr[2]];
+    bool aA0 = active(ea.first), aA1 = active(ea.second), aB0 = active(eb.first), aB1 = active(eb.second);
+    if (!aA0 && !aA1 && !aB0 && !aB1) return;
+    V<T> a0 = anchor[ea.first], a1 = anchor[ea.second], b0 = anchor[eb.first], b1 = anchor[eb.second];
+    T s_, t_;
+    segSeg(a0, a1, b0, b1, s_, t_);
+    V<T> ca = a0 + (a1 - a0) * s_, cb = b0 + (b1 - b0) * t_;
+    V<T> nh = ca - cb;
+    T gap = lenv(nh);
+    if (!std::isfinite(gap)) {
+      datInvalid++;
+      datNonfinite++;
+      lowerT(ea.first, T(0)); lowerT(ea.second, T(0)); lowerT(eb.first, T(0)); lowerT(eb.second, T(0));
+      return;
+    }
+    if (gap > rq) return;
+    V<T> da0 = pos[ea.first] - a0, da1 = pos[ea.second] - a1, db0 = pos[eb.first] - b0, db1 = pos[eb.second] - b1;
+    V<T> n;
+    if (gap > eps) {
+      n = nh / gap;
+    } else {
+      if (!eeFallbackNormal(a0, a1, b0, b1, (da0 + da1 - db0 - db1) * T(0.5), n)) {
+        datInvalid++;
+        datEEDeg++;
+        lowerT(ea.first, T(0)); lowerT(ea.second, T(0)); lowerT(eb.first, T(0)); lowerT(eb.second, T(0));
+        return;
+      }
+      datEEDeg++;
+    }
+    T apA = std::max(std::max(-dotv(n, da0), -dotv(n, da1)), T(0));
+    T apB = std::max(std::max(dotv(n, db0), dotv(n, db1)), T(0));
+    T lambda = apA + apB == T(0) ? T(0.5) : clampT(apB / (apA + apB), T(0.05), T(0.95));
+    V<T> plane = gap > eps ? cb + nh * lambda : cb;
+    if (apA > T(0)) {
+      if (aA0) atomicT(ea.first, a0, da0, n, plane, T(1));
+      if (aA1) atomicT(ea.second, a1, da1, n, plane, T(1));
+    }
+    if (apB > T(0)) {
+      if (aB0) atomicT(eb.first, b0, db0, n, plane, T(-1));
+      if (aB1) atomicT(eb.second, b1, db1, n, plane, T(-1));
+    }
+  }
+  // apply one vertex's fraction and the displacement cap (dat_apply / dat_apply_color)
+  void datApply(uint32_t v, const std::vector<V<T>>& anchor) {
+    V<T> d = pos[v] - anchor[v];
+    bool unsafeCap = datTotal > maxDatPairs || datPeak > maxDatPairs || datOverflow > 0 || datInvalid > 0;
+    T t = datT[v];
+    datT[v] = T(1);
+    if (unsafeCap) t = T(0);
+    if (!finite3(d) || !std::isfinite(t)) {
+      datInvalid++;
+      datNonfinite++;
+      d = V<T>();
+      t = T(0);
+    }
+    bool truncated = t < T(1);
+    d = d * clampT(t, T(0), T(1));
+    T maxD = T(0.5) * (T)kDatRelax * (T)datQueryRadius;
+    T d2 = dotv(d, d);
+    if (d2 > maxD * maxD) {
+      d = d * (maxD / std::sqrt(d2));
+      truncated = true;
+    }
+    if (truncated) {
+      pos[v] = anchor[v] + d;
+      datTrunc++;
+    }
+  }
+  // the full pass after the predictor: every pair, every surface particle
+  void datPredictorPass() {
+    for (const auto& pr : datPairs) datReducePair(pr, initLin, -1);
+    for (uint32_t v : particles) {
+      datApply(v, initLin);
+      snapLin[v] = pos[v];  // the first iteration starts from the truncated pose
+    }
+  }
+  // the pass after color c's primal: the color's surface particles
+  void datColorPass(int c) {
+    for (const auto& pr : datPairs) datReducePair(pr, datAnchor, c);
+    for (uint32_t v : buckets[c])
+      if (clothGroup[v] != 0u) datApply(v, datAnchor);
   }
 
   // OGC contact row (40_solver.metal:1237-1300); `snap` reads same-color partners from the snapshot
@@ -2203,6 +2408,7 @@ struct Oracle : oracle_base {
       const long n = (long)bk.size();
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1 && n > 256)
       for (long i = 0; i < n; i++) primal(bk[(size_t)i]);
+      if (nTris && n > 0) datColorPass(c);
     }
     dual();
   }
@@ -2222,12 +2428,22 @@ struct Oracle : oracle_base {
     soft.clear();
     softCandidates = softRT = 0;
     datOverflow = datNonfinite = 0;
-    if (nTris) rtEmit();
+    datPeak = datPeakVT = datInvalid = datTrunc = datVTDeg = datEEDeg = 0;
+    // cloth: the pair query at the step-start pose, rigid-feature contacts,
+    // the predictor, its full truncation pass, then the query at the accepted
+    // predictor pose feeds the contacts and the per-color passes, anchored
+    // there (GPUSolver.swift:3685-3870)
+    if (nTris) {
+      datBuild();
+      rtEmit();
+    }
     warmstartConstraints();
     warmstartBodies();
     if (nTris) {
+      datPredictorPass();
       datBuild();
       datContacts();
+      for (uint32_t v : particles) datAnchor[v] = pos[v];
     }
     softAdj.assign(nb, {});
     for (uint32_t i = 0; i < soft.size(); i++)
@@ -2256,6 +2472,11 @@ struct Oracle : oracle_base {
     } else {
       for (int it = 0; it < set.iterations; it++) sweep();
     }
+    // a failed surface query restores the step-start surface
+    // (dat_restore_failed_surface, 27_planar_dat.metal:1289-1310)
+    if (nTris && (softCandidates > maxSoft || datPeak > maxDatPairs || datOverflow > 0 || datInvalid > 0))
+      for (uint32_t i = 0; i < nb; i++)
+        if (radiusSigned[i] < 0) pos[i] = initLin[i];
     // BDF1 finalize (40_solver.metal:2108-2150)
     T dt = set.dt;
     T maxSpeed = std::max(T(30), T(1.5) * (T)cellSize / dt * T(0.5));
@@ -2384,6 +2605,15 @@ int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, in
     s->stepToResidual(tol, max_iterations, check_every);
     if (iterations) *iterations = s->itersUsed;
     if (residual) *residual = s->lastResidual;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_set_dat_test_hooks(oracle_solver* o, float queryRadius, int32_t pairCapacity, int32_t softCapacity) {
+  auto go = [&](auto* s) {
+    if (queryRadius > 0) s->datQueryRadius = queryRadius;
+    if (pairCapacity > 0) s->maxDatPairs = std::min<uint32_t>(s->maxDatPairs, (uint32_t)pairCapacity);
+    if (softCapacity > 0) s->maxSoft = std::min<uint32_t>(s->maxSoft, (uint32_t)softCapacity);
   };
   if (o->f) go(o->f); else go(o->d);
   return 0;
@@ -2555,12 +2785,24 @@ int oracle_dat_pairs_at(oracle_solver* o, const float* pos3, uint32_t cap, uint3
 int oracle_cloth_stats(oracle_solver* o, int32_t* datPairs, int32_t* datVT, int32_t* softCandidates, int32_t* softRT,
                        int32_t* overflow, int32_t* nonfinite) {
   auto go = [&](auto* s) {
-    if (datPairs) *datPairs = (int32_t)s->datTotal;
-    if (datVT) *datVT = (int32_t)s->datVT;
+    if (datPairs) *datPairs = (int32_t)s->datPeak;
+    if (datVT) *datVT = (int32_t)s->datPeakVT;
     if (softCandidates) *softCandidates = (int32_t)s->softCandidates;
     if (softRT) *softRT = (int32_t)s->softRT;
     if (overflow) *overflow = (int32_t)s->datOverflow;
     if (nonfinite) *nonfinite = (int32_t)s->datNonfinite;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_dat_stats(oracle_solver* o, int32_t* out) {
+  auto go = [&](auto* s) {
+    out[0] = (int32_t)s->datTrunc;
+    out[1] = (int32_t)s->datInvalid;
+    out[2] = (int32_t)s->datVTDeg;
+    out[3] = (int32_t)s->datEEDeg;
+    out[4] = (int32_t)s->datNonfinite;
+    out[5] = (int32_t)s->datOverflow;
   };
   if (o->f) go(o->f); else go(o->d);
   return 0;

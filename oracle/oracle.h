@@ -51,6 +51,12 @@ int oracle_get_soft_contacts(oracle_solver* o, uint32_t capacity, uint32_t* n, v
 int oracle_dat_pairs_at(oracle_solver* o, const float* pos3, uint32_t capacity, uint32_t* n, uint32_t* pairs3);
 int oracle_cloth_stats(oracle_solver* o, int32_t* dat_pairs, int32_t* dat_vt, int32_t* soft_candidates,
                        int32_t* soft_rt, int32_t* overflow, int32_t* nonfinite);
+/* Planar-DAT truncation counters of the last frame: truncations, invalid
+ * anchors, V-T and E-E degeneracies, non-finite values, grid overflows. */
+int oracle_dat_stats(oracle_solver* o, int32_t* out6);
+/* Test hooks of the Planar-DAT fixtures: query radius override (> 0), and
+ * pair / soft-contact capacity caps (> 0), as the backend's. */
+int oracle_set_dat_test_hooks(oracle_solver* o, float query_radius, int32_t pair_capacity, int32_t soft_capacity);
 
 /* unit restatements for pinning */
 void oracle_splitmix64(uint64_t seed, uint32_t n, uint64_t* out_u64, float* out_float);

@@ -32,9 +32,39 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_capi.Settings) == 15 * 4
     assert C.sizeof(_capi.Options) == 7 * 4
     assert C.sizeof(_capi.ClothInfo) == 9 * 4 + 4 * 4
-    # int64 frame + 7 counters + 64 color counts + 4 + 4 cloth counters, 8-byte aligned
-    assert C.sizeof(_capi.Stats) == 8 + 4 * (7 + 64 + 4 + 4 + 1 + 2)
+    # int64 frame + 7 counters + 64 color counts + 4 + 4 cloth counters + 3
+    # learned-start fields + 5 truncation counters, tail-padded to 8 bytes
+    assert C.sizeof(_capi.Stats) == 8 + 4 * (7 + 64 + 4 + 4 + 1 + 2 + 5) + 4
     assert C.sizeof(_capi.Failure) == 4 + 4 + 8 * 3 + 4 + 160 + 4
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """The ctypes mirror against what a C compiler makes of include/avbd.h:
+    size of every struct and the offset of every field."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"avbd_settings": _capi.Settings, "avbd_options": _capi.Options, "avbd_stats": _capi.Stats,
+               "avbd_failure": _capi.Failure, "avbd_cloth_info": _capi.ClothInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "avbd.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cc, "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
 
 
 def test_defaults_mirror_simsettings():
