@@ -144,6 +144,9 @@ SIGNATURES = {
     "avbd_test_deny_stage": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "avbd_test_fail_completion": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "avbd_test_set_dat_query_radius": (C.c_int, [C.c_void_p, C.c_float]),
+    "avbd_capture_speculation_snapshot": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "avbd_restore_speculation_snapshot": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "avbd_destroy_speculation_snapshot": (None, [C.c_void_p]),
     "avbd_mlp_create": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "avbd_mlp_info": (C.c_int, [C.c_void_p, I32P, I32P]),
     "avbd_mlp_destroy": (None, [C.c_void_p]),

@@ -371,12 +371,41 @@ class GPUSolver:
         codes = {v: k for k, v in _capi.FAILURE_NAMES.items()}
         self._check(self._lib.avbd_test_fail_completion(self._h, codes[kind], int(frame)), "failCompletionForTesting")
 
+    # -- speculation snapshots (GPUSolver.swift:2948-3057) ---------------------
+    def captureRigidSpeculationSnapshot(self, into=None):
+        """Device-resident snapshot of the rigid scene's temporal state;
+        `into` (a snapshot of this solver) is overwritten in place."""
+        h = C.c_void_p(into._h.value if into is not None else None)
+        self._check(self._lib.avbd_capture_speculation_snapshot(self._h, C.byref(h)),
+                    "captureRigidSpeculationSnapshot")
+        if into is not None:
+            return into
+        return RigidSpeculationSnapshot(self, h)
+
+    def restoreRigidSpeculationSnapshot(self, snapshot):
+        self._check(self._lib.avbd_restore_speculation_snapshot(self._h, snapshot._h),
+                    "restoreRigidSpeculationSnapshot")
+
     def setDATQueryRadiusForTesting(self, r):
         """params.planarDATQueryRadius = r (PlanarDATTests.swift:930); r <= 0 restores."""
         self._check(self._lib.avbd_test_set_dat_query_radius(self._h, float(r)), "setDATQueryRadiusForTesting")
 
     def denyStageForTesting(self, stage, frame=-1):
         self._lib.avbd_test_deny_stage(self._h, stage.encode() if stage else None, int(frame))
+
+
+class RigidSpeculationSnapshot:
+    """Opaque snapshot handle; keeps its solver alive while it exists."""
+
+    def __init__(self, solver, handle):
+        self._solver = solver
+        self._h = handle
+        self._lib = solver._lib
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.avbd_destroy_speculation_snapshot(self._h)
+            self._h = None
 
 
 def version():

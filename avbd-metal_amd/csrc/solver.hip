@@ -2586,6 +2586,126 @@ extern "C" avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, i
   return AVBD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// speculation snapshots (GPUSolver.swift:2948-3057): the temporal state of a
+// rigid scene -- body poses / velocities / inertial targets, joint and
+// spring multipliers, both manifold buffers, the pair map, the persistent
+// colors, the counters -- copied device to device into one HBM block, so a
+// rollout can fork and rewind without crossing PCIe
+// ---------------------------------------------------------------------------
+struct avbd_snapshot {
+  const avbd_solver* owner = nullptr;
+  void* block = nullptr;
+  size_t bytes = 0;
+  int64_t frameIndex = 0;
+  avbd_stats stats{};
+};
+
+namespace {
+
+// the snapshot's buffers, in a fixed order, with the manifold / soft roles
+// of the current frame
+std::vector<std::pair<void*, size_t>> speculation_buffers(const avbd_solver* s) {
+  const DeviceState& S = s->S;
+  const size_t nb = s->nb;
+  std::vector<std::pair<void*, size_t>> b = {
+      {S.posLin, 16 * nb}, {S.posAng, 16 * nb}, {S.initLin, 16 * nb}, {S.initAng, 16 * nb},
+      {S.inertLin, 16 * nb}, {S.inertAng, 16 * nb}, {S.velLin, 16 * nb}, {S.velAng, 16 * nb},
+      {S.prevVelLin, 16 * nb}, {S.posLinSnap, 16 * nb}, {S.posAngSnap, 16 * nb},
+      {S.joints, sizeof(JointRec) * std::max<size_t>(1, s->nj)},
+      {S.springs, sizeof(SpringRec) * std::max<size_t>(1, s->ns)},
+      {S.manifolds, sizeof(ManifoldRec) * (size_t)s->maxPairs},
+      {S.prevManifolds, sizeof(ManifoldRec) * (size_t)s->maxPairs},
+      {S.mapKeyA, 4 * (size_t)s->mapCapacity}, {S.mapKeyB, 4 * (size_t)s->mapCapacity},
+      {S.mapVal, 4 * (size_t)s->mapCapacity}, {S.colors, 4 * nb}, {S.colorsB, 4 * nb},
+      {S.counters, 4 * (size_t)CTR_TOTAL}};
+  if (s->W.corr) {  // the learned warm start's correction history
+    b.push_back({s->W.corr, 16 * nb});
+    b.push_back({s->W.corrPrev, 16 * nb});
+    b.push_back({s->W.corrAng, 16 * nb});
+  }
+  return b;
+}
+
+size_t spec_align(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_snapshot** out) {
+  if (!s || !out) return AVBD_ERR_INVALID_ARGUMENT;
+  if (s->nTris || s->nt) return AVBD_ERR_UNSUPPORTED;  // rigid scenes only, as the reference
+  if (*out && (*out)->owner != s) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  const auto bufs = speculation_buffers(s);
+  size_t bytes = 0;
+  for (const auto& b : bufs) bytes += spec_align(b.second);
+  avbd_snapshot* snap = *out;
+  bool fresh = false;
+  if (!snap) {
+    snap = new avbd_snapshot();
+    snap->owner = s;
+    fresh = true;
+  }
+  if (snap->bytes != bytes) {  // grows when a learned warm start was attached since
+    if (snap->block) hipFree(snap->block);
+    snap->block = nullptr;
+    snap->bytes = 0;
+    if (hipMalloc(&snap->block, bytes) != hipSuccess) {
+      hipGetLastError();
+      if (fresh) delete snap;
+      return AVBD_ERR_ALLOC;
+    }
+    snap->bytes = bytes;
+  }
+  size_t off = 0;
+  for (const auto& b : bufs) {
+    if (hipMemcpyAsync((char*)snap->block + off, b.first, b.second, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+      return AVBD_ERR_HIP;
+    off += spec_align(b.second);
+  }
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return AVBD_ERR_HIP;
+  snap->frameIndex = s->frameIndex;
+  snap->stats = s->stats;
+  *out = snap;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_restore_speculation_snapshot(avbd_solver* s, const avbd_snapshot* snap) {
+  if (!s || !snap || snap->owner != s) return AVBD_ERR_INVALID_ARGUMENT;
+  if (s->nTris || s->nt) return AVBD_ERR_UNSUPPORTED;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  // manifold and soft buffers alternate roles every frame (the captured
+  // graphs bake the roles by frame parity): put them back in the roles of
+  // the snapshot's frame before copying
+  if (((s->frameIndex - snap->frameIndex) & 1) != 0) {
+    std::swap(s->S.manifolds, s->S.prevManifolds);
+    std::swap(s->S.soft, s->S.prevSoft);
+  }
+  const auto bufs = speculation_buffers(s);
+  size_t bytes = 0;
+  for (const auto& b : bufs) bytes += spec_align(b.second);
+  if (bytes != snap->bytes) return AVBD_ERR_INVALID_ARGUMENT;  // a warm start attached or detached since
+  size_t off = 0;
+  for (const auto& b : bufs) {
+    if (hipMemcpyAsync(b.first, (const char*)snap->block + off, b.second, hipMemcpyDeviceToDevice, s->stream) !=
+        hipSuccess)
+      return AVBD_ERR_HIP;
+    off += spec_align(b.second);
+  }
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return AVBD_ERR_HIP;
+  s->frameIndex = snap->frameIndex;
+  s->stats = snap->stats;
+  return AVBD_OK;
+}
+
+extern "C" void avbd_destroy_speculation_snapshot(avbd_snapshot* snap) {
+  if (!snap) return;
+  if (snap->block) hipFree(snap->block);
+  delete snap;
+}
+
 extern "C" avbd_status avbd_test_set_dat_query_radius(avbd_solver* s, float r) {
   if (!s || !(r == r)) return AVBD_ERR_INVALID_ARGUMENT;
   s->datQueryRadius = r > 0.0f ? r : s->datQueryRadiusDerived;

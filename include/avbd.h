@@ -386,6 +386,27 @@ avbd_status avbd_test_fail_completion(avbd_solver* s, int32_t kind, int64_t fram
  * grid keeps the cell size derived at create.  r <= 0 restores the derived rq. */
 avbd_status avbd_test_set_dat_query_radius(avbd_solver* s, float r);
 
+/* ---- Speculation snapshots ------------------------------------------------
+ * RigidSpeculationSnapshot (GPUSolver.swift:2948-2982): the temporal solver
+ * state of a rigid scene -- poses, velocities and the adaptive velocity
+ * history, inertial targets, joint / spring multipliers and penalties, both
+ * contact-manifold buffers and the persistence map, the incremental colors,
+ * the frame counters, frame index and stats (plus the learned warm start's
+ * correction history when one is attached).  Authored scene data and
+ * settings are not included.  The snapshot lives in device memory and is
+ * bound to the solver that captured it. */
+typedef struct avbd_snapshot avbd_snapshot;
+/* captureRigidSpeculationSnapshot() -- GPUSolver.swift:2984-3012.  Waits for
+ * the frames in flight.  *out == NULL allocates a snapshot; a snapshot of
+ * this solver passed in *out is overwritten in place (no allocation).  A
+ * scene with cloth triangles or tets returns AVBD_ERR_UNSUPPORTED (the
+ * reference's precondition). */
+avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_snapshot** out);
+/* restoreRigidSpeculationSnapshot(_:) -- GPUSolver.swift:3014-3057.  Later
+ * frames replay the captured rollout bit for bit given the same inputs. */
+avbd_status avbd_restore_speculation_snapshot(avbd_solver* s, const avbd_snapshot* snap);
+void avbd_destroy_speculation_snapshot(avbd_snapshot* snap);
+
 /* ---- Learned warm start (no counterpart in the reference) ----------------
  * A per-body MLP 24 -> 128 -> 128 -> 3 (particles) or 6 (rigid bodies:
  * linear, angular), ReLU, BF16 weights and activations, FP32 accumulation on
