@@ -113,6 +113,11 @@ class PhysicsScene:
         return self.addBody((diameter, diameter, diameter), density, friction, position, velocity=velocity,
                             shape=SPHERE)
 
+    def addCapsule(self, length, radius, density, friction, position, rotation=(0, 0, 0, 1), velocity=(0, 0, 0)):
+        """Scene.swift:686-693: size = (cylinder length along local z, radius, 0)."""
+        return self.addBody((length, radius, 0.0), density, friction, position, rotation=rotation,
+                            velocity=velocity, shape=CAPSULE)
+
     def addCollider(self, body, size, friction=None, localPosition=(0, 0, 0), localRotation=(0, 0, 0, 1),
                     shape=BOX, collisionGroup=0, collidesWithSharedGeometry=True, collisionEnabled=True,
                     dynamicFriction=None):

@@ -572,6 +572,12 @@ __device__ void np_support_edge(const NPBox& b, int k, V3 dir, V3& e0, V3& e1) {
   e1 = c + b.ax[k] * np_half(b, k);
 }
 
+__device__ __forceinline__ V3 np_closest_on_segment(V3 p, V3 a, V3 b) {
+  V3 ab = b - a;
+  float t = clampf(dot(p - a, ab) / fmaxf(dot(ab, ab), 1e-12f), 0.0f, 1.0f);
+  return a + ab * t;
+}
+
 __device__ void np_closest_seg_seg(V3 p0, V3 p1, V3 q0, V3 q1, V3& c0, V3& c1) {
   V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
   float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
@@ -649,10 +655,108 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
     NPContact cs[kMaxContacts];
     int count = 0;
     V3 nrm;  // from B toward A
+    const bool capA = (tA & kShapeKindMask) == 3, capB = (tB & kShapeKindMask) == 3;
 
-    if (sphA || sphB) {
+    if (capA || capB || sphA || sphB) {
       float rA = shA.x * 0.5f, rB = shB.x * 0.5f;
-      if (sphA && sphB) {
+      float bestDRound;  // proximity bound of the warm-start match
+      if (capA || capB) {
+        // capsule pairs (30_narrowphase.metal:510-745): vs sphere and vs
+        // capsule one closest-point contact, vs box an alternating
+        // clamp <-> segment projection from three seeds
+        const bool cIsA = capA;
+        const F4 shC = cIsA ? shA : shB, shO = cIsA ? shB : shA;
+        const F4 qC = cIsA ? qA : qB, qO = cIsA ? qB : qA;
+        const V3 pC = cIsA ? A.c : B.c, pO = cIsA ? B.c : A.c;
+        const float rc = shC.y;
+        const V3 axC = q_rotate(qC, v3(0, 0, 1));
+        const V3 c0 = pC - axC * (shC.x * 0.5f), c1 = pC + axC * (shC.x * 0.5f);
+        const uint32_t otherType = (cIsA ? tB : tA) & kShapeKindMask;
+        const V3 relCO = cIsA ? (vB - vA) : (vA - vB);
+        V3 nCO = v3(0, 0, 1);  // capsule -> other
+        NPContact hits[3];     // xA on the capsule, xB on the other shape
+        int nh = 0;
+        if (otherType == 1) {
+          const float rs = shO.x * 0.5f;
+          V3 q = np_closest_on_segment(pO, c0, c1);
+          V3 d = pO - q;
+          float dist = length(d);
+          V3 nT = dist > 1e-6f ? d / dist : v3(0, 0, 1);
+          if (dist <= rc + rs + np_detect_margin(nT, relCO, P, shC.w, shO.w)) {
+            nCO = nT;
+            hits[0].xA = q + nCO * rc;
+            hits[0].xB = pO - nCO * rs;
+            hits[0].feature = 0u;
+            nh = 1;
+          }
+        } else if (otherType == 3) {
+          const float ro = shO.y;
+          const V3 axO = q_rotate(qO, v3(0, 0, 1));
+          const V3 o0 = pO - axO * (shO.x * 0.5f), o1 = pO + axO * (shO.x * 0.5f);
+          V3 qa, qb;
+          np_closest_seg_seg(c0, c1, o0, o1, qa, qb);
+          V3 d = qb - qa;
+          float dist = length(d);
+          V3 nT = dist > 1e-6f ? d / dist : v3(0, 0, 1);
+          if (dist <= rc + ro + np_detect_margin(nT, relCO, P, shC.w, shO.w)) {
+            nCO = nT;
+            hits[0].xA = qa + nCO * rc;
+            hits[0].xB = qb - nCO * ro;
+            hits[0].feature = 0u;
+            nh = 1;
+          }
+        } else {
+          const F4 qOc = q_conj(qO);
+          const V3 halfO = xyz(shO) * 0.5f;
+          V3 nAccum = v3s(0);
+          for (int seed = 0; seed < 3 && nh < 3; seed++) {
+            V3 q = c0 + (c1 - c0) * ((float)seed * 0.5f);
+            V3 bw = q;
+            for (int k = 0; k < 5; k++) {
+              V3 lb = clamp3(q_rotate(qOc, q - pO), -halfO, halfO);
+              bw = q_rotate(qO, lb) + pO;
+              q = np_closest_on_segment(bw, c0, c1);
+            }
+            V3 d = bw - q;
+            float dist = length(d);
+            V3 n;
+            if (dist < 1e-6f) {
+              // segment inside the box: push out along the least-penetrated face
+              V3 lb = q_rotate(qOc, q - pO);
+              V3 pen = halfO - vabs(lb);
+              int axis = 0;
+              if (pen.y < comp(pen, axis)) axis = 1;
+              if (pen.z < comp(pen, axis)) axis = 2;
+              V3 nl = v3s(0);
+              setcomp(nl, axis, comp(lb, axis) >= 0.0f ? 1.0f : -1.0f);
+              n = -q_rotate(qO, nl);
+              bw = q - n * comp(pen, axis);
+            } else {
+              n = d / dist;
+              if (dist - rc > np_detect_margin(n, relCO, P, shC.w, shO.w)) continue;
+            }
+            bool dup = false;
+            for (int h = 0; h < nh; h++)
+              if (length(hits[h].xA - (q + n * rc)) < 0.4f * rc + 0.05f) { dup = true; break; }
+            if (dup) continue;
+            hits[nh].xA = q + n * rc;
+            hits[nh].xB = bw;
+            hits[nh].feature = (uint32_t)seed;
+            nAccum += n;
+            nh++;
+          }
+          if (nh > 0) nCO = normalize(nAccum);
+        }
+        if (nh == 0) { outM.header = U4{ba, bb, 0, 0}; continue; }
+        nrm = cIsA ? -nCO : nCO;
+        for (int h = 0; h < nh; h++) {
+          cs[h].xA = cIsA ? hits[h].xA : hits[h].xB;
+          cs[h].xB = cIsA ? hits[h].xB : hits[h].xA;
+          cs[h].feature = hits[h].feature;
+        }
+        count = nh;
+        bestDRound = 0.6f * (rc + 0.2f);
+      } else if (sphA && sphB) {
         V3 d = A.c - B.c;
         float dist = length(d);
         V3 nT = dist > 1e-9f ? d / dist : v3(0, 0, 1);
@@ -703,6 +807,7 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
         cs[0].feature = 0;
         count = 1;
       }
+      if (!(capA || capB)) bestDRound = 0.5f * fmaxf(rA, rB);
       // round-shape tail: anchors stored as world offsets when flagged
       V3 t1, t2;
       orthonormal(nrm, t1, t2);
@@ -721,7 +826,7 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
         if (prevIdx >= 0) {
           const ManifoldRec& pm = S.prevManifolds[prevIdx];
           uint32_t pn = pm.header.z;
-          float bestD = 0.5f * fmaxf(rA, rB);
+          float bestD = bestDRound;
           int bestJ = -1;
           for (uint32_t j = 0; j < pn; j++) {
             float dj = length(xyz(pm.c[j].rA) - rA_);
@@ -735,8 +840,12 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
             V3 lt = t1o * lambda.y + t2o * lambda.z;
             lambda.y = dot(t1, lt);
             lambda.z = dot(t2, lt);
+            // body-local anchors inside the Coulomb cone are kept; a
+            // world-offset (rolling) anchor of a rigid round shape is
+            // rebuilt -- capsules keep anchors only when neither is one
             bool partA = shA.w < 0.0f, partB = shB.w < 0.0f;
-            if ((partA || !roundA) && (partB || !roundB)) {
+            bool keep = (capA || capB) ? (!roundA && !roundB) : ((partA || !roundA) && (partB || !roundB));
+            if (keep) {
               stick = pm.c[bestJ].rB.w;
               if (stick != 0.0f) {
                 rA_ = xyz(pm.c[bestJ].rA);

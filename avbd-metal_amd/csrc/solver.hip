@@ -512,7 +512,8 @@ avbd_status validate(const avbd_scene_desc* d) {
     if (d->collider_body[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
     uint32_t sh = d->collider_shape[i];
     bool enabled = !d->collider_enabled || d->collider_enabled[i];
-    if (enabled && sh != AVBD_SHAPE_BOX && sh != AVBD_SHAPE_SPHERE) return AVBD_ERR_UNSUPPORTED;
+    if (enabled && sh != AVBD_SHAPE_BOX && sh != AVBD_SHAPE_SPHERE && sh != AVBD_SHAPE_CAPSULE)
+      return AVBD_ERR_UNSUPPORTED;
   }
   for (uint32_t i = 0; i < d->num_joints; i++) {
     if (d->joint_body_b[i] < 0 || (uint32_t)d->joint_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;

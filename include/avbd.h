@@ -39,7 +39,7 @@ typedef enum avbd_status {
     AVBD_ERR_INVALID_ARGUMENT = 1,  /* precondition violated (Scene.swift preconditions) */
     AVBD_ERR_NO_DEVICE = 2,         /* AVBDError.noDevice          GPUSolver.swift:641 */
     AVBD_ERR_ALLOC = 3,             /* AVBDError.allocFailed       GPUSolver.swift:642 */
-    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus, capsule, hull) */
+    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus, convex hull) */
     AVBD_ERR_RUNTIME_FAILURE = 5,   /* a RuntimeFailure is latched; see avbd_last_failure */
     AVBD_ERR_HIP = 6                /* HIP API error outside a latched frame */
 } avbd_status;

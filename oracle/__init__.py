@@ -64,6 +64,7 @@ def load():
         "oracle_cloth_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p]),
         "oracle_dat_stats": (C.c_int, [vp, i32p]),
         "oracle_set_dat_test_hooks": (C.c_int, [vp, C.c_float, C.c_int32, C.c_int32]),
+        "oracle_set_joint_world_anchors": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_float)]),
         "oracle_dat_pairs_at": (C.c_int, [vp, fp, C.c_uint32, u32p, u32p]),
         "oracle_membrane": (None, [dp, dp, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_hinge": (C.c_int, [fp, C.c_float, C.c_float, fp, fp]),
@@ -225,6 +226,11 @@ class OracleSolver:
         self._lib.oracle_cloth_stats(self._h, *[C.byref(x) for x in v])
         return dict(zip(["dat_pairs", "dat_vt_pairs", "soft_candidates", "num_soft_rt", "overflow", "nonfinite"],
                         [x.value for x in v]))
+
+    def setJointWorldAnchors(self, joints, points):
+        j = np.ascontiguousarray(joints, np.uint32)
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        self._lib.oracle_set_joint_world_anchors(self._h, _p(j, C.c_uint32), len(j), _p(p, C.c_float))
 
     def setDatTestHooks(self, queryRadius=0.0, pairCapacity=0, softCapacity=0):
         """Planar-DAT query radius override and pair / soft capacity caps (0 keeps)."""

@@ -336,6 +336,14 @@ struct Oracle : oracle_base {
         m = den > 0 ? 4.0f / 3.0f * pi * rr * rr * rr * den : 0.0f;
         ix = iy = iz = 0.4f * m * rr * rr;
         r = rr;
+      } else if (sh == AVBD_SHAPE_CAPSULE) {
+        // GPUSolver.swift:940-945: cylinder of length L plus two hemispheres
+        float L = sx, rr = sy;
+        m = den > 0 ? pi * rr * rr * (L + 4.0f * rr / 3.0f) * den : 0.0f;
+        float iAxis = 0.5f * m * rr * rr, iPerp = m * (L * L / 12.0f + rr * rr / 4.0f);
+        ix = iy = iPerp;
+        iz = iAxis;
+        r = L / 2.0f + rr;
       } else {
         return 4;
       }
@@ -369,6 +377,7 @@ struct Oracle : oracle_base {
       if (sh == AVBD_SHAPE_SPHERE) { sy = sx; sz = sx; }
       float r;
       if (sh == AVBD_SHAPE_SPHERE) r = sx / 2.0f;
+      else if (sh == AVBD_SHAPE_CAPSULE) r = sx / 2.0f + sy;
       else { float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f; r = std::sqrt(hx * hx + hy * hy + hz * hz); }
       rad32[i] = r;
       bool en = !d->collider_enabled || d->collider_enabled[i];
@@ -975,9 +984,95 @@ struct Oracle : oracle_base {
     auto warmPen = [&](V<T> p) {
       return V<T>(clampT(p.x * gam, fl.x, T(1e10)), clampT(p.y * gam, fl.y, T(1e6)), clampT(p.z * gam, fl.z, T(1e6)));
     };
-    if (sphA || sphB) {
+    bool capA = (cFlags[ia] & 0xF) == 3, capB = (cFlags[ib] & 0xF) == 3;
+    if (capA || capB || sphA || sphB) {
       T rA = cSize[ia].x * T(0.5), rB = cSize[ib].x * T(0.5);
-      if (sphA && sphB) {
+      T bestDRound;
+      if (capA || capB) {
+        // capsule pairs (30_narrowphase.metal:510-745)
+        bool cIsA = capA;
+        uint32_t ic = cIsA ? ia : ib, io = cIsA ? ib : ia;
+        Q<T> qC = cIsA ? qA : qB, qO = cIsA ? qB : qA;
+        V<T> pC = cIsA ? A.c : B.c, pO = cIsA ? B.c : A.c;
+        T rc = cSize[ic].y;
+        V<T> axC = qrot(qC, V<T>(0, 0, 1));
+        V<T> c0 = pC - axC * (cSize[ic].x * T(0.5)), c1 = pC + axC * (cSize[ic].x * T(0.5));
+        uint32_t other = cFlags[io] & 0xF;
+        V<T> relCO = cIsA ? (vB - vA) : (vA - vB);
+        V<T> nCO(0, 0, 1);
+        std::vector<Pt> hits;  // xA on the capsule, xB on the other shape
+        auto onSeg = [](const V<T>& p, const V<T>& a, const V<T>& b) {
+          V<T> ab = b - a;
+          T t = clampT(dotv(p - a, ab) / std::max(dotv(ab, ab), T(1e-12)), T(0), T(1));
+          return a + ab * t;
+        };
+        if (other == 1) {
+          T rs = cSize[io].x * T(0.5);
+          V<T> q = onSeg(pO, c0, c1);
+          V<T> d = pO - q;
+          T dist = lenv(d);
+          V<T> nT = dist > T(1e-6) ? d / dist : V<T>(0, 0, 1);
+          if (dist <= rc + rs + detectMargin(nT, relCO, cRad[ic], cRad[io])) {
+            nCO = nT;
+            hits.push_back(Pt{q + nCO * rc, pO - nCO * rs, 0});
+          }
+        } else if (other == 3) {
+          T ro = cSize[io].y;
+          V<T> axO = qrot(qO, V<T>(0, 0, 1));
+          V<T> o0 = pO - axO * (cSize[io].x * T(0.5)), o1 = pO + axO * (cSize[io].x * T(0.5));
+          V<T> qa, qb;
+          segSeg(c0, c1, o0, o1, qa, qb);
+          V<T> d = qb - qa;
+          T dist = lenv(d);
+          V<T> nT = dist > T(1e-6) ? d / dist : V<T>(0, 0, 1);
+          if (dist <= rc + ro + detectMargin(nT, relCO, cRad[ic], cRad[io])) {
+            nCO = nT;
+            hits.push_back(Pt{qa + nCO * rc, qb - nCO * ro, 0});
+          }
+        } else {
+          Q<T> qOc = qconj(qO);
+          V<T> h = cSize[io] * T(0.5);
+          V<T> nAcc;
+          for (int seed = 0; seed < 3 && hits.size() < 3; seed++) {
+            V<T> q = c0 + (c1 - c0) * (T(seed) * T(0.5));
+            V<T> bw = q;
+            for (int k = 0; k < 5; k++) {
+              V<T> l = qrot(qOc, q - pO);
+              V<T> lb(clampT(l.x, -h.x, h.x), clampT(l.y, -h.y, h.y), clampT(l.z, -h.z, h.z));
+              bw = qrot(qO, lb) + pO;
+              q = onSeg(bw, c0, c1);
+            }
+            V<T> d = bw - q;
+            T dist = lenv(d);
+            V<T> n;
+            if (dist < T(1e-6)) {
+              V<T> lb = qrot(qOc, q - pO);
+              V<T> pen = h - V<T>(std::fabs(lb.x), std::fabs(lb.y), std::fabs(lb.z));
+              int ax = 0;
+              if (pen.y < pen[ax]) ax = 1;
+              if (pen.z < pen[ax]) ax = 2;
+              V<T> nl;
+              nl[ax] = lb[ax] >= 0 ? T(1) : T(-1);
+              n = -qrot(qO, nl);
+              bw = q - n * pen[ax];
+            } else {
+              n = d / dist;
+              if (dist - rc > detectMargin(n, relCO, cRad[ic], cRad[io])) continue;
+            }
+            bool dup = false;
+            for (const Pt& hp : hits)
+              if (lenv(hp.xA - (q + n * rc)) < T(0.4) * rc + T(0.05)) { dup = true; break; }
+            if (dup) continue;
+            hits.push_back(Pt{q + n * rc, bw, (uint32_t)seed});
+            nAcc = nAcc + n;
+          }
+          if (!hits.empty()) nCO = normv(nAcc);
+        }
+        if (hits.empty()) return;
+        nrm = cIsA ? -nCO : nCO;
+        for (const Pt& hp : hits) cs.push_back(Pt{cIsA ? hp.xA : hp.xB, cIsA ? hp.xB : hp.xA, hp.f});
+        bestDRound = T(0.6) * (rc + T(0.2));
+      } else if (sphA && sphB) {
         V<T> d = A.c - B.c;
         T dist = lenv(d);
         V<T> nT = dist > T(1e-9) ? d / dist : V<T>(0, 0, 1);
@@ -1016,6 +1111,7 @@ struct Oracle : oracle_base {
         nrm = sIsA ? nW : -nW;
         cs.push_back(Pt{sIsA ? xS : qW, sIsA ? qW : xS, 0});
       }
+      if (!(capA || capB)) bestDRound = T(0.5) * std::max(rA, rB);
       V<T> t1, t2;
       basis(nrm, t1, t2);
       int prev = prevIndex();
@@ -1030,7 +1126,7 @@ struct Oracle : oracle_base {
         T stick = 0;
         if (prev >= 0) {
           const Manifold<T>& pm = prevManifolds[prev];
-          T bestD = T(0.5) * std::max(rA, rB);
+          T bestD = bestDRound;
           int bj = -1;
           for (uint32_t j = 0; j < pm.n; j++) {
             T dj = lenv(pm.c[j].rA - rA_);
@@ -1044,7 +1140,8 @@ struct Oracle : oracle_base {
             lam.y = dotv(t1, lt);
             lam.z = dotv(t2, lt);
             bool pA = cRad[ia] < 0, pB = cRad[ib] < 0;
-            if ((pA || !roundA) && (pB || !roundB)) {
+            bool keep = (capA || capB) ? (!roundA && !roundB) : ((pA || !roundA) && (pB || !roundB));
+            if (keep) {
               stick = pm.c[bj].stick;
               if (stick != 0) { rA_ = pm.c[bj].rA; rB_ = pm.c[bj].rB; }
             }
@@ -2605,6 +2702,18 @@ int oracle_step_to_residual(oracle_solver* o, double tol, int max_iterations, in
     s->stepToResidual(tol, max_iterations, check_every);
     if (iterations) *iterations = s->itersUsed;
     if (residual) *residual = s->lastResidual;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+int oracle_set_joint_world_anchors(oracle_solver* o, const uint32_t* joints, uint32_t n, const float* pts) {
+  // setJointWorldAnchors (GPUSolver.swift:2481-2489): the world point of a
+  // world-anchored joint
+  auto go = [&](auto* s) {
+    for (uint32_t k = 0; k < n; k++) {
+      if (joints[k] >= s->joints.size()) continue;
+      s->joints[joints[k]].rA = decltype(s->joints[0].rA)(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+    }
   };
   if (o->f) go(o->f); else go(o->d);
   return 0;

@@ -56,6 +56,8 @@ int oracle_cloth_stats(oracle_solver* o, int32_t* dat_pairs, int32_t* dat_vt, in
 int oracle_dat_stats(oracle_solver* o, int32_t* out6);
 /* Test hooks of the Planar-DAT fixtures: query radius override (> 0), and
  * pair / soft-contact capacity caps (> 0), as the backend's. */
+/* setJointWorldAnchors: world points of world-anchored joints. */
+int oracle_set_joint_world_anchors(oracle_solver* o, const uint32_t* joints, uint32_t n, const float* pts);
 int oracle_set_dat_test_hooks(oracle_solver* o, float query_radius, int32_t pair_capacity, int32_t soft_capacity);
 
 /* unit restatements for pinning */
