@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libavbd_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_COLORS = 64
 
 # avbd_status
@@ -67,7 +67,8 @@ class SceneDesc(C.Structure):
                 ("num_tets", C.c_uint32), ("tet_ids", U32P), ("tet_mu", FP), ("tet_lambda", FP),
                 ("num_tris", C.c_uint32), ("tri_ids", U32P), ("tri_mu", FP), ("tri_lambda", FP),
                 ("tri_bend", FP),
-                ("num_exclusions", C.c_uint32), ("exclusion_pairs", U32P)]
+                ("num_exclusions", C.c_uint32), ("exclusion_pairs", U32P),
+                ("collider_hull_count", U32P), ("hull_vertices", FP)]
 
 
 class Options(C.Structure):

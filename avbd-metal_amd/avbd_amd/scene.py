@@ -120,7 +120,7 @@ class PhysicsScene:
 
     def addCollider(self, body, size, friction=None, localPosition=(0, 0, 0), localRotation=(0, 0, 0, 1),
                     shape=BOX, collisionGroup=0, collidesWithSharedGeometry=True, collisionEnabled=True,
-                    dynamicFriction=None):
+                    dynamicFriction=None, convexHullVertices=()):
         bd = self.bodies[body]
         fr = bd["friction"] if friction is None else friction
         df = (bd["dynamicFriction"] if dynamicFriction is None else dynamicFriction)
@@ -130,8 +130,29 @@ class PhysicsScene:
         self.colliders.append(dict(body=body, size=size, friction=float(fr), dynamicFriction=float(df),
                                    localPosition=tuple(localPosition), localRotation=tuple(localRotation),
                                    shape=shape, group=int(collisionGroup), shared=bool(collidesWithSharedGeometry),
-                                   enabled=bool(collisionEnabled), worldRound=False))
+                                   enabled=bool(collisionEnabled), worldRound=False,
+                                   hull=[tuple(float(x) for x in v) for v in convexHullVertices]))
         return len(self.colliders) - 1
+
+    def addConvexCollider(self, body, vertices, friction=None, dynamicFriction=None, localPosition=(0, 0, 0),
+                          localRotation=(0, 0, 0, 1), collisionGroup=0, collidesWithSharedGeometry=True,
+                          collisionEnabled=True):
+        """Scene.swift:645-679: a cooked convex hull (4..64 vertices) stored
+        about its AABB centre, the offset folded into the local position."""
+        v = np.asarray(vertices, np.float32).reshape(-1, 3)
+        assert 4 <= len(v) <= 64 and np.all(np.isfinite(v)), "convex collider requires 4...64 finite vertices"
+        lo, hi = v.min(axis=0), v.max(axis=0)
+        center = (lo + hi) * np.float32(0.5)
+        x, y, z, w = (float(c) for c in localRotation)
+        c = center.astype(np.float64)
+        t = 2.0 * np.cross([x, y, z], c)
+        rc = c + w * t + np.cross([x, y, z], t)        # localRotation.act(center)
+        lp = tuple(float(a) + float(b) for a, b in zip(localPosition, rc))
+        return self.addCollider(body, tuple(float(a) for a in hi - lo), friction=friction, localPosition=lp,
+                                localRotation=localRotation, shape=BOX, collisionGroup=collisionGroup,
+                                collidesWithSharedGeometry=collidesWithSharedGeometry,
+                                collisionEnabled=collisionEnabled, dynamicFriction=dynamicFriction,
+                                convexHullVertices=[tuple(float(a) for a in q) for q in (v - center)])
 
     # -- Scene.swift:684-702: density chosen so the sphere volume formula gives `mass`
     def addParticle(self, radius, mass, position, friction=0.5, velocity=(0, 0, 0)):
@@ -221,6 +242,10 @@ class PhysicsScene:
             d.collider_enabled = ptr(arr([1 if c["enabled"] else 0 for c in CL], np.uint8), C.c_uint8)
             d.collider_world_round_anchor = ptr(arr([1 if c["worldRound"] else 0 for c in CL], np.uint8),
                                                 C.c_uint8)
+            hulls = [c.get("hull", []) for c in CL]
+            if any(hulls):
+                d.collider_hull_count = ptr(arr([len(h) for h in hulls], np.uint32), C.c_uint32)
+                d.hull_vertices = ptr(arr([v for h in hulls for v in h], np.float32), C.c_float)
         J = self.joints
         d.num_joints = len(J)
         if J:

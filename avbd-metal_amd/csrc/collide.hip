@@ -572,6 +572,49 @@ __device__ void np_support_edge(const NPBox& b, int k, V3 dir, V3& e0, V3& e1) {
   e1 = c + b.ax[k] * np_half(b, k);
 }
 
+// one convex-hull vertex against an oriented box (30_narrowphase.metal:
+// 189-247): inside, the nearest exit face; outside, the clamped point within
+// the velocity-aware detection margin
+struct NPHullCand {
+  V3 xHull, xBox, boxToHull;
+  float sep;
+  uint32_t feature;
+};
+__device__ __forceinline__ bool np_hull_vertex(V3 xw, uint32_t vid, const NPBox& box, F4 qBox, V3 rel, float hullR,
+                                               float boxR, const StepParams& P, NPHullCand& out) {
+  const V3 local = q_rotate(q_conj(qBox), xw - box.c);
+  const V3 cl = clamp3(local, -box.h, box.h);
+  V3 nL, bL;
+  float sep;
+  const bool inside = local.x >= -box.h.x && local.y >= -box.h.y && local.z >= -box.h.z && local.x <= box.h.x &&
+                      local.y <= box.h.y && local.z <= box.h.z;
+  if (inside) {
+    V3 clr = box.h - vabs(local);
+    int axis = 0;
+    if (clr.y < comp(clr, axis)) axis = 1;
+    if (clr.z < comp(clr, axis)) axis = 2;
+    nL = v3s(0);
+    setcomp(nL, axis, comp(local, axis) >= 0.0f ? 1.0f : -1.0f);
+    bL = local;
+    setcomp(bL, axis, comp(nL, axis) * comp(box.h, axis));
+    sep = -comp(clr, axis);
+  } else {
+    V3 d = local - cl;
+    float dist = length(d);
+    if (dist < 1e-8f) return false;
+    nL = d / dist;
+    if (dist > np_detect_margin(q_rotate(qBox, nL), rel, P, hullR, boxR)) return false;
+    bL = cl;
+    sep = dist;
+  }
+  out.xHull = xw;
+  out.xBox = q_rotate(qBox, bL) + box.c;
+  out.boxToHull = q_rotate(qBox, nL);
+  out.sep = sep;
+  out.feature = (4u << 24) | (vid & 0x00FFFFFFu);
+  return true;
+}
+
 __device__ __forceinline__ V3 np_closest_on_segment(V3 p, V3 a, V3 b) {
   V3 ab = b - a;
   float t = clampf(dot(p - a, ab) / fmaxf(dot(ab, ab), 1e-12f), 0.0f, 1.0f);
@@ -656,6 +699,13 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
     int count = 0;
     V3 nrm;  // from B toward A
     const bool capA = (tA & kShapeKindMask) == 3, capB = (tB & kShapeKindMask) == 3;
+    const bool hullA = (tA & kShapeKindMask) == 4, hullB = (tB & kShapeKindMask) == 4;
+    // convex hulls collide with boxes only (30_narrowphase.metal:404-410);
+    // other hull pairs are refused, not approximated by the hull's box
+    if ((hullA || hullB) && ((hullA && hullB) || ((hullA ? tB : tA) & kShapeKindMask) != 0)) {
+      outM.header = U4{ba, bb, 0, 0};
+      continue;
+    }
 
     if (capA || capB || sphA || sphB) {
       float rA = shA.x * 0.5f, rB = shB.x * 0.5f;
@@ -871,6 +921,70 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
       continue;
     }
 
+    if (hullA || hullB) {
+      // convex hull vs box (30_narrowphase.metal:411-497): the deepest hull
+      // vertex, then up to kMaxContacts - 1 more within 0.75 margin of it
+      // along the same face normal, each maximizing its least tangential
+      // distance to those chosen (ties to the lower vertex id)
+      const bool hIsA = hullA;
+      const uint32_t ih = hIsA ? ia : ib;
+      const F4 qH = hIsA ? qA : qB, qBox = hIsA ? qB : qA;
+      const V3 pH = hIsA ? A.c : B.c;
+      const NPBox& box = hIsA ? B : A;
+      const V3 rel = hIsA ? (vA - vB) : (vB - vA);
+      const float hullR = fabsf(hIsA ? shA.w : shB.w), boxR = fabsf(hIsA ? shB.w : shA.w);
+      const U2 range = S.colliderHullRange[ih];
+      NPHullCand best, chosen[kMaxContacts];
+      bool haveBest = false;
+      for (uint32_t v = 0; v < range.y; v++) {
+        NPHullCand c;
+        if (!np_hull_vertex(pH + q_rotate(qH, xyz(S.hullVerts[range.x + v])), v, box, qBox, rel, hullR, boxR, P, c))
+          continue;
+        if (!haveBest || c.sep < best.sep - 1e-8f || (fabsf(c.sep - best.sep) <= 1e-8f && c.feature < best.feature)) {
+          best = c;
+          haveBest = true;
+        }
+      }
+      if (!haveBest) { outM.header = U4{ba, bb, 0, 0}; continue; }
+      chosen[0] = best;
+      int nc = 1;
+      const float band = 0.75f * P.collisionMargin;
+      while (nc < kMaxContacts) {
+        NPHullCand next;
+        bool haveNext = false;
+        float bestSpread = -1.0f;
+        for (uint32_t v = 0; v < range.y; v++) {
+          NPHullCand c;
+          if (!np_hull_vertex(pH + q_rotate(qH, xyz(S.hullVerts[range.x + v])), v, box, qBox, rel, hullR, boxR, P, c))
+            continue;
+          if (c.sep > best.sep + band || dot(c.boxToHull, best.boxToHull) < 0.98f) continue;
+          bool dup = false;
+          float minSpread = 3.402823466e38f;
+          for (int j = 0; j < nc; j++) {
+            if (c.feature == chosen[j].feature) { dup = true; break; }
+            V3 d = c.xHull - chosen[j].xHull;
+            d = d - best.boxToHull * dot(best.boxToHull, d);
+            minSpread = fminf(minSpread, dot(d, d));
+          }
+          if (dup) continue;
+          if (!haveNext || minSpread > bestSpread + 1e-10f ||
+              (fabsf(minSpread - bestSpread) <= 1e-10f && c.feature < next.feature)) {
+            next = c;
+            bestSpread = minSpread;
+            haveNext = true;
+          }
+        }
+        if (!haveNext || bestSpread < kMergeDistSq) break;
+        chosen[nc++] = next;
+      }
+      for (int i = 0; i < nc; i++) {
+        cs[i].xA = hIsA ? chosen[i].xHull : chosen[i].xBox;
+        cs[i].xB = hIsA ? chosen[i].xBox : chosen[i].xHull;
+        cs[i].feature = chosen[i].feature;
+      }
+      count = nc;
+      nrm = hIsA ? best.boxToHull : -best.boxToHull;
+    } else {
     // ---- box-box SAT ----
     NPBest bestFace, bestEdge;
     bestFace.valid = false; bestFace.sep = -3.402823466e38f;
@@ -944,6 +1058,7 @@ __global__ void __launch_bounds__(64) k_np_collide(DeviceState S, StepParams P) 
       if (count == 0) np_add(cs, count, mids, np_support(A, best.n), np_support(B, -best.n), prefix);
     }
     nrm = -best.n;
+    }  // box-box SAT
     V3 t1, t2;
     orthonormal(nrm, t1, t2);
     int prevIdx = pair_map_find(S, P.mapCapacity, ia, ib);

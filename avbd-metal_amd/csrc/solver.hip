@@ -508,13 +508,20 @@ avbd_status validate(const avbd_scene_desc* d) {
     if (d->body_shape[i] > AVBD_SHAPE_CAPSULE) return AVBD_ERR_INVALID_ARGUMENT;
   if (d->num_colliders && (!d->collider_body || !d->collider_size || !d->collider_friction || !d->collider_shape))
     return AVBD_ERR_INVALID_ARGUMENT;
+  uint64_t hullTotal = 0;
   for (uint32_t i = 0; i < d->num_colliders; i++) {
     if (d->collider_body[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
     uint32_t sh = d->collider_shape[i];
     bool enabled = !d->collider_enabled || d->collider_enabled[i];
     if (enabled && sh != AVBD_SHAPE_BOX && sh != AVBD_SHAPE_SPHERE && sh != AVBD_SHAPE_CAPSULE)
       return AVBD_ERR_UNSUPPORTED;
+    // convex hulls: at most 64 finite vertices (GPUSolver.swift:1024-1025)
+    const uint32_t hn = d->collider_hull_count ? d->collider_hull_count[i] : 0u;
+    if (hn > 64u || (hn && !d->hull_vertices)) return AVBD_ERR_INVALID_ARGUMENT;
+    hullTotal += hn;
   }
+  for (uint64_t k = 0; k < 3 * hullTotal; k++)
+    if (!isfinite(d->hull_vertices[k])) return AVBD_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < d->num_joints; i++) {
     if (d->joint_body_b[i] < 0 || (uint32_t)d->joint_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
     if (d->joint_body_a[i] >= 0 && (uint32_t)d->joint_body_a[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
@@ -939,6 +946,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   const uint32_t nc = d->num_colliders;
   std::vector<uint32_t> co(nc), ct(nc), cg(nc), csc(nc);
   std::vector<F4> cs(nc), cp(nc), cq(nc), cf(nc);
+  // convex hull colliders: (first vertex, count) per collider, vertices collider-local
+  std::vector<U2> hullRange(nc, U2{0u, 0u});
+  std::vector<F4> hullVerts;
   std::vector<float> radii;
   std::vector<float> crad(nc);
   uint32_t enabledCount = 0;
@@ -953,6 +963,16 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     else if (shape == AVBD_SHAPE_TORUS) r = size.x + size.y;
     else if (shape == AVBD_SHAPE_CAPSULE) r = size.x / 2.0f + size.y;
     else { V3 h = size * 0.5f; r = sqrtf(h.x * h.x + h.y * h.y + h.z * h.z); }
+    const uint32_t hullN = d->collider_hull_count ? d->collider_hull_count[i] : 0u;
+    if (hullN) {  // the hull's radius is its farthest vertex (GPUSolver.swift:994-997)
+      hullRange[i] = U2{(uint32_t)hullVerts.size(), hullN};
+      r = 0.0f;
+      for (uint32_t k = 0; k < hullN; k++) {
+        V3 v = rd3(d->hull_vertices, hullRange[i].x + k);
+        hullVerts.push_back(f4(v, 0.0f));
+        r = fmaxf(r, length(v));
+      }
+    }
     crad[i] = r;
     bool particle = d->body_is_particle && d->body_is_particle[b];
     bool enabled = !d->collider_enabled || d->collider_enabled[i];
@@ -967,7 +987,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     float qn = sqrtf(lq.x * lq.x + lq.y * lq.y + lq.z * lq.z + lq.w * lq.w);
     cq[i] = F4{lq.x / qn, lq.y / qn, lq.z / qn, lq.w / qn};
     cf[i] = F4{fr, dfr, 0, 0};
-    uint32_t flags = shape;
+    uint32_t flags = hullN ? 4u : shape;
     if (particle) flags |= kShapeParticle;
     if (d->collider_world_round_anchor && d->collider_world_round_anchor[i]) flags |= kShapeWorldRound;
     ct[i] = flags;
@@ -1326,6 +1346,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.posAngSnap, nb)
   ALLOC(S.colliderOwner, nc) ALLOC(S.colliderShapeType, nc) ALLOC(S.colliderGroup, nc) ALLOC(S.colliderShared, nc)
   ALLOC(S.colliderShape, nc) ALLOC(S.colliderLocalPos, nc) ALLOC(S.colliderLocalRot, nc)
+  ALLOC(S.colliderHullRange, std::max<uint32_t>(1, nc)) ALLOC(S.hullVerts, std::max<size_t>(1, hullVerts.size()))
   ALLOC(S.colliderFriction, nc)
   ALLOC(S.hashedIdx, std::max<size_t>(1, hashed.size())) ALLOC(S.globalIdx, std::max<size_t>(1, globals.size()))
   ALLOC(S.cellCount, s->gridHashSize) ALLOC(S.cellStart, s->gridHashSize) ALLOC(S.cellRigid, s->gridHashSize)
@@ -1402,6 +1423,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.prevVelLin, pv) || upload(S.props, pr) || upload(S.shape, sh) || upload(S.gravityScale, gs) ||
       upload(S.clothGroup, clothGroup) || upload(S.colliderOwner, co) || upload(S.colliderShapeType, ct) ||
       upload(S.colliderGroup, cg) || upload(S.colliderShared, csc) || upload(S.colliderShape, cs) ||
+      upload(S.colliderHullRange, hullRange) || upload(S.hullVerts, hullVerts) ||
       upload(S.colliderLocalPos, cp) || upload(S.colliderLocalRot, cq) || upload(S.colliderFriction, cf) ||
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
       upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||

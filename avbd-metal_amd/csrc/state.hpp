@@ -20,6 +20,8 @@ struct DeviceState {
   // colliders
   uint32_t *colliderOwner, *colliderShapeType, *colliderGroup, *colliderShared;
   F4 *colliderShape, *colliderLocalPos, *colliderLocalRot, *colliderFriction;
+  U2* colliderHullRange;      // convex hulls: (first vertex, count), count 0 = primitive
+  F4* hullVerts;              // hull vertices, collider-local
   // broad phase
   uint32_t *hashedIdx, *globalIdx;
   uint32_t* hashedRigid;      // hashed colliders whose owner is not a particle

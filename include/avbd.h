@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 1
+#define AVBD_ABI_VERSION 2
 #define AVBD_MAX_COLORS 64
 
 /* ---- status codes ------------------------------------------------------ */
@@ -171,6 +171,13 @@ typedef struct avbd_scene_desc {
     /* SceneCollisionExclusion, Scene.swift:466-475 */
     uint32_t num_exclusions;
     const uint32_t* exclusion_pairs;      /* [e*2] */
+
+    /* SceneCollider.convexHullVertices, Scene.swift:127: per collider a
+     * vertex count (0 = its primitive shape, at most 64), then the vertices
+     * of every hull collider in collider order, collider-local.  A hull
+     * collides with boxes only (30_narrowphase.metal:404-410).  NULL -> none. */
+    const uint32_t* collider_hull_count;  /* [m] */
+    const float* hull_vertices;           /* [sum(count)*3] */
 } avbd_scene_desc;
 
 /* Solver creation options (GPUSolver.init maxPairsPerBody, GPUSolver.swift:331) */

@@ -258,6 +258,7 @@ struct Oracle : oracle_base {
   std::vector<uint32_t> clothGroup;
   // colliders
   std::vector<uint32_t> cOwner, cGroup, cShared, cFlags;
+  std::vector<std::vector<V<T>>> cHull;  // convex hull vertices per collider (collider-local), empty = primitive
   std::vector<V<T>> cSize, cLocal;
   std::vector<Q<T>> cLocalRot;
   std::vector<T> cRad, cStatF, cDynF;
@@ -366,6 +367,8 @@ struct Oracle : oracle_base {
     }
     // colliders (GPUSolver.swift:971-1090)
     cOwner.resize(nc); cGroup.resize(nc); cShared.resize(nc); cFlags.resize(nc); cSize.resize(nc); cLocal.resize(nc);
+    cHull.assign(nc, {});
+    uint32_t hullBase = 0;
     cLocalRot.resize(nc); cRad.resize(nc); cStatF.resize(nc); cDynF.resize(nc);
     std::vector<float> radii;
     uint32_t enabled = 0;
@@ -379,6 +382,16 @@ struct Oracle : oracle_base {
       if (sh == AVBD_SHAPE_SPHERE) r = sx / 2.0f;
       else if (sh == AVBD_SHAPE_CAPSULE) r = sx / 2.0f + sy;
       else { float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f; r = std::sqrt(hx * hx + hy * hy + hz * hz); }
+      const uint32_t hn = d->collider_hull_count ? d->collider_hull_count[i] : 0u;
+      if (hn) {  // GPUSolver.swift:994-997: the farthest vertex bounds the hull
+        r = 0.0f;
+        for (uint32_t k = 0; k < hn; k++) {
+          const float* v = d->hull_vertices + 3 * (hullBase + k);
+          cHull[i].push_back(V<T>(v[0], v[1], v[2]));
+          r = std::max(r, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        }
+        hullBase += hn;
+      }
       rad32[i] = r;
       bool en = !d->collider_enabled || d->collider_enabled[i];
       cOwner[i] = b;
@@ -392,7 +405,7 @@ struct Oracle : oracle_base {
       cLocalRot[i] = Q<T>{lq.x / qn, lq.y / qn, lq.z / qn, lq.w / qn};
       cStatF[i] = d->collider_friction[i];
       cDynF[i] = d->collider_dynamic_friction ? d->collider_dynamic_friction[i] : d->collider_friction[i];
-      uint32_t f = sh;
+      uint32_t f = hn ? 4u : sh;
       if (particle[b]) f |= 0x10u;
       if (d->collider_world_round_anchor && d->collider_world_round_anchor[i]) f |= 0x20u;
       cFlags[i] = f;
@@ -985,6 +998,9 @@ struct Oracle : oracle_base {
       return V<T>(clampT(p.x * gam, fl.x, T(1e10)), clampT(p.y * gam, fl.y, T(1e6)), clampT(p.z * gam, fl.z, T(1e6)));
     };
     bool capA = (cFlags[ia] & 0xF) == 3, capB = (cFlags[ib] & 0xF) == 3;
+    bool hullA = (cFlags[ia] & 0xF) == 4, hullB = (cFlags[ib] & 0xF) == 4;
+    // hulls collide with boxes only (30_narrowphase.metal:404-410)
+    if ((hullA || hullB) && ((hullA && hullB) || (cFlags[hullA ? ib : ia] & 0xF) != 0)) return;
     if (capA || capB || sphA || sphB) {
       T rA = cSize[ia].x * T(0.5), rB = cSize[ib].x * T(0.5);
       T bestDRound;
@@ -1158,6 +1174,90 @@ struct Oracle : oracle_base {
       }
       return;
     }
+    if (hullA || hullB) {
+      // convex hull vs box (30_narrowphase.metal:189-247, 411-497)
+      bool hIsA = hullA;
+      uint32_t ih = hIsA ? ia : ib;
+      Q<T> qH = hIsA ? qA : qB, qBox = hIsA ? qB : qA;
+      V<T> pH = hIsA ? A.c : B.c;
+      const Box& box = hIsA ? B : A;
+      V<T> rel = hIsA ? (vA - vB) : (vB - vA);
+      T hullR = std::fabs(cRad[ih]), boxR = std::fabs(cRad[hIsA ? ib : ia]);
+      struct Cand { V<T> xHull, xBox, n; T sep; uint32_t f; };
+      auto vertex = [&](uint32_t vid, Cand& c) -> bool {
+        V<T> xw = pH + qrot(qH, cHull[ih][vid]);
+        V<T> l = qrot(qconj(qBox), xw - box.c);
+        V<T> cl(clampT(l.x, -box.h.x, box.h.x), clampT(l.y, -box.h.y, box.h.y), clampT(l.z, -box.h.z, box.h.z));
+        V<T> nL, bL;
+        T sep;
+        bool inside = l.x >= -box.h.x && l.y >= -box.h.y && l.z >= -box.h.z && l.x <= box.h.x && l.y <= box.h.y &&
+                      l.z <= box.h.z;
+        if (inside) {
+          V<T> clr = box.h - V<T>(std::fabs(l.x), std::fabs(l.y), std::fabs(l.z));
+          int ax = 0;
+          if (clr.y < clr[ax]) ax = 1;
+          if (clr.z < clr[ax]) ax = 2;
+          nL = V<T>();
+          nL[ax] = l[ax] >= 0 ? T(1) : T(-1);
+          bL = l;
+          bL[ax] = nL[ax] * box.h[ax];
+          sep = -clr[ax];
+        } else {
+          V<T> dd = l - cl;
+          T dist = lenv(dd);
+          if (dist < T(1e-8)) return false;
+          nL = dd / dist;
+          if (dist > detectMargin(qrot(qBox, nL), rel, hullR, boxR)) return false;
+          bL = cl;
+          sep = dist;
+        }
+        c = Cand{xw, qrot(qBox, bL) + box.c, qrot(qBox, nL), sep, (4u << 24) | (vid & 0x00FFFFFFu)};
+        return true;
+      };
+      const uint32_t nv = (uint32_t)cHull[ih].size();
+      Cand best{};
+      bool have = false;
+      for (uint32_t v = 0; v < nv; v++) {
+        Cand c;
+        if (!vertex(v, c)) continue;
+        if (!have || c.sep < best.sep - T(1e-8) || (std::fabs(c.sep - best.sep) <= T(1e-8) && c.f < best.f)) {
+          best = c;
+          have = true;
+        }
+      }
+      if (!have) return;
+      std::vector<Cand> chosen{best};
+      T band = T(0.75) * (T)set.collision_margin;
+      while (chosen.size() < 8) {
+        Cand next{};
+        bool haveNext = false;
+        T bestSpread = T(-1);
+        for (uint32_t v = 0; v < nv; v++) {
+          Cand c;
+          if (!vertex(v, c)) continue;
+          if (c.sep > best.sep + band || dotv(c.n, best.n) < T(0.98)) continue;
+          bool dup = false;
+          T minSpread = std::numeric_limits<T>::max();
+          for (const Cand& ch : chosen) {
+            if (c.f == ch.f) { dup = true; break; }
+            V<T> dd = c.xHull - ch.xHull;
+            dd = dd - best.n * dotv(best.n, dd);
+            minSpread = std::min(minSpread, dotv(dd, dd));
+          }
+          if (dup) continue;
+          if (!haveNext || minSpread > bestSpread + T(1e-10) ||
+              (std::fabs(minSpread - bestSpread) <= T(1e-10) && c.f < next.f)) {
+            next = c;
+            bestSpread = minSpread;
+            haveNext = true;
+          }
+        }
+        if (!haveNext || bestSpread < T(1e-6)) break;
+        chosen.push_back(next);
+      }
+      for (const Cand& ch : chosen) cs.push_back(Pt{hIsA ? ch.xHull : ch.xBox, hIsA ? ch.xBox : ch.xHull, ch.f});
+      nrm = hIsA ? best.n : -best.n;
+    } else {
     // box-box SAT
     Best bf{0, 0, 0, -std::numeric_limits<T>::max(), V<T>(), false}, be = bf;
     T cap = specCap(cRad[ia], cRad[ib]);
@@ -1219,6 +1319,7 @@ struct Oracle : oracle_base {
       if (cs.empty()) addPt(cs, mids, support(A, best.n), support(B, -best.n), prefix);
     }
     nrm = -best.n;
+    }  // box-box SAT
     V<T> t1, t2;
     basis(nrm, t1, t2);
     int prev = prevIndex();

@@ -118,3 +118,33 @@ def capsule_segment(pos, rot, length):
     ax = np.array([2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)])
     c = np.asarray(pos, np.float64)
     return c - ax * length / 2, c + ax * length / 2
+
+
+# -- convex hulls ---------------------------------------------------------------
+CUBE = [(-0.5, -0.5, -0.5), (0.5, -0.5, -0.5), (-0.5, 0.5, -0.5), (0.5, 0.5, -0.5),
+        (-0.5, -0.5, 0.5), (0.5, -0.5, 0.5), (-0.5, 0.5, 0.5), (0.5, 0.5, 0.5)]
+
+
+def hull_cube():
+    """testConvexHullColliderRestsOnBoxTerrain (GPUSolverTests.swift:537-570)."""
+    s = PhysicsScene("convex-hull-box-contact", SimSettings(dt=1.0 / 120.0, gravity=-9.81, iterations=20))
+    ground = s.addBody((20, 20, 0.2), 0.0, 1.0, (0, 0, -0.1))
+    body = s.addBody((1, 1, 1), 0.0, 1.0, (0, 0, 2), mass=1.0, diagonalInertia=(1 / 6, 1 / 6, 1 / 6),
+                     collisionEnabled=False)
+    s.addConvexCollider(body, CUBE, friction=1.0)
+    return s, ground, body
+
+
+def hull_wedge():
+    """A triangular-prism hull (6 vertices, an off-centre AABB) dropped tilted
+    onto the ground next to a sphere it overlaps: hull-box contacts with
+    up to 8 chosen vertices, the hull-sphere pair refused."""
+    s = PhysicsScene("hull wedge", SimSettings(dt=1.0 / 120.0, iterations=20))
+    s.addBody((20, 20, 0.2), 0.0, 0.8, (0, 0, -0.1))
+    body = s.addBody((0.6, 0.4, 0.3), 0.0, 0.8, (0, 0, 0.6), rotation=quat(0.3, (1, 0.5, 0)), mass=2.0,
+                     diagonalInertia=(0.05, 0.08, 0.1), collisionEnabled=False)
+    wedge = [(-0.3, -0.2, 0.0), (0.3, -0.2, 0.0), (-0.3, 0.2, 0.0), (0.3, 0.2, 0.0), (-0.3, 0.0, 0.3),
+             (0.3, 0.0, 0.3)]
+    s.addConvexCollider(body, wedge, friction=0.8, localPosition=(0.05, 0, 0))
+    sph = s.addSphere(0.3, 1.0, 0.5, (0.05, 0.0, 0.75))
+    return s, body, sph

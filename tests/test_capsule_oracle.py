@@ -113,3 +113,31 @@ def test_capsule_capsule_normal_matches_brute_force_closest_points():
         i, j = np.unravel_index(np.argmin(d), d.shape)
         want = (P[i] - Q[j]) / np.linalg.norm(P[i] - Q[j])      # from B toward A
         assert np.dot(nrm[0], want) > 0.999, trial
+
+
+def test_convex_hull_rests_on_box_terrain():
+    # GPUSolverTests.testConvexHullColliderRestsOnBoxTerrain (:537-570)
+    s, ground, body = F.hull_cube()
+    o = oracle.OracleSolver(s)
+    o.step(360)
+    p, r, _, _ = o.bodyStates()
+    assert abs(p[body, 2] - 0.5) < 0.06
+    assert abs(r[body, 0]) < 0.05 and abs(r[body, 1]) < 0.05
+    h, nrm, _ = o.manifolds()
+    act = [(int(a), int(b), int(n)) for a, b, n in h if n > 0]
+    assert (ground, body, 4) in act                  # the four bottom vertices
+
+
+def test_hull_pairs_other_than_box_are_refused():
+    s, body, sph = F.hull_wedge()
+    o = oracle.OracleSolver(s)
+    o.step()
+    h, _, _ = o.manifolds()
+    pairs = {(int(a), int(b)): int(n) for a, b, n in h}
+    assert pairs.get((body, sph), pairs.get((sph, body))) == 0     # overlapping, but no contacts
+    o.step(240)
+    p = o.positions()
+    assert np.all(np.isfinite(p))
+    h, _, _ = o.manifolds()
+    ground_contacts = [int(n) for a, b, n in h if {int(a), int(b)} == {0, body}]
+    assert ground_contacts and 2 <= ground_contacts[0] <= 8

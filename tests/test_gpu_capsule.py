@@ -120,3 +120,33 @@ def test_reference_capsule_contracts():
     p, _, v, _ = g.bodyStates()
     flush_print(f"  foot creep {p[foot, 0] - settled[0]:.2e} m, speed {np.linalg.norm(v[foot]):.2e}")
     assert abs(p[foot, 0] - settled[0]) < 0.001 and np.linalg.norm(v[foot]) < 0.03
+
+
+def test_convex_hull_contacts():
+    _case("convex hull vs box terrain (GPUSolverTests.swift:537-570), hull vs sphere refused, parity")
+    s, ground, body = F.hull_cube()
+    g = GPUSolver(s)
+    g.step(360)
+    p, r, _, _ = g.bodyStates()
+    assert abs(p[body, 2] - 0.5) < 0.06 and abs(r[body, 0]) < 0.05 and abs(r[body, 1]) < 0.05
+    m = g.manifolds()
+    act = [k for k in range(len(m)) if m["header"][k][2] > 0]
+    assert len(act) == 1 and tuple(m["header"][act[0]][:3]) == (ground, body, 4)
+    feats = sorted(int(x) for x in m["c"][act[0]]["rA"][:4, 3].view(np.uint32))
+    assert feats == [(4 << 24) | v for v in (0, 1, 2, 3)]      # the four bottom hull vertices
+
+    s, body, sph = F.hull_wedge()
+    g = GPUSolver(s)
+    o = oracle.OracleSolver(s)
+    for k in range(90):
+        g.step()
+        o.step()
+        gm = g.manifolds()
+        h, nrm, _ = o.manifolds()
+        assert np.array_equal(gm["header"][:, :3], h), k
+        for a, b, n in h:
+            if {int(a), int(b)} == {body, sph}:
+                assert n == 0, k
+    err = float(np.abs(g.positions() - o.positions()).max())
+    flush_print(f"  wedge after 90 steps: max|x_gpu - x_oracle| = {err:.3e} m")
+    assert err <= 1e-3
