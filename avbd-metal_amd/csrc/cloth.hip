@@ -179,8 +179,11 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
       dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
     }
     if (dup) continue;
-    if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[h], 1u)] = entry;
-    else atomicAdd(&S.elemCellCount[h], 1u);
+    // counts and cursors are (triangles, edges) per bucket, interleaved, so
+    // one scan lays out every bucket as its triangles, then its edges
+    const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
+    if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
+    else atomicAdd(&S.elemCellCount[slot], 1u);
   }
 }
 
@@ -279,7 +282,7 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
     }
     Cell3 cell = cell_of(p, P.elemCellSize);
     uint32_t h = cell_hash(cell.x, cell.y, cell.z, 0u, P.elemHashSize);
-    uint32_t s = S.elemCellStart[h], e = s + S.elemCellCount[h];
+    uint32_t s = S.elemCellStart[2u * h], e = S.elemCellStart[2u * h + 1u];  // the bucket's triangles
     uint32_t ns = S.nbrStart[v], ne = ns + S.nbrCount[v];
     for (uint32_t base = s; base < e; base += width) {
       uint32_t k = base + lane;
@@ -309,7 +312,7 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
         int x = cell_add(ar.c0.x, ix), y = cell_add(ar.c0.y, iy), z = cell_add(ar.c0.z, iz);
         Cell3 cell{x, y, z};
         uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
-        uint32_t s = S.elemCellStart[h], e = s + S.elemCellCount[h];
+        uint32_t s = S.elemCellStart[2u * h + 1u], e = S.elemCellStart[2u * h + 2u];  // the bucket's edges
         for (uint32_t base = s; base < e; base += width) {
           uint32_t k = base + lane;
           uint32_t entry = k < e ? S.elemCells[k] : 0u;
@@ -1115,10 +1118,12 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   if (P.numTris == 0) return;
   uint32_t nElem = P.numTris + P.numEdges;
   uint32_t eb = (nElem + 255) / 256;
+  const uint32_t slots = 2u * P.elemHashSize;
+  launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
   hipLaunchKernelGGL(k_el_bin<false>, dim3(eb), dim3(256), 0, st, S, P);
-  launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, P.elemHashSize, S.scanScratch);
-  hipLaunchKernelGGL(k_copy_u32, dim3(cap_blocks(P.elemHashSize, gridCap)), dim3(256), 0, st, S.elemCellCursor,
-                     S.elemCellStart, P.elemHashSize);
+  launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, slots + 1u, S.scanScratch);
+  hipLaunchKernelGGL(k_copy_u32, dim3(cap_blocks(slots, gridCap)), dim3(256), 0, st, S.elemCellCursor,
+                     S.elemCellStart, slots);
   hipLaunchKernelGGL(k_el_bin<true>, dim3(eb), dim3(256), 0, st, S, P);
   uint32_t nProd = P.numParticles + P.numEdges;
   uint32_t pb = (nProd + 3) / 4;

@@ -230,10 +230,9 @@ using StageFn = std::function<void(const char*)>;
 
 void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P, const StageFn& stage) {
   hipStream_t st = s->stream;
-  uint32_t* zp[6] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.elemCellCount, S.fineCount};
-  uint32_t zn[6] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->nTris ? s->elemHashSize : 0u,
-                    s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
-  launch_zero_multi(st, zp, zn, 6);
+  uint32_t* zp[5] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.fineCount};
+  uint32_t zn[5] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
+  launch_zero_multi(st, zp, zn, 5);
   if (s->nTris) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
   launch_broadphase(st, S, P);
   stage("broadphase");
@@ -255,7 +254,6 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   if (s->nTris) {
     launch_dat_predictor_pass(st, S, P, s->gridCap);
     stage("dat-predict");
-    launch_fill_u32(st, S.elemCellCount, s->elemHashSize, 0u);
     launch_dat_pairs(st, S, P, s->gridCap);
     stage("dat-accepted-pairs");
     launch_dat_contacts(st, S, P, s->gridCap);
@@ -1387,8 +1385,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.nbrStart, cl ? nb : 1u) ALLOC(S.nbrCount, cl ? nb : 1u)
     ALLOC(S.nbrList, std::max<size_t>(1, ch.nbrList.size())) ALLOC(S.particleIdx, std::max<uint32_t>(1, s->nParticles))
     ALLOC(S.selfCollision, cl ? nb : 1u)
-    ALLOC(S.elemCellCount, cl ? s->elemHashSize : 1u) ALLOC(S.elemCellStart, cl ? s->elemHashSize : 1u)
-    ALLOC(S.elemCellCursor, cl ? s->elemHashSize : 1u) ALLOC(S.elemCells, std::max<uint64_t>(1, 64ull * nElem))
+    ALLOC(S.elemCellCount, cl ? 2 * s->elemHashSize + 1 : 1u) ALLOC(S.elemCellStart, cl ? 2 * s->elemHashSize + 1 : 1u)
+    ALLOC(S.elemCellCursor, cl ? 2 * s->elemHashSize : 1u) ALLOC(S.elemCells, std::max<uint64_t>(1, 64ull * nElem))
     ALLOC(S.elemBox, std::max<uint32_t>(2, 2 * nElem))
     ALLOC(S.datAnchor, cl ? nb : 1u) ALLOC(S.datT, cl ? nb : 1u)
     ALLOC(S.datCount, nProd + 1) ALLOC(S.datStart, nProd + 1)
@@ -1401,7 +1399,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.velScratch, cl ? nb : 1u)
   }
   uint32_t scanMax = std::max<uint32_t>(std::max<uint32_t>(s->gridHashSize, producers), std::max(nb, kMaxColors * bb));
-  scanMax = std::max<uint32_t>(scanMax, std::max(s->elemHashSize, s->nParticles + s->nEdges + 1));
+  scanMax = std::max<uint32_t>(scanMax, std::max(2 * s->elemHashSize + 1, s->nParticles + s->nEdges + 1));
   scanMax = std::max<uint32_t>(scanMax, std::max(s->nTris, s->maxDatPairs) + 1);
   ALLOC(S.scanScratch, scan_scratch_words(scanMax) + 16)
   s->residualBlocks = bb;

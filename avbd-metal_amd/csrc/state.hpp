@@ -69,6 +69,7 @@ struct DeviceState {
   uint32_t *nbrStart, *nbrCount, *nbrList;  // sorted topological 1-ring of every body
   uint32_t* particleIdx;      // surface particles, ascending
   uint32_t* selfCollision;    // per body: its surface self-collides
+  // element grid: counts / starts / cursors per (bucket, triangles | edges), interleaved
   uint32_t *elemCellCount, *elemCellStart, *elemCellCursor, *elemCells;
   F4* elemBox;                // per element (triangles, then edges): padded box lo, hi
   F4* datAnchor;              // Planar-DAT anchor pose of the color passes (accepted predictor pose)
