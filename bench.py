@@ -350,12 +350,15 @@ def main():
     if args.workload == "soft_block" and args.scale_cells > 0 and args.scale_cells != args.cells:
         from avbd_amd import demos as _demos
         big = _demos.soft_block_scene(cells=args.scale_cells)
+        t_create = time.perf_counter()
         gb = GPUSolver(big, lanesPerBody=args.lanes, sweepMode=args.sweep)
+        create_ms = (time.perf_counter() - t_create) * 1e3
         gb.step(3)
         gb.synchronize()
         roof = primal_roofline(gb, f"primal_soft_block_{args.scale_cells + 1}_pmc.json")
         roof["workload"] = (f"soft block {args.scale_cells}^3 cells ({big.numDynamic} vertices, {len(big.tets)} tets), "
                             f"the same kernel at the 1M-tet size")
+        roof["create_ms"] = round(create_ms, 1)   # scene packing + upload precompute + coloring + autotune
         out["roofline_at_scale"] = roof
         gb.close()
     if args.workload == "soft_block" and not args.no_learned:
