@@ -828,7 +828,7 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
   avbd_solver* s = new avbd_solver();
   if (opts) s->opts = *opts;
   else avbd_options_default(&s->opts);
-  if (s->opts.max_pairs_per_body <= 0) s->opts.max_pairs_per_body = 16;
+  if (s->opts.max_pairs_per_body < 0) s->opts.max_pairs_per_body = 16;  // 0 is a valid request (4096-pair floor)
   s->settings = d->settings;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
@@ -1474,6 +1474,10 @@ avbd_status static_coloring(avbd_solver* s) {
       if (hipStreamSynchronize(st) != hipSuccess) return AVBD_ERR_HIP;
     }
   }
+  // the loop can also end on its round bound between two checks: the last
+  // passes must finish before the colors are read (the stream does not
+  // synchronize with the null stream)
+  if (hipStreamSynchronize(st) != hipSuccess) return AVBD_ERR_HIP;
   std::vector<uint32_t> colors(s->nb);
   if (hipMemcpy(colors.data(), in, 4 * s->nb, hipMemcpyDeviceToHost) != hipSuccess) return AVBD_ERR_HIP;
   if (in != S.colors && hipMemcpy(S.colors, in, 4 * s->nb, hipMemcpyDeviceToDevice) != hipSuccess)

@@ -182,7 +182,7 @@ typedef struct avbd_scene_desc {
 
 /* Solver creation options (GPUSolver.init maxPairsPerBody, GPUSolver.swift:331) */
 typedef struct avbd_options {
-    int32_t max_pairs_per_body;        /* 16 */
+    int32_t max_pairs_per_body;        /* 16; maxPairs = max(4096, colliders * this), negative -> 16 */
     int32_t lanes_per_body;            /* 0 = autotune; else 4, 8, 16 or 32 lanes per body in the primal sweep */
     int32_t sweep_mode;                /* 0 = autotune, 1 = one launch per color, 2 = persistent cooperative sweep */
     int32_t use_graph;                 /* 1 = capture each step in a hipGraph (default), 0 = eager launches */

@@ -2778,8 +2778,8 @@ int oracle_create(const avbd_scene_desc* scene, int precision, int maxPairsPerBo
   oracle_solver* o = new oracle_solver();
   o->precision = precision == 64 ? 64 : 32;
   int rc;
-  if (o->precision == 64) { o->d = new Oracle<double>(); rc = o->d->init(scene, maxPairsPerBody > 0 ? maxPairsPerBody : 16); }
-  else { o->f = new Oracle<float>(); rc = o->f->init(scene, maxPairsPerBody > 0 ? maxPairsPerBody : 16); }
+  if (o->precision == 64) { o->d = new Oracle<double>(); rc = o->d->init(scene, maxPairsPerBody >= 0 ? maxPairsPerBody : 16); }
+  else { o->f = new Oracle<float>(); rc = o->f->init(scene, maxPairsPerBody >= 0 ? maxPairsPerBody : 16); }
   if (rc) { oracle_destroy(o); *out = nullptr; return rc; }
   *out = o;
   return 0;

@@ -11,7 +11,7 @@ import pytest
 import oracle
 from oracle import mlp as omlp
 from avbd_amd import AVBDError, GPUSolver, RuntimeFailure, demos
-from avbd_amd.scene import SimSettings
+from avbd_amd.scene import PhysicsScene, SceneJoint, SimSettings
 
 from conftest import flush_print
 
@@ -466,3 +466,47 @@ def test_motor_setters_states_and_preconditions():
     h.setBodyPoses([4], [[2.0, 2.0, 1.0]], [[0, 0, 0, 1]])
     p, _, v, _ = h.bodyStates()
     assert np.allclose(p[4], [2, 2, 1]) and np.all(v[4] == 0)
+
+
+def _rigid_capacity_scene(shared_statics):
+    # rigidCapacityScene (GPUSolverTests.swift:251-270)
+    s = PhysicsScene("rigid-capacity", SimSettings(gravity=0.0, iterations=1))
+    for _ in range(shared_statics):
+        s.addBody((0.1, 0.1, 0.1), 0.0, 0.0, (0, 0, 0), shape=1)
+    for g in range(1, 65):
+        s.addBody((0.1, 0.1, 0.1), 1.0, 0.0, (0, 0, 0), shape=1, collisionGroup=g)
+    return s
+
+
+def test_reference_capacity_and_coloring_failures():
+    _case("exact pair capacity 4096 valid / 4160 poisons; unresolved coloring; static palette exhaustion")
+    g = GPUSolver(_rigid_capacity_scene(64), maxPairsPerBody=0)
+    g.step()
+    g.synchronize()
+    st = g.stats()
+    assert st.pair_candidates == 4096 and st.num_pairs == 4096 and g.runtimeFailure is None
+    g = GPUSolver(_rigid_capacity_scene(65), maxPairsPerBody=0)
+    g.step()
+    with pytest.raises(RuntimeFailure) as e:
+        g.synchronize()
+    assert (e.value.kind, e.value.frame, e.value.required, e.value.capacity) == ("rigidPairCapacity", 1, 4160, 4096)
+    assert g.stats().pair_candidates == 4160 and g.stats().num_pairs == 4096
+    # GPUSolverTests.testUnresolvedDynamicColoringPoisonsTheSolve (:305-326)
+    s = PhysicsScene("color-capacity", SimSettings(gravity=0.0, iterations=1))
+    for _ in range(65):
+        s.addBody((0.1, 0.1, 0.1), 1.0, 0.0, (0, 0, 0), shape=1)
+    g = GPUSolver(s)
+    g.step()
+    with pytest.raises(RuntimeFailure) as e:
+        g.synchronize()
+    assert e.value.kind == "unresolvedColoring" and e.value.frame == 1 and e.value.required > 0
+    # GPUSolverTests.testStaticColorPaletteExhaustionFailsDuringInitialization (:342-364)
+    s = PhysicsScene("static-color-capacity", SimSettings(gravity=0.0))
+    ps = [s.addParticle(0.01, 0.01, (float(i), 0.0, 1.0), friction=0.0) for i in range(65)]
+    s.addTri((ps[0], ps[1], ps[2]))
+    for a in range(65):
+        for b in range(a + 1, 65):
+            s.addJoint(SceneJoint(ps[a], ps[b], (0, 0, 0), (0, 0, 0), stiffnessLin=1.0))
+    with pytest.raises(RuntimeFailure) as e:
+        GPUSolver(s)
+    assert (e.value.kind, e.value.required, e.value.capacity) == ("staticColorCapacity", 65, 64)

@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from avbd_amd import demos
-from avbd_amd.scene import PhysicsScene, SimSettings
+from avbd_amd.scene import PhysicsScene, SceneJoint, SimSettings
 
 
 # -- SplitMix64 (Random.swift:3-21): published reference outputs for seed 0 --
@@ -282,3 +282,36 @@ def test_threaded_sweep_is_identical_to_serial():
     b.step(4)
     for x, y in zip(a.bodyStates(), b.bodyStates()):
         assert np.array_equal(x, y)
+
+
+def _rigid_capacity_scene(shared_statics):
+    # rigidCapacityScene (GPUSolverTests.swift:251-270): group-0 statics shared
+    # by 64 isolated replicas, demand exactly 64 * shared_statics
+    s = PhysicsScene("rigid-capacity", SimSettings(gravity=0.0, iterations=1))
+    for _ in range(shared_statics):
+        s.addBody((0.1, 0.1, 0.1), 0.0, 0.0, (0, 0, 0), shape=1)
+    for g in range(1, 65):
+        s.addBody((0.1, 0.1, 0.1), 1.0, 0.0, (0, 0, 0), shape=1, collisionGroup=g)
+    return s
+
+
+@pytest.mark.parametrize("statics,demand", [(64, 4096), (65, 4160)])
+def test_exact_rigid_pair_capacity(statics, demand):
+    # GPUSolverTests.testExactRigidPairCapacityIsValid / testRigidPairOverflowPoisons... (:272-303)
+    o = oracle.OracleSolver(_rigid_capacity_scene(statics), maxPairsPerBody=0)
+    o.step()
+    st = o.stats()
+    assert st["pair_candidates"] == demand
+    assert st["num_pairs"] == min(demand, 4096)
+
+
+def test_static_palette_exhaustion_fails_at_create():
+    # GPUSolverTests.testStaticColorPaletteExhaustionFailsDuringInitialization (:342-364)
+    s = PhysicsScene("static-color-capacity", SimSettings(gravity=0.0))
+    ps = [s.addParticle(0.01, 0.01, (float(i), 0.0, 1.0), friction=0.0) for i in range(65)]
+    s.addTri((ps[0], ps[1], ps[2]))
+    for a in range(65):
+        for b in range(a + 1, 65):
+            s.addJoint(SceneJoint(ps[a], ps[b], (0, 0, 0), (0, 0, 0), stiffnessLin=1.0))
+    with pytest.raises(RuntimeError):
+        oracle.OracleSolver(s)
