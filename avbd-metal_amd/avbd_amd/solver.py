@@ -144,6 +144,14 @@ class GPUSolver:
         self._check(self._lib.avbd_get_body_mass(self._h, self.bodyCount, _fp(m), _fp(inertia)), "bodyMass")
         return m if i is None else m[i]
 
+    def bodyDiagonalInertia(self, i=None):
+        """bodyDiagonalInertia (GPUSolver.swift:4461): principal moments."""
+        m = np.zeros(self.bodyCount, np.float32)
+        inertia = np.zeros((self.bodyCount, 3), np.float32)
+        self._check(self._lib.avbd_get_body_mass(self._h, self.bodyCount, _fp(m), _fp(inertia)),
+                    "bodyDiagonalInertia")
+        return inertia if i is None else inertia[i]
+
     def setBodyStates(self, idx, pos, rot, vel, angvel):
         idx = np.ascontiguousarray(idx, dtype=np.uint32)
         pos = np.ascontiguousarray(pos, dtype=np.float32)

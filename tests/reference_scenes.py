@@ -1,0 +1,119 @@
+"""Small scenes of the reference's GPU solver tests
+(Tests/PhysicsAVBDTests/GPUSolverTests.swift), restated with their
+thresholds; shared by the oracle (CPU) and backend (GPU) tests."""
+import math
+
+from avbd_amd.scene import SPHERE, PhysicsScene, SimSettings
+
+EXP_M2 = math.exp(-2.0)
+
+
+def gravity_scale():
+    """testPerBodyGravityScaleMatchesCPUAndGPU (:7-30)."""
+    s = PhysicsScene("gravity-scale", SimSettings(dt=1.0 / 100.0, iterations=4))
+    s.addBody((0.1, 0.1, 0.1), 1000.0, 0.0, (0, 0, 2), shape=SPHERE, gravityScale=0.0)
+    s.addBody((0.1, 0.1, 0.1), 1000.0, 0.0, (2, 0, 2), shape=SPHERE, gravityScale=1.0)
+    return s
+
+
+def linear_damping():
+    """testRigidLinearDampingMatchesCPUAndGPU (:32-53): v = exp(-2) after 1 s."""
+    s = PhysicsScene("rigid-damping", SimSettings(gravity=0.0, dt=1.0 / 120.0, rigidLinearDamping=2.0))
+    s.addBody((0.2, 0.2, 0.2), 1000.0, 0.0, (0, 0, 0), velocity=(1, 0, 0), shape=SPHERE)
+    return s
+
+
+def angular_damping():
+    """testRigidAngularDampingMatchesCPUAndGPU (:55-84): w = exp(-2) after 1 s."""
+    s = PhysicsScene("rigid-angular-damping", SimSettings(gravity=0.0, dt=1.0 / 120.0, rigidAngularDamping=2.0))
+    s.addBody((0.3, 0.2, 0.1), 1000.0, 0.0, (0, 0, 0))
+    return s
+
+
+def particle_undamped():
+    """testRigidDampingDoesNotDampParticles (:86-106)."""
+    s = PhysicsScene("rigid-damping-particle-isolation",
+                     SimSettings(gravity=0.0, dt=1.0 / 120.0, rigidLinearDamping=2.0, rigidAngularDamping=2.0))
+    s.addParticle(0.04, 0.01, (0, 0, 0), friction=0.0, velocity=(1, 0, 0))
+    return s
+
+
+def disconnected_soft_mesh():
+    """testDisconnectedSoftMeshUsesStructuralContactCapacity (:386-403): maxSoft 2600."""
+    s = PhysicsScene("disconnected-soft-capacity")
+    for t in range(50):
+        ids = [s.addParticle(0.01, 0.01, (2.0 * t, float(c), 1.0), friction=0.0) for c in range(3)]
+        s.addTri(tuple(ids))
+    return s
+
+
+def explicit_inertia():
+    """testExplicitImportedInertiaOverridesCollisionPrimitiveDensity (:486-506)."""
+    s = PhysicsScene("explicit-inertia")
+    s.addBody((0.3, 0.2, 0.1), 0.0, 0.5, (0, 0, 10), mass=5.39, diagonalInertia=(0.0490211, 0.0445821, 0.00824619))
+    return s
+
+
+def offset_compound():
+    """testOffsetCompoundCollidersSupportOneRigidBody (:508-535): COM rests near 0.90 m."""
+    s = PhysicsScene("compound-collider-support", SimSettings(dt=1.0 / 120.0, iterations=20))
+    s.addBody((20, 20, 0.1), 0.0, 1.0, (0, 0, -0.05))
+    b = s.addBody((0.4, 0.4, 0.4), 0.0, 0.9, (0, 0, 1.5), mass=2.0, diagonalInertia=(0.25, 0.25, 0.25),
+                  collisionEnabled=False)
+    s.addCollider(b, (0.30, 0.30, 0.20), localPosition=(-0.35, 0, -0.80))
+    s.addCollider(b, (0.30, 0.30, 0.20), localPosition=(0.35, 0, -0.80))
+    return s, b
+
+
+def soft_free_fall():
+    """testSoftParticlesFreeFallLikeRigidBodiesWithDamping (:623-656)."""
+    s = PhysicsScene("soft-rigid-freefall", SimSettings(particleDamping=20.0))
+    rigid = s.addBody((0.4, 0.4, 0.4), 1.0, 0.5, (-2, 0, 20))
+    particle = s.addParticle(0.08, 0.02, (0, 0, 20))
+    tet = [s.addParticle(0.05, 0.02, p) for p in ((2, 0, 19.75), (2.4, 0, 19.75), (2, 0.4, 19.75), (2, 0, 20.75))]
+    s.addTet(tuple(tet), 2000.0, 20000.0)
+    return s, rigid, particle, tet
+
+
+def free_fall_z(steps, dt=1.0 / 60.0, g=-10.0, z0=20.0):
+    v, z = 0.0, z0
+    for _ in range(steps):
+        v += g * dt
+        z += v * dt
+    return z
+
+
+def _floor(s):
+    s.addBody((200, 200, 2), 0.0, 0.7, (0, 0, -1))
+
+
+def fast_box_floor():
+    """testFastBoxDoesNotStartBelowFloor / ...DoesNotAddSpuriousSpin (:666-693)."""
+    s = PhysicsScene("fast-floor-box")
+    _floor(s)
+    b = s.addBody((1, 1, 1), 1.0, 0.5, (0, 0, 0.72), velocity=(0, 0, -18))
+    return s, b
+
+
+def fast_box_wall():
+    """testFastBoxDoesNotTunnelThroughWall (:695-710)."""
+    s = PhysicsScene("fast-wall-box", SimSettings(gravity=0.0))
+    s.addBody((0.2, 20, 20), 0.0, 0.7, (0, 0, 0))
+    b = s.addBody((1, 1, 1), 1.0, 0.5, (-0.72, 0, 0), velocity=(18, 0, 0))
+    return s, b
+
+
+def fast_opposing_boxes():
+    """testFastOpposingDynamicBoxesDoNotTunnelThroughEachOther (:712-739)."""
+    s = PhysicsScene("fast-opposing-boxes", SimSettings(gravity=0.0, iterations=20))
+    left = s.addBody((1, 1, 1), 1.0, 0.0, (-0.6, 0, 0), velocity=(42, 0, 0))
+    right = s.addBody((1, 1, 1), 1.0, 0.0, (0.6, 0, 0), velocity=(-42, 0, 0))
+    return s, left, right
+
+
+def fast_particle_floor():
+    """testFastParticleDoesNotStartBelowFloor (:741-755)."""
+    s = PhysicsScene("fast-floor-particle")
+    _floor(s)
+    p = s.addParticle(0.05, 0.02, (0, 0, 0.16), friction=0.5, velocity=(0, 0, -8))
+    return s, p
