@@ -117,3 +117,66 @@ def fast_particle_floor():
     _floor(s)
     p = s.addParticle(0.05, 0.02, (0, 0, 0.16), friction=0.5, velocity=(0, 0, -8))
     return s, p
+
+
+def _quat(angle, axis):
+    n = math.sqrt(sum(a * a for a in axis))
+    s = math.sin(angle / 2)
+    return (axis[0] / n * s, axis[1] / n * s, axis[2] / n * s, math.cos(angle / 2))
+
+
+def slide(mu, vx=5.0):
+    """testFrictionStopsSlidingBox / testFrictionMagnitudeMatchesCoulomb
+    (ConvergenceTests.swift:51-97)."""
+    s = PhysicsScene("slide")
+    s.addBody((200, 200, 2), 0.0, mu, (0, 0, -1))
+    s.addBody((1, 1, 1), 1.0, mu, (0, 0, 0.5), velocity=(vx, 0, 0))
+    return s
+
+
+def welded_pusht():
+    """testWeldedPushTCompoundDissipatesSlidingEnergy (ConvergenceTests.swift:99-145)."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("welded-pusht-friction", SimSettings(iterations=16, betaLin=20000.0, lambdaMax=600.0))
+    s.addBody((20, 20, 2), 0.0, 1.3, (0, 0, -1))
+    bar = s.addBody((1, 0.25, 0.18), 3.0, 1.1, (0, 0.125, 0.09), collisionGroup=1)
+    stem = s.addBody((0.25, 0.65, 0.18), 3.0, 1.1, (0, -0.325, 0.09), collisionGroup=1)
+    pb, ps = s.bodies[bar]["position"], s.bodies[stem]["position"]
+    mid = tuple(0.5 * (a + b) for a, b in zip(pb, ps))
+    s.addJoint(SceneJoint(bar, stem, tuple(m - a for m, a in zip(mid, pb)), tuple(m - b for m, b in zip(mid, ps)),
+                          stiffnessLin=math.inf, stiffnessAng=math.inf))
+    s.addJoint(SceneJoint(bar, stem, (0, 0, 0), (0, 0, 0), stiffnessLin=0.0, stiffnessAng=0.0))
+    return s, bar, stem
+
+
+def sphere_on_ground():
+    """testSphereRestsOnGround (SphereTests.swift:7-17)."""
+    s = PhysicsScene("s")
+    s.addBody((100, 100, 2), 0.0, 0.5, (0, 0, -1))
+    s.addSphere(1.0, 1.0, 0.5, (0, 0, 2))
+    return s
+
+
+def sphere_incline(theta=0.25):
+    """testSphereRollsWithoutSlip (SphereTests.swift:19-39)."""
+    s = PhysicsScene("r")
+    s.addBody((40, 10, 1), 0.0, 0.6, (0, 0, 0), rotation=_quat(theta, (0, 1, 0)))
+    s.addSphere(1.0, 1.0, 0.6, (0, 0, 1.2))
+    return s
+
+
+def sliding_sphere():
+    """testSlidingSphereStaysOnGround (SphereTests.swift:41-49)."""
+    s = PhysicsScene("sl")
+    s.addBody((200, 200, 2), 0.0, 0.3, (0, 0, -1))
+    s.addSphere(0.7, 1.0, 0.3, (0, 0, 1), velocity=(8, 0, 0))
+    return s
+
+
+def sphere_sphere():
+    """testSphereSphereCollision (SphereTests.swift:51-65)."""
+    s = PhysicsScene("ss")
+    s.addBody((100, 100, 2), 0.0, 0.5, (0, 0, -1))
+    s.addSphere(1.0, 1.0, 0.5, (0, 0, 0.5))
+    s.addSphere(1.0, 1.0, 0.5, (0.1, 0, 1.6))
+    return s

@@ -4,7 +4,11 @@ backend through the C ABI (GPU): per-body gravity scale, rigid linear /
 angular damping against exp(-2), particles exempt from rigid damping, soft
 particles and tets free-falling with particle damping, offset compound
 colliders, fast boxes and particles against floors, walls and each other,
-the structural soft-contact capacity, explicit inertia."""
+the structural soft-contact capacity, explicit inertia; and from
+ConvergenceTests.swift / SphereTests.swift the friction stop distance and
+Coulomb deceleration, a welded compound dissipating its sliding energy,
+sphere rest, rolling without slip (v = w r, 5/7 g sin theta), a sliding
+sphere staying on the floor and two spheres not interpenetrating."""
 import numpy as np
 import pytest
 
@@ -127,3 +131,67 @@ def test_explicit_inertia_overrides_density():
     g = GPUSolver(R.explicit_inertia())
     assert abs(g.bodyMass(0) - 5.39) < 1e-6
     np.testing.assert_allclose(g.bodyDiagonalInertia(0), [0.0490211, 0.0445821, 0.00824619], atol=1e-7)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_friction_stop_distance_and_coulomb_deceleration(kind):
+    # ConvergenceTests.swift:51-97
+    stops = []
+    for mu in (0.2, 0.8):
+        x = make(kind, R.slide(mu))
+        x.step(300)
+        p, _, v, _ = states(x)
+        assert np.linalg.norm(v[1]) < 0.1
+        want = 25.0 / (2 * mu * 10)
+        assert abs(p[1, 0] - want) < 0.3 * want
+        stops.append(p[1, 0])
+    assert stops[0] > 2 * stops[1]
+    x = make(kind, R.slide(0.3, vx=6.0))
+    x.step(5)
+    v0 = states(x)[2][1, 0]
+    x.step(30)
+    v1 = states(x)[2][1, 0]
+    decel = (v0 - v1) / (30 / 60.0)
+    assert abs(decel - 3.0) < 0.25 * 3.0
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_welded_compound_dissipates_sliding_energy(kind):
+    # ConvergenceTests.swift:99-145
+    s, bar, stem = R.welded_pusht()
+    x = make(kind, s)
+    p, r, _, _ = states(x)
+    idx = np.array([bar, stem], np.uint32)
+    x.setBodyStates(idx, p[idx], r[idx], np.array([[0.8, 0, 0]] * 2, np.float32),
+                    np.array([[0, 0, 0.46]] * 2, np.float32))
+    x.step(64)
+    _, _, v, w = states(x)
+    assert np.linalg.norm(0.5 * (v[bar] + v[stem])) < 0.1
+    assert np.linalg.norm(0.5 * (w[bar] + w[stem])) < 0.1
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_sphere_contracts(kind):
+    # SphereTests.swift:7-65
+    x = make(kind, R.sphere_on_ground())
+    x.step(240)
+    p, _, v, _ = states(x)
+    assert abs(p[1, 2] - 0.5) < 0.04 and np.linalg.norm(v[1]) < 0.05
+    x = make(kind, R.sphere_incline())
+    x.step(20)
+    v0 = np.linalg.norm(states(x)[2][1])
+    x.step(60)
+    _, _, v, w = states(x)
+    v1, w1 = np.linalg.norm(v[1]), np.linalg.norm(w[1])
+    assert abs(v1 / max(w1, 0.01) - 0.5) < 0.05                      # v = w r
+    accel = (v1 - v0) / (60 / 60.0)
+    analytic = 5.0 / 7.0 * 10 * np.sin(0.25)
+    assert abs(accel - analytic) < 0.2 * analytic
+    x = make(kind, R.sliding_sphere())
+    x.step(300)
+    assert abs(x.positions()[1, 2] - 0.35) < 0.05
+    x = make(kind, R.sphere_sphere())
+    x.step(300)
+    p = x.positions()
+    assert np.linalg.norm(p[1] - p[2]) > 0.93
+    assert abs(p[1, 2] - 0.5) < 0.05 and abs(p[2, 2] - 0.5) < 0.05
