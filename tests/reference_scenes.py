@@ -180,3 +180,117 @@ def sphere_sphere():
     s.addSphere(1.0, 1.0, 0.5, (0, 0, 0.5))
     s.addSphere(1.0, 1.0, 0.5, (0.1, 0, 1.6))
     return s
+
+
+# -- Demos.swift scenes the ConvergenceTests / RigidCorrectnessTests use ----------
+def _ground(s, friction=0.7):
+    s.addBody((200, 200, 2), 0.0, friction, (0, 0, -1))
+
+
+def _rot_act(q, v):
+    x, y, z, w = q
+    u = (x, y, z)
+    t = [2 * (u[1] * v[2] - u[2] * v[1]), 2 * (u[2] * v[0] - u[0] * v[2]), 2 * (u[0] * v[1] - u[1] * v[0])]
+    return (v[0] + w * t[0] + (u[1] * t[2] - u[2] * t[1]), v[1] + w * t[1] + (u[2] * t[0] - u[0] * t[2]),
+            v[2] + w * t[2] + (u[0] * t[1] - u[1] * t[0]))
+
+
+def slopefriction(count=8):
+    """Demos.slopefriction (Demos.swift:245-276): a 30 degree mu = 1 ramp with
+    boxes whose pair friction sqrt(mu) sweeps 0.30..0.78 around tan 30."""
+    from avbd_amd.scene import PhysicsScene as PS
+    s = PS("slopefriction", SimSettings(iterations=10))
+    _ground(s, 0.8)
+    tilt = math.pi / 6
+    rot = _quat(tilt, (0, 1, 0))
+    slope_t, slope_len = 0.4, 32.0
+    cz = slope_len / 2 * math.sin(tilt) + 0.4
+    s.addBody((slope_len, count * 1.3 + 1.5, slope_t), 0.0, 1.0, (0, 0, cz), rotation=rot)
+    n = max(2, count)
+    side = 0.8
+    lift = _rot_act(rot, (0, 0, slope_t / 2 + side / 2 + 0.01))
+    for k in range(n):
+        pair = 0.30 + 0.48 * k / (n - 1)
+        y = (k - (n - 1) / 2) * 1.3
+        off = _rot_act(rot, (-slope_len * 0.32, y, 0))
+        s.addBody((side, side, side), 1.0, pair * pair,
+                  (off[0] + lift[0], off[1] + lift[1], cz + off[2] + lift[2]), rotation=rot)
+    return s
+
+
+def ratiostack(levels=5, growth=1.45):
+    """Demos.ratiostack (Demos.swift:278-300)."""
+    s = PhysicsScene("ratiostack", SimSettings(iterations=25, betaLin=40000.0))
+    _ground(s, 0.8)
+    side, z = 0.5, 0.0
+    for _ in range(max(2, levels)):
+        z += side / 2
+        s.addBody((side * 0.98,) * 3, 1.0, 0.8, (0, 0, z))
+        z += side / 2
+        side *= growth
+    return s
+
+
+def pendulum(links, mass_ratio):
+    """Demos.pendulum (Demos.swift:314-341): a chain of links ending in a heavy bob."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("pendulum")
+    L = 0.5
+    az = links * L + 2.0
+    prev = -1
+    for i in range(links):
+        x = (i + 1) * L
+        idx = s.addBody((L, 0.1, 0.1), 1.0, 0.5, (x - L / 2, 0, az))
+        s.addJoint(SceneJoint(prev, idx, (0, 0, az) if prev < 0 else (L / 2, 0, 0), (-L / 2, 0, 0)))
+        prev = idx
+    bob = 1.0
+    density = mass_ratio * (L * 0.1 * 0.1) / bob ** 3
+    b = s.addBody((bob,) * 3, density, 0.5, (links * L + bob / 2, 0, az))
+    s.addJoint(SceneJoint(prev, b, (L / 2, 0, 0), (-bob / 2, 0, 0)))
+    return s, az
+
+
+def chain(links):
+    """Demos.chain (Demos.swift:343-358)."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("chain")
+    _ground(s)
+    L = 0.6
+    prev = -1
+    for i in range(links):
+        z = 20 - i * L
+        idx = s.addBody((0.15, 0.15, L), 1.0, 0.5, (0, 0, z - L / 2))
+        s.addJoint(SceneJoint(prev, idx, (0, 0, 20) if prev < 0 else (0, 0, -L / 2), (0, 0, L / 2)))
+        prev = idx
+    return s
+
+
+def spring_ratio(blocks=3):
+    """Demos.springRatio (Demos.swift:385-400): springs of 1e6 / 1e2 stiffness."""
+    s = PhysicsScene("spring")
+    z_top = 4 + 2 * blocks
+    prev = s.addBody((1, 1, 0.5), 0.0, 0.5, (0, 0, z_top))
+    for i in range(1, blocks):
+        b = s.addBody((1, 1, 0.5), 1.0, 0.5, (0, 0, z_top - i * 2))
+        s.addSpring(prev, b, (0, 0, 0), (0, 0, 0), 1e6 if i % 2 == 1 else 1e2)
+        prev = b
+    return s, z_top
+
+
+def stack(height):
+    """Demos.stack (Demos.swift:214-222)."""
+    s = PhysicsScene("stack")
+    _ground(s)
+    for i in range(height):
+        s.addBody((1, 1, 1), 1.0, 0.7, (0, 0, 0.5 + i))
+    return s
+
+
+def pyramid(base):
+    """Demos.pyramid (Demos.swift:302-312)."""
+    s = PhysicsScene("pyramid")
+    _ground(s)
+    for j in range(base):
+        for i in range(base - j):
+            s.addBody((0.95, 0.95, 0.95), 1.0, 0.8, (i + j / 2 - base / 2, 0, 0.5 + j))
+    return s

@@ -8,7 +8,10 @@ the structural soft-contact capacity, explicit inertia; and from
 ConvergenceTests.swift / SphereTests.swift the friction stop distance and
 Coulomb deceleration, a welded compound dissipating its sliding energy,
 sphere rest, rolling without slip (v = w r, 5/7 g sin theta), a sliding
-sphere staying on the floor and two spheres not interpenetrating."""
+sphere staying on the floor and two spheres not interpenetrating; the
+static-friction sweep, ratio stack, high-mass-ratio pendulum, chain error
+decay, spring stiffness ratio, low-iteration stack and determinism of
+RigidCorrectnessTests.swift / ConvergenceTests.swift on their demo scenes."""
 import numpy as np
 import pytest
 
@@ -195,3 +198,67 @@ def test_sphere_contracts(kind):
     p = x.positions()
     assert np.linalg.norm(p[1] - p[2]) > 0.93
     assert abs(p[1, 2] - 0.5) < 0.05 and abs(p[2, 2] - 0.5) < 0.05
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_static_friction_sweep_and_ratio_stack(kind):
+    # RigidCorrectnessTests.swift:14-47
+    s = R.slopefriction(8)
+    x = make(kind, s)
+    p0 = x.positions().copy()
+    x.step(400)
+    p = x.positions()
+    for k in (0, 1):
+        assert p[2 + k, 0] - p0[2 + k, 0] > 0.5, k             # slick boxes slide
+    for k in (6, 7):
+        assert np.linalg.norm(p[2 + k] - p0[2 + k]) < 0.08, k   # grippy boxes hold
+    s = R.ratiostack(5)
+    x = make(kind, s)
+    top = len(s.bodies) - 1
+    z0 = float(x.positions()[top, 2])
+    x.step(600)
+    p = x.positions()[top]
+    assert abs(p[2] - z0) < 0.1 * z0 and np.hypot(p[0], p[1]) < 0.25
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_chains_springs_and_low_iteration_stacks(kind):
+    # ConvergenceTests.swift:13-49, 147-162, 182-193
+    s, az = R.pendulum(20, 1000.0)
+    s.settings.iterations = 20
+    x = make(kind, s)
+    x.step(240)
+    bob = x.positions()[len(s.bodies) - 1]
+    assert np.linalg.norm(bob - np.array([0, 0, 10])) < (20 * 0.5 + 0.5) * 1.05
+    if kind == "gpu":
+        assert x.maxConstraintError() < 0.15
+    s = R.chain(20)
+    s.settings.iterations = 5
+    x = make(kind, s)
+    x.step(240)
+    if kind == "gpu":
+        assert x.maxConstraintError() < 0.02
+    s, zt = R.spring_ratio()
+    s.settings.iterations = 10
+    x = make(kind, s)
+    x.step(600)
+    p, _, v, _ = states(x)
+    assert abs(np.linalg.norm(p[1] - np.array([0, 0, 10])) - 2.0) < 0.005
+    assert np.linalg.norm(v[1]) < 0.05
+    s = R.stack(8)
+    s.settings.iterations = 2
+    x = make(kind, s)
+    x.step(600)
+    p = x.positions()
+    for i in range(8):
+        assert np.linalg.norm(p[1 + i]) < 50 and p[1 + i, 2] > -1
+
+
+@pytest.mark.gpu
+def test_pyramid_bitwise_determinism():
+    # ConvergenceTests.testDeterminismGPU (:221-240)
+    from avbd_amd import GPUSolver
+    a, b = GPUSolver(R.pyramid(5)), GPUSolver(R.pyramid(5))
+    a.step(60)
+    b.step(60)
+    assert np.array_equal(a.positions(), b.positions())
