@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libavbd_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_COLORS = 64
 
 # avbd_status
@@ -103,7 +103,8 @@ class ClothInfo(C.Structure):
                 ("num_membranes", C.c_uint32), ("num_bends", C.c_uint32), ("elem_hash_size", C.c_uint32),
                 ("max_soft", C.c_uint32), ("soft_map_capacity", C.c_uint32), ("max_dat_pairs", C.c_uint32),
                 ("elem_cell_size", C.c_float), ("elem_margin", C.c_float), ("dat_query_radius", C.c_float),
-                ("surface_contact_cell_size", C.c_float)]
+                ("surface_contact_cell_size", C.c_float), ("num_contact_edges", C.c_uint32),
+                ("pair_stream", C.c_uint32), ("dat_on", C.c_uint32)]
 
 
 # name -> (restype, argtypes); the complete exported surface of avbd.h
@@ -135,6 +136,7 @@ SIGNATURES = {
     "avbd_step_to_residual": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.c_int32, I32P, FP]),
     "avbd_get_cloth_info": (C.c_int, [C.c_void_p, C.POINTER(ClothInfo), U32P, U32P]),
     "avbd_get_dat_pairs": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
+    "avbd_get_collision_tris": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
     "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),

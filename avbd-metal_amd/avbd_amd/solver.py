@@ -286,6 +286,14 @@ class GPUSolver:
         out["particles"] = parts[:info.num_particles]
         return out
 
+    def collisionTris(self):
+        """Collision triangles: the authored ones, then the tet boundary faces (GPUSolver.swift:872-897)."""
+        n = C.c_uint32(0)
+        self._check(self._lib.avbd_get_collision_tris(self._h, 0, C.byref(n), None), "collision tris")
+        t = np.zeros((max(1, n.value), 3), np.uint32)
+        self._check(self._lib.avbd_get_collision_tris(self._h, n.value, C.byref(n), _u32p(t)), "collision tris")
+        return t[:n.value]
+
     def datPairs(self):
         """Planar-DAT pair stream of the last frame: rows (kind 0 V-T / 1 E-E, owner, other)."""
         n = C.c_uint32()

@@ -950,6 +950,14 @@ __device__ bool dat_contact(const DeviceState& S, const StepParams& P, U4 pair, 
   uint32_t ea = pair.y, eb = pair.z;
   if (ea >= P.numEdges || eb >= P.numEdges) return false;
   U2 ai = S.edges[ea], bi = S.edges[eb];
+  // E-E contacts between authored edges, or within a self-colliding
+  // surface; other pairs (tet-boundary edges) only bound the motion
+  // (27_planar_dat.metal:196-201)
+  if (ea >= P.numContactEdges || eb >= P.numContactEdges) {
+    const uint32_t g = S.clothGroup[ai.x];
+    const bool same = g != 0u && S.clothGroup[ai.y] == g && S.clothGroup[bi.x] == g && S.clothGroup[bi.y] == g;
+    if (!(same && S.selfCollision[ai.x] != 0u)) return false;
+  }
   V3 a0 = P3(S, ai.x), a1 = P3(S, ai.y), b0 = P3(S, bi.x), b1 = P3(S, bi.y);
   float s, t;
   closest_seg_seg(a0, a1, b0, b1, s, t);
@@ -1049,9 +1057,9 @@ __global__ void __launch_bounds__(256) k_dat_contacts(DeviceState S, StepParams 
 
 // total soft demand; clamp the working count (25_cloth.metal:1149-1163)
 __global__ void k_soft_finalize(DeviceState S, StepParams P) {
-  uint32_t total = S.datStart[P.numParticles + P.numEdges];
+  uint32_t total = P.pairStream ? S.datStart[P.numParticles + P.numEdges] : 0u;
   uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
-  uint32_t demand = S.counters[CTR_SOFT_RT] + S.softStart[n];
+  uint32_t demand = S.counters[CTR_SOFT_RT] + (P.pairStream ? S.softStart[n] : 0u);
   S.counters[CTR_SOFT_CANDIDATES] = demand;
   S.counters[CTR_SOFT] = demand < P.maxSoft ? demand : P.maxSoft;
 }
@@ -1115,7 +1123,7 @@ void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P) {
 }
 
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  if (P.numTris == 0) return;
+  if (!P.pairStream) return;
   uint32_t nElem = P.numTris + P.numEdges;
   uint32_t eb = (nElem + 255) / 256;
   const uint32_t slots = 2u * P.elemHashSize;
@@ -1133,20 +1141,24 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   hipLaunchKernelGGL(k_dat_finalize, dim3(1), dim3(1), 0, st, S, P);
 }
 
+// the pair stream's contacts after the rigid-feature ones (without the
+// stream, the rigid-feature contacts alone), then the persistence map
 void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
   if (P.numTris == 0) return;
-  uint32_t blocks = cap_blocks(P.maxDatPairs, gridCap);
-  hipLaunchKernelGGL(k_dat_contacts<false>, dim3(blocks), dim3(256), 0, st, S, P);
-  launch_exclusive_scan_dev(st, S.softFlag, S.softStart, P.maxDatPairs + 1, S.datStart + P.numParticles + P.numEdges,
-                            P.maxDatPairs, S.scanScratch);
-  hipLaunchKernelGGL(k_dat_contacts<true>, dim3(blocks), dim3(256), 0, st, S, P);
+  if (P.pairStream) {
+    uint32_t blocks = cap_blocks(P.maxDatPairs, gridCap);
+    hipLaunchKernelGGL(k_dat_contacts<false>, dim3(blocks), dim3(256), 0, st, S, P);
+    launch_exclusive_scan_dev(st, S.softFlag, S.softStart, P.maxDatPairs + 1,
+                              S.datStart + P.numParticles + P.numEdges, P.maxDatPairs, S.scanScratch);
+    hipLaunchKernelGGL(k_dat_contacts<true>, dim3(blocks), dim3(256), 0, st, S, P);
+  }
   hipLaunchKernelGGL(k_soft_finalize, dim3(1), dim3(1), 0, st, S, P);
   hipLaunchKernelGGL(k_softmap_clear, dim3(cap_blocks(P.softMapCapacity, gridCap)), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_softmap_insert, dim3(cap_blocks(P.maxSoft, gridCap)), dim3(256), 0, st, S, P);
 }
 
 void launch_dat_predictor_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  if (P.numTris == 0) return;
+  if (!P.datOn) return;
   hipLaunchKernelGGL(k_dat_reduce, dim3(cap_blocks(P.maxDatPairs, gridCap)), dim3(256), 0, st, S, P,
                      (const F4*)S.initLin, kDatFullPass);
   if (P.numParticles)
@@ -1155,13 +1167,13 @@ void launch_dat_predictor_pass(hipStream_t st, const DeviceState& S, const StepP
 }
 
 void launch_dat_reanchor(hipStream_t st, const DeviceState& S, const StepParams& P) {
-  if (P.numTris == 0 || P.numParticles == 0) return;
+  if (!P.datOn || P.numParticles == 0) return;
   hipLaunchKernelGGL(k_dat_reanchor, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
 }
 
 void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t color, uint32_t start,
                            uint32_t count, uint32_t gridCap) {
-  if (P.numTris == 0 || count == 0) return;
+  if (!P.datOn || count == 0) return;
   hipLaunchKernelGGL(k_dat_reduce, dim3(cap_blocks(P.maxDatPairs, gridCap)), dim3(256), 0, st, S, P,
                      (const F4*)S.datAnchor, color);
   hipLaunchKernelGGL(k_dat_apply, dim3((count + 255) / 256), dim3(256), 0, st, S, P, (const F4*)S.datAnchor, 0u,
@@ -1169,7 +1181,7 @@ void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParam
 }
 
 void launch_dat_restore(hipStream_t st, const DeviceState& S, const StepParams& P) {
-  if (P.numTris == 0) return;
+  if (!P.datOn) return;
   hipLaunchKernelGGL(k_dat_restore, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
 }
 

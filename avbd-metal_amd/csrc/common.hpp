@@ -388,6 +388,9 @@ struct StepParams {
   uint32_t staticPalette;   // colors fixed for the scene's lifetime (tets / cloth present)
   // cloth surfaces (Planar-DAT pair stream + OGC element contacts)
   uint32_t numTris, numEdges, numParticles, elemHashSize, maxSoft, softMapCapacity, maxDatPairs;
+  uint32_t numContactEdges;  // edges [0, numContactEdges) carry E-E contacts; later ones bound motion only
+  uint32_t pairStream;       // the V-T / E-E pair stream and its contacts run
+  uint32_t datOn;            // the Planar-DAT limiter passes run
   float elemCellSize, elemMargin, datQueryRadius, surfaceContactCellSize, clothViscosity;
   float fineCellSize;       // > 0: candidate search grid of 2 x the largest hashed radius
 };

@@ -61,6 +61,9 @@ struct avbd_solver {
   int allMode = kModeGeneral;               // common mode of every used color, else general
   // cloth surfaces
   uint32_t nTris = 0, nEdges = 0, nParticles = 0, nMembranes = 0, nBends = 0;
+  uint32_t numContactEdges = 0;  // authored-triangle edges: the E-E contact prefix of the edge list
+  bool pairStream = false;       // V-T / E-E pair stream and its contacts (some pair can be accepted)
+  bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
   uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
   float elemCellSize = 0.0f, elemMargin = 0.0f, datQueryRadius = 0.0f, surfaceContactCellSize = 0.0f;
   float datQueryRadiusDerived = 0.0f;  // rq derived at create (a test hook may replace datQueryRadius)
@@ -196,6 +199,9 @@ StepParams make_params(const avbd_solver* s) {
   P.staticPalette = s->dynamicColoring ? 0u : 1u;
   P.numTris = s->nTris;
   P.numEdges = s->nEdges;
+  P.numContactEdges = s->numContactEdges;
+  P.datOn = s->datOn ? 1u : 0u;
+  P.pairStream = s->pairStream ? 1u : 0u;
   P.numParticles = s->nParticles;
   P.elemHashSize = s->elemHashSize;
   P.maxSoft = s->maxSoft;
@@ -233,7 +239,7 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   uint32_t* zp[5] = {S.counters, S.cellCount, S.cellRigid, S.dAdjCount, S.fineCount};
   uint32_t zn[5] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
   launch_zero_multi(st, zp, zn, 5);
-  if (s->nTris) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
+  if (s->datOn) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
   launch_broadphase(st, S, P);
   stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
@@ -244,7 +250,7 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   // full Planar-DAT pass; the query again at the accepted predictor pose,
   // which feeds the OGC contacts and every per-color pass, anchored there
   if (s->nTris) {
-    launch_dat_pairs(st, S, P, s->gridCap);
+    if (s->datOn) launch_dat_pairs(st, S, P, s->gridCap);
     stage("dat-pairs");
     launch_rt_emit(st, S, P);
     stage("rt-emit");
@@ -252,13 +258,13 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   launch_warmstart(st, S, P);
   stage("warmstart");
   if (s->nTris) {
-    launch_dat_predictor_pass(st, S, P, s->gridCap);
+    if (s->datOn) launch_dat_predictor_pass(st, S, P, s->gridCap);
     stage("dat-predict");
     launch_dat_pairs(st, S, P, s->gridCap);
     stage("dat-accepted-pairs");
     launch_dat_contacts(st, S, P, s->gridCap);
     stage("vt-ee-emit");
-    launch_dat_reanchor(st, S, P);
+    if (s->datOn) launch_dat_reanchor(st, S, P);
     stage("dat-reanchor");
   }
   launch_contact_adjacency(st, S, P, s->gridCap);
@@ -313,7 +319,7 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
       for (uint32_t c = 0; c < s->staticUsedColors; c++) {
         launch_primal_color(st, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes, s->colorMode[c]);
         // Planar-DAT after every color (GPUSolver.swift:4196-4229)
-        if (s->nTris)
+        if (s->datOn)
           launch_dat_color_pass(st, S, P, c, s->staticColorStarts[c], s->staticColorCounts[c], s->gridCap);
       }
     }
@@ -531,33 +537,44 @@ avbd_status validate(const avbd_scene_desc* d) {
   if (d->num_tris > 0 && !d->tri_ids) return AVBD_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < d->num_tris * 3; i++)
     if (d->tri_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
-  // cloth surfaces run the Planar-DAT path, which the reference selects for
-  // authored triangles without tets; tet surfaces with element contacts
-  // (isotropic-DAT trackers) are not in this backend
-  if (d->num_tris > 0 && d->num_tets > 0) return AVBD_ERR_UNSUPPORTED;
   for (uint32_t i = 0; i < d->num_exclusions * 2; i++)
     if (d->exclusion_pairs[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
   return AVBD_OK;
 }
 
-// Boundary faces of the tet mesh (faces used by exactly one tet),
-// GPUSolver.swift:875-897; only their vertex membership matters here.
-std::vector<uint8_t> tet_surface_flags(const avbd_scene_desc* d) {
-  std::map<std::array<uint32_t, 3>, int> faces;
-  static const int fi[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+// Boundary faces of the tet meshes (faces used by exactly one tet), wound
+// outward against the tet's opposite vertex and sorted; they are collision
+// triangles after the authored ones (GPUSolver.swift:384-390, 872-897).
+std::vector<std::array<uint32_t, 3>> tet_boundary_faces(const avbd_scene_desc* d) {
+  struct Face {
+    std::array<uint32_t, 3> tri;
+    uint32_t opp;
+    int count;
+  };
+  std::map<std::array<uint32_t, 3>, Face> faces;
+  static const int fi[4][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 3, 1}, {1, 2, 3, 0}};
   for (uint32_t t = 0; t < d->num_tets; t++) {
     const uint32_t* ids = d->tet_ids + 4 * t;
     for (int f = 0; f < 4; f++) {
-      std::array<uint32_t, 3> k{ids[fi[f][0]], ids[fi[f][1]], ids[fi[f][2]]};
+      std::array<uint32_t, 3> tri{ids[fi[f][0]], ids[fi[f][1]], ids[fi[f][2]]};
+      std::array<uint32_t, 3> k = tri;
       std::sort(k.begin(), k.end());
-      faces[k]++;
+      auto it = faces.find(k);
+      if (it != faces.end()) it->second.count++;
+      else faces.emplace(k, Face{tri, ids[fi[f][3]], 1});
     }
   }
-  std::vector<uint8_t> flag(d->num_bodies, 0);
-  for (auto& kv : faces)
-    if (kv.second == 1)
-      for (uint32_t v : kv.first) flag[v] = 1;
-  return flag;
+  std::vector<std::array<uint32_t, 3>> out;
+  for (auto& kv : faces) {
+    if (kv.second.count != 1) continue;
+    std::array<uint32_t, 3> t = kv.second.tri;
+    V3 pa = rd3(d->body_position, t[0]);
+    V3 n = cross(rd3(d->body_position, t[1]) - pa, rd3(d->body_position, t[2]) - pa);
+    if (dot(n, rd3(d->body_position, kv.second.opp) - pa) > 0.0f) std::swap(t[1], t[2]);
+    out.push_back(t);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -580,6 +597,9 @@ struct ClothHost {
   std::vector<MembraneHost> membranes;
   std::vector<BendHost> bends;
   uint32_t elemHashSize = 64, maxDatPairs = 1, maxSoft = 1, softMapCapacity = 64;
+  uint32_t numContactEdges = 0;  // edges of authored triangles (E-E contacts); tet-boundary edges follow
+  bool pairStream = false;       // some V-T / E-E pair can be accepted: two surfaces, or a self-colliding one
+  bool datOn = false;            // Planar-DAT limiter: a self-colliding (authored) surface exists
   float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
 };
 
@@ -588,16 +608,25 @@ float body_pos_dist(const avbd_scene_desc* d, uint32_t a, uint32_t b) {
   return sqrtf(d3.x * d3.x + d3.y * d3.y + d3.z * d3.z);
 }
 
+// Collision triangles are the authored ones, then the tet boundary faces
+// (GPUSolver.swift:1289-1334): membranes and hinges come from the authored
+// triangles only, E-E contacts from their edges only (the tet-boundary edges
+// follow them in the edge list and feed the Planar-DAT pair stream alone).
 void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
-  const uint32_t nb = d->num_bodies, nr = d->num_tris;
+  const uint32_t nb = d->num_bodies, na = d->num_tris;
   auto isParticle = [&](uint32_t i) { return d->body_is_particle && d->body_is_particle[i]; };
   auto sizeX = [&](uint32_t i) { return d->body_size[3 * i]; };
+  std::vector<std::array<uint32_t, 3>> coll(na);
+  for (uint32_t t = 0; t < na; t++) coll[t] = {d->tri_ids[3 * t], d->tri_ids[3 * t + 1], d->tri_ids[3 * t + 2]};
+  for (const auto& f : tet_boundary_faces(d)) coll.push_back(f);
+  const uint32_t nr = (uint32_t)coll.size();
   c.tris.resize(nr);
   std::set<uint64_t> seen;
   std::vector<uint64_t> keys;  // unique edges in first-appearance order
   float maxElemR = 0.0f, maxPartR = 0.0f;
   for (uint32_t t = 0; t < nr; t++) {
-    uint32_t a = d->tri_ids[3 * t], b = d->tri_ids[3 * t + 1], cc = d->tri_ids[3 * t + 2];
+    if (t == na) c.numContactEdges = (uint32_t)keys.size();
+    uint32_t a = coll[t][0], b = coll[t][1], cc = coll[t][2];
     c.tris[t] = U4{a, b, cc, 0};
     uint32_t e[3][2] = {{a, b}, {b, cc}, {a, cc}};
     for (auto& uv : e) {
@@ -610,6 +639,7 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
     maxPartR = fmaxf(maxPartR, thick);
     maxElemR = fmaxf(maxElemR, fmaxf(length(m - pa), fmaxf(length(m - pb), length(m - pc))) + thick);
   }
+  if (nr == na) c.numContactEdges = (uint32_t)keys.size();
   c.edges.resize(keys.size());
   std::vector<std::vector<uint32_t>> nbr(nb);
   float edgeLenSum = 0.0f;
@@ -631,7 +661,8 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
     c.nbrList.insert(c.nbrList.end(), nbr[i].begin(), nbr[i].end());
   }
   // connected surfaces (union-find over triangles), numbered by first
-  // triangle; authored sheets self-collide (GPUSolver.swift:1476-1530)
+  // triangle; a surface holding an authored triangle self-collides, a tet
+  // boundary does not (GPUSolver.swift:1476-1530)
   std::vector<uint32_t> parent(nb);
   for (uint32_t i = 0; i < nb; i++) parent[i] = i;
   std::function<uint32_t(uint32_t)> find = [&](uint32_t x) {
@@ -648,6 +679,8 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
   }
   c.group.assign(nb, 0);
   c.selfCollision.assign(nb, 0);
+  std::set<uint32_t> selfRoots;
+  for (uint32_t t = 0; t < na; t++) selfRoots.insert(find(c.tris[t].x));
   std::map<uint32_t, uint32_t> gidx;
   std::vector<uint8_t> surface(nb, 0);
   for (const U4& t : c.tris) {
@@ -657,15 +690,23 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
     for (uint32_t v : {t.x, t.y, t.z})
       if (isParticle(v)) {
         c.group[v] = it->second;
-        c.selfCollision[v] = 1;
+        c.selfCollision[v] = selfRoots.count(root) ? 1u : 0u;
         surface[v] = 1;
       }
   }
+  // The pair stream (and its contacts) runs wherever a V-T / E-E pair can
+  // pass the topology filters: two surfaces, or a self-colliding one.  The
+  // Planar-DAT limiter runs where the reference selects it automatically,
+  // scenes with self-colliding (authored) surfaces (GPUSolver.swift:93-99);
+  // closed tet surfaces get the reference's isotropic bound there, which is
+  // not built: between tet bodies the contacts act without a limiter.
+  c.pairStream = gidx.size() >= 2 || !selfRoots.empty();
+  c.datOn = !selfRoots.empty();
   for (uint32_t i = 0; i < nb; i++)
     if (surface[i]) c.particles.push_back(i);
   // membranes: rest 2x2 in the triangle's own frame, inverse column major
   std::map<uint64_t, std::vector<std::pair<uint32_t, float>>> edgeTris;  // edge -> (opposite, bend)
-  for (uint32_t t = 0; t < nr; t++) {
+  for (uint32_t t = 0; t < na; t++) {
     float mu = d->tri_mu ? d->tri_mu[t] : 0.0f;
     if (!(mu > 0.0f)) continue;
     float lam = d->tri_lambda ? d->tri_lambda[t] : mu;
@@ -1163,15 +1204,18 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   for (size_t i = 0; i < ex.size(); i++) exr[i] = U2{(uint32_t)(ex[i] >> 32), (uint32_t)(ex[i] & 0xFFFFFFFFu)};
   s->nExcl = (uint32_t)ex.size();
 
-  // ---- soft collision groups: tet boundary particles of one connected
-  // surface share a group (GPUSolver.swift:1476-1522); one union-find over
-  // boundary faces
+  // ---- collision surfaces: authored triangles and tet boundary faces; the
+  // particles of one connected surface share a soft collision group
+  // (GPUSolver.swift:1476-1522)
   std::vector<uint32_t> clothGroup(nb, 0);
   ClothHost ch;
-  if (d->num_tris > 0) {
+  if (d->num_tris > 0 || nt > 0) {
     build_cloth(d, ch);
     clothGroup = ch.group;
-    s->nTris = d->num_tris;
+    s->nTris = (uint32_t)ch.tris.size();
+    s->numContactEdges = ch.numContactEdges;
+    s->datOn = ch.datOn;
+    s->pairStream = ch.pairStream;
     s->nEdges = (uint32_t)ch.edges.size();
     s->nParticles = (uint32_t)ch.particles.size();
     s->nMembranes = (uint32_t)ch.membranes.size();
@@ -1189,29 +1233,6 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->datQueryRadius = ch.datQueryRadius;
     s->datQueryRadiusDerived = ch.datQueryRadius;
     s->surfaceContactCellSize = ch.surfaceContactCellSize;
-  } else if (nt > 0) {
-    std::vector<uint8_t> surf = tet_surface_flags(d);
-    std::vector<uint32_t> parent(nb);
-    for (uint32_t i = 0; i < nb; i++) parent[i] = i;
-    auto find = [&](uint32_t x) {
-      while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
-      return x;
-    };
-    for (uint32_t t = 0; t < nt; t++) {
-      const uint32_t* id = d->tet_ids + 4 * t;
-      uint32_t r = find(id[0]);
-      for (int k = 1; k < 4; k++) parent[find(id[k])] = r;
-    }
-    std::map<uint32_t, uint32_t> gidx;
-    for (uint32_t i = 0; i < nb; i++) {
-      if (!surf[i]) continue;
-      bool particle = d->body_is_particle && d->body_is_particle[i];
-      if (!particle) continue;
-      uint32_t root = find(i);
-      auto it = gidx.find(root);
-      if (it == gidx.end()) it = gidx.emplace(root, (uint32_t)gidx.size() + 1).first;
-      clothGroup[i] = it->second;
-    }
   }
 
   // ---- static topology CSR (sorted by entry: joints, springs, tets)
@@ -1379,7 +1400,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.hashedRigid, std::max<size_t>(1, hashedRigid.size()))
   ALLOC(S.clothSlots, cslots.size())
   {
-    const bool cl = d->num_tris > 0;
+    const bool cl = s->nTris > 0;
     const uint32_t nElem = cl ? s->nTris + s->nEdges : 0u, nProd = cl ? s->nParticles + s->nEdges : 0u;
     ALLOC(S.tris, std::max<uint32_t>(1, s->nTris)) ALLOC(S.edges, std::max<uint32_t>(1, s->nEdges))
     ALLOC(S.nbrStart, cl ? nb : 1u) ALLOC(S.nbrCount, cl ? nb : 1u)
@@ -1578,7 +1599,7 @@ void autotune(avbd_solver* s) {
   // dynamic coloring with per-color launches would dispatch all 64 colors
   if (s->dynamicColoring && wantMode == 0) modeCands = {2, 1};
   // cloth: the Planar-DAT passes run between the colors' launches
-  if (s->nTris) modeCands = {1};
+  if (s->datOn) modeCands = {1};
   if (laneCands.size() == 1 && modeCands.size() == 1) {
     s->lanes = laneCands[0];
     s->sweepMode = modeCands[0];
@@ -2215,6 +2236,9 @@ extern "C" avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info
     info->elem_margin = s->elemMargin;
     info->dat_query_radius = s->datQueryRadius;
     info->surface_contact_cell_size = s->surfaceContactCellSize;
+    info->num_contact_edges = s->numContactEdges;
+    info->pair_stream = s->pairStream ? 1u : 0u;
+    info->dat_on = s->datOn ? 1u : 0u;
   }
   if (edges && s->nEdges && hipMemcpy(edges, s->S.edges, 8 * (size_t)s->nEdges, hipMemcpyDeviceToHost))
     return AVBD_ERR_HIP;
@@ -2224,11 +2248,26 @@ extern "C" avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info
   return AVBD_OK;
 }
 
+extern "C" avbd_status avbd_get_collision_tris(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* tris) {
+  if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
+  *n = s->nTris;
+  uint32_t m = std::min(s->nTris, capacity);
+  if (!tris || !m) return AVBD_OK;
+  std::vector<U4> tmp(m);
+  if (hipMemcpy(tmp.data(), s->S.tris, sizeof(U4) * m, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  for (uint32_t i = 0; i < m; i++) {
+    tris[3 * i] = tmp[i].x;
+    tris[3 * i + 1] = tmp[i].y;
+    tris[3 * i + 2] = tmp[i].z;
+  }
+  return AVBD_OK;
+}
+
 extern "C" avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* pairs) {
   if (!s || !n) return AVBD_ERR_INVALID_ARGUMENT;
   sync_all(s);
   *n = 0;
-  if (!s->nTris) return AVBD_OK;
+  if (!s->pairStream) return AVBD_OK;
   uint32_t cnt = 0;
   // the stream of the accepted predictor pose (its demand is the working count)
   if (hipMemcpy(&cnt, s->S.datStart + s->nParticles + s->nEdges, 4, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;

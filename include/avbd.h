@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 2
+#define AVBD_ABI_VERSION 3
 #define AVBD_MAX_COLORS 64
 
 /* ---- status codes ------------------------------------------------------ */
@@ -324,17 +324,31 @@ avbd_status avbd_get_manifolds(avbd_solver* s, uint32_t capacity, uint32_t* n, v
 avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t max_iterations, int32_t check_every,
                                   int32_t* iterations_used, float* residual);
 
-/* Cloth surface topology derived at create (GPUSolver.swift:1289-1640): counts
- * of collision triangles, unique edges, surface particles, membrane and bend
- * elements, and the grid / query parameters.  Any output pointer may be NULL;
+/* Collision-surface topology derived at create (GPUSolver.swift:384-390,
+ * 1289-1640): counts of collision triangles (the authored ones, then the
+ * boundary faces of the tet meshes), unique edges, surface particles,
+ * membrane and bend elements, and the grid / query parameters.
+ * num_contact_edges: the edges of authored triangles, a prefix of edges[]
+ * (E-E contacts; the tet-boundary edges after it only bound motion);
+ * pair_stream: 1 when the V-T / E-E pair stream and its contacts run (two
+ * or more surfaces, or a self-colliding one); dat_on: 1 when the Planar-DAT
+ * limiter runs (a self-colliding, i.e. authored, surface exists: the
+ * reference's automatic selection, GPUSolver.swift:93-99; its isotropic
+ * bound for closed tet surfaces is not built).  Any output pointer may be NULL;
  * edges[2*num_edges] (ascending vertex ids, edge order of the pair stream),
  * particles[num_particles]. */
 typedef struct avbd_cloth_info {
     uint32_t num_tris, num_edges, num_particles, num_membranes, num_bends;
     uint32_t elem_hash_size, max_soft, soft_map_capacity, max_dat_pairs;
     float elem_cell_size, elem_margin, dat_query_radius, surface_contact_cell_size;
+    uint32_t num_contact_edges, pair_stream, dat_on;
 } avbd_cloth_info;
 avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
+
+/* Collision triangles (trisBuf, GPUSolver.swift:1289-1310): tris[3*n], the
+ * authored ones in scene order, then the tet boundary faces wound outward in
+ * ascending (a, b, c) order (tetBoundaryFaces, GPUSolver.swift:872-897). */
+avbd_status avbd_get_collision_tris(avbd_solver* s, uint32_t capacity, uint32_t* n, uint32_t* tris);
 
 /* Planar-DAT pair stream of the last frame (dat_build_vt_pairs /
  * dat_build_ee_pairs, 27_planar_dat.metal:35-229), the query at the accepted

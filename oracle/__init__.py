@@ -59,6 +59,7 @@ def load():
         "oracle_solve6x6": (None, [dp, dp, dp, C.c_int]),
         "oracle_cell_hash": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32]),
         "oracle_get_cloth_info": (C.c_int, [vp, vp, u32p, u32p]),
+        "oracle_get_collision_tris": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
         "oracle_get_dat_pairs": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
         "oracle_get_soft_contacts": (C.c_int, [vp, C.c_uint32, u32p, vp]),
         "oracle_cloth_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p]),
@@ -179,6 +180,13 @@ class OracleSolver:
         out = np.zeros(len(q), np.int32)
         self._lib.oracle_map_find(self._h, _p(q, C.c_uint32), len(q), _p(out, C.c_int32))
         return out
+
+    def collisionTris(self):
+        n = C.c_uint32(0)
+        self._lib.oracle_get_collision_tris(self._h, 0, C.byref(n), None)
+        t = np.zeros((max(1, n.value), 3), np.uint32)
+        self._lib.oracle_get_collision_tris(self._h, n.value, C.byref(n), _p(t, C.c_uint32))
+        return t[:n.value]
 
     def clothInfo(self):
         import sys

@@ -276,7 +276,9 @@ struct Oracle : oracle_base {
   using Membrane = MembraneT<T>;
   using Bend = BendT<T>;
   using Soft = SoftT<T>;
-  uint32_t nTris = 0;
+  uint32_t nTris = 0, numContactEdges = 0;
+  bool pairStream = false;  // the V-T / E-E pair stream and its contacts: some pair can pass the filters
+  bool datOn = false;       // the Planar-DAT limiter: a self-colliding (authored) surface exists
   std::vector<std::array<uint32_t, 3>> tris;
   std::vector<std::pair<uint32_t, uint32_t>> edges;
   std::vector<uint32_t> nbrStart, nbrCount, nbrList, particles, selfColl, hashedRigidList;
@@ -531,34 +533,10 @@ struct Oracle : oracle_base {
       ex.insert({std::min(a, b), std::max(a, b)});
     }
     excl.assign(ex.begin(), ex.end());
-    // soft collision group of tet boundary particles (GPUSolver.swift:1476-1522)
+    // collision surfaces: authored triangles and tet boundary faces; soft
+    // collision groups over them (GPUSolver.swift:1476-1522)
     clothGroup.assign(nb, 0);
-    if (d->num_tris > 0) {
-      if (d->num_tets > 0) return 4;  // tet surfaces with element contacts: not restated
-      clothInit(d);
-    } else if (!tets.empty()) {
-      std::map<std::array<uint32_t, 3>, int> faces;
-      const int fi[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
-      for (auto& t : tets)
-        for (int f = 0; f < 4; f++) {
-          std::array<uint32_t, 3> k{t.ids[fi[f][0]], t.ids[fi[f][1]], t.ids[fi[f][2]]};
-          std::sort(k.begin(), k.end());
-          faces[k]++;
-        }
-      std::vector<uint32_t> parent(nb);
-      for (uint32_t i = 0; i < nb; i++) parent[i] = i;
-      std::function<uint32_t(uint32_t)> find = [&](uint32_t x) { return parent[x] == x ? x : (parent[x] = find(parent[x])); };
-      for (auto& t : tets) for (int k = 1; k < 4; k++) parent[find(t.ids[k])] = find(t.ids[0]);
-      std::map<uint32_t, uint32_t> gid;
-      std::vector<uint8_t> surf(nb, 0);
-      for (auto& kv : faces) if (kv.second == 1) for (uint32_t v : kv.first) surf[v] = 1;
-      for (uint32_t i = 0; i < nb; i++) {
-        if (!surf[i] || !particle[i]) continue;
-        uint32_t r = find(i);
-        if (!gid.count(r)) { uint32_t nid = (uint32_t)gid.size() + 1; gid[r] = nid; }
-        clothGroup[i] = gid[r];
-      }
-    }
+    if (d->num_tris > 0 || !tets.empty()) clothInit(d);
     // static force adjacency, sorted: joints, springs, tets
     staticAdj.assign(nb, {});
     for (uint32_t i = 0; i < joints.size(); i++) {
@@ -606,8 +584,45 @@ struct Oracle : oracle_base {
   // Surface topology, membranes, hinges, grid / query parameters and
   // capacities (GPUSolver.swift:391-443, 1289-1690).  Element constants are
   // derived in double and rounded to float, as the records store them.
+  // Boundary faces of the tet meshes: faces of exactly one tet, wound so the
+  // normal points away from that tet's fourth vertex, in ascending
+  // (a, b, c) order (GPUSolver.swift:872-897).
+  static std::vector<std::array<uint32_t, 3>> tetBoundaryFaces(const avbd_scene_desc* d) {
+    std::map<std::array<uint32_t, 3>, std::pair<std::array<uint32_t, 4>, int>> seen;  // sorted -> (tri + opposite, uses)
+    const int fv[4][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 3, 1}, {1, 2, 3, 0}};
+    for (uint32_t t = 0; t < d->num_tets; t++) {
+      const uint32_t* q = d->tet_ids + 4 * t;
+      for (auto& f : fv) {
+        std::array<uint32_t, 3> key{q[f[0]], q[f[1]], q[f[2]]};
+        std::sort(key.begin(), key.end());
+        auto it = seen.find(key);
+        if (it == seen.end()) seen[key] = {{q[f[0]], q[f[1]], q[f[2]], q[f[3]]}, 1};
+        else it->second.second++;
+      }
+    }
+    auto P = [&](uint32_t i) {
+      return V<float>(d->body_position[3 * i], d->body_position[3 * i + 1], d->body_position[3 * i + 2]);
+    };
+    std::vector<std::array<uint32_t, 3>> out;
+    for (auto& kv : seen) {
+      if (kv.second.second != 1) continue;
+      auto f = kv.second.first;
+      V<float> n = crossv(P(f[1]) - P(f[0]), P(f[2]) - P(f[0]));
+      if (dotv(n, P(f[3]) - P(f[0])) > 0.0f) std::swap(f[1], f[2]);
+      out.push_back({f[0], f[1], f[2]});
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+
   void clothInit(const avbd_scene_desc* d) {
-    nTris = d->num_tris;
+    // collision triangles: the authored ones, then the tet boundary faces;
+    // membranes and hinges from the authored ones only (GPUSolver.swift:1289-1334)
+    const uint32_t nAuthored = d->num_tris;
+    std::vector<std::array<uint32_t, 3>> coll;
+    for (uint32_t t = 0; t < nAuthored; t++) coll.push_back({d->tri_ids[3 * t], d->tri_ids[3 * t + 1], d->tri_ids[3 * t + 2]});
+    for (const auto& f : tetBoundaryFaces(d)) coll.push_back(f);
+    nTris = (uint32_t)coll.size();
     auto sizeX = [&](uint32_t i) { return d->body_size[3 * i]; };
     auto bp = [&](uint32_t i, int k) { return d->body_position[3 * i + k]; };
     auto fdist = [&](uint32_t a, uint32_t b) {
@@ -617,7 +632,9 @@ struct Oracle : oracle_base {
     std::set<uint64_t> seen;
     float maxElemR = 0.0f, maxPartR = 0.0f;
     for (uint32_t t = 0; t < nTris; t++) {
-      std::array<uint32_t, 3> id{d->tri_ids[3 * t], d->tri_ids[3 * t + 1], d->tri_ids[3 * t + 2]};
+      // edges of the authored triangles come first: the E-E contact prefix
+      if (t == nAuthored) numContactEdges = (uint32_t)edges.size();
+      std::array<uint32_t, 3> id = coll[t];
       tris.push_back(id);
       uint32_t ev[3][2] = {{id[0], id[1]}, {id[1], id[2]}, {id[0], id[2]}};
       for (auto& e : ev) {
@@ -634,6 +651,7 @@ struct Oracle : oracle_base {
       maxPartR = std::max(maxPartR, thick);
       maxElemR = std::max(maxElemR, std::max(dm(id[0]), std::max(dm(id[1]), dm(id[2]))) + thick);
     }
+    if (nTris == nAuthored) numContactEdges = (uint32_t)edges.size();
     std::vector<std::vector<uint32_t>> nb1(nb);
     float edgeLenSum = 0.0f;
     for (auto& e : edges) {
@@ -651,26 +669,35 @@ struct Oracle : oracle_base {
       nbrCount[i] = (uint32_t)nb1[i].size();
       nbrList.insert(nbrList.end(), nb1[i].begin(), nb1[i].end());
     }
-    // connected sheets, numbered by their first triangle; authored sheets self-collide
+    // connected surfaces, numbered by their first triangle; a surface with an
+    // authored triangle self-collides, a tet boundary does not (GPUSolver.swift:1476-1522)
     std::vector<uint32_t> parent(nb);
     for (uint32_t i = 0; i < nb; i++) parent[i] = i;
     std::function<uint32_t(uint32_t)> find = [&](uint32_t x) { return parent[x] == x ? x : (parent[x] = find(parent[x])); };
     for (auto& t : tris) { uint32_t r = find(t[0]); parent[find(t[1])] = r; parent[find(t[2])] = r; }
     selfColl.assign(nb, 0);
+    std::set<uint32_t> selfRoots;
+    for (uint32_t t = 0; t < nAuthored; t++) selfRoots.insert(find(tris[t][0]));
     std::map<uint32_t, uint32_t> gid;
     std::vector<uint8_t> surf(nb, 0);
     for (auto& t : tris) {
       uint32_t r = find(t[0]);
       if (!gid.count(r)) { uint32_t nid = (uint32_t)gid.size() + 1; gid[r] = nid; }
       for (uint32_t v : t)
-        if (particle[v]) { clothGroup[v] = gid[r]; selfColl[v] = 1; surf[v] = 1; }
+        if (particle[v]) { clothGroup[v] = gid[r]; selfColl[v] = selfRoots.count(r) ? 1 : 0; surf[v] = 1; }
     }
+    // The pair stream runs where a pair can pass the topology filters; the
+    // Planar-DAT limiter where the reference selects it automatically, with
+    // self-colliding (authored) surfaces (GPUSolver.swift:93-99).  Closed tet
+    // surfaces get the reference's isotropic bound there, not restated.
+    pairStream = gid.size() >= 2 || !selfRoots.empty();
+    datOn = !selfRoots.empty();
     for (uint32_t i = 0; i < nb; i++) if (surf[i]) particles.push_back(i);
     datAnchor.assign(nb, V<T>());
     datT.assign(nb, T(1));
     // StVK membranes: rest 2x2 in the triangle frame, inverse column major
     std::map<uint64_t, std::vector<std::pair<uint32_t, float>>> edgeTris;
-    for (uint32_t t = 0; t < nTris; t++) {
+    for (uint32_t t = 0; t < nAuthored; t++) {
       float mu = d->tri_mu ? d->tri_mu[t] : 0.0f;
       if (!(mu > 0.0f)) continue;
       float lam = d->tri_lambda ? d->tri_lambda[t] : mu, bend = d->tri_bend ? d->tri_bend[t] : 0.0f;
@@ -2016,6 +2043,13 @@ struct Oracle : oracle_base {
     }
     uint32_t ea = pr[1], eb = pr[2];
     auto ai = edges[ea], bi = edges[eb];
+    // contacts between authored edges or within a self-colliding surface;
+    // tet-boundary edge pairs only bound the motion (27_planar_dat.metal:196-201)
+    if (ea >= numContactEdges || eb >= numContactEdges) {
+      uint32_t g = clothGroup[ai.first];
+      bool same = g != 0 && clothGroup[ai.second] == g && clothGroup[bi.first] == g && clothGroup[bi.second] == g;
+      if (!(same && selfColl[ai.first])) return false;
+    }
     V<T> a0 = pos[ai.first], a1 = pos[ai.second], b0 = pos[bi.first], b1 = pos[bi.second];
     T s_, t_;
     segSeg(a0, a1, b0, b1, s_, t_);
@@ -2606,7 +2640,7 @@ struct Oracle : oracle_base {
       const long n = (long)bk.size();
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1 && n > 256)
       for (long i = 0; i < n; i++) primal(bk[(size_t)i]);
-      if (nTris && n > 0) datColorPass(c);
+      if (datOn && n > 0) datColorPass(c);
     }
     dual();
   }
@@ -2632,17 +2666,17 @@ struct Oracle : oracle_base {
     // predictor pose feeds the contacts and the per-color passes, anchored
     // there (GPUSolver.swift:3685-3870)
     if (nTris) {
-      datBuild();
+      if (datOn) datBuild();
       rtEmit();
     }
     warmstartConstraints();
     warmstartBodies();
-    if (nTris) {
-      datPredictorPass();
+    if (datOn) datPredictorPass();
+    if (pairStream) {
       datBuild();
       datContacts();
-      for (uint32_t v : particles) datAnchor[v] = pos[v];
     }
+    if (datOn) for (uint32_t v : particles) datAnchor[v] = pos[v];
     softAdj.assign(nb, {});
     for (uint32_t i = 0; i < soft.size(); i++)
       for (int k = 0; k < 4; k++)
@@ -2672,7 +2706,7 @@ struct Oracle : oracle_base {
     }
     // a failed surface query restores the step-start surface
     // (dat_restore_failed_surface, 27_planar_dat.metal:1289-1310)
-    if (nTris && (softCandidates > maxSoft || datPeak > maxDatPairs || datOverflow > 0 || datInvalid > 0))
+    if (datOn && (softCandidates > maxSoft || datPeak > maxDatPairs || datOverflow > 0 || datInvalid > 0))
       for (uint32_t i = 0; i < nb; i++)
         if (radiusSigned[i] < 0) pos[i] = initLin[i];
     // BDF1 finalize (40_solver.metal:2108-2150)
@@ -2913,6 +2947,16 @@ int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* 
   if (o->f) go(o->f); else go(o->d);
   return 0;
 }
+int oracle_get_collision_tris(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* tris) {
+  auto go = [&](auto* s) {
+    *n = (uint32_t)s->tris.size();
+    for (uint32_t i = 0; i < std::min(*n, capacity); i++)
+      for (int k = 0; k < 3; k++) tris[3 * i + k] = s->tris[i][k];
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+
 int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles) {
   auto go = [&](auto* s) {
     if (info) {
@@ -2929,6 +2973,9 @@ int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edg
       info->elem_margin = s->elemMargin;
       info->dat_query_radius = s->datQueryRadius;
       info->surface_contact_cell_size = s->surfaceContactCellSize;
+      info->num_contact_edges = s->numContactEdges;
+      info->pair_stream = s->pairStream ? 1u : 0u;
+      info->dat_on = s->datOn ? 1u : 0u;
     }
     if (edges)
       for (size_t i = 0; i < s->edges.size(); i++) { edges[2 * i] = s->edges[i].first; edges[2 * i + 1] = s->edges[i].second; }

@@ -294,3 +294,87 @@ def pyramid(base):
         for i in range(base - j):
             s.addBody((0.95, 0.95, 0.95), 1.0, 0.8, (i + j / 2 - base / 2, 0, 0.5 + j))
     return s
+
+
+# -- SoftBodyTests.swift scenes --------------------------------------------------
+def add_soft_lattice(s, center, nx, ny, nz, spacing, mu=3000.0, lam=30000.0, massPerNode=0.05, friction=0.8):
+    """Demos.addSoftBlock (Demos5.swift:76-145): a particle lattice, five
+    tets per cell with the split alternating on (i + j + k) parity, and inert
+    (stiffness 0) joints between lattice neighbours as collision exclusions.
+    Returns the particle ids in (i, j, k) order."""
+    from avbd_amd.scene import SceneJoint
+    half = [(n - 1) * spacing / 2 for n in (nx, ny, nz)]
+    ids = {}
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                p = (center[0] + i * spacing - half[0], center[1] + j * spacing - half[1],
+                     center[2] + k * spacing - half[2])
+                ids[i, j, k] = s.addParticle(spacing * 0.32, massPerNode, p, friction=friction)
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            for k in range(nz - 1):
+                c = {(a, b, d): ids[i + a, j + b, k + d] for a in (0, 1) for b in (0, 1) for d in (0, 1)}
+                if (i + j + k) % 2 == 0:
+                    split = [(c[0, 0, 0], c[1, 0, 0], c[0, 1, 0], c[0, 0, 1]), (c[1, 1, 0], c[1, 0, 0], c[0, 1, 0], c[1, 1, 1]),
+                             (c[1, 0, 1], c[1, 0, 0], c[0, 0, 1], c[1, 1, 1]), (c[0, 1, 1], c[0, 1, 0], c[0, 0, 1], c[1, 1, 1]),
+                             (c[1, 0, 0], c[0, 1, 0], c[0, 0, 1], c[1, 1, 1])]
+                else:
+                    split = [(c[1, 0, 0], c[0, 0, 0], c[1, 1, 0], c[1, 0, 1]), (c[0, 1, 0], c[0, 0, 0], c[1, 1, 0], c[0, 1, 1]),
+                             (c[0, 0, 1], c[0, 0, 0], c[1, 0, 1], c[0, 1, 1]), (c[1, 1, 1], c[1, 1, 0], c[1, 0, 1], c[0, 1, 1]),
+                             (c[0, 0, 0], c[1, 1, 0], c[1, 0, 1], c[0, 1, 1])]
+                for t in split:
+                    s.addTet(t, mu, lam)
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                for di, dj, dk in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)):
+                    if i + di < nx and j + dj < ny and k + dk < nz:
+                        s.addJoint(SceneJoint(ids[i, j, k], ids[i + di, j + dj, k + dk], (0, 0, 0), (0, 0, 0),
+                                              stiffnessLin=0.0, stiffnessAng=0.0))
+    return [ids[i, j, k] for i in range(nx) for j in range(ny) for k in range(nz)]
+
+
+def particle_rod_chain():
+    """testParticleRodChainInextensible (SoftBodyTests.swift:28-51): twelve
+    particles joined by hard rods, the first pinned to the world."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("pc", SimSettings(iterations=20))
+    prev = -1
+    for k in range(12):
+        p = s.addParticle(0.05, 0.02, (k * 0.25, 0, 4))
+        if k == 0:
+            s.addJoint(SceneJoint(-1, p, (0, 0, 4), (0, 0, 0)))
+        else:
+            s.addSpring(prev, p, (0, 0, 0), (0, 0, 0), 1e5, hard=True)
+        prev = p
+    return s
+
+
+def pinned_tet():
+    """testTetVolumePreservation (SoftBodyTests.swift:53-68): one Neo-Hookean
+    tet of edge 0.5 hanging from a pinned corner."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("tet1", SimSettings(iterations=20))
+    ids = [s.addParticle(0.05, 0.1, p) for p in ((0, 0, 3), (0.5, 0, 3), (0, 0.5, 3), (0, 0, 3.5))]
+    s.addJoint(SceneJoint(-1, ids[0], (0, 0, 3), (0, 0, 0)))
+    s.addTet(tuple(ids), 2000.0, 20000.0)
+    return s, ids
+
+
+def ball_on_soft_block():
+    """testRigidRestsOnSoftBlock (SoftBodyTests.swift:70-83)."""
+    s = PhysicsScene("ballonblock", SimSettings(iterations=20, betaLin=20000.0, lambdaMax=800.0))
+    _ground(s)
+    add_soft_lattice(s, (0, 0, 1.2), 4, 4, 4, 0.34, mu=4000.0, lam=40000.0)
+    ball = s.addSphere(1.1, 3.0, 0.6, (0, 0, 4.2))
+    return s, ball
+
+
+def stacked_soft_blocks():
+    """testSoftBlocksDontInterpenetrate (SoftBodyTests.swift:98-123)."""
+    s = PhysicsScene("sbsb", SimSettings(iterations=20))
+    _ground(s)
+    add_soft_lattice(s, (0, 0, 0.7), 4, 4, 4, 0.34, mu=4000.0, lam=40000.0, massPerNode=0.05)
+    top = add_soft_lattice(s, (0.15, 0.1, 2.3), 3, 3, 3, 0.34, mu=4000.0, lam=40000.0, massPerNode=0.05)
+    return s, top

@@ -262,3 +262,39 @@ def test_pyramid_bitwise_determinism():
     a.step(60)
     b.step(60)
     assert np.array_equal(a.positions(), b.positions())
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_particle_rods_and_pinned_tet(kind):
+    # SoftBodyTests.swift:28-68
+    x = make(kind, R.particle_rod_chain())
+    x.step(300)
+    p, r, _, _ = states(x)
+    d = np.linalg.norm(p[1:12] - p[:11], axis=1)
+    assert np.all(np.abs(d - 0.25) < 0.01), d
+    assert abs(r[5, 3] - 1.0) < 1e-4                      # particles carry no rotation
+    s, ids = R.pinned_tet()
+    x = make(kind, s)
+    x.step(300)
+    a, b, c, d = x.positions()[ids].astype(np.float64)
+    v = abs(np.dot(b - a, np.cross(c - a, d - a))) / 6
+    assert abs(v * 6 / 0.5 ** 3 - 1.0) < 0.05
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_ball_rests_on_soft_block(kind):
+    # SoftBodyTests.testRigidRestsOnSoftBlock (:70-83)
+    s, ball = R.ball_on_soft_block()
+    x = make(kind, s)
+    x.step(500)
+    assert x.positions()[ball, 2] > 0.8
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_soft_blocks_do_not_interpenetrate(kind):
+    # SoftBodyTests.testSoftBlocksDontInterpenetrate (:98-123)
+    s, top = R.stacked_soft_blocks()
+    x = make(kind, s)
+    x.step(400)
+    p = x.positions()[top]
+    assert p[:, 2].min() > 0.95 and p[:, 2].mean() > 1.15
