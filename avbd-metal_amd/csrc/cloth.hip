@@ -154,8 +154,27 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
   V3 lo, hi;
   elem_bounds(S, P, gid, lo, hi);
   if (!SCATTER) {
-    S.elemBox[2 * gid] = f4(lo, 0.0f);
+    // lo.w carries the element's surface group when all its vertices share
+    // one (else 0): the queries test the same-component filter from it
+    uint32_t ge;
+    if (gid < P.numTris) {
+      U4 t = S.tris[gid];
+      ge = S.clothGroup[t.x];
+      if (S.clothGroup[t.y] != ge || S.clothGroup[t.z] != ge) ge = 0u;
+    } else {
+      U2 e = S.edges[gid - P.numTris];
+      ge = S.clothGroup[e.x];
+      if (S.clothGroup[e.y] != ge) ge = 0u;
+    }
+    S.elemBox[2 * gid] = f4(lo, as_float(ge));
     S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
+  }
+  // an element of a surface with no other surface in reach is never a
+  // candidate of an accepted pair: its counters above, no bucket entries
+  bool unreached = false;
+  if (P.groupSkip) {
+    const uint32_t gb = as_uint(S.elemBox[2 * gid].w);
+    unreached = gb != 0u && S.groupNear[gb] == 0u;
   }
   if (!finite3(lo) || !finite3(hi)) {
     if (!SCATTER) {
@@ -166,6 +185,7 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
   }
   CellRange r = range_of(lo, hi, P.elemCellSize);
   if (!SCATTER && r.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+  if (unreached) return;
   const Cell3 c0 = r.c0;
   const int nx = r.sx + 1, ny = r.sy + 1, nz = r.sz + 1;
   const int n = nx * ny * nz;  // <= 64
@@ -184,6 +204,105 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
     const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
     if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
     else atomicAdd(&S.elemCellCount[slot], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-surface reach: the particle box of every surface group (atomic min /
+// max on order-preserving keys of the coordinates), then whether another
+// group's box lies within the query radius (grown by a rounding margin)
+// ---------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ uint32_t order_key(float f) {
+  uint32_t u = as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) { return as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k); }
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_group_clear(DeviceState S, StepParams P) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > P.numGroups) return;
+  for (int k = 0; k < 3; k++) {
+    S.groupBox[6 * g + k] = 0xFFFFFFFFu;
+    S.groupBox[6 * g + 3 + k] = 0u;
+  }
+  S.groupNear[g] = 0u;
+}
+
+// surface particles are ordered by body id, so a wave holds one group or
+// two: the groups of its first and last lanes are reduced across the wave
+// and committed by one lane each; lanes of any other group commit alone
+__global__ void __launch_bounds__(256) k_group_box(DeviceState S, StepParams P) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < P.numParticles;
+  uint32_t g = 0;
+  bool fin = false;
+  uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  if (in) {
+    const uint32_t v = S.particleIdx[i];
+    g = S.clothGroup[v];
+    const V3 p = P3(S, v);
+    fin = finite3(p);
+    if (!fin) S.groupNear[g] = 1u;  // the queries run for this surface and report it
+    else {
+      k[0] = k[3] = order_key(p.x);
+      k[1] = k[4] = order_key(p.y);
+      k[2] = k[5] = order_key(p.z);
+    }
+  }
+  const bool live = in && fin;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t ends[2] = {(uint32_t)__shfl((int)g, 0), (uint32_t)__shfl((int)g, 63)};
+  for (int w = 0; w < 2; w++) {
+    const uint32_t ge = ends[w];
+    if (w == 1 && ge == ends[0]) break;
+    const bool mine = live && g == ge;
+    uint32_t r[6];
+    for (int j = 0; j < 3; j++) {
+      r[j] = mine ? k[j] : 0xFFFFFFFFu;
+      r[3 + j] = mine ? k[3 + j] : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      for (int j = 0; j < 3; j++) {
+        r[j] = min(r[j], (uint32_t)__shfl_xor((int)r[j], o));
+        r[3 + j] = max(r[3 + j], (uint32_t)__shfl_xor((int)r[3 + j], o));
+      }
+    const bool any = __any(mine);
+    if (lane == 0 && any)
+      for (int j = 0; j < 3; j++) {
+        atomicMin(&S.groupBox[6 * ge + j], r[j]);
+        atomicMax(&S.groupBox[6 * ge + 3 + j], r[3 + j]);
+      }
+  }
+  if (!live || g == ends[0] || g == ends[1]) return;
+  for (int j = 0; j < 3; j++) {
+    atomicMin(&S.groupBox[6 * g + j], k[j]);
+    atomicMax(&S.groupBox[6 * g + 3 + j], k[3 + j]);
+  }
+}
+
+// one block: every group's box staged in LDS (at most kMaxSkipGroups)
+__global__ void __launch_bounds__(256) k_group_near(DeviceState S, StepParams P) {
+  __shared__ float box[6 * (kMaxSkipGroups + 1)];
+  for (uint32_t i = threadIdx.x; i < 6 * (P.numGroups + 1); i += blockDim.x) box[i] = key_value(S.groupBox[i]);
+  __syncthreads();
+  // margin: the radius, plus rounding of the distances and closest points
+  // the queries compute (relative 1e-6 of the coordinates, ~16 ulp)
+  const float r = P.datQueryRadius * 1.001f + 1.0e-5f;
+  for (uint32_t g = threadIdx.x + 1u; g <= P.numGroups; g += blockDim.x) {
+    bool near = false;
+    for (uint32_t h = 1; h <= P.numGroups && !near; h++) {
+      if (h == g) continue;
+      bool apart = false;
+      for (int k = 0; k < 3; k++) {
+        const float gl = box[6 * g + k], gh = box[6 * g + 3 + k], hl = box[6 * h + k], hh = box[6 * h + 3 + k];
+        const float m = r + 1.0e-6f * (fabsf(gl) + fabsf(gh) + fabsf(hl) + fabsf(hh));
+        apart = apart || hl > gh + m || hh < gl - m;
+      }
+      near = !apart;
+    }
+    if (near) S.groupNear[g] = 1u;
   }
 }
 
@@ -213,10 +332,14 @@ __device__ __forceinline__ bool boxes_apart(V3 alo, V3 ahi, F4 lo, F4 hi, float 
          alo.y > hi.y + slack || alo.z > hi.z + slack;
 }
 
+// `noSelf`: the producer's surface group when that surface does not
+// self-collide (else 0); a candidate of that group fails the same-component
+// filter, tested first from the group stored with the element's box.
 __device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell, uint32_t ns,
-                          uint32_t ne, uint32_t entry, float q2) {
+                          uint32_t ne, uint32_t entry, float q2, uint32_t noSelf) {
   if ((entry & kElemEdgeBit) != 0u || entry >= P.numTris) return false;
   const F4 blo = S.elemBox[2 * entry], bhi = S.elemBox[2 * entry + 1];
+  if (noSelf != 0u && as_uint(blo.w) == noSelf) return false;
   if (outside_box(p, blo, bhi, kBoxSlack)) return false;
   U4 t = S.tris[entry];
   if (t.x == v || t.y == v || t.z == v) return false;
@@ -236,11 +359,12 @@ __device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams
 
 __device__ __forceinline__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
                           const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1, Cell3 cell,
-                          uint32_t entry, float q2) {
+                          uint32_t entry, float q2, uint32_t noSelf) {
   if ((entry & kElemEdgeBit) == 0u) return false;
   uint32_t other = entry & ~kElemEdgeBit;
   if (other <= gid || other >= P.numEdges) return false;
   const F4 blo = S.elemBox[2 * (P.numTris + other)], bhi = S.elemBox[2 * (P.numTris + other) + 1];
+  if (noSelf != 0u && as_uint(blo.w) == noSelf) return false;
   if (boxes_apart(vmin(a0, a1), vmax(a0, a1), blo, bhi, kBoxSlack)) return false;
   U2 b = S.edges[other];
   if (b.x == a.x || b.x == a.y || b.y == a.x || b.y == a.y) return false;
@@ -280,14 +404,16 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
       }
       return;
     }
+    if (P.groupSkip && S.groupNear[S.clothGroup[v]] == 0u) return;
     Cell3 cell = cell_of(p, P.elemCellSize);
     uint32_t h = cell_hash(cell.x, cell.y, cell.z, 0u, P.elemHashSize);
     uint32_t s = S.elemCellStart[2u * h], e = S.elemCellStart[2u * h + 1u];  // the bucket's triangles
     uint32_t ns = S.nbrStart[v], ne = ns + S.nbrCount[v];
+    const uint32_t noSelf = S.selfCollision[v] == 0u ? S.clothGroup[v] : 0u;
     for (uint32_t base = s; base < e; base += width) {
       uint32_t k = base + lane;
       uint32_t entry = k < e ? S.elemCells[k] : 0u;
-      bool acc = k < e && vt_accept(S, P, v, p, cell, ns, ne, entry, q2);
+      bool acc = k < e && vt_accept(S, P, v, p, cell, ns, ne, entry, q2, noSelf);
       f(acc, entry);
     }
     return;
@@ -304,8 +430,11 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
   }
   CellRange ar = range_of(vmin(a0, a1), vmax(a0, a1), P.elemCellSize);
   if (diag && lane == 0 && ar.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+  if (P.groupSkip && S.groupNear[S.clothGroup[a.x]] == 0u) return;
   uint32_t ns0 = S.nbrStart[a.x], ne0 = ns0 + S.nbrCount[a.x];
   uint32_t ns1 = S.nbrStart[a.y], ne1 = ns1 + S.nbrCount[a.y];
+  const uint32_t ga = S.clothGroup[a.x];
+  const uint32_t noSelf = (S.selfCollision[a.x] == 0u && S.clothGroup[a.y] == ga) ? ga : 0u;
   for (int iz = 0; iz <= ar.sz; iz++)
     for (int iy = 0; iy <= ar.sy; iy++)
       for (int ix = 0; ix <= ar.sx; ix++) {
@@ -316,7 +445,7 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
         for (uint32_t base = s; base < e; base += width) {
           uint32_t k = base + lane;
           uint32_t entry = k < e ? S.elemCells[k] : 0u;
-          bool acc = k < e && ee_accept(S, P, gid, a, a0, a1, ar, ns0, ne0, ns1, ne1, cell, entry, q2);
+          bool acc = k < e && ee_accept(S, P, gid, a, a0, a1, ar, ns0, ne0, ns1, ne1, cell, entry, q2, noSelf);
           f(acc, entry & ~kElemEdgeBit);
         }
       }
@@ -1127,6 +1256,11 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   uint32_t nElem = P.numTris + P.numEdges;
   uint32_t eb = (nElem + 255) / 256;
   const uint32_t slots = 2u * P.elemHashSize;
+  if (P.groupSkip) {
+    hipLaunchKernelGGL(k_group_clear, dim3((P.numGroups + 256) / 256), dim3(256), 0, st, S, P);
+    hipLaunchKernelGGL(k_group_box, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
+    hipLaunchKernelGGL(k_group_near, dim3(1), dim3(256), 0, st, S, P);
+  }
   launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
   hipLaunchKernelGGL(k_el_bin<false>, dim3(eb), dim3(256), 0, st, S, P);
   launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, slots + 1u, S.scanScratch);

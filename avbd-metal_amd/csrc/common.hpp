@@ -391,6 +391,8 @@ struct StepParams {
   uint32_t numContactEdges;  // edges [0, numContactEdges) carry E-E contacts; later ones bound motion only
   uint32_t pairStream;       // the V-T / E-E pair stream and its contacts run
   uint32_t datOn;            // the Planar-DAT limiter passes run
+  uint32_t numGroups;        // surface groups (1-based ids)
+  uint32_t groupSkip;        // queries skip producers of a group with no other group in reach
   float elemCellSize, elemMargin, datQueryRadius, surfaceContactCellSize, clothViscosity;
   float fineCellSize;       // > 0: candidate search grid of 2 x the largest hashed radius
 };

@@ -64,6 +64,8 @@ struct avbd_solver {
   uint32_t numContactEdges = 0;  // authored-triangle edges: the E-E contact prefix of the edge list
   bool pairStream = false;       // V-T / E-E pair stream and its contacts (some pair can be accepted)
   bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
+  uint32_t numGroups = 0;        // connected collision surfaces
+  bool groupSkip = false;        // the queries skip surfaces with no other surface in reach
   uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
   float elemCellSize = 0.0f, elemMargin = 0.0f, datQueryRadius = 0.0f, surfaceContactCellSize = 0.0f;
   float datQueryRadiusDerived = 0.0f;  // rq derived at create (a test hook may replace datQueryRadius)
@@ -202,6 +204,8 @@ StepParams make_params(const avbd_solver* s) {
   P.numContactEdges = s->numContactEdges;
   P.datOn = s->datOn ? 1u : 0u;
   P.pairStream = s->pairStream ? 1u : 0u;
+  P.numGroups = s->numGroups;
+  P.groupSkip = s->groupSkip ? 1u : 0u;
   P.numParticles = s->nParticles;
   P.elemHashSize = s->elemHashSize;
   P.maxSoft = s->maxSoft;
@@ -600,6 +604,8 @@ struct ClothHost {
   uint32_t numContactEdges = 0;  // edges of authored triangles (E-E contacts); tet-boundary edges follow
   bool pairStream = false;       // some V-T / E-E pair can be accepted: two surfaces, or a self-colliding one
   bool datOn = false;            // Planar-DAT limiter: a self-colliding (authored) surface exists
+  uint32_t numGroups = 0;        // connected surfaces
+  bool allParticleVerts = true;  // every collision-triangle vertex is a particle
   float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
 };
 
@@ -702,6 +708,9 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
   // not built: between tet bodies the contacts act without a limiter.
   c.pairStream = gidx.size() >= 2 || !selfRoots.empty();
   c.datOn = !selfRoots.empty();
+  c.numGroups = (uint32_t)gidx.size();
+  for (const U4& t : c.tris)
+    if (!isParticle(t.x) || !isParticle(t.y) || !isParticle(t.z)) c.allParticleVerts = false;
   for (uint32_t i = 0; i < nb; i++)
     if (surface[i]) c.particles.push_back(i);
   // membranes: rest 2x2 in the triangle's own frame, inverse column major
@@ -1215,6 +1224,12 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->nTris = (uint32_t)ch.tris.size();
     s->numContactEdges = ch.numContactEdges;
     s->datOn = ch.datOn;
+    s->numGroups = ch.numGroups;
+    // Without self-collision a pair joins two surfaces, both within the
+    // query radius of each other's particle box: the queries skip every
+    // producer of a surface with no other surface in reach (same pairs and
+    // counters, less work when the surfaces are apart)
+    s->groupSkip = !ch.datOn && ch.pairStream && ch.allParticleVerts && ch.numGroups <= kMaxSkipGroups;
     s->pairStream = ch.pairStream;
     s->nEdges = (uint32_t)ch.edges.size();
     s->nParticles = (uint32_t)ch.particles.size();
@@ -1418,6 +1433,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.softMapKeyA, cl ? s->softMapCapacity : 1u) ALLOC(S.softMapKeyB, cl ? s->softMapCapacity : 1u)
     ALLOC(S.softMapVal, cl ? s->softMapCapacity : 1u)
     ALLOC(S.velScratch, cl ? nb : 1u)
+    ALLOC(S.groupBox, s->groupSkip ? 6 * (s->numGroups + 1) : 1u) ALLOC(S.groupNear, s->groupSkip ? s->numGroups + 1 : 1u)
   }
   uint32_t scanMax = std::max<uint32_t>(std::max<uint32_t>(s->gridHashSize, producers), std::max(nb, kMaxColors * bb));
   scanMax = std::max<uint32_t>(scanMax, std::max(2 * s->elemHashSize + 1, s->nParticles + s->nEdges + 1));

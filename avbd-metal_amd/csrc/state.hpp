@@ -71,7 +71,7 @@ struct DeviceState {
   uint32_t* selfCollision;    // per body: its surface self-collides
   // element grid: counts / starts / cursors per (bucket, triangles | edges), interleaved
   uint32_t *elemCellCount, *elemCellStart, *elemCellCursor, *elemCells;
-  F4* elemBox;                // per element (triangles, then edges): padded box lo, hi
+  F4* elemBox;                // per element (triangles, then edges): padded box lo, hi; lo.w = surface group bits (0: mixed)
   F4* datAnchor;              // Planar-DAT anchor pose of the color passes (accepted predictor pose)
   uint32_t* datT;             // per body: step fraction of the running pass (f32 bits, 1.0 between passes)
   uint32_t *datCount, *datStart;  // per producer: surface particles, then edges (+1 total)
@@ -80,6 +80,8 @@ struct DeviceState {
   SoftRec *soft, *prevSoft;
   uint32_t *softMapKeyA, *softMapKeyB, *softMapVal;
   F4* velScratch;             // unsmoothed velocities for the 1-ring viscosity gather
+  uint32_t* groupBox;         // per surface group (1-based): box of its particles, 6 order-preserving keys
+  uint32_t* groupNear;        // per surface group: 1 when another group's box lies within the query radius
   // misc
   uint32_t* colorFlags;       // per-pass changed flags of the dynamic coloring
   uint32_t* counters;

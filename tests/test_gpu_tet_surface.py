@@ -120,3 +120,24 @@ def test_ball_on_block_feature_contacts():
                 first[name] = k
     flush_print(f"  first rigid-feature contact frame {first}, ball z {g.positions()[ball, 2]:.3f}")
     assert "gpu" in first and first.get("gpu") == first.get("oracle")
+
+
+def test_surface_reach_skip_keeps_pairs_and_counters():
+    # two stacked blocks and a third far away: the queries skip the far
+    # block's producers; the stream and the frame's counters equal the
+    # oracle's (which queries every producer) on every step
+    flush_print("[gpu case] stacked blocks + a distant block: per-surface reach skip")
+    s, top = R.stacked_soft_blocks()
+    R.add_soft_lattice(s, (6.0, 0, 0.7), 3, 3, 3, 0.34, mu=4000.0, lam=40000.0)
+    g, o = GPUSolver(s), oracle.OracleSolver(s)
+    for k in range(150):
+        g.step()
+        o.step()
+        if k % 10 == 9 or k > 110:
+            pg, po = g.datPairs(), o.datPairs()
+            assert np.array_equal(pg, po), k
+            sg, so = g.stats(), o.clothStats()
+            assert (sg.dat_pairs, sg.dat_vt_pairs, sg.dat_nonfinite) == \
+                (so["dat_pairs"], so["dat_vt_pairs"], so["nonfinite"]), k
+    flush_print(f"  pairs at the end: {len(g.datPairs())}")
+    assert len(g.datPairs()) > 0
