@@ -282,28 +282,23 @@ __global__ void __launch_bounds__(256) k_group_box(DeviceState S, StepParams P) 
   }
 }
 
-// one block: every group's box staged in LDS (at most kMaxSkipGroups)
+// one thread per ordered pair of groups (at most kMaxSkipGroups^2)
 __global__ void __launch_bounds__(256) k_group_near(DeviceState S, StepParams P) {
-  __shared__ float box[6 * (kMaxSkipGroups + 1)];
-  for (uint32_t i = threadIdx.x; i < 6 * (P.numGroups + 1); i += blockDim.x) box[i] = key_value(S.groupBox[i]);
-  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.numGroups * P.numGroups) return;
+  const uint32_t g = i / P.numGroups + 1u, h = i % P.numGroups + 1u;
+  if (g == h) return;
   // margin: the radius, plus rounding of the distances and closest points
   // the queries compute (relative 1e-6 of the coordinates, ~16 ulp)
   const float r = P.datQueryRadius * 1.001f + 1.0e-5f;
-  for (uint32_t g = threadIdx.x + 1u; g <= P.numGroups; g += blockDim.x) {
-    bool near = false;
-    for (uint32_t h = 1; h <= P.numGroups && !near; h++) {
-      if (h == g) continue;
-      bool apart = false;
-      for (int k = 0; k < 3; k++) {
-        const float gl = box[6 * g + k], gh = box[6 * g + 3 + k], hl = box[6 * h + k], hh = box[6 * h + 3 + k];
-        const float m = r + 1.0e-6f * (fabsf(gl) + fabsf(gh) + fabsf(hl) + fabsf(hh));
-        apart = apart || hl > gh + m || hh < gl - m;
-      }
-      near = !apart;
-    }
-    if (near) S.groupNear[g] = 1u;
+  bool apart = false;
+  for (int k = 0; k < 3; k++) {
+    const float gl = key_value(S.groupBox[6 * g + k]), gh = key_value(S.groupBox[6 * g + 3 + k]);
+    const float hl = key_value(S.groupBox[6 * h + k]), hh = key_value(S.groupBox[6 * h + 3 + k]);
+    const float m = r + 1.0e-6f * (fabsf(gl) + fabsf(gh) + fabsf(hl) + fabsf(hh));
+    apart = apart || hl > gh + m || hh < gl - m;
   }
+  if (!apart) S.groupNear[g] = 1u;
 }
 
 __global__ void __launch_bounds__(256) k_copy_u32(uint32_t* dst, const uint32_t* src, uint32_t n) {
@@ -1259,7 +1254,7 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   if (P.groupSkip) {
     hipLaunchKernelGGL(k_group_clear, dim3((P.numGroups + 256) / 256), dim3(256), 0, st, S, P);
     hipLaunchKernelGGL(k_group_box, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
-    hipLaunchKernelGGL(k_group_near, dim3(1), dim3(256), 0, st, S, P);
+    hipLaunchKernelGGL(k_group_near, dim3((P.numGroups * P.numGroups + 255) / 256), dim3(256), 0, st, S, P);
   }
   launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
   hipLaunchKernelGGL(k_el_bin<false>, dim3(eb), dim3(256), 0, st, S, P);
