@@ -324,14 +324,16 @@ def drape_scene(res=24, friction=0.6, membrane_mu=300.0, bend=5.0e-4, structural
     return s
 
 
-def cloth_metrics(scene, positions):
+def cloth_metrics(scene, positions, tris=None):
     """debugClothMetrics (GPUSolver.swift:2687-2768): the minimum surface gap
     |p - closest(p, T)| - (r_p + r_T) over particles and non-adjacent
     triangles, and the largest relative elongation of stiff springs
-    (stiffness >= 1000 or hard)."""
+    (stiffness >= 1000 or hard).  `tris`: the collision triangles (a
+    solver's collisionTris(), tet boundary faces included); the scene's
+    authored triangles when omitted."""
     x = np.asarray(positions, np.float64)
     rad = np.array([b["size"][0] / 2 for b in scene.bodies])
-    tris = np.array([t[0] for t in scene.tris], np.int64)
+    tris = np.array([t[0] for t in scene.tris] if tris is None else tris, np.int64)
     nbr = {}
     for t in tris:
         for a in t:

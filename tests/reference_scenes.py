@@ -378,3 +378,49 @@ def stacked_soft_blocks():
     add_soft_lattice(s, (0, 0, 0.7), 4, 4, 4, 0.34, mu=4000.0, lam=40000.0, massPerNode=0.05)
     top = add_soft_lattice(s, (0.15, 0.1, 2.3), 3, 3, 3, 0.34, mu=4000.0, lam=40000.0, massPerNode=0.05)
     return s, top
+
+
+def add_cloth_grid(s, positions, thickness, massPerNode, friction=0.7, structuralK=5000.0, membraneMu=300.0,
+                   membraneBend=5e-4):
+    """Demos.addClothGrid (Demos7.swift:17-85) with hard structural rods and
+    membranes (its defaults): particles, hard rods along both grid
+    directions, two StVK / Bergou triangles per cell on alternating
+    diagonals.  Returns the particle grid."""
+    nu, nv = len(positions), len(positions[0])
+    grid = [[s.addParticle(thickness, massPerNode, positions[i][j], friction=friction) for j in range(nv)]
+            for i in range(nu)]
+    for i in range(nu):
+        for j in range(nv):
+            if i + 1 < nu:
+                s.addSpring(grid[i][j], grid[i + 1][j], (0, 0, 0), (0, 0, 0), structuralK, hard=True)
+            if j + 1 < nv:
+                s.addSpring(grid[i][j], grid[i][j + 1], (0, 0, 0), (0, 0, 0), structuralK, hard=True)
+    for i in range(nu - 1):
+        for j in range(nv - 1):
+            v00, v10, v01, v11 = grid[i][j], grid[i + 1][j], grid[i][j + 1], grid[i + 1][j + 1]
+            if (i + j) % 2 == 0:
+                s.addTri((v00, v10, v11), membraneMu, membraneMu, membraneBend)
+                s.addTri((v00, v11, v01), membraneMu, membraneMu, membraneBend)
+            else:
+                s.addTri((v00, v10, v01), membraneMu, membraneMu, membraneBend)
+                s.addTri((v10, v11, v01), membraneMu, membraneMu, membraneBend)
+    return grid
+
+
+def clothcombo(res=16, friction=0.8, membraneMu=300.0, bend=5e-4, blockMu=4000.0):
+    """Demos.clothcombo (Demos7.swift:520-565): a 7^3 soft block, a membrane
+    cloth dropped over it and a box and a sphere dropped on top -- tets,
+    cloth and rigid bodies in one solve."""
+    s = PhysicsScene("clothcombo", SimSettings(iterations=20, betaLin=20000.0, particleDamping=1.5,
+                                               clothViscosity=0.25, lambdaMax=1.0e6))
+    _ground(s, friction)
+    add_soft_lattice(s, (0, 0, 0.7), 7, 7, 7, 0.17, mu=blockMu, lam=10 * blockMu, massPerNode=0.0095)
+    n = max(16, res)
+    size = 2.6
+    h = size / (n - 1)
+    r = min(0.05, 0.3 * h)
+    pos = [[(i * h - size / 2, j * h - size / 2, 1.6) for j in range(n)] for i in range(n)]
+    add_cloth_grid(s, pos, r, 0.01, friction=friction, membraneMu=membraneMu, membraneBend=bend)
+    s.addBody((0.45, 0.45, 0.45), 2.0, 0.7, (-0.1, 0.1, 2.6))
+    s.addSphere(0.5, 1.5, 0.5, (0.5, -0.4, 3.1))
+    return s

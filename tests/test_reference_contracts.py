@@ -298,3 +298,21 @@ def test_soft_blocks_do_not_interpenetrate(kind):
     x.step(400)
     p = x.positions()[top]
     assert p[:, 2].min() > 0.95 and p[:, 2].mean() > 1.15
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_combined_tet_cloth_rigid_stable(kind):
+    # ClothTests.testCombinedTetClothRigidStable (:156-178): tets, a membrane
+    # cloth and rigid bodies in one solve come to rest
+    from avbd_amd import demos
+    s = R.clothcombo(16)
+    g = make(kind, s)
+    g.step(600)
+    p, _, v, _ = g.bodyStates()
+    parts = [b for b in range(len(s.bodies)) if s.bodies[b]["particle"]]
+    mass = [s.bodies[b]["density"] * 4.0 / 3.0 * np.pi * (s.bodies[b]["size"][0] / 2) ** 3 for b in parts]
+    ke = sum(0.5 * m * float(np.dot(v[b], v[b])) for m, b in zip(mass, parts))
+    gap, stretch = demos.cloth_metrics(s, p, g.collisionTris())
+    assert ke < 0.2 and gap > -0.06 and stretch < 0.02
+    for b in (len(s.bodies) - 2, len(s.bodies) - 1):
+        assert p[b, 2] > 0.1 and np.linalg.norm(v[b]) < 0.2
