@@ -424,3 +424,54 @@ def clothcombo(res=16, friction=0.8, membraneMu=300.0, bend=5e-4, blockMu=4000.0
     s.addBody((0.45, 0.45, 0.45), 2.0, 0.7, (-0.1, 0.1, 2.6))
     s.addSphere(0.5, 1.5, 0.5, (0.5, -0.4, 3.1))
     return s
+
+
+def boxoncloth(res=20, massRatio=8.0, friction=0.8, membraneMu=300.0, bend=5e-4):
+    """Demos.boxoncloth (Demos7.swift:152-194): a cloth laid over a static
+    pedestal and a box of `massRatio` times the cloth's mass dropped on it."""
+    s = PhysicsScene("boxoncloth", SimSettings(iterations=20, betaLin=20000.0, particleDamping=1.5,
+                                               clothViscosity=0.25))
+    _ground(s, friction)
+    s.addBody((1.2, 1.2, 0.8), 0.0, 0.6, (0, 0, 0.4))
+    n = max(16, res)
+    size = 2.6
+    h = size / (n - 1)
+    r = min(0.05, 0.3 * h)
+    pos = [[(i * h - size / 2, j * h - size / 2, 0.8 + r + 0.02) for j in range(n)] for i in range(n)]
+    add_cloth_grid(s, pos, r, 0.01, friction=friction, membraneMu=membraneMu, membraneBend=bend)
+    side = 0.55
+    s.addBody((side, side, side), massRatio * n * n * 0.01 / side ** 3, 0.7, (0, 0, 1.45))
+    return s
+
+
+def hammock(res=16, friction=0.8, membraneMu=300.0, bend=5e-4):
+    """Demos.hammock (Demos7.swift:200-262) with one cargo box: a sagging
+    cloth pinned to the world along both short ends."""
+    from avbd_amd.demos import SplitMix64
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("hammock", SimSettings(iterations=20, betaLin=20000.0, particleDamping=1.5,
+                                            clothViscosity=0.25))
+    _ground(s, friction)
+    nu, nv = max(16, res), max(10, res * 2 // 3)
+    h = 0.14
+    r = min(0.05, 0.3 * h)
+    span = (nu - 1) * h
+    sag = 0.22 * span
+    spanZ = sag + 1.1
+    pos = []
+    for i in range(nu):
+        t = i / (nu - 1) * 2 - 1
+        dip = sag * (1 - t * t)
+        pos.append([(i * h - span / 2, j * h - (nv - 1) * h / 2, spanZ - dip) for j in range(nv)])
+    grid = add_cloth_grid(s, pos, r, 0.012, friction=friction, membraneMu=membraneMu, membraneBend=bend)
+    for j in range(nv):
+        for i in (0, nu - 1):
+            node = grid[i][j]
+            s.addJoint(SceneJoint(-1, node, s.bodies[node]["position"], (0, 0, 0)))
+    rng = SplitMix64(7)
+    jx = (rng.nextFloat() - 0.5) * 0.2
+    jy = (rng.nextFloat() - 0.5) * 0.3
+    angle = rng.nextFloat() * 0.5
+    ax = (rng.nextFloat(), rng.nextFloat(), 1.0)
+    s.addBody((0.5, 0.5, 0.5), 8.0, 0.6, (jx, jy, spanZ + 0.6), rotation=_quat(angle, ax))
+    return s

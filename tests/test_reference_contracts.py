@@ -316,3 +316,24 @@ def test_combined_tet_cloth_rigid_stable(kind):
     assert ke < 0.2 and gap > -0.06 and stretch < 0.02
     for b in (len(s.bodies) - 2, len(s.bodies) - 1):
         assert p[b, 2] > 0.1 and np.linalg.norm(v[b]) < 0.2
+
+
+@pytest.mark.gpu
+def test_heavy_box_on_cloth_and_hammock():
+    # ClothTests.testHeavyBoxRestsOnDrapedCloth (:36-56) and
+    # testHammockInextensible (:115-128)
+    from avbd_amd import demos
+    s = R.boxoncloth(20, 8.0)
+    g = make("gpu", s)
+    g.step(480)
+    p, _, v, _ = g.bodyStates()
+    box = len(s.bodies) - 1
+    gap, _ = demos.cloth_metrics(s, p, g.collisionTris())
+    assert p[box, 2] > 1.05 and max(abs(p[box, 0]), abs(p[box, 1])) < 0.3 and np.linalg.norm(v[box]) < 0.1
+    assert gap > -0.05 and g.stats().num_soft > 10
+    s = R.hammock(16)
+    g = make("gpu", s)
+    g.step(480)
+    p = g.positions()
+    gap, stretch = demos.cloth_metrics(s, p, g.collisionTris())
+    assert stretch < 0.02 and gap > -0.06 and p[len(s.bodies) - 1, 2] > 0.5
