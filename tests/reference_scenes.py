@@ -475,3 +475,73 @@ def hammock(res=16, friction=0.8, membraneMu=300.0, bend=5e-4):
     ax = (rng.nextFloat(), rng.nextFloat(), 1.0)
     s.addBody((0.5, 0.5, 0.5), 8.0, 0.6, (jx, jy, spanZ + 0.6), rotation=_quat(angle, ax))
     return s
+
+
+def clothfold(res=36, friction=0.9, membraneMu=300.0, bend=5e-4):
+    """Demos.clothfold (Demos7.swift:90-145): a 2.2 m towel laid in an S
+    fold (two half-turns of radius rb) just above the ground."""
+    s = PhysicsScene("clothfold", SimSettings(iterations=10, betaLin=20000.0, particleDamping=1.5,
+                                              clothViscosity=0.25))
+    _ground(s, friction)
+    nu, nv = max(36, res), max(18, res // 2)
+    total = 2.2
+    h = total / (nu - 1)
+    r = min(0.045, 0.36 * h)
+    rb = max(0.055, 2.6 * r)
+    straight = (total - 2 * math.pi * rb) / 3
+
+    def profile(a):
+        a1 = straight
+        b1 = a1 + math.pi * rb
+        a2 = b1 + straight
+        b2 = a2 + math.pi * rb
+        if a < a1:
+            return a, 0.0
+        if a < b1:
+            t = (a - a1) / rb
+            return a1 + rb * math.sin(t), rb - rb * math.cos(t)
+        if a < a2:
+            return a1 - (a - b1), 2 * rb
+        if a < b2:
+            t = (a - a2) / rb
+            return a1 - straight - rb * math.sin(t), 2 * rb + rb * (1 - math.cos(t))
+        return a1 - straight + (a - b2), 4 * rb
+
+    pos = []
+    for i in range(nu):
+        x, z = profile(i * h)
+        pos.append([(x - straight / 2, j * h - (nv - 1) * h / 2, z + r + 0.02) for j in range(nv)])
+    add_cloth_grid(s, pos, r, 0.5 * h * h, friction=friction, membraneMu=membraneMu, membraneBend=bend)
+    return s
+
+
+def flagwhip(res=14, kick=3.0, membraneMu=300.0, bend=5e-4):
+    """Demos.flagwhip (Demos7.swift:307-350): a hanging sheet pinned at its
+    two top corners, kicked sideways with a velocity growing down the sheet."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("flagwhip", SimSettings(iterations=10, betaLin=20000.0, particleDamping=1.5,
+                                             clothViscosity=0.25))
+    _ground(s, 0.6)
+    n = max(12, res)
+    h = 2.0 / (n - 1)
+    r = min(0.05, 0.3 * h)
+    pos = [[(0.0, i * h - 1.0, 3.2 - j * h) for j in range(n)] for i in range(n)]
+    grid = add_cloth_grid(s, pos, r, 0.008, friction=0.6, membraneMu=membraneMu, membraneBend=bend)
+    for i in (0, n - 1):
+        node = grid[i][0]
+        s.addJoint(SceneJoint(-1, node, s.bodies[node]["position"], (0, 0, 0)))
+    for i in range(n):
+        for j in range(n):
+            s.bodies[grid[i][j]]["velocity"] = (kick * j / (n - 1), 0.0, 0.0)
+    return s
+
+
+def particle_ke(scene, v):
+    """Kinetic energy of the scene's particles (mass from the density the
+    particle was given, as Scene.addParticle derives it)."""
+    ke = 0.0
+    for b, body in enumerate(scene.bodies):
+        if body["particle"]:
+            m = body["density"] * 4.0 / 3.0 * math.pi * (body["size"][0] / 2) ** 3
+            ke += 0.5 * m * float(v[b][0] ** 2 + v[b][1] ** 2 + v[b][2] ** 2)
+    return ke

@@ -337,3 +337,32 @@ def test_heavy_box_on_cloth_and_hammock():
     p = g.positions()
     gap, stretch = demos.cloth_metrics(s, p, g.collisionTris())
     assert stretch < 0.02 and gap > -0.06 and p[len(s.bodies) - 1, 2] > 0.5
+
+
+@pytest.mark.gpu
+def test_folded_cloth_and_whipping_flag():
+    # ClothTests.testFoldedClothNoInterpenetration (:12-32) and
+    # testFlagWhipNoEnergyGrowth (:131-152)
+    from avbd_amd import demos
+    from conftest import flush_print
+    s = R.clothfold(36)
+    g = make("gpu", s)
+    g.step(600)
+    p, _, v, _ = g.bodyStates()
+    gap, stretch = demos.cloth_metrics(s, p, g.collisionTris())
+    ke = R.particle_ke(s, v)
+    flush_print(f"  fold: gap {gap:.4f}, stretch {stretch:.4f}, soft {g.stats().num_soft}, KE {ke:.2e}")
+    assert gap > -0.023 and stretch < 0.05 and g.stats().num_soft > 100 and ke < 0.01
+    s = R.flagwhip(14)
+    g = make("gpu", s)
+    windows, cur = [], 0.0
+    for f in range(1800):
+        g.step()
+        if f % 10 == 0:
+            cur = max(cur, R.particle_ke(s, g.bodyStates()[2]))
+        if (f + 1) % 600 == 0:
+            windows.append(cur)
+            cur = 0.0
+    _, stretch = demos.cloth_metrics(s, g.positions(), g.collisionTris())
+    flush_print(f"  flag: KE window maxima {windows}, stretch {stretch:.4f}")
+    assert windows[2] < windows[0] * 1.05 and stretch < 0.02
