@@ -545,3 +545,29 @@ def particle_ke(scene, v):
             m = body["density"] * 4.0 / 3.0 * math.pi * (body["size"][0] / 2) ** 3
             ke += 0.5 * m * float(v[b][0] ** 2 + v[b][1] ** 2 + v[b][2] ** 2)
     return ke
+
+
+def eecross(length=18, wid=4):
+    """Demos.eecross (Demos7.swift:456-516): strip A pinned taut just above
+    the ground, strip B across it held by four soft drag joints on its
+    leading edge.  Returns (scene, drag joints, A nodes, B nodes)."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("eecross", SimSettings(iterations=20, betaLin=20000.0, particleDamping=1.5,
+                                            clothViscosity=0.25))
+    _ground(s, 0.3)
+    h, r, zA = 0.12, 0.04, 0.12
+    posA = [[(i * h - (length - 1) * h / 2, j * h - (wid - 1) * h / 2, zA) for j in range(wid)] for i in range(length)]
+    gridA = add_cloth_grid(s, posA, r, 0.01, friction=0.3)
+    for i in (0, length - 1):
+        for j in range(wid):
+            node = gridA[i][j]
+            s.addJoint(SceneJoint(-1, node, s.bodies[node]["position"], (0, 0, 0)))
+    posB = [[(j * h - (wid - 1) * h / 2, i * h - (length - 1) * h / 2, zA + 2 * r + 0.02) for j in range(wid)]
+            for i in range(length)]
+    gridB = add_cloth_grid(s, posB, r, 0.01, friction=0.3)
+    drag = []
+    for j in range(wid):
+        node = gridB[length - 1][j]
+        drag.append(len(s.joints))
+        s.addJoint(SceneJoint(-1, node, s.bodies[node]["position"], (0, 0, 0), stiffnessLin=3000.0, stiffnessAng=0.0))
+    return s, drag, [n for row in gridA for n in row], [n for row in gridB for n in row]

@@ -366,3 +366,46 @@ def test_folded_cloth_and_whipping_flag():
     _, stretch = demos.cloth_metrics(s, g.positions(), g.collisionTris())
     flush_print(f"  flag: KE window maxima {windows}, stretch {stretch:.4f}")
     assert windows[2] < windows[0] * 1.05 and stretch < 0.02
+
+
+@pytest.mark.gpu
+def test_dragged_strips_do_not_pass_at_corners():
+    # ClothTests.testDraggedStripsDontPassAtCorners (:58-113): strip B is
+    # dragged 1.5 m across taut strip A; no B node may ever sit under an
+    # interior point of an A triangle, and E-E contacts must fire
+    from conftest import flush_print
+    s, drag, aNodes, bNodes = R.eecross()
+    g = make("gpu", s)
+    g.step(120)
+    anchors = np.array([s.joints[j].rA for j in drag], np.float64)
+    aSet = set(aNodes)
+    aTris = np.array([t[0] for t in s.tris if all(v in aSet for v in t[0])], np.int64)
+    bNodes = np.array(bNodes)
+    violations = sawEE = 0
+    for f in range(300):
+        anchors[:, 1] -= 0.30 / 60
+        g.setJointWorldAnchors(drag, anchors)
+        g.step()
+        x = g.positions().astype(np.float64)
+        p = x[bNodes]
+        p = p[(np.abs(p[:, 0]) < 1.0) & (np.abs(p[:, 1]) < 0.25) & (np.abs(p[:, 2] - 0.12) < 0.25)]
+        a, b, c = x[aTris[:, 0]], x[aTris[:, 1]], x[aTris[:, 2]]
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        for q in p:
+            sd = np.sum((q - a) * n, axis=1)
+            proj = q - n * sd[:, None]
+            v0, v1, v2 = b - a, c - a, proj - a
+            d00, d01, d11 = np.sum(v0 * v0, 1), np.sum(v0 * v1, 1), np.sum(v1 * v1, 1)
+            d20, d21 = np.sum(v2 * v0, 1), np.sum(v2 * v1, 1)
+            den = d00 * d11 - d01 * d01
+            ok = np.abs(den) > 1e-12
+            v = np.where(ok, (d11 * d20 - d01 * d21) / np.where(ok, den, 1), 0)
+            w = np.where(ok, (d00 * d21 - d01 * d20) / np.where(ok, den, 1), 0)
+            violations += int(np.sum((sd < -1e-4) & ok & (v > 0.1) & (w > 0.1) & (1 - v - w > 0.1)))
+        if f % 30 == 0:
+            sc = g.softContacts()
+            kinds = (sc["anchorA"][:, 3].view(np.uint32) >> 2) & 7
+            sawEE += int(np.sum(kinds == 3))
+    flush_print(f"  violations {violations}, E-E contacts seen {sawEE}")
+    assert violations == 0 and sawEE > 0
