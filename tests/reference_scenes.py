@@ -571,3 +571,64 @@ def eecross(length=18, wid=4):
         drag.append(len(s.joints))
         s.addJoint(SceneJoint(-1, node, s.bodies[node]["position"], (0, 0, 0), stiffnessLin=3000.0, stiffnessAng=0.0))
     return s, drag, [n for row in gridA for n in row], [n for row in gridB for n in row]
+
+
+def add_cloth(s, origin, dirU, dirV, nu, nv, spacing, thickness=0.06, massPerNode=0.02, friction=0.7):
+    """Demos.addCloth (Demos5.swift:15-74): a particle sheet spanned by
+    dirU / dirV with StVK + Bergou triangles (mu = lambda = 300, bend 5e-4)
+    on alternating diagonals and hard rods (cap 5000) along the grid; the
+    skin radius stays under 0.38 spacing."""
+    t = min(thickness, 0.38 * spacing)
+    grid = [[s.addParticle(t, massPerNode, tuple(origin[k] + dirU[k] * i * spacing + dirV[k] * j * spacing
+                                                 for k in range(3)), friction=friction)
+             for j in range(nv)] for i in range(nu)]
+    for i in range(nu - 1):
+        for j in range(nv - 1):
+            v00, v10, v01, v11 = grid[i][j], grid[i + 1][j], grid[i][j + 1], grid[i + 1][j + 1]
+            if (i + j) % 2 == 0:
+                s.addTri((v00, v10, v11), 300.0, 300.0, 5e-4)
+                s.addTri((v00, v11, v01), 300.0, 300.0, 5e-4)
+            else:
+                s.addTri((v00, v10, v01), 300.0, 300.0, 5e-4)
+                s.addTri((v10, v11, v01), 300.0, 300.0, 5e-4)
+    for i in range(nu):
+        for j in range(nv):
+            if i + 1 < nu:
+                s.addSpring(grid[i][j], grid[i + 1][j], (0, 0, 0), (0, 0, 0), 5000.0, hard=True)
+            if j + 1 < nv:
+                s.addSpring(grid[i][j], grid[i][j + 1], (0, 0, 0), (0, 0, 0), 5000.0, hard=True)
+    return grid
+
+
+def flag_on_pole(res=14):
+    """Demos.cloth(res:ball: false) (Demos5.swift:243-305): a flag hard-attached
+    at two points to a deformable pole of ten ball-jointed rigid segments
+    with finite angular stiffness.  Returns (scene, top attachment particle)."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("cloth", SimSettings(iterations=20, betaLin=20000.0, particleDamping=1.5, clothViscosity=0.25,
+                                          betaAng=20000.0, lambdaMax=500.0))
+    _ground(s, 0.8)
+    segs, segH = 10, 0.55
+    prev = top = -1
+    for k in range(segs):
+        seg = s.addBody((0.16, 0.16, segH * 0.98), 40.0, 0.5, (0, 0, (k + 0.5) * segH))
+        if k == 0:
+            s.addJoint(SceneJoint(-1, seg, (0, 0, 0), (0, 0, -segH / 2), stiffnessLin=math.inf, stiffnessAng=math.inf))
+        else:
+            s.addJoint(SceneJoint(prev, seg, (0, 0, segH / 2), (0, 0, -segH / 2), stiffnessLin=math.inf,
+                                  stiffnessAng=80000.0))
+            s.addJoint(SceneJoint(prev, seg, (0, 0, 0), (0, 0, 0), stiffnessLin=0.0, stiffnessAng=0.0))
+        prev = top = seg
+    n = max(10, res)
+    spacing = 2.6 / (n - 1)
+    u = (0.55, 0.2, -0.81)
+    un = math.sqrt(sum(c * c for c in u))
+    grid = add_cloth(s, (0.12, 0, segs * segH - 0.15), tuple(c / un for c in u), (0, 0, -1), n, n, spacing,
+                     massPerNode=0.007)
+    s.addJoint(SceneJoint(top, grid[0][0], (0.12, 0, segH / 2 - 0.15), (0, 0, 0)))
+    low = top - int(2.6 / segH)
+    s.addJoint(SceneJoint(low, grid[0][n - 1], (0.12, 0, 0), (0, 0, 0)))
+    for row in grid:
+        for node in row:
+            s.bodies[node]["velocity"] = (0.8, 0.5, 0.0)
+    return s, grid[0][0]

@@ -409,3 +409,12 @@ def test_dragged_strips_do_not_pass_at_corners():
             sawEE += int(np.sum(kinds == 3))
     flush_print(f"  violations {violations}, E-E contacts seen {sawEE}")
     assert violations == 0 and sawEE > 0
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_flag_stays_attached_to_pole(kind):
+    # SoftBodyTests.testFlagStaysAttachedToPole (:384-397)
+    s, attach = R.flag_on_pole(14)
+    x = make(kind, s)
+    x.step(500)
+    assert x.positions()[attach, 2] > 3.5
