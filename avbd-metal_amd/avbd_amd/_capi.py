@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libavbd_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_COLORS = 64
 
 # avbd_status
@@ -68,7 +68,7 @@ class SceneDesc(C.Structure):
                 ("num_tris", C.c_uint32), ("tri_ids", U32P), ("tri_mu", FP), ("tri_lambda", FP),
                 ("tri_bend", FP),
                 ("num_exclusions", C.c_uint32), ("exclusion_pairs", U32P),
-                ("collider_hull_count", U32P), ("hull_vertices", FP)]
+                ("collider_hull_count", U32P), ("hull_vertices", FP), ("joint_motor_rate", FP)]
 
 
 class Options(C.Structure):
@@ -137,6 +137,7 @@ SIGNATURES = {
     "avbd_get_cloth_info": (C.c_int, [C.c_void_p, C.POINTER(ClothInfo), U32P, U32P]),
     "avbd_get_dat_pairs": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
     "avbd_get_collision_tris": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
+    "avbd_get_motor_targets": (C.c_int, [C.c_void_p, C.c_uint32, FP]),
     "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),

@@ -71,6 +71,7 @@ class SceneJoint:
     armature: float = 0.0
     limitLo: float = 1.0
     limitHi: float = -1.0
+    motorRate: float = 0.0   # legacy servo rate (rad/s) of an implicit-PD motor (Scene.swift:300-314)
 
 
 @dataclass
@@ -174,6 +175,9 @@ class PhysicsScene:
     def addJoint(self, j: SceneJoint):
         if j.motorTorque > 0:
             assert j.hingeAxis is not None, "a powered joint must define a hinge axis"
+        assert math.isfinite(j.motorRate), "motor rate must be finite"
+        if j.motorRate != 0 and j.motorTorque > 0:
+            assert j.motorMode == 0, "legacy motor rates require implicit position PD"
         self.joints.append(j)
         return len(self.joints) - 1
 
@@ -262,6 +266,8 @@ class PhysicsScene:
             d.joint_motor = ptr(arr([(j.motorTarget, j.motorTorque, j.motorStiffness, j.motorDamping, j.armature,
                                       j.motorMode) for j in J], np.float32), C.c_float)
             d.joint_limits = ptr(arr([(j.limitLo, j.limitHi) for j in J], np.float32), C.c_float)
+            if any(j.motorRate != 0 for j in J):
+                d.joint_motor_rate = ptr(arr([j.motorRate for j in J], np.float32), C.c_float)
         S = self.springs
         d.num_springs = len(S)
         if S:

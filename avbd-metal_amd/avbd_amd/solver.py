@@ -286,6 +286,12 @@ class GPUSolver:
         out["particles"] = parts[:info.num_particles]
         return out
 
+    def motorTargets(self, n):
+        """motorTargetForTesting: the motor targets of joints [0, n)."""
+        t = np.zeros(max(1, n), np.float32)
+        self._check(self._lib.avbd_get_motor_targets(self._h, n, _fp(t)), "motor targets")
+        return t[:n]
+
     def collisionTris(self):
         """Collision triangles: the authored ones, then the tet boundary faces (GPUSolver.swift:872-897)."""
         n = C.c_uint32(0)

@@ -26,7 +26,9 @@ void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, 
 void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
 // element grid + V-T / E-E pair stream at the current (predictor) pose
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
-constexpr uint32_t kMaxSkipGroups = 1024;  // the per-surface reach test runs up to this many surfaces
+constexpr uint32_t kMaxSkipGroups = 1024;
+void launch_rate_motors(hipStream_t st, const DeviceState& S, const uint32_t* joints, const float* rates, uint32_t n,
+                        float dt);  // the per-surface reach test runs up to this many surfaces
 // OGC records from the pair stream, soft counts, soft persistence map rebuild
 void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 // Planar-DAT truncation: the full pass after the predictor (anchor = step

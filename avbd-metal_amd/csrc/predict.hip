@@ -120,4 +120,24 @@ void launch_warmstart(hipStream_t st, const DeviceState& S, const StepParams& P)
     hipLaunchKernelGGL(k_warmstart_bodies, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
 }
 
+// Legacy servo rates (GPUSolver.swift:4407-4416): before the frame, each
+// rate motor's target advances by rate * dt and is wrapped to [-pi, pi] by
+// subtracting 2 pi times the nearest integer (ties away from zero); single
+// roundings, no contraction, as the host arithmetic it restates.
+__global__ void __launch_bounds__(256) k_rate_motors(DeviceState S, const uint32_t* joints, const float* rates,
+                                                     uint32_t n, float dt) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  JointRec& j = S.joints[joints[k]];
+  const float twoPi = 6.28318530717958647692f;
+  float target = __fadd_rn(j.motor.x, __fmul_rn(rates[k], dt));
+  target = __fsub_rn(target, __fmul_rn(twoPi, roundf(__fdiv_rn(target, twoPi))));
+  j.motor.x = target;
+}
+
+void launch_rate_motors(hipStream_t st, const DeviceState& S, const uint32_t* joints, const float* rates, uint32_t n,
+                        float dt) {
+  hipLaunchKernelGGL(k_rate_motors, dim3((n + 255) / 256), dim3(256), 0, st, S, joints, rates, n, dt);
+}
+
 }  // namespace avbd

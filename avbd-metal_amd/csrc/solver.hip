@@ -64,6 +64,10 @@ struct avbd_solver {
   uint32_t numContactEdges = 0;  // authored-triangle edges: the E-E contact prefix of the edge list
   bool pairStream = false;       // V-T / E-E pair stream and its contacts (some pair can be accepted)
   bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
+  std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
+  std::vector<float> rateValues;
+  uint32_t* rateJointDev = nullptr;
+  float* rateValueDev = nullptr;
   uint32_t numGroups = 0;        // connected collision surfaces
   bool groupSkip = false;        // the queries skip surfaces with no other surface in reach
   uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
@@ -244,6 +248,8 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   uint32_t zn[5] = {CTR_TOTAL, s->gridHashSize, s->gridHashSize, s->nb, s->fineCellSize > 0.0f ? s->gridHashSize : 0u};
   launch_zero_multi(st, zp, zn, 5);
   if (s->datOn) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
+  if (!s->rateJoints.empty())
+    launch_rate_motors(st, S, s->rateJointDev, s->rateValueDev, (uint32_t)s->rateJoints.size(), P.dt);
   launch_broadphase(st, S, P);
   stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
@@ -1110,6 +1116,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     float motorTorque = mo ? mo[1] : 0.0f;
     float lo = d->joint_limits ? d->joint_limits[2 * i] : 1.0f;
     float hi = d->joint_limits ? d->joint_limits[2 * i + 1] : -1.0f;
+    const float rate = d->joint_motor_rate ? d->joint_motor_rate[i] : 0.0f;
+    if (!std::isfinite(rate)) return AVBD_ERR_INVALID_ARGUMENT;
     if (hinge) {
       V3 ax = normalize(rd3(d->joint_hinge_axis, i));
       g.hingeAxis = f4(ax, 1.0f);
@@ -1119,6 +1127,11 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
         g.limits = F4{lo, hi, mo[3], 0.0f};
         int mode = (int)mo[5];
         g.header.w |= (mode == 1 ? kMotorExplicitPD : (mode == 2 ? kMotorVelocity : kMotorImplicitPD));
+        if (rate != 0.0f) {
+          if (mode != 0) return AVBD_ERR_INVALID_ARGUMENT;  // legacy rates need implicit position PD
+          s->rateJoints.push_back(i);
+          s->rateValues.push_back(rate);
+        }
       }
       if (lo < hi && motorTorque == 0.0f) g.limits = F4{lo, hi, 0.0f, 0.0f};
     } else if (motorTorque > 0.0f) {
@@ -1414,6 +1427,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.colorFlags, 32) ALLOC(S.counters, CTR_TOTAL)
   ALLOC(S.hashedRigid, std::max<size_t>(1, hashedRigid.size()))
   ALLOC(S.clothSlots, cslots.size())
+  ALLOC(s->rateJointDev, std::max<size_t>(1, s->rateJoints.size()))
+  ALLOC(s->rateValueDev, std::max<size_t>(1, s->rateValues.size()))
   {
     const bool cl = s->nTris > 0;
     const uint32_t nElem = cl ? s->nTris + s->nEdges : 0u, nProd = cl ? s->nParticles + s->nEdges : 0u;
@@ -1463,7 +1478,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
       upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||
       upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.tetSlots, slots) ||
-      upload(S.hashedRigid, hashedRigid) || upload(S.clothSlots, cslots) || upload(S.tris, ch.tris) ||
+      upload(S.hashedRigid, hashedRigid) || upload(S.clothSlots, cslots) ||
+      upload(s->rateJointDev, s->rateJoints) || upload(s->rateValueDev, s->rateValues) || upload(S.tris, ch.tris) ||
       upload(S.edges, ch.edges) || upload(S.nbrStart, ch.nbrStart) || upload(S.nbrCount, ch.nbrCount) ||
       upload(S.nbrList, ch.nbrList) || upload(S.particleIdx, ch.particles) ||
       upload(S.selfCollision, ch.selfCollision) ||
@@ -2261,6 +2277,15 @@ extern "C" avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info
   if (particles && s->nParticles &&
       hipMemcpy(particles, s->S.particleIdx, 4 * (size_t)s->nParticles, hipMemcpyDeviceToHost))
     return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_motor_targets(avbd_solver* s, uint32_t n, float* targets) {
+  if (!s || (n && !targets) || n > s->nj) return AVBD_ERR_INVALID_ARGUMENT;
+  sync_all(s);
+  std::vector<JointRec> j(n);
+  if (n && hipMemcpy(j.data(), s->S.joints, sizeof(JointRec) * n, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  for (uint32_t i = 0; i < n; i++) targets[i] = j[i].motor.x;
   return AVBD_OK;
 }
 

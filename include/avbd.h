@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 3
+#define AVBD_ABI_VERSION 4
 #define AVBD_MAX_COLORS 64
 
 /* ---- status codes ------------------------------------------------------ */
@@ -178,6 +178,11 @@ typedef struct avbd_scene_desc {
      * collides with boxes only (30_narrowphase.metal:404-410).  NULL -> none. */
     const uint32_t* collider_hull_count;  /* [m] */
     const float* hull_vertices;           /* [sum(count)*3] */
+    /* legacy servo rate (rad/s) of implicit-PD hinge motors: before every
+     * frame the target advances by rate*dt, wrapped to [-pi, pi]
+     * (SceneJoint.motorRate, Scene.swift:300-314; GPUSolver.swift:1154-1158,
+     * 4359-4416); NULL -> 0 */
+    const float* joint_motor_rate;        /* [j] */
 } avbd_scene_desc;
 
 /* Solver creation options (GPUSolver.init maxPairsPerBody, GPUSolver.swift:331) */
@@ -261,6 +266,9 @@ avbd_status avbd_set_joint_world_anchors(avbd_solver* s, const uint32_t* joints,
 avbd_status avbd_get_motor_states(avbd_solver* s, const uint32_t* joints, uint32_t n, float* angle,
                                   float* velocity);
 /* debugBrokenJoints / debugMaxLambda -- GPUSolver.swift:2652-2672 */
+/* Motor targets (JointGPU.motor.x) of joints [0, n): motorTargetForTesting. */
+avbd_status avbd_get_motor_targets(avbd_solver* s, uint32_t n, float* targets);
+
 avbd_status avbd_get_joint_diagnostics(avbd_solver* s, int32_t* broken, float* max_lambda_lin,
                                        float* max_lambda_ang);
 

@@ -60,6 +60,7 @@ def load():
         "oracle_cell_hash": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32]),
         "oracle_get_cloth_info": (C.c_int, [vp, vp, u32p, u32p]),
         "oracle_get_collision_tris": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
+        "oracle_get_motor_targets": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_float)]),
         "oracle_get_dat_pairs": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
         "oracle_get_soft_contacts": (C.c_int, [vp, C.c_uint32, u32p, vp]),
         "oracle_cloth_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p]),
@@ -180,6 +181,11 @@ class OracleSolver:
         out = np.zeros(len(q), np.int32)
         self._lib.oracle_map_find(self._h, _p(q, C.c_uint32), len(q), _p(out, C.c_int32))
         return out
+
+    def motorTargets(self, n):
+        t = np.zeros(max(1, n), np.float32)
+        self._lib.oracle_get_motor_targets(self._h, n, _p(t, C.c_float))
+        return t[:n]
 
     def collisionTris(self):
         n = C.c_uint32(0)

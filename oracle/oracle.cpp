@@ -186,6 +186,7 @@ struct Manifold {
 };
 template <typename T>
 struct Joint {
+  float rate;  // legacy servo rate of an implicit-PD motor (0: none)
   uint32_t a, b, broken, flags;
   V<T> rA, rB;
   T stiffLin, stiffAng, torqueArm, fracture;
@@ -466,6 +467,7 @@ struct Oracle : oracle_base {
         j.armature = mo ? mo[4] : 0.0f;
         if (mo && mo[1] > 0) {
           j.motorTarget = mo[0]; j.motorTorque = mo[1]; j.kp = mo[2]; j.kd = mo[3];
+          j.rate = d->joint_motor_rate ? d->joint_motor_rate[i] : 0.0f;
           j.limLo = lo; j.limHi = hi;
           int mode = (int)mo[5];
           j.flags |= (mode == 1 ? (2u << 4) : (mode == 2 ? (3u << 4) : (1u << 4)));
@@ -2653,6 +2655,15 @@ struct Oracle : oracle_base {
   }
 
   void step() {
+    // legacy servo rates advance their targets before the frame, wrapped to
+    // [-pi, pi] in Float (GPUSolver.swift:4359-4416)
+    for (auto& j : joints)
+      if (j.rate != 0.0f) {
+        const float twoPi = 6.28318530717958647692f;
+        float t = (float)j.motorTarget + j.rate * (float)set.dt;
+        t = t - twoPi * std::round(t / twoPi);
+        j.motorTarget = (T)t;
+      }
     frame++;
     broadphase();
     manifolds.assign(pairs.size(), Manifold<T>{});
@@ -2947,6 +2958,14 @@ int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* 
   if (o->f) go(o->f); else go(o->d);
   return 0;
 }
+int oracle_get_motor_targets(oracle_solver* o, uint32_t n, float* targets) {
+  auto go = [&](auto* s) {
+    for (uint32_t i = 0; i < n && i < s->joints.size(); i++) targets[i] = (float)s->joints[i].motorTarget;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+
 int oracle_get_collision_tris(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* tris) {
   auto go = [&](auto* s) {
     *n = (uint32_t)s->tris.size();

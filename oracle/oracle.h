@@ -47,6 +47,7 @@ int oracle_map_find(oracle_solver* o, const uint32_t* pairs, uint32_t n, int32_t
  * element contacts of the last frame (112-byte records as the backend's) and counts */
 int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
 int oracle_get_collision_tris(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* tris);
+int oracle_get_motor_targets(oracle_solver* o, uint32_t n, float* targets);
 int oracle_get_dat_pairs(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* pairs3);
 int oracle_get_soft_contacts(oracle_solver* o, uint32_t capacity, uint32_t* n, void* records);
 int oracle_dat_pairs_at(oracle_solver* o, const float* pos3, uint32_t capacity, uint32_t* n, uint32_t* pairs3);

@@ -632,3 +632,13 @@ def flag_on_pole(res=14):
         for node in row:
             s.bodies[node]["velocity"] = (0.8, 0.5, 0.0)
     return s, grid[0][0]
+
+
+def legacy_rate_motor(dt=0.25, rate=2.0):
+    """testLegacyRateMotorAdvancesWrappedServoTarget (GPUSolverTests.swift:148-168)."""
+    from avbd_amd.scene import SceneJoint
+    s = PhysicsScene("legacy-rate-motor", SimSettings(dt=dt, gravity=0.0))
+    b = s.addBody((0.2, 0.2, 0.2), 1.0, 0.0, (0, 0, 0))
+    s.addJoint(SceneJoint(-1, b, (0, 0, 0), (0, 0, 0), stiffnessLin=math.inf, stiffnessAng=math.inf,
+                          hingeAxis=(0, 0, 1), motorTorque=10.0, motorStiffness=400.0, motorRate=rate))
+    return s

@@ -418,3 +418,20 @@ def test_flag_stays_attached_to_pole(kind):
     x = make(kind, s)
     x.step(500)
     assert x.positions()[attach, 2] > 3.5
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_legacy_rate_motor_advances_wrapped_target(kind):
+    # GPUSolverTests.testLegacyRateMotorAdvancesWrappedServoTarget (:148-168),
+    # and the wrap to [-pi, pi] over later frames against float64
+    x = make(kind, R.legacy_rate_motor())
+    assert x.motorTargets(1)[0] == 0.0
+    x.step()
+    assert abs(x.motorTargets(1)[0] - 0.5) <= 1e-6
+    want = 0.5
+    for _ in range(12):
+        x.step()
+        want += 0.5
+        want -= 2 * np.pi * np.floor(want / (2 * np.pi) + 0.5)
+        assert abs(x.motorTargets(1)[0] - want) < 1e-5
+        assert -np.pi - 1e-6 <= x.motorTargets(1)[0] <= np.pi + 1e-6
