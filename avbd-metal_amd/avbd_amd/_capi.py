@@ -3,8 +3,10 @@
 The HIP library is the only compute path: if it is missing this module raises,
 it never substitutes a CPU implementation.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
@@ -179,6 +181,26 @@ WS_FORCE = 4
 WS_SMOOTH_SHIFT = 8
 
 _lib = None
+
+
+# Handles still alive when the interpreter exits are closed by an atexit
+# hook, snapshots first, then solvers, then networks (a solver may hold a
+# network attached as its warm start), while the HIP runtime is still up.
+_live = {"snapshot": weakref.WeakSet(), "solver": weakref.WeakSet(), "network": weakref.WeakSet()}
+
+
+def track(kind, obj):
+    _live[kind].add(obj)
+
+
+@atexit.register
+def _close_all():
+    for kind in ("snapshot", "solver", "network"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 def load(path=None):

@@ -61,8 +61,15 @@ class WarmStartMLP:
         o, n = C.c_int32(), C.c_int32()
         self._lib.avbd_mlp_info(h, C.byref(o), C.byref(n))
         self.outputs, self.numParams = o.value, n.value
+        _capi.track("network", self)
 
     def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
         self.close()
 
     def close(self):

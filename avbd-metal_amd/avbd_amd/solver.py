@@ -47,8 +47,15 @@ class GPUSolver:
         if st != _capi.OK:
             self._h = None
             raise _capi.AVBDError(st, "avbd_create")
+        _capi.track("solver", self)
 
     def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
         self.close()
 
     def close(self):
@@ -423,11 +430,21 @@ class RigidSpeculationSnapshot:
         self._solver = solver
         self._h = handle
         self._lib = solver._lib
+        _capi.track("snapshot", self)
 
     def __del__(self):
+        self.close()
+
+    def close(self):
         if getattr(self, "_h", None):
             self._lib.avbd_destroy_speculation_snapshot(self._h)
             self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def version():

@@ -725,6 +725,7 @@ __device__ void dat_reduce_pair(const DeviceState& S, const StepParams& P, const
 // one thread per pair of the stream in use; `color` = kDatFullPass after
 // the predictor, else the static color whose primal just ran
 __global__ void __launch_bounds__(256) k_dat_reduce(DeviceState S, StepParams P, const F4* anchor, uint32_t color) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
   const uint32_t total = S.datStart[P.numParticles + P.numEdges];
   const uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -735,6 +736,7 @@ __global__ void __launch_bounds__(256) k_dat_reduce(DeviceState S, StepParams P,
 // snapshot); color pass: the color's bodies from colorWork[start, start+count)
 __global__ void __launch_bounds__(256) k_dat_apply(DeviceState S, StepParams P, const F4* anchor, uint32_t full,
                                                    uint32_t start, uint32_t count) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t v;
   if (full) {
@@ -771,6 +773,7 @@ __global__ void __launch_bounds__(256) k_dat_apply(DeviceState S, StepParams P, 
   if (truncated) {
     const F4 np = f4(x0 + d, cur.w);
     S.posLin[v] = np;
+    S.posLo[v] = F4{0.0f, 0.0f, 0.0f, 0.0f};
     if (full) S.posLinSnap[v] = np;
     atomicAdd(&S.counters[CTR_DAT_TRUNC], 1u);
   }
@@ -789,7 +792,10 @@ __global__ void __launch_bounds__(256) k_dat_restore(DeviceState S, StepParams P
   if (b >= P.numBodies || S.shape[b].w >= 0.0f) return;
   const bool failed = S.counters[CTR_SOFT_CANDIDATES] > P.maxSoft || S.counters[CTR_DAT_PAIRS] > P.maxDatPairs ||
                       S.counters[CTR_DAT_OVERFLOW] > 0u || S.counters[CTR_DAT_INVALID] > 0u;
-  if (failed) S.posLin[b] = f4(xyz(S.initLin[b]), S.posLin[b].w);
+  if (failed) {
+    S.posLin[b] = f4(xyz(S.initLin[b]), S.posLin[b].w);
+    S.posLo[b] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -78,7 +78,21 @@ enum Counter : uint32_t {
   CTR_DAT_TRUNC,           // surface particles truncated by a Planar-DAT pass
   CTR_DAT_VT_DEGEN,        // coincident V-T anchors (fallback normal)
   CTR_DAT_EE_DEGEN,        // coincident / parallel E-E anchors (fallback normal)
-  CTR_TOTAL = 16 + 2 * kMaxColors + 12
+  // residual-terminated frames: chunks of sweeps each end with a residual
+  // check; once CTR_CONV_DONE is set every later sweep kernel of the frame
+  // returns at entry
+  CTR_CONV_DONE,           // 0 iterating, 1 residual <= tolerance, 2 iteration cap reached
+  CTR_CONV_SEQ,            // residual checks completed this frame
+  CTR_CONV_ITERS,          // sweeps run this frame
+  CTR_CONV_RES,            // residual of the last check (f32 bits)
+  CTR_TOTAL = 16 + 2 * kMaxColors + 16
+};
+
+// Progress of a residual-terminated frame, written by the device into mapped
+// host memory after every residual check (seq last, with system-scope release).
+struct ConvHost {
+  uint32_t seq, done, iters;
+  float residual;
 };
 
 // ---------------------------------------------------------------------------
@@ -109,6 +123,19 @@ AVBD_HD V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
 AVBD_HD V3& operator-=(V3& a, V3 b) { a = a - b; return a; }
 AVBD_HD V3& operator*=(V3& a, float s) { a = a * s; return a; }
 AVBD_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Error-free sum (Knuth's TwoSum): s = f32(a + b) and a + b = s + e exactly.
+// Additions only, so contraction cannot fuse any of it.
+AVBD_HD void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+AVBD_HD void two_sum3(V3 a, V3 b, V3& s, V3& e) {
+  two_sum(a.x, b.x, s.x, e.x);
+  two_sum(a.y, b.y, s.y, e.y);
+  two_sum(a.z, b.z, s.z, e.z);
+}
 AVBD_HD V3 cross(V3 a, V3 b) {
   return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }

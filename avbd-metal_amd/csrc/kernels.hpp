@@ -26,9 +26,10 @@ void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, 
 void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
 // element grid + V-T / E-E pair stream at the current (predictor) pose
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
-constexpr uint32_t kMaxSkipGroups = 1024;
+constexpr uint32_t kMaxSkipGroups = 1024;  // the per-surface reach test runs up to this many surfaces
+// legacy servo rates: each listed joint's motor target advances by rate * dt, wrapped to [-pi, pi]
 void launch_rate_motors(hipStream_t st, const DeviceState& S, const uint32_t* joints, const float* rates, uint32_t n,
-                        float dt);  // the per-surface reach test runs up to this many surfaces
+                        float dt);
 // OGC records from the pair stream, soft counts, soft persistence map rebuild
 void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 // Planar-DAT truncation: the full pass after the predictor (anchor = step
@@ -61,7 +62,10 @@ void launch_primal_tail(hipStream_t st, const DeviceState& S, const StepParams& 
 constexpr uint32_t kDynamicColorLaunches = 8;  // dynamic palette: colors launched one kernel each
 void launch_dual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 void launch_finalize(hipStream_t st, const DeviceState& S, const StepParams& P);
-void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks);
+// guard: skip once a residual-terminated frame has ended its iterations
+void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks, bool guard = false);
+void launch_residual_check(hipStream_t st, const DeviceState& S, uint32_t blocks, float tol, uint32_t maxIt,
+                           uint32_t n, ConvHost* host);
 void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t* out, uint32_t gridCap);
 // persistent cooperative sweep: all iterations x colors x dual in one launch
 hipError_t launch_persistent_solve(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t numColors,

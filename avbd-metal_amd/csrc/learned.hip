@@ -73,9 +73,9 @@ __global__ void __launch_bounds__(256) k_ws_features(DeviceState S, StepParams P
 }
 
 // sums of the residual partials: |g|^2 into res[slot], |m g_grav|^2 into res[2]
-__global__ void __launch_bounds__(256) k_ws_reduce(const float* partial, uint32_t blocks, float* res, int slot) {
-  __shared__ float red[2][256];
-  float g2 = 0.0f, w2 = 0.0f;
+__global__ void __launch_bounds__(256) k_ws_reduce(const double* partial, uint32_t blocks, float* res, int slot) {
+  __shared__ double red[2][256];
+  double g2 = 0.0, w2 = 0.0;
   for (uint32_t b = threadIdx.x; b < blocks; b += 256) {
     g2 += partial[2 * b];
     w2 += partial[2 * b + 1];
@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(256) k_ws_reduce(const float* partial, uint32_
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    res[slot] = red[0][0];
-    res[2] = red[1][0];
+    res[slot] = (float)red[0][0];
+    res[2] = (float)red[1][0];
   }
 }
 
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_ws_target(DeviceState S, WarmState W) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W.n) return;
   const uint32_t b = W.idx[i];
-  const V3 t = (xyz(S.posLin[b]) - xyz(W.guess[i])) * (1.0f / W.L);
+  const V3 t = ((xyz(S.posLin[b]) - xyz(W.guess[i])) + xyz(S.posLo[b])) * (1.0f / W.L);
   W.corrPrev[b] = W.corr[b];
   W.corr[b] = f4(t, 0.0f);
   if (W.log) {
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_ws_target_rigid(DeviceState S, WarmStat
   if (i >= W.nR) return;
   const uint32_t b = W.ridx[i];
   const float invL = 1.0f / W.L;
-  const V3 tl = (xyz(S.posLin[b]) - xyz(W.rGuessLin[i])) * invL;
+  const V3 tl = ((xyz(S.posLin[b]) - xyz(W.rGuessLin[i])) + xyz(S.posLo[b])) * invL;
   const V3 ta = q_sub(S.posAng[b], W.rGuessAng[i]) * (rigid_radius(S, b) * invL);
   W.corr[b] = f4(tl, 0.0f);
   W.corrAng[b] = f4(ta, 0.0f);
@@ -247,7 +247,7 @@ void launch_ws_features(hipStream_t st, const DeviceState& S, const StepParams& 
   if (W.n) hipLaunchKernelGGL(k_ws_features, grid_n(W.n), dim3(256), 0, st, S, P, W);
 }
 void launch_ws_reduce(hipStream_t st, const DeviceState& S, uint32_t blocks, const WarmState& W, int slot) {
-  hipLaunchKernelGGL(k_ws_reduce, dim3(1), dim3(256), 0, st, (const float*)S.residualPartial, blocks, W.res, slot);
+  hipLaunchKernelGGL(k_ws_reduce, dim3(1), dim3(256), 0, st, (const double*)S.residualPartial, blocks, W.res, slot);
 }
 void launch_ws_apply(hipStream_t st, const DeviceState& S, const WarmState& W) {
   if (!W.n) return;

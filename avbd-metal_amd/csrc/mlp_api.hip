@@ -7,9 +7,14 @@
 #include <vector>
 
 #include "../../include/avbd.h"
+#include "lifetime.hpp"
 #include "mlp.hpp"
 
 using namespace avbd;
+
+namespace avbd {
+void release_network(avbd_mlp* m);
+}
 
 namespace {
 
@@ -95,7 +100,7 @@ extern "C" avbd_status avbd_mlp_create(uint64_t seed, int32_t outputs, avbd_mlp*
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return AVBD_ERR_NO_DEVICE;
   avbd_mlp* m = new avbd_mlp();
   auto fail = [&](avbd_status st) {
-    avbd_mlp_destroy(m);
+    release_network(m);
     return st;
   };
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return fail(AVBD_ERR_HIP);
@@ -128,11 +133,14 @@ extern "C" avbd_status avbd_mlp_create(uint64_t seed, int32_t outputs, avbd_mlp*
   fill(kMlpOffW3, kMlpHidden, kMlpHidden * outputs, 0.1f);
   avbd_status st = avbd_mlp_set_params(m, p.data());
   if (st != AVBD_OK) return fail(st);
+  live_add(m, HandleKind::Network);
   *out = m;
   return AVBD_OK;
 }
 
-extern "C" void avbd_mlp_destroy(avbd_mlp* m) {
+namespace avbd {
+
+void release_network(avbd_mlp* m) {
   if (!m) return;
   if (m->stream) hipStreamSynchronize(m->stream);
   release(m->trainAllocs);
@@ -140,6 +148,13 @@ extern "C" void avbd_mlp_destroy(avbd_mlp* m) {
   release(m->allocs);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
+}
+
+}  // namespace avbd
+
+extern "C" void avbd_mlp_destroy(avbd_mlp* m) {
+  // a handle already released at process exit is not touched again
+  if (m && live_remove(m)) release_network(m);
 }
 
 extern "C" avbd_status avbd_mlp_info(const avbd_mlp* m, int32_t* outputs, int32_t* num_params) {

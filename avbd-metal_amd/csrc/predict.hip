@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(256) k_warmstart_bodies(DeviceState S, StepPar
     S.posLin[gid] = newL;
     S.posAng[gid] = newA;
   }
+  S.posLo[gid] = F4{0.0f, 0.0f, 0.0f, 0.0f};
   S.posLinSnap[gid] = newL;
   S.posAngSnap[gid] = newA;
 }

@@ -438,7 +438,7 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   acc_zero(acc);
   if (active) {
     PoseView pv{S.posLin, S.posAng, S.posLinSnap, S.posAngSnap, true};
-    const V3 xs = xyz(S.posLin[body]);
+    const V3 xs = xyz(S.posLin[body]), ls = xyz(S.posLo[body]);
     const uint32_t ss = work.y, sc = work.z;
     const uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
     if (SOFT) {
@@ -453,18 +453,23 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
           uint32_t k = base + i * L;
           if (k < sc) r[i] = S.tetSlots[ss + k];
         }
-        V3 x[B][3];
+        V3 x[B][3], lo[B][3];
 #pragma unroll
         for (int i = 0; i < B; i++) {
           if (base + i * L < sc) {
             x[i][0] = xyz(S.posLin[r[i].other.x]);
             x[i][1] = xyz(S.posLin[r[i].other.y]);
             x[i][2] = xyz(S.posLin[r[i].other.z]);
+            lo[i][0] = xyz(S.posLo[r[i].other.x]);
+            lo[i][1] = xyz(S.posLo[r[i].other.y]);
+            lo[i][2] = xyz(S.posLo[r[i].other.z]);
           }
         }
 #pragma unroll
         for (int i = 0; i < B; i++)
-          if (base + i * L < sc) stamp_tet_slot_x(r[i], xs, x[i][0], x[i][1], x[i][2], acc);
+          if (base + i * L < sc)
+            stamp_tet_slot_e(r[i], split_edge(x[i][0], lo[i][0], xs, ls), split_edge(x[i][1], lo[i][1], xs, ls),
+                             split_edge(x[i][2], lo[i][2], xs, ls), acc);
       }
       // contact entries continue the lane round-robin after the tets
       for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L) {
@@ -473,7 +478,7 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
           stamp_soft(S.soft[entry & kSoftIndexMask], body, (entry >> kSoftSameShift) & 15u, pv, S.initLin, S.initAng,
                      P.alpha, acc);
         else
-          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc);
+          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc, ls);
       }
     } else if (MODE == kModeParticle) {
       // particles: cloth slots inline, any other static entry generically
@@ -481,7 +486,7 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
         uint32_t entry = S.sAdjList[ss + k];
         uint32_t kind = entry >> kAdjShift;
         if (kind >= FK_MEMBRANE) stamp_cloth_slot(kind, S.clothSlots[ss + k], pv, xs, S.springs, P.alpha, acc);
-        else stamp_entry(S, pv, entry, ss + k, body, xs, false, P.alpha, P.dt, acc);
+        else stamp_entry(S, pv, entry, ss + k, body, xs, ls, false, P.alpha, P.dt, acc);
       }
       for (uint32_t k = (lane + L - (sc % L)) % L; k < dc; k += L) {
         uint32_t entry = S.dAdjList[ds + k];
@@ -489,12 +494,12 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
           stamp_soft(S.soft[entry & kSoftIndexMask], body, (entry >> kSoftSameShift) & 15u, pv, S.initLin, S.initAng,
                      P.alpha, acc);
         else
-          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc);
+          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc, ls);
       }
     } else {
       for (uint32_t k = lane; k < sc + dc; k += L) {
         uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
-        stamp_entry(S, pv, entry, ss + k, body, xs, rigid, P.alpha, P.dt, acc);
+        stamp_entry(S, pv, entry, ss + k, body, xs, ls, rigid, P.alpha, P.dt, acc);
       }
     }
   }
@@ -519,10 +524,11 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   }
   if (!active || lane != 0) return;
   F4 pl = S.posLin[body];
+  const V3 lo = xyz(S.posLo[body]);
   float mass = pl.w, dt2 = P.dt * P.dt;
   float md = mass / dt2;
   acc.l00 += md; acc.l11 += md; acc.l22 += md;
-  acc.gl += (xyz(pl) - xyz(S.inertLin[body])) * md;
+  acc.gl += ((xyz(pl) - xyz(S.inertLin[body])) + lo) * md;
   V3 dxL, dxA = v3s(0);
   float rad = fabsf(S.shape[body].w);
   if (rigid) {
@@ -542,7 +548,11 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   float maxLin = 0.35f * rad;
   float l2 = dot(dxL, dxL);
   if (l2 > maxLin * maxLin) dxL *= maxLin * rsq(l2);
-  S.posLin[body] = f4(xyz(pl) + dxL, mass);
+  // (hi, lo) + dx, renormalised exactly: hi = f32(hi + t), lo = the error term
+  V3 hiN, loN;
+  two_sum3(xyz(pl), lo + dxL, hiN, loN);
+  S.posLin[body] = f4(hiN, mass);
+  S.posLo[body] = f4(loN, 0.0f);
   if (rigid) S.posAng[body] = q_addw(S.posAng[body], dxA);
 }
 
@@ -564,6 +574,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
 template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count,
                                                 uint32_t base) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
   const uint32_t groupsPerBlock = 256 / L;
   uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   uint32_t slot = blk * groupsPerBlock + threadIdx.x / L;
@@ -591,6 +602,7 @@ __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uin
 // does not depend on which kernel processed a color.
 template <int L>
 __global__ void __launch_bounds__(256) k_primal_tail(DeviceState S, StepParams P, uint32_t first) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;
   const uint32_t groupsPerBlock = 256 / L;
   const uint32_t lane = threadIdx.x & (L - 1), grp = threadIdx.x / L;
   const uint32_t used = S.colorStart[kMaxColors + 1];
@@ -789,6 +801,7 @@ __device__ __forceinline__ void dual_item(const DeviceState& S, const StepParams
 }
 
 __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;
   uint32_t numPairs = S.counters[CTR_PAIRS];
   uint32_t total = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
   uint32_t stride = gridDim.x * blockDim.x;
@@ -818,7 +831,7 @@ __global__ void __launch_bounds__(256) k_finalize(DeviceState S, StepParams P) {
   S.prevVelLin[gid] = vOld;
   F4 pl = S.posLin[gid];
   if (pl.w <= 0.0f) return;
-  V3 v = (xyz(pl) - xyz(S.initLin[gid])) / P.dt;
+  V3 v = ((xyz(pl) - xyz(S.initLin[gid])) + xyz(S.posLo[gid])) / P.dt;
   float s2 = dot(v, v);
   if (s2 > P.maxSpeed * P.maxSpeed) v *= P.maxSpeed * rsq(s2);
   float shw = S.shape[gid].w;
@@ -843,10 +856,12 @@ void launch_finalize(hipStream_t st, const DeviceState& S, const StepParams& P) 
 // iterate (inertia + every stamp, no solve); block partials of |g|^2 and
 // |m g_grav|^2 in residualPartial[2*block + {0,1}]
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_residual(DeviceState S, StepParams P) {
-  __shared__ float red[2][256];
+// guard: part of a residual-terminated frame's chunk, skipped once it has ended
+__global__ void __launch_bounds__(256) k_residual(DeviceState S, StepParams P, uint32_t guard) {
+  if (guard && S.counters[CTR_CONV_DONE] != 0u) return;
+  __shared__ double red[2][256];
   uint32_t body = blockIdx.x * blockDim.x + threadIdx.x;
-  float g2 = 0.0f, w2 = 0.0f;
+  double g2 = 0.0, w2 = 0.0;
   if (body < P.numBodies && S.posLin[body].w > 0.0f) {
     bool rigid = S.shape[body].w >= 0.0f;
     Acc acc;
@@ -854,21 +869,21 @@ __global__ void __launch_bounds__(256) k_residual(DeviceState S, StepParams P) {
     PoseView pv{S.posLin, S.posAng, S.posLin, S.posAng, false};
     uint32_t ss = S.sAdjStart[body], sc = S.sAdjCount[body];
     uint32_t ds = S.dAdjStart[body], dc = S.dAdjCount[body];
-    const V3 xs = xyz(S.posLin[body]);
+    const V3 xs = xyz(S.posLin[body]), ls = xyz(S.posLo[body]);
     for (uint32_t k = 0; k < sc + dc; k++) {
       uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
-      stamp_entry(S, pv, entry, ss + k, body, xs, rigid, P.alpha, P.dt, acc);
+      stamp_entry(S, pv, entry, ss + k, body, xs, ls, rigid, P.alpha, P.dt, acc);
     }
     F4 pl = S.posLin[body];
     float md = pl.w / (P.dt * P.dt);
-    V3 g = acc.gl + (xyz(pl) - xyz(S.inertLin[body])) * md;
-    g2 = dot(g, g);
+    V3 g = acc.gl + ((xyz(pl) - xyz(S.inertLin[body])) + ls) * md;
+    g2 = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
     if (rigid) {
       M3 inertia = m3_scale(world_inertia(S.posAng[body], xyz(S.props[body])), 1.0f / (P.dt * P.dt));
       V3 ga = acc.ga + m3_mul(inertia, q_sub(S.posAng[body], S.inertAng[body]));
-      g2 += dot(ga, ga);
+      g2 += (double)ga.x * ga.x + (double)ga.y * ga.y + (double)ga.z * ga.z;
     }
-    float mg = pl.w * P.gravity * S.gravityScale[body];
+    double mg = (double)pl.w * P.gravity * S.gravityScale[body];
     w2 = mg * mg;
   }
   red[0][threadIdx.x] = g2;
@@ -887,8 +902,55 @@ __global__ void __launch_bounds__(256) k_residual(DeviceState S, StepParams P) {
   }
 }
 
-void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks) {
-  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, st, S, P);
+void launch_residual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t blocks, bool guard) {
+  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, st, S, P, guard ? 1u : 0u);
+}
+
+// The residual check ending a chunk of `n` sweeps: r = sqrt(sum |g|^2 /
+// sum |m g_grav|^2) from the block partials (f64); sets CTR_CONV_DONE when
+// r <= tol (1) or the frame has run maxIt sweeps (2), and publishes the
+// progress to the mapped host record.  A check after the frame has ended
+// only advances the sequence number.
+__global__ void __launch_bounds__(256) k_residual_check(DeviceState S, uint32_t blocks, float tol, uint32_t maxIt,
+                                                        uint32_t n, ConvHost* host) {
+  __shared__ double red[2][256];
+  uint32_t* c = S.counters;
+  const bool ended = c[CTR_CONV_DONE] != 0u;
+  double g2 = 0.0, w2 = 0.0;
+  if (!ended)
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256) {
+      g2 += S.residualPartial[2 * b];
+      w2 += S.residualPartial[2 * b + 1];
+    }
+  red[0][threadIdx.x] = g2;
+  red[1][threadIdx.x] = w2;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const uint32_t seq = c[CTR_CONV_SEQ] + 1u;
+  c[CTR_CONV_SEQ] = seq;
+  if (!ended) {
+    const double r = red[1][0] > 0.0 ? sqrt(red[0][0] / red[1][0]) : sqrt(red[0][0]);
+    const uint32_t it = c[CTR_CONV_ITERS] + n;
+    c[CTR_CONV_ITERS] = it;
+    c[CTR_CONV_RES] = as_uint((float)r);
+    c[CTR_CONV_DONE] = r <= (double)tol ? 1u : (it >= maxIt ? 2u : 0u);
+  }
+  host->done = c[CTR_CONV_DONE];
+  host->iters = c[CTR_CONV_ITERS];
+  host->residual = as_float(c[CTR_CONV_RES]);
+  __hip_atomic_store(&host->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_residual_check(hipStream_t st, const DeviceState& S, uint32_t blocks, float tol, uint32_t maxIt,
+                           uint32_t n, ConvHost* host) {
+  hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(256), 0, st, S, blocks, tol, maxIt, n, host);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,6 +998,7 @@ void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P
 template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P, uint32_t numColors,
                                                     uint32_t barrierBase) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;  // uniform across the grid: no barrier is entered
   const uint32_t groupsPerBlock = 256 / L;
   const uint32_t lane = threadIdx.x & (L - 1);
   const uint32_t gslot = xcd_block(blockIdx.x, gridDim.x) * groupsPerBlock + threadIdx.x / L;

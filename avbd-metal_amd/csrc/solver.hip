@@ -18,6 +18,7 @@
 
 #include "../../include/avbd.h"
 #include "kernels.hpp"
+#include "lifetime.hpp"
 #include "mlp.hpp"
 
 // upload-time geometry must round exactly as written (radii feed the
@@ -66,6 +67,7 @@ struct avbd_solver {
   bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
   std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
   std::vector<float> rateValues;
+  F4* jointPenInit = nullptr;      // upload-time joint penalties (lin [nj], ang [nj]), restored on reset
   uint32_t* rateJointDev = nullptr;
   float* rateValueDev = nullptr;
   uint32_t numGroups = 0;        // connected collision surfaces
@@ -114,8 +116,18 @@ struct avbd_solver {
   // scratch for queries
   uint32_t* devScratchU32 = nullptr;
   uint32_t devScratchWords = 0;
-  float* residualPartialHost = nullptr;
+  double* residualPartialHost = nullptr;
   uint32_t residualBlocks = 0;
+  // residual-terminated frames: the device's progress record (mapped host
+  // memory) and the captured pre-solve / chunk / post-solve graphs of the
+  // even and odd frame roles, for one (chunk, tolerance, cap) key
+  ConvHost* convHost = nullptr;
+  ConvHost* convDev = nullptr;
+  hipGraphExec_t resPre[2] = {nullptr, nullptr}, resChunk[2] = {nullptr, nullptr}, resPost[2] = {nullptr, nullptr};
+  bool resValid = false;
+  int32_t resCheck = 0, resMax = 0;
+  float resTol = 0.0f;
+  int64_t resBase = 1;  // frame whose roles resPre/resChunk/resPost[0] were captured with
   uint32_t cus = 256;
   // learned warm start
   avbd_mlp* mlp = nullptr;       // 3 outputs: dynamic particles
@@ -368,8 +380,30 @@ void invalidate_graphs(avbd_solver* s) {
   for (int i = 0; i < 2; i++) {
     if (s->graphs[i]) hipGraphExecDestroy(s->graphs[i]);
     s->graphs[i] = nullptr;
+    for (hipGraphExec_t* g : {&s->resPre[i], &s->resChunk[i], &s->resPost[i]}) {
+      if (*g) hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
   }
   s->graphsValid = false;
+  s->resValid = false;
+}
+
+// captures what `body` enqueues on the solver stream into an executable graph
+hipError_t capture_graph(avbd_solver* s, const std::function<hipError_t()>& body, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t eq = body();
+  e = hipStreamEndCapture(s->stream, &g);
+  if (eq != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return eq;
+  }
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  return e;
 }
 
 hipError_t build_graphs(avbd_solver* s) {
@@ -871,6 +905,7 @@ namespace {
 avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d);
 avbd_status static_coloring(avbd_solver* s);
 void autotune(avbd_solver* s);
+void release_solver(avbd_solver* s);
 
 }  // namespace
 
@@ -892,10 +927,11 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
   }
   avbd_status st = build_static_and_upload(s, d);
   if (st != AVBD_OK) {
-    avbd_destroy(s);
+    release_solver(s);
     return st;
   }
   *out = s;
+  live_add(s, HandleKind::Solver);
   st = static_coloring(s);
   if (st == AVBD_ERR_RUNTIME_FAILURE) return st;  // failure latched; caller reads it then destroys
   if (st != AVBD_OK) {
@@ -911,6 +947,13 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
 }
 
 extern "C" void avbd_destroy(avbd_solver* s) {
+  // a handle already released at process exit is not touched again
+  if (s && live_remove(s)) release_solver(s);
+}
+
+namespace {
+
+void release_solver(avbd_solver* s) {
   if (!s) return;
   if (s->stream) hipStreamSynchronize(s->stream);
   invalidate_graphs(s);
@@ -919,11 +962,10 @@ extern "C" void avbd_destroy(avbd_solver* s) {
     if (s->frameDone[i]) hipEventDestroy(s->frameDone[i]);
   if (s->pinnedCounters) hipHostFree(s->pinnedCounters);
   if (s->residualPartialHost) hipHostFree(s->residualPartialHost);
+  if (s->convHost) hipHostFree(s->convHost);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
 }
-
-namespace {
 
 avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   const uint32_t nb = d->num_bodies;
@@ -1086,6 +1128,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   // ---- joints (GPUSolver.swift:1092-1193)
   const uint32_t nj = d->num_joints;
   std::vector<JointRec> jr(nj);
+  std::vector<F4> penInit(2 * (size_t)std::max<uint32_t>(1, nj), F4{0, 0, 0, 0});
   s->jointHeaders.resize(nj);
   for (uint32_t i = 0; i < nj; i++) {
     JointRec g;
@@ -1149,6 +1192,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     if (ka > 0.0f && isfinite(ka)) { float k = fminf(ka, 1e9f); g.penaltyAng = F4{k, k, k, k}; }
     jr[i] = g;
     s->jointHeaders[i] = g.header;
+    // initialJointPenaltyLin / Ang: restored by setBodyStates
+    penInit[i] = g.penaltyLin;
+    penInit[nj + i] = g.penaltyAng;
   }
 
   // ---- springs (GPUSolver.swift:1195-1212)
@@ -1387,7 +1433,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   hipError_t e = hipSuccess;
 #define ALLOC(ptr, n) \
   if ((e = dev_alloc(s, &ptr, (n))) != hipSuccess) return AVBD_ERR_ALLOC;
-  ALLOC(S.posLin, nb) ALLOC(S.posAng, nb) ALLOC(S.initLin, nb) ALLOC(S.initAng, nb) ALLOC(S.inertLin, nb)
+  ALLOC(S.posLin, nb) ALLOC(S.posLo, nb) ALLOC(S.posAng, nb) ALLOC(S.initLin, nb) ALLOC(S.initAng, nb) ALLOC(S.inertLin, nb)
   ALLOC(S.inertAng, nb) ALLOC(S.velLin, nb) ALLOC(S.velAng, nb) ALLOC(S.prevVelLin, nb) ALLOC(S.props, nb)
   ALLOC(S.shape, nb) ALLOC(S.gravityScale, nb) ALLOC(S.clothGroup, nb) ALLOC(S.posLinSnap, nb)
   ALLOC(S.posAngSnap, nb)
@@ -1414,6 +1460,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.manifolds, s->maxPairs) ALLOC(S.prevManifolds, s->maxPairs)
   ALLOC(S.mapKeyA, s->mapCapacity) ALLOC(S.mapKeyB, s->mapCapacity) ALLOC(S.mapVal, s->mapCapacity)
   ALLOC(S.tets, std::max<uint32_t>(1, nt)) ALLOC(S.joints, std::max<uint32_t>(1, nj))
+  ALLOC(s->jointPenInit, 2 * (size_t)std::max<uint32_t>(1, nj))
   ALLOC(S.springs, std::max<uint32_t>(1, ns))
   ALLOC(S.sAdjStart, nb) ALLOC(S.sAdjCount, nb) ALLOC(S.sAdjList, std::max<size_t>(1, sList.size()))
   ALLOC(S.tetSlots, slots.size())
@@ -1476,7 +1523,8 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.colliderHullRange, hullRange) || upload(S.hullVerts, hullVerts) ||
       upload(S.colliderLocalPos, cp) || upload(S.colliderLocalRot, cq) || upload(S.colliderFriction, cf) ||
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
-      upload(S.tets, tr) || upload(S.joints, jr) || upload(S.springs, sr) || upload(S.sAdjStart, sStart) ||
+      upload(S.tets, tr) || upload(S.joints, jr) || upload(s->jointPenInit, penInit) || upload(S.springs, sr) ||
+      upload(S.sAdjStart, sStart) ||
       upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.tetSlots, slots) ||
       upload(S.hashedRigid, hashedRigid) || upload(S.clothSlots, cslots) ||
       upload(s->rateJointDev, s->rateJoints) || upload(s->rateValueDev, s->rateValues) || upload(S.tris, ch.tris) ||
@@ -1491,8 +1539,14 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       hipSuccess)
     return AVBD_ERR_ALLOC;
   memset(s->pinnedCounters, 0, 2 * CTR_TOTAL * sizeof(uint32_t));
-  if (hipHostMalloc((void**)&s->residualPartialHost, 2 * bb * sizeof(float), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void**)&s->residualPartialHost, 2 * bb * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return AVBD_ERR_ALLOC;
+  // written by the device's residual checks, read by the host while kernels run
+  if (hipHostMalloc((void**)&s->convHost, sizeof(ConvHost), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->convDev, s->convHost, 0) != hipSuccess)
+    return AVBD_ERR_ALLOC;
+  memset(s->convHost, 0, sizeof(ConvHost));
   for (int i = 0; i < 2; i++)
     if (hipEventCreateWithFlags(&s->frameDone[i], hipEventDisableTiming) != hipSuccess) return AVBD_ERR_HIP;
   int dev = 0, cus = 0;
@@ -1655,7 +1709,7 @@ void autotune(avbd_solver* s) {
   }
   // backup
   std::vector<std::pair<void*, size_t>> bufs = {
-      {S.posLin, nb * 16}, {S.posAng, nb * 16}, {S.initLin, nb * 16}, {S.initAng, nb * 16},
+      {S.posLin, nb * 16}, {S.posLo, nb * 16}, {S.posAng, nb * 16}, {S.initLin, nb * 16}, {S.initAng, nb * 16},
       {S.inertLin, nb * 16}, {S.inertAng, nb * 16}, {S.posLinSnap, nb * 16}, {S.posAngSnap, nb * 16},
       {S.joints, std::max<uint32_t>(1, s->nj) * sizeof(JointRec)},
       {S.springs, std::max<uint32_t>(1, s->ns) * sizeof(SpringRec)}, {S.counters, CTR_TOTAL * 4},
@@ -1815,43 +1869,123 @@ extern "C" avbd_status avbd_submit_step(avbd_solver* s) {
 
 extern "C" avbd_status avbd_step(avbd_solver* s) { return avbd_submit_step(s); }
 
-// Iterate one frame until the residual reaches `tol` (or max_iterations):
-// chunks of `check_every` sweeps, each followed by the residual kernel and a
-// host readback of its block sums.  Eager launches; the frame is complete
-// (and retired) on return.
+// Residual-terminated frames.  Iterations run in chunks of `check_every`
+// sweeps; each chunk ends with the residual kernel and a one-block check
+// that sets the frame's done flag on the device (r <= tol or the sweep cap)
+// and publishes its progress to mapped host memory.  Every sweep kernel
+// returns at entry once the flag is set, so chunks can be queued ahead
+// without a host round trip per chunk: the host keeps a few chunks in
+// flight, watches the progress record, and stops queueing at the first done
+// check.  The result is that of the first chunk whose check passes, the
+// same as checking on the host after every chunk.  Pre-solve, chunk and
+// post-solve are captured graphs (even / odd frame roles) unless graphs are
+// off; a final partial chunk (cap not a multiple of check_every) is
+// enqueued directly.
+namespace {
+
+constexpr uint32_t kChunksInFlight = 3;
+
+hipError_t enqueue_chunk(avbd_solver* s, const DeviceState& S, const StepParams& P, uint32_t n, float tol,
+                         uint32_t maxIt) {
+  hipError_t e = enqueue_iterations(s, S, P, n, true);
+  if (e != hipSuccess) return e;
+  launch_residual(s->stream, S, P, s->residualBlocks, true);
+  launch_residual_check(s->stream, S, s->residualBlocks, tol, maxIt, n, s->convDev);
+  return hipGetLastError();
+}
+
+hipError_t build_residual_graphs(avbd_solver* s, float tol, int32_t maxIt, int32_t check) {
+  for (int i = 0; i < 2; i++)
+    for (hipGraphExec_t* g : {&s->resPre[i], &s->resChunk[i], &s->resPost[i]}) {
+      if (*g) hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+  s->resValid = false;
+  DeviceState S = s->S;
+  const StepParams P = make_params(s);
+  auto none = [](const char*) {};
+  for (int p = 0; p < 2; p++) {
+    hipError_t e = capture_graph(s, [&]() { enqueue_pre_solve(s, S, P, none); return hipGetLastError(); },
+                                 &s->resPre[p]);
+    if (e == hipSuccess)
+      e = capture_graph(s, [&]() { return enqueue_chunk(s, S, P, (uint32_t)check, tol, (uint32_t)maxIt); },
+                        &s->resChunk[p]);
+    if (e == hipSuccess)
+      e = capture_graph(s, [&]() { enqueue_post_solve(s, S, P, none); return hipGetLastError(); }, &s->resPost[p]);
+    if (e != hipSuccess) return e;
+    std::swap(S.manifolds, S.prevManifolds);
+    std::swap(S.soft, S.prevSoft);
+  }
+  s->resValid = true;
+  s->resCheck = check;
+  s->resMax = maxIt;
+  s->resTol = tol;
+  s->resBase = s->frameIndex + 1;
+  return hipSuccess;
+}
+
+}  // namespace
+
 extern "C" avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t max_iterations, int32_t check_every,
                                              int32_t* iterations_used, float* residual) {
   if (!s || !(tol > 0.0f) || max_iterations <= 0 || check_every <= 0) return AVBD_ERR_INVALID_ARGUMENT;
   avbd_status st0 = sync_all(s);
   if (st0 != AVBD_OK) return st0;
-  int64_t frame = s->frameIndex + 1;
-  int slot = (int)(frame & 1);
+  const int64_t frame = s->frameIndex + 1;
+  const int slot = (int)(frame & 1);
   hipStream_t st = s->stream;
-  DeviceState S = s->S;
-  StepParams P = make_params(s);
-  auto none = [](const char*) {};
-  enqueue_pre_solve(s, S, P, none);
-  int32_t it = 0;
-  double r = INFINITY;
-  while (it < max_iterations) {
-    int32_t n = std::min(check_every, max_iterations - it);
-    HIP_OR_LATCH(s, enqueue_iterations(s, S, P, (uint32_t)n, true), frame);
-    it += n;
-    launch_residual(st, S, P, s->residualBlocks);
-    HIP_OR_LATCH(s,
-                 hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 8 * s->residualBlocks,
-                                hipMemcpyDeviceToHost, st),
-                 frame);
-    HIP_OR_LATCH(s, hipStreamSynchronize(st), frame);
-    double g2 = 0.0, w2 = 0.0;
-    for (uint32_t b = 0; b < s->residualBlocks; b++) {
-      g2 += s->residualPartialHost[2 * b];
-      w2 += s->residualPartialHost[2 * b + 1];
+  bool useGraph = s->opts.use_graph && !s->graphsDisabled && !s->profiling;
+  if (useGraph && !(s->resValid && s->resCheck == check_every && s->resMax == max_iterations && s->resTol == tol)) {
+    if (build_residual_graphs(s, tol, max_iterations, check_every) != hipSuccess) {
+      (void)hipGetLastError();
+      invalidate_graphs(s);
+      s->graphsDisabled = true;
+      useGraph = false;
     }
-    r = w2 > 0.0 ? sqrt(g2 / w2) : sqrt(g2);
-    if (r <= tol) break;
   }
-  enqueue_post_solve(s, S, P, none);
+  const int parity = useGraph ? (int)((frame - s->resBase) & 1) : 0;
+  const DeviceState S = s->S;
+  const StepParams P = make_params(s);
+  auto none = [](const char*) {};
+  // the device has retired every earlier check (sync_all above)
+  memset(s->convHost, 0, sizeof(ConvHost));
+  if (useGraph) HIP_OR_LATCH(s, hipGraphLaunch(s->resPre[parity], st), frame);
+  else enqueue_pre_solve(s, S, P, none);
+  const uint32_t full = (uint32_t)(max_iterations / check_every), tail = (uint32_t)(max_iterations % check_every);
+  const uint32_t total = full + (tail ? 1u : 0u);
+  uint32_t launched = 0, seen = 0;
+  volatile ConvHost* h = s->convHost;
+  for (;;) {
+    while (launched < total && launched - seen < kChunksInFlight) {
+      if (launched < full && useGraph) {
+        HIP_OR_LATCH(s, hipGraphLaunch(s->resChunk[parity], st), frame);
+      } else {
+        const uint32_t n = launched < full ? (uint32_t)check_every : tail;
+        HIP_OR_LATCH(s, enqueue_chunk(s, S, P, n, tol, (uint32_t)max_iterations), frame);
+      }
+      launched++;
+    }
+    // wait for the next check to land (or the stream to go idle / fail)
+    for (uint32_t spin = 0;; spin++) {
+      const uint32_t q = __atomic_load_n(&s->convHost->seq, __ATOMIC_ACQUIRE);
+      if (q != seen) {
+        seen = q;
+        break;
+      }
+      if ((spin & 255u) == 255u) {
+        hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) {
+          seen = __atomic_load_n(&s->convHost->seq, __ATOMIC_ACQUIRE);
+          break;
+        }
+        if (e != hipErrorNotReady) HIP_OR_LATCH(s, e, frame);
+      }
+      __builtin_ia32_pause();
+    }
+    if (h->done != 0u || seen >= total || (seen == launched && launched == total)) break;
+  }
+  if (useGraph) HIP_OR_LATCH(s, hipGraphLaunch(s->resPost[parity], st), frame);
+  else enqueue_post_solve(s, S, P, none);
   HIP_OR_LATCH(s, hipGetLastError(), frame);
   std::swap(s->S.manifolds, s->S.prevManifolds);
   std::swap(s->S.soft, s->S.prevSoft);
@@ -1863,9 +1997,10 @@ extern "C" avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t 
   s->slotBusy[slot] = true;
   s->slotFrame[slot] = frame;
   s->frameIndex = frame;
-  if (iterations_used) *iterations_used = it;
-  if (residual) *residual = (float)r;
-  return sync_all(s);
+  avbd_status out = sync_all(s);
+  if (iterations_used) *iterations_used = (int32_t)s->pinnedCounters[slot * CTR_TOTAL + CTR_CONV_ITERS];
+  if (residual) *residual = as_float(s->pinnedCounters[slot * CTR_TOTAL + CTR_CONV_RES]);
+  return out;
 }
 
 extern "C" avbd_status avbd_synchronize(avbd_solver* s) {
@@ -1971,6 +2106,7 @@ __global__ void k_write_states(DeviceState S, const uint32_t* idx, uint32_t n, c
   uint32_t i = idx[k];
   float m = S.posLin[i].w;
   S.posLin[i] = F4{p[3 * k], p[3 * k + 1], p[3 * k + 2], m};
+  S.posLo[i] = F4{0.0f, 0.0f, 0.0f, 0.0f};
   S.posAng[i] = F4{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
   S.velLin[i] = F4{v[3 * k], v[3 * k + 1], v[3 * k + 2], 0.0f};
   S.velAng[i] = F4{w[3 * k], w[3 * k + 1], w[3 * k + 2], 0.0f};
@@ -1991,49 +2127,31 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
     ids[k] = i;
     flag[i] = 1u;
   }
-  // joint penalties at upload (initialJointPenaltyLin/Ang) are re-derived
-  // from the stored records' flags: keep a device copy made at create
-  static_assert(sizeof(F4) == 16, "F4");
-  uint32_t* dIdx = nullptr;
-  uint32_t* dFlag = nullptr;
-  float* dBuf = nullptr;
-  F4* dPen = nullptr;
-  size_t fl = (size_t)n * 13;
-  if (hipMalloc(&dIdx, 4 * n) || hipMalloc(&dFlag, 4 * s->nb) || hipMalloc(&dBuf, 4 * fl) ||
-      hipMalloc(&dPen, 32 * std::max<uint32_t>(1, s->nj)))
+  // one staging block: ids [n], flags [nb], pos / rot / vel / angvel [13 n]
+  const size_t fl = (size_t)n * 13;
+  const size_t words = (size_t)n + s->nb + fl;
+  void* block = nullptr;
+  if (hipMalloc(&block, 4 * words) != hipSuccess) {
+    (void)hipGetLastError();
     return AVBD_ERR_ALLOC;
-  std::vector<float> buf(fl);
-  memcpy(buf.data(), pos3, 12 * n);
-  memcpy(buf.data() + 3 * n, rot4, 16 * n);
-  memcpy(buf.data() + 7 * n, vel3, 12 * n);
-  memcpy(buf.data() + 10 * n, angvel3, 12 * n);
-  hipMemcpy(dIdx, ids.data(), 4 * n, hipMemcpyHostToDevice);
-  hipMemcpy(dFlag, flag.data(), 4 * s->nb, hipMemcpyHostToDevice);
-  hipMemcpy(dBuf, buf.data(), 4 * fl, hipMemcpyHostToDevice);
-  // initial joint penalties: recompute as upload does (floor or stiffness)
-  std::vector<F4> pen(2 * std::max<uint32_t>(1, s->nj));
-  if (s->nj) {
-    std::vector<JointRec> jr;
-    read_dev(s, jr, s->S.joints, s->nj);
-    for (uint32_t i = 0; i < s->nj; i++) {
-      const JointRec& g = jr[i];
-      uint32_t a = g.header.x, b = g.header.y;
-      float mA = a != kWorld ? s->mass[a] : 0.0f, mB = s->mass[b];
-      float eff = 0.0f;
-      bool any = false;
-      if (mA > 0.0f) { eff = mA; any = true; }
-      if (mB > 0.0f) { eff = any ? fminf(eff, mB) : mB; any = true; }
-      float floorK = fminf(1.0e9f, fmaxf(1.0f, eff / fmaxf(s->settings.dt * s->settings.dt, 1.0e-12f)));
-      F4 pl{0, 0, 0, 0}, pa{0, 0, 0, 0};
-      if (g.header.w & 1u) pl = F4{floorK, floorK, floorK, floorK};
-      else if (g.rA.w > 0.0f) { float k = fminf(g.rA.w, 1e9f); pl = F4{k, k, k, k}; }
-      if (g.header.w & 2u) pa = F4{floorK, floorK, floorK, floorK};
-      else if (g.rB.w > 0.0f) { float k = fminf(g.rB.w, 1e9f); pa = F4{k, k, k, k}; }
-      pen[i] = pl;
-      pen[s->nj + i] = pa;
-    }
-    hipMemcpy(dPen, pen.data(), 32 * s->nj, hipMemcpyHostToDevice);
   }
+  uint32_t* dIdx = (uint32_t*)block;
+  uint32_t* dFlag = dIdx + n;
+  float* dBuf = (float*)(dFlag + s->nb);
+  std::vector<float> buf(fl);
+  memcpy(buf.data(), pos3, 12 * (size_t)n);
+  memcpy(buf.data() + 3 * (size_t)n, rot4, 16 * (size_t)n);
+  memcpy(buf.data() + 7 * (size_t)n, vel3, 12 * (size_t)n);
+  memcpy(buf.data() + 10 * (size_t)n, angvel3, 12 * (size_t)n);
+  if (hipMemcpy(dIdx, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dFlag, flag.data(), 4 * (size_t)s->nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dBuf, buf.data(), 4 * fl, hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(block);
+    return AVBD_ERR_HIP;
+  }
+  // the incident joints get back the penalties stored at upload
+  // (initialJointPenaltyLin / Ang), whatever dt the settings hold now
+  F4* dPen = s->jointPenInit;
   StepParams P = make_params(s);
   hipLaunchKernelGGL(k_write_states, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dIdx, n, dBuf,
                      dBuf + 3 * n, dBuf + 7 * n, dBuf + 10 * n);
@@ -2047,10 +2165,7 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
     hipMemsetAsync(s->W.corrAng, 0, 16 * (size_t)s->nb, s->stream);
   }
   hipError_t e = hipStreamSynchronize(s->stream);
-  hipFree(dIdx);
-  hipFree(dFlag);
-  hipFree(dBuf);
-  hipFree(dPen);
+  hipFree(block);
   if (e != hipSuccess) return latch(s, AVBD_FAIL_COMMAND_EXECUTION, s->frameIndex, 0, 0, (int)e, "set_body_states");
   return AVBD_OK;
 }
@@ -2067,15 +2182,20 @@ __global__ void k_set_joint_field(DeviceState S, const uint32_t* j, const float*
 }
 avbd_status set_joint_field(avbd_solver* s, const uint32_t* joints, uint32_t n, const float* v, int field) {
   const uint32_t width = field == 2 ? 3u : 1u;
-  uint32_t* dj = nullptr;
-  float* dv = nullptr;
-  if (hipMalloc(&dj, 4 * n) || hipMalloc(&dv, 4 * width * n)) return AVBD_ERR_ALLOC;
-  hipMemcpy(dj, joints, 4 * n, hipMemcpyHostToDevice);
-  hipMemcpy(dv, v, 4 * width * n, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_set_joint_field, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dj, dv, n, field);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  hipFree(dj);
-  hipFree(dv);
+  void* block = nullptr;  // joint ids [n], then values [width n]
+  if (hipMalloc(&block, 4 * (size_t)n * (1 + width)) != hipSuccess) {
+    (void)hipGetLastError();
+    return AVBD_ERR_ALLOC;
+  }
+  uint32_t* dj = (uint32_t*)block;
+  float* dv = (float*)(dj + n);
+  hipError_t e = hipMemcpy(dj, joints, 4 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv, v, 4 * (size_t)width * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_set_joint_field, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->S, dj, dv, n, field);
+    e = hipStreamSynchronize(s->stream);
+  }
+  hipFree(block);
   return e == hipSuccess ? AVBD_OK : AVBD_ERR_HIP;
 }
 // the preconditions of GPUSolver.swift:2496-2590 as argument errors; kept
@@ -2387,8 +2507,8 @@ extern "C" avbd_status avbd_residual(avbd_solver* s, float* out) {
   std::swap(S.soft, S.prevSoft);
   StepParams P = make_params(s);
   launch_residual(s->stream, S, P, s->residualBlocks);
-  if (hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 8 * s->residualBlocks, hipMemcpyDeviceToHost,
-                     s->stream) ||
+  if (hipMemcpyAsync(s->residualPartialHost, S.residualPartial, 16 * (size_t)s->residualBlocks,
+                     hipMemcpyDeviceToHost, s->stream) ||
       hipStreamSynchronize(s->stream))
     return AVBD_ERR_HIP;
   double g2 = 0.0, w2 = 0.0;
@@ -2709,12 +2829,11 @@ struct avbd_snapshot {
 namespace {
 
 // the snapshot's buffers, in a fixed order, with the manifold / soft roles
-// of the current frame
-std::vector<std::pair<void*, size_t>> speculation_buffers(const avbd_solver* s) {
-  const DeviceState& S = s->S;
+// of S (the solver's current roles, or the roles a restore will commit)
+std::vector<std::pair<void*, size_t>> speculation_buffers(const avbd_solver* s, const DeviceState& S) {
   const size_t nb = s->nb;
   std::vector<std::pair<void*, size_t>> b = {
-      {S.posLin, 16 * nb}, {S.posAng, 16 * nb}, {S.initLin, 16 * nb}, {S.initAng, 16 * nb},
+      {S.posLin, 16 * nb}, {S.posLo, 16 * nb}, {S.posAng, 16 * nb}, {S.initLin, 16 * nb}, {S.initAng, 16 * nb},
       {S.inertLin, 16 * nb}, {S.inertAng, 16 * nb}, {S.velLin, 16 * nb}, {S.velAng, 16 * nb},
       {S.prevVelLin, 16 * nb}, {S.posLinSnap, 16 * nb}, {S.posAngSnap, 16 * nb},
       {S.joints, sizeof(JointRec) * std::max<size_t>(1, s->nj)},
@@ -2742,7 +2861,7 @@ extern "C" avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_sn
   if (*out && (*out)->owner != s) return AVBD_ERR_INVALID_ARGUMENT;
   avbd_status st = sync_all(s);
   if (st != AVBD_OK) return st;
-  const auto bufs = speculation_buffers(s);
+  const auto bufs = speculation_buffers(s, s->S);
   size_t bytes = 0;
   for (const auto& b : bufs) bytes += spec_align(b.second);
   avbd_snapshot* snap = *out;
@@ -2751,6 +2870,8 @@ extern "C" avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_sn
     snap = new avbd_snapshot();
     snap->owner = s;
     fresh = true;
+  } else if (!live_contains(snap)) {
+    return AVBD_ERR_INVALID_ARGUMENT;
   }
   if (snap->bytes != bytes) {  // grows when a learned warm start was attached since
     if (snap->block) hipFree(snap->block);
@@ -2772,6 +2893,7 @@ extern "C" avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_sn
   if (hipStreamSynchronize(s->stream) != hipSuccess) return AVBD_ERR_HIP;
   snap->frameIndex = s->frameIndex;
   snap->stats = s->stats;
+  if (fresh) live_add(snap, HandleKind::Snapshot);
   *out = snap;
   return AVBD_OK;
 }
@@ -2779,16 +2901,20 @@ extern "C" avbd_status avbd_capture_speculation_snapshot(avbd_solver* s, avbd_sn
 extern "C" avbd_status avbd_restore_speculation_snapshot(avbd_solver* s, const avbd_snapshot* snap) {
   if (!s || !snap || snap->owner != s) return AVBD_ERR_INVALID_ARGUMENT;
   if (s->nTris || s->nt) return AVBD_ERR_UNSUPPORTED;
+  if (!live_contains(snap)) return AVBD_ERR_INVALID_ARGUMENT;
   avbd_status st = sync_all(s);
   if (st != AVBD_OK) return st;
   // manifold and soft buffers alternate roles every frame (the captured
-  // graphs bake the roles by frame parity): put them back in the roles of
-  // the snapshot's frame before copying
+  // graphs bake the roles by frame parity): the restore copies into the
+  // roles of the snapshot's frame and commits those roles only after every
+  // copy has completed, so a rejected or failed restore leaves the solver
+  // as it was
+  DeviceState R = s->S;
   if (((s->frameIndex - snap->frameIndex) & 1) != 0) {
-    std::swap(s->S.manifolds, s->S.prevManifolds);
-    std::swap(s->S.soft, s->S.prevSoft);
+    std::swap(R.manifolds, R.prevManifolds);
+    std::swap(R.soft, R.prevSoft);
   }
-  const auto bufs = speculation_buffers(s);
+  const auto bufs = speculation_buffers(s, R);
   size_t bytes = 0;
   for (const auto& b : bufs) bytes += spec_align(b.second);
   if (bytes != snap->bytes) return AVBD_ERR_INVALID_ARGUMENT;  // a warm start attached or detached since
@@ -2800,16 +2926,50 @@ extern "C" avbd_status avbd_restore_speculation_snapshot(avbd_solver* s, const a
     off += spec_align(b.second);
   }
   if (hipStreamSynchronize(s->stream) != hipSuccess) return AVBD_ERR_HIP;
+  s->S.manifolds = R.manifolds;
+  s->S.prevManifolds = R.prevManifolds;
+  s->S.soft = R.soft;
+  s->S.prevSoft = R.prevSoft;
   s->frameIndex = snap->frameIndex;
   s->stats = snap->stats;
   return AVBD_OK;
 }
 
-extern "C" void avbd_destroy_speculation_snapshot(avbd_snapshot* snap) {
-  if (!snap) return;
+namespace {
+
+void release_snapshot(avbd_snapshot* snap) {
   if (snap->block) hipFree(snap->block);
   delete snap;
 }
+
+}  // namespace
+
+extern "C" void avbd_destroy_speculation_snapshot(avbd_snapshot* snap) {
+  if (snap && live_remove(snap)) release_snapshot(snap);
+}
+
+namespace avbd {
+
+void release_network(avbd_mlp* m);  // mlp_api.hip
+
+void release_all_at_exit() {
+  LiveRegistry& r = live_registry();
+  std::vector<LiveHandle> h;
+  {
+    std::lock_guard<std::mutex> g(r.mu);
+    h.swap(r.live);
+    r.exiting = true;
+  }
+  // snapshots and solvers before the networks a solver may reference
+  for (const LiveHandle& x : h)
+    if (x.kind == HandleKind::Snapshot) release_snapshot((avbd_snapshot*)x.p);
+  for (const LiveHandle& x : h)
+    if (x.kind == HandleKind::Solver) release_solver((avbd_solver*)x.p);
+  for (const LiveHandle& x : h)
+    if (x.kind == HandleKind::Network) release_network((avbd_mlp*)x.p);
+}
+
+}  // namespace avbd
 
 extern "C" avbd_status avbd_test_set_dat_query_radius(avbd_solver* s, float r) {
   if (!s || !(r == r)) return AVBD_ERR_INVALID_ARGUMENT;

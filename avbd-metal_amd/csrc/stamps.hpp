@@ -97,8 +97,8 @@ __device__ __forceinline__ void stamp_tet(const TetRec& t, uint32_t self, const 
 }
 
 // The same stamp from a stamp-major slot: one gather level.
-__device__ __forceinline__ void stamp_tet_slot_x(const TetSlot& t, V3 xs, V3 x1, V3 x2, V3 x3, Acc& acc) {
-  V3 e1 = x1 - xs, e2 = x2 - xs, e3 = x3 - xs;
+// e1..e3: the edges from the slot's own vertex to the other three
+__device__ __forceinline__ void stamp_tet_slot_e(const TetSlot& t, V3 e1, V3 e2, V3 e3, Acc& acc) {
   V3 b1 = xyz(t.b1), b2 = xyz(t.b2), b3 = xyz(t.b3);
   float muV = t.b1.w, lamV = t.b2.w, alphaT = t.b3.w;
   V3 f0 = e1 * b1.x + e2 * b2.x + e3 * b3.x;
@@ -113,8 +113,18 @@ __device__ __forceinline__ void stamp_tet_slot_x(const TetSlot& t, V3 xs, V3 x1,
   acc_lin_diag(acc, muV * dot(w, w));
   acc_lin_outer(acc, g, lamV);
 }
-__device__ __forceinline__ void stamp_tet_slot(const TetSlot& t, V3 xs, const F4* pos, Acc& acc) {
-  stamp_tet_slot_x(t, xs, xyz(pos[t.other.x]), xyz(pos[t.other.y]), xyz(pos[t.other.z]), acc);
+// edge from a vertex at (xs + ls) to one at (x + l), high and low parts
+// differenced separately: neighbouring f32 positions subtract exactly, so
+// the edge keeps the resolution of the low parts
+__device__ __forceinline__ V3 split_edge(V3 x, V3 l, V3 xs, V3 ls) { return (x - xs) + (l - ls); }
+__device__ __forceinline__ void stamp_tet_slot_x(const TetSlot& t, V3 xs, V3 x1, V3 x2, V3 x3, Acc& acc) {
+  stamp_tet_slot_e(t, x1 - xs, x2 - xs, x3 - xs, acc);
+}
+__device__ __forceinline__ void stamp_tet_slot(const TetSlot& t, V3 xs, V3 ls, const F4* pos, const F4* lo,
+                                               Acc& acc) {
+  stamp_tet_slot_e(t, split_edge(xyz(pos[t.other.x]), xyz(lo[t.other.x]), xs, ls),
+                   split_edge(xyz(pos[t.other.y]), xyz(lo[t.other.y]), xs, ls),
+                   split_edge(xyz(pos[t.other.z]), xyz(lo[t.other.z]), xs, ls), acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -143,8 +153,11 @@ __device__ __forceinline__ V3 contact_force(const ManifoldRec& m, int ci, V3 dqA
   return F;
 }
 
+// loSelf: the low part of the stamping body's position (added to its own
+// displacement from the step start)
 __device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t self, bool rigidSelf, const PoseView& pv,
-                                               const F4* initLin, const F4* initAng, float alpha, Acc& acc) {
+                                               const F4* initLin, const F4* initAng, float alpha, Acc& acc,
+                                               V3 loSelf) {
   uint32_t a = m.header.x, b = m.header.y, n = m.header.z;
   bool isA = self == a;
   bool rA_round = (m.header.w & 2u) != 0, rB_round = (m.header.w & 4u) != 0;
@@ -153,6 +166,8 @@ __device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t se
   V3 dqAL = xyz(pAL) - xyz(initLin[a]);
   V3 dqAA = q_sub(qA, initAng[a]);
   V3 dqBL = xyz(pBL) - xyz(initLin[b]);
+  if (isA) dqAL += loSelf;
+  else dqBL += loSelf;
   V3 dqBA = q_sub(qB, initAng[b]);
   V3 nrm = xyz(m.basisN), t1 = xyz(m.basisT1), t2 = cross(nrm, t1);
   float s = isA ? 1.0f : -1.0f;
@@ -515,13 +530,15 @@ __device__ __forceinline__ void stamp_soft(const SoftRec& sc, uint32_t self, uin
 
 // One adjacency entry; `slot` is the entry's static-CSR position (tets and
 // cloth elements read their stamp-major record there).
+// xs, ls: the stamping body's position, high and low parts
 __device__ __forceinline__ void stamp_entry(const DeviceState& S, const PoseView& pv, uint32_t entry, uint32_t slot,
-                                            uint32_t self, V3 xs, bool rigidSelf, float alpha, float dt, Acc& acc) {
+                                            uint32_t self, V3 xs, V3 ls, bool rigidSelf, float alpha, float dt,
+                                            Acc& acc) {
   uint32_t kind = entry >> kAdjShift, idx = entry & kAdjIndexMask;
   if (kind == FK_TET) {
-    stamp_tet_slot(S.tetSlots[slot], xs, S.posLin, acc);
+    stamp_tet_slot(S.tetSlots[slot], xs, ls, S.posLin, S.posLo, acc);
   } else if (kind == FK_MANIFOLD) {
-    stamp_manifold(S.manifolds[idx], self, rigidSelf, pv, S.initLin, S.initAng, alpha, acc);
+    stamp_manifold(S.manifolds[idx], self, rigidSelf, pv, S.initLin, S.initAng, alpha, acc, ls);
   } else if (kind == FK_JOINT) {
     const JointRec& j = S.joints[idx];
     if (j.header.z == 0) stamp_joint(j, self, S.posLin, S.posAng, S.initAng, alpha, dt, acc);  // skip broken

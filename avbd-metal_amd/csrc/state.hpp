@@ -12,6 +12,11 @@ struct alignas(8) U2 {
 struct DeviceState {
   // bodies (one F4 per body, each field its own array)
   F4 *posLin, *posAng, *initLin, *initAng, *inertLin, *inertAng;
+  // low part of the current linear position: the iterate is posLin + posLo
+  // exactly (posLin holds it rounded to f32).  Zeroed at the frame start;
+  // the primal carries the sum of its corrections here, so the elastic and
+  // inertial terms see differences finer than one f32 ulp of the position
+  F4* posLo;
   F4 *velLin, *velAng, *prevVelLin;
   F4 *props;          // xyz principal moment, w friction
   F4 *shape;          // xyz size, w = +radius (rigid) / -radius (particle)
@@ -86,7 +91,7 @@ struct DeviceState {
   uint32_t* colorFlags;       // per-pass changed flags of the dynamic coloring
   uint32_t* counters;
   uint32_t* scanScratch;      // block sums for the scan
-  float* residualPartial;     // per-block residual partial sums
+  double* residualPartial;    // per-block residual partial sums (|g|^2, |m g_grav|^2)
   uint32_t dAdjCapacity;
 };
 

@@ -39,7 +39,7 @@ typedef enum avbd_status {
     AVBD_ERR_INVALID_ARGUMENT = 1,  /* precondition violated (Scene.swift preconditions) */
     AVBD_ERR_NO_DEVICE = 2,         /* AVBDError.noDevice          GPUSolver.swift:641 */
     AVBD_ERR_ALLOC = 3,             /* AVBDError.allocFailed       GPUSolver.swift:642 */
-    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus, convex hull) */
+    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus colliders; hull pairs other than hull-box) */
     AVBD_ERR_RUNTIME_FAILURE = 5,   /* a RuntimeFailure is latched; see avbd_last_failure */
     AVBD_ERR_HIP = 6                /* HIP API error outside a latched frame */
 } avbd_status;
@@ -265,10 +265,9 @@ avbd_status avbd_set_joint_world_anchors(avbd_solver* s, const uint32_t* joints,
  * velocity projected on the child's world hinge axis. */
 avbd_status avbd_get_motor_states(avbd_solver* s, const uint32_t* joints, uint32_t n, float* angle,
                                   float* velocity);
-/* debugBrokenJoints / debugMaxLambda -- GPUSolver.swift:2652-2672 */
 /* Motor targets (JointGPU.motor.x) of joints [0, n): motorTargetForTesting. */
 avbd_status avbd_get_motor_targets(avbd_solver* s, uint32_t n, float* targets);
-
+/* debugBrokenJoints / debugMaxLambda -- GPUSolver.swift:2652-2672 */
 avbd_status avbd_get_joint_diagnostics(avbd_solver* s, int32_t* broken, float* max_lambda_lin,
                                        float* max_lambda_ang);
 

@@ -251,6 +251,9 @@ struct Oracle : oracle_base {
   uint32_t nb = 0, nc = 0;
   // bodies
   std::vector<V<T>> pos, vel, angVel, prevVel, initLin, inertLin, moment, size;
+  // low part of the iterate: the position is pos + lo exactly (pos rounded);
+  // zero at the frame start, carried by the primal update (TwoSum)
+  std::vector<V<T>> lo;
   std::vector<Q<T>> rot, initAng, inertAng;
   std::vector<V<T>> snapLin;
   std::vector<Q<T>> snapAng;
@@ -320,7 +323,7 @@ struct Oracle : oracle_base {
     nc = d->num_colliders;
     auto r3 = [](const float* p, uint32_t i) { return V<T>(p[3 * i], p[3 * i + 1], p[3 * i + 2]); };
     auto r4 = [](const float* p, uint32_t i) { return Q<T>{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; };
-    pos.resize(nb); vel.resize(nb); angVel.resize(nb); prevVel.resize(nb); initLin.resize(nb); inertLin.resize(nb);
+    pos.resize(nb); lo.assign(nb, V<T>()); vel.resize(nb); angVel.resize(nb); prevVel.resize(nb); initLin.resize(nb); inertLin.resize(nb);
     moment.resize(nb); size.resize(nb); rot.resize(nb); initAng.resize(nb); inertAng.resize(nb);
     mass.resize(nb); radiusSigned.resize(nb); gravityScale.resize(nb); friction.resize(nb); particle.resize(nb);
     const float pi = 3.14159265358979f;
@@ -1453,6 +1456,7 @@ struct Oracle : oracle_base {
       if (!std::isfinite(w)) w = 0;
       initLin[i] = pos[i];
       initAng[i] = rot[i];
+      lo[i] = V<T>();
       if (m > 0) {
         pos[i] = pos[i] + vel[i] * dt + V<T>(0, 0, g) * (w * dt * dt);
         rot[i] = inertAng[i];
@@ -1477,10 +1481,11 @@ struct Oracle : oracle_base {
   }
 
   void stampTet(const Tet<T>& t, uint32_t self, Sys& s) const {
-    V<T> x0 = pos[t.ids[0]], x1 = pos[t.ids[1]], x2 = pos[t.ids[2]], x3 = pos[t.ids[3]];
+    // edges from vertex 0, high and low parts differenced separately
+    auto edge = [&](uint32_t k) { return (pos[t.ids[k]] - pos[t.ids[0]]) + (lo[t.ids[k]] - lo[t.ids[0]]); };
     T muV = t.muV, lamV = t.lamV;
     T aT = T(1) + muV / std::max(lamV, T(1e-9));
-    V<T> d0 = x1 - x0, d1 = x2 - x0, d2 = x3 - x0;
+    V<T> d0 = edge(1), d1 = edge(2), d2 = edge(3);
     V<T> f[3];
     for (int c = 0; c < 3; c++) f[c] = d0 * t.r[0][c] + d1 * t.r[1][c] + d2 * t.r[2][c];
     V<T> j0 = crossv(f[1], f[2]), j1 = crossv(f[2], f[0]), j2 = crossv(f[0], f[1]);
@@ -1523,6 +1528,8 @@ struct Oracle : oracle_base {
     V<T> pAL = pL(m.a, self), pBL = pL(m.b, self);
     Q<T> qA = pA(m.a, self), qB = pA(m.b, self);
     V<T> dqAL = pAL - initLin[m.a], dqBL = pBL - initLin[m.b];
+    if (isA) dqAL = dqAL + lo[self];   // the stamping body's own low part
+    else dqBL = dqBL + lo[self];
     V<T> dqAA = qsub(qA, initAng[m.a]), dqBA = qsub(qB, initAng[m.b]);
     V<T> n = m.nrm, t1 = m.t1, t2 = crossv(n, t1);
     T sg = isA ? T(1) : T(-1);
@@ -2302,6 +2309,7 @@ struct Oracle : oracle_base {
     }
     if (truncated) {
       pos[v] = anchor[v] + d;
+      lo[v] = V<T>();
       datTrunc++;
     }
   }
@@ -2440,7 +2448,7 @@ struct Oracle : oracle_base {
     s.ll = M<T>::diag(V<T>(1, 1, 1) * (m / dt2));
     M<T> In = worldInertia(rot[body], moment[body]) * (T(1) / dt2);
     s.aa = In;
-    s.gl = (pos[body] - inertLin[body]) * (m / dt2);
+    s.gl = ((pos[body] - inertLin[body]) + lo[body]) * (m / dt2);
     s.ga = In.mulv(qsub(rot[body], inertAng[body]));
     // reference order of the sorted CSR: joints, springs, manifolds, tets
     for (uint32_t e : staticAdj[body]) {
@@ -2481,7 +2489,16 @@ struct Oracle : oracle_base {
     if (l2 > maxLin * maxLin) dl = dl * (maxLin / std::sqrt(l2));
     T a2 = dotv(da, da);
     if (a2 > T(0.25)) da = da * (T(0.5) / std::sqrt(a2));
-    pos[body] = pos[body] + dl;
+    // (pos, lo) + dl renormalised exactly (TwoSum per component)
+    V<T> t = lo[body] + dl;
+    for (int k = 0; k < 3; k++) {
+      T a = pos[body][k], b = t[k];
+      T sum = a + b, bb = sum - a;
+      T err = (a - (sum - bb)) + (b - bb);
+      if (k == 0) { pos[body].x = sum; lo[body].x = err; }
+      else if (k == 1) { pos[body].y = sum; lo[body].y = err; }
+      else { pos[body].z = sum; lo[body].z = err; }
+    }
     if (!part) rot[body] = qaddw(rot[body], da);
   }
 
@@ -2719,7 +2736,10 @@ struct Oracle : oracle_base {
     // (dat_restore_failed_surface, 27_planar_dat.metal:1289-1310)
     if (datOn && (softCandidates > maxSoft || datPeak > maxDatPairs || datOverflow > 0 || datInvalid > 0))
       for (uint32_t i = 0; i < nb; i++)
-        if (radiusSigned[i] < 0) pos[i] = initLin[i];
+        if (radiusSigned[i] < 0) {
+          pos[i] = initLin[i];
+          lo[i] = V<T>();
+        }
     // BDF1 finalize (40_solver.metal:2108-2150)
     T dt = set.dt;
     T maxSpeed = std::max(T(30), T(1.5) * (T)cellSize / dt * T(0.5));
@@ -2727,7 +2747,7 @@ struct Oracle : oracle_base {
       V<T> vOld = vel[i];
       prevVel[i] = vOld;
       if (!(mass[i] > 0)) continue;
-      V<T> v = (pos[i] - initLin[i]) / dt;
+      V<T> v = ((pos[i] - initLin[i]) + lo[i]) / dt;
       T s2 = dotv(v, v);
       if (s2 > maxSpeed * maxSpeed) v = v * (maxSpeed / std::sqrt(s2));
       if (radiusSigned[i] < 0 && set.particle_damping > 0) {
@@ -2789,6 +2809,7 @@ struct Oracle : oracle_base {
     for (uint32_t k = 0; k < n; k++) {
       uint32_t i = idx ? idx[k] : k;
       pos[i] = V<T>(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+      lo[i] = V<T>();
       rot[i] = Q<T>{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
       vel[i] = V<T>(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
       angVel[i] = V<T>(w[3 * k], w[3 * k + 1], w[3 * k + 2]);

@@ -100,3 +100,38 @@ def test_snapshot_is_bound_to_rigid_scenes_and_its_solver():
     snap = a.captureRigidSpeculationSnapshot()
     with pytest.raises(AVBDError):
         b.restoreRigidSpeculationSnapshot(snap)
+
+
+def _manifold_bytes(g):
+    recs = g.manifolds()
+    return np.asarray(recs).tobytes() if not isinstance(recs, bytes) else recs
+
+
+@pytest.mark.parametrize("odd", [True, False])
+def test_rejected_restore_leaves_the_solver_unchanged(odd):
+    """A restore that is refused (a warm start attached since the capture
+    changes the snapshot's size) leaves the manifold buffer roles, the
+    manifolds and the next frames exactly as a solver that never tried it,
+    for odd and even frame distances."""
+    from avbd_amd import WarmStartMLP
+    _case(f"speculation snapshot: rejected restore is side-effect free (odd distance={odd})")
+    m = WarmStartMLP(seed=3, outputs=6)
+    a, b = GPUSolver(_scene()), GPUSolver(_scene())
+    for g in (a, b):
+        _run(g, 3, 0)
+    snap = a.captureRigidSpeculationSnapshot()
+    n = 3 if odd else 4
+    for g in (a, b):
+        _run(g, n, 3)
+        g.setWarmStart(rigid=m, learned=False, log=True)
+    with pytest.raises(AVBDError):
+        a.restoreRigidSpeculationSnapshot(snap)
+    assert _manifold_bytes(a) == _manifold_bytes(b)
+    sa, sb = _run(a, 3, 3 + n), _run(b, 3, 3 + n)
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
+    assert _manifold_bytes(a) == _manifold_bytes(b)
+    snap.close()
+    a.close()
+    b.close()
+    m.close()
