@@ -142,6 +142,7 @@ SIGNATURES = {
     "avbd_get_motor_targets": (C.c_int, [C.c_void_p, C.c_uint32, FP]),
     "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
+    "avbd_soft_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),
     "avbd_residual": (C.c_int, [C.c_void_p, FP]),
     "avbd_last_failure": (C.c_int, [C.c_void_p, C.POINTER(Failure)]),

@@ -331,6 +331,15 @@ class GPUSolver:
                     "mapFind")
         return out
 
+    def softMapFind(self, keys):
+        """Element-contact persistence map: (keyA, keyB) rows -> index of the
+        last frame's element contact with that key (softContacts order), or -1."""
+        q = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 2)
+        out = np.zeros(len(q), np.int32)
+        self._check(self._lib.avbd_soft_map_find(self._h, _u32p(q), len(q),
+                                                 out.ctypes.data_as(C.POINTER(C.c_int32))), "softMapFind")
+        return out
+
     def maxConstraintError(self):
         v = C.c_float()
         self._check(self._lib.avbd_max_constraint_error(self._h, C.byref(v)), "maxConstraintError")

@@ -1194,6 +1194,13 @@ __global__ void k_soft_finalize(DeviceState S, StepParams P) {
   S.counters[CTR_SOFT] = demand < P.maxSoft ? demand : P.maxSoft;
 }
 
+// debug lookup: (keyA, keyB) -> index of the previous frame's record, or -1
+__global__ void __launch_bounds__(256) k_softmap_query(DeviceState S, StepParams P, const U2* q, int32_t* out,
+                                                       uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = soft_map_find(S, P, q[i].x, q[i].y);
+}
+
 __global__ void __launch_bounds__(256) k_softmap_clear(DeviceState S, StepParams P) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.softMapCapacity; i += gridDim.x * blockDim.x)
     S.softMapKeyA[i] = 0u;
@@ -1324,6 +1331,12 @@ void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P)
   if (P.numTris == 0 || P.numParticles == 0 || !(P.clothViscosity > 0.0f)) return;
   hipLaunchKernelGGL(k_visc_copy, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
   hipLaunchKernelGGL(k_viscosity, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
+}
+
+void launch_soft_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, const U2* q, int32_t* out,
+                          uint32_t n) {
+  if (n == 0 || P.softMapCapacity == 0) return;
+  hipLaunchKernelGGL(k_softmap_query, dim3((n + 255) / 256), dim3(256), 0, st, S, P, q, out, n);
 }
 
 }  // namespace avbd

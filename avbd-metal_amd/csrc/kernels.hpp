@@ -41,6 +41,9 @@ void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParam
                            uint32_t count, uint32_t gridCap);
 void launch_dat_restore(hipStream_t st, const DeviceState& S, const StepParams& P);
 void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P);
+// debug lookup in the element-contact persistence map (keys -> previous record index or -1)
+void launch_soft_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, const U2* q, int32_t* out,
+                          uint32_t n);
 
 // solve.hip
 void launch_warmstart(hipStream_t st, const DeviceState& S, const StepParams& P);

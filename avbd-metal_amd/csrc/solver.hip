@@ -2462,21 +2462,41 @@ extern "C" avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity,
   return AVBD_OK;
 }
 
-extern "C" avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out) {
+// Runs one of the persistence-map lookup kernels over host query pairs.
+static avbd_status map_query(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out, bool soft) {
   if (!s || (n && (!pairs || !out))) return AVBD_ERR_INVALID_ARGUMENT;
   if (n == 0) return AVBD_OK;
   sync_all(s);
   U2* dq = nullptr;
   int32_t* dout = nullptr;
-  if (hipMalloc(&dq, 8 * n) || hipMalloc(&dout, 4 * n)) return AVBD_ERR_ALLOC;
-  hipMemcpy(dq, pairs, 8 * n, hipMemcpyHostToDevice);
-  StepParams P = make_params(s);
-  launch_map_find(s->stream, s->S, P, dq, dout, n);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost);
+  if (hipMalloc(&dq, 8ull * n) != hipSuccess) return AVBD_ERR_ALLOC;
+  if (hipMalloc(&dout, 4ull * n) != hipSuccess) {
+    hipFree(dq);
+    return AVBD_ERR_ALLOC;
+  }
+  hipError_t e = hipMemcpy(dq, pairs, 8ull * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    StepParams P = make_params(s);
+    if (soft) {
+      if (P.softMapCapacity && s->S.softMapKeyA) launch_soft_map_find(s->stream, s->S, P, dq, dout, n);
+      else e = hipMemset(dout, 0xFF, 4ull * n);  // no element contacts: every key is absent
+    } else {
+      launch_map_find(s->stream, s->S, P, dq, dout, n);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 4ull * n, hipMemcpyDeviceToHost);
   hipFree(dq);
   hipFree(dout);
   return e == hipSuccess ? AVBD_OK : AVBD_ERR_HIP;
+}
+
+extern "C" avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out) {
+  return map_query(s, pairs, n, out, false);
+}
+
+extern "C" avbd_status avbd_soft_map_find(avbd_solver* s, const uint32_t* keys, uint32_t n, int32_t* out) {
+  return map_query(s, keys, n, out, true);
 }
 
 extern "C" avbd_status avbd_max_constraint_error(avbd_solver* s, float* out) {
@@ -2551,7 +2571,6 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   if (!s || reps <= 0) return AVBD_ERR_INVALID_ARGUMENT;
   avbd_status st = sync_all(s);
   if (st != AVBD_OK) return st;
-  if (s->dynamicColoring) return AVBD_ERR_UNSUPPORTED;
   DeviceState S = s->S;
   std::swap(S.manifolds, S.prevManifolds);  // the last frame's contacts
   std::swap(S.soft, S.prevSoft);
@@ -2599,11 +2618,20 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   // the `reps` sweeps are captured in one hipGraph and replayed (warm-up
   // replay, then one timed replay), so the events bracket device-side
   // dispatch only, as the frame's own graph does
+  // a dynamic palette runs its first colors one launch each and the rest in
+  // one ordered tail launch, as the frame does
   auto sweeps = [&]() {
-    for (int r = 0; r < reps; r++)
-      for (uint32_t c = 0; c < s->staticUsedColors; c++)
-        launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes,
-                            s->colorMode[c]);
+    for (int r = 0; r < reps; r++) {
+      if (s->dynamicColoring) {
+        for (uint32_t c = 0; c < kDynamicColorLaunches; c++)
+          launch_primal_color(s->stream, S, P, c, 0xFFFFFFFFu, 0u, s->lanes, kModeGeneral);
+        launch_primal_tail(s->stream, S, P, kDynamicColorLaunches, s->lanes);
+      } else {
+        for (uint32_t c = 0; c < s->staticUsedColors; c++)
+          launch_primal_color(s->stream, S, P, c, s->staticColorCounts[c], s->staticColorStarts[c], s->lanes,
+                              s->colorMode[c]);
+      }
+    }
   };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2618,6 +2646,16 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
     hipGetLastError();
     return AVBD_ERR_HIP;
   }
+  // a residual-terminated frame leaves its end flag set until the next
+  // frame starts, and every sweep kernel returns at entry while it is set:
+  // clear it for the timed sweeps and put it back afterwards
+  uint32_t convDone = 0;
+  he = hipMemcpy(&convDone, s->S.counters + CTR_CONV_DONE, 4, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemsetAsync(s->S.counters + CTR_CONV_DONE, 0, 4, s->stream);
+  if (he != hipSuccess) {
+    hipGraphExecDestroy(exec);
+    return AVBD_ERR_HIP;
+  }
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -2626,13 +2664,14 @@ extern "C" avbd_status avbd_time_primal(avbd_solver* s, int32_t reps, float* avg
   hipGraphLaunch(exec, s->stream);
   hipEventRecord(e1, s->stream);
   he = hipEventSynchronize(e1);
+  if (he == hipSuccess) he = hipMemcpy(s->S.counters + CTR_CONV_DONE, &convDone, 4, hipMemcpyHostToDevice);
   float ms = 0.0f;
   hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipGraphExecDestroy(exec);
   if (he != hipSuccess) return AVBD_ERR_HIP;
-  int32_t launches = (int32_t)s->staticUsedColors;
+  int32_t launches = s->dynamicColoring ? (int32_t)kDynamicColorLaunches + 1 : (int32_t)s->staticUsedColors;
   if (avg_launch_us) *avg_launch_us = ms * 1000.0f / (float)(reps * launches);
   if (sweep_bytes) *sweep_bytes = bytes;
   if (launches_per_sweep) *launches_per_sweep = launches;

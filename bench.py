@@ -1,25 +1,33 @@
-"""Benchmark: AVBD step throughput on the 21^3-vertex soft block.
+"""Benchmark: AVBD step throughput and step rate to residual 1e-4 on one MI355X.
 
-Workload (one step = the whole AVBD frame: hash, pairs, narrow phase,
-persistence map, inertial predict, contact adjacency, 20 iterations of the
-static-colored primal sweep + dual, BDF1 finalize): a 20x20x20-cell tet block
-(9261 vertices, 48000 Kuhn tets, 1 m), Stable Neo-Hookean E = 1e5 Pa,
-nu = 0.4, 1000 kg/m^3, dropped from 0.5 m with seeded N(0, 0.1) m/s velocity
-onto the ground, dt = 1/60.  Inputs are resident on the GPU; the timed region
-is K steps bracketed by device synchronisation.
+Headline workload (one step = the whole AVBD frame: hash, pairs, narrow
+phase, persistence map, inertial predict, contact adjacency, 20 iterations
+of the static-colored primal sweep + dual, BDF1 finalize): the soft block --
+a 20x20x20-cell tet grid (9261 vertices, 48000 Kuhn tets, 1 m), Stable
+Neo-Hookean E = 1e5 Pa, nu = 0.4, 1000 kg/m^3, dropped from 0.5 m with seeded
+N(0, 0.1) m/s velocity onto the ground, dt = 1/60.  Inputs are resident on the
+GPU; the timed region is K steps bracketed by device synchronisation.
+value = dynamic vertices x iterations x steps / wall time.
 
-Prints one JSON line: value = vertex-iters/s (dynamic vertices x iterations
-x steps / wall time), the step rate when every frame iterates to residual
-1e-4 (cap 200 sweeps), the primal color-sweep roofline measured live with
-HIP events, the oracle CPU baseline on a bounded sample, and the learned
-warm start against the plain one (frames iterated to residual 1e-3 -- the
-FP32 iterate of this block floors near 5e-4 -- and the residual after the
-fixed 20 sweeps), with the MLP forward's matrix-core roofline.
-
---workload cloth / batched run the 256^2 cloth drape or the 4096 batched
-rigid scenes instead (reported the same way; the default is the soft block).
+The same JSON line carries:
+  * roofline: the primal color sweep (the dominant kernel) timed live with
+    HIP events on the solver stream, algorithmic bytes per launch from the
+    per-stamp record model, DRAM traffic from the committed PMC summary;
+  * config.residual_1e-4: frames of the block iterated until the residual is
+    <= 1e-4 (sweep cap --res-cap, default 1000: the FP32 and FP64 oracles need
+    110-370 sweeps per frame on this block, beyond the metric's 200), with
+    the plain gamma warm start and with the learned start, and their ratio;
+  * config.error_vs_fp64 for every configuration: the backend's trajectory
+    against the FP64 oracle rerun of the same frames;
+  * config.configs: the 10-cube stack (GPU and CPU ms/step), the 256^2
+    cloth drape and the 4096 batched rigid scenes with their own
+    rooflines, the learned start evaluated on the 1M-tet block and the 256^2
+    cloth, and batched + learned vs per-scene plain solves at 1e-4;
+  * cpu_baseline: the oracle (the C++ restatement of the reference's CPU
+    path) on one core and on every allowed core, with the CPU model.
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -31,13 +39,19 @@ for p in (ROOT, os.path.join(ROOT, "avbd-metal_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 METRIC = "vertex-iters/s + ms/step to residual 1e-4; batched+learned vs plain AVBD"
+MLP_FLOPS_PER_ROW = 2 * (24 * 128 + 128 * 128 + 128 * 3)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles(name):
     """HBM bytes per primal launch from a committed rocprofv3 PMC summary, when present."""
-    path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path):
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
         return None
     try:
         with open(path) as f:
@@ -46,37 +60,53 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(scene, unit, target_s=10.0):
-    """Oracle (C++ FP32 restatement of the reference CPU path) on the host:
-    one core (the serial Gauss-Seidel, the reported value), then every color
-    split over the host's cores (identical results), each on a bounded
-    sample of about target_s seconds."""
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = nproc
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(allowed, omp) if omp > 0 else allowed
+    return model, nproc, allowed, max(1, threads)
+
+
+def rms(a, b):
+    import numpy as np
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+
+
+def error_vs_fp64(scene, steps, threads):
+    """The backend, the FP32 oracle and the FP64 oracle stepped `steps`
+    frames from the same initial state: RMS / max position error against
+    FP64 (metres)."""
+    import numpy as np
     import oracle
+    from avbd_amd import GPUSolver
 
-    def rate(threads, budget):
-        o = oracle.OracleSolver(scene)
-        o.set_threads(threads)
-        o.step()  # warm-up step (first contact detection, caches)
-        t0 = time.perf_counter()
-        o.step()
-        one = time.perf_counter() - t0
-        n = max(1, min(500, int(budget / max(one, 1e-3))))
-        t0 = time.perf_counter()
-        o.step(n)
-        dt = time.perf_counter() - t0
-        return scene.numDynamic * scene.settings.iterations * n / dt, n, dt
-
-    v1, n1, dt1 = rate(1, target_s)
-    cores = max(1, min(16, os.cpu_count() or 1))
-    out = {"value": v1, "unit": unit, "cores": 1, "kind": "port",
-           "sample": f"{n1} steps of the same workload after 2 warm-up steps (about {target_s:.0f} s of work), "
-                     f"serial oracle FP32, {dt1:.2f} s"}
-    if cores > 1:
-        vn, nn, dtn = rate(cores, target_s / 2)
-        out["all_cores"] = {"value": vn, "cores": cores,
-                            "sample": f"{nn} steps, each color's bodies split over {cores} OpenMP threads, "
-                                      f"{dtn:.2f} s"}
-    return out
+    with GPUSolver(scene) as g:
+        o32 = oracle.OracleSolver(scene)
+        o64 = oracle.OracleSolver(scene, precision=64)
+        o32.set_threads(threads)
+        o64.set_threads(threads)
+        g.step(steps)
+        o32.step(steps)
+        o64.step(steps)
+        pg = g.positions()
+    p32, p64 = o32.positions(), o64.bodyStates64()[0]
+    return {"steps": steps, "gpu_vs_fp64_rms_m": rms(pg, p64),
+            "gpu_vs_fp64_max_m": float(np.abs(pg - p64).max()),
+            "oracle_fp32_vs_fp64_rms_m": rms(p32, p64), "gpu_vs_oracle_fp32_rms_m": rms(pg, p32)}
 
 
 def primal_roofline(solver, pmc):
@@ -87,187 +117,271 @@ def primal_roofline(solver, pmc):
     bytes_per_launch = tp["sweep_bytes"] / tp["launches_per_sweep"]
     achieved = bytes_per_launch / (tp["avg_launch_us"] * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(pmc) if pmc else None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(pmc),
             "kernel": f"k_primal<{st.lanes_per_body}>", "avg_launch_us": round(tp["avg_launch_us"], 3),
-            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
+            "launches_per_sweep": tp["launches_per_sweep"]}
 
 
-def learned_comparison(args):
-    """Learned vs plain warm start: training pairs from the plain solver on
-    other seeds of the --train-cells block (every frame iterated to residual
-    2e-4 or 200 sweeps), GPU training, then frames of the --eval-cells block
-    (default: the metric's scene) iterated to `tol` with and without the
-    learned start (plain fallback active), the residual after the fixed
-    iteration count, and the MLP forward roofline on the evaluated rows."""
+def timed_steps(solver, steps, warmup):
+    for _ in range(warmup):
+        solver.submitStep()
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.submitStep()
+    solver.synchronize()
+    return time.perf_counter() - t0
+
+
+def frames_to_tol(solver, frames, tol, cap, check=10, learned=False):
+    """`frames` frames, each iterated until residual <= tol (at most `cap`
+    sweeps, checked on the device every `check` sweeps)."""
     import numpy as np
-    from avbd_amd import GPUSolver, WarmStartMLP, demos
+    its, res, kept = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        n, r = solver.stepToResidual(tol, cap, check)
+        its.append(n)
+        res.append(r)
+        if learned:
+            kept.append(solver.stats().ws_accepted)
+    wall = time.perf_counter() - t0
+    its, res = np.array(its), np.array(res)
+    out = {"ms_per_step": round(wall * 1e3 / frames, 4), "sweeps_mean": round(float(its.mean()), 2),
+           "sweeps_max": int(its.max()), "reached_fraction": round(float(np.mean(res <= tol)), 4),
+           "residual_median": float(np.median(res))}
+    if learned:
+        out["learned_start_kept"] = round(float(np.mean(kept)), 3)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# learned warm start
+# ---------------------------------------------------------------------------
+def train_network(args):
+    """Training pairs from the plain solver on other seeds of the 20^3 soft
+    block and of 64^2 sheets of the cloth-drape generator, every frame
+    iterated to the label tolerance; GPU Adam on the project's MFMA forward /
+    backward kernels."""
+    from avbd_amd import WarmStartMLP, demos
     from avbd_amd.learned import collect_samples
+    import numpy as np
 
     t0 = time.perf_counter()
-    train = [demos.soft_block_scene(cells=args.train_cells, seed=10 + k) for k in range(args.train_seeds)]
-    X, T, label_its = collect_samples(train, args.train_frames)
+    blocks = [demos.soft_block_scene(cells=20, seed=10 + k) for k in range(args.train_seeds)]
+    Xb, Tb, itb = collect_samples(blocks, args.train_frames, tol=args.label_tol, max_iterations=args.label_cap,
+                                  check_every=10)
+    sheets = [demos.cloth_scene(n=64, size=0.5, height=0.3, radius=0.002, sphere_radius=0.1, seed=20 + k)
+              for k in range(args.train_cloth_seeds)]
+    Xc, Tc, itc = collect_samples(sheets, args.train_frames, tol=args.label_tol, max_iterations=args.label_cap,
+                                  check_every=10) if sheets else (np.zeros((0, 24), np.float32),
+                                                                  np.zeros((0, 3), np.float32), 0.0)
+    X, T = np.concatenate([Xb, Xc]), np.concatenate([Tb, Tc])
     t_collect = time.perf_counter() - t0
-    m = WarmStartMLP(seed=1)
+    net = WarmStartMLP(seed=1)
     t0 = time.perf_counter()
-    losses = m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
-    t_train = time.perf_counter() - t0
-
-    eval_cells = args.eval_cells or args.cells
-    eval_scene = demos.soft_block_scene(cells=eval_cells)
-    nv = eval_scene.numDynamic
-
-    def run(learned):
-        s = eval_scene
-        g = GPUSolver(s)
-        if learned:
-            g.setWarmStart(m, smooth=2)
-        its, acc, reached = [], [], 0
-        t0 = time.perf_counter()
-        for _ in range(args.learned_frames):
-            n, r = g.stepToResidual(args.learned_tol, 200, 5)
-            its.append(n)
-            reached += r <= args.learned_tol
-            acc.append(g.stats().ws_accepted)
-        wall = time.perf_counter() - t0
-        g2 = GPUSolver(s)
-        if learned:
-            g2.setWarmStart(m, smooth=2)
-        res = []
-        for _ in range(args.learned_frames):
-            g2.step()
-            res.append(g2.residual())
-        return {"iterations_mean": round(float(np.mean(its)), 2), "ms_per_step": round(wall * 1000 / len(its), 4),
-                "reached_fraction": reached / len(its),
-                "learned_start_kept": round(float(np.mean(acc)), 3) if learned else None,
-                "residual_after_fixed_iterations_mean": float(np.mean(res)),
-                "residual_after_fixed_iterations_median": float(np.median(res))}
-
-    plain, learned = run(False), run(True)
-    us = m.timeForward(nv, reps=50)
-    flops = 2.0 * nv * (24 * 128 + 128 * 128 + 128 * 3)
-    tflops = flops / (us * 1e-6) / 1e12
-    out = {"evaluated_on": f"soft block {eval_cells}^3 cells ({nv} vertices, {len(eval_scene.tets)} tets)",
-           "trained_on": f"soft block {args.train_cells}^3 cells, seeds 10..{9 + args.train_seeds}",
-           "tolerance": args.learned_tol, "frames": args.learned_frames, "max_iterations": 200,
-           "training": {"samples": int(len(X)), "scenes": f"{args.train_seeds} seeds x {args.train_frames} frames",
-                        "label_sweeps_mean": round(label_its, 1), "collect_s": round(t_collect, 2),
-                        "epochs": args.epochs, "train_s": round(t_train, 2),
-                        "loss_first_last": [float(losses[0]), float(losses[-1])], "target_variance": float(T.var())},
-           "plain": plain, "learned": learned,
-           "step_rate_ratio": round(plain["ms_per_step"] / learned["ms_per_step"], 3),
-           "iteration_ratio": round(plain["iterations_mean"] / learned["iterations_mean"], 3)}
-    roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
-            "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_mlp_forward",
-            "avg_launch_us": round(us, 3), "rows": nv, "flops_per_row": 2 * (24 * 128 + 128 * 128 + 128 * 3)}
-    return out, roof
+    losses = net.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
+    info = {"samples": int(len(X)), "block_samples": int(len(Xb)), "cloth_samples": int(len(Xc)),
+            "scenes": f"soft block 20^3 seeds 10..{9 + args.train_seeds} and 64^2 cloth seeds "
+                      f"20..{19 + args.train_cloth_seeds}, {args.train_frames} frames each",
+            "label_tolerance": args.label_tol, "label_sweep_cap": args.label_cap,
+            "label_sweeps_mean": {"block": round(itb, 1), "cloth": round(itc, 1)},
+            "collect_s": round(t_collect, 2), "epochs": args.epochs, "train_s": round(time.perf_counter() - t0, 2),
+            "loss_first_last": [float(losses[0]), float(losses[-1])], "target_variance": float(T.var())}
+    return net, info
 
 
-def batched_learned_comparison(args):
-    """Batched + learned vs plain AVBD on the same uneven soft blocks: the
-    plain path steps each block with its own solver (plain warm start), the
-    batched paths step all blocks in one solver, without and with the learned
-    start (trained on other draws of the same generator).  Frames iterate to
-    residual `tol` (per solver: each block for the plain path, all blocks
-    together for the batched paths), cap 200 sweeps."""
+def plain_vs_learned(scene, net, frames, tol, cap, fixed_res=True):
+    """The same frames of `scene` iterated to `tol` from the plain start and
+    from the learned start (plain fallback active); also the residual after
+    the scene's fixed iteration count."""
     import numpy as np
-    from avbd_amd import GPUSolver, WarmStartMLP, demos
-    from avbd_amd.learned import collect_samples
+    from avbd_amd import GPUSolver
 
-    n, frames, tol = args.batch_scenes, args.batch_frames, args.learned_tol
-    t0 = time.perf_counter()
-    train = [demos.batched_soft_scene(n, seed_base=(6 << 32) + 1000 * k) for k in range(2)]
-    X, T, _ = collect_samples(train, args.train_frames)
-    m = WarmStartMLP(seed=1)
-    m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
-    t_train = time.perf_counter() - t0
+    out = {}
+    for name, learned in (("plain", False), ("learned", True)):
+        with GPUSolver(scene) as g:
+            if learned:
+                g.setWarmStart(net, smooth=2)
+            r = frames_to_tol(g, frames, tol, cap, learned=learned)
+        if fixed_res:
+            with GPUSolver(scene) as g:
+                if learned:
+                    g.setWarmStart(net, smooth=2)
+                res = []
+                for _ in range(frames):
+                    g.step()
+                    res.append(g.residual())
+                r["residual_after_fixed_iterations_median"] = float(np.median(res))
+        out[name] = r
+    out["step_rate_ratio"] = round(out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)
+    out["sweep_ratio"] = round(out["plain"]["sweeps_mean"] / max(out["learned"]["sweeps_mean"], 1e-9), 3)
+    return out
 
-    def frames_to_tol(solvers):
+
+def batched_vs_plain(args, net):
+    """Batched + learned vs plain AVBD on the same uneven soft blocks at
+    residual `tol`: the plain path steps every block with its own solver
+    (plain warm start, graph-captured frames terminated on the device), the
+    batched paths step all blocks in one solver, without and with the
+    learned start."""
+    from avbd_amd import GPUSolver, demos
+
+    n, frames, tol, cap = args.batch_scenes, args.batch_frames, 1e-4, args.res_cap
+    specs = demos.batched_soft_scene(n, separate=True)
+    with contextlib.ExitStack() as es:
+        sep = [es.enter_context(GPUSolver(sc)) for sc in specs]
+        import numpy as np
         its = []
         t0 = time.perf_counter()
         for _ in range(frames):
-            for g in solvers:
-                k, _ = g.stepToResidual(tol, 200, 5)
+            for g in sep:
+                k, _ = g.stepToResidual(tol, cap, 10)
                 its.append(k)
-        return (time.perf_counter() - t0) * 1000.0 / frames, float(np.mean(its))
-
-    sep = [GPUSolver(sc) for sc in demos.batched_soft_scene(n, separate=True)]
-    ms_plain, it_plain = frames_to_tol(sep)
-    for g in sep:
-        g.close()
+        ms_plain = (time.perf_counter() - t0) * 1e3 / frames
+        it_plain = float(np.mean(its))
     scene = demos.batched_soft_scene(n)
-    gb = GPUSolver(scene)
-    ms_batched, it_batched = frames_to_tol([gb])
-    gl = GPUSolver(scene)
-    gl.setWarmStart(m, smooth=2)
-    ms_learned, it_learned = frames_to_tol([gl])
+    with GPUSolver(scene) as gb:
+        b = frames_to_tol(gb, frames, tol, cap)
+    with GPUSolver(scene) as gl:
+        gl.setWarmStart(net, smooth=2)
+        bl = frames_to_tol(gl, frames, tol, cap, learned=True)
     return {"scenes": n, "vertices": scene.numDynamic, "tets": len(scene.tets), "frames": frames,
-            "tolerance": tol, "train_s": round(t_train, 2), "train_samples": int(len(X)),
-            "plain_per_scene_solvers": {"ms_per_step": round(ms_plain, 3), "iterations_mean": round(it_plain, 2)},
-            "batched_plain": {"ms_per_step": round(ms_batched, 3), "iterations_mean": round(it_batched, 2)},
-            "batched_learned": {"ms_per_step": round(ms_learned, 3), "iterations_mean": round(it_learned, 2)},
-            "step_rate_ratio_batched_learned_vs_plain": round(ms_plain / ms_learned, 2),
-            "step_rate_ratio_batching_only": round(ms_plain / ms_batched, 2),
-            "step_rate_ratio_learned_only": round(ms_batched / ms_learned, 3)}
+            "tolerance": tol, "sweep_cap": cap,
+            "plain_per_scene_solvers": {"ms_per_step": round(ms_plain, 3), "sweeps_mean": round(it_plain, 2)},
+            "batched_plain": b, "batched_learned": bl,
+            "step_rate_ratio_batched_learned_vs_plain": round(ms_plain / bl["ms_per_step"], 2),
+            "step_rate_ratio_batching_only": round(ms_plain / b["ms_per_step"], 2),
+            "step_rate_ratio_learned_only": round(b["ms_per_step"] / bl["ms_per_step"], 3)}
 
 
-def rigid_learned_comparison(args, scene):
-    """Batched rigid scenes with the learned start of the rigid bodies (a
-    6-output network trained on other draws of the generator, labels from the
-    plain solver at 100 sweeps) against the plain guess: the residual after
-    the fixed iteration count over `frames` frames."""
+# ---------------------------------------------------------------------------
+# other configurations
+# ---------------------------------------------------------------------------
+def config_stack(args, threads):
+    """10 unit cubes stacked with +-1 cm jitter, mu = 0.5, 10 iterations,
+    240 steps, on the GPU and on the oracle (the CPU path), with the FP64 rerun."""
     import numpy as np
-    from avbd_amd import GPUSolver, WarmStartMLP, demos
-    from avbd_amd.learned import collect_samples
+    import oracle
+    from avbd_amd import GPUSolver, demos
 
+    s = demos.stack_scene()
+    with GPUSolver(s) as g:
+        wall = timed_steps(g, 240, 10)
+        pg = g.positions()
+    o = oracle.OracleSolver(s)
+    o.step(10)
     t0 = time.perf_counter()
-    train = [demos.batched_scene(64, seed_base=(7 << 32) + 1000 * k)[0] for k in range(3)]
-    X, T, _ = collect_samples(train, 60, tol=1e-4, max_iterations=100, check_every=10, rigid=True)
-    m = WarmStartMLP(seed=1, outputs=6)
-    losses = m.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=2)
-    t_train = time.perf_counter() - t0
-
-    def run(learned):
-        g = GPUSolver(scene)
-        if learned:
-            g.setWarmStart(rigid=m)
-        res, kept = [], []
-        t0 = time.perf_counter()
-        for _ in range(args.learned_frames):
-            g.step()
-            res.append(g.residual())
-            kept.append(g.stats().ws_accepted)
-        wall = time.perf_counter() - t0
-        return {"residual_after_fixed_iterations_median": float(np.median(res)),
-                "residual_after_fixed_iterations_mean": float(np.mean(res)),
-                "ms_per_step_with_residual_readback": round(wall * 1000 / len(res), 4),
-                "learned_start_kept": round(float(np.mean(kept)), 3) if learned else None}
-
-    plain, learned = run(False), run(True)
-    return {"trained_on": "3 x 64 batched rigid scenes (other seeds), 60 frames, labels at 100 sweeps",
-            "train_samples": int(len(X)), "train_s": round(t_train, 2),
-            "loss_first_last": [float(losses[0]), float(losses[-1])], "frames": args.learned_frames,
-            "plain": plain, "learned": learned,
-            "residual_ratio_median": round(plain["residual_after_fixed_iterations_median"] /
-                                           max(learned["residual_after_fixed_iterations_median"], 1e-30), 3)}
+    o.step(240)
+    cpu_ms = (time.perf_counter() - t0) * 1e3 / 240
+    o64 = oracle.OracleSolver(s, precision=64)
+    o64.step(250)
+    p64 = o64.bodyStates64()[0]
+    top = int(np.argmax(p64[:, 2]))
+    return {"workload": "10-cube stack (1 m cubes, +-1 cm jitter, mu 0.5), 10 iterations, dt 1/60",
+            "gpu_ms_per_step": round(wall * 1e3 / 240, 4), "cpu_oracle_ms_per_step": round(cpu_ms, 4),
+            "cpu_cores": 1,
+            "error_vs_fp64": {"steps": 250, "gpu_vs_fp64_rms_m": rms(pg, p64),
+                              "gpu_vs_fp64_max_m": float(np.abs(pg - p64).max()),
+                              "top_cube_height_fp64_m": float(p64[top, 2])}}
 
 
-def build(workload, cells):
+def config_cloth(args, threads):
+    """The 256^2 cloth drape onto a fixed sphere, 30 iterations."""
+    from avbd_amd import GPUSolver, demos
+
+    s = demos.cloth_scene()
+    with GPUSolver(s) as g:
+        wall = timed_steps(g, args.side_steps, 5)
+        st = g.stats()
+        roof = primal_roofline(g, "primal_cloth_256_pmc.json")
+        res = g.residual()
+    it = s.settings.iterations
+    return {"workload": f"cloth drape 256^2 ({s.numDynamic} vertices, {len(s.tris)} triangles), "
+                        f"{it} iterations, dt 1/60",
+            "ms_per_step": round(wall * 1e3 / args.side_steps, 4),
+            "vertex_iters_per_s": round(s.numDynamic * it * args.side_steps / wall, 1),
+            "soft_contacts_last_frame": st.num_soft, "dat_pairs_last_frame": st.dat_pairs,
+            "residual_after_fixed_iterations": res, "roofline": roof,
+            "error_vs_fp64": error_vs_fp64(s, args.fp64_cloth_steps, threads)}
+
+
+def config_batched(args, threads):
+    """4096 batched uneven rigid scenes in one solver, 10 iterations."""
+    import numpy as np
+    import oracle
+    from avbd_amd import GPUSolver, demos
+
+    s, _ = demos.batched_scene()
+    with GPUSolver(s) as g:
+        wall = timed_steps(g, args.side_steps, 5)
+        st = g.stats()
+        roof = primal_roofline(g, None)
+    it = s.settings.iterations
+    out = {"workload": f"4096 batched uneven rigid scenes ({s.numDynamic} bodies, {len(s.joints)} hinges), "
+                       f"{it} iterations, dt 1/60, contact-aware dynamic coloring",
+           "ms_per_step": round(wall * 1e3 / args.side_steps, 4),
+           "body_iters_per_s": round(s.numDynamic * it * args.side_steps / wall, 1),
+           "colors": st.num_colors, "lanes_per_body": st.lanes_per_body,
+           "sweep_mode": "persistent" if st.sweep_mode == 2 else "per-color launches", "roofline": roof}
+    small, _ = demos.batched_scene(64)
+    out["error_vs_fp64"] = dict(error_vs_fp64(small, args.fp64_batched_steps, threads),
+                                scenes="the first 64 of the 4096 scenes")
+    # why these scenes are not iterated to 1e-4: the FP64 oracle's residual after 2000 sweeps
+    o64 = oracle.OracleSolver(demos.batched_scene(8)[0], precision=64)
+    o64.set_threads(threads)
+    o64.step(3)
+    n, r = o64.stepToResidual(1e-4, 2000, 50)
+    out["fp64_residual_after_2000_sweeps"] = {"scenes": 8, "frame": 4, "sweeps": n, "residual": r}
+    return out
+
+
+def config_learned_at_scale(args, net):
+    """The learned start (trained on the 20^3 block and 64^2 sheets)
+    evaluated on the 1M-tet block (55^3 cells) and the 256^2 cloth."""
     from avbd_amd import demos
 
-    if workload == "cloth":
-        s = demos.cloth_scene()
-        desc = (f"cloth drape 256^2 ({s.numDynamic} vertices, {len(s.tris)} triangles, {len(s.springs)} "
-                f"structural springs) onto a fixed r = 0.5 m sphere, {s.settings.iterations} iterations, dt 1/60")
-        return s, desc, "vertex-iters/s", None
-    if workload == "batched":
-        s, ranges = demos.batched_scene()
-        desc = (f"4096 batched uneven rigid scenes ({s.numDynamic} bodies, {len(s.joints)} hinges) in one solver, "
-                f"{s.settings.iterations} iterations, dt 1/60, contact-aware dynamic coloring")
-        return s, desc, "body-iters/s", None
-    s = demos.soft_block_scene(cells=cells)
-    desc = (f"soft block {cells}^3 cells, {s.numDynamic} vertices, {len(s.tets)} SNH tets, "
-            f"{s.settings.iterations} iterations, dt 1/60, static coloring")
-    return s, desc, "vertex-iters/s", f"primal_soft_block_{cells + 1}_pmc.json"
+    big = demos.soft_block_scene(cells=55)
+    out = {"block_55": dict(plain_vs_learned(big, net, args.c5_frames, 1e-4, args.res_cap),
+                            workload=f"soft block 55^3 cells ({big.numDynamic} vertices, {len(big.tets)} tets)")}
+    cloth = demos.cloth_scene()
+    out["cloth_256"] = dict(plain_vs_learned(cloth, net, args.c5_frames, 1e-4, args.c5_cloth_cap),
+                            workload="cloth drape 256^2")
+    return out
+
+
+def cpu_baseline(scene, unit, threads_all, target_s=10.0):
+    """Oracle (C++ FP32 restatement) on the host: one core (the serial
+    Gauss-Seidel, the reported value), then each color split over every
+    allowed core (identical results), each on a bounded sample."""
+    import oracle
+
+    def rate(threads, budget):
+        o = oracle.OracleSolver(scene)
+        o.set_threads(threads)
+        o.step()
+        t0 = time.perf_counter()
+        o.step()
+        one = time.perf_counter() - t0
+        n = max(1, min(500, int(budget / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        o.step(n)
+        dt = time.perf_counter() - t0
+        return scene.numDynamic * scene.settings.iterations * n / dt, n, dt
+
+    model, nproc, allowed, _ = cpu_info()
+    v1, n1, dt1 = rate(1, target_s)
+    out = {"value": v1, "unit": unit, "cores": 1, "kind": "port",
+           "sample": f"{n1} steps of the same workload after 2 warm-up steps (about {target_s:.0f} s of work), "
+                     f"serial oracle FP32, {dt1:.2f} s",
+           "cpu_model": model, "nproc": nproc, "allowed_cpus": allowed}
+    if threads_all > 1:
+        vn, nn, dtn = rate(threads_all, target_s / 2)
+        out["all_cores"] = {"value": vn, "cores": threads_all,
+                            "sample": f"{nn} steps, each color's bodies split over {threads_all} OpenMP threads "
+                                      f"(every allowed CPU, OMP_NUM_THREADS respected), {dtn:.2f} s"}
+    return out
 
 
 def main():
@@ -276,19 +390,25 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cells", type=int, default=20)
-    ap.add_argument("--workload", choices=["soft_block", "cloth", "batched"], default="soft_block")
-    ap.add_argument("--residual-steps", type=int, default=30, help="frames iterated to residual 1e-4")
+    ap.add_argument("--res-frames", type=int, default=40, help="frames of the block iterated to residual 1e-4")
+    ap.add_argument("--res-cap", type=int, default=1000, help="sweep cap of a frame iterated to 1e-4")
+    ap.add_argument("--quick", action="store_true", help="headline, roofline and residual only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain warm start comparison")
-    ap.add_argument("--learned-tol", type=float, default=1e-3)
-    ap.add_argument("--learned-frames", type=int, default=60)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-seeds", type=int, default=3)
-    ap.add_argument("--train-frames", type=int, default=60)
+    ap.add_argument("--train-cloth-seeds", type=int, default=2)
+    ap.add_argument("--train-frames", type=int, default=40)
+    ap.add_argument("--label-tol", type=float, default=1e-5)
+    ap.add_argument("--label-cap", type=int, default=1500)
     ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--train-cells", type=int, default=20, help="block of the learned start's training data")
-    ap.add_argument("--eval-cells", type=int, default=0, help="block it is evaluated on (0 = --cells)")
-    ap.add_argument("--batch-scenes", type=int, default=64, help="uneven soft blocks of the batched comparison")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=30)
+    ap.add_argument("--c5-frames", type=int, default=20)
+    ap.add_argument("--c5-cloth-cap", type=int, default=300)
+    ap.add_argument("--side-steps", type=int, default=60, help="timed steps of the cloth / batched sub-results")
+    ap.add_argument("--fp64-steps", type=int, default=30)
+    ap.add_argument("--fp64-cloth-steps", type=int, default=3)
+    ap.add_argument("--fp64-batched-steps", type=int, default=30)
     ap.add_argument("--scale-cells", type=int, default=55, help="block of the at-scale primal roofline (0 = skip)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body (0 = autotune)")
     ap.add_argument("--sweep", type=int, default=0, help="1 per-color launches, 2 persistent (0 = autotune)")
@@ -298,78 +418,79 @@ def main():
               f"(independent scenes would shard per GPU with no per-step exchange)", file=sys.stderr)
         sys.exit(2)
 
-    from avbd_amd import GPUSolver
+    from avbd_amd import GPUSolver, demos
 
-    scene, desc, unit, pmc = build(args.workload, args.cells)
-    solver = GPUSolver(scene, lanesPerBody=args.lanes, sweepMode=args.sweep)
-    st = solver.stats()
-    for _ in range(args.warmup):
-        solver.submitStep()
-    solver.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solver.submitStep()
-    solver.synchronize()
-    wall = time.perf_counter() - t0
-    iters = scene.settings.iterations
-    nv = scene.numDynamic
-    value = nv * iters * args.steps / wall
-    ms_step = wall * 1000.0 / args.steps
-    residual = solver.residual()
-    st = solver.stats()
-
-    # the second half of the metric: frames iterated until residual <= 1e-4 (cap 200)
-    res = None
-    if args.residual_steps > 0:
-        its, reached = [], 0
-        t0 = time.perf_counter()
-        for _ in range(args.residual_steps):
-            n, r = solver.stepToResidual(1e-4, 200, 5)
-            its.append(n)
-            reached += r <= 1e-4
-        rwall = time.perf_counter() - t0
-        res = {"tolerance": 1e-4, "max_iterations": 200, "check_every": 5, "steps": args.residual_steps,
-               "ms_per_step": round(rwall * 1000.0 / args.residual_steps, 4),
-               "iterations_mean": round(sum(its) / len(its), 2), "reached_fraction": reached / len(its)}
-
-    out = {"metric": METRIC, "value": round(value, 1), "unit": unit, "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+    model, nproc, allowed, threads = cpu_info()
+    scene = demos.soft_block_scene(cells=args.cells)
+    iters, nv = scene.settings.iterations, scene.numDynamic
+    desc = (f"soft block {args.cells}^3 cells, {nv} vertices, {len(scene.tets)} SNH tets, "
+            f"{iters} iterations, dt 1/60, static coloring")
+    out = {"metric": METRIC, "unit": "vertex-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (seeded SplitMix64 scene)",
-           "config": {"workload": desc, "dynamic_bodies": nv, "iterations": iters,
-                      "colors": st.num_colors, "lanes_per_body": st.lanes_per_body,
-                      "sweep_mode": "persistent" if st.sweep_mode == 2 else "per-color launches",
-                      "end_residual": residual, "residual_1e-4": res}}
-    if args.workload == "cloth":
-        out["config"].update({"soft_contacts_last_frame": st.num_soft, "dat_pairs_last_frame": st.dat_pairs})
+           "data": "synthetic (seeded SplitMix64 scenes; random-init network)"}
+    with GPUSolver(scene, lanesPerBody=args.lanes, sweepMode=args.sweep) as solver:
+        wall = timed_steps(solver, args.steps, args.warmup)
+        out["value"] = round(nv * iters * args.steps / wall, 1)
+        out["ms_per_step"] = round(wall * 1e3 / args.steps, 4)
+        st = solver.stats()
+        cfg = {"workload": desc, "dynamic_bodies": nv, "iterations": iters, "colors": st.num_colors,
+               "lanes_per_body": st.lanes_per_body,
+               "sweep_mode": "persistent" if st.sweep_mode == 2 else "per-color launches",
+               "end_residual": solver.residual()}
+        out["roofline"] = primal_roofline(solver, f"primal_soft_block_{args.cells + 1}_pmc.json")
+    out["config"] = cfg
+    log(f"headline {out['ms_per_step']} ms/step, primal frac {out['roofline']['frac']}")
+    cfg["error_vs_fp64"] = error_vs_fp64(scene, args.fp64_steps, threads)
 
-    # roofline of the dominant kernel (per-color primal sweep), HIP events on the solver stream
-    if args.workload != "batched":
-        out["roofline"] = primal_roofline(solver, pmc)
-    # the same kernel where a color fills the chip: the 1M-tet block (56^3 vertices)
-    if args.workload == "soft_block" and args.scale_cells > 0 and args.scale_cells != args.cells:
-        from avbd_amd import demos as _demos
-        big = _demos.soft_block_scene(cells=args.scale_cells)
-        t_create = time.perf_counter()
-        gb = GPUSolver(big, lanesPerBody=args.lanes, sweepMode=args.sweep)
-        create_ms = (time.perf_counter() - t_create) * 1e3
-        gb.step(3)
-        gb.synchronize()
-        roof = primal_roofline(gb, f"primal_soft_block_{args.scale_cells + 1}_pmc.json")
-        roof["workload"] = (f"soft block {args.scale_cells}^3 cells ({big.numDynamic} vertices, {len(big.tets)} tets), "
-                            f"the same kernel at the 1M-tet size")
-        roof["create_ms"] = round(create_ms, 1)   # scene packing + upload precompute + coloring + autotune
-        out["roofline_at_scale"] = roof
-        gb.close()
-    if args.workload == "soft_block" and not args.no_learned:
-        cmp, roof = learned_comparison(args)
-        out["config"]["learned_vs_plain"] = cmp
-        out["roofline_mlp"] = roof
-        out["config"]["batched_learned_vs_plain"] = batched_learned_comparison(args)
-    if args.workload == "batched" and not args.no_learned:
-        out["config"]["rigid_learned_vs_plain"] = rigid_learned_comparison(args, scene)
+    net = None
+    if not args.no_learned:
+        net, cfg["learned_training"] = train_network(args)
+        log(f"trained on {cfg['learned_training']['samples']} pairs")
+    res = plain_vs_learned(scene, net, args.res_frames, 1e-4, args.res_cap, fixed_res=False) if net else None
+    if res is None:
+        with GPUSolver(scene) as g:
+            res = {"plain": frames_to_tol(g, args.res_frames, 1e-4, args.res_cap)}
+    res.update({"tolerance": 1e-4, "sweep_cap": args.res_cap, "frames": args.res_frames, "check_every": 10,
+                "note": "frames from the drop through the landing; the FP32 / FP64 oracles need 110-370 "
+                        "sweeps per frame here, so the metric's 200-sweep cap is raised"})
+    cfg["residual_1e-4"] = res
+    log(f"residual 1e-4: {json.dumps(res)}")
+
+    if not args.quick:
+        if args.scale_cells > 0 and args.scale_cells != args.cells:
+            big = demos.soft_block_scene(cells=args.scale_cells)
+            t_create = time.perf_counter()
+            with GPUSolver(big, lanesPerBody=args.lanes, sweepMode=args.sweep) as gb:
+                create_ms = (time.perf_counter() - t_create) * 1e3
+                gb.step(3)
+                gb.synchronize()
+                roof = primal_roofline(gb, f"primal_soft_block_{args.scale_cells + 1}_pmc.json")
+            roof["workload"] = (f"soft block {args.scale_cells}^3 cells ({big.numDynamic} vertices, "
+                                f"{len(big.tets)} tets), the same kernel at the 1M-tet size")
+            roof["create_ms"] = round(create_ms, 1)
+            out["roofline_at_scale"] = roof
+        configs = {"stack_10_cubes": config_stack(args, threads)}
+        log("stack done")
+        configs["cloth_256"] = config_cloth(args, threads)
+        log("cloth done")
+        configs["batched_rigid_4096"] = config_batched(args, threads)
+        log("batched rigid done")
+        if net is not None:
+            configs["learned_at_scale"] = config_learned_at_scale(args, net)
+            log("learned at scale done")
+            configs["batched_soft_learned_vs_plain"] = batched_vs_plain(args, net)
+            log("batched done")
+            us = net.timeForward(nv, reps=50)
+            tflops = MLP_FLOPS_PER_ROW * nv / (us * 1e-6) / 1e12
+            out["roofline_mlp"] = {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5),
+                                   "traffic": None, "kernel": "k_mlp_forward", "avg_launch_us": round(us, 3),
+                                   "rows": nv, "flops_per_row": MLP_FLOPS_PER_ROW}
+        cfg["configs"] = configs
+    if net is not None:
+        net.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, unit)
+        out["cpu_baseline"] = cpu_baseline(scene, "vertex-iters/s", threads)
     print(json.dumps(out), flush=True)
 
 

@@ -375,6 +375,11 @@ avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity, uint32_t* 
  * frame, or -1. */
 avbd_status avbd_map_find(avbd_solver* s, const uint32_t* pairs, uint32_t n, int32_t* out);
 
+/* Element-contact persistence map lookup (softMapFind, 25_cloth.metal:87-99):
+ * for each (keyA, keyB) returns the index of the last frame's element
+ * contact (avbd_get_soft_contacts order) carrying that key, or -1. */
+avbd_status avbd_soft_map_find(avbd_solver* s, const uint32_t* keys, uint32_t n, int32_t* out);
+
 /* maxConstraintErrorChecked() -- GPUSolver.swift:4829 (diag_error kernel 40_solver.metal:2189) */
 avbd_status avbd_max_constraint_error(avbd_solver* s, float* out);
 

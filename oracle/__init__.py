@@ -54,6 +54,7 @@ def load():
         "oracle_residual": (C.c_int, [vp, dp]),
         "oracle_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p]),
         "oracle_map_find": (C.c_int, [vp, u32p, C.c_uint32, i32p]),
+        "oracle_get_manifold_keys": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
         "oracle_splitmix64": (None, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), fp]),
         "oracle_snh_tet": (None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_solve6x6": (None, [dp, dp, dp, C.c_int]),
@@ -175,6 +176,14 @@ class OracleSolver:
         self._lib.oracle_get_manifolds(self._h, n.value, C.byref(n), _p(h, C.c_uint32), _p(nrm, C.c_float),
                                        _p(lp, C.c_float))
         return h[:n.value], nrm[:n.value], lp[:n.value]
+
+    def manifoldKeys(self):
+        """Feature keys of the last frame's contacts: (n, 8), 0xFFFFFFFF past each count."""
+        n = C.c_uint32()
+        self._lib.oracle_get_manifold_keys(self._h, 0, C.byref(n), None)
+        out = np.zeros((max(1, n.value), 8), np.uint32)
+        self._lib.oracle_get_manifold_keys(self._h, n.value, C.byref(n), _p(out, C.c_uint32))
+        return out[:n.value]
 
     def mapFind(self, pairs):
         q = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)

@@ -2979,6 +2979,17 @@ int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* 
   if (o->f) go(o->f); else go(o->d);
   return 0;
 }
+// feature keys of the last frame's contacts, 8 per manifold (0xFFFFFFFF past the count)
+int oracle_get_manifold_keys(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* keys8) {
+  auto go = [&](auto* s) {
+    const auto& ms = s->prevManifolds;
+    *n = (uint32_t)ms.size();
+    for (uint32_t i = 0; i < *n && i < cap && keys8; i++)
+      for (uint32_t k = 0; k < 8; k++) keys8[8 * i + k] = k < ms[i].n ? ms[i].c[k].feature : 0xFFFFFFFFu;
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
 int oracle_get_motor_targets(oracle_solver* o, uint32_t n, float* targets) {
   auto go = [&](auto* s) {
     for (uint32_t i = 0; i < n && i < s->joints.size(); i++) targets[i] = (float)s->joints[i].motorTarget;

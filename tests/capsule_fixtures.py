@@ -97,6 +97,20 @@ def pile(n=40, seed=21):
     return s
 
 
+def box_pile(n=36, seed=33):
+    """Boxes of mixed aspect ratios and orientations dropped in a loose
+    column: face-face, face-edge and edge-edge box contacts."""
+    s = PhysicsScene("box pile", SimSettings(iterations=10))
+    s.addBody((20, 20, 2), 0.0, 0.6, (0, 0, -1))
+    rng = SplitMix64(seed)
+    for i in range(n):
+        x, y = rng.uniform(-0.35, 0.35), rng.uniform(-0.35, 0.35)
+        q = quat(rng.uniform(0, math.pi), (rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)))
+        size = (rng.uniform(0.1, 0.5), rng.uniform(0.1, 0.3), rng.uniform(0.05, 0.3))
+        s.addBody(size, 1.0, 0.5, (x, y, 0.3 + 0.3 * i), rotation=q)
+    return s
+
+
 def capsule_mass(length, radius, density):
     """GPUSolver.swift:940-945 in float64."""
     m = math.pi * radius * radius * (length + 4 * radius / 3) * density
