@@ -23,8 +23,9 @@ ERR_ALLOC = 3
 ERR_UNSUPPORTED = 4
 ERR_RUNTIME_FAILURE = 5
 ERR_HIP = 6
+ERR_TOO_LARGE = 7
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "NO_DEVICE", 3: "ALLOC", 4: "UNSUPPORTED",
-                5: "RUNTIME_FAILURE", 6: "HIP"}
+                5: "RUNTIME_FAILURE", 6: "HIP", 7: "TOO_LARGE"}
 
 # avbd_failure_kind
 FAILURE_NAMES = {0: "none", 1: "commandExecution", 2: "rigidPairCapacity", 3: "softContactCapacity",
@@ -123,6 +124,7 @@ SIGNATURES = {
     "avbd_get_body_states": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP, FP, FP]),
     "avbd_get_body_mass": (C.c_int, [C.c_void_p, C.c_uint32, FP, FP]),
     "avbd_set_body_states": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP, FP, FP]),
+    "avbd_set_body_states_noalloc": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP, FP, FP, FP]),
     "avbd_set_motor_targets": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
     "avbd_set_motor_torques": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),
     "avbd_set_motor_velocities": (C.c_int, [C.c_void_p, U32P, C.c_uint32, FP]),

@@ -12,6 +12,7 @@
 #include <array>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -68,6 +69,8 @@ struct avbd_solver {
   std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
   std::vector<float> rateValues;
   F4* jointPenInit = nullptr;      // upload-time joint penalties (lin [nj], ang [nj]), restored on reset
+  int device = 0;                  // the HIP device current at create
+  uint32_t* fixedScratch = nullptr;  // this device's address of the no-allocation staging array
   uint32_t* rateJointDev = nullptr;
   float* rateValueDev = nullptr;
   uint32_t numGroups = 0;        // connected collision surfaces
@@ -909,6 +912,35 @@ void release_solver(avbd_solver* s);
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Staging memory of avbd_set_body_states_noalloc: one fixed array per GPU,
+// resolved to its device address once per device (at the first create on
+// that device) and shared by every solver on it until the process ends.
+// ---------------------------------------------------------------------------
+constexpr size_t kFixedScratchWords = size_t(1) << 20;  // 4 MiB
+static __device__ __align__(256) uint32_t scratch[kFixedScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+uint32_t* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device` (the current device).  The name
+// itself is a host-side stand-in, never passed to a kernel.
+static hipError_t resolve_fixed_scratch(int device, uint32_t** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[device]) {
+    uint32_t* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    g_scratchPtr[device] = ptr;
+  }
+  *out = g_scratchPtr[device];
+  return hipSuccess;
+}
+
 extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options* opts, avbd_solver** out) {
   if (!out) return AVBD_ERR_INVALID_ARGUMENT;
   *out = nullptr;
@@ -917,6 +949,12 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AVBD_ERR_NO_DEVICE;
   avbd_solver* s = new avbd_solver();
+  hipGetDevice(&s->device);
+  if (resolve_fixed_scratch(s->device, &s->fixedScratch) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return AVBD_ERR_NO_DEVICE;
+  }
   if (opts) s->opts = *opts;
   else avbd_options_default(&s->opts);
   if (s->opts.max_pairs_per_body < 0) s->opts.max_pairs_per_body = 16;  // 0 is a valid request (4096-pair floor)
@@ -2166,6 +2204,70 @@ extern "C" avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx,
   }
   hipError_t e = hipStreamSynchronize(s->stream);
   hipFree(block);
+  if (e != hipSuccess) return latch(s, AVBD_FAIL_COMMAND_EXECUTION, s->frameIndex, 0, 0, (int)e, "set_body_states");
+  return AVBD_OK;
+}
+
+namespace {
+// body ids of a reset: the caller's list, or 0..n-1 when it passed none
+__global__ void k_iota(uint32_t* out, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = k;
+}
+__global__ void k_flag_bodies(const uint32_t* idx, uint32_t n, uint32_t* flag) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flag[idx[k]] = 1u;
+}
+}  // namespace
+
+// avbd_set_body_states with its staging (ids, per-body flags, the 13 floats
+// per body) in the device's fixed scratch array instead of a fresh
+// allocation; the same kernels do the write and the warm-start clearing.
+extern "C" avbd_status avbd_set_body_states_noalloc(avbd_solver* s, const uint32_t* idx, uint32_t n,
+                                                    const float* pos3, const float* rot4, const float* vel3,
+                                                    const float* angvel3) {
+  if (!s || !pos3 || !rot4 || !vel3 || !angvel3) return AVBD_ERR_INVALID_ARGUMENT;
+  if (n == 0) return AVBD_OK;
+  if ((size_t)n * 14 + s->nb > kFixedScratchWords) return AVBD_ERR_TOO_LARGE;
+  if (!s->fixedScratch) return AVBD_ERR_NO_DEVICE;
+  if (idx)
+    for (uint32_t k = 0; k < n; k++)
+      if (idx[k] >= s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!idx && n > s->nb) return AVBD_ERR_INVALID_ARGUMENT;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  hipStream_t q = s->stream;
+  uint32_t* dIdx = s->fixedScratch;                 // [n]
+  uint32_t* dFlag = dIdx + n;                       // [nb], cleared below
+  float* dBuf = (float*)(dFlag + s->nb);            // pos [3n], rot [4n], vel [3n], angvel [3n]
+  const size_t n3 = 12 * (size_t)n, n4 = 16 * (size_t)n;
+  hipError_t e = hipSuccess;
+  if (idx) e = hipMemcpyAsync(dIdx, idx, 4 * (size_t)n, hipMemcpyHostToDevice, q);
+  else hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, q, dIdx, n);
+  if (e == hipSuccess) e = hipMemcpyAsync(dBuf, pos3, n3, hipMemcpyHostToDevice, q);
+  if (e == hipSuccess) e = hipMemcpyAsync(dBuf + 3 * (size_t)n, rot4, n4, hipMemcpyHostToDevice, q);
+  if (e == hipSuccess) e = hipMemcpyAsync(dBuf + 7 * (size_t)n, vel3, n3, hipMemcpyHostToDevice, q);
+  if (e == hipSuccess) e = hipMemcpyAsync(dBuf + 10 * (size_t)n, angvel3, n3, hipMemcpyHostToDevice, q);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(q);
+    return AVBD_ERR_HIP;
+  }
+  // the flags keep the previous call's contents: clear them with a fill kernel
+  launch_fill_u32(q, dFlag, s->nb, 0u);
+  hipLaunchKernelGGL(k_flag_bodies, dim3((n + 255) / 256), dim3(256), 0, q, dIdx, n, dFlag);
+  F4* dPen = s->jointPenInit;
+  StepParams P = make_params(s);
+  hipLaunchKernelGGL(k_write_states, dim3((n + 255) / 256), dim3(256), 0, q, s->S, dIdx, n, dBuf,
+                     dBuf + 3 * n, dBuf + 7 * n, dBuf + 10 * n);
+  uint32_t total = std::max(std::max(s->nj, s->ns), s->maxPairs);
+  hipLaunchKernelGGL(k_reset_incident, dim3((total + 255) / 256), dim3(256), 0, q, s->S, P, dFlag, dPen,
+                     dPen + s->nj, s->S.prevManifolds, s->maxPairs);
+  if (s->wsBuilt) {
+    launch_fill_u32(q, (uint32_t*)s->W.corr, 4 * s->nb, 0u);
+    launch_fill_u32(q, (uint32_t*)s->W.corrPrev, 4 * s->nb, 0u);
+    launch_fill_u32(q, (uint32_t*)s->W.corrAng, 4 * s->nb, 0u);
+  }
+  e = hipStreamSynchronize(q);
   if (e != hipSuccess) return latch(s, AVBD_FAIL_COMMAND_EXECUTION, s->frameIndex, 0, 0, (int)e, "set_body_states");
   return AVBD_OK;
 }

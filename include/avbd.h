@@ -41,7 +41,8 @@ typedef enum avbd_status {
     AVBD_ERR_ALLOC = 3,             /* AVBDError.allocFailed       GPUSolver.swift:642 */
     AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus colliders; hull pairs other than hull-box) */
     AVBD_ERR_RUNTIME_FAILURE = 5,   /* a RuntimeFailure is latched; see avbd_last_failure */
-    AVBD_ERR_HIP = 6                /* HIP API error outside a latched frame */
+    AVBD_ERR_HIP = 6,               /* HIP API error outside a latched frame */
+    AVBD_ERR_TOO_LARGE = 7          /* request exceeds a fixed capacity (avbd_set_body_states_noalloc) */
 } avbd_status;
 
 /* RuntimeFailure kinds, GPUSolver.swift:663-758 */
@@ -242,6 +243,18 @@ avbd_status avbd_get_body_mass(avbd_solver* s, uint32_t n, float* mass, float* i
 avbd_status avbd_set_body_states(avbd_solver* s, const uint32_t* idx, uint32_t n,
                                  const float* pos3, const float* rot4,
                                  const float* vel3, const float* angvel3);
+
+/* No counterpart in the reference.  avbd_set_body_states for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight RL
+ * reset loops, harnesses that forbid hipMalloc): identical effect, but the
+ * staging lives in a fixed 4 MiB array per GPU (resolved at the first
+ * avbd_create on that device) and nothing is allocated or freed.  Capacity:
+ * 14 * n + (bodies in the solver) <= 1048576 words, else AVBD_ERR_TOO_LARGE.
+ * The array is shared by every solver on the GPU and keeps the last call's
+ * contents: calls that overlap on one GPU (other threads) race. */
+avbd_status avbd_set_body_states_noalloc(avbd_solver* s, const uint32_t* idx, uint32_t n,
+                                         const float* pos3, const float* rot4, const float* vel3,
+                                         const float* angvel3);
 /* setBodyPoses(_:) -- GPUSolver.swift:2399-2404: setBodyStates with zero velocities. */
 avbd_status avbd_set_body_poses(avbd_solver* s, const uint32_t* idx, uint32_t n, const float* pos3,
                                 const float* rot4);
