@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libavbd_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_COLORS = 64
 
 # avbd_status
@@ -107,7 +107,8 @@ class ClothInfo(C.Structure):
                 ("max_soft", C.c_uint32), ("soft_map_capacity", C.c_uint32), ("max_dat_pairs", C.c_uint32),
                 ("elem_cell_size", C.c_float), ("elem_margin", C.c_float), ("dat_query_radius", C.c_float),
                 ("surface_contact_cell_size", C.c_float), ("num_contact_edges", C.c_uint32),
-                ("pair_stream", C.c_uint32), ("dat_on", C.c_uint32)]
+                ("pair_stream", C.c_uint32), ("dat_on", C.c_uint32),
+                ("iso_on", C.c_uint32)]
 
 
 # name -> (restype, argtypes); the complete exported surface of avbd.h
@@ -143,6 +144,7 @@ SIGNATURES = {
     "avbd_get_collision_tris": (C.c_int, [C.c_void_p, C.c_uint32, U32P, U32P]),
     "avbd_get_motor_targets": (C.c_int, [C.c_void_p, C.c_uint32, FP]),
     "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
+    "avbd_get_iso_state": (C.c_int, [C.c_void_p, U32P, U32P, FP, FP, U32P]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_soft_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),

@@ -324,6 +324,25 @@ class GPUSolver:
                     "soft contacts")
         return out[:n.value]
 
+    def isoState(self):
+        """Isotropic limiter state after the last frame: tracked triangles
+        (nb, 4), tracked edges (num_contact_edges, 4), squared clearance
+        bounds (nb,), anchor poses (nb, 3), the frame's truncations and
+        refreshes; None when the mode is off."""
+        info = self.clothInfo()
+        if not info["iso_on"]:
+            return None
+        nb, ne = self.bodyCount, info["num_contact_edges"]
+        vt = np.zeros((max(1, nb), 4), np.uint32)
+        ee = np.zeros((max(1, ne), 4), np.uint32)
+        bound = np.zeros(max(1, nb), np.float32)
+        anchor = np.zeros((max(1, nb), 3), np.float32)
+        counts = np.zeros(2, np.uint32)
+        self._check(self._lib.avbd_get_iso_state(self._h, _u32p(vt), _u32p(ee), _fp(bound), _fp(anchor), _u32p(counts)),
+                    "iso state")
+        return {"vt_track": vt[:nb], "ee_track": ee[:ne], "bound": bound[:nb], "anchor": anchor[:nb],
+                "truncations": int(counts[0]), "refreshes": int(counts[1])}
+
     def mapFind(self, pairs):
         q = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
         out = np.zeros(len(q), np.int32)

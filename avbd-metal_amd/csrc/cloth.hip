@@ -169,13 +169,6 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
     S.elemBox[2 * gid] = f4(lo, as_float(ge));
     S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
   }
-  // an element of a surface with no other surface in reach is never a
-  // candidate of an accepted pair: its counters above, no bucket entries
-  bool unreached = false;
-  if (P.groupSkip) {
-    const uint32_t gb = as_uint(S.elemBox[2 * gid].w);
-    unreached = gb != 0u && S.groupNear[gb] == 0u;
-  }
   if (!finite3(lo) || !finite3(hi)) {
     if (!SCATTER) {
       atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
@@ -185,7 +178,6 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
   }
   CellRange r = range_of(lo, hi, P.elemCellSize);
   if (!SCATTER && r.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
-  if (unreached) return;
   const Cell3 c0 = r.c0;
   const int nx = r.sx + 1, ny = r.sy + 1, nz = r.sz + 1;
   const int n = nx * ny * nz;  // <= 64
@@ -205,100 +197,6 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
     if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
     else atomicAdd(&S.elemCellCount[slot], 1u);
   }
-}
-
-// ---------------------------------------------------------------------------
-// per-surface reach: the particle box of every surface group (atomic min /
-// max on order-preserving keys of the coordinates), then whether another
-// group's box lies within the query radius (grown by a rounding margin)
-// ---------------------------------------------------------------------------
-namespace {
-__device__ __forceinline__ uint32_t order_key(float f) {
-  uint32_t u = as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float key_value(uint32_t k) { return as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k); }
-}  // namespace
-
-__global__ void __launch_bounds__(256) k_group_clear(DeviceState S, StepParams P) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g > P.numGroups) return;
-  for (int k = 0; k < 3; k++) {
-    S.groupBox[6 * g + k] = 0xFFFFFFFFu;
-    S.groupBox[6 * g + 3 + k] = 0u;
-  }
-  S.groupNear[g] = 0u;
-}
-
-// surface particles are ordered by body id, so a wave holds one group or
-// two: the groups of its first and last lanes are reduced across the wave
-// and committed by one lane each; lanes of any other group commit alone
-__global__ void __launch_bounds__(256) k_group_box(DeviceState S, StepParams P) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < P.numParticles;
-  uint32_t g = 0;
-  bool fin = false;
-  uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-  if (in) {
-    const uint32_t v = S.particleIdx[i];
-    g = S.clothGroup[v];
-    const V3 p = P3(S, v);
-    fin = finite3(p);
-    if (!fin) S.groupNear[g] = 1u;  // the queries run for this surface and report it
-    else {
-      k[0] = k[3] = order_key(p.x);
-      k[1] = k[4] = order_key(p.y);
-      k[2] = k[5] = order_key(p.z);
-    }
-  }
-  const bool live = in && fin;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t ends[2] = {(uint32_t)__shfl((int)g, 0), (uint32_t)__shfl((int)g, 63)};
-  for (int w = 0; w < 2; w++) {
-    const uint32_t ge = ends[w];
-    if (w == 1 && ge == ends[0]) break;
-    const bool mine = live && g == ge;
-    uint32_t r[6];
-    for (int j = 0; j < 3; j++) {
-      r[j] = mine ? k[j] : 0xFFFFFFFFu;
-      r[3 + j] = mine ? k[3 + j] : 0u;
-    }
-    for (int o = 32; o > 0; o >>= 1)
-      for (int j = 0; j < 3; j++) {
-        r[j] = min(r[j], (uint32_t)__shfl_xor((int)r[j], o));
-        r[3 + j] = max(r[3 + j], (uint32_t)__shfl_xor((int)r[3 + j], o));
-      }
-    const bool any = __any(mine);
-    if (lane == 0 && any)
-      for (int j = 0; j < 3; j++) {
-        atomicMin(&S.groupBox[6 * ge + j], r[j]);
-        atomicMax(&S.groupBox[6 * ge + 3 + j], r[3 + j]);
-      }
-  }
-  if (!live || g == ends[0] || g == ends[1]) return;
-  for (int j = 0; j < 3; j++) {
-    atomicMin(&S.groupBox[6 * g + j], k[j]);
-    atomicMax(&S.groupBox[6 * g + 3 + j], k[3 + j]);
-  }
-}
-
-// one thread per ordered pair of groups (at most kMaxSkipGroups^2)
-__global__ void __launch_bounds__(256) k_group_near(DeviceState S, StepParams P) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.numGroups * P.numGroups) return;
-  const uint32_t g = i / P.numGroups + 1u, h = i % P.numGroups + 1u;
-  if (g == h) return;
-  // margin: the radius, plus rounding of the distances and closest points
-  // the queries compute (relative 1e-6 of the coordinates, ~16 ulp)
-  const float r = P.datQueryRadius * 1.001f + 1.0e-5f;
-  bool apart = false;
-  for (int k = 0; k < 3; k++) {
-    const float gl = key_value(S.groupBox[6 * g + k]), gh = key_value(S.groupBox[6 * g + 3 + k]);
-    const float hl = key_value(S.groupBox[6 * h + k]), hh = key_value(S.groupBox[6 * h + 3 + k]);
-    const float m = r + 1.0e-6f * (fabsf(gl) + fabsf(gh) + fabsf(hl) + fabsf(hh));
-    apart = apart || hl > gh + m || hh < gl - m;
-  }
-  if (!apart) S.groupNear[g] = 1u;
 }
 
 __global__ void __launch_bounds__(256) k_copy_u32(uint32_t* dst, const uint32_t* src, uint32_t n) {
@@ -399,7 +297,6 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
       }
       return;
     }
-    if (P.groupSkip && S.groupNear[S.clothGroup[v]] == 0u) return;
     Cell3 cell = cell_of(p, P.elemCellSize);
     uint32_t h = cell_hash(cell.x, cell.y, cell.z, 0u, P.elemHashSize);
     uint32_t s = S.elemCellStart[2u * h], e = S.elemCellStart[2u * h + 1u];  // the bucket's triangles
@@ -425,7 +322,6 @@ __device__ void dat_visit(const DeviceState& S, const StepParams& P, uint32_t pr
   }
   CellRange ar = range_of(vmin(a0, a1), vmax(a0, a1), P.elemCellSize);
   if (diag && lane == 0 && ar.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
-  if (P.groupSkip && S.groupNear[S.clothGroup[a.x]] == 0u) return;
   uint32_t ns0 = S.nbrStart[a.x], ne0 = ns0 + S.nbrCount[a.x];
   uint32_t ns1 = S.nbrStart[a.y], ne1 = ns1 + S.nbrCount[a.y];
   const uint32_t ga = S.clothGroup[a.x];
@@ -1189,7 +1085,8 @@ __global__ void __launch_bounds__(256) k_dat_contacts(DeviceState S, StepParams 
 __global__ void k_soft_finalize(DeviceState S, StepParams P) {
   uint32_t total = P.pairStream ? S.datStart[P.numParticles + P.numEdges] : 0u;
   uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
-  uint32_t demand = S.counters[CTR_SOFT_RT] + (P.pairStream ? S.softStart[n] : 0u);
+  uint32_t demand = S.counters[CTR_SOFT_RT] + (P.pairStream ? S.softStart[n] : 0u) +
+                    (P.isoOn ? S.counters[CTR_SOFT_ISO] : 0u);
   S.counters[CTR_SOFT_CANDIDATES] = demand;
   S.counters[CTR_SOFT] = demand < P.maxSoft ? demand : P.maxSoft;
 }
@@ -1220,6 +1117,499 @@ __global__ void __launch_bounds__(256) k_softmap_insert(DeviceState S, StepParam
       h = (h + 1u) & (P.softMapCapacity - 1u);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// isotropic OGC limiter (25_cloth.metal:350-940): every surface particle
+// tracks its four closest collision triangles and every authored edge its
+// four closest edges across frames.  Per frame a small candidate pool is
+// evaluated -- the tracked set, the triangles adjacent to the best, the mesh
+// neighbours' best -- and the element grid is scanned on staggered reseeds
+// or fast relative motion.  Each observed distance between elements outside
+// each other's 2-ring lowers the squared clearance bound of the vertices
+// involved (atomic min on the f32 bits: non-negative floats order as
+// integers); the contacts come from the tracked sets.  Every tracker reads
+// the previous frame's tracked sets (vtPrev / eePrev, not the ones being
+// rewritten) and ties in distance keep the lower element id, so the result
+// is a function of the inputs only.  Producers are the surface particles,
+// then the authored edges; the records follow the rigid-feature ones in
+// producer order.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kOgcFarBits = 0x7F7F7F7Fu;  // cleared bound, 3.3961514e38
+
+struct Best4 {
+  uint32_t id0, id1, id2, id3;
+  float d0, d1, d2, d3;
+};
+__device__ __forceinline__ void b4_init(Best4& b) {
+  b.id0 = b.id1 = b.id2 = b.id3 = kNone;
+  b.d0 = b.d1 = b.d2 = b.d3 = kFltMax;
+}
+__device__ __forceinline__ bool b4_has(const Best4& b, uint32_t t) {
+  return b.id0 == t || b.id1 == t || b.id2 == t || b.id3 == t;
+}
+// (d, id) strictly before (e, jd); a NaN distance is before nothing
+__device__ __forceinline__ bool b4_less(float d, uint32_t id, float e, uint32_t jd) {
+  return d < e || (d == e && id < jd);
+}
+// keeps the four smallest (d2, id): the candidate replaces the last entry
+// and sinks by compare-and-swap (fixed slots, so the set stays in registers)
+__device__ __forceinline__ void b4_insert(Best4& b, uint32_t t, float d2) {
+  if (!b4_less(d2, t, b.d3, b.id3)) return;
+  b.id3 = t;
+  b.d3 = d2;
+  if (!b4_less(b.d3, b.id3, b.d2, b.id2)) return;
+  { uint32_t i = b.id2; float d = b.d2; b.id2 = b.id3; b.d2 = b.d3; b.id3 = i; b.d3 = d; }
+  if (!b4_less(b.d2, b.id2, b.d1, b.id1)) return;
+  { uint32_t i = b.id1; float d = b.d1; b.id1 = b.id2; b.d1 = b.d2; b.id2 = i; b.d2 = d; }
+  if (!b4_less(b.d1, b.id1, b.d0, b.id0)) return;
+  { uint32_t i = b.id0; float d = b.d0; b.id0 = b.id1; b.d0 = b.d1; b.id1 = i; b.d1 = d; }
+}
+__device__ __forceinline__ uint32_t u4_at(U4 u, int k) { return k == 0 ? u.x : (k == 1 ? u.y : (k == 2 ? u.z : u.w)); }
+
+__device__ __forceinline__ float contact_pad(const DeviceState& S, const StepParams& P, uint32_t a, uint32_t b) {
+  return fmaxf(fabsf(S.shape[a].w), fabsf(S.shape[b].w)) + P.elemMargin;
+}
+
+// The 27-cell reseed and the bounds cap use the contact cell of the mode.
+__device__ __forceinline__ bool iso_reseed(const StepParams& P, uint32_t id, uint32_t frame, V3 relV, bool missing,
+                                           bool solid) {
+  const float vth = 0.25f * P.surfaceContactCellSize / fmaxf(P.dt, 1e-6f);
+  const uint32_t period = dot(relV, relV) > vth * vth ? 1u : 7u;
+  return ((id + frame) & period) == 0u || (missing && !solid) || (missing && solid && ((id + frame) & 3u) == 0u);
+}
+
+struct VtQuery {
+  uint32_t v, gv, ns, ne, n2s, n2e;
+  bool solid;
+  V3 p;
+  float cap2;
+};
+
+// one candidate triangle of a V-T tracker (25_cloth.metal:423-463)
+__device__ __forceinline__ void vt_consider(const DeviceState& S, const StepParams& P, const VtQuery& q, Best4& best,
+                                            float& own, uint32_t t) {
+  if (t >= P.numTris || b4_has(best, t)) return;
+  const U4 tv = S.tris[t];
+  if (tv.x == q.v || tv.y == q.v || tv.z == q.v) return;
+  if (q.solid && S.clothGroup[tv.x] == q.gv && S.clothGroup[tv.y] == q.gv && S.clothGroup[tv.z] == q.gv &&
+      S.clothVert[tv.x] == 0u && S.clothVert[tv.y] == 0u && S.clothVert[tv.z] == 0u)
+    return;
+  V3 bary;
+  const V3 c = closest_pt_triangle(q.p, P3(S, tv.x), P3(S, tv.y), P3(S, tv.z), bary);
+  const V3 d = q.p - c;
+  const float d2 = dot(d, d);
+  if (!nbr_contains(S.nbrList, q.ns, q.ne, tv.x) && !nbr_contains(S.nbrList, q.ns, q.ne, tv.y) &&
+      !nbr_contains(S.nbrList, q.ns, q.ne, tv.z))
+    b4_insert(best, t, d2);
+  if (d2 < q.cap2 && !nbr_contains(S.nbr2List, q.n2s, q.n2e, tv.x) &&
+      !nbr_contains(S.nbr2List, q.n2s, q.n2e, tv.y) && !nbr_contains(S.nbr2List, q.n2s, q.n2e, tv.z)) {
+    own = fminf(own, d2);
+    const uint32_t bits = as_uint(d2);
+    atomicMin(&S.ogcBound[tv.x], bits);
+    atomicMin(&S.ogcBound[tv.y], bits);
+    atomicMin(&S.ogcBound[tv.z], bits);
+  }
+}
+
+struct EeQuery {
+  uint32_t e, ga, ns0, ne0, ns1, ne1, n2s, n2e;
+  U2 a;
+  bool solid;
+  V3 a0, a1;
+  float cap2;
+};
+
+// one candidate edge of an E-E tracker (25_cloth.metal:697-737)
+__device__ __forceinline__ void ee_consider(const DeviceState& S, const StepParams& P, const EeQuery& q, Best4& best,
+                                            float& own, uint32_t o) {
+  if (o >= P.numContactEdges || o == q.e || b4_has(best, o)) return;
+  const U2 b = S.edges[o];
+  if (b.x == q.a.x || b.x == q.a.y || b.y == q.a.x || b.y == q.a.y) return;
+  if (q.solid && S.clothGroup[b.x] == q.ga && S.clothGroup[b.y] == q.ga && S.clothVert[b.x] == 0u &&
+      S.clothVert[b.y] == 0u)
+    return;
+  float s, t;
+  const V3 b0 = P3(S, b.x), b1 = P3(S, b.y);
+  closest_seg_seg(q.a0, q.a1, b0, b1, s, t);
+  const V3 d = (q.a0 + (q.a1 - q.a0) * s) - (b0 + (b1 - b0) * t);
+  const float d2 = dot(d, d);
+  if (!nbr_contains(S.nbrList, q.ns0, q.ne0, b.x) && !nbr_contains(S.nbrList, q.ns0, q.ne0, b.y) &&
+      !nbr_contains(S.nbrList, q.ns1, q.ne1, b.x) && !nbr_contains(S.nbrList, q.ns1, q.ne1, b.y))
+    b4_insert(best, o, d2);
+  if (d2 < q.cap2 && !nbr_contains(S.nbr2List, q.n2s, q.n2e, b.x) && !nbr_contains(S.nbr2List, q.n2s, q.n2e, b.y)) {
+    own = fminf(own, d2);
+    const uint32_t bits = as_uint(d2);
+    atomicMin(&S.ogcBound[b.x], bits);
+    atomicMin(&S.ogcBound[b.y], bits);
+  }
+}
+
+// V-T contacts of particle v's tracked triangles (25_cloth.metal:532-629);
+// returns how many it emits, written from soft[base] when EMIT
+template <bool EMIT>
+__device__ uint32_t vt_emit(const DeviceState& S, const StepParams& P, uint32_t v, U4 tracked, uint32_t base) {
+  const V3 pv = P3(S, v);
+  const float rv = fabsf(S.shape[v].w), mv = S.posLin[v].w;
+  const V3 velV = xyz(S.velLin[v]);
+  uint32_t n = 0;
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++) {
+    const uint32_t t = u4_at(tracked, bi);
+    if (t >= P.numTris) break;
+    const U4 tv = S.tris[t];
+    const V3 a = P3(S, tv.x), b = P3(S, tv.y), c = P3(S, tv.z);
+    V3 bary;
+    const V3 q = closest_pt_triangle(pv, a, b, c, bary);
+    const V3 d = pv - q;
+    const float dist2 = dot(d, d);
+    const float rt = fmaxf(fabsf(S.shape[tv.x].w), fmaxf(fabsf(S.shape[tv.y].w), fabsf(S.shape[tv.z].w)));
+    const float hSum = rv + rt;
+    const V3 velT = (xyz(S.velLin[tv.x]) + xyz(S.velLin[tv.y]) + xyz(S.velLin[tv.z])) / 3.0f;
+    const float inflate = fminf(length(velV - velT) * P.dt, 0.3f * P.surfaceContactCellSize);
+    const float detect = hSum + P.elemMargin + inflate;
+    if (dist2 > detect * detect) continue;
+    V3 triN = cross(b - a, c - a);
+    const float tnLen = length(triN);
+    if (tnLen < 1e-12f) continue;
+    float minMass = mv > 0.0f ? mv : kFltMax;
+    const float mA = S.posLin[tv.x].w, mB = S.posLin[tv.y].w, mC = S.posLin[tv.z].w;
+    if (mA > 0.0f) minMass = fminf(minMass, mA);
+    if (mB > 0.0f) minMass = fminf(minMass, mB);
+    if (mC > 0.0f) minMass = fminf(minMass, mC);
+    if (minMass == kFltMax) continue;
+    if (EMIT) {
+      const uint32_t slot = base + n;
+      if (slot < P.maxSoft) {
+        const float dist = sqrtf(dist2);
+        triN = triN / tnLen;
+        const float side = dot(d, triN) >= 0.0f ? 1.0f : -1.0f;
+        const uint32_t keyA = (kSoftVT << 28) | v, keyB = t;
+        const bool deep = bary.x > 0.02f && bary.y > 0.02f && bary.z > 0.02f;
+        const float mem = deep ? soft_prev_side(S, P, keyA, keyB, side) : side;
+        V3 nrm;
+        float g;
+        if (dist > 1e-7f && side == mem) { nrm = d / dist; g = dist; }
+        else { nrm = triN * mem; g = -dist; }
+        const float fricT = S.props[tv.x].w * bary.x + S.props[tv.y].w * bary.y + S.props[tv.z].w * bary.z;
+        const float friction = sqrtf(fmaxf(S.props[v].w * fricT, 0.0f));
+        S.soft[slot] = soft_record(S, P, U4{v, tv.x, tv.y, tv.z}, F4{1.0f, -bary.x, -bary.y, -bary.z}, nrm, friction,
+                                   v3s(0), g - hSum + P.elemMargin, hSum - P.elemMargin, soft_lambda_cap(P, minMass),
+                                   minMass, kSoftVT, false, false, mem < 0.0f, keyA, keyB);
+      }
+    }
+    n++;
+  }
+  return n;
+}
+
+// E-E contacts of edge e's tracked edges, each pair once by its lower edge
+// (25_cloth.metal:789-853)
+template <bool EMIT>
+__device__ uint32_t ee_emit(const DeviceState& S, const StepParams& P, uint32_t e, U4 tracked, uint32_t base) {
+  const U2 ea = S.edges[e];
+  const V3 a0 = P3(S, ea.x), a1 = P3(S, ea.y);
+  const float rA = fmaxf(fabsf(S.shape[ea.x].w), fabsf(S.shape[ea.y].w));
+  const V3 velA = (xyz(S.velLin[ea.x]) + xyz(S.velLin[ea.y])) * 0.5f;
+  uint32_t n = 0;
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++) {
+    const uint32_t o = u4_at(tracked, bi);
+    if (o >= P.numContactEdges) break;
+    if (o <= e) continue;
+    const U2 eb = S.edges[o];
+    const V3 b0 = P3(S, eb.x), b1 = P3(S, eb.y);
+    float s, t;
+    closest_seg_seg(a0, a1, b0, b1, s, t);
+    if (s < kEeInteriorEps || s > 1.0f - kEeInteriorEps || t < kEeInteriorEps || t > 1.0f - kEeInteriorEps) continue;
+    const V3 d = (a0 + (a1 - a0) * s) - (b0 + (b1 - b0) * t);
+    const float dist = length(d);
+    const float rB = fmaxf(fabsf(S.shape[eb.x].w), fabsf(S.shape[eb.y].w));
+    const float hSum = rA + rB;
+    const V3 velB = (xyz(S.velLin[eb.x]) + xyz(S.velLin[eb.y])) * 0.5f;
+    const float inflate = fminf(length(velA - velB) * P.dt, 0.3f * P.surfaceContactCellSize);
+    if (dist - hSum > P.elemMargin + inflate) continue;
+    if (dist < 1e-7f) continue;
+    float minMass = kFltMax;
+    const float m0 = S.posLin[ea.x].w, m1 = S.posLin[ea.y].w, m2 = S.posLin[eb.x].w, m3 = S.posLin[eb.y].w;
+    if (m0 > 0.0f) minMass = fminf(minMass, m0);
+    if (m1 > 0.0f) minMass = fminf(minMass, m1);
+    if (m2 > 0.0f) minMass = fminf(minMass, m2);
+    if (m3 > 0.0f) minMass = fminf(minMass, m3);
+    if (minMass == kFltMax) continue;
+    if (EMIT) {
+      const uint32_t slot = base + n;
+      if (slot < P.maxSoft) {
+        const V3 nGeo = d / dist;
+        const uint32_t keyA = (kSoftEE << 28) | e, keyB = o;
+        V3 nrm = nGeo;
+        float g = dist;
+        const int prev = soft_map_find(S, P, keyA, keyB);
+        if (prev >= 0 && dot(nGeo, xyz(S.prevSoft[prev].normal)) < 0.0f) {
+          nrm = -nGeo;
+          g = -dist;
+        }
+        const float friction = sqrtf(fmaxf(
+            (S.props[ea.x].w + S.props[ea.y].w) * 0.5f * (S.props[eb.x].w + S.props[eb.y].w) * 0.5f, 0.0f));
+        S.soft[slot] = soft_record(S, P, U4{ea.x, ea.y, eb.x, eb.y}, F4{1.0f - s, s, -(1.0f - t), -t}, nrm, friction,
+                                   v3s(0), g - hSum + P.elemMargin, hSum - P.elemMargin, soft_lambda_cap(P, minMass),
+                                   minMass, kSoftEE, false, false, false, keyA, keyB);
+      }
+    }
+    n++;
+  }
+  return n;
+}
+
+}  // namespace
+
+// Per frame, before the trackers: the previous frame's tracked sets, the
+// frame number the staggered reseeds follow (the frame index before this
+// step), and cleared bounds.
+__global__ void __launch_bounds__(256) k_iso_begin(DeviceState S, StepParams P) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) S.isoFrame[0] += 1u;
+  if (i < P.numBodies) {
+    S.vtPrev[i] = S.vtTrack[i];
+    S.ogcBound[i] = kOgcFarBits;
+  }
+  if (i < P.numContactEdges) S.eePrev[i] = S.eeTrack[i];
+}
+
+// element grid at the contact reach: each triangle / authored edge in every
+// distinct bucket of its AABB padded by its largest vertex radius and the
+// margin, at most 3 cells per axis (25_cloth.metal:147-267, isotropic
+// branch); a non-finite element registers nowhere
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= P.numTris + P.numContactEdges) return;
+  V3 lo, hi;
+  float pad;
+  uint32_t entry;
+  if (gid < P.numTris) {
+    const U4 t = S.tris[gid];
+    const V3 a = P3(S, t.x), b = P3(S, t.y), c = P3(S, t.z);
+    pad = fmaxf(fabsf(S.shape[t.x].w), fmaxf(fabsf(S.shape[t.y].w), fabsf(S.shape[t.z].w))) + P.elemMargin;
+    lo = vmin(vmin(a, b), c);
+    hi = vmax(vmax(a, b), c);
+    entry = gid;
+  } else {
+    const U2 e = S.edges[gid - P.numTris];
+    const V3 a = P3(S, e.x), b = P3(S, e.y);
+    pad = contact_pad(S, P, e.x, e.y);
+    lo = vmin(a, b);
+    hi = vmax(a, b);
+    entry = kElemEdgeBit | (gid - P.numTris);
+  }
+  lo = lo - v3s(pad);
+  hi = hi + v3s(pad);
+  if (!finite3(lo) || !finite3(hi)) return;
+  const float cs = P.surfaceContactCellSize;
+  const Cell3 c0 = cell_of(lo, cs), raw = cell_of(hi, cs);
+  auto span = [](long long d) { return d < 0 ? 0 : (d > 2 ? 2 : (int)d); };
+  const int nx = span((long long)raw.x - c0.x) + 1, ny = span((long long)raw.y - c0.y) + 1,
+            nz = span((long long)raw.z - c0.z) + 1;
+  const int n = nx * ny * nz;  // <= 27
+  for (int i = 0; i < n; i++) {
+    const int x = cell_add(c0.x, i % nx), y = cell_add(c0.y, (i / nx) % ny), z = cell_add(c0.z, i / (nx * ny));
+    const uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
+    bool dup = false;
+    for (int j = 0; j < i && !dup; j++) {
+      const int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
+      dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
+    }
+    if (dup) continue;
+    const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
+    if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
+    else atomicAdd(&S.elemCellCount[slot], 1u);
+  }
+}
+
+// Trackers (pass 1: tracked sets, bounds, contact counts into softFlag) and
+// emission (pass 2: records from the tracked sets at the rigid-feature count
+// plus softStart).  One thread per producer.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nProd = P.numParticles + P.numContactEdges;
+  if (!EMIT && p == 0) S.softFlag[nProd] = 0u;
+  if (p >= nProd) return;
+  if (EMIT) {
+    const uint32_t base = S.counters[CTR_SOFT_RT] + S.softStart[p];
+    if (p < P.numParticles) {
+      const uint32_t v = S.particleIdx[p];
+      vt_emit<true>(S, P, v, S.vtTrack[v], base);
+    } else {
+      const uint32_t e = p - P.numParticles;
+      ee_emit<true>(S, P, e, S.eeTrack[e], base);
+    }
+    if (p == nProd - 1u) S.counters[CTR_SOFT_ISO] = S.softStart[nProd];
+    return;
+  }
+  const uint32_t frame = S.isoFrame[0] - 1u;
+  const float cs = P.surfaceContactCellSize;
+  Best4 best;
+  b4_init(best);
+  float own = as_float(kOgcFarBits);
+  if (p < P.numParticles) {
+    VtQuery q;
+    q.v = S.particleIdx[p];
+    q.p = P3(S, q.v);
+    q.gv = S.clothGroup[q.v];
+    q.solid = q.gv != 0u && S.clothVert[q.v] == 0u;
+    q.ns = S.nbrStart[q.v];
+    q.ne = q.ns + S.nbrCount[q.v];
+    q.n2s = S.nbr2Start[q.v];
+    q.n2e = q.n2s + S.nbr2Count[q.v];
+    q.cap2 = cs * cs;
+    const U4 tr = S.vtPrev[q.v];
+    vt_consider(S, P, q, best, own, tr.x);
+    vt_consider(S, P, q, best, own, tr.y);
+    vt_consider(S, P, q, best, own, tr.z);
+    vt_consider(S, P, q, best, own, tr.w);
+    if (tr.x < P.numTris) {
+      const U4 adj = S.triAdj[tr.x];
+      vt_consider(S, P, q, best, own, adj.x);
+      vt_consider(S, P, q, best, own, adj.y);
+      vt_consider(S, P, q, best, own, adj.z);
+    }
+    for (uint32_t k = q.ns; k < q.ne && k < q.ns + 8u; k++) vt_consider(S, P, q, best, own, S.vtPrev[S.nbrList[k]].x);
+    V3 relV = xyz(S.velLin[q.v]);
+    if (best.id0 < P.numTris) {
+      const U4 bt = S.tris[best.id0];
+      relV = relV - (xyz(S.velLin[bt.x]) + xyz(S.velLin[bt.y]) + xyz(S.velLin[bt.z])) / 3.0f;
+    }
+    if (iso_reseed(P, q.v, frame, relV, best.id0 == kNone, q.solid)) {
+      const Cell3 cc = cell_of(q.p, cs);
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            const uint32_t h = cell_hash(cell_add(cc.x, dx), cell_add(cc.y, dy), cell_add(cc.z, dz), 0u,
+                                         P.elemHashSize);
+            const uint32_t s0 = S.elemCellStart[2u * h], e0 = S.elemCellStart[2u * h + 1u];
+            for (uint32_t k = s0; k < e0; k++) vt_consider(S, P, q, best, own, S.elemCells[k]);
+          }
+    }
+    const U4 kept = U4{best.id0, best.id1, best.id2, best.id3};
+    S.vtTrack[q.v] = kept;
+    if (own < as_float(kOgcFarBits)) atomicMin(&S.ogcBound[q.v], as_uint(own));
+    S.softFlag[p] = vt_emit<false>(S, P, q.v, kept, 0u);
+    return;
+  }
+  EeQuery q;
+  q.e = p - P.numParticles;
+  q.a = S.edges[q.e];
+  q.a0 = P3(S, q.a.x);
+  q.a1 = P3(S, q.a.y);
+  q.ga = S.clothGroup[q.a.x];
+  q.solid = q.ga != 0u && S.clothGroup[q.a.y] == q.ga && S.clothVert[q.a.x] == 0u && S.clothVert[q.a.y] == 0u;
+  q.ns0 = S.nbrStart[q.a.x];
+  q.ne0 = q.ns0 + S.nbrCount[q.a.x];
+  q.ns1 = S.nbrStart[q.a.y];
+  q.ne1 = q.ns1 + S.nbrCount[q.a.y];
+  q.n2s = S.nbr2Start[q.a.x];
+  q.n2e = q.n2s + S.nbr2Count[q.a.x];
+  q.cap2 = cs * cs;
+  const U4 tr = S.eePrev[q.e];
+  ee_consider(S, P, q, best, own, tr.x);
+  ee_consider(S, P, q, best, own, tr.y);
+  ee_consider(S, P, q, best, own, tr.z);
+  ee_consider(S, P, q, best, own, tr.w);
+  for (int side = 0; side < 2; side++) {
+    const uint32_t vv = side == 0 ? q.a.x : q.a.y;
+    const uint32_t es = S.vEdgeStart[vv], ec = min(S.vEdgeCount[vv], 6u);
+    for (uint32_t k = es; k < es + ec; k++) {
+      const uint32_t ie = S.vEdgeList[k];
+      if (ie < P.numContactEdges) ee_consider(S, P, q, best, own, S.eePrev[ie].x);
+    }
+  }
+  const V3 velA = (xyz(S.velLin[q.a.x]) + xyz(S.velLin[q.a.y])) * 0.5f;
+  V3 relV = velA;
+  if (best.id0 < P.numContactEdges) {
+    const U2 bt = S.edges[best.id0];
+    relV = relV - (xyz(S.velLin[bt.x]) + xyz(S.velLin[bt.y])) * 0.5f;
+  }
+  if (iso_reseed(P, q.e, frame, relV, best.id0 == kNone, q.solid)) {
+    const Cell3 cc = cell_of((q.a0 + q.a1) * 0.5f, cs);
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const uint32_t h =
+              cell_hash(cell_add(cc.x, dx), cell_add(cc.y, dy), cell_add(cc.z, dz), 0u, P.elemHashSize);
+          const uint32_t s0 = S.elemCellStart[2u * h + 1u], e0 = S.elemCellStart[2u * h + 2u];
+          for (uint32_t k = s0; k < e0; k++) ee_consider(S, P, q, best, own, S.elemCells[k] & ~kElemEdgeBit);
+        }
+  }
+  const U4 kept = U4{best.id0, best.id1, best.id2, best.id3};
+  S.eeTrack[q.e] = kept;
+  if (own < as_float(kOgcFarBits)) {
+    atomicMin(&S.ogcBound[q.a.x], as_uint(own));
+    atomicMin(&S.ogcBound[q.a.y], as_uint(own));
+  }
+  S.softFlag[p] = ee_emit<false>(S, P, q.e, kept, 0u);
+}
+
+// Conditional in-iteration refresh (25_cloth.metal:857-939): when more than
+// 1 % of the surface particles were held back by their bound since the last
+// check, each re-anchors at its live pose with its bound recomputed from the
+// tracked sets.  Every workgroup reads the truncation count, then counts
+// itself in; the last one to arrive clears both counters for the next
+// iteration.
+__global__ void __launch_bounds__(256) k_ogc_refresh(DeviceState S, StepParams P) {
+  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
+  __shared__ uint32_t fire;
+  if (threadIdx.x == 0) {
+    const uint32_t exceeded = __hip_atomic_load(&S.counters[CTR_OGC], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    fire = (uint64_t)exceeded * 100u > (uint64_t)P.numParticles ? 1u : 0u;
+    __threadfence();
+    if (atomicAdd(&S.counters[CTR_OGC_ARRIVE], 1u) == gridDim.x - 1u) {
+      S.counters[CTR_OGC] = 0u;
+      S.counters[CTR_OGC_ARRIVE] = 0u;
+      if (fire) S.counters[CTR_OGC_REFRESH] += 1u;
+    }
+  }
+  __syncthreads();
+  if (!fire) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.numParticles) return;
+  const uint32_t v = S.particleIdx[i];
+  const V3 pv = P3(S, v);
+  const uint32_t n2s = S.nbr2Start[v], n2e = n2s + S.nbr2Count[v];
+  float dmin = 3.0e38f;
+  const U4 tr = S.vtTrack[v];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t t = u4_at(tr, k);
+    if (t >= P.numTris) break;
+    const U4 tv = S.tris[t];
+    if (nbr_contains(S.nbr2List, n2s, n2e, tv.x) || nbr_contains(S.nbr2List, n2s, n2e, tv.y) ||
+        nbr_contains(S.nbr2List, n2s, n2e, tv.z))
+      continue;
+    V3 bary;
+    const V3 d = pv - closest_pt_triangle(pv, P3(S, tv.x), P3(S, tv.y), P3(S, tv.z), bary);
+    dmin = fminf(dmin, dot(d, d));
+  }
+  const uint32_t es = S.vEdgeStart[v], ec = min(S.vEdgeCount[v], 6u);
+  for (uint32_t k = es; k < es + ec; k++) {
+    const uint32_t me = S.vEdgeList[k];
+    if (me >= P.numContactEdges) continue;
+    const uint32_t other = S.eeTrack[me].x;
+    if (other >= P.numContactEdges) continue;
+    const U2 eo = S.edges[other];
+    if (nbr_contains(S.nbr2List, n2s, n2e, eo.x) || nbr_contains(S.nbr2List, n2s, n2e, eo.y)) continue;
+    const U2 em = S.edges[me];
+    const V3 a0 = P3(S, em.x), a1 = P3(S, em.y), b0 = P3(S, eo.x), b1 = P3(S, eo.y);
+    float s, t;
+    closest_seg_seg(a0, a1, b0, b1, s, t);
+    const V3 d = (a0 + (a1 - a0) * s) - (b0 + (b1 - b0) * t);
+    dmin = fminf(dmin, dot(d, d));
+  }
+  S.ogcBound[v] = as_uint(dmin);
+  S.ogcPrev[v] = S.posLin[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -1264,11 +1654,6 @@ void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P,
   uint32_t nElem = P.numTris + P.numEdges;
   uint32_t eb = (nElem + 255) / 256;
   const uint32_t slots = 2u * P.elemHashSize;
-  if (P.groupSkip) {
-    hipLaunchKernelGGL(k_group_clear, dim3((P.numGroups + 256) / 256), dim3(256), 0, st, S, P);
-    hipLaunchKernelGGL(k_group_box, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
-    hipLaunchKernelGGL(k_group_near, dim3((P.numGroups * P.numGroups + 255) / 256), dim3(256), 0, st, S, P);
-  }
   launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
   hipLaunchKernelGGL(k_el_bin<false>, dim3(eb), dim3(256), 0, st, S, P);
   launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, slots + 1u, S.scanScratch);
@@ -1325,6 +1710,30 @@ void launch_dat_color_pass(hipStream_t st, const DeviceState& S, const StepParam
 void launch_dat_restore(hipStream_t st, const DeviceState& S, const StepParams& P) {
   if (!P.datOn) return;
   hipLaunchKernelGGL(k_dat_restore, dim3((P.numBodies + 255) / 256), dim3(256), 0, st, S, P);
+}
+
+// the isotropic mode's per-frame work at the step-start pose, after the
+// rigid-feature records: tracked sets, bounds and the tracked contacts
+void launch_iso_track(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
+  if (!P.isoOn) return;
+  const uint32_t nBegin = P.numBodies > P.numContactEdges ? P.numBodies : P.numContactEdges;
+  hipLaunchKernelGGL(k_iso_begin, dim3((nBegin + 255) / 256), dim3(256), 0, st, S, P);
+  const uint32_t nElem = P.numTris + P.numContactEdges, slots = 2u * P.elemHashSize;
+  launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
+  hipLaunchKernelGGL(k_iso_bin<false>, dim3((nElem + 255) / 256), dim3(256), 0, st, S, P);
+  launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, slots + 1u, S.scanScratch);
+  hipLaunchKernelGGL(k_copy_u32, dim3(cap_blocks(slots, gridCap)), dim3(256), 0, st, S.elemCellCursor,
+                     S.elemCellStart, slots);
+  hipLaunchKernelGGL(k_iso_bin<true>, dim3((nElem + 255) / 256), dim3(256), 0, st, S, P);
+  const uint32_t nProd = P.numParticles + P.numContactEdges;
+  hipLaunchKernelGGL(k_iso_track<false>, dim3((nProd + 255) / 256), dim3(256), 0, st, S, P);
+  launch_exclusive_scan(st, S.softFlag, S.softStart, nProd + 1u, S.scanScratch);
+  hipLaunchKernelGGL(k_iso_track<true>, dim3((nProd + 255) / 256), dim3(256), 0, st, S, P);
+}
+
+void launch_ogc_refresh(hipStream_t st, const DeviceState& S, const StepParams& P) {
+  if (!P.isoOn || P.numParticles == 0) return;
+  hipLaunchKernelGGL(k_ogc_refresh, dim3((P.numParticles + 255) / 256), dim3(256), 0, st, S, P);
 }
 
 void launch_viscosity(hipStream_t st, const DeviceState& S, const StepParams& P) {

@@ -85,7 +85,13 @@ enum Counter : uint32_t {
   CTR_CONV_SEQ,            // residual checks completed this frame
   CTR_CONV_ITERS,          // sweeps run this frame
   CTR_CONV_RES,            // residual of the last check (f32 bits)
-  CTR_TOTAL = 16 + 2 * kMaxColors + 16
+  // isotropic OGC limiter
+  CTR_SOFT_ISO,            // tracked V-T / E-E records (after the rigid-feature ones)
+  CTR_OGC,                 // truncations since the last refresh check
+  CTR_OGC_ARRIVE,          // workgroups of the running refresh check that have read CTR_OGC
+  CTR_OGC_TRUNC,           // truncations this frame
+  CTR_OGC_REFRESH,         // refreshes fired this frame
+  CTR_TOTAL = 16 + 2 * kMaxColors + 24
 };
 
 // Progress of a residual-terminated frame, written by the device into mapped
@@ -418,8 +424,7 @@ struct StepParams {
   uint32_t numContactEdges;  // edges [0, numContactEdges) carry E-E contacts; later ones bound motion only
   uint32_t pairStream;       // the V-T / E-E pair stream and its contacts run
   uint32_t datOn;            // the Planar-DAT limiter passes run
-  uint32_t numGroups;        // surface groups (1-based ids)
-  uint32_t groupSkip;        // queries skip producers of a group with no other group in reach
+  uint32_t isoOn;            // the isotropic OGC limiter and its tracked contacts run
   float elemCellSize, elemMargin, datQueryRadius, surfaceContactCellSize, clothViscosity;
   float fineCellSize;       // > 0: candidate search grid of 2 x the largest hashed radius
 };

@@ -26,12 +26,16 @@ void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, 
 void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
 // element grid + V-T / E-E pair stream at the current (predictor) pose
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
-constexpr uint32_t kMaxSkipGroups = 1024;  // the per-surface reach test runs up to this many surfaces
 // legacy servo rates: each listed joint's motor target advances by rate * dt, wrapped to [-pi, pi]
 void launch_rate_motors(hipStream_t st, const DeviceState& S, const uint32_t* joints, const float* rates, uint32_t n,
                         float dt);
 // OGC records from the pair stream, soft counts, soft persistence map rebuild
 void launch_dat_contacts(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+// isotropic limiter: tracked sets, clearance bounds and tracked contacts at
+// the step-start pose (after the rigid-feature records); the conditional
+// bound refresh after each iteration's dual
+void launch_iso_track(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
+void launch_ogc_refresh(hipStream_t st, const DeviceState& S, const StepParams& P);
 // Planar-DAT truncation: the full pass after the predictor (anchor = step
 // start), the anchor at the accepted predictor pose, the pass after each
 // static color, and the failed-surface restore before finalize

@@ -101,9 +101,23 @@ __global__ void __launch_bounds__(256) k_warmstart_bodies(DeviceState S, StepPar
   if (!finite_bits(w)) w = 0.0f;
   S.initLin[gid] = f4(xyz(pl), 0.0f);
   S.initAng[gid] = pa;
+  // isotropic limiter: the step-start pose anchors the bounds of every
+  // particle (pinned ones included), and the guess stays within the bound
+  // (40_solver.metal:697-717)
+  const bool isoPart = P.isoOn != 0u && S.shape[gid].w < 0.0f;
+  if (isoPart) S.ogcPrev[gid] = f4(xyz(pl), 0.0f);
   F4 newL = pl, newA = pa;
   if (mass > 0.0f) {
     V3 guess = xyz(pl) + vl * dt + v3(0, 0, g) * (w * dt * dt);
+    if (isoPart) {
+      const float d2 = as_float(S.ogcBound[gid]);
+      if (d2 < 1e37f) {
+        const float bv = fmaxf(0.45f * sqrtf(d2), 0.2f * fabsf(S.shape[gid].w));
+        const V3 tot = guess - xyz(pl);
+        const float t2 = dot(tot, tot);
+        if (t2 > bv * bv) guess = xyz(pl) + tot * (bv / sqrtf(t2));
+      }
+    }
     newL = f4(guess, mass);
     newA = qi;
     S.posLin[gid] = newL;

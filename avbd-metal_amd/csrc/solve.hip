@@ -424,6 +424,26 @@ __device__ __forceinline__ V3 group_sum3(V3 v) {
   return v3(group_sum<L>(v.x), group_sum<L>(v.y), group_sum<L>(v.z));
 }
 
+// Isotropic limiter (40_solver.metal:1361-1382): a particle's total
+// displacement from the pose its bound holds around stays within
+// max(0.45 sqrt(bound), 0.2 r); each truncation is counted for the refresh
+// check and the frame's diagnostics.
+__device__ __forceinline__ V3 ogc_truncate(const DeviceState& S, uint32_t body, V3 cur, V3 dx, float rad) {
+  const float d2 = as_float(S.ogcBound[body]);
+  if (!(d2 < 1e37f)) return dx;
+  const float bv = fmaxf(0.45f * sqrtf(d2), 0.2f * rad);
+  const V3 anchor = xyz(S.ogcPrev[body]);
+  V3 tot = (cur + dx) - anchor;
+  const float t2 = dot(tot, tot);
+  if (t2 > bv * bv) {
+    tot = tot * (bv / sqrtf(t2));
+    dx = (anchor + tot) - cur;
+    atomicAdd(&S.counters[CTR_OGC], 1u);
+    atomicAdd(&S.counters[CTR_OGC_TRUNC], 1u);
+  }
+  return dx;
+}
+
 // Gathers and stamps one body's entries (lane-strided), reduces, and lane 0
 // solves and writes.  Must be reached by whole waves (EXEC full) because of
 // the ballot and DPP reductions; inactive groups pass active = false.
@@ -548,6 +568,7 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   float maxLin = 0.35f * rad;
   float l2 = dot(dxL, dxL);
   if (l2 > maxLin * maxLin) dxL *= maxLin * rsq(l2);
+  if (P.isoOn && !rigid) dxL = ogc_truncate(S, body, xyz(pl), dxL, rad);
   // (hi, lo) + dx, renormalised exactly: hi = f32(hi + t), lo = the error term
   V3 hiN, loN;
   two_sum3(xyz(pl), lo + dxL, hiN, loN);

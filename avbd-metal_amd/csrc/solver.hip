@@ -66,6 +66,8 @@ struct avbd_solver {
   uint32_t numContactEdges = 0;  // authored-triangle edges: the E-E contact prefix of the edge list
   bool pairStream = false;       // V-T / E-E pair stream and its contacts (some pair can be accepted)
   bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
+  bool isoOn = false;            // isotropic OGC limiter and its tracked V-T / E-E contacts
+  uint32_t lastOgcTrunc = 0, lastOgcRefresh = 0;  // the last retired frame's truncations / refreshes
   std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
   std::vector<float> rateValues;
   F4* jointPenInit = nullptr;      // upload-time joint penalties (lin [nj], ang [nj]), restored on reset
@@ -73,8 +75,6 @@ struct avbd_solver {
   uint32_t* fixedScratch = nullptr;  // this device's address of the no-allocation staging array
   uint32_t* rateJointDev = nullptr;
   float* rateValueDev = nullptr;
-  uint32_t numGroups = 0;        // connected collision surfaces
-  bool groupSkip = false;        // the queries skip surfaces with no other surface in reach
   uint32_t elemHashSize = 0, maxSoft = 0, softMapCapacity = 0, maxDatPairs = 0;
   float elemCellSize = 0.0f, elemMargin = 0.0f, datQueryRadius = 0.0f, surfaceContactCellSize = 0.0f;
   float datQueryRadiusDerived = 0.0f;  // rq derived at create (a test hook may replace datQueryRadius)
@@ -222,9 +222,8 @@ StepParams make_params(const avbd_solver* s) {
   P.numEdges = s->nEdges;
   P.numContactEdges = s->numContactEdges;
   P.datOn = s->datOn ? 1u : 0u;
+  P.isoOn = s->isoOn ? 1u : 0u;
   P.pairStream = s->pairStream ? 1u : 0u;
-  P.numGroups = s->numGroups;
-  P.groupSkip = s->groupSkip ? 1u : 0u;
   P.numParticles = s->nParticles;
   P.elemHashSize = s->elemHashSize;
   P.maxSoft = s->maxSoft;
@@ -279,6 +278,8 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
     stage("dat-pairs");
     launch_rt_emit(st, S, P);
     stage("rt-emit");
+    launch_iso_track(st, S, P, s->gridCap);
+    stage("vt-ee-track");
   }
   launch_warmstart(st, S, P);
   stage("warmstart");
@@ -349,6 +350,7 @@ hipError_t enqueue_iterations(avbd_solver* s, const DeviceState& S, const StepPa
       }
     }
     launch_dual(st, S, P, s->gridCap);
+    launch_ogc_refresh(st, S, P);
   }
   return hipSuccess;
 }
@@ -467,6 +469,8 @@ avbd_status retire_slot(avbd_solver* s, int slot) {
   s->stats.dat_vt_degeneracies = (int32_t)c[CTR_DAT_VT_DEGEN];
   s->stats.dat_ee_degeneracies = (int32_t)c[CTR_DAT_EE_DEGEN];
   s->stats.dat_nonfinite = (int32_t)c[CTR_DAT_NONFINITE];
+  s->lastOgcTrunc = c[CTR_OGC_TRUNC];
+  s->lastOgcRefresh = c[CTR_OGC_REFRESH];
   s->stats.color_conflicts = (int32_t)c[CTR_COLOR_CONFLICTS];
   if (s->dynamicColoring) {
     int maxUsed = -1;
@@ -647,14 +651,88 @@ struct ClothHost {
   uint32_t numContactEdges = 0;  // edges of authored triangles (E-E contacts); tet-boundary edges follow
   bool pairStream = false;       // some V-T / E-E pair can be accepted: two surfaces, or a self-colliding one
   bool datOn = false;            // Planar-DAT limiter: a self-colliding (authored) surface exists
-  uint32_t numGroups = 0;        // connected surfaces
-  bool allParticleVerts = true;  // every collision-triangle vertex is a particle
+  bool isoOn = false;            // isotropic OGC limiter: tets, and two surfaces or an authored one
+  // isotropic mode topology: triangle edge-adjacency, 2-rings, incident
+  // edges, authored-vertex flags
+  std::vector<U4> triAdj;
+  std::vector<uint32_t> nbr2Start, nbr2Count, nbr2List, vEdgeStart, vEdgeCount, vEdgeList, clothVert;
   float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
 };
 
 float body_pos_dist(const avbd_scene_desc* d, uint32_t a, uint32_t b) {
   V3 d3 = rd3(d->body_position, a) - rd3(d->body_position, b);
   return sqrtf(d3.x * d3.x + d3.y * d3.y + d3.z * d3.z);
+}
+
+// Static data of the isotropic mode (GPUSolver.swift:1336-1460): triangle
+// adjacency over shared edges (filled in ascending edge-key order), sorted
+// 2-rings (the 1-ring and its 1-rings, the body itself excluded), incident
+// edges of every body in edge order, and the vertices of authored triangles.
+void build_iso_topology(ClothHost& c, uint32_t nb, uint32_t na) {
+  const uint32_t nr = (uint32_t)c.tris.size();
+  std::map<uint64_t, std::pair<int64_t, int64_t>> edgeTris;
+  for (uint32_t t = 0; t < nr; t++) {
+    const U4 id = c.tris[t];
+    const uint32_t ev[3][2] = {{id.x, id.y}, {id.y, id.z}, {id.x, id.z}};
+    for (const auto& e : ev) {
+      const uint64_t k = (uint64_t)std::min(e[0], e[1]) << 32 | std::max(e[0], e[1]);
+      auto it = edgeTris.find(k);
+      if (it == edgeTris.end()) edgeTris.emplace(k, std::make_pair((int64_t)t, (int64_t)-1));
+      else it->second.second = t;
+    }
+  }
+  c.triAdj.assign(std::max<uint32_t>(1, nr), U4{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu});
+  std::vector<int> fill(std::max<uint32_t>(1, nr), 0);
+  auto put = [&](uint32_t t, uint32_t o) {
+    if (fill[t] == 0) c.triAdj[t].x = o;
+    else if (fill[t] == 1) c.triAdj[t].y = o;
+    else if (fill[t] == 2) c.triAdj[t].z = o;
+    else return;
+    fill[t]++;
+  };
+  for (const auto& kv : edgeTris) {
+    if (kv.second.second < 0) continue;
+    put((uint32_t)kv.second.first, (uint32_t)kv.second.second);
+    put((uint32_t)kv.second.second, (uint32_t)kv.second.first);
+  }
+  c.nbr2Start.assign(nb, 0);
+  c.nbr2Count.assign(nb, 0);
+  c.nbr2List.clear();
+  std::vector<uint32_t> two;
+  for (uint32_t i = 0; i < nb; i++) {
+    c.nbr2Start[i] = (uint32_t)c.nbr2List.size();
+    if (c.nbrCount[i] == 0) continue;
+    two.clear();
+    for (uint32_t k = 0; k < c.nbrCount[i]; k++) {
+      const uint32_t n1 = c.nbrList[c.nbrStart[i] + k];
+      two.push_back(n1);
+      for (uint32_t q = 0; q < c.nbrCount[n1]; q++) two.push_back(c.nbrList[c.nbrStart[n1] + q]);
+    }
+    std::sort(two.begin(), two.end());
+    two.erase(std::unique(two.begin(), two.end()), two.end());
+    two.erase(std::remove(two.begin(), two.end(), i), two.end());
+    c.nbr2Count[i] = (uint32_t)two.size();
+    c.nbr2List.insert(c.nbr2List.end(), two.begin(), two.end());
+  }
+  std::vector<std::vector<uint32_t>> ve(nb);
+  for (uint32_t e = 0; e < (uint32_t)c.edges.size(); e++) {
+    ve[c.edges[e].x].push_back(e);
+    ve[c.edges[e].y].push_back(e);
+  }
+  c.vEdgeStart.assign(nb, 0);
+  c.vEdgeCount.assign(nb, 0);
+  c.vEdgeList.clear();
+  for (uint32_t i = 0; i < nb; i++) {
+    c.vEdgeStart[i] = (uint32_t)c.vEdgeList.size();
+    c.vEdgeCount[i] = (uint32_t)ve[i].size();
+    c.vEdgeList.insert(c.vEdgeList.end(), ve[i].begin(), ve[i].end());
+  }
+  c.clothVert.assign(nb, 0);
+  for (uint32_t t = 0; t < na; t++) {
+    c.clothVert[c.tris[t].x] = 1;
+    c.clothVert[c.tris[t].y] = 1;
+    c.clothVert[c.tris[t].z] = 1;
+  }
 }
 
 // Collision triangles are the authored ones, then the tet boundary faces
@@ -743,17 +821,25 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
         surface[v] = 1;
       }
   }
-  // The pair stream (and its contacts) runs wherever a V-T / E-E pair can
-  // pass the topology filters: two surfaces, or a self-colliding one.  The
-  // Planar-DAT limiter runs where the reference selects it automatically,
-  // scenes with self-colliding (authored) surfaces (GPUSolver.swift:93-99);
-  // closed tet surfaces get the reference's isotropic bound there, which is
-  // not built: between tet bodies the contacts act without a limiter.
-  c.pairStream = gidx.size() >= 2 || !selfRoots.empty();
+  // The surface limiter follows the reference's automatic selection
+  // (GPUSolver.swift:93-99): Planar-DAT, with its V-T / E-E pair stream and
+  // the stream's contacts, for authored (self-colliding) surfaces without
+  // tets; the isotropic bound wherever tets exist -- the scene format
+  // carries no volumetric self-collision flag -- with its own tracked
+  // contacts.  With one closed surface and no authored triangle every
+  // candidate pair of the isotropic trackers is a same-solid pair, which they
+  // skip: the mode is then left off, an identity on the results.
   c.datOn = !selfRoots.empty();
-  c.numGroups = (uint32_t)gidx.size();
-  for (const U4& t : c.tris)
-    if (!isParticle(t.x) || !isParticle(t.y) || !isParticle(t.z)) c.allParticleVerts = false;
+  c.pairStream = c.datOn;
+  {
+    uint32_t np0 = 0;
+    for (uint32_t i = 0; i < nb; i++) np0 += surface[i];
+    c.isoOn = d->num_tets > 0 && np0 > 0 && (gidx.size() >= 2 || na > 0);
+  }
+  if (c.isoOn) {
+    c.datOn = false;
+    c.pairStream = false;
+  }
   for (uint32_t i = 0; i < nb; i++)
     if (surface[i]) c.particles.push_back(i);
   // membranes: rest 2x2 in the triangle's own frame, inverse column major
@@ -867,6 +953,12 @@ void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
   uint64_t budget = std::min<uint64_t>(pairCap, 8 * np + 8 * ne);
   c.maxDatPairs = (uint32_t)std::min<uint64_t>(pairCap, 0xFFFFFFF0ull);
   c.maxSoft = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, 4ull * nr + budget), kMaxSoftRecords - 1);
+  if (c.isoOn) {
+    // tracked-set capacity (GPUSolver.swift:439-442)
+    c.maxSoft = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, 4 * np + 4ull * nr + 4ull * c.numContactEdges),
+                                             kMaxSoftRecords - 1);
+    build_iso_topology(c, nb, na);
+  }
   c.softMapCapacity = next_pow2(std::max<uint32_t>(64u, 2u * c.maxSoft));
 }
 
@@ -1321,12 +1413,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->nTris = (uint32_t)ch.tris.size();
     s->numContactEdges = ch.numContactEdges;
     s->datOn = ch.datOn;
-    s->numGroups = ch.numGroups;
-    // Without self-collision a pair joins two surfaces, both within the
-    // query radius of each other's particle box: the queries skip every
-    // producer of a surface with no other surface in reach (same pairs and
-    // counters, less work when the surfaces are apart)
-    s->groupSkip = !ch.datOn && ch.pairStream && ch.allParticleVerts && ch.numGroups <= kMaxSkipGroups;
+    s->isoOn = ch.isoOn;
     s->pairStream = ch.pairStream;
     s->nEdges = (uint32_t)ch.edges.size();
     s->nParticles = (uint32_t)ch.particles.size();
@@ -1527,13 +1614,22 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.datAnchor, cl ? nb : 1u) ALLOC(S.datT, cl ? nb : 1u)
     ALLOC(S.datCount, nProd + 1) ALLOC(S.datStart, nProd + 1)
     ALLOC(S.datPairs, cl ? s->maxDatPairs : 1u)
-    uint32_t flagN = cl ? std::max(s->nTris, s->maxDatPairs) + 1 : 1u;
+    uint32_t flagN = cl ? std::max(std::max(s->nTris, s->maxDatPairs), s->nParticles + s->numContactEdges) + 1 : 1u;
     ALLOC(S.softFlag, flagN) ALLOC(S.softStart, flagN)
     ALLOC(S.soft, cl ? s->maxSoft : 1u) ALLOC(S.prevSoft, cl ? s->maxSoft : 1u)
     ALLOC(S.softMapKeyA, cl ? s->softMapCapacity : 1u) ALLOC(S.softMapKeyB, cl ? s->softMapCapacity : 1u)
     ALLOC(S.softMapVal, cl ? s->softMapCapacity : 1u)
     ALLOC(S.velScratch, cl ? nb : 1u)
-    ALLOC(S.groupBox, s->groupSkip ? 6 * (s->numGroups + 1) : 1u) ALLOC(S.groupNear, s->groupSkip ? s->numGroups + 1 : 1u)
+    const bool iso = s->isoOn;
+    const uint32_t nce = std::max<uint32_t>(1, s->numContactEdges);
+    ALLOC(S.triAdj, iso ? std::max<uint32_t>(1, s->nTris) : 1u)
+    ALLOC(S.nbr2Start, iso ? nb : 1u) ALLOC(S.nbr2Count, iso ? nb : 1u)
+    ALLOC(S.nbr2List, std::max<size_t>(1, ch.nbr2List.size()))
+    ALLOC(S.vEdgeStart, iso ? nb : 1u) ALLOC(S.vEdgeCount, iso ? nb : 1u)
+    ALLOC(S.vEdgeList, std::max<size_t>(1, ch.vEdgeList.size())) ALLOC(S.clothVert, iso ? nb : 1u)
+    ALLOC(S.vtTrack, iso ? nb : 1u) ALLOC(S.vtPrev, iso ? nb : 1u)
+    ALLOC(S.eeTrack, iso ? nce : 1u) ALLOC(S.eePrev, iso ? nce : 1u)
+    ALLOC(S.ogcBound, iso ? nb : 1u) ALLOC(S.ogcPrev, iso ? nb : 1u) ALLOC(S.isoFrame, 1u)
   }
   uint32_t scanMax = std::max<uint32_t>(std::max<uint32_t>(s->gridHashSize, producers), std::max(nb, kMaxColors * bb));
   scanMax = std::max<uint32_t>(scanMax, std::max(2 * s->elemHashSize + 1, s->nParticles + s->nEdges + 1));
@@ -1572,6 +1668,21 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.posLinSnap, pl) ||
       upload(S.posAngSnap, pa))
     return AVBD_ERR_HIP;
+
+  if (s->isoOn) {
+    // isotropic mode: topology, empty tracked sets, cleared bounds
+    const std::vector<U4> none4(std::max<uint32_t>(s->nb, s->numContactEdges),
+                                U4{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu});
+    const std::vector<U4> noneB(none4.begin(), none4.begin() + s->nb);
+    const std::vector<U4> noneE(none4.begin(), none4.begin() + s->numContactEdges);
+    const std::vector<uint32_t> far(s->nb, 0x7F7F7F7Fu);
+    if (upload(S.triAdj, ch.triAdj) || upload(S.nbr2Start, ch.nbr2Start) || upload(S.nbr2Count, ch.nbr2Count) ||
+        upload(S.nbr2List, ch.nbr2List) || upload(S.vEdgeStart, ch.vEdgeStart) ||
+        upload(S.vEdgeCount, ch.vEdgeCount) || upload(S.vEdgeList, ch.vEdgeList) ||
+        upload(S.clothVert, ch.clothVert) || upload(S.vtTrack, noneB) || upload(S.vtPrev, noneB) ||
+        upload(S.eeTrack, noneE) || upload(S.eePrev, noneE) || upload(S.ogcBound, far))
+      return AVBD_ERR_HIP;
+  }
 
   if (hipHostMalloc((void**)&s->pinnedCounters, 2 * CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault) !=
       hipSuccess)
@@ -1723,7 +1834,7 @@ void autotune(avbd_solver* s) {
   // dynamic coloring with per-color launches would dispatch all 64 colors
   if (s->dynamicColoring && wantMode == 0) modeCands = {2, 1};
   // cloth: the Planar-DAT passes run between the colors' launches
-  if (s->datOn) modeCands = {1};
+  if (s->datOn || s->isoOn) modeCands = {1};
   if (laneCands.size() == 1 && modeCands.size() == 1) {
     s->lanes = laneCands[0];
     s->sweepMode = modeCands[0];
@@ -2493,6 +2604,7 @@ extern "C" avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info
     info->num_contact_edges = s->numContactEdges;
     info->pair_stream = s->pairStream ? 1u : 0u;
     info->dat_on = s->datOn ? 1u : 0u;
+    info->iso_on = s->isoOn ? 1u : 0u;
   }
   if (edges && s->nEdges && hipMemcpy(edges, s->S.edges, 8 * (size_t)s->nEdges, hipMemcpyDeviceToHost))
     return AVBD_ERR_HIP;
@@ -2561,6 +2673,32 @@ extern "C" avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity,
   // the frame just retired wrote into what is now prevSoft
   if (records && m && hipMemcpy(records, s->S.prevSoft, sizeof(SoftRec) * m, hipMemcpyDeviceToHost))
     return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_iso_state(avbd_solver* s, uint32_t* vt_track, uint32_t* ee_track, float* bound,
+                                          float* anchor, uint32_t* counts) {
+  if (!s) return AVBD_ERR_INVALID_ARGUMENT;
+  if (!s->isoOn) return AVBD_ERR_UNSUPPORTED;
+  avbd_status st = sync_all(s);
+  if (st != AVBD_OK) return st;
+  const size_t nb = s->nb, ne = s->numContactEdges;
+  if (vt_track && hipMemcpy(vt_track, s->S.vtTrack, 16 * nb, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  if (ee_track && ne && hipMemcpy(ee_track, s->S.eeTrack, 16 * ne, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  if (bound && hipMemcpy(bound, s->S.ogcBound, 4 * nb, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+  if (anchor) {
+    std::vector<F4> tmp(nb);
+    if (hipMemcpy(tmp.data(), s->S.ogcPrev, 16 * nb, hipMemcpyDeviceToHost)) return AVBD_ERR_HIP;
+    for (size_t i = 0; i < nb; i++) {
+      anchor[3 * i] = tmp[i].x;
+      anchor[3 * i + 1] = tmp[i].y;
+      anchor[3 * i + 2] = tmp[i].z;
+    }
+  }
+  if (counts) {
+    counts[0] = s->lastOgcTrunc;
+    counts[1] = s->lastOgcRefresh;
+  }
   return AVBD_OK;
 }
 

@@ -85,8 +85,16 @@ struct DeviceState {
   SoftRec *soft, *prevSoft;
   uint32_t *softMapKeyA, *softMapKeyB, *softMapVal;
   F4* velScratch;             // unsmoothed velocities for the 1-ring viscosity gather
-  uint32_t* groupBox;         // per surface group (1-based): box of its particles, 6 order-preserving keys
-  uint32_t* groupNear;        // per surface group: 1 when another group's box lies within the query radius
+  // isotropic OGC limiter
+  U4* triAdj;                 // per collision triangle: its edge neighbours (ascending edge order, 0xFFFFFFFF: none)
+  uint32_t *nbr2Start, *nbr2Count, *nbr2List;  // sorted topological 2-ring of every body (itself excluded)
+  uint32_t *vEdgeStart, *vEdgeCount, *vEdgeList;  // incident edges of every body, in edge order
+  uint32_t* clothVert;        // per body: 1 for a vertex of an authored triangle
+  U4 *vtTrack, *vtPrev;       // per body: tracked triangles of this frame / of the previous one
+  U4 *eeTrack, *eePrev;       // per contact edge: tracked edges of this frame / of the previous one
+  uint32_t* ogcBound;         // per body: squared clearance bound (f32 bits, atomic min)
+  F4* ogcPrev;                // per body: the pose the bounds hold around
+  uint32_t* isoFrame;         // frames stepped so far (the trackers' reseed stagger)
   // misc
   uint32_t* colorFlags;       // per-pass changed flags of the dynamic coloring
   uint32_t* counters;

@@ -7,7 +7,7 @@ set -u
 cd "$(dirname "$0")/../.."
 export TMPDIR=/tmp
 OUT=gpurun_out
-BENCH="bench.py --workload cloth --steps 4 --warmup 40 --no-cpu --residual-steps 0 --lanes ${LANES:-8} --sweep ${SWEEP:-1}"
+BENCH="bench.py --workload cloth --steps 4 --warmup 40 --no-cpu --lanes ${LANES:-8} --sweep ${SWEEP:-1}"
 timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/prof_cloth -o trace -- python3 $BENCH > $OUT/prof_cloth.log 2>&1
 rc=$?; tail -2 $OUT/prof_cloth.log; [ -f $OUT/prof_cloth/trace_kernel_stats.csv ] || exit $rc
 python3 avbd-metal_amd/prof/summarize.py --kernels-only "$OUT/prof_cloth/trace_kernel_stats.csv" \

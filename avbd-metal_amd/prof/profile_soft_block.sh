@@ -11,7 +11,7 @@ cd "$(dirname "$0")/../.."
 export TMPDIR=/tmp
 OUT=gpurun_out
 CELLS=${CELLS:-20}
-BENCH="bench.py --cells $CELLS --steps 4 --warmup 2 --no-cpu --no-learned --residual-steps 0 --scale-cells 0 --lanes ${LANES:-32} --sweep ${SWEEP:-1}"
+BENCH="bench.py --cells $CELLS --steps 4 --warmup 2 --no-cpu --no-learned --quick --res-frames 0 --fp64-steps 0 --lanes ${LANES:-32} --sweep ${SWEEP:-1}"
 /opt/rocm/bin/hipcc -O2 --offload-arch=gfx950 -o avbd-metal_amd/lib/copy_calib avbd-metal_amd/prof/copy_calib.hip || exit 1
 echo "pass 1: kernel trace"
 timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/prof_trace -o trace -- python3 $BENCH > $OUT/prof_trace.log 2>&1

@@ -261,30 +261,35 @@ def batched_vs_plain(args, net):
 # ---------------------------------------------------------------------------
 def config_stack(args, threads):
     """10 unit cubes stacked with +-1 cm jitter, mu = 0.5, 10 iterations,
-    240 steps, on the GPU and on the oracle (the CPU path), with the FP64 rerun."""
+    240 steps: the backend, and on one host core the reference's own CPU path
+    (CPUSolver semantics), each against its FP64 rerun."""
     import numpy as np
     import oracle
     from avbd_amd import GPUSolver, demos
 
     s = demos.stack_scene()
     with GPUSolver(s) as g:
-        wall = timed_steps(g, 240, 10)
-        pg = g.positions()
-    o = oracle.OracleSolver(s)
-    o.step(10)
+        wall = timed_steps(g, 240, 0)
+        pg = g.positions().astype(np.float64)
+    cpu = oracle.CpuPathSolver(s)
     t0 = time.perf_counter()
-    o.step(240)
+    cpu.step(240)
     cpu_ms = (time.perf_counter() - t0) * 1e3 / 240
+    cpu64 = oracle.CpuPathSolver(s, precision=64)
+    cpu64.step(240)
     o64 = oracle.OracleSolver(s, precision=64)
-    o64.step(250)
+    o64.step(240)
     p64 = o64.bodyStates64()[0]
-    top = int(np.argmax(p64[:, 2]))
-    return {"workload": "10-cube stack (1 m cubes, +-1 cm jitter, mu 0.5), 10 iterations, dt 1/60",
-            "gpu_ms_per_step": round(wall * 1e3 / 240, 4), "cpu_oracle_ms_per_step": round(cpu_ms, 4),
-            "cpu_cores": 1,
-            "error_vs_fp64": {"steps": 250, "gpu_vs_fp64_rms_m": rms(pg, p64),
+    pc, pc64 = cpu.positions(), cpu64.positions()
+    return {"workload": "10-cube stack (1 m cubes, +-1 cm jitter, mu 0.5), 10 iterations, dt 1/60, 240 steps",
+            "gpu_ms_per_step": round(wall * 1e3 / 240, 4),
+            "cpu_path_ms_per_step": round(cpu_ms, 4), "cpu_path_cores": 1,
+            "cpu_path": "the reference's CPU solver semantics restated (plain LDL^T, O(n^2) broad phase, "
+                        "index-order Gauss-Seidel)",
+            "error_vs_fp64": {"steps": 240, "gpu_vs_fp64_rms_m": rms(pg, p64),
                               "gpu_vs_fp64_max_m": float(np.abs(pg - p64).max()),
-                              "top_cube_height_fp64_m": float(p64[top, 2])}}
+                              "cpu_path_fp32_vs_fp64_rms_m": rms(pc, pc64)},
+            "gpu_vs_cpu_path_max_m": float(np.linalg.norm(pg - pc, axis=1).max())}
 
 
 def config_cloth(args, threads):
@@ -390,6 +395,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cells", type=int, default=20)
+    ap.add_argument("--workload", choices=["soft_block", "cloth", "batched"], default="soft_block",
+                    help="the headline workload; cloth / batched report their step rate and roofline only")
     ap.add_argument("--res-frames", type=int, default=40, help="frames of the block iterated to residual 1e-4")
     ap.add_argument("--res-cap", type=int, default=1000, help="sweep cap of a frame iterated to 1e-4")
     ap.add_argument("--quick", action="store_true", help="headline, roofline and residual only")
@@ -421,11 +428,25 @@ def main():
     from avbd_amd import GPUSolver, demos
 
     model, nproc, allowed, threads = cpu_info()
-    scene = demos.soft_block_scene(cells=args.cells)
+    unit, pmc = "vertex-iters/s", f"primal_soft_block_{args.cells + 1}_pmc.json"
+    if args.workload == "cloth":
+        scene = demos.cloth_scene()
+        desc = (f"cloth drape 256^2 ({scene.numDynamic} vertices, {len(scene.tris)} triangles) onto a fixed "
+                f"r = 0.5 m sphere, {scene.settings.iterations} iterations, dt 1/60")
+        pmc = "primal_cloth_256_pmc.json"
+    elif args.workload == "batched":
+        scene = demos.batched_scene()[0]
+        desc = (f"4096 batched uneven rigid scenes ({scene.numDynamic} bodies, {len(scene.joints)} hinges), "
+                f"{scene.settings.iterations} iterations, dt 1/60, contact-aware dynamic coloring")
+        unit, pmc = "body-iters/s", None
+    else:
+        scene = demos.soft_block_scene(cells=args.cells)
+        desc = (f"soft block {args.cells}^3 cells, {scene.numDynamic} vertices, {len(scene.tets)} SNH tets, "
+                f"{scene.settings.iterations} iterations, dt 1/60, static coloring")
     iters, nv = scene.settings.iterations, scene.numDynamic
-    desc = (f"soft block {args.cells}^3 cells, {nv} vertices, {len(scene.tets)} SNH tets, "
-            f"{iters} iterations, dt 1/60, static coloring")
-    out = {"metric": METRIC, "unit": "vertex-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    if args.workload != "soft_block":
+        args.quick, args.no_learned, args.res_frames, args.fp64_steps = True, True, 0, 0
+    out = {"metric": METRIC, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded SplitMix64 scenes; random-init network)"}
     with GPUSolver(scene, lanesPerBody=args.lanes, sweepMode=args.sweep) as solver:
@@ -437,24 +458,27 @@ def main():
                "lanes_per_body": st.lanes_per_body,
                "sweep_mode": "persistent" if st.sweep_mode == 2 else "per-color launches",
                "end_residual": solver.residual()}
-        out["roofline"] = primal_roofline(solver, f"primal_soft_block_{args.cells + 1}_pmc.json")
+        out["roofline"] = primal_roofline(solver, pmc)
     out["config"] = cfg
     log(f"headline {out['ms_per_step']} ms/step, primal frac {out['roofline']['frac']}")
-    cfg["error_vs_fp64"] = error_vs_fp64(scene, args.fp64_steps, threads)
+    if args.fp64_steps > 0:
+        cfg["error_vs_fp64"] = error_vs_fp64(scene, args.fp64_steps, threads)
 
     net = None
     if not args.no_learned:
         net, cfg["learned_training"] = train_network(args)
         log(f"trained on {cfg['learned_training']['samples']} pairs")
-    res = plain_vs_learned(scene, net, args.res_frames, 1e-4, args.res_cap, fixed_res=False) if net else None
-    if res is None:
-        with GPUSolver(scene) as g:
-            res = {"plain": frames_to_tol(g, args.res_frames, 1e-4, args.res_cap)}
-    res.update({"tolerance": 1e-4, "sweep_cap": args.res_cap, "frames": args.res_frames, "check_every": 10,
-                "note": "frames from the drop through the landing; the FP32 / FP64 oracles need 110-370 "
-                        "sweeps per frame here, so the metric's 200-sweep cap is raised"})
-    cfg["residual_1e-4"] = res
-    log(f"residual 1e-4: {json.dumps(res)}")
+    if args.res_frames > 0:
+        if net is not None:
+            res = plain_vs_learned(scene, net, args.res_frames, 1e-4, args.res_cap, fixed_res=False)
+        else:
+            with GPUSolver(scene) as g:
+                res = {"plain": frames_to_tol(g, args.res_frames, 1e-4, args.res_cap)}
+        res.update({"tolerance": 1e-4, "sweep_cap": args.res_cap, "frames": args.res_frames, "check_every": 10,
+                    "note": "frames from the drop through the landing; the FP32 / FP64 oracles need 110-370 "
+                            "sweeps per frame here, so the metric's 200-sweep cap is raised"})
+        cfg["residual_1e-4"] = res
+        log(f"residual 1e-4: {json.dumps(res)}")
 
     if not args.quick:
         if args.scale_cells > 0 and args.scale_cells != args.cells:
@@ -490,7 +514,7 @@ def main():
     if net is not None:
         net.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, "vertex-iters/s", threads)
+        out["cpu_baseline"] = cpu_baseline(scene, unit, threads)
     print(json.dumps(out), flush=True)
 
 

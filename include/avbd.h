@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 4
+#define AVBD_ABI_VERSION 5
 #define AVBD_MAX_COLORS 64
 
 /* ---- status codes ------------------------------------------------------ */
@@ -350,18 +350,19 @@ avbd_status avbd_step_to_residual(avbd_solver* s, float tol, int32_t max_iterati
  * membrane and bend elements, and the grid / query parameters.
  * num_contact_edges: the edges of authored triangles, a prefix of edges[]
  * (E-E contacts; the tet-boundary edges after it only bound motion);
- * pair_stream: 1 when the V-T / E-E pair stream and its contacts run (two
- * or more surfaces, or a self-colliding one); dat_on: 1 when the Planar-DAT
- * limiter runs (a self-colliding, i.e. authored, surface exists: the
- * reference's automatic selection, GPUSolver.swift:93-99; its isotropic
- * bound for closed tet surfaces is not built).  Any output pointer may be NULL;
+ * pair_stream: 1 when the V-T / E-E pair stream and its contacts run (with
+ * the Planar-DAT limiter); dat_on: 1 when the Planar-DAT limiter runs (a
+ * self-colliding, i.e. authored, surface and no tets); iso_on: 1 when the isotropic OGC limiter and its tracked V-T /
+ * E-E contacts run (tets and either two or more surfaces or an authored
+ * one) -- the reference's automatic selection, GPUSolver.swift:93-99.
+ * Any output pointer may be NULL;
  * edges[2*num_edges] (ascending vertex ids, edge order of the pair stream),
  * particles[num_particles]. */
 typedef struct avbd_cloth_info {
     uint32_t num_tris, num_edges, num_particles, num_membranes, num_bends;
     uint32_t elem_hash_size, max_soft, soft_map_capacity, max_dat_pairs;
     float elem_cell_size, elem_margin, dat_query_radius, surface_contact_cell_size;
-    uint32_t num_contact_edges, pair_stream, dat_on;
+    uint32_t num_contact_edges, pair_stream, dat_on, iso_on;
 } avbd_cloth_info;
 avbd_status avbd_get_cloth_info(avbd_solver* s, avbd_cloth_info* info, uint32_t* edges, uint32_t* particles);
 
@@ -382,6 +383,17 @@ avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, u
  * raw 112-byte records -- ids u32[4], normal+friction, anchorA+flags, weights,
  * C0+lambdaCap, lambda+keyA, penalty+keyB (float4 each). */
 avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records);
+
+/* Isotropic limiter state after the last frame (vtTrackBuf, eeTrackBuf,
+ * boundsBuf, ogcPrevBuf of GPUSolver.swift:1336-1460, 25_cloth.metal:
+ * 350-940): vt_track[4*num_bodies] tracked triangles per body (0xFFFFFFFF:
+ * none), ee_track[4*num_contact_edges] tracked edges per authored edge,
+ * bound[num_bodies] squared clearance bounds (3.3961514e38: none),
+ * anchor[3*num_bodies] the poses the bounds hold around, counts[2] the
+ * truncations and refreshes of the frame.  Any output may be NULL;
+ * AVBD_ERR_UNSUPPORTED when the mode is off (avbd_cloth_info.iso_on). */
+avbd_status avbd_get_iso_state(avbd_solver* s, uint32_t* vt_track, uint32_t* ee_track, float* bound, float* anchor,
+                               uint32_t* counts);
 
 /* Persistence map lookup (pairMapFind, 20_broadphase.metal:596): for each
  * (a,b) collider pair returns the manifold index of the map built by the last

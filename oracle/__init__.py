@@ -55,6 +55,7 @@ def load():
         "oracle_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p]),
         "oracle_map_find": (C.c_int, [vp, u32p, C.c_uint32, i32p]),
         "oracle_get_manifold_keys": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
+        "oracle_get_iso_state": (C.c_int, [vp, u32p, u32p, fp, fp, u32p]),
         "oracle_splitmix64": (None, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), fp]),
         "oracle_snh_tet": (None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_solve6x6": (None, [dp, dp, dp, C.c_int]),
@@ -71,6 +72,12 @@ def load():
         "oracle_dat_pairs_at": (C.c_int, [vp, fp, C.c_uint32, u32p, u32p]),
         "oracle_membrane": (None, [dp, dp, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
         "oracle_hinge": (C.c_int, [fp, C.c_float, C.c_float, fp, fp]),
+        "oracle_cpu_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "oracle_cpu_destroy": (None, [vp]),
+        "oracle_cpu_step": (C.c_int, [vp, C.c_int]),
+        "oracle_cpu_get_body_states_f64": (C.c_int, [vp, C.c_uint32, dp, dp, dp, dp]),
+        "oracle_cpu_num_manifolds": (C.c_int, [vp]),
+        "oracle_cpu_max_constraint_error": (C.c_double, [vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -82,6 +89,47 @@ def load():
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+class CpuPathSolver:
+    """The reference's own CPU AVBD path (makeCPUSolver semantics: plain
+    LDL^T, no trust region, O(n^2) broad phase, index-order Gauss-Seidel,
+    lambda_max contact cap, diagonal hinge block, bilateral springs), FP32 or
+    FP64.  Rigid boxes and spheres, joints and springs."""
+
+    def __init__(self, scene, precision=32):
+        self._lib = load()
+        desc, self._keep = scene.to_desc()
+        h = C.c_void_p()
+        rc = self._lib.oracle_cpu_create(C.byref(desc), int(precision), C.byref(h))
+        if rc != 0:
+            raise ValueError(f"scene outside the CPU path ({rc})")
+        self._h = h
+        self.bodyCount = len(scene.bodies)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.oracle_cpu_destroy(self._h)
+            self._h = None
+
+    def step(self, n=1):
+        self._lib.oracle_cpu_step(self._h, int(n))
+
+    def bodyStates64(self):
+        n = self.bodyCount
+        p, r, v, w = (np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3)))
+        self._lib.oracle_cpu_get_body_states_f64(self._h, n, _p(p, C.c_double), _p(r, C.c_double),
+                                                 _p(v, C.c_double), _p(w, C.c_double))
+        return p, r, v, w
+
+    def positions(self):
+        return self.bodyStates64()[0]
+
+    def numManifolds(self):
+        return self._lib.oracle_cpu_num_manifolds(self._h)
+
+    def maxConstraintError(self):
+        return self._lib.oracle_cpu_max_constraint_error(self._h)
 
 
 class OracleSolver:
@@ -184,6 +232,24 @@ class OracleSolver:
         out = np.zeros((max(1, n.value), 8), np.uint32)
         self._lib.oracle_get_manifold_keys(self._h, n.value, C.byref(n), _p(out, C.c_uint32))
         return out[:n.value]
+
+    def isoState(self):
+        """Isotropic limiter state after the last frame: tracked triangles
+        (nb, 4), tracked edges (num_contact_edges, 4), squared clearance
+        bounds (nb,), anchor poses (nb, 3), the frame's truncations and
+        refreshes; None when the mode is off."""
+        info = self.clothInfo()
+        nb, ne = self.bodyCount, info["num_contact_edges"]
+        vt = np.zeros((max(1, nb), 4), np.uint32)
+        ee = np.zeros((max(1, ne), 4), np.uint32)
+        bound = np.zeros(max(1, nb), np.float32)
+        anchor = np.zeros((max(1, nb), 3), np.float32)
+        counts = np.zeros(2, np.uint32)
+        if self._lib.oracle_get_iso_state(self._h, _p(vt, C.c_uint32), _p(ee, C.c_uint32), _p(bound, C.c_float),
+                                          _p(anchor, C.c_float), _p(counts, C.c_uint32)) != 0:
+            return None
+        return {"vt_track": vt[:nb], "ee_track": ee[:ne], "bound": bound[:nb], "anchor": anchor[:nb],
+                "truncations": int(counts[0]), "refreshes": int(counts[1])}
 
     def mapFind(self, pairs):
         q = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)

@@ -281,14 +281,27 @@ struct Oracle : oracle_base {
   using Bend = BendT<T>;
   using Soft = SoftT<T>;
   uint32_t nTris = 0, numContactEdges = 0;
-  bool pairStream = false;  // the V-T / E-E pair stream and its contacts: some pair can pass the filters
+  bool pairStream = false;  // the V-T / E-E pair stream and its contacts (with the Planar-DAT limiter)
   bool datOn = false;       // the Planar-DAT limiter: a self-colliding (authored) surface exists
+  // isotropic OGC limiter (the reference's automatic choice where tets exist,
+  // GPUSolver.swift:93-102): tracked V-T / E-E contacts and per-vertex bounds
+  bool isoOn = false;
+  std::vector<std::array<uint32_t, 4>> triAdj;         // up to 3 edge-adjacent collision triangles
+  std::vector<uint32_t> nbr2Start, nbr2Count, nbr2List;  // sorted 2-ring per vertex
+  std::vector<uint32_t> vEdgeStart, vEdgeCount, vEdgeList;  // incident edges per vertex
+  std::vector<uint8_t> clothVert;                        // vertex of an authored triangle
+  std::vector<std::array<uint32_t, 4>> vtTrack, vtTrackPrev, eeTrack, eeTrackPrev;
+  std::vector<T> ogcBound;                               // squared clearance bound per body
+  std::vector<V<T>> ogcPrev;                             // the pose the bounds hold around
+  uint64_t ogcExceeded = 0;                              // primal truncations since the last refresh check
+  uint32_t ogcTruncFrame = 0, ogcRefreshFrame = 0;       // this frame's truncations / refreshes
   std::vector<std::array<uint32_t, 3>> tris;
   std::vector<std::pair<uint32_t, uint32_t>> edges;
   std::vector<uint32_t> nbrStart, nbrCount, nbrList, particles, selfColl, hashedRigidList;
   std::vector<Membrane> membranes;
   std::vector<Bend> bends;
   float elemMargin = 0, datQueryRadius = 0, elemCellSize = 0, surfaceContactCellSize = 0;
+  uint32_t elemHashSize = 64;
   uint32_t maxDatPairs = 1, maxSoft = 1;
   std::vector<std::array<uint32_t, 3>> datPairs;
   uint32_t datTotal = 0, datVT = 0, datOverflow = 0, datNonfinite = 0, softCandidates = 0, softRT = 0;
@@ -691,13 +704,26 @@ struct Oracle : oracle_base {
       for (uint32_t v : t)
         if (particle[v]) { clothGroup[v] = gid[r]; selfColl[v] = selfRoots.count(r) ? 1 : 0; surf[v] = 1; }
     }
-    // The pair stream runs where a pair can pass the topology filters; the
-    // Planar-DAT limiter where the reference selects it automatically, with
-    // self-colliding (authored) surfaces (GPUSolver.swift:93-99).  Closed tet
-    // surfaces get the reference's isotropic bound there, not restated.
-    pairStream = gid.size() >= 2 || !selfRoots.empty();
     datOn = !selfRoots.empty();
+    pairStream = datOn;  // the V-T / E-E pair stream runs with the Planar-DAT limiter
     for (uint32_t i = 0; i < nb; i++) if (surf[i]) particles.push_back(i);
+    // surface limiter (GPUSolver.swift:93-102): Planar-DAT for authored
+    // (self-colliding) surfaces without tets, the isotropic OGC bound
+    // wherever tets exist (the scene format carries no volumetric
+    // self-collision flag, so tet bodies never self-collide); the isotropic
+    // mode emits its own tracked contacts.
+    // With one closed surface and no authored triangle every candidate pair
+    // of the isotropic trackers is a same-solid pair, which they skip: no
+    // bound, no contact, no truncation -- the mode is then left off, an
+    // identity on the results.
+    isoOn = d->num_tets > 0 && !particles.empty() && (gid.size() >= 2 || nAuthored > 0);
+    if (isoOn) {
+      datOn = false;
+      pairStream = false;
+    }
+    clothVert.assign(nb, 0);
+    for (uint32_t t = 0; t < nAuthored; t++)
+      for (uint32_t v : tris[t]) clothVert[v] = 1;
     datAnchor.assign(nb, V<T>());
     datT.assign(nb, T(1));
     // StVK membranes: rest 2x2 in the triangle frame, inverse column major
@@ -777,11 +803,81 @@ struct Oracle : oracle_base {
     float reach = (2.0f * maxPartR + elemMargin) / 0.7f;
     surfaceContactCellSize = std::max(0.02f, std::max(std::max(reach, meanEdge * 1.05f), maxElemR + maxPartR + elemMargin));
     elemCellSize = std::max(0.02f, std::max(std::max(reach, meanEdge * 1.05f), 2.0f * maxElemR + datQueryRadius));
+    elemHashSize = nextPow2(std::max<uint32_t>(64u, 8u * nTris));
     uint64_t np = particles.size(), ne = edges.size();
     uint64_t cap = std::max<uint64_t>(1, 64 * np + 128 * ne);
     maxDatPairs = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFF0ull);
     maxSoft = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, 4ull * nTris + std::min<uint64_t>(cap, 8 * np + 8 * ne)),
                                            (1u << 24) - 1);
+    if (isoOn) {
+      // tracked-set capacity (GPUSolver.swift:439-440)
+      maxSoft = (uint32_t)std::min<uint64_t>(
+          std::max<uint64_t>(1, 4ull * particles.size() + 4ull * nTris + 4ull * numContactEdges), (1u << 24) - 1);
+      isoTopology();
+    }
+  }
+
+  // Static data of the isotropic mode (GPUSolver.swift:1336-1460): triangle
+  // adjacency over shared edges (filled in ascending edge order), sorted
+  // 2-rings, incident-edge lists in edge order, tracking state empty.
+  void isoTopology() {
+    std::map<uint64_t, std::pair<int64_t, int64_t>> edgeTris;
+    for (uint32_t t = 0; t < nTris; t++) {
+      const auto& id = tris[t];
+      uint32_t ev[3][2] = {{id[0], id[1]}, {id[1], id[2]}, {id[0], id[2]}};
+      for (auto& e : ev) {
+        uint64_t k = (uint64_t)std::min(e[0], e[1]) << 32 | std::max(e[0], e[1]);
+        auto it = edgeTris.find(k);
+        if (it == edgeTris.end()) edgeTris[k] = {t, -1};
+        else it->second.second = t;
+      }
+    }
+    triAdj.assign(std::max<uint32_t>(1, nTris), {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu});
+    std::vector<int> fill(std::max<uint32_t>(1, nTris), 0);
+    for (const auto& kv : edgeTris) {
+      if (kv.second.second < 0) continue;
+      uint32_t a = (uint32_t)kv.second.first, b = (uint32_t)kv.second.second;
+      if (fill[a] < 3) triAdj[a][fill[a]++] = b;
+      if (fill[b] < 3) triAdj[b][fill[b]++] = a;
+    }
+    nbr2Start.assign(nb, 0);
+    nbr2Count.assign(nb, 0);
+    nbr2List.clear();
+    for (uint32_t i = 0; i < nb; i++) {
+      nbr2Start[i] = (uint32_t)nbr2List.size();
+      if (nbrCount[i] == 0) continue;
+      std::set<uint32_t> two;
+      for (uint32_t k = 0; k < nbrCount[i]; k++) {
+        uint32_t n1 = nbrList[nbrStart[i] + k];
+        if (n1 != i) two.insert(n1);
+        for (uint32_t q = 0; q < nbrCount[n1]; q++) {
+          uint32_t n2 = nbrList[nbrStart[n1] + q];
+          if (n2 != i) two.insert(n2);
+        }
+      }
+      nbr2Count[i] = (uint32_t)two.size();
+      nbr2List.insert(nbr2List.end(), two.begin(), two.end());
+    }
+    std::vector<std::vector<uint32_t>> ve(nb);
+    for (uint32_t e = 0; e < edges.size(); e++) {
+      ve[edges[e].first].push_back(e);
+      ve[edges[e].second].push_back(e);
+    }
+    vEdgeStart.assign(nb, 0);
+    vEdgeCount.assign(nb, 0);
+    vEdgeList.clear();
+    for (uint32_t i = 0; i < nb; i++) {
+      vEdgeStart[i] = (uint32_t)vEdgeList.size();
+      vEdgeCount[i] = (uint32_t)ve[i].size();
+      vEdgeList.insert(vEdgeList.end(), ve[i].begin(), ve[i].end());
+    }
+    const std::array<uint32_t, 4> none{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    vtTrack.assign(nb, none);
+    vtTrackPrev.assign(nb, none);
+    eeTrack.assign(std::max<uint32_t>(1, numContactEdges), none);
+    eeTrackPrev.assign(std::max<uint32_t>(1, numContactEdges), none);
+    ogcBound.assign(nb, kOgcFar);
+    ogcPrev.assign(nb, V<T>());
   }
 
   void rebuildBuckets() {
@@ -1457,8 +1553,19 @@ struct Oracle : oracle_base {
       initLin[i] = pos[i];
       initAng[i] = rot[i];
       lo[i] = V<T>();
+      const bool isoPart = isoOn && radiusSigned[i] < 0;
+      if (isoPart) ogcPrev[i] = pos[i];
       if (m > 0) {
-        pos[i] = pos[i] + vel[i] * dt + V<T>(0, 0, g) * (w * dt * dt);
+        V<T> guess = pos[i] + vel[i] * dt + V<T>(0, 0, g) * (w * dt * dt);
+        // the predictor stays within the clearance bound of its particle
+        // (40_solver.metal:706-716)
+        if (isoPart && ogcBound[i] < T(1e37)) {
+          T bv = std::max(T(0.45) * std::sqrt(ogcBound[i]), T(0.2) * std::fabs(radiusSigned[i]));
+          V<T> tot = guess - pos[i];
+          T t2 = dotv(tot, tot);
+          if (t2 > bv * bv) guess = pos[i] + tot * (bv / std::sqrt(t2));
+        }
+        pos[i] = guess;
         rot[i] = inertAng[i];
       }
     }
@@ -2124,6 +2231,368 @@ struct Oracle : oracle_base {
     return true;
   }
 
+  // -------------------------------------------------------------------------
+  // isotropic OGC mode (25_cloth.metal:350-940, 40_solver.metal:697-717,
+  // 1361-1382, 1676-1681; order GPUSolver.swift:3686-3756, 4243-4275).
+  // Each surface particle keeps its four closest collision triangles across
+  // frames and each authored edge its four closest edges; per frame a small
+  // candidate pool is re-evaluated -- the tracked set, the triangles adjacent
+  // to the best, the mesh neighbours' best -- and the hashed element grid is
+  // scanned on staggered reseeds.  Every observed vertex-facet / edge-edge
+  // distance outside the 2-ring lowers the squared clearance bound of the
+  // vertices involved; the predictor and every primal step keep a particle
+  // within max(0.45 sqrt(bound), 0.2 r) of the pose the bounds hold around,
+  // and a refresh re-anchors that pose once more than 1 % of the particles
+  // were held back.  Contacts are emitted from the tracked sets.
+  // Deterministic where the reference is not: every tracker reads the
+  // previous frame's tracked sets (the reference reads its neighbours' as
+  // they are being rewritten) and ties in distance keep the lower element id.
+  // -------------------------------------------------------------------------
+  static constexpr double kOgcFar = 3.3961514e38;  // the bounds' clear value (f32 bits 0x7F7F7F7F)
+  struct Best4 {
+    uint32_t id[4];
+    T d2[4];
+  };
+  static void b4Init(Best4& b) {
+    for (int i = 0; i < 4; i++) { b.id[i] = 0xFFFFFFFFu; b.d2[i] = (T)std::numeric_limits<float>::max(); }
+  }
+  static bool b4Has(const Best4& b, uint32_t t) { return b.id[0] == t || b.id[1] == t || b.id[2] == t || b.id[3] == t; }
+  // the kept set is the four smallest (d2, id); a NaN distance never enters
+  static void b4Insert(Best4& b, uint32_t t, T d2) {
+    if (!(d2 < b.d2[3] || (d2 == b.d2[3] && t < b.id[3]))) return;
+    int i = 3;
+    for (; i > 0 && (b.d2[i - 1] > d2 || (b.d2[i - 1] == d2 && b.id[i - 1] > t)); i--) {
+      b.id[i] = b.id[i - 1];
+      b.d2[i] = b.d2[i - 1];
+    }
+    b.id[i] = t;
+    b.d2[i] = d2;
+  }
+  bool inList(const std::vector<uint32_t>& list, uint32_t s, uint32_t n, uint32_t x) const {
+    return std::binary_search(list.begin() + s, list.begin() + s + n, x);
+  }
+  bool in1(uint32_t v, uint32_t x) const { return inList(nbrList, nbrStart[v], nbrCount[v], x); }
+  bool in2(uint32_t v, uint32_t x) const { return inList(nbr2List, nbr2Start[v], nbr2Count[v], x); }
+  void ogcLower(uint32_t v, T d2) {
+    if (d2 < ogcBound[v]) ogcBound[v] = d2;
+  }
+  // hashed element grid at the contact reach: triangles, then authored edges
+  // (entry | 0x80000000), each binned into at most 3 cells per axis
+  std::vector<std::vector<uint32_t>> isoBuckets;
+  std::array<int, 3> isoCell(const V<T>& p) const {
+    const float cs = surfaceContactCellSize;
+    return {(int)std::floor((float)p.x / cs), (int)std::floor((float)p.y / cs), (int)std::floor((float)p.z / cs)};
+  }
+  void isoGrid() {
+    isoBuckets.assign(elemHashSize, {});
+    const float em = elemMargin;
+    for (uint32_t g = 0; g < nTris + numContactEdges; g++) {
+      V<T> lo, hi;
+      T pad;
+      uint32_t entry;
+      if (g < nTris) {
+        const auto& t = tris[g];
+        pad = (T)(std::max(std::fabs((float)radiusSigned[t[0]]),
+                           std::max(std::fabs((float)radiusSigned[t[1]]), std::fabs((float)radiusSigned[t[2]]))) + em);
+        lo = V<T>(std::min(std::min(pos[t[0]].x, pos[t[1]].x), pos[t[2]].x),
+                  std::min(std::min(pos[t[0]].y, pos[t[1]].y), pos[t[2]].y),
+                  std::min(std::min(pos[t[0]].z, pos[t[1]].z), pos[t[2]].z));
+        hi = V<T>(std::max(std::max(pos[t[0]].x, pos[t[1]].x), pos[t[2]].x),
+                  std::max(std::max(pos[t[0]].y, pos[t[1]].y), pos[t[2]].y),
+                  std::max(std::max(pos[t[0]].z, pos[t[1]].z), pos[t[2]].z));
+        entry = g;
+      } else {
+        const auto& e = edges[g - nTris];
+        pad = (T)(std::max(std::fabs((float)radiusSigned[e.first]), std::fabs((float)radiusSigned[e.second])) + em);
+        lo = V<T>(std::min(pos[e.first].x, pos[e.second].x), std::min(pos[e.first].y, pos[e.second].y),
+                  std::min(pos[e.first].z, pos[e.second].z));
+        hi = V<T>(std::max(pos[e.first].x, pos[e.second].x), std::max(pos[e.first].y, pos[e.second].y),
+                  std::max(pos[e.first].z, pos[e.second].z));
+        entry = 0x80000000u | (g - nTris);
+      }
+      lo = lo - V<T>(pad, pad, pad);
+      hi = hi + V<T>(pad, pad, pad);
+      if (!finite3(lo) || !finite3(hi)) continue;  // a non-finite element registers nowhere
+      auto c0 = isoCell(lo), c1 = isoCell(hi);
+      for (int k = 0; k < 3; k++) c1[k] = std::min(c1[k], c0[k] + 2);
+      std::vector<uint32_t> seen;
+      for (int z = c0[2]; z <= c1[2]; z++)
+        for (int y = c0[1]; y <= c1[1]; y++)
+          for (int x = c0[0]; x <= c1[0]; x++) {
+            uint32_t h = cellHash32(x, y, z, 0u, elemHashSize);
+            if (std::find(seen.begin(), seen.end(), h) != seen.end()) continue;
+            seen.push_back(h);
+            isoBuckets[h].push_back(entry);
+          }
+    }
+  }
+  bool isoReseed(uint32_t id, const V<T>& relV, bool missing, bool solid) const {
+    T vth = T(0.25f * surfaceContactCellSize / std::max((float)set.dt, 1e-6f));
+    uint32_t period = dotv(relV, relV) > vth * vth ? 1u : 7u;
+    return ((id + isoFrame) & period) == 0u || (missing && !solid) || (missing && solid && ((id + isoFrame) & 3u) == 0u);
+  }
+  uint32_t isoFrame = 0;
+
+  void isoVertex(uint32_t v) {
+    const V<T> pv = pos[v];
+    const uint32_t gv = clothGroup[v];
+    const bool solidV = gv != 0 && clothVert[v] == 0;
+    const T cap2 = (T)surfaceContactCellSize * (T)surfaceContactCellSize;
+    Best4 best;
+    b4Init(best);
+    T own = (T)kOgcFar;
+    auto consider = [&](uint32_t t) {
+      if (t >= nTris || b4Has(best, t)) return;
+      const auto& tv = tris[t];
+      if (tv[0] == v || tv[1] == v || tv[2] == v) return;
+      if (solidV && clothGroup[tv[0]] == gv && clothGroup[tv[1]] == gv && clothGroup[tv[2]] == gv &&
+          clothVert[tv[0]] == 0 && clothVert[tv[1]] == 0 && clothVert[tv[2]] == 0)
+        return;
+      V<T> bary;
+      V<T> q = closestTri(pv, pos[tv[0]], pos[tv[1]], pos[tv[2]], bary);
+      V<T> dd = pv - q;
+      T d2 = dotv(dd, dd);
+      if (!in1(v, tv[0]) && !in1(v, tv[1]) && !in1(v, tv[2])) b4Insert(best, t, d2);
+      if (d2 < cap2 && !in2(v, tv[0]) && !in2(v, tv[1]) && !in2(v, tv[2])) {
+        own = std::min(own, d2);
+        ogcLower(tv[0], d2);
+        ogcLower(tv[1], d2);
+        ogcLower(tv[2], d2);
+      }
+    };
+    const auto& tracked = vtTrackPrev[v];
+    for (int i = 0; i < 4; i++) consider(tracked[i]);
+    if (tracked[0] < nTris)
+      for (int i = 0; i < 3; i++) consider(triAdj[tracked[0]][i]);
+    for (uint32_t k = 0; k < nbrCount[v] && k < 8; k++) consider(vtTrackPrev[nbrList[nbrStart[v] + k]][0]);
+    V<T> relV = vel[v];
+    if (best.id[0] < nTris) {
+      const auto& bt = tris[best.id[0]];
+      relV = relV - (vel[bt[0]] + vel[bt[1]] + vel[bt[2]]) / T(3);
+    }
+    if (isoReseed(v, relV, best.id[0] == 0xFFFFFFFFu, solidV)) {
+      auto cc = isoCell(pv);
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++)
+            for (uint32_t entry : isoBuckets[cellHash32(cc[0] + dx, cc[1] + dy, cc[2] + dz, 0u, elemHashSize)])
+              if (!(entry & 0x80000000u)) consider(entry);
+    }
+    vtTrack[v] = {best.id[0], best.id[1], best.id[2], best.id[3]};
+    if (own < (T)kOgcFar) ogcLower(v, own);
+    // emission from the tracked set (25_cloth.metal:532-629)
+    const T big = std::numeric_limits<T>::max(), em = (T)elemMargin;
+    const T rv = std::fabs(radiusSigned[v]);
+    for (int bi = 0; bi < 4; bi++) {
+      uint32_t t = best.id[bi];
+      if (t >= nTris) break;
+      const auto& tv = tris[t];
+      V<T> a = pos[tv[0]], b = pos[tv[1]], c = pos[tv[2]];
+      V<T> bary;
+      V<T> q = closestTri(pv, a, b, c, bary);
+      V<T> dd = pv - q;
+      T dist2 = dotv(dd, dd);
+      T rt = std::max(std::fabs(radiusSigned[tv[0]]), std::max(std::fabs(radiusSigned[tv[1]]), std::fabs(radiusSigned[tv[2]])));
+      T hSum = rv + rt;
+      V<T> velT = (vel[tv[0]] + vel[tv[1]] + vel[tv[2]]) / T(3);
+      T inflate = std::min(lenv(vel[v] - velT) * (T)set.dt, T(0.3) * (T)surfaceContactCellSize);
+      T detect = hSum + em + inflate;
+      if (dist2 > detect * detect) continue;
+      T dist = std::sqrt(dist2);
+      V<T> triN = crossv(b - a, c - a);
+      T tnl = lenv(triN);
+      if (tnl < T(1e-12)) continue;
+      triN = triN / tnl;
+      T side = dotv(dd, triN) >= 0 ? T(1) : T(-1);
+      uint32_t keyA = (1u << 28) | v, keyB = t;
+      bool deep = bary.x > T(0.02) && bary.y > T(0.02) && bary.z > T(0.02);
+      T mem = deep ? prevSide(keyA, keyB, side) : side;
+      V<T> n;
+      T g;
+      if (dist > T(1e-7) && side == mem) { n = dd / dist; g = dist; }
+      else { n = triN * mem; g = -dist; }
+      T fricT = friction[tv[0]] * bary.x + friction[tv[1]] * bary.y + friction[tv[2]] * bary.z;
+      T fric = std::sqrt(std::max(friction[v] * fricT, T(0)));
+      T minMass = mass[v] > 0 ? mass[v] : big;
+      for (int j = 0; j < 3; j++) if (mass[tv[j]] > 0) minMass = std::min(minMass, mass[tv[j]]);
+      if (minMass == big) continue;
+      uint32_t ids[4] = {v, tv[0], tv[1], tv[2]};
+      T w[4] = {T(1), -bary.x, -bary.y, -bary.z};
+      Soft r = makeSoft(ids, w, n, fric, V<T>(), g - hSum + em, hSum - em, minMass, 1u, false, false, mem < 0, keyA,
+                        keyB);
+      softCandidates++;
+      if (soft.size() < maxSoft) soft.push_back(r);
+    }
+  }
+
+  void isoEdge(uint32_t e) {
+    const auto eA = edges[e];
+    const V<T> a0 = pos[eA.first], a1 = pos[eA.second];
+    const uint32_t gA = clothGroup[eA.first];
+    const bool solidA = gA != 0 && clothGroup[eA.second] == gA && clothVert[eA.first] == 0 && clothVert[eA.second] == 0;
+    const T cap2 = (T)surfaceContactCellSize * (T)surfaceContactCellSize;
+    Best4 best;
+    b4Init(best);
+    T own = (T)kOgcFar;
+    auto consider = [&](uint32_t o) {
+      if (o >= numContactEdges || o == e || b4Has(best, o)) return;
+      const auto eB = edges[o];
+      if (eB.first == eA.first || eB.first == eA.second || eB.second == eA.first || eB.second == eA.second) return;
+      if (solidA && clothGroup[eB.first] == gA && clothGroup[eB.second] == gA && clothVert[eB.first] == 0 &&
+          clothVert[eB.second] == 0)
+        return;
+      T s_, t_;
+      V<T> b0 = pos[eB.first], b1 = pos[eB.second];
+      segSeg(a0, a1, b0, b1, s_, t_);
+      V<T> dd = (a0 + (a1 - a0) * s_) - (b0 + (b1 - b0) * t_);
+      T d2 = dotv(dd, dd);
+      if (!in1(eA.first, eB.first) && !in1(eA.first, eB.second) && !in1(eA.second, eB.first) &&
+          !in1(eA.second, eB.second))
+        b4Insert(best, o, d2);
+      if (d2 < cap2 && !in2(eA.first, eB.first) && !in2(eA.first, eB.second)) {
+        own = std::min(own, d2);
+        ogcLower(eB.first, d2);
+        ogcLower(eB.second, d2);
+      }
+    };
+    const auto& tracked = eeTrackPrev[e];
+    for (int i = 0; i < 4; i++) consider(tracked[i]);
+    for (int side = 0; side < 2; side++) {
+      uint32_t vv = side == 0 ? eA.first : eA.second;
+      uint32_t cnt = std::min<uint32_t>(vEdgeCount[vv], 6u);
+      for (uint32_t k = 0; k < cnt; k++) {
+        uint32_t ie = vEdgeList[vEdgeStart[vv] + k];
+        if (ie < numContactEdges) consider(eeTrackPrev[ie][0]);
+      }
+    }
+    V<T> velA = (vel[eA.first] + vel[eA.second]) * T(0.5);
+    V<T> relV = velA;
+    if (best.id[0] < numContactEdges) {
+      const auto bt = edges[best.id[0]];
+      relV = relV - (vel[bt.first] + vel[bt.second]) * T(0.5);
+    }
+    if (isoReseed(e, relV, best.id[0] == 0xFFFFFFFFu, solidA)) {
+      auto cc = isoCell((a0 + a1) * T(0.5));
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++)
+            for (uint32_t entry : isoBuckets[cellHash32(cc[0] + dx, cc[1] + dy, cc[2] + dz, 0u, elemHashSize)])
+              if (entry & 0x80000000u) consider(entry & ~0x80000000u);
+    }
+    eeTrack[e] = {best.id[0], best.id[1], best.id[2], best.id[3]};
+    if (own < (T)kOgcFar) {
+      ogcLower(eA.first, own);
+      ogcLower(eA.second, own);
+    }
+    // emission, each pair once by its lower edge (25_cloth.metal:790-853)
+    const T big = std::numeric_limits<T>::max(), em = (T)elemMargin;
+    const T rA = std::max(std::fabs(radiusSigned[eA.first]), std::fabs(radiusSigned[eA.second]));
+    for (int bi = 0; bi < 4; bi++) {
+      uint32_t o = best.id[bi];
+      if (o >= numContactEdges) break;
+      if (o <= e) continue;
+      const auto eB = edges[o];
+      V<T> b0 = pos[eB.first], b1 = pos[eB.second];
+      T s_, t_;
+      segSeg(a0, a1, b0, b1, s_, t_);
+      if (s_ < T(0.02) || s_ > T(1) - T(0.02) || t_ < T(0.02) || t_ > T(1) - T(0.02)) continue;
+      V<T> dd = (a0 + (a1 - a0) * s_) - (b0 + (b1 - b0) * t_);
+      T dist = lenv(dd);
+      T rB = std::max(std::fabs(radiusSigned[eB.first]), std::fabs(radiusSigned[eB.second]));
+      T hSum = rA + rB;
+      V<T> velB = (vel[eB.first] + vel[eB.second]) * T(0.5);
+      T inflate = std::min(lenv(velA - velB) * (T)set.dt, T(0.3) * (T)surfaceContactCellSize);
+      if (dist - hSum > em + inflate) continue;
+      if (dist < T(1e-7)) continue;
+      V<T> nGeo = dd / dist;
+      uint32_t keyA = (3u << 28) | e, keyB = o;
+      V<T> n = nGeo;
+      T g = dist;
+      int prev = softFind(keyA, keyB);
+      if (prev >= 0 && dotv(nGeo, prevSoft[prev].n) < 0) { n = -nGeo; g = -dist; }
+      T fric = std::sqrt(std::max((friction[eA.first] + friction[eA.second]) * T(0.5) *
+                                  (friction[eB.first] + friction[eB.second]) * T(0.5), T(0)));
+      T minMass = big;
+      uint32_t ids[4] = {eA.first, eA.second, eB.first, eB.second};
+      for (int j = 0; j < 4; j++) if (mass[ids[j]] > 0) minMass = std::min(minMass, mass[ids[j]]);
+      if (minMass == big) continue;
+      T w[4] = {T(1) - s_, s_, -(T(1) - t_), -t_};
+      Soft r = makeSoft(ids, w, n, fric, V<T>(), g - hSum + em, hSum - em, minMass, 3u, false, false, false, keyA, keyB);
+      softCandidates++;
+      if (soft.size() < maxSoft) soft.push_back(r);
+    }
+  }
+
+  void isoEmit() {
+    isoFrame = (uint32_t)(frame - 1);  // the frame index before this step
+    ogcBound.assign(nb, (T)kOgcFar);
+    ogcExceeded = 0;
+    ogcTruncFrame = ogcRefreshFrame = 0;
+    std::swap(vtTrack, vtTrackPrev);
+    std::swap(eeTrack, eeTrackPrev);
+    isoGrid();
+    for (uint32_t v : particles) isoVertex(v);
+    for (uint32_t e = 0; e < numContactEdges; e++) isoEdge(e);
+  }
+  // the displacement cap (40_solver.metal:1361-1382): returns the kept dx
+  V<T> ogcTruncate(uint32_t body, const V<T>& cur, V<T> dx) {
+    T d2 = ogcBound[body];
+    if (!(d2 < T(1e37))) return dx;
+    T bv = std::max(T(0.45) * std::sqrt(d2), T(0.2) * std::fabs(radiusSigned[body]));
+    V<T> tot = (cur + dx) - ogcPrev[body];
+    T t2 = dotv(tot, tot);
+    if (t2 > bv * bv) {
+      tot = tot * (bv / std::sqrt(t2));
+      dx = (ogcPrev[body] + tot) - cur;
+#pragma omp atomic
+      ogcExceeded++;
+#pragma omp atomic
+      ogcTruncFrame++;
+    }
+    return dx;
+  }
+  // Alg 3 refresh (25_cloth.metal:857-939): when more than 1 % of the
+  // particles were held back, bounds are recomputed from the tracked sets at
+  // the live pose, which becomes the new anchor
+  void ogcRefresh() {
+    const bool fire = ogcExceeded * 100u > particles.size();
+    ogcExceeded = 0;
+    if (!fire) return;
+    ogcRefreshFrame++;
+    for (uint32_t v : particles) {
+      V<T> pv = pos[v];
+      T dmin = T(3.0e38);
+      const auto& tr = vtTrack[v];
+      for (int i = 0; i < 4; i++) {
+        uint32_t t = tr[i];
+        if (t >= nTris) break;
+        const auto& tv = tris[t];
+        if (in2(v, tv[0]) || in2(v, tv[1]) || in2(v, tv[2])) continue;
+        V<T> bary;
+        V<T> q = closestTri(pv, pos[tv[0]], pos[tv[1]], pos[tv[2]], bary);
+        V<T> dd = pv - q;
+        dmin = std::min(dmin, dotv(dd, dd));
+      }
+      uint32_t cnt = std::min<uint32_t>(vEdgeCount[v], 6u);
+      for (uint32_t k = 0; k < cnt; k++) {
+        uint32_t me = vEdgeList[vEdgeStart[v] + k];
+        if (me >= numContactEdges) continue;
+        uint32_t other = eeTrack[me][0];
+        if (other >= numContactEdges) continue;
+        const auto eo = edges[other];
+        if (in2(v, eo.first) || in2(v, eo.second)) continue;
+        const auto em_ = edges[me];
+        T s_, t_;
+        V<T> a0 = pos[em_.first], a1 = pos[em_.second], b0 = pos[eo.first], b1 = pos[eo.second];
+        segSeg(a0, a1, b0, b1, s_, t_);
+        V<T> dd = (a0 + (a1 - a0) * s_) - (b0 + (b1 - b0) * t_);
+        dmin = std::min(dmin, dotv(dd, dd));
+      }
+      ogcBound[v] = dmin;
+      ogcPrev[v] = pos[v];
+    }
+  }
+
   void datContacts() {
     for (const auto& pr : datPairs) {
       Soft r;
@@ -2489,6 +2958,7 @@ struct Oracle : oracle_base {
     if (l2 > maxLin * maxLin) dl = dl * (maxLin / std::sqrt(l2));
     T a2 = dotv(da, da);
     if (a2 > T(0.25)) da = da * (T(0.5) / std::sqrt(a2));
+    if (part && isoOn) dl = ogcTruncate(body, pos[body], dl);
     // (pos, lo) + dl renormalised exactly (TwoSum per component)
     V<T> t = lo[body] + dl;
     for (int k = 0; k < 3; k++) {
@@ -2662,6 +3132,7 @@ struct Oracle : oracle_base {
       if (datOn && n > 0) datColorPass(c);
     }
     dual();
+    if (isoOn) ogcRefresh();
   }
   void stepToResidual(double tol, int maxIt, int every) {
     tolerance = tol;
@@ -2696,6 +3167,7 @@ struct Oracle : oracle_base {
     if (nTris) {
       if (datOn) datBuild();
       rtEmit();
+      if (isoOn) isoEmit();
     }
     warmstartConstraints();
     warmstartBodies();
@@ -2990,6 +3462,25 @@ int oracle_get_manifold_keys(oracle_solver* o, uint32_t cap, uint32_t* n, uint32
   if (o->f) go(o->f); else go(o->d);
   return 0;
 }
+int oracle_get_iso_state(oracle_solver* o, uint32_t* vt_track, uint32_t* ee_track, float* bound, float* anchor,
+                         uint32_t* counts) {
+  auto go = [&](auto* s) {
+    if (!s->isoOn) return -1;
+    for (uint32_t i = 0; i < s->nb; i++) {
+      for (int k = 0; k < 4 && vt_track; k++) vt_track[4 * i + k] = s->vtTrack[i][k];
+      if (bound) bound[i] = (float)s->ogcBound[i];
+      for (int k = 0; k < 3 && anchor; k++) anchor[3 * i + k] = (float)s->ogcPrev[i][k];
+    }
+    for (uint32_t e = 0; e < s->numContactEdges && ee_track; e++)
+      for (int k = 0; k < 4; k++) ee_track[4 * e + k] = s->eeTrack[e][k];
+    if (counts) {
+      counts[0] = s->ogcTruncFrame;
+      counts[1] = s->ogcRefreshFrame;
+    }
+    return 0;
+  };
+  return o->f ? go(o->f) : go(o->d);
+}
 int oracle_get_motor_targets(oracle_solver* o, uint32_t n, float* targets) {
   auto go = [&](auto* s) {
     for (uint32_t i = 0; i < n && i < s->joints.size(); i++) targets[i] = (float)s->joints[i].motorTarget;
@@ -3027,6 +3518,7 @@ int oracle_get_cloth_info(oracle_solver* o, avbd_cloth_info* info, uint32_t* edg
       info->num_contact_edges = s->numContactEdges;
       info->pair_stream = s->pairStream ? 1u : 0u;
       info->dat_on = s->datOn ? 1u : 0u;
+      info->iso_on = s->isoOn ? 1u : 0u;
     }
     if (edges)
       for (size_t i = 0; i < s->edges.size(); i++) { edges[2 * i] = s->edges[i].first; edges[2 * i + 1] = s->edges[i].second; }

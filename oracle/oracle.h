@@ -38,6 +38,12 @@ int oracle_get_cell_lists(oracle_solver* o, uint32_t* hash_size, uint32_t* num_h
 int oracle_get_pairs(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* pairs);
 /* manifold summary of the last frame: per pair (bodyA, bodyB, numContacts) and contact normals */
 int oracle_get_manifold_keys(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* keys8);
+/* isotropic limiter state after the last frame: tracked triangles [4 nb],
+ * tracked edges [4 num_contact_edges], squared clearance bounds [nb], anchor
+ * poses [3 nb], the frame's truncations and refreshes [2]; -1 when the mode
+ * is off */
+int oracle_get_iso_state(oracle_solver* o, uint32_t* vt_track, uint32_t* ee_track, float* bound, float* anchor,
+                         uint32_t* counts);
 int oracle_get_manifolds(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* header3, float* normal3,
                          float* lambda_pen);
 int oracle_residual(oracle_solver* o, double* out);
@@ -74,6 +80,17 @@ void oracle_membrane(const double* x9, const double* dm4, double muA, double lam
                      double* grad3, double* hess9);
 int oracle_hinge(const float* x12, float k1, float k2, float* K4, float* s);
 uint32_t oracle_cell_hash(float px, float py, float pz, float cell_size, uint32_t group, uint32_t hash_size);
+
+/* The reference's CPU AVBD path (CPUSolver semantics, cpu_path.cpp): rigid
+ * boxes / spheres, joints and springs; returns 4 for scenes outside it. */
+typedef struct oracle_cpu_solver oracle_cpu_solver;
+int oracle_cpu_create(const avbd_scene_desc* scene, int precision, oracle_cpu_solver** out);
+void oracle_cpu_destroy(oracle_cpu_solver* o);
+int oracle_cpu_step(oracle_cpu_solver* o, int n);
+int oracle_cpu_get_body_states_f64(oracle_cpu_solver* o, uint32_t n, double* pos3, double* rot4, double* vel3,
+                                   double* angvel3);
+int oracle_cpu_num_manifolds(oracle_cpu_solver* o);
+double oracle_cpu_max_constraint_error(oracle_cpu_solver* o);
 
 #ifdef __cplusplus
 }

@@ -642,3 +642,18 @@ def legacy_rate_motor(dt=0.25, rate=2.0):
     s.addJoint(SceneJoint(-1, b, (0, 0, 0), (0, 0, 0), stiffnessLin=math.inf, stiffnessAng=math.inf,
                           hingeAxis=(0, 0, 1), motorTorque=10.0, motorStiffness=400.0, motorRate=rate))
     return s
+
+
+def sheets_and_block(n=7, h=0.06):
+    """Two n x n cloth sheets, the upper turned a quarter and dropped on the
+    lower, and a small tet block beside them: the tets select the isotropic
+    limiter (GPUSolver.swift:93-99), the sheets give it authored V-T and E-E
+    pairs (a test scene of this backend, built from the Demos helpers)."""
+    s = PhysicsScene("sheets+block", SimSettings(iterations=8))
+    _ground(s)
+    lower = [[(x * h - 0.18, y * h - 0.18, 0.3) for y in range(n)] for x in range(n)]
+    upper = [[(y * h - 0.17, x * h - 0.19, 0.36) for y in range(n)] for x in range(n)]
+    add_cloth_grid(s, lower, 0.008, 0.004)
+    add_cloth_grid(s, upper, 0.008, 0.004)
+    add_soft_lattice(s, (1.0, 0, 0.4), 2, 2, 2, 0.2)
+    return s

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     assert C.sizeof(_capi.Settings) == 15 * 4
     assert C.sizeof(_capi.Options) == 7 * 4
-    assert C.sizeof(_capi.ClothInfo) == 9 * 4 + 4 * 4 + 3 * 4
+    assert C.sizeof(_capi.ClothInfo) == 9 * 4 + 4 * 4 + 4 * 4
     # int64 frame + 7 counters + 64 color counts + 4 + 4 cloth counters + 3
     # learned-start fields + 5 truncation counters, tail-padded to 8 bytes
     assert C.sizeof(_capi.Stats) == 8 + 4 * (7 + 64 + 4 + 4 + 1 + 2 + 5) + 4

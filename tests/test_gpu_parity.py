@@ -171,7 +171,10 @@ def test_warm_started_one_iteration_frames(name):
     act = h[:, 2] > 0
     assert act.sum() > 0
     lam_g, pen_g = gm["c"]["lambda"][act, 0, 0], gm["c"]["penalty"][act, 0, 0]
-    np.testing.assert_allclose(lam_g, lp[act, 0], rtol=1e-3, atol=1e-6)
+    # a multiplier is penalty x constraint value, so small ones carry the
+    # absolute error of the largest penalty times the 1e-5 m agreement of the
+    # positions: 1e-3 relative plus 1e-5 of the largest multiplier
+    np.testing.assert_allclose(lam_g, lp[act, 0], rtol=1e-3, atol=1e-5 * float(np.abs(lp[act, 0]).max()))
     np.testing.assert_allclose(pen_g, lp[act, 1], rtol=1e-3)
     flush_print(f"  40 frames, worst max|x_gpu - x_oracle| = {worst:.3e} m, {int(act.sum())} manifolds compared")
 

@@ -4,7 +4,7 @@ against the closed form (6 (n-1)^2 squares, two triangles each), a closed
 2-manifold (every edge on two faces, V - E + F = 2), outward winding on a
 convex block, ascending order; the surface groups, self-collision, pair
 stream and limiter selection (GPUSolver.swift:93-99) of one block, two
-blocks and a block beside a cloth sheet; and the soft-soft contacts that
+blocks and a block beside a cloth sheet; and the tracked V-T contacts that
 keep a dropped block on another."""
 import numpy as np
 
@@ -74,7 +74,7 @@ def test_surface_groups_and_limiter_selection():
     o = oracle.OracleSolver(s)
     info = o.clothInfo()
     assert info["num_tris"] == 108 + 48 and info["num_contact_edges"] == 0
-    assert info["pair_stream"] == 1 and info["dat_on"] == 0        # closed tet surfaces: no Planar-DAT
+    assert (info["pair_stream"], info["dat_on"], info["iso_on"]) == (0, 0, 1)   # closed tet surfaces: isotropic
     # a cloth sheet beside a block: its triangles first, its edges the contact prefix
     s = PhysicsScene("sheet+block", SimSettings(iterations=5))
     R._ground(s)
@@ -90,22 +90,21 @@ def test_surface_groups_and_limiter_selection():
     tris = o.collisionTris()
     assert info["num_tris"] == 8 + 12 and info["num_contact_edges"] == 16 and info["num_membranes"] == 8
     assert np.array_equal(tris[:8], np.array([t[0] for t in s.tris], np.uint32))
-    assert info["pair_stream"] == 1 and info["dat_on"] == 1        # an authored, self-colliding sheet
+    assert (info["pair_stream"], info["dat_on"], info["iso_on"]) == (0, 0, 1)   # tets select the isotropic mode
 
 
 def test_dropped_block_rests_on_block_through_vt_contacts():
     s, top = R.stacked_soft_blocks()
     o = oracle.OracleSolver(s)
     o.step(120)
-    pairs = o.datPairs()
-    assert len(pairs) > 0 and set(pairs[:, 0].tolist()) <= {0, 1}
-    # every accepted pair joins the two blocks (no same-block pair on a
-    # non-self-colliding surface)
-    tris = o.collisionTris()
+    sc = o.softContacts()
+    vt = sc[(sc["lambda"][:, 3].view(np.uint32) >> 28) == 1]
+    assert len(vt) > 0
+    # every tracked V-T contact joins the two blocks (no same-block pair on
+    # a non-self-colliding surface)
     top = set(top)
-    for kind, a, b in pairs.tolist():
-        if kind == 0:
-            assert (a in top) != (int(tris[b][0]) in top)
+    for ids in vt["ids"].tolist():
+        assert (ids[0] in top) != (ids[1] in top)
     o.step(280)
     p = o.positions()[sorted(top)]
     assert p[:, 2].min() > 0.95 and p[:, 2].mean() > 1.15
