@@ -324,6 +324,15 @@ class GPUSolver:
                     "soft contacts")
         return out[:n.value]
 
+    def createProfile(self):
+        """[(phase, ms)] of this solver's creation, in order."""
+        out, i = [], 0
+        name, ms = C.c_char_p(), C.c_float()
+        while self._lib.avbd_get_create_profile(self._h, i, C.byref(name), C.byref(ms)) == 0:
+            out.append((name.value.decode(), ms.value))
+            i += 1
+        return out
+
     def isoState(self):
         """Isotropic limiter state after the last frame: tracked triangles
         (nb, 4), tracked edges (num_contact_edges, 4), squared clearance

@@ -21,6 +21,18 @@ void launch_map_rebuild(hipStream_t st, const DeviceState& S, const StepParams& 
 void launch_map_find(hipStream_t st, const DeviceState& S, const StepParams& P, const U2* q, int32_t* out,
                      uint32_t n);
 
+// upload.hip: boundary faces of the tet meshes, wound outward, ascending;
+// false when the scene is outside the device path (2^21 or more bodies) or
+// a HIP call fails, and the caller takes the host path
+bool gpu_tet_boundary_faces(const uint32_t* tetIds, uint32_t nt, const float* bodyPos, uint32_t nb, hipStream_t st,
+                            std::vector<std::array<uint32_t, 3>>& out);
+
+// upload.hip: per-tet rest shape into tetsDev and the slot of every tet entry
+// of the (already uploaded) static adjacency into slotsDev
+hipError_t gpu_tet_precompute(const uint32_t* tetIds, const float* mu, const float* lam, uint32_t nt,
+                              const float* bodyPos, uint32_t nb, const uint32_t* sStartDev, const uint32_t* sCountDev,
+                              const uint32_t* sListDev, TetRec* tetsDev, TetSlot* slotsDev, hipStream_t st);
+
 // cloth.hip (no-ops without cloth triangles)
 // rigid feature vs triangle contacts at the step-start pose: the prefix of this frame's soft records
 void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);

@@ -12,6 +12,7 @@
 #include <array>
 #include <functional>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <set>
 #include <string>
@@ -68,6 +69,9 @@ struct avbd_solver {
   bool datOn = false;            // Planar-DAT limiter passes (a self-colliding surface exists)
   bool isoOn = false;            // isotropic OGC limiter and its tracked V-T / E-E contacts
   uint32_t lastOgcTrunc = 0, lastOgcRefresh = 0;  // the last retired frame's truncations / refreshes
+  // wall time of the create phases (host clock, device work synchronised)
+  std::vector<std::pair<const char*, double>> createPhases;
+  std::chrono::steady_clock::time_point phaseT0;
   std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
   std::vector<float> rateValues;
   F4* jointPenInit = nullptr;      // upload-time joint penalties (lin [nj], ang [nj]), restored on reset
@@ -583,6 +587,7 @@ avbd_status validate(const avbd_scene_desc* d) {
   }
   for (uint32_t i = 0; i < d->num_springs; i++)
     if (d->spring_body_a[i] >= d->num_bodies || d->spring_body_b[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  if (d->num_tets > 0 && (!d->tet_ids || !d->tet_mu || !d->tet_lambda)) return AVBD_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < d->num_tets * 4; i++)
     if (d->tet_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
   if (d->num_tris > 0 && !d->tri_ids) return AVBD_ERR_INVALID_ARGUMENT;
@@ -739,13 +744,13 @@ void build_iso_topology(ClothHost& c, uint32_t nb, uint32_t na) {
 // (GPUSolver.swift:1289-1334): membranes and hinges come from the authored
 // triangles only, E-E contacts from their edges only (the tet-boundary edges
 // follow them in the edge list and feed the Planar-DAT pair stream alone).
-void build_cloth(const avbd_scene_desc* d, ClothHost& c) {
+void build_cloth(const avbd_scene_desc* d, ClothHost& c, const std::vector<std::array<uint32_t, 3>>& boundary) {
   const uint32_t nb = d->num_bodies, na = d->num_tris;
   auto isParticle = [&](uint32_t i) { return d->body_is_particle && d->body_is_particle[i]; };
   auto sizeX = [&](uint32_t i) { return d->body_size[3 * i]; };
   std::vector<std::array<uint32_t, 3>> coll(na);
   for (uint32_t t = 0; t < na; t++) coll[t] = {d->tri_ids[3 * t], d->tri_ids[3 * t + 1], d->tri_ids[3 * t + 2]};
-  for (const auto& f : tet_boundary_faces(d)) coll.push_back(f);
+  for (const auto& f : boundary) coll.push_back(f);
   const uint32_t nr = (uint32_t)coll.size();
   c.tris.resize(nr);
   std::set<uint64_t> seen;
@@ -998,6 +1003,15 @@ const char* avbd_version(void) { return "avbd-mi355x 0.1 (gfx950, wave64)"; }
 namespace {
 
 avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d);
+
+// closes the running create phase under `name` (after the stream drains, so
+// device work is charged to the phase that enqueued it)
+void create_phase(avbd_solver* s, const char* name) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  const auto t = std::chrono::steady_clock::now();
+  s->createPhases.emplace_back(name, std::chrono::duration<double, std::milli>(t - s->phaseT0).count());
+  s->phaseT0 = t;
+}
 avbd_status static_coloring(avbd_solver* s);
 void autotune(avbd_solver* s);
 void release_solver(avbd_solver* s);
@@ -1041,6 +1055,7 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AVBD_ERR_NO_DEVICE;
   avbd_solver* s = new avbd_solver();
+  s->phaseT0 = std::chrono::steady_clock::now();
   hipGetDevice(&s->device);
   if (resolve_fixed_scratch(s->device, &s->fixedScratch) != hipSuccess) {
     (void)hipGetLastError();
@@ -1055,6 +1070,7 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
     delete s;
     return AVBD_ERR_NO_DEVICE;
   }
+  create_phase(s, "stream");
   avbd_status st = build_static_and_upload(s, d);
   if (st != AVBD_OK) {
     release_solver(s);
@@ -1063,6 +1079,7 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
   *out = s;
   live_add(s, HandleKind::Solver);
   st = static_coloring(s);
+  create_phase(s, "static coloring");
   if (st == AVBD_ERR_RUNTIME_FAILURE) return st;  // failure latched; caller reads it then destroys
   if (st != AVBD_OK) {
     avbd_destroy(s);
@@ -1070,6 +1087,7 @@ extern "C" avbd_status avbd_create(const avbd_scene_desc* d, const avbd_options*
     return st;
   }
   autotune(s);
+  create_phase(s, "autotune");
   s->stats.lanes_per_body = s->lanes;
   s->stats.sweep_mode = s->sweepMode;
   s->stats.uses_dynamic_coloring = s->dynamicColoring ? 1 : 0;
@@ -1168,6 +1186,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->inertia[3 * i + 2] = mom.z;
   }
 
+  create_phase(s, "bodies");
   // ---- colliders (GPUSolver.swift:971-1090)
   const uint32_t nc = d->num_colliders;
   std::vector<uint32_t> co(nc), ct(nc), cg(nc), csc(nc);
@@ -1255,6 +1274,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   s->mapCapacity = next_pow2(2 * s->maxPairs);
   s->gridHashSize = next_pow2(std::max<uint32_t>(64u, 2u * std::max<uint32_t>(1u, enabledCount)));
 
+  create_phase(s, "colliders");
   // ---- joints (GPUSolver.swift:1092-1193)
   const uint32_t nj = d->num_joints;
   std::vector<JointRec> jr(nj);
@@ -1327,6 +1347,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     penInit[nj + i] = g.penaltyAng;
   }
 
+  create_phase(s, "joints");
   // ---- springs (GPUSolver.swift:1195-1212)
   const uint32_t ns = d->num_springs;
   std::vector<SpringRec> sr(ns);
@@ -1348,37 +1369,17 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->springEndsHost.push_back(U2{a, b});
   }
 
-  // ---- tets (GPUSolver.swift:1214-1236): Dm^-1 rows, 6V, mu V, lambda V
+  create_phase(s, "springs");
+  // ---- tets (GPUSolver.swift:1214-1236): the rest shapes (Dm^-1 rows, 6V,
+  // mu V, lambda V) are computed on the device once the adjacency is there
   const uint32_t nt = d->num_tets;
-  std::vector<TetRec> tr(nt);
+  s->tetIdsHost.resize(nt);
   for (uint32_t t = 0; t < nt; t++) {
     const uint32_t* id = d->tet_ids + 4 * t;
-    double x0[3], D[3][3];  // D columns d0 d1 d2
-    for (int k = 0; k < 3; k++) x0[k] = d->body_position[3 * id[0] + k];
-    for (int c = 0; c < 3; c++)
-      for (int k = 0; k < 3; k++) D[k][c] = (double)d->body_position[3 * id[c + 1] + k] - x0[k];
-    double det = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) - D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
-                 D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
-    double inv[3][3];
-    inv[0][0] = (D[1][1] * D[2][2] - D[1][2] * D[2][1]) / det;
-    inv[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) / det;
-    inv[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) / det;
-    inv[1][0] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) / det;
-    inv[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det;
-    inv[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det;
-    inv[2][0] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) / det;
-    inv[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det;
-    inv[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det;
-    double vol = fabs(det) / 6.0;
-    TetRec g;
-    g.ids = U4{id[0], id[1], id[2], id[3]};
-    s->tetIdsHost.push_back(g.ids);
-    g.r0 = F4{(float)inv[0][0], (float)inv[0][1], (float)inv[0][2], (float)det};
-    g.r1 = F4{(float)inv[1][0], (float)inv[1][1], (float)inv[1][2], (float)(d->tet_mu[t] * vol)};
-    g.r2 = F4{(float)inv[2][0], (float)inv[2][1], (float)inv[2][2], (float)(d->tet_lambda[t] * vol)};
-    tr[t] = g;
+    s->tetIdsHost[t] = U4{id[0], id[1], id[2], id[3]};
   }
 
+  create_phase(s, "tets");
   // ---- exclusions (GPUSolver.swift:1265-1288): sorted unique (lo, hi)
   std::vector<uint64_t> ex;
   for (uint32_t i = 0; i < nj; i++) {
@@ -1401,6 +1402,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   std::vector<U2> exr(ex.size());
   for (size_t i = 0; i < ex.size(); i++) exr[i] = U2{(uint32_t)(ex[i] >> 32), (uint32_t)(ex[i] & 0xFFFFFFFFu)};
   s->nExcl = (uint32_t)ex.size();
+  create_phase(s, "exclusions");
 
   // ---- collision surfaces: authored triangles and tet boundary faces; the
   // particles of one connected surface share a soft collision group
@@ -1408,7 +1410,13 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   std::vector<uint32_t> clothGroup(nb, 0);
   ClothHost ch;
   if (d->num_tris > 0 || nt > 0) {
-    build_cloth(d, ch);
+    std::vector<std::array<uint32_t, 3>> boundary;
+    if (nt && !gpu_tet_boundary_faces(d->tet_ids, nt, d->body_position, nb, s->stream, boundary)) {
+      (void)hipGetLastError();
+      boundary = tet_boundary_faces(d);
+    }
+    create_phase(s, "tet boundary faces");
+    build_cloth(d, ch, boundary);
     clothGroup = ch.group;
     s->nTris = (uint32_t)ch.tris.size();
     s->numContactEdges = ch.numContactEdges;
@@ -1434,76 +1442,49 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     s->surfaceContactCellSize = ch.surfaceContactCellSize;
   }
 
-  // ---- static topology CSR (sorted by entry: joints, springs, tets)
-  std::vector<std::vector<uint32_t>> adj(nb);
+  create_phase(s, "collision surfaces");
+  // ---- static topology CSR (per body: joints, springs, tets, membranes,
+  // bends, each ascending), counted then filled in that order
   auto dyn = [&](uint32_t b) { return b != kWorld && pl[b].w > 0.0f; };
-  for (uint32_t i = 0; i < nj; i++) {
-    const JointRec& g = jr[i];
-    if (g.rA.w == 0.0f && g.rB.w == 0.0f && g.motor.y == 0.0f) continue;  // inert slot
-    uint32_t e = (FK_JOINT << kAdjShift) | i;
-    if (dyn(g.header.x)) adj[g.header.x].push_back(e);
-    if (dyn(g.header.y)) adj[g.header.y].push_back(e);
-  }
-  // particle-particle springs without offsets take a stamp-major slot
   auto zero3 = [](F4 v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; };
-  for (uint32_t i = 0; i < ns; i++) {
-    bool slot = s->isParticle[sr[i].header.x] && s->isParticle[sr[i].header.y] && zero3(sr[i].rA) && zero3(sr[i].rB);
-    uint32_t e = ((slot ? FK_PSPRING : FK_SPRING) << kAdjShift) | i;
-    if (dyn(sr[i].header.x)) adj[sr[i].header.x].push_back(e);
-    if (dyn(sr[i].header.y)) adj[sr[i].header.y].push_back(e);
-  }
-  for (uint32_t t = 0; t < nt; t++) {
-    uint32_t e = (FK_TET << kAdjShift) | t;
-    const uint32_t* id = d->tet_ids + 4 * t;
-    for (int k = 0; k < 4; k++)
-      if (dyn(id[k])) adj[id[k]].push_back(e);
-  }
-  for (uint32_t m = 0; m < ch.membranes.size(); m++)
-    for (int k = 0; k < 3; k++)
-      if (dyn(ch.membranes[m].ids[k])) adj[ch.membranes[m].ids[k]].push_back((FK_MEMBRANE << kAdjShift) | m);
-  for (uint32_t m = 0; m < ch.bends.size(); m++)
-    for (int k = 0; k < 4; k++)
-      if (dyn(ch.bends[m].ids[k])) adj[ch.bends[m].ids[k]].push_back((FK_BEND << kAdjShift) | m);
-  std::vector<uint32_t> sStart(nb), sCount(nb), sList;
-  for (uint32_t b = 0; b < nb; b++) {
-    sStart[b] = (uint32_t)sList.size();
-    sCount[b] = (uint32_t)adj[b].size();
-    sList.insert(sList.end(), adj[b].begin(), adj[b].end());
+  std::vector<uint32_t> sStart(nb), sCount(nb, 0), sList;
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<uint32_t> cur;
+    if (pass == 1) {
+      uint32_t run = 0;
+      for (uint32_t b = 0; b < nb; b++) { sStart[b] = run; run += sCount[b]; }
+      sList.assign(run, 0u);
+      cur = sStart;
+    }
+    auto add = [&](uint32_t b, uint32_t e) {
+      if (!dyn(b)) return;
+      if (pass == 0) sCount[b]++;
+      else sList[cur[b]++] = e;
+    };
+    for (uint32_t i = 0; i < nj; i++) {
+      const JointRec& g = jr[i];
+      if (g.rA.w == 0.0f && g.rB.w == 0.0f && g.motor.y == 0.0f) continue;  // inert slot
+      add(g.header.x, (FK_JOINT << kAdjShift) | i);
+      add(g.header.y, (FK_JOINT << kAdjShift) | i);
+    }
+    // particle-particle springs without offsets take a stamp-major slot
+    for (uint32_t i = 0; i < ns; i++) {
+      bool slot = s->isParticle[sr[i].header.x] && s->isParticle[sr[i].header.y] && zero3(sr[i].rA) && zero3(sr[i].rB);
+      uint32_t e = ((slot ? FK_PSPRING : FK_SPRING) << kAdjShift) | i;
+      add(sr[i].header.x, e);
+      add(sr[i].header.y, e);
+    }
+    for (uint32_t t = 0; t < nt; t++) {
+      const uint32_t* id = d->tet_ids + 4 * t;
+      for (int k = 0; k < 4; k++) add(id[k], (FK_TET << kAdjShift) | t);
+    }
+    for (uint32_t m = 0; m < ch.membranes.size(); m++)
+      for (int k = 0; k < 3; k++) add(ch.membranes[m].ids[k], (FK_MEMBRANE << kAdjShift) | m);
+    for (uint32_t m = 0; m < ch.bends.size(); m++)
+      for (int k = 0; k < 4; k++) add(ch.bends[m].ids[k], (FK_BEND << kAdjShift) | m);
   }
   s->sAdjCount = sCount;
   s->sAdjListHost = sList;
-  // stamp-major tet slots at each tet entry's CSR position: the other three
-  // vertices and their shape-function gradients b (rows of Dm^-1; vertex 0
-  // takes minus their sum)
-  std::vector<TetSlot> slots(std::max<size_t>(1, sList.size()));
-  memset(slots.data(), 0, slots.size() * sizeof(TetSlot));
-  for (uint32_t b = 0; b < nb; b++) {
-    for (uint32_t k = 0; k < sCount[b]; k++) {
-      uint32_t e = sList[sStart[b] + k];
-      if ((e >> kAdjShift) != FK_TET) continue;
-      uint32_t t = e & kAdjIndexMask;
-      const TetRec& g = tr[t];
-      uint32_t ids[4] = {g.ids.x, g.ids.y, g.ids.z, g.ids.w};
-      V3 grad[4];
-      grad[1] = xyz(g.r0);
-      grad[2] = xyz(g.r1);
-      grad[3] = xyz(g.r2);
-      grad[0] = -(grad[1] + grad[2] + grad[3]);
-      TetSlot sl;
-      uint32_t o[3];
-      V3 ob[3];
-      int n = 0;
-      for (int q = 0; q < 4; q++)
-        if (ids[q] != b && n < 3) { o[n] = ids[q]; ob[n] = grad[q]; n++; }
-      float muV = g.r1.w, lamV = g.r2.w;
-      sl.other = U4{o[0], o[1], o[2], t};
-      sl.b1 = f4(ob[0], muV);
-      sl.b2 = f4(ob[1], lamV);
-      sl.b3 = f4(ob[2], 1.0f + muV / fmaxf(lamV, 1e-9f));
-      slots[sStart[b] + k] = sl;
-    }
-  }
-
   // cloth slots at each membrane / bend / particle-spring entry's CSR
   // position (same-color partner bits are added once the palette is known)
   bool anySlot = false;
@@ -1553,6 +1534,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   s->clothSlotsHost = cslots;
   s->sAdjStartHost = sStart;
 
+  create_phase(s, "static CSR and slots");
   // ---- device buffers
   DeviceState& S = s->S;
   hipError_t e = hipSuccess;
@@ -1588,7 +1570,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(s->jointPenInit, 2 * (size_t)std::max<uint32_t>(1, nj))
   ALLOC(S.springs, std::max<uint32_t>(1, ns))
   ALLOC(S.sAdjStart, nb) ALLOC(S.sAdjCount, nb) ALLOC(S.sAdjList, std::max<size_t>(1, sList.size()))
-  ALLOC(S.tetSlots, slots.size())
+  ALLOC(S.tetSlots, std::max<size_t>(1, sList.size()))
   ALLOC(S.dAdjCount, nb) ALLOC(S.dAdjStart, nb) ALLOC(S.dAdjCursor, nb)
   S.dAdjCapacity = 2 * s->maxPairs + 4 * s->maxSoft;
   ALLOC(S.dAdjList, S.dAdjCapacity)
@@ -1640,6 +1622,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   s->devScratchWords = std::max<uint32_t>(2 * s->maxPairs, 64);
   ALLOC(s->devScratchU32, s->devScratchWords)
 #undef ALLOC
+  create_phase(s, "device allocation");
 
   // static metadata of the hashed colliders and the collider -> hashed index map
   std::vector<U4> hmeta(std::max<size_t>(1, hashed.size()), U4{0, 0, 0, 0});
@@ -1657,9 +1640,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.colliderHullRange, hullRange) || upload(S.hullVerts, hullVerts) ||
       upload(S.colliderLocalPos, cp) || upload(S.colliderLocalRot, cq) || upload(S.colliderFriction, cf) ||
       upload(S.hashedIdx, hashed) || upload(S.globalIdx, globals) || upload(S.exclusions, exr) ||
-      upload(S.tets, tr) || upload(S.joints, jr) || upload(s->jointPenInit, penInit) || upload(S.springs, sr) ||
+      upload(S.joints, jr) || upload(s->jointPenInit, penInit) || upload(S.springs, sr) ||
       upload(S.sAdjStart, sStart) ||
-      upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) || upload(S.tetSlots, slots) ||
+      upload(S.sAdjCount, sCount) || upload(S.sAdjList, sList) ||
       upload(S.hashedRigid, hashedRigid) || upload(S.clothSlots, cslots) ||
       upload(s->rateJointDev, s->rateJoints) || upload(s->rateValueDev, s->rateValues) || upload(S.tris, ch.tris) ||
       upload(S.edges, ch.edges) || upload(S.nbrStart, ch.nbrStart) || upload(S.nbrCount, ch.nbrCount) ||
@@ -1667,6 +1650,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       upload(S.selfCollision, ch.selfCollision) ||
       upload(S.posLinSnap, pl) ||
       upload(S.posAngSnap, pa))
+    return AVBD_ERR_HIP;
+  if (nt && gpu_tet_precompute(d->tet_ids, d->tet_mu, d->tet_lambda, nt, d->body_position, nb, S.sAdjStart,
+                               S.sAdjCount, S.sAdjList, S.tets, S.tetSlots, s->stream) != hipSuccess)
     return AVBD_ERR_HIP;
 
   if (s->isoOn) {
@@ -1684,6 +1670,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
       return AVBD_ERR_HIP;
   }
 
+  create_phase(s, "uploads");
   if (hipHostMalloc((void**)&s->pinnedCounters, 2 * CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault) !=
       hipSuccess)
     return AVBD_ERR_ALLOC;
@@ -1703,6 +1690,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   s->gridCap = (uint32_t)std::max(256, 8 * cus);
   s->cus = (uint32_t)std::max(1, cus);
+  create_phase(s, "runtime resources");
   return AVBD_OK;
 }
 
@@ -2673,6 +2661,13 @@ extern "C" avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity,
   // the frame just retired wrote into what is now prevSoft
   if (records && m && hipMemcpy(records, s->S.prevSoft, sizeof(SoftRec) * m, hipMemcpyDeviceToHost))
     return AVBD_ERR_HIP;
+  return AVBD_OK;
+}
+
+extern "C" avbd_status avbd_get_create_profile(avbd_solver* s, uint32_t index, const char** name, float* ms) {
+  if (!s || index >= s->createPhases.size()) return AVBD_ERR_INVALID_ARGUMENT;
+  if (name) *name = s->createPhases[index].first;
+  if (ms) *ms = (float)s->createPhases[index].second;
   return AVBD_OK;
 }
 

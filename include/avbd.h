@@ -384,6 +384,12 @@ avbd_status avbd_get_dat_pairs(avbd_solver* s, uint32_t capacity, uint32_t* n, u
  * C0+lambdaCap, lambda+keyA, penalty+keyB (float4 each). */
 avbd_status avbd_get_soft_contacts(avbd_solver* s, uint32_t capacity, uint32_t* n, void* records);
 
+/* No counterpart in the reference.  Wall time of avbd_create's phases
+ * (host clock; each phase's device work is drained before it closes), in
+ * order: phase `index` -> a static name and its milliseconds;
+ * AVBD_ERR_INVALID_ARGUMENT past the last phase. */
+avbd_status avbd_get_create_profile(avbd_solver* s, uint32_t index, const char** name, float* ms);
+
 /* Isotropic limiter state after the last frame (vtTrackBuf, eeTrackBuf,
  * boundsBuf, ogcPrevBuf of GPUSolver.swift:1336-1460, 25_cloth.metal:
  * 350-940): vt_track[4*num_bodies] tracked triangles per body (0xFFFFFFFF:

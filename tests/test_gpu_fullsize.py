@@ -8,7 +8,10 @@
   equal the oracle's, one frame later positions agree within 1e-4 m.
 * The 4096 batched rigid scenes: the first frame's hash cell lists, rigid
   pair stream, manifold headers and feature keys, and the contact-aware
-  dynamic palette equal the oracle's."""
+  dynamic palette equal the oracle's.
+* The 1M-tet block (55^3 cells): the collision triangles the device
+  extracts from the tet meshes at create (sort-based boundary faces), the
+  surface edges and particles, and the static palette equal the oracle's."""
 import numpy as np
 import pytest
 
@@ -96,3 +99,18 @@ def test_batched_4096_first_frame_bit_exact():
         assert np.array_equal(g.debugColors(), o.debugColors())
         flush_print(f"  {len(s.bodies)} bodies, {len(h)} pairs, {int((h[:, 2] > 0).sum())} active manifolds, "
                     f"{g.stats().num_colors} colors")
+
+
+def test_million_tet_block_surface_and_palette():
+    flush_print("[gpu case] 55^3 block: device-extracted boundary faces, surface topology, palette bit-exact")
+    s = demos.soft_block_scene(cells=55)
+    with GPUSolver(s) as g:
+        o = oracle.OracleSolver(s)
+        tg, to = g.collisionTris(), o.collisionTris()
+        flush_print(f"  {len(tg)} boundary faces (oracle {len(to)}); create phases {g.createProfile()}")
+        assert len(to) == 6 * 55 * 55 * 2
+        assert np.array_equal(tg, to)
+        cg, co = g.clothInfo(), o.clothInfo()
+        assert np.array_equal(cg["edges"], co["edges"])
+        assert np.array_equal(cg["particles"], co["particles"])
+        assert np.array_equal(g.debugColors(), o.debugColors())
