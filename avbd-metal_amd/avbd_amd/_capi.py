@@ -146,6 +146,9 @@ SIGNATURES = {
     "avbd_get_soft_contacts": (C.c_int, [C.c_void_p, C.c_uint32, U32P, C.c_void_p]),
     "avbd_get_iso_state": (C.c_int, [C.c_void_p, U32P, U32P, FP, FP, U32P]),
     "avbd_get_create_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_char_p), FP]),
+    "avbd_scene_write": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "avbd_scene_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "avbd_scene_free": (None, [C.c_void_p]),
     "avbd_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_soft_map_find": (C.c_int, [C.c_void_p, U32P, C.c_uint32, I32P]),
     "avbd_max_constraint_error": (C.c_int, [C.c_void_p, FP]),

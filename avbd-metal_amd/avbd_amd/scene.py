@@ -7,7 +7,9 @@ Float arithmetic that the reference performs in Float (particle density,
 sphere size normalisation) is done in float32 here too.
 """
 import ctypes as C
+import itertools
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -200,12 +202,31 @@ class PhysicsScene:
         self.exclusions.append((bodyA, bodyB))
 
     # -- wire format ------------------------------------------------------------
+    def save(self, path):
+        """Writes the scene's wire format as a versioned scene file
+        (avbd_scene_write); SceneFile(path) loads it back."""
+        lib = _capi.load()
+        desc, keep = self.to_desc()
+        st = lib.avbd_scene_write(C.byref(desc), os.fsencode(path))
+        if st != _capi.OK:
+            raise _capi.AVBDError(st, f"avbd_scene_write {path}")
+
     def to_desc(self):
         """Returns (SceneDesc, keepalive list)."""
         keep = []
 
         def arr(values, dtype, width=None):
-            a = np.ascontiguousarray(np.asarray(values, dtype=dtype).reshape(-1))
+            # rows of equal-length tuples stream through fromiter (no
+            # intermediate object array); anything else goes through asarray
+            if len(values) and isinstance(values[0], (tuple, list)):
+                w = len(values[0])
+                if sum(map(len, values)) == w * len(values) and min(map(len, values)) == w:
+                    a = np.fromiter(itertools.chain.from_iterable(values), dtype=dtype, count=w * len(values))
+                else:
+                    a = np.asarray(values, dtype=dtype).reshape(-1)
+            else:
+                a = np.asarray(values, dtype=dtype).reshape(-1)
+            a = np.ascontiguousarray(a)
             keep.append(a)
             return a
 
@@ -306,3 +327,34 @@ class PhysicsScene:
             else:
                 n += b["density"] > 0
         return n
+
+
+class SceneFile:
+    """A scene loaded from a scene file (avbd_scene_read): usable wherever a
+    PhysicsScene goes into a solver; the descriptor is owned by the library
+    and released with this object."""
+
+    def __init__(self, path):
+        self._lib = _capi.load()
+        p = C.c_void_p()
+        st = self._lib.avbd_scene_read(os.fsencode(path), C.byref(p))
+        if st != _capi.OK:
+            raise _capi.AVBDError(st, f"avbd_scene_read {path}")
+        self._p = p
+        self.desc = C.cast(p, C.POINTER(_capi.SceneDesc)).contents
+        self.settings = SimSettings.from_c(self.desc.settings)
+
+    @property
+    def num_bodies(self):
+        return int(self.desc.num_bodies)
+
+    def to_desc(self):
+        return self.desc, [self]
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.avbd_scene_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()

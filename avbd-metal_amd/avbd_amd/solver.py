@@ -38,7 +38,7 @@ class GPUSolver:
         h = C.c_void_p()
         st = self._lib.avbd_create(C.byref(desc), C.byref(opts), C.byref(h))
         self._h = h
-        self.bodyCount = len(scene.bodies)
+        self.bodyCount = int(desc.num_bodies)
         if st == _capi.ERR_RUNTIME_FAILURE and h:
             f = self._failure()
             self._lib.avbd_destroy(h)

@@ -186,6 +186,19 @@ typedef struct avbd_scene_desc {
     const float* joint_motor_rate;        /* [j] */
 } avbd_scene_desc;
 
+/* Scene files: a versioned on-disk form of avbd_scene_desc (no reference
+ * counterpart; for fixtures and for large scenes that should load without
+ * being rebuilt).  avbd_scene_write stores every present array with the
+ * settings and counts; avbd_scene_read checks the magic, the file version
+ * and every array's id, element size and length against the counts, and
+ * returns a descriptor owned by the library (release it with
+ * avbd_scene_free; avbd_create copies what it needs).  Unreadable,
+ * truncated or inconsistent files give AVBD_ERR_INVALID_ARGUMENT. */
+#define AVBD_SCENE_FILE_VERSION 1
+avbd_status avbd_scene_write(const avbd_scene_desc* d, const char* path);
+avbd_status avbd_scene_read(const char* path, avbd_scene_desc** out);
+void avbd_scene_free(avbd_scene_desc* d);
+
 /* Solver creation options (GPUSolver.init maxPairsPerBody, GPUSolver.swift:331) */
 typedef struct avbd_options {
     int32_t max_pairs_per_body;        /* 16; maxPairs = max(4096, colliders * this), negative -> 16 */

@@ -105,7 +105,7 @@ class CpuPathSolver:
         if rc != 0:
             raise ValueError(f"scene outside the CPU path ({rc})")
         self._h = h
-        self.bodyCount = len(scene.bodies)
+        self.bodyCount = int(desc.num_bodies)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -144,7 +144,7 @@ class OracleSolver:
         if rc != 0:
             raise RuntimeError(f"oracle_create failed ({rc})")
         self._h = h
-        self.bodyCount = len(scene.bodies)
+        self.bodyCount = int(desc.num_bodies)
         self.precision = precision
 
     def __del__(self):
