@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libavbd_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_COLORS = 64
 
 # avbd_status
@@ -71,7 +71,8 @@ class SceneDesc(C.Structure):
                 ("num_tris", C.c_uint32), ("tri_ids", U32P), ("tri_mu", FP), ("tri_lambda", FP),
                 ("tri_bend", FP),
                 ("num_exclusions", C.c_uint32), ("exclusion_pairs", U32P),
-                ("collider_hull_count", U32P), ("hull_vertices", FP), ("joint_motor_rate", FP)]
+                ("collider_hull_count", U32P), ("hull_vertices", FP), ("joint_motor_rate", FP),
+                ("num_spinners", C.c_uint32), ("spinner_body", U32P), ("spinner_axis_omega", FP)]
 
 
 class Options(C.Structure):

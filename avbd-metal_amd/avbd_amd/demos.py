@@ -387,3 +387,138 @@ def _closest_points(p, a, b, c):
         out = np.where(((d3 >= 0) & (d4 <= d3))[:, None], b, out)
         out = np.where(((d1 <= 0) & (d2 <= 0))[:, None], a, out)
     return out
+
+
+def _quat(angle, axis):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    h = 0.5 * angle
+    return (float(a[0] * math.sin(h)), float(a[1] * math.sin(h)), float(a[2] * math.sin(h)), float(math.cos(h)))
+
+
+def _qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    q = np.array([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                  aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz])
+    return tuple(float(v) for v in q / np.linalg.norm(q))
+
+
+def chainmail_scene(rings=6, drops=3):
+    """Mail of interlocked tori hung on flexible posts (Demos2.swift:318-430
+    layout): an n x n grid of flat rings, joined by upright rings along x and
+    y between neighbours; posts at the corner and mid-edge cells carry welded
+    cross pegs just under the sheet; seeded spheres and cubes drop on top."""
+    s = PhysicsScene("chainmail", SimSettings(iterations=15, betaLin=20000.0, lambdaMax=1500.0))
+    add_ground(s, 0.6)
+    R, r, sheetZ, pitch = 0.45, 0.15, 8.0, 1.15
+    n = rings
+    extent = (n - 1) * pitch / 2
+    mid = (n - 1) // 2
+
+    def add_post(x, y):
+        segH, cross = 1.6, 0.22
+        pegZ, topZ = sheetZ - 0.3, sheetZ + 1.0
+        prev = s.addBody((cross, cross, segH), 0.0, 0.4, (x, y, segH / 2))
+        z = segH
+        while z < topZ:
+            h = min(segH, topZ - z)
+            seg = s.addBody((cross, cross, h), 4.0, 0.4, (x, y, z + h / 2))
+            s.addJoint(SceneJoint(prev, seg, (0, 0, s.bodies[prev]["size"][2] / 2), (0, 0, -h / 2),
+                                  stiffnessLin=math.inf, stiffnessAng=math.inf))
+            prev = seg
+            z += h
+        host = prev
+        for axis in range(2):
+            size = (1.1, 0.14, 0.14) if axis == 0 else (0.14, 1.1, 0.14)
+            peg = s.addBody(size, 2.0, 0.5, (x, y, pegZ))
+            segCenterZ = s.bodies[host]["position"][2]
+            off = (0.4, 0, 0) if axis == 0 else (0, 0.4, 0)
+            s.addJoint(SceneJoint(host, peg, (0, 0, pegZ - segCenterZ), (0, 0, 0),
+                                  stiffnessLin=math.inf, stiffnessAng=math.inf))
+            s.addJoint(SceneJoint(host, peg, (off[0], off[1], pegZ - segCenterZ), off,
+                                  stiffnessLin=math.inf, stiffnessAng=math.inf))
+
+    posts = set()
+    for j in range(n):
+        for i in range(n):
+            corner = i in (0, n - 1) and j in (0, n - 1)
+            edgeMid = (i == mid and j in (0, n - 1)) or (j == mid and i in (0, n - 1))
+            if corner or edgeMid:
+                posts.add(j * n + i)
+    for cell in sorted(posts):
+        add_post((cell % n) * pitch - extent, (cell // n) * pitch - extent)
+    qX = _quat(math.pi / 2, (1, 0, 0))
+    qY = _qmul(_quat(math.pi / 2, (0, 0, 1)), qX)
+    for j in range(n):
+        for i in range(n):
+            cx, cy = i * pitch - extent, j * pitch - extent
+            s.addTorus(R, r, 1.0, 0.3, (cx, cy, sheetZ))
+            if i + 1 < n:
+                s.addTorus(R, r, 1.0, 0.3, (cx + pitch / 2, cy, sheetZ), rotation=qX)
+            if j + 1 < n:
+                s.addTorus(R, r, 1.0, 0.3, (cx, cy + pitch / 2, sheetZ), rotation=qY)
+    rng = SplitMix64(11)
+    for k in range(drops):
+        sz = float(np.float32(0.9) + rng.nextFloat() * np.float32(0.7))
+        pos = (float((rng.nextFloat() - np.float32(0.5)) * np.float32(n * pitch * 0.4)),
+               float((rng.nextFloat() - np.float32(0.5)) * np.float32(n * pitch * 0.4)), sheetZ + 4 + k * 2)
+        if k % 2 == 0:
+            s.addSphere(sz, 1.0, 0.5, pos)
+        else:
+            s.addBody((sz, sz, sz), 1.0, 0.5, pos)
+    return s
+
+
+def treadmill_scene(boxes=8):
+    """A plank belt driven by kinematic spinning drums (Demos2.swift:492-580
+    layout): 26 planks hinged in a closed loop around two wheels, each wheel
+    four static octagon slabs spinning at 1.2 rad/s about y, slick support and
+    return beds, and seeded cargo cubes dropped on the upstream end."""
+    s = PhysicsScene("treadmill")
+    add_ground(s)
+    wheelX, wheelZ, wrapR, beltW = 2.6, 2.6, 1.0, 3.2
+    span = 2 * wheelX
+    circumference = 2 * span + 2 * math.pi * wrapR
+    plankCount = 26
+    plankLen = circumference / plankCount
+
+    def path_point(t):
+        t = math.fmod(t, circumference)
+        if t < 0:
+            t += circumference
+        if t < span:
+            return (-wheelX + t, 0.0, wheelZ + wrapR), 0.0
+        t -= span
+        if t < math.pi * wrapR:
+            a = math.pi / 2 - t / wrapR
+            return (wheelX + wrapR * math.cos(a), 0.0, wheelZ + wrapR * math.sin(a)), math.pi / 2 - a
+        t -= math.pi * wrapR
+        if t < span:
+            return (wheelX - t, 0.0, wheelZ - wrapR), math.pi
+        t -= span
+        a = -math.pi / 2 - t / wrapR
+        return (-wheelX + wrapR * math.cos(a), 0.0, wheelZ + wrapR * math.sin(a)), math.pi / 2 - a
+
+    planks = []
+    for i in range(plankCount):
+        pos, pitchA = path_point((i + 0.5) * plankLen)
+        planks.append(s.addBody((plankLen * 0.98, beltW, 0.1), 0.6, 0.9, pos, rotation=_quat(pitchA, (0, 1, 0))))
+    for i in range(plankCount):
+        nx = (i + 1) % plankCount
+        for yo in (-beltW * 0.4, beltW * 0.4):
+            s.addJoint(SceneJoint(planks[i], planks[nx], (plankLen / 2, yo, 0), (-plankLen / 2, yo, 0)))
+    for wx in (-wheelX, wheelX):
+        for k in range(4):
+            idx = s.addBody((2 * (wrapR - 0.06), beltW * 0.8, 0.3), 0.0, 1.2, (wx, 0, wheelZ),
+                            rotation=_quat(k * math.pi / 4, (0, 1, 0)))
+            s.addSpinner(idx, (0, 1, 0), 1.2)
+    s.addBody((span - 2.6, beltW, 0.25), 0.0, 0.02, (0, 0, wheelZ + wrapR - 0.26))
+    s.addBody((span - 2.6, beltW, 0.25), 0.0, 0.02, (0, 0, wheelZ - wrapR - 0.45))
+    rng = SplitMix64(21)
+    for k in range(boxes):
+        e = float(np.float32(0.5) + rng.nextFloat() * np.float32(0.25))
+        x = -wheelX + 0.7 + float(rng.nextFloat() - np.float32(0.5)) * 0.6
+        y = float(rng.nextFloat() - np.float32(0.5)) * (beltW - 1)
+        s.addBody((e, e, e), 0.8, 0.9, (x, y, wheelZ + wrapR + 1.0 + k * 0.8))
+    return s

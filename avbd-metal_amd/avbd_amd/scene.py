@@ -89,6 +89,7 @@ class PhysicsScene:
         self.tets = []
         self.tris = []
         self.exclusions = []
+        self.spinners = []     # (body, unit axis, omega)
 
     # -- Scene.swift:566-589 -------------------------------------------------
     def addBody(self, size, density, friction, position, rotation=(0, 0, 0, 1), velocity=(0, 0, 0),
@@ -115,6 +116,19 @@ class PhysicsScene:
     def addSphere(self, diameter, density, friction, position, velocity=(0, 0, 0)):
         return self.addBody((diameter, diameter, diameter), density, friction, position, velocity=velocity,
                             shape=SPHERE)
+
+    def addTorus(self, major, minor, density, friction, position, rotation=(0, 0, 0, 1), velocity=(0, 0, 0)):
+        """Scene.swift:696-702: size = (major radius, minor radius, 0), spine in the local xy plane."""
+        return self.addBody((major, minor, 0.0), density, friction, position, rotation=rotation, velocity=velocity,
+                            shape=TORUS)
+
+    def addSpinner(self, body, axis, omega):
+        """SceneSpinner, Scene.swift:459-469: a kinematic spinner turning `body`
+        by omega rad/s about `axis` (normalised in float32)."""
+        a = np.asarray(axis, np.float32)
+        a = a / np.float32(np.sqrt(np.float32(a[0] * a[0] + a[1] * a[1]) + np.float32(a[2] * a[2])))
+        self.spinners.append((int(body), tuple(float(x) for x in a), float(omega)))
+        return len(self.spinners) - 1
 
     def addCapsule(self, length, radius, density, friction, position, rotation=(0, 0, 0, 1), velocity=(0, 0, 0)):
         """Scene.swift:686-693: size = (cylinder length along local z, radius, 0)."""
@@ -316,6 +330,11 @@ class PhysicsScene:
         d.num_exclusions = len(E)
         if E:
             d.exclusion_pairs = ptr(arr(E, np.uint32), C.c_uint32)
+        K = self.spinners
+        d.num_spinners = len(K)
+        if K:
+            d.spinner_body = ptr(arr([k[0] for k in K], np.uint32), C.c_uint32)
+            d.spinner_axis_omega = ptr(arr([(*k[1], k[2]) for k in K], np.float32), C.c_float)
         return d, keep
 
     @property

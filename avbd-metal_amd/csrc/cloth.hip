@@ -787,7 +787,19 @@ __device__ int rt_features(const DeviceState& S, uint32_t c, V3 center, F4 rot, 
     out[0] = RtFeature{center, size.x * 0.5f, 8u};
     return 1;
   }
-  if (kind != 0u) return 0;
+  if (kind == 3u) {
+    // capsule: spheres at its two ends and middle (25_cloth.metal:957-970)
+    const float half = size.x * 0.5f, rc = size.y;
+    const V3 ax = q_rotate(rot, v3(0.0f, 0.0f, 1.0f));
+    int n = 0;
+    for (int i = -1; i <= 1; i++) {
+      const V3 w = center + ax * (half * (float)i);
+      if (length(w - m) > rT + rc + 4.0f * margin) continue;
+      out[n++] = RtFeature{w, rc, (uint32_t)(10 + i)};
+    }
+    return n;
+  }
+  if (kind != 0u) return 0;  // torus: its particle contacts come from the rigid narrow phase
   V3 h3 = size * 0.5f;
   int n = 0;
   for (uint32_t i = 0; i < 8u; i++) {

@@ -39,6 +39,8 @@ void launch_rt_emit(hipStream_t st, const DeviceState& S, const StepParams& P);
 // element grid + V-T / E-E pair stream at the current (predictor) pose
 void launch_dat_pairs(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap);
 // legacy servo rates: each listed joint's motor target advances by rate * dt, wrapped to [-pi, pi]
+void launch_spinners(hipStream_t st, const DeviceState& S, const uint32_t* bodies, const uint32_t* start,
+                     const F4* dq, uint32_t n);
 void launch_rate_motors(hipStream_t st, const DeviceState& S, const uint32_t* joints, const float* rates, uint32_t n,
                         float dt);
 // OGC records from the pair stream, soft counts, soft persistence map rebuild

@@ -717,6 +717,212 @@ __device__ __forceinline__ bool round_pair(const PairView& V, const Obb& A, cons
   return true;
 }
 
+// ---- torus pairs (30_narrowphase.metal:292-336, 749-965) ----
+// The torus is the set of points within r of its spine, the circle of radius
+// R about the collider's local z axis through its center.
+struct TorusGeo {
+  V3 c, axis, u, v;
+  float R, r;
+};
+__device__ __forceinline__ TorusGeo torus_at(V3 p, F4 q, float R, float r) {
+  TorusGeo t;
+  t.c = p;
+  t.axis = q_rotate(q, v3(0.0f, 0.0f, 1.0f));
+  const bool xMajor = fabsf(t.axis.x) > fabsf(t.axis.z);
+  t.u = normalize(pick(xMajor, v3(-t.axis.y, t.axis.x, 0.0f), v3(0.0f, -t.axis.z, t.axis.y)));
+  t.v = cross(t.axis, t.u);
+  t.R = R;
+  t.r = r;
+  return t;
+}
+// nearest spine point (the spine point along u for points on the axis)
+__device__ __forceinline__ V3 spine_nearest(const TorusGeo& t, V3 p) {
+  V3 d = p - t.c;
+  d = d - t.axis * dot(t.axis, d);
+  const float l = length(d);
+  return pick(l < 1e-6f, t.c + t.u * t.R, t.c + d / l * t.R);
+}
+
+// What the torus is tested against: a box (nearest point by clamping), a
+// capsule (its segment) or another torus (its spine).
+struct TorusOther {
+  bool isBox, isCap;
+  V3 p, half, s0, s1;
+  F4 q, qc;
+  TorusGeo t2;
+  float r;  // tube / capsule radius, 0 for a box
+};
+__device__ __forceinline__ V3 torus_other_nearest(const TorusOther& O, V3 q) {
+  if (O.isBox) return q_rotate(O.q, clamp_box(q_rotate(O.qc, q - O.p), O.half)) + O.p;
+  if (O.isCap) return nearest_on_segment(q, O.s0, O.s1);
+  return spine_nearest(O.t2, q);
+}
+
+// Lane k evaluates spine samples k and k + 16 (angles 2 pi i / 32 from the
+// cos / sin table); the four greedy picks are row arg-mins of the sample gap
+// (lowest sample on ties), each masking the samples within 3 of it; lane j
+// then refines pick j by two alternating projections and a ballot packs the
+// kept hits in pick order.  The sphere case is closed form on lane 0.
+// Hits carry (xA, xB) on colliders A / B; `nrm` points from B toward A.
+__device__ __forceinline__ bool torus_pair(const PairView& V, const StepParams& P, const float* trig, Hit& h, V3& nrm,
+                                           float& matchRadius) {
+  const int k = row_lane();
+  const uint32_t kindA = V.ta & kShapeKindMask, kindB = V.tb & kShapeKindMask;
+  const bool tIsA = kindA == 2u;
+  const F4 shT = pick(tIsA, V.shA, V.shB), shO = pick(tIsA, V.shB, V.shA);
+  const V3 pO = pick(tIsA, V.cB, V.cA);
+  const TorusGeo T = torus_at(pick(tIsA, V.cA, V.cB), pick(tIsA, V.qA, V.qB), shT.x, shT.y);
+  const uint32_t other = tIsA ? kindB : kindA;
+  const float detectM = fmaxf(P.collisionMargin, 0.8f * T.r);  // fixed reach: a fraction of the tube radius
+  matchRadius = 1.1f * shT.x;
+  bool kept = false;
+  V3 xT = v3s(0), xO = v3s(0), n = v3s(0);
+  float sep = 0.0f;
+  uint32_t feat = 0;
+  if (other == 1u) {
+    if (k == 0) {
+      const float rs = shO.x * 0.5f;
+      const V3 spine = spine_nearest(T, pO);
+      const V3 d = pO - spine;
+      const float dist = length(d);
+      if (dist <= T.r + rs + detectM) {
+        const V3 nts = pick(dist > 1e-6f, d / dist, v3(0.0f, 0.0f, 1.0f));  // torus -> sphere
+        xT = spine + nts * T.r;
+        xO = pO - nts * rs;
+        n = -nts;
+        sep = dist - T.r - rs;
+        kept = true;
+      }
+    }
+  } else {
+    TorusOther O;
+    O.isBox = other == 0u;
+    O.isCap = other == 3u;
+    O.p = pO;
+    O.q = pick(tIsA, V.qB, V.qA);
+    O.qc = q_conj(O.q);
+    O.half = xyz(shO) * 0.5f;
+    O.r = 0.0f;
+    O.s0 = O.s1 = v3s(0);
+    O.t2 = TorusGeo{};
+    if (O.isCap) {
+      const float ch = shO.x * 0.5f;
+      const V3 ax = q_rotate(O.q, v3(0.0f, 0.0f, 1.0f));
+      O.s0 = pO - ax * ch;
+      O.s1 = pO + ax * ch;
+      O.r = shO.y;
+    } else if (!O.isBox) {
+      O.t2 = torus_at(pO, O.q, shO.x, shO.y);
+      O.r = O.t2.r;
+    }
+    const V3 q0 = T.c + (T.u * trig[2 * k] + T.v * trig[2 * k + 1]) * T.R;
+    const V3 q1 = T.c + (T.u * trig[2 * k + 32] + T.v * trig[2 * k + 33]) * T.R;
+    const float d0 = length(q0 - torus_other_nearest(O, q0));
+    const float d1 = length(q1 - torus_other_nearest(O, q1));
+    bool u0 = false, u1 = false;
+    int mine = -1, picks = 0;
+    for (int pk = 0; pk < 4; pk++) {
+      float bd = kHuge;
+      int bi = -1;
+      if (!u0 && d0 < bd) { bd = d0; bi = k; }
+      if (!u1 && d1 < bd) { bd = d1; bi = k + 16; }
+      for (int o = 8; o > 0; o >>= 1) {
+        const float d2 = __shfl_xor(bd, o, kRow);
+        const int i2 = __shfl_xor(bi, o, kRow);
+        if (i2 >= 0 && (bi < 0 || d2 < bd || (d2 == bd && i2 < bi))) {
+          bd = d2;
+          bi = i2;
+        }
+      }
+      bd = row_get(bd, 0);
+      bi = row_get(bi, 0);
+      if (bi < 0 || bd - T.r - O.r > detectM) break;
+      int c0 = abs(k - bi), c1 = abs(k + 16 - bi);
+      c0 = min(c0, 32 - c0);
+      c1 = min(c1, 32 - c1);
+      u0 = u0 || c0 <= 3;
+      u1 = u1 || c1 <= 3;
+      if (k == pk) mine = bi;
+      picks++;
+    }
+    if (k < picks) {
+      V3 q = T.c + (T.u * trig[2 * mine] + T.v * trig[2 * mine + 1]) * T.R;
+      V3 b = torus_other_nearest(O, q);
+      for (int r2 = 0; r2 < 2; r2++) {
+        q = spine_nearest(T, b);
+        b = torus_other_nearest(O, q);
+      }
+      const V3 d = q - b;
+      const float dist = length(d);
+      V3 xo = b;
+      if (dist < 1e-6f) {
+        if (O.isBox) {
+          // spine point inside the box: out through the nearest face
+          const V3 lb = q_rotate(O.qc, q - O.p);
+          const V3 pen = O.half - v3(fabsf(lb.x), fabsf(lb.y), fabsf(lb.z));
+          const int ax = min_axis(pen);
+          V3 nl = v3s(0);
+          setcomp(nl, ax, comp(lb, ax) >= 0.0f ? 1.0f : -1.0f);
+          n = q_rotate(O.q, nl);
+          xo = q + n * comp(pen, ax);
+        } else {
+          const V3 other2 = pick(O.isCap, normalize(O.s1 - O.s0), O.t2.axis);
+          n = normalize(cross(T.axis, other2) + v3(1e-4f, 0.0f, 0.0f));
+        }
+        kept = true;
+      } else {
+        kept = !(dist - T.r - O.r > detectM);
+        n = d / dist;  // other -> torus
+      }
+      xT = q - n * T.r;
+      xO = pick(O.isBox, xo, xo + n * O.r);
+      sep = dist - T.r - O.r;
+      feat = (uint32_t)mine;
+    }
+  }
+  const uint32_t bits = row_bits(kept);
+  if (bits == 0u) return false;
+  // deepest hit (first in pick order on ties) sets the normal
+  bool ok = kept && sep < kHuge;
+  float best = ok ? sep : kHuge;
+  int lane = ok ? k : 16;
+  for (int o = 8; o > 0; o >>= 1) {
+    const bool ok2 = __shfl_xor((int)ok, o, kRow) != 0;
+    const float s2 = __shfl_xor(best, o, kRow);
+    const int l2 = __shfl_xor(lane, o, kRow);
+    if (ok2 && (!ok || s2 < best || (s2 == best && l2 < lane))) {
+      ok = true;
+      best = s2;
+      lane = l2;
+    }
+  }
+  lane = row_get(lane, 0);
+  if (lane >= 16) lane = __ffs(bits) - 1;
+  const V3 nDeep = row_get(n, lane);
+  nrm = pick(tIsA, nDeep, -nDeep);
+  h.on = kept;
+  h.slot = __popc(bits & ((1u << k) - 1u));
+  h.xA = pick(tIsA, xT, xO);
+  h.xB = pick(tIsA, xO, xT);
+  h.key = feat;
+  return true;
+}
+
+// Kinematic spinner surface motion as a tangential target
+// (30_narrowphase.metal:274-284): each spinning body's surface displacement
+// over the step at the contact points (offsets from the body origins),
+// pre-divided by 1 - alpha because the solver discounts C0 by alpha.
+__device__ __forceinline__ void spin_shift(const DeviceState& S, const StepParams& P, uint32_t ba, uint32_t bb, V3 offA,
+                                           V3 offB, V3 t1, V3 t2, F4& C0) {
+  const F4 wA = S.spinVel[ba], wB = S.spinVel[bb];
+  const V3 sA = pick(wA.w != 0.0f, cross(xyz(wA), offA) * P.dt, v3s(0));
+  const V3 sB = pick(wB.w != 0.0f, cross(xyz(wB), offB) * P.dt, v3s(0));
+  const V3 rel = sB - sA;
+  const float kk = 1.0f / fmaxf(1.0f - P.alpha, 0.01f);
+  C0.y = C0.y - dot(t1, rel) * kk;
+  C0.z = C0.z - dot(t2, rel) * kk;
+}
+
 __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P) {
   __shared__ F4 scratch[kRowsPerGroup][kPolyMax];
   F4* lds = scratch[threadIdx.x >> 4];
@@ -730,7 +936,8 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
     const Obb B = make_obb(V.cB, V.qB, xyz(V.shB));
     const uint32_t kindA = V.ta & kShapeKindMask, kindB = V.tb & kShapeKindMask;
     const bool hullA = kindA == 4u, hullB = kindB == 4u;
-    const bool roundPair = kindA == 1u || kindB == 1u || kindA == 3u || kindB == 3u;
+    const bool torusPair = kindA == 2u || kindB == 2u;
+    const bool roundPair = !torusPair && (kindA == 1u || kindB == 1u || kindA == 3u || kindB == 3u);
     if (k == 0) M.colliderPair = U4{V.ca, V.cb, 0, 0};
 
     Hit h;
@@ -744,6 +951,8 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
     const bool refused = (hullA || hullB) && ((hullA && hullB) || (hullA ? kindB : kindA) != 0u);
     if (refused) {
       touching = false;
+    } else if (torusPair) {
+      touching = torus_pair(V, P, S.torusTrig, h, nrm, matchRadius);
     } else if (roundPair) {
       touching = round_pair(V, A, B, P, h, nrm, matchRadius);
     } else if (hullA || hullB) {
@@ -784,8 +993,8 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
     V3 t1, t2;
     orthonormal(nrm, t1, t2);
     const F4 fA = S.colliderFriction[V.ca], fB = S.colliderFriction[V.cb];
-    const bool worldA = roundPair && (V.ta & kShapeWorldRound) != 0u;
-    const bool worldB = roundPair && (V.tb & kShapeWorldRound) != 0u;
+    const bool worldA = (roundPair || torusPair) && (V.ta & kShapeWorldRound) != 0u;
+    const bool worldB = (roundPair || torusPair) && (V.tb & kShapeWorldRound) != 0u;
     if (k == 0) {
       const uint32_t flags = 1u | (worldA ? 2u : 0u) | (worldB ? 4u : 0u);
       M.header = U4{V.ba, V.bb, count, flags};
@@ -803,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
     if (prev >= 0) {
       const ManifoldRec& Q = S.prevManifolds[prev];
       const uint32_t qn = Q.header.z;
-      if (roundPair) {
+      if (roundPair || torusPair) {
         // nearest previous anchor on A within the match radius; its
         // tangential multiplier is carried into this frame's basis
         float bestD = matchRadius;
@@ -824,9 +1033,10 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
           lam.z = dot(t2, ft);
           // a sticking anchor is reused unless it is a world offset of a
           // rigid round shape (capsule pairs: unless either is a world offset)
+          // (torus contacts never stick: their anchors are this frame's hits)
           const bool capsule = kindA == 3u || kindB == 3u;
-          const bool reuse = capsule ? (!worldA && !worldB)
-                                     : ((V.shA.w < 0.0f || !worldA) && (V.shB.w < 0.0f || !worldB));
+          const bool reuse = !torusPair && (capsule ? (!worldA && !worldB)
+                                                    : ((V.shA.w < 0.0f || !worldA) && (V.shB.w < 0.0f || !worldB)));
           if (reuse) {
             stick = Q.c[bj].rB.w;
             if (stick != 0.0f) {
@@ -849,14 +1059,16 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
         }
       }
     }
-    const V3 wA = pick(worldA, V.pA + ancA, xform(V.pA, V.oA, ancA));
-    const V3 wB = pick(worldB, V.pB + ancB, xform(V.pB, V.oB, ancB));
+    // torus contacts measure the gap between this frame's hit points directly
+    const V3 wA = pick(torusPair, h.xA, pick(worldA, V.pA + ancA, xform(V.pA, V.oA, ancA)));
+    const V3 wB = pick(torusPair, h.xB, pick(worldB, V.pB + ancB, xform(V.pB, V.oB, ancB)));
     const V3 gap = wA - wB;
     const V3 floorK = contact_penalty_floor(S, V.ba, V.bb, P);
     ContactRec& c = M.c[h.slot];
     c.rA = f4(ancA, as_float(h.key));
     c.rB = f4(ancB, stick);
     c.C0 = f4(v3(dot(nrm, gap) + P.collisionMargin, dot(t1, gap), dot(t2, gap)), 0.0f);
+    if (S.spinVel) spin_shift(S, P, V.ba, V.bb, wA - V.pA, wB - V.pB, t1, t2, c.C0);
     c.lambda = f4(lam * (P.alpha * P.gamma), 0.0f);
     c.penalty = f4(v3(clampf(pen.x * P.gamma, floorK.x, kPenaltyMax), clampf(pen.y * P.gamma, floorK.y, kPenaltyMaxT),
                       clampf(pen.z * P.gamma, floorK.z, kPenaltyMaxT)),
@@ -864,7 +1076,30 @@ __global__ void __launch_bounds__(256) k_np_collide(DeviceState S, StepParams P)
   }
 }
 
+// Kinematic spinners (GPUSolver.swift:4418-4427): before the frame each
+// spinning body's orientation becomes normalize(dq q) for each of its
+// spinners in order, dq the host-computed rotation by omega dt about the
+// spinner axis.  One thread per spinning body.
+__global__ void __launch_bounds__(64) k_spinners(DeviceState S, const uint32_t* bodies, const uint32_t* start,
+                                                 const F4* dq, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t b = bodies[t];
+  F4 q = S.posAng[b];
+  for (uint32_t k = start[t]; k < start[t + 1]; k++) {
+    const F4 r = q_mul(dq[k], q);
+    const float sc = 1.0f / sqrtf(r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w);
+    q = F4{r.x * sc, r.y * sc, r.z * sc, r.w * sc};
+  }
+  S.posAng[b] = q;
+}
+
 }  // namespace
+
+void launch_spinners(hipStream_t st, const DeviceState& S, const uint32_t* bodies, const uint32_t* start,
+                     const F4* dq, uint32_t n) {
+  hipLaunchKernelGGL(k_spinners, dim3((n + 63) / 64), dim3(64), 0, st, S, bodies, start, dq, n);
+}
 
 void launch_narrowphase(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
   uint32_t blocks = (P.maxPairs + kRowsPerGroup - 1) / kRowsPerGroup;

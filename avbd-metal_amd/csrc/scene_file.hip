@@ -3,8 +3,8 @@
 //
 // Layout (little-endian): the 8-byte magic "AVBDSCEN", the file version,
 // the descriptor ABI version it was written from, the 15 settings fields
-// (4 bytes each), the eight counts (bodies, colliders, joints, springs,
-// tets, triangles, exclusions, hull vertices), the number of arrays, then
+// (4 bytes each), the nine counts (bodies, colliders, joints, springs,
+// tets, triangles, exclusions, hull vertices, spinners), the number of arrays, then
 // per present array: its id, element size, element count and data padded
 // to 8 bytes.  Absent (NULL) arrays are not written and read back as NULL.
 // The reader checks every id, element size and count against the counts,
@@ -20,9 +20,9 @@
 namespace {
 
 constexpr char kMagic[8] = {'A', 'V', 'B', 'D', 'S', 'C', 'E', 'N'};
-constexpr uint32_t kFileVersion = 1;
+constexpr uint32_t kFileVersion = AVBD_SCENE_FILE_VERSION;
 
-enum Dim : uint32_t { kBodies, kColliders, kJoints, kSprings, kTets, kTris, kExclusions, kHullVerts, kDims };
+enum Dim : uint32_t { kBodies, kColliders, kJoints, kSprings, kTets, kTris, kExclusions, kHullVerts, kSpinners, kDims };
 
 struct Field {
   uint32_t id;
@@ -86,6 +86,8 @@ const Field kFields[] = {
     F(82, tri_lambda, float, kTris, 1),
     F(83, tri_bend, float, kTris, 1),
     F(90, exclusion_pairs, uint32_t, kExclusions, 2),
+    F(100, spinner_body, uint32_t, kSpinners, 1),
+    F(101, spinner_axis_omega, float, kSpinners, 4),
 };
 #undef F
 
@@ -107,6 +109,7 @@ void counts_of(const avbd_scene_desc* d, uint64_t n[kDims]) {
   if (d->collider_hull_count)
     for (uint32_t c = 0; c < d->num_colliders; c++) hv += d->collider_hull_count[c];
   n[kHullVerts] = hv;
+  n[kSpinners] = d->num_spinners;
 }
 
 // what avbd_scene_read hands out: the descriptor, then its arrays' storage
@@ -182,6 +185,7 @@ extern "C" avbd_status avbd_scene_read(const char* path, avbd_scene_desc** out) 
   d.num_tets = cnt[kTets];
   d.num_tris = cnt[kTris];
   d.num_exclusions = cnt[kExclusions];
+  d.num_spinners = cnt[kSpinners];
   // read every record, then check the hull vertex count against the counts
   struct Rec {
     const Field* fd;

@@ -74,6 +74,14 @@ struct avbd_solver {
   std::chrono::steady_clock::time_point phaseT0;
   std::vector<uint32_t> rateJoints;  // implicit-PD motors with a legacy servo rate
   std::vector<float> rateValues;
+  // kinematic spinners grouped by body (bodies in first-appearance order,
+  // each body's spinners in scene order): CSR over spinAxisOmega / spinDq
+  std::vector<uint32_t> spinBodies, spinStart;
+  std::vector<F4> spinAxisOmega;
+  uint32_t* spinBodyDev = nullptr;
+  uint32_t* spinStartDev = nullptr;
+  F4* spinDqDev = nullptr;
+  float spinDt = 0.0f;  // the step dt spinDqDev was computed for
   F4* jointPenInit = nullptr;      // upload-time joint penalties (lin [nj], ang [nj]), restored on reset
   int device = 0;                  // the HIP device current at create
   uint32_t* fixedScratch = nullptr;  // this device's address of the no-allocation staging array
@@ -268,6 +276,8 @@ void enqueue_pre_solve(avbd_solver* s, const DeviceState& S, const StepParams& P
   if (s->datOn) launch_fill_u32(st, S.datT, s->nb, 0x3F800000u);  // 1.0f: no pass has lowered a fraction
   if (!s->rateJoints.empty())
     launch_rate_motors(st, S, s->rateJointDev, s->rateValueDev, (uint32_t)s->rateJoints.size(), P.dt);
+  if (!s->spinBodies.empty())
+    launch_spinners(st, S, s->spinBodyDev, s->spinStartDev, s->spinDqDev, (uint32_t)s->spinBodies.size());
   launch_broadphase(st, S, P);
   stage("broadphase");
   launch_narrowphase(st, S, P, s->gridCap);
@@ -572,8 +582,7 @@ avbd_status validate(const avbd_scene_desc* d) {
     if (d->collider_body[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
     uint32_t sh = d->collider_shape[i];
     bool enabled = !d->collider_enabled || d->collider_enabled[i];
-    if (enabled && sh != AVBD_SHAPE_BOX && sh != AVBD_SHAPE_SPHERE && sh != AVBD_SHAPE_CAPSULE)
-      return AVBD_ERR_UNSUPPORTED;
+    if (enabled && sh > AVBD_SHAPE_CAPSULE) return AVBD_ERR_INVALID_ARGUMENT;
     // convex hulls: at most 64 finite vertices (GPUSolver.swift:1024-1025)
     const uint32_t hn = d->collider_hull_count ? d->collider_hull_count[i] : 0u;
     if (hn > 64u || (hn && !d->hull_vertices)) return AVBD_ERR_INVALID_ARGUMENT;
@@ -595,7 +604,29 @@ avbd_status validate(const avbd_scene_desc* d) {
     if (d->tri_ids[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < d->num_exclusions * 2; i++)
     if (d->exclusion_pairs[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+  if (d->num_spinners > 0 && (!d->spinner_body || !d->spinner_axis_omega)) return AVBD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < d->num_spinners; i++) {
+    if (d->spinner_body[i] >= d->num_bodies) return AVBD_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < 4; k++)
+      if (!isfinite(d->spinner_axis_omega[4 * i + k])) return AVBD_ERR_INVALID_ARGUMENT;
+  }
   return AVBD_OK;
+}
+
+// A spinner's per-frame rotation (x, y, z, w) by omega dt about its unit
+// axis, in float as the reference's host computes it before each frame
+// (GPUSolver.swift:4418-4427).
+F4 spinner_rotation(F4 axisOmega, float dt) {
+  const float angle = axisOmega.w * dt;
+  const double half = 0.5 * (double)angle;
+  const float sn = (float)std::sin(half), cs = (float)std::cos(half);
+  return F4{axisOmega.x * sn, axisOmega.y * sn, axisOmega.z * sn, cs};
+}
+
+std::vector<F4> spinner_rotations(const avbd_solver* s, float dt) {
+  std::vector<F4> dq(s->spinAxisOmega.size());
+  for (size_t k = 0; k < dq.size(); k++) dq[k] = spinner_rotation(s->spinAxisOmega[k], dt);
+  return dq;
 }
 
 // Boundary faces of the tet meshes (faces used by exactly one tet), wound
@@ -1187,6 +1218,25 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   }
 
   create_phase(s, "bodies");
+  // ---- kinematic spinners (GPUSolver.swift:628-637), grouped by body
+  {
+    std::vector<int32_t> slot(d->num_spinners ? nb : 0, -1);
+    std::vector<std::vector<F4>> per;
+    for (uint32_t k = 0; k < d->num_spinners; k++) {
+      const uint32_t b = d->spinner_body[k];
+      if (slot[b] < 0) {
+        slot[b] = (int32_t)s->spinBodies.size();
+        s->spinBodies.push_back(b);
+        per.emplace_back();
+      }
+      per[slot[b]].push_back(rd4(d->spinner_axis_omega, k));
+    }
+    s->spinStart.assign(1, 0u);
+    for (const auto& v : per) {
+      s->spinAxisOmega.insert(s->spinAxisOmega.end(), v.begin(), v.end());
+      s->spinStart.push_back((uint32_t)s->spinAxisOmega.size());
+    }
+  }
   // ---- colliders (GPUSolver.swift:971-1090)
   const uint32_t nc = d->num_colliders;
   std::vector<uint32_t> co(nc), ct(nc), cg(nc), csc(nc);
@@ -1583,6 +1633,17 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
   ALLOC(S.clothSlots, cslots.size())
   ALLOC(s->rateJointDev, std::max<size_t>(1, s->rateJoints.size()))
   ALLOC(s->rateValueDev, std::max<size_t>(1, s->rateValues.size()))
+  ALLOC(s->spinBodyDev, std::max<size_t>(1, s->spinBodies.size()))
+  ALLOC(s->spinStartDev, s->spinStart.size())
+  ALLOC(s->spinDqDev, std::max<size_t>(1, s->spinAxisOmega.size()))
+  if (!s->spinBodies.empty()) {
+    ALLOC(S.spinVel, nb)
+  }
+  bool anyTorus = false;
+  for (uint32_t i = 0; i < nc; i++) anyTorus = anyTorus || d->collider_shape[i] == AVBD_SHAPE_TORUS;
+  if (anyTorus) {
+    ALLOC(S.torusTrig, 64)
+  }
   {
     const bool cl = s->nTris > 0;
     const uint32_t nElem = cl ? s->nTris + s->nEdges : 0u, nProd = cl ? s->nParticles + s->nEdges : 0u;
@@ -1633,6 +1694,28 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     c2h[c] = (uint32_t)k;
   }
   if (upload(S.hashMeta, hmeta) || upload(S.colliderToHashed, c2h)) return AVBD_ERR_HIP;
+  if (!s->spinBodies.empty()) {
+    // surface angular velocity per body: the last spinner of a body sets it
+    std::vector<F4> sv(nb, F4{0.0f, 0.0f, 0.0f, 0.0f});
+    for (uint32_t k = 0; k < d->num_spinners; k++) {
+      const F4 ao = rd4(d->spinner_axis_omega, k);
+      sv[d->spinner_body[k]] = F4{ao.x * ao.w, ao.y * ao.w, ao.z * ao.w, 1.0f};
+    }
+    s->spinDt = s->settings.dt;
+    if (upload(S.spinVel, sv) || upload(s->spinBodyDev, s->spinBodies) || upload(s->spinStartDev, s->spinStart) ||
+        upload(s->spinDqDev, spinner_rotations(s, s->spinDt)))
+      return AVBD_ERR_HIP;
+  }
+  if (anyTorus) {
+    // cos / sin of the spine sample angles 2 pi i / 32, rounded to float
+    std::vector<float> trig(64);
+    for (int i = 0; i < 32; i++) {
+      const double a = 6.283185307179586476925 * (double)i / 32.0;
+      trig[2 * i] = (float)std::cos(a);
+      trig[2 * i + 1] = (float)std::sin(a);
+    }
+    if (upload(S.torusTrig, trig)) return AVBD_ERR_HIP;
+  }
   if (upload(S.posLin, pl) || upload(S.posAng, pa) || upload(S.velLin, vl) || upload(S.velAng, va) ||
       upload(S.prevVelLin, pv) || upload(S.props, pr) || upload(S.shape, sh) || upload(S.gravityScale, gs) ||
       upload(S.clothGroup, clothGroup) || upload(S.colliderOwner, co) || upload(S.colliderShapeType, ct) ||
@@ -2156,6 +2239,11 @@ extern "C" avbd_status avbd_set_settings(avbd_solver* s, const avbd_settings* in
   if (!(in->dt > 0.0f) || in->iterations < 0 || !(in->collision_margin >= 0.0f)) return AVBD_ERR_INVALID_ARGUMENT;
   avbd_status st = sync_all(s);
   s->settings = *in;
+  if (!s->spinBodies.empty() && s->settings.dt != s->spinDt) {
+    // the spinners' per-frame rotations follow the step size
+    s->spinDt = s->settings.dt;
+    if (upload(s->spinDqDev, spinner_rotations(s, s->spinDt))) return AVBD_ERR_HIP;
+  }
   invalidate_graphs(s);
   return st == AVBD_ERR_RUNTIME_FAILURE ? st : AVBD_OK;
 }

@@ -27,6 +27,8 @@ struct DeviceState {
   F4 *colliderShape, *colliderLocalPos, *colliderLocalRot, *colliderFriction;
   U2* colliderHullRange;      // convex hulls: (first vertex, count), count 0 = primitive
   F4* hullVerts;              // hull vertices, collider-local
+  float* torusTrig;           // cos, sin of the 32 torus spine sample angles (torus colliders only)
+  F4* spinVel;                // kinematic spinners: per body axis * omega, w = 1 if spinning; null if none
   // broad phase
   uint32_t *hashedIdx, *globalIdx;
   uint32_t* hashedRigid;      // hashed colliders whose owner is not a particle

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 5
+#define AVBD_ABI_VERSION 6
 #define AVBD_MAX_COLORS 64
 
 /* ---- status codes ------------------------------------------------------ */
@@ -184,6 +184,13 @@ typedef struct avbd_scene_desc {
      * (SceneJoint.motorRate, Scene.swift:300-314; GPUSolver.swift:1154-1158,
      * 4359-4416); NULL -> 0 */
     const float* joint_motor_rate;        /* [j] */
+    /* SceneSpinner, Scene.swift:459-469: kinematic spinners.  Before every
+     * frame the body's orientation turns by omega*dt about the unit axis
+     * (GPUSolver.swift:4406-4430), and contact targets against it move
+     * with its surface (30_narrowphase.metal:270-284). */
+    uint32_t num_spinners;
+    const uint32_t* spinner_body;         /* [k] */
+    const float* spinner_axis_omega;      /* [k*4] unit axis xyz, omega (rad/s) */
 } avbd_scene_desc;
 
 /* Scene files: a versioned on-disk form of avbd_scene_desc (no reference
@@ -194,7 +201,7 @@ typedef struct avbd_scene_desc {
  * returns a descriptor owned by the library (release it with
  * avbd_scene_free; avbd_create copies what it needs).  Unreadable,
  * truncated or inconsistent files give AVBD_ERR_INVALID_ARGUMENT. */
-#define AVBD_SCENE_FILE_VERSION 1
+#define AVBD_SCENE_FILE_VERSION 2
 avbd_status avbd_scene_write(const avbd_scene_desc* d, const char* path);
 avbd_status avbd_scene_read(const char* path, avbd_scene_desc** out);
 void avbd_scene_free(avbd_scene_desc* d);

@@ -55,6 +55,7 @@ def load():
         "oracle_stats": (C.c_int, [vp, i32p, i32p, i32p, i32p, i32p]),
         "oracle_map_find": (C.c_int, [vp, u32p, C.c_uint32, i32p]),
         "oracle_get_manifold_keys": (C.c_int, [vp, C.c_uint32, u32p, u32p]),
+        "oracle_get_manifold_records": (C.c_int, [vp, C.c_uint32, u32p, C.c_void_p]),
         "oracle_get_iso_state": (C.c_int, [vp, u32p, u32p, fp, fp, u32p]),
         "oracle_splitmix64": (None, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), fp]),
         "oracle_snh_tet": (None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
@@ -224,6 +225,22 @@ class OracleSolver:
         self._lib.oracle_get_manifolds(self._h, n.value, C.byref(n), _p(h, C.c_uint32), _p(nrm, C.c_float),
                                        _p(lp, C.c_float))
         return h[:n.value], nrm[:n.value], lp[:n.value]
+
+    # the 704-byte manifold record: header (a, b, count, flags), collider
+    # pair, (normal, dynamic friction), (t1, static friction), 8 contacts
+    MANIFOLD_DTYPE = np.dtype([("header", np.uint32, 4), ("colliderPair", np.uint32, 4), ("basisN", np.float32, 4),
+                               ("basisT1", np.float32, 4),
+                               ("c", [("rA", np.float32, 4), ("rB", np.float32, 4), ("C0", np.float32, 4),
+                                      ("lambda", np.float32, 4), ("penalty", np.float32, 4)], 8)])
+
+    def manifoldRecords(self):
+        """The last frame's manifolds in the backend's record layout
+        (rA.w holds the feature key bits, rB.w the stick flag)."""
+        n = C.c_uint32()
+        self._lib.oracle_get_manifold_records(self._h, 0, C.byref(n), None)
+        out = np.zeros(max(1, n.value), self.MANIFOLD_DTYPE)
+        self._lib.oracle_get_manifold_records(self._h, n.value, C.byref(n), out.ctypes.data_as(C.c_void_p))
+        return out[:n.value]
 
     def manifoldKeys(self):
         """Feature keys of the last frame's contacts: (n, 8), 0xFFFFFFFF past each count."""

@@ -15,8 +15,9 @@
 //  * a contact partner that shares the body's color is read from the
 //    start-of-iteration pose snapshot (the backend closes the reference's
 //    same-color Jacobi race this way; the oracle mirrors the rule).
-// Out of scope here, as in the backend: surface truncation passes (isotropic
-// OGC bounds, Planar-DAT reduce/apply), torus / capsule / hull colliders.
+// Covered: box / sphere / capsule / torus / convex-hull colliders, kinematic
+// spinners, joints, springs, tets, cloth (Planar-DAT and the isotropic OGC
+// limiter), the automatic limiter selection and the CPU path's step order.
 #include "oracle.h"
 
 #include <math.h>
@@ -32,6 +33,18 @@
 #include <map>
 #include <set>
 #include <vector>
+
+// the spinner's per-frame rotation (x, y, z, w) by omega * dt about its unit
+// axis, in float as the host computes it before every frame
+static void oracle_spinner_rotation(const float* axisOmega, float dt, float* dq) {
+  const float angle = axisOmega[3] * dt;
+  const double half = 0.5 * (double)angle;
+  const float sn = (float)std::sin(half), cs = (float)std::cos(half);
+  dq[0] = axisOmega[0] * sn;
+  dq[1] = axisOmega[1] * sn;
+  dq[2] = axisOmega[2] * sn;
+  dq[3] = cs;
+}
 
 namespace {
 
@@ -266,6 +279,13 @@ struct Oracle : oracle_base {
   std::vector<V<T>> cSize, cLocal;
   std::vector<Q<T>> cLocalRot;
   std::vector<T> cRad, cStatF, cDynF;
+  // kinematic spinners (GPUSolver.swift:628-637): spinner k turns body
+  // spinBody[k] about a unit axis at omega rad/s; spinW / spinOn are the
+  // per-body angular velocity the contacts' surface shift reads
+  std::vector<uint32_t> spinBody;
+  std::vector<std::array<float, 4>> spinAxisOmega;
+  std::vector<V<T>> spinW;
+  std::vector<uint8_t> spinOn;
   std::vector<uint32_t> hashed, globals;
   uint32_t nHashedRigid = 0;
   float cellSize = 0.1f;
@@ -364,6 +384,14 @@ struct Oracle : oracle_base {
         ix = iy = iPerp;
         iz = iAxis;
         r = L / 2.0f + rr;
+      } else if (sh == AVBD_SHAPE_TORUS) {
+        // GPUSolver.swift:933-939: solid torus, spine radius R, tube radius r
+        float R = sx, rr = sy;
+        m = den > 0 ? 2.0f * pi * pi * R * rr * rr * den : 0.0f;
+        float iDia = m * (R * R / 2.0f + 5.0f * rr * rr / 8.0f), iAxis = m * (R * R + 3.0f * rr * rr / 4.0f);
+        ix = iy = iDia;
+        iz = iAxis;
+        r = R + rr;
       } else {
         return 4;
       }
@@ -400,6 +428,7 @@ struct Oracle : oracle_base {
       float r;
       if (sh == AVBD_SHAPE_SPHERE) r = sx / 2.0f;
       else if (sh == AVBD_SHAPE_CAPSULE) r = sx / 2.0f + sy;
+      else if (sh == AVBD_SHAPE_TORUS) r = sx + sy;
       else { float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f; r = std::sqrt(hx * hx + hy * hy + hz * hz); }
       const uint32_t hn = d->collider_hull_count ? d->collider_hull_count[i] : 0u;
       if (hn) {  // GPUSolver.swift:994-997: the farthest vertex bounds the hull
@@ -451,6 +480,18 @@ struct Oracle : oracle_base {
       if (!particle[cOwner[i]]) { nHashedRigid++; hashedRigidList.push_back(i); }
     maxPairs = std::max<uint32_t>(4096u, enabled * (uint32_t)maxPairsPerBody);
     hashSize = nextPow2(std::max<uint32_t>(64u, 2u * std::max<uint32_t>(1u, enabled)));
+    // spinners: the last spinner of a body sets its surface angular velocity
+    spinW.assign(nb, V<T>());
+    spinOn.assign(nb, 0);
+    for (uint32_t k = 0; k < d->num_spinners; k++) {
+      const uint32_t b = d->spinner_body[k];
+      if (b >= nb) return 5;
+      const float* a = d->spinner_axis_omega + 4 * k;
+      spinBody.push_back(b);
+      spinAxisOmega.push_back({a[0], a[1], a[2], a[3]});
+      spinW[b] = V<T>(a[0] * a[3], a[1] * a[3], a[2] * a[3]);
+      spinOn[b] = 1;
+    }
     // joints (GPUSolver.swift:1092-1193)
     for (uint32_t i = 0; i < d->num_joints; i++) {
       Joint<T> j{};
@@ -995,6 +1036,22 @@ struct Oracle : oracle_base {
     T ap = std::max(T(0), -dotv(rv, n));
     return (T)set.collision_margin + std::min(ap * (T)set.dt, specCap(ra, rb));
   }
+  // kinematic spinner surface motion as a tangential target: the surface
+  // displacement of each spinning body over the step, pre-divided by
+  // (1 - alpha) because the solver discounts C0 by alpha
+  // (30_narrowphase.metal:274-284); offA / offB are the contact points
+  // relative to the body origins
+  void spinShift(uint32_t ba, uint32_t bb, const V<T>& offA, const V<T>& offB, const V<T>& t1, const V<T>& t2,
+                 V<T>& C0) const {
+    if (spinBody.empty()) return;
+    const T dt = (T)set.dt;
+    V<T> sA = spinOn[ba] ? crossv(spinW[ba], offA) * dt : V<T>();
+    V<T> sB = spinOn[bb] ? crossv(spinW[bb], offB) * dt : V<T>();
+    V<T> rel = sB - sA;
+    T k = T(1) / std::max(T(1) - (T)set.alpha, T(0.01));
+    C0.y = C0.y - dotv(t1, rel) * k;
+    C0.z = C0.z - dotv(t2, rel) * k;
+  }
   V<T> penFloor(uint32_t a, uint32_t b) const {
     T big = std::numeric_limits<T>::max();
     T mA = mass[a], mB = mass[b];
@@ -1098,6 +1155,198 @@ struct Oracle : oracle_base {
     e1 = c + b.ax[k] * half(b, k);
   }
 
+  // ---- torus pairs (30_narrowphase.metal:292-336, 749-965) ----
+  // The torus is the set of points within r of its spine, the circle of
+  // radius R about the collider's local z axis through its center.
+  struct Torus { V<T> c, axis, u, v; T R, r; };
+  static Torus torusAt(const V<T>& p, const Q<T>& q, T R, T r) {
+    Torus t;
+    t.c = p;
+    t.axis = qrot(q, V<T>(0, 0, 1));
+    V<T> u = std::fabs(t.axis.x) > std::fabs(t.axis.z) ? V<T>(-t.axis.y, t.axis.x, 0) : V<T>(0, -t.axis.z, t.axis.y);
+    t.u = normv(u);
+    t.v = crossv(t.axis, t.u);
+    t.R = R;
+    t.r = r;
+    return t;
+  }
+  // nearest spine point (the spine point along u for points on the axis)
+  static V<T> spineNearest(const Torus& t, const V<T>& p) {
+    V<T> d = p - t.c;
+    d = d - t.axis * dotv(t.axis, d);
+    T l = lenv(d);
+    return l < T(1e-6) ? t.c + t.u * t.R : t.c + d / l * t.R;
+  }
+  // cos / sin of the 32 spine sample angles 2 pi i / 32, rounded to float
+  static const float* torusTrig() {
+    static float tab[64];
+    static bool done = false;
+    if (!done) {
+      for (int i = 0; i < 32; i++) {
+        const double a = 6.283185307179586476925 * (double)i / 32.0;
+        tab[2 * i] = (float)std::cos(a);
+        tab[2 * i + 1] = (float)std::sin(a);
+      }
+      done = true;
+    }
+    return tab;
+  }
+  void torusPair(uint32_t ba, uint32_t bb, uint32_t ia, uint32_t ib, bool tIsA, const V<T>& cA, const Q<T>& qA,
+                 const V<T>& cB, const Q<T>& qB, T statF, T dynF, Manifold<T>& out) {
+    struct Hit { V<T> xT, xO, n; T sep; uint32_t f; };  // n: other -> torus
+    Hit hits[4];
+    int nHits = 0;
+    const uint32_t it = tIsA ? ia : ib, io = tIsA ? ib : ia;
+    const Q<T> qO = tIsA ? qB : qA;
+    const V<T> pT = tIsA ? cA : cB, pO = tIsA ? cB : cA;
+    const Torus tor = torusAt(pT, tIsA ? qA : qB, cSize[it].x, cSize[it].y);
+    const uint32_t other = cFlags[io] & 0xF;
+    // detection reach: fixed, a fraction of the tube radius
+    const T detectM = std::max((T)set.collision_margin, T(0.8) * tor.r);
+    if (other == 1) {
+      // torus - sphere: exact, through the nearest spine point
+      const T rs = cSize[io].x * T(0.5);
+      V<T> spine = spineNearest(tor, pO);
+      V<T> d = pO - spine;
+      T dist = lenv(d);
+      if (dist <= tor.r + rs + detectM) {
+        V<T> n = dist > T(1e-6) ? d / dist : V<T>(0, 0, 1);  // torus -> sphere
+        hits[0] = Hit{spine + n * tor.r, pO - n * rs, -n, dist - tor.r - rs, 0u};
+        nHits = 1;
+      }
+    } else {
+      // box / capsule / torus: 32 spine samples, each with its exact nearest
+      // point on the other shape; the deepest samples at least 4 apart are
+      // refined by two alternating projections
+      const bool isBox = other == 0, isCap = other == 3;
+      const Q<T> qOc = qconj(qO);
+      const V<T> halfO = cSize[io] * T(0.5);
+      T otherR = 0;
+      V<T> s0, s1;
+      Torus t2{};
+      if (isCap) {
+        const T ch = cSize[io].x * T(0.5);
+        V<T> ax = qrot(qO, V<T>(0, 0, 1));
+        s0 = pO - ax * ch;
+        s1 = pO + ax * ch;
+        otherR = cSize[io].y;
+      } else if (!isBox) {
+        t2 = torusAt(pO, qO, cSize[io].x, cSize[io].y);
+        otherR = t2.r;
+      }
+      auto nearestOther = [&](const V<T>& q) -> V<T> {
+        if (isBox) {
+          V<T> l = qrot(qOc, q - pO);
+          V<T> cl(clampT(l.x, -halfO.x, halfO.x), clampT(l.y, -halfO.y, halfO.y), clampT(l.z, -halfO.z, halfO.z));
+          return qrot(qO, cl) + pO;
+        }
+        if (isCap) {
+          V<T> ab = s1 - s0;
+          T t = clampT(dotv(q - s0, ab) / std::max(dotv(ab, ab), T(1e-12)), T(0), T(1));
+          return s0 + ab * t;
+        }
+        return spineNearest(t2, q);
+      };
+      const float* trig = torusTrig();
+      auto sample = [&](int i) { return tor.c + (tor.u * (T)trig[2 * i] + tor.v * (T)trig[2 * i + 1]) * tor.R; };
+      T dists[32];
+      bool used[32];
+      for (int i = 0; i < 32; i++) {
+        V<T> q = sample(i);
+        dists[i] = lenv(q - nearestOther(q));
+        used[i] = false;
+      }
+      for (int pick = 0; pick < 4; pick++) {
+        int best = -1;
+        T bd = std::numeric_limits<T>::max();
+        for (int i = 0; i < 32; i++)
+          if (!used[i] && dists[i] < bd) { bd = dists[i]; best = i; }
+        if (best < 0 || bd - tor.r - otherR > detectM) break;
+        for (int k = -3; k <= 3; k++) used[(best + k + 32) % 32] = true;
+        V<T> q = sample(best);
+        V<T> b = nearestOther(q);
+        for (int r2 = 0; r2 < 2; r2++) {
+          q = spineNearest(tor, b);
+          b = nearestOther(q);
+        }
+        V<T> d = q - b;
+        T dist = lenv(d);
+        V<T> n, xO = b;
+        if (dist < T(1e-6)) {
+          if (isBox) {
+            // spine point inside the box: out through the nearest face
+            V<T> lb = qrot(qOc, q - pO);
+            V<T> pen = halfO - V<T>(std::fabs(lb.x), std::fabs(lb.y), std::fabs(lb.z));
+            int ax = 0;
+            if (pen.y < pen[ax]) ax = 1;
+            if (pen.z < pen[ax]) ax = 2;
+            V<T> nl;
+            nl[ax] = lb[ax] >= 0 ? T(1) : T(-1);
+            n = qrot(qO, nl);
+            xO = q + n * pen[ax];
+          } else if (isCap) {
+            n = normv(crossv(tor.axis, normv(s1 - s0)) + V<T>(T(1e-4), 0, 0));
+          } else {
+            n = normv(crossv(tor.axis, t2.axis) + V<T>(T(1e-4), 0, 0));
+          }
+        } else {
+          if (dist - tor.r - otherR > detectM) continue;
+          n = d / dist;  // other -> torus
+        }
+        hits[nHits++] = Hit{q - n * tor.r, isBox ? xO : xO + n * otherR, n, dist - tor.r - otherR, (uint32_t)best};
+      }
+    }
+    if (nHits == 0) return;
+    int deep = 0;
+    T minSep = std::numeric_limits<T>::max();
+    for (int h = 0; h < nHits; h++)
+      if (hits[h].sep < minSep) { minSep = hits[h].sep; deep = h; }
+    const V<T> nrm = tIsA ? hits[deep].n : -hits[deep].n;  // B -> A
+    V<T> t1, t2;
+    basis(nrm, t1, t2);
+    auto pit = prevMap.find({ia, ib});
+    const int prev = pit == prevMap.end() ? -1 : pit->second;
+    const bool roundA = (cFlags[ia] & 0x20) != 0, roundB = (cFlags[ib] & 0x20) != 0;
+    out.n = (uint32_t)nHits;
+    out.flags = 1u | (roundA ? 2u : 0u) | (roundB ? 4u : 0u);
+    out.nrm = nrm; out.t1 = t1; out.dynF = dynF; out.statF = statF;
+    const T warm = (T)set.alpha * (T)set.gamma;
+    const V<T> fl = penFloor(ba, bb);
+    const T gam = set.gamma;
+    for (int i = 0; i < nHits; i++) {
+      const V<T> xAw = tIsA ? hits[i].xT : hits[i].xO, xBw = tIsA ? hits[i].xO : hits[i].xT;
+      const V<T> rA_ = roundA ? xAw - pos[ba] : qrot(qconj(rot[ba]), xAw - pos[ba]);
+      const V<T> rB_ = roundB ? xBw - pos[bb] : qrot(qconj(rot[bb]), xBw - pos[bb]);
+      V<T> lam, pen;
+      if (prev >= 0) {
+        // nearest previous contact on A within 1.1 R carries its multipliers,
+        // the tangential part turned into this frame's basis; never sticks
+        const Manifold<T>& pm = prevManifolds[prev];
+        T bestD = T(1.1) * cSize[it].x;
+        int bj = -1;
+        for (uint32_t j = 0; j < pm.n; j++) {
+          T dj = lenv(pm.c[j].rA - rA_);
+          if (dj < bestD) { bestD = dj; bj = (int)j; }
+        }
+        if (bj >= 0) {
+          lam = pm.c[bj].lambda;
+          pen = pm.c[bj].penalty;
+          V<T> t2o = crossv(pm.nrm, pm.t1);
+          V<T> lt = pm.t1 * lam.y + t2o * lam.z;
+          lam.y = dotv(t1, lt);
+          lam.z = dotv(t2, lt);
+        }
+      }
+      const V<T> d = xAw - xBw;
+      Contact<T>& c = out.c[i];
+      c.rA = rA_; c.rB = rB_; c.feature = hits[i].f; c.stick = 0;
+      c.C0 = V<T>(dotv(nrm, d) + (T)set.collision_margin, dotv(t1, d), dotv(t2, d));
+      spinShift(ba, bb, xAw - pos[ba], xBw - pos[bb], t1, t2, c.C0);
+      c.lambda = lam * warm;
+      c.penalty = V<T>(clampT(pen.x * gam, fl.x, T(1e10)), clampT(pen.y * gam, fl.y, T(1e6)), clampT(pen.z * gam, fl.z, T(1e6)));
+    }
+  }
+
   void collide(uint32_t pi, Manifold<T>& out) {
     uint32_t ia = pairs[pi].first, ib = pairs[pi].second;
     uint32_t ba = cOwner[ia], bb = cOwner[ib];
@@ -1129,6 +1378,11 @@ struct Oracle : oracle_base {
     bool hullA = (cFlags[ia] & 0xF) == 4, hullB = (cFlags[ib] & 0xF) == 4;
     // hulls collide with boxes only (30_narrowphase.metal:404-410)
     if ((hullA || hullB) && ((hullA && hullB) || (cFlags[hullA ? ib : ia] & 0xF) != 0)) return;
+    bool torA = (cFlags[ia] & 0xF) == 2, torB = (cFlags[ib] & 0xF) == 2;
+    if (torA || torB) {
+      torusPair(ba, bb, ia, ib, torA, A.c, qA, B.c, qB, statF, dynF, out);
+      return;
+    }
     if (capA || capB || sphA || sphB) {
       T rA = cSize[ia].x * T(0.5), rB = cSize[ib].x * T(0.5);
       T bestDRound;
@@ -1297,6 +1551,7 @@ struct Oracle : oracle_base {
         Contact<T>& c = out.c[i];
         c.rA = rA_; c.rB = rB_; c.feature = cs[i].f; c.stick = stick;
         c.C0 = V<T>(dotv(nrm, d) + (T)set.collision_margin, dotv(t1, d), dotv(t2, d));
+        spinShift(ba, bb, xAw - pos[ba], xBw - pos[bb], t1, t2, c.C0);
         c.lambda = lam * warm;
         c.penalty = warmPen(pen);
       }
@@ -1469,10 +1724,12 @@ struct Oracle : oracle_base {
           }
         }
       }
-      V<T> d = xformv(pos[ba], rot[ba], rA) - xformv(pos[bb], rot[bb], rB);
+      V<T> xAw = xformv(pos[ba], rot[ba], rA), xBw = xformv(pos[bb], rot[bb], rB);
+      V<T> d = xAw - xBw;
       Contact<T>& c = out.c[i];
       c.rA = rA; c.rB = rB; c.feature = cs[i].f; c.stick = stick;
       c.C0 = V<T>(dotv(nrm, d) + (T)set.collision_margin, dotv(t1, d), dotv(t2, d));
+      spinShift(ba, bb, xAw - pos[ba], xBw - pos[bb], t1, t2, c.C0);
       c.lambda = lam * warm;
       c.penalty = warmPen(pen);
     }
@@ -1947,7 +2204,16 @@ struct Oracle : oracle_base {
         uint32_t fid[4];
         int nf = 0;
         if (kind == 1) { fw[0] = cen; fr[0] = cSize[col].x * T(0.5); fid[0] = 8; nf = 1; }
-        else if (kind == 0) {
+        else if (kind == 3) {
+          // capsule: spheres at its two ends and middle (25_cloth.metal:957-970)
+          T half = cSize[col].x * T(0.5), rc = cSize[col].y;
+          V<T> ax = qrot(cr, V<T>(0, 0, 1));
+          for (int i = -1; i <= 1; i++) {
+            V<T> w = cen + ax * (half * T(i));
+            if (lenv(w - m) > rT + rc + T(4) * margin) continue;
+            fw[nf] = w; fr[nf] = rc; fid[nf] = (uint32_t)(10 + i); nf++;
+          }
+        } else if (kind == 0) {
           V<T> h3 = cSize[col] * T(0.5);
           for (uint32_t i = 0; i < 8; i++) {
             V<T> loc((i & 1) ? h3.x : -h3.x, (i & 2) ? h3.y : -h3.y, (i & 4) ? h3.z : -h3.z);
@@ -3152,6 +3418,16 @@ struct Oracle : oracle_base {
         t = t - twoPi * std::round(t / twoPi);
         j.motorTarget = (T)t;
       }
+    // kinematic spinners turn before the frame: q <- normalize(dq q), dq the
+    // rotation by omega dt about the spinner axis (GPUSolver.swift:4418-4427)
+    for (size_t k = 0; k < spinBody.size(); k++) {
+      float dq[4];
+      oracle_spinner_rotation(spinAxisOmega[k].data(), set.dt, dq);
+      Q<T>& q = rot[spinBody[k]];
+      Q<T> r = qmul(Q<T>{(T)dq[0], (T)dq[1], (T)dq[2], (T)dq[3]}, q);
+      T sc = T(1) / std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w);
+      q = Q<T>{r.x * sc, r.y * sc, r.z * sc, r.w * sc};
+    }
     frame++;
     broadphase();
     manifolds.assign(pairs.size(), Manifold<T>{});
@@ -3446,6 +3722,37 @@ int oracle_get_manifolds(oracle_solver* o, uint32_t cap, uint32_t* n, uint32_t* 
       if (h3) { h3[3 * i] = ms[i].a; h3[3 * i + 1] = ms[i].b; h3[3 * i + 2] = ms[i].n; }
       if (n3) { n3[3 * i] = (float)ms[i].nrm.x; n3[3 * i + 1] = (float)ms[i].nrm.y; n3[3 * i + 2] = (float)ms[i].nrm.z; }
       if (lp) { lp[2 * i] = ms[i].n ? (float)ms[i].c[0].lambda.x : 0.0f; lp[2 * i + 1] = ms[i].n ? (float)ms[i].c[0].penalty.x : 0.0f; }
+    }
+  };
+  if (o->f) go(o->f); else go(o->d);
+  return 0;
+}
+// the last frame's manifolds as 704-byte records: header (a, b, count,
+// flags), collider pair, (normal, dynamic friction), (t1, static friction),
+// then 8 contacts of (rA, feature bits), (rB, stick), C0, lambda, penalty
+int oracle_get_manifold_records(oracle_solver* o, uint32_t cap, uint32_t* n, void* records) {
+  auto go = [&](auto* s) {
+    const auto& ms = s->prevManifolds;
+    *n = (uint32_t)ms.size();
+    for (uint32_t i = 0; i < *n && i < cap && records; i++) {
+      uint32_t* u = (uint32_t*)records + 176 * i;
+      float* f = (float*)u;
+      const auto& m = ms[i];
+      u[0] = m.a; u[1] = m.b; u[2] = m.n; u[3] = m.flags;
+      u[4] = m.ca; u[5] = m.cb; u[6] = 0; u[7] = 0;
+      f[8] = (float)m.nrm.x; f[9] = (float)m.nrm.y; f[10] = (float)m.nrm.z; f[11] = (float)m.dynF;
+      f[12] = (float)m.t1.x; f[13] = (float)m.t1.y; f[14] = (float)m.t1.z; f[15] = (float)m.statF;
+      for (uint32_t k = 0; k < 8; k++) {
+        float* c = f + 16 + 20 * k;
+        for (int j = 0; j < 20; j++) c[j] = 0.0f;
+        if (k >= m.n) continue;
+        const auto& q = m.c[k];
+        c[0] = (float)q.rA.x; c[1] = (float)q.rA.y; c[2] = (float)q.rA.z; memcpy(&c[3], &q.feature, 4);
+        c[4] = (float)q.rB.x; c[5] = (float)q.rB.y; c[6] = (float)q.rB.z; c[7] = (float)q.stick;
+        c[8] = (float)q.C0.x; c[9] = (float)q.C0.y; c[10] = (float)q.C0.z;
+        c[12] = (float)q.lambda.x; c[13] = (float)q.lambda.y; c[14] = (float)q.lambda.z;
+        c[16] = (float)q.penalty.x; c[17] = (float)q.penalty.y; c[18] = (float)q.penalty.z;
+      }
     }
   };
   if (o->f) go(o->f); else go(o->d);

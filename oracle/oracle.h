@@ -37,6 +37,7 @@ int oracle_get_cell_lists(oracle_solver* o, uint32_t* hash_size, uint32_t* num_h
                           uint32_t* cell_start, uint32_t* cell_count, uint32_t* cell_bodies);
 int oracle_get_pairs(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* pairs);
 /* manifold summary of the last frame: per pair (bodyA, bodyB, numContacts) and contact normals */
+int oracle_get_manifold_records(oracle_solver* o, uint32_t capacity, uint32_t* n, void* records704);
 int oracle_get_manifold_keys(oracle_solver* o, uint32_t capacity, uint32_t* n, uint32_t* keys8);
 /* isotropic limiter state after the last frame: tracked triangles [4 nb],
  * tracked edges [4 num_contact_edges], squared clearance bounds [nb], anchor

@@ -162,3 +162,45 @@ def hull_wedge():
     s.addConvexCollider(body, wedge, friction=0.8, localPosition=(0.05, 0, 0))
     sph = s.addSphere(0.3, 1.0, 0.5, (0.05, 0.0, 0.75))
     return s, body, sph
+
+
+def triangle_over_capsule(x=0.5, gap=0.003, side=0.08, rp=0.002, rc=0.1):
+    """One free particle triangle (flat, edge `side`, particle radius rp) with
+    its centroid above x on a static capsule along x (length 1, radius rc,
+    axis at z = 0.5), its plane `gap` above the capsule top; zero gravity.
+    The capsule's rigid features against the triangle are the spheres at its
+    two ends (x = -0.5, 0.5) and middle (x = 0)."""
+    s = PhysicsScene("triangle over capsule", SimSettings(gravity=0.0, iterations=4))
+    cap = s.addCapsule(1.0, rc, 0.0, 0.6, (0, 0, 0.5), rotation=ALONG_X)
+    z = 0.5 + rc + gap
+    h = side * math.sqrt(3.0) / 2
+    ids = [s.addParticle(rp, 0.001, (x + dx, dy, z)) for dx, dy in ((-side / 2, -h / 3), (side / 2, -h / 3),
+                                                                     (0.0, 2 * h / 3))]
+    s.addTri(tuple(ids))
+    return s, cap, ids
+
+
+def sheet_over_capsule(n=14, size=0.5, rp=0.003):
+    """A particle sheet (springs + collision triangles) dropping across a
+    static capsule bar and onto the ground."""
+    s = PhysicsScene("sheet over capsule", SimSettings(iterations=20, particleDamping=1.5))
+    s.addBody((20, 20, 2), 0.0, 0.6, (0, 0, -1))
+    s.addCapsule(0.8, 0.06, 0.0, 0.6, (0, 0, 0.1), rotation=ALONG_X)
+    h = size / (n - 1)
+    base = len(s.bodies)
+    for i in range(n):
+        for j in range(n):
+            s.addParticle(rp, 0.3 * h * h, (i * h - size / 2, j * h - size / 2, 0.2), friction=0.6)
+    idx = lambda i, j: base + i * n + j
+    for i in range(n):
+        for j in range(n):
+            if i + 1 < n:
+                s.addSpring(idx(i, j), idx(i + 1, j), (0, 0, 0), (0, 0, 0), 1e4)
+            if j + 1 < n:
+                s.addSpring(idx(i, j), idx(i, j + 1), (0, 0, 0), (0, 0, 0), 1e4)
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a, b, c, d = idx(i, j), idx(i + 1, j), idx(i, j + 1), idx(i + 1, j + 1)
+            s.addTri((a, b, d), 300, 300, 1e-3)
+            s.addTri((a, d, c), 300, 300, 1e-3)
+    return s

@@ -141,3 +141,30 @@ def test_hull_pairs_other_than_box_are_refused():
     h, _, _ = o.manifolds()
     ground_contacts = [int(n) for a, b, n in h if {int(a), int(b)} == {0, body}]
     assert ground_contacts and 2 <= ground_contacts[0] <= 8
+
+
+def _rt_contacts(o):
+    sc = o.softContacts()
+    keyA = sc["lambda"][:, 3].view(np.uint32)
+    return sc[(keyA >> 28) == 2]
+
+
+def test_capsule_rigid_features_against_a_triangle():
+    """25_cloth.metal:957-970: a capsule meets cloth triangles through three
+    spheres of its radius, at its ends (features 9, 11) and middle (10);
+    a sphere is used when its center is within the triangle's reach."""
+    for x, feat in ((0.5, 11), (-0.5, 9), (0.0, 10)):
+        gap, rp, rc = 0.003, 0.002, 0.1
+        s, cap, ids = F.triangle_over_capsule(x=x, gap=gap, rp=rp, rc=rc)
+        o = oracle.OracleSolver(s)
+        o.step()
+        rt = _rt_contacts(o)
+        assert len(rt) == 1, (x, len(rt))
+        c = rt[0]
+        assert c["lambda"][3:4].view(np.uint32)[0] == (2 << 28) | cap
+        assert c["penalty"][3:4].view(np.uint32)[0] == 0 * 16 + feat
+        assert list(c["ids"]) == [cap] + ids
+        np.testing.assert_allclose(abs(float(c["normal"][2])), 1.0, atol=1e-6)
+        # gap between the feature sphere and the triangle's particle shell
+        margin = o.clothInfo()["elem_margin"]
+        np.testing.assert_allclose(float(c["C0"][0]), gap - rp + margin, atol=2e-6)

@@ -150,3 +150,45 @@ def test_convex_hull_contacts():
     err = float(np.abs(g.positions() - o.positions()).max())
     flush_print(f"  wedge after 90 steps: max|x_gpu - x_oracle| = {err:.3e} m")
     assert err <= 1e-3
+
+
+def test_capsule_rigid_features_against_cloth():
+    _case("capsule rigid features vs cloth triangles (25_cloth.metal:957-970): soft contacts vs the oracle")
+    for x in (0.5, -0.5, 0.0):
+        s, cap, ids = F.triangle_over_capsule(x=x)
+        with GPUSolver(s) as g:
+            o = oracle.OracleSolver(s)
+            g.step()
+            o.step()
+            sg, so = g.softContacts(), o.softContacts()
+            assert len(sg) == len(so) == 1
+            assert np.array_equal(sg["ids"], so["ids"])
+            for f in ("lambda", "penalty", "anchorA"):
+                assert np.array_equal(sg[f][:, 3].view(np.uint32), so[f][:, 3].view(np.uint32)), f
+            np.testing.assert_allclose(sg["C0"], so["C0"], rtol=1e-5, atol=1e-7)
+    s = F.sheet_over_capsule()
+    src = oracle.OracleSolver(s)
+    src.step(40)
+    p, r, v, w = src.bodyStates()
+    idx = np.arange(len(s.bodies), dtype=np.uint32)
+    with GPUSolver(s) as g:
+        o = oracle.OracleSolver(s)
+        g.setBodyStates(idx, p, r, v, w)
+        o.setBodyStates(idx, p, r, v, w)
+        st = g.settings
+        st.iterations = 1
+        g.settings = st
+        o.set_settings(st)
+        g.step()
+        o.step()
+        sg, so = g.softContacts(), o.softContacts()
+        keyA = so["lambda"][:, 3].view(np.uint32)
+        flush_print(f"  sheet: {len(so)} element contacts, {int(((keyA >> 28) == 2).sum())} capsule-feature contacts")
+        assert int(((keyA >> 28) == 2).sum()) > 0
+        assert len(sg) == len(so)
+        assert np.array_equal(sg["ids"], so["ids"])
+        for f in ("lambda", "penalty", "anchorA"):
+            assert np.array_equal(sg[f][:, 3].view(np.uint32), so[f][:, 3].view(np.uint32)), f
+        err = float(np.abs(g.positions() - o.positions()).max())
+        flush_print(f"  max|x_gpu - x_oracle| = {err:.3e} m")
+        assert err <= 1e-5

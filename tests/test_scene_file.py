@@ -17,8 +17,9 @@ from avbd_amd import PhysicsScene, SceneFile, SceneJoint, SimSettings, _capi, de
 from avbd_amd.scene import CAPSULE, SPHERE
 
 FIELDS = [f for f, t in _capi.SceneDesc._fields_ if f.startswith(("body_", "collider_", "joint_", "spring_", "tet_",
-                                                                   "tri_", "exclusion_", "hull_"))]
-COUNTS = ["num_bodies", "num_colliders", "num_joints", "num_springs", "num_tets", "num_tris", "num_exclusions"]
+                                                                   "tri_", "exclusion_", "hull_", "spinner_"))]
+COUNTS = ["num_bodies", "num_colliders", "num_joints", "num_springs", "num_tets", "num_tris", "num_exclusions",
+          "num_spinners"]
 
 
 def _everything():
@@ -38,6 +39,8 @@ def _everything():
     s.addTri((p[0], p[1], p[4]), mu=100.0, bend=1e-3)
     s.addTri((p[0], p[4], p[3]), mu=100.0, bend=1e-3)
     s.addCollisionExclusion(a, ids[0])
+    t = s.addTorus(0.45, 0.12, 0.0, 0.5, (3, 0, 0.5))
+    s.addSpinner(t, (0, 0, 2), 0.8)
     return s
 
 
@@ -52,11 +55,11 @@ def _arrays(desc):
         base = f.split("_")[0]
         cnt = {"body": n["num_bodies"], "collider": n["num_colliders"], "joint": n["num_joints"],
                "spring": n["num_springs"], "tet": n["num_tets"], "tri": n["num_tris"],
-               "exclusion": n["num_exclusions"]}.get(base)
+               "exclusion": n["num_exclusions"], "spinner": n["num_spinners"]}.get(base)
         width = {"body_size": 3, "body_position": 3, "body_rotation": 4, "body_velocity": 3, "body_inertia": 3,
                  "collider_size": 3, "collider_local_position": 3, "collider_local_rotation": 4, "joint_ra": 3,
                  "joint_rb": 3, "joint_hinge_axis": 3, "joint_motor": 6, "joint_limits": 2, "spring_ra": 3,
-                 "spring_rb": 3, "tet_ids": 4, "tri_ids": 3, "exclusion_pairs": 2}.get(f, 1)
+                 "spring_rb": 3, "tet_ids": 4, "tri_ids": 3, "exclusion_pairs": 2, "spinner_axis_omega": 4}.get(f, 1)
         if f == "hull_vertices":
             hc = desc.collider_hull_count
             cnt, width = sum(hc[i] for i in range(n["num_colliders"])), 3
@@ -84,7 +87,8 @@ def test_round_trip_every_array(tmp_path, name):
             assert a1[k] is not None and a0[k].dtype == a1[k].dtype, k
             assert np.array_equal(a0[k].view(np.uint8), a1[k].view(np.uint8)), k
     if name == "everything":
-        assert all(a0[k] is not None for k in ("hull_vertices", "joint_motor_rate", "tri_bend", "exclusion_pairs"))
+        assert all(a0[k] is not None for k in ("hull_vertices", "joint_motor_rate", "tri_bend", "exclusion_pairs",
+                                               "spinner_axis_omega"))
 
 
 def test_loaded_file_steps_like_the_scene(tmp_path):
@@ -109,7 +113,7 @@ def test_bad_files_are_refused(tmp_path):
         "empty": b"",
     }
     # the first array record's element size (bodies' sizes: 4-byte floats)
-    hdr = 8 + 8 + 60 + 32 + 4
+    hdr = 8 + 8 + 60 + 36 + 4
     cases["element size"] = raw[: hdr + 4] + struct.pack("<I", 8) + raw[hdr + 8:]
     cases["length"] = raw[: hdr + 8] + struct.pack("<Q", 999) + raw[hdr + 16:]
     for name, data in cases.items():
