@@ -621,7 +621,7 @@ __device__ void dat_reduce_pair(const DeviceState& S, const StepParams& P, const
 // one thread per pair of the stream in use; `color` = kDatFullPass after
 // the predictor, else the static color whose primal just ran
 __global__ void __launch_bounds__(256) k_dat_reduce(DeviceState S, StepParams P, const F4* anchor, uint32_t color) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
   const uint32_t total = S.datStart[P.numParticles + P.numEdges];
   const uint32_t n = total < P.maxDatPairs ? total : P.maxDatPairs;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -632,7 +632,7 @@ __global__ void __launch_bounds__(256) k_dat_reduce(DeviceState S, StepParams P,
 // snapshot); color pass: the color's bodies from colorWork[start, start+count)
 __global__ void __launch_bounds__(256) k_dat_apply(DeviceState S, StepParams P, const F4* anchor, uint32_t full,
                                                    uint32_t start, uint32_t count) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t v;
   if (full) {
@@ -1572,7 +1572,7 @@ __global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P) 
 // itself in; the last one to arrive clears both counters for the next
 // iteration.
 __global__ void __launch_bounds__(256) k_ogc_refresh(DeviceState S, StepParams P) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
   __shared__ uint32_t fire;
   if (threadIdx.x == 0) {
     const uint32_t exceeded = __hip_atomic_load(&S.counters[CTR_OGC], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

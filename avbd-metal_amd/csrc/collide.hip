@@ -200,11 +200,9 @@ constexpr uint32_t kFineCap = 128;  // accepted candidates one wave sorts in LDS
 // accepted partner lies within the radius sum, hence in an adjacent hash
 // cell.  More than kFineCap accepted candidates fall back to the walk.
 template <bool EMIT>
-__global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
-  __shared__ uint64_t fineKeys[4][2][kFineCap];
+__device__ __forceinline__ void bp_producer(const DeviceState& S, const StepParams& P, uint32_t producer,
+                                            uint64_t (*fineKeys)[2][kFineCap]) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t producer = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (producer >= P.numHashed + P.numGlobals) return;  // whole wave
   const uint64_t ltMask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
   const uint32_t base = EMIT ? S.pairStart[producer] : 0u;
   uint32_t count = 0;
@@ -403,6 +401,78 @@ __global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
   if (!EMIT && lane == 0) S.pairCount[producer] = count;
 }
 
+// The wave path: one wave per producer of bpWave (the producers not on the
+// lane list: globals, and hashed producers of dense packings).
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
+  __shared__ uint64_t fineKeys[4][2][kFineCap];
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= P.numBpWave) return;  // whole wave
+  bp_producer<EMIT>(S, P, S.bpWave[w], fineKeys);
+}
+
+// Hashed producers whose candidates are few, one lane each, with the wave
+// path's filters and order: its 27 neighbour cells in (dz, dy, dx) order (a
+// hash met earlier in that order is not walked again), each bucket in
+// collider order, then the globals.  The lane list (built at create) holds
+// every hashed producer when the hash buckets are small (no search grid), and
+// otherwise the soft-group particles of scenes without hashed non-particle
+// colliders, which pair with globals only; dense packings stay on the wave
+// path, where 64 lanes share a producer's candidates.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_bp_lane(DeviceState S, StepParams P) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.numBpLane) return;
+  const uint32_t producer = S.bpLane[i];
+  const uint32_t a = S.hashedIdx[producer];
+  const U4 ma = S.hashMeta[producer];
+  const F4 ca = S.hashCenter[producer];
+  const uint32_t ownerA = ma.x, grpA = ma.y, shA = ma.z & 1u, ga = ma.w;
+  const bool aDyn = (ma.z & 2u) != 0;
+  const V3 pa = xyz(ca);
+  const float ra = ca.w;
+  const uint32_t base = EMIT ? S.pairStart[producer] : 0u;
+  uint32_t count = 0;
+  auto emit = [&](uint32_t x, uint32_t y) {
+    if (EMIT && base + count < P.maxPairs) S.pairs[base + count] = U2{x, y};
+    count++;
+  };
+  if (!(ga != 0 && P.numHashedRigid == 0)) {
+    int cx, cy, cz;
+    cell_coord(pa, P.cellSize, cx, cy, cz);
+    uint32_t h[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) h[q] = cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, grpA, P.gridHashSize);
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      bool dup = false;
+#pragma unroll
+      for (int j = 0; j < q; j++) dup |= h[j] == h[q];
+      if (dup || (ga != 0 && S.cellRigid[h[q]] == 0)) continue;
+      const uint32_t cs = S.cellStart[h[q]], ce = cs + S.cellCount[h[q]];
+      for (uint32_t k = cs; k < ce; k++) {
+        const uint32_t b = S.cellBodies[k];
+        if (b <= a) continue;
+        const U4 mb = S.cellMeta[k];
+        const F4 cb = S.cellPos[k];
+        if (!(ga != 0 && mb.w != 0) &&
+            pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, mb.x, mb.y, mb.z & 1u, (mb.z & 2u) != 0, xyz(cb),
+                        cb.w))
+          emit(a, b);
+      }
+    }
+  }
+  for (uint32_t g = 0; g < P.numGlobals; g++) {
+    const uint32_t b = S.globalIdx[g];
+    const uint32_t ownerB = S.colliderOwner[b];
+    if (!(ga != 0 && S.clothGroup[ownerB] == ga) &&
+        pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, ownerB, S.colliderGroup[b], S.colliderShared[b],
+                    S.posLin[ownerB].w > 0.0f, collider_center(S, b), fabsf(S.colliderShape[b].w)))
+      emit(a < b ? a : b, a < b ? b : a);
+  }
+  if (!EMIT) S.pairCount[producer] = count;
+}
+
 __global__ void k_bp_finalize(DeviceState S, StepParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t producers = P.numHashed + P.numGlobals;
@@ -469,9 +539,15 @@ void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P
   }
   uint32_t producers = P.numHashed + P.numGlobals;
   if (producers > 0) {
-    hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for(producers, 4)), dim3(256), 0, st, S, P);
+    if (P.numBpLane > 0)
+      hipLaunchKernelGGL(k_bp_lane<false>, dim3(blocks_for(P.numBpLane, 256)), dim3(256), 0, st, S, P);
+    if (P.numBpWave > 0)
+      hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for(P.numBpWave, 4)), dim3(256), 0, st, S, P);
     launch_exclusive_scan(st, S.pairCount, S.pairStart, producers, S.scanScratch);
-    hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(producers, 4)), dim3(256), 0, st, S, P);
+    if (P.numBpLane > 0)
+      hipLaunchKernelGGL(k_bp_lane<true>, dim3(blocks_for(P.numBpLane, 256)), dim3(256), 0, st, S, P);
+    if (P.numBpWave > 0)
+      hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(P.numBpWave, 4)), dim3(256), 0, st, S, P);
   }
   hipLaunchKernelGGL(k_bp_finalize, dim3(1), dim3(64), 0, st, S, P);
 }

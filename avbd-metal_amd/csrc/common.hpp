@@ -417,6 +417,11 @@ struct StepParams {
   float particleDamping, rigidLinearDamping, rigidAngularDamping;
   uint32_t numBodies, numColliders, numJoints, numSprings, numTets;
   uint32_t mapCapacity, maxPairs, gridHashSize, numHashed, numGlobals, numHashedRigid;
+  uint32_t numBpLane, numBpWave;  // pair producers on the lane / wave paths
+  // 1 in the iterate-to-residual chunks: sweep kernels first read the
+  // frame's end flag (CTR_CONV_DONE) and exit once it is set; 0 elsewhere,
+  // where the flag is never set, so the kernels skip that dependent load
+  uint32_t convCheck;
   uint32_t frictionCombineMode, numExclusions, iterations;
   uint32_t staticPalette;   // colors fixed for the scene's lifetime (tets / cloth present)
   // cloth surfaces (Planar-DAT pair stream + OGC element contacts)

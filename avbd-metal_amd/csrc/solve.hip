@@ -426,9 +426,9 @@ __device__ __forceinline__ V3 group_sum3(V3 v) {
 
 // Isotropic limiter (40_solver.metal:1361-1382): a particle's total
 // displacement from the pose its bound holds around stays within
-// max(0.45 sqrt(bound), 0.2 r); each truncation is counted for the refresh
-// check and the frame's diagnostics.
-__device__ __forceinline__ V3 ogc_truncate(const DeviceState& S, uint32_t body, V3 cur, V3 dx, float rad) {
+// max(0.45 sqrt(bound), 0.2 r); `cut` reports a truncation (counted for the
+// refresh check and the frame's diagnostics by ogc_count).
+__device__ __forceinline__ V3 ogc_truncate(const DeviceState& S, uint32_t body, V3 cur, V3 dx, float rad, bool& cut) {
   const float d2 = as_float(S.ogcBound[body]);
   if (!(d2 < 1e37f)) return dx;
   const float bv = fmaxf(0.45f * sqrtf(d2), 0.2f * rad);
@@ -438,10 +438,20 @@ __device__ __forceinline__ V3 ogc_truncate(const DeviceState& S, uint32_t body, 
   if (t2 > bv * bv) {
     tot = tot * (bv / sqrtf(t2));
     dx = (anchor + tot) - cur;
-    atomicAdd(&S.counters[CTR_OGC], 1u);
-    atomicAdd(&S.counters[CTR_OGC_TRUNC], 1u);
+    cut = true;
   }
   return dx;
+}
+// The wave's truncations in one pair of atomics (the lanes still running
+// here are the solving lanes of the wave's bodies): per-body atomics on
+// the two shared counters serialize when most particles are held back.
+__device__ __forceinline__ void ogc_count(const DeviceState& S, bool cut) {
+  const uint64_t m = __ballot(cut);
+  if (m != 0ull && __lane_id() == (uint32_t)(__ffsll((unsigned long long)__activemask()) - 1)) {
+    const uint32_t n = (uint32_t)__popcll(m);
+    atomicAdd(&S.counters[CTR_OGC], n);
+    atomicAdd(&S.counters[CTR_OGC_TRUNC], n);
+  }
 }
 
 // Gathers and stamps one body's entries (lane-strided), reduces, and lane 0
@@ -568,7 +578,11 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
   float maxLin = 0.35f * rad;
   float l2 = dot(dxL, dxL);
   if (l2 > maxLin * maxLin) dxL *= maxLin * rsq(l2);
-  if (P.isoOn && !rigid) dxL = ogc_truncate(S, body, xyz(pl), dxL, rad);
+  if (P.isoOn) {
+    bool cut = false;
+    if (!rigid) dxL = ogc_truncate(S, body, xyz(pl), dxL, rad, cut);
+    ogc_count(S, cut);
+  }
   // (hi, lo) + dx, renormalised exactly: hi = f32(hi + t), lo = the error term
   V3 hiN, loN;
   two_sum3(xyz(pl), lo + dxL, hiN, loN);
@@ -595,7 +609,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
 template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uint32_t color, uint32_t count,
                                                 uint32_t base) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;  // the frame's residual check has ended its iterations
   const uint32_t groupsPerBlock = 256 / L;
   uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   uint32_t slot = blk * groupsPerBlock + threadIdx.x / L;
@@ -623,7 +637,7 @@ __global__ void __launch_bounds__(256) k_primal(DeviceState S, StepParams P, uin
 // does not depend on which kernel processed a color.
 template <int L>
 __global__ void __launch_bounds__(256) k_primal_tail(DeviceState S, StepParams P, uint32_t first) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;
   const uint32_t groupsPerBlock = 256 / L;
   const uint32_t lane = threadIdx.x & (L - 1), grp = threadIdx.x / L;
   const uint32_t used = S.colorStart[kMaxColors + 1];
@@ -822,7 +836,7 @@ __device__ __forceinline__ void dual_item(const DeviceState& S, const StepParams
 }
 
 __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;
   uint32_t numPairs = S.counters[CTR_PAIRS];
   uint32_t total = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
   uint32_t stride = gridDim.x * blockDim.x;
@@ -1019,7 +1033,7 @@ void launch_diag_error(hipStream_t st, const DeviceState& S, const StepParams& P
 template <int L, int MODE>
 __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P, uint32_t numColors,
                                                     uint32_t barrierBase) {
-  if (S.counters[CTR_CONV_DONE] != 0u) return;  // uniform across the grid: no barrier is entered
+  if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;  // uniform across the grid: no barrier is entered
   const uint32_t groupsPerBlock = 256 / L;
   const uint32_t lane = threadIdx.x & (L - 1);
   const uint32_t gslot = xcd_block(blockIdx.x, gridDim.x) * groupsPerBlock + threadIdx.x / L;

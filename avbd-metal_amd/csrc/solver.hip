@@ -50,6 +50,7 @@ struct avbd_solver {
   avbd_options opts{};
   uint32_t nb = 0, nc = 0, nj = 0, ns = 0, nt = 0;
   uint32_t nHashed = 0, nGlobal = 0, nHashedRigid = 0, nExcl = 0;
+  uint32_t numBpLane = 0, numBpWave = 0;  // pair producers on the lane / wave paths
   uint32_t maxPairs = 0, mapCapacity = 0, gridHashSize = 0;
   float cellSize = 0.1f;
   float fineCellSize = 0.0f;  // candidate search grid (0 = off)
@@ -226,6 +227,8 @@ StepParams make_params(const avbd_solver* s) {
   P.numHashed = s->nHashed;
   P.numGlobals = s->nGlobal;
   P.numHashedRigid = s->nHashedRigid;
+  P.numBpLane = s->numBpLane;
+  P.numBpWave = s->numBpWave;
   P.frictionCombineMode = s->settings.friction_combine_mode;
   P.numExclusions = s->nExcl;
   P.iterations = (uint32_t)std::max(0, s->settings.iterations);
@@ -1612,6 +1615,19 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.largeCells, hashed.size() / 33 + 1)
   }
   uint32_t producers = s->nHashed + s->nGlobal;
+  // pair producers: one lane each where the candidates are few -- every
+  // hashed producer when buckets are small (no search grid), else the
+  // soft-group particles of scenes with no hashed non-particle collider
+  // (they pair with globals only); one wave each for the rest and the globals
+  std::vector<uint32_t> bpLane, bpWave;
+  for (size_t k = 0; k < hashed.size(); k++) {
+    const bool lane = s->fineCellSize <= 0.0f || (clothGroup[co[hashed[k]]] != 0u && s->nHashedRigid == 0);
+    (lane ? bpLane : bpWave).push_back((uint32_t)k);
+  }
+  for (uint32_t g = 0; g < s->nGlobal; g++) bpWave.push_back(s->nHashed + g);
+  s->numBpLane = (uint32_t)bpLane.size();
+  s->numBpWave = (uint32_t)bpWave.size();
+  ALLOC(S.bpLane, std::max<size_t>(1, bpLane.size())) ALLOC(S.bpWave, std::max<size_t>(1, bpWave.size()))
   ALLOC(S.pairCount, std::max<uint32_t>(1, producers)) ALLOC(S.pairStart, std::max<uint32_t>(1, producers))
   ALLOC(S.pairs, s->maxPairs) ALLOC(S.exclusions, std::max<size_t>(1, exr.size()))
   ALLOC(S.manifolds, s->maxPairs) ALLOC(S.prevManifolds, s->maxPairs)
@@ -1693,7 +1709,9 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     hmeta[k] = U4{b, cg[c], csc[c] | ((pl[b].w > 0.0f) ? 2u : 0u), clothGroup[b]};
     c2h[c] = (uint32_t)k;
   }
-  if (upload(S.hashMeta, hmeta) || upload(S.colliderToHashed, c2h)) return AVBD_ERR_HIP;
+  if (upload(S.hashMeta, hmeta) || upload(S.colliderToHashed, c2h) || upload(S.bpLane, bpLane) ||
+      upload(S.bpWave, bpWave))
+    return AVBD_ERR_HIP;
   if (!s->spinBodies.empty()) {
     // surface angular velocity per body: the last spinner of a body sets it
     std::vector<F4> sv(nb, F4{0.0f, 0.0f, 0.0f, 0.0f});
@@ -2107,9 +2125,11 @@ constexpr uint32_t kChunksInFlight = 3;
 
 hipError_t enqueue_chunk(avbd_solver* s, const DeviceState& S, const StepParams& P, uint32_t n, float tol,
                          uint32_t maxIt) {
-  hipError_t e = enqueue_iterations(s, S, P, n, true);
+  StepParams Pc = P;
+  Pc.convCheck = 1u;  // the chunk's kernels stop once a check has ended the frame
+  hipError_t e = enqueue_iterations(s, S, Pc, n, true);
   if (e != hipSuccess) return e;
-  launch_residual(s->stream, S, P, s->residualBlocks, true);
+  launch_residual(s->stream, S, Pc, s->residualBlocks, true);
   launch_residual_check(s->stream, S, s->residualBlocks, tol, maxIt, n, s->convDev);
   return hipGetLastError();
 }

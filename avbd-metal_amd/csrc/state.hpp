@@ -49,6 +49,9 @@ struct DeviceState {
   U2* fineSlot;
   uint32_t* largeCells;       // buckets of more than 32 colliders (sorted by a workgroup each)
   uint32_t *pairCount, *pairStart;
+  // pair producers by path (built at create): bpLane one lane each, bpWave
+  // one wave each
+  uint32_t *bpLane, *bpWave;
   U2* pairs;
   U2* exclusions;
   // contacts

@@ -24,6 +24,15 @@ for C in FETCH_SIZE WRITE_SIZE; do
   timeout -s KILL 60 rocprofv3 --pmc $C --output-format csv -d $OUT/prof_$C -o copy -- ./avbd-metal_amd/lib/copy_calib > $OUT/prof_${C}_copy.log 2>&1
   rc=$?; tail -1 $OUT/prof_${C}_copy.log; [ -f $OUT/prof_$C/copy_counter_collection.csv ] || exit $rc
 done
+# cross-check of the byte counters: the L2 -> fabric read requests by size
+# class (128 B requests count in TCC_BUBBLE) and the DRAM-bound ones
+XC="TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_BUBBLE_sum TCC_EA0_RDREQ_DRAM_sum"
+echo "pass: read-request counters bench"
+timeout -s KILL 120 rocprofv3 --pmc $XC --output-format csv -d $OUT/prof_XC -o bench -- python3 $BENCH > $OUT/prof_XC.log 2>&1
+rc=$?; tail -1 $OUT/prof_XC.log; [ -f $OUT/prof_XC/bench_counter_collection.csv ] || exit $rc
+echo "pass: read-request counters copy calibration"
+timeout -s KILL 60 rocprofv3 --pmc $XC --output-format csv -d $OUT/prof_XC -o copy -- ./avbd-metal_amd/lib/copy_calib > $OUT/prof_XC_copy.log 2>&1
+rc=$?; tail -1 $OUT/prof_XC_copy.log; [ -f $OUT/prof_XC/copy_counter_collection.csv ] || exit $rc
 echo "pass: wave / LDS counters"
 timeout -s KILL 120 rocprofv3 --pmc SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT GRBM_GUI_ACTIVE \
   --output-format csv -d $OUT/prof_SQ -o bench -- python3 $BENCH > $OUT/prof_SQ.log 2>&1

@@ -76,6 +76,30 @@ def main():
                   "primal_bytes": b * 1024.0 * factor if (b is not None and factor) else None}
         lines.append(f"# {c}: k_primal avg {b} KiB over {nb} dispatches; copy kernel {k} KiB for {COPY_BYTES} "
                      f"bytes -> factor {factor}")
+    xc_dir = os.path.join(out, "prof_XC")
+    if os.path.exists(os.path.join(xc_dir, "bench_counter_collection.csv")):
+        # read requests by size class: bytes = 32 * RDREQ_32B + 128 * BUBBLE
+        # + 64 * (the rest), the expression FETCH_SIZE itself uses; the DRAM
+        # request count beside it, per k_primal launch and for the copy
+        xc = {}
+        for tag, flt in (("bench", "k_primal"), ("copy", "k_copy")):
+            path = os.path.join(xc_dir, f"{tag}_counter_collection.csv")
+            v = {c: counter_avg(path, c, flt)[0] for c in ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum",
+                                                        "TCC_BUBBLE_sum", "TCC_EA0_RDREQ_DRAM_sum")}
+            if None not in v.values():
+                rest = v["TCC_EA0_RDREQ_sum"] - v["TCC_BUBBLE_sum"] - v["TCC_EA0_RDREQ_32B_sum"]
+                v["bytes_by_size_class"] = 32 * v["TCC_EA0_RDREQ_32B_sum"] + 128 * v["TCC_BUBBLE_sum"] + 64 * rest
+                v["dram_requests_x128"] = 128 * v["TCC_EA0_RDREQ_DRAM_sum"]
+            xc["primal" if tag == "bench" else "copy"] = v
+        res["read_request_cross_check"] = xc
+        cp = xc.get("copy", {})
+        if cp.get("bytes_by_size_class"):
+            lines.append(f"# read requests, copy of {COPY_BYTES} B: {cp['bytes_by_size_class']:.0f} B by size class, "
+                         f"{cp['dram_requests_x128']:.0f} B as 128 B DRAM requests")
+        pr = xc.get("primal", {})
+        if pr.get("bytes_by_size_class"):
+            lines.append(f"# read requests, k_primal per launch: {pr['bytes_by_size_class']:.0f} B by size class, "
+                         f"{pr['dram_requests_x128']:.0f} B as 128 B DRAM requests")
     sq_path = os.path.join(out, "prof_SQ", "bench_counter_collection.csv")
     if os.path.exists(sq_path):
         sq = {}

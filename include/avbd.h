@@ -39,7 +39,7 @@ typedef enum avbd_status {
     AVBD_ERR_INVALID_ARGUMENT = 1,  /* precondition violated (Scene.swift preconditions) */
     AVBD_ERR_NO_DEVICE = 2,         /* AVBDError.noDevice          GPUSolver.swift:641 */
     AVBD_ERR_ALLOC = 3,             /* AVBDError.allocFailed       GPUSolver.swift:642 */
-    AVBD_ERR_UNSUPPORTED = 4,       /* scene feature outside this backend (torus colliders; hull pairs other than hull-box) */
+    AVBD_ERR_UNSUPPORTED = 4,       /* request outside the scene's configuration (snapshots of soft scenes; isotropic state with the mode off) */
     AVBD_ERR_RUNTIME_FAILURE = 5,   /* a RuntimeFailure is latched; see avbd_last_failure */
     AVBD_ERR_HIP = 6,               /* HIP API error outside a latched frame */
     AVBD_ERR_TOO_LARGE = 7          /* request exceeds a fixed capacity (avbd_set_body_states_noalloc) */

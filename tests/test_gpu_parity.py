@@ -70,8 +70,24 @@ def _granular(n=1500, seed=12):
     return s
 
 
+def _block_with_rigid(seed=13):
+    """A 9^3 soft block with small hashed rigid spheres and cubes resting on
+    and beside it: some tet particles have a rigid collider in a neighbour
+    cell (pair search on the wave path), the rest only the ground."""
+    s = _block(8, 5)
+    rng = demos.SplitMix64(seed)
+    for i in range(12):
+        p = (rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), rng.uniform(1.45, 1.6))
+        if i % 2:
+            s.addSphere(0.1, 500.0, 0.5, p)
+        else:
+            s.addBody((0.08, 0.08, 0.08), 500.0, 0.5, p)
+    return s
+
+
 def test_hash_cells_and_pairs_bitexact():
     for name, s in [("stack 10", demos.stack_scene(10)), ("soft block 9^3", _block(8, 5)),
+                    ("soft block 9^3 + small rigid bodies", _block_with_rigid()),
                     ("soft block 21^3", _block(20, 2)), ("rigid cloud 300", _cloud()),
                     ("granular 1500 + 50 small rigid (search grid)", _granular()),
                     ("soft block 41^3 (search grid, 64-collider buckets)", _block(40, 1))]:
