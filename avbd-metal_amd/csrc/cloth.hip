@@ -1201,6 +1201,25 @@ struct VtQuery {
   float cap2;
 };
 
+// Solid key of an element: the closed surface's group when all its vertices
+// lie on that surface and none on an authored triangle, else all ones.  A
+// solid query skips exactly the candidates whose key is its own group, so a
+// bucket whose least and greatest key are both that group is skipped whole.
+__device__ __forceinline__ uint32_t solid_key3(const DeviceState& S, uint32_t a, uint32_t b, uint32_t c) {
+  const uint32_t g = S.clothGroup[a];
+  return (g != 0u && S.clothGroup[b] == g && S.clothGroup[c] == g && S.clothVert[a] == 0u && S.clothVert[b] == 0u &&
+          S.clothVert[c] == 0u)
+             ? g
+             : kNone;
+}
+__device__ __forceinline__ uint32_t solid_key2(const DeviceState& S, uint32_t a, uint32_t b) {
+  const uint32_t g = S.clothGroup[a];
+  return (g != 0u && S.clothGroup[b] == g && S.clothVert[a] == 0u && S.clothVert[b] == 0u) ? g : kNone;
+}
+__device__ __forceinline__ bool own_solid_bucket(const DeviceState& S, uint32_t slot, bool solid, uint32_t g) {
+  return solid && S.isoSlotLo[slot] == g && S.isoSlotHi[slot] == g;
+}
+
 // one candidate triangle of a V-T tracker (25_cloth.metal:423-463)
 __device__ __forceinline__ void vt_consider(const DeviceState& S, const StepParams& P, const VtQuery& q, Best4& best,
                                             float& own, uint32_t t) {
@@ -1401,7 +1420,7 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
   if (gid >= P.numTris + P.numContactEdges) return;
   V3 lo, hi;
   float pad;
-  uint32_t entry;
+  uint32_t entry, key = kNone;
   if (gid < P.numTris) {
     const U4 t = S.tris[gid];
     const V3 a = P3(S, t.x), b = P3(S, t.y), c = P3(S, t.z);
@@ -1409,6 +1428,7 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
     lo = vmin(vmin(a, b), c);
     hi = vmax(vmax(a, b), c);
     entry = gid;
+    if (!SCATTER) key = solid_key3(S, t.x, t.y, t.z);
   } else {
     const U2 e = S.edges[gid - P.numTris];
     const V3 a = P3(S, e.x), b = P3(S, e.y);
@@ -1416,6 +1436,7 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
     lo = vmin(a, b);
     hi = vmax(a, b);
     entry = kElemEdgeBit | (gid - P.numTris);
+    if (!SCATTER) key = solid_key2(S, e.x, e.y);
   }
   lo = lo - v3s(pad);
   hi = hi + v3s(pad);
@@ -1437,7 +1458,11 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
     if (dup) continue;
     const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
     if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
-    else atomicAdd(&S.elemCellCount[slot], 1u);
+    else {
+      atomicAdd(&S.elemCellCount[slot], 1u);
+      atomicMin(&S.isoSlotLo[slot], key);
+      atomicMax(&S.isoSlotHi[slot], key);
+    }
   }
 }
 
@@ -1502,6 +1527,7 @@ __global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P) 
           for (int dx = -1; dx <= 1; dx++) {
             const uint32_t h = cell_hash(cell_add(cc.x, dx), cell_add(cc.y, dy), cell_add(cc.z, dz), 0u,
                                          P.elemHashSize);
+            if (own_solid_bucket(S, 2u * h, q.solid, q.gv)) continue;  // only candidates the solid filter rejects
             const uint32_t s0 = S.elemCellStart[2u * h], e0 = S.elemCellStart[2u * h + 1u];
             for (uint32_t k = s0; k < e0; k++) vt_consider(S, P, q, best, own, S.elemCells[k]);
           }
@@ -1552,6 +1578,7 @@ __global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P) 
         for (int dx = -1; dx <= 1; dx++) {
           const uint32_t h =
               cell_hash(cell_add(cc.x, dx), cell_add(cc.y, dy), cell_add(cc.z, dz), 0u, P.elemHashSize);
+          if (own_solid_bucket(S, 2u * h + 1u, q.solid, q.ga)) continue;
           const uint32_t s0 = S.elemCellStart[2u * h + 1u], e0 = S.elemCellStart[2u * h + 2u];
           for (uint32_t k = s0; k < e0; k++) ee_consider(S, P, q, best, own, S.elemCells[k] & ~kElemEdgeBit);
         }
@@ -1732,6 +1759,8 @@ void launch_iso_track(hipStream_t st, const DeviceState& S, const StepParams& P,
   hipLaunchKernelGGL(k_iso_begin, dim3((nBegin + 255) / 256), dim3(256), 0, st, S, P);
   const uint32_t nElem = P.numTris + P.numContactEdges, slots = 2u * P.elemHashSize;
   launch_fill_u32(st, S.elemCellCount, slots + 1u, 0u);
+  launch_fill_u32(st, S.isoSlotLo, slots, kNone);
+  launch_fill_u32(st, S.isoSlotHi, slots, 0u);
   hipLaunchKernelGGL(k_iso_bin<false>, dim3((nElem + 255) / 256), dim3(256), 0, st, S, P);
   launch_exclusive_scan(st, S.elemCellCount, S.elemCellStart, slots + 1u, S.scanScratch);
   hipLaunchKernelGGL(k_copy_u32, dim3(cap_blocks(slots, gridCap)), dim3(256), 0, st, S.elemCellCursor,

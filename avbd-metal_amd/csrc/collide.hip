@@ -401,24 +401,46 @@ __device__ __forceinline__ void bp_producer(const DeviceState& S, const StepPara
   if (!EMIT && lane == 0) S.pairCount[producer] = count;
 }
 
-// The wave path: one wave per producer of bpWave (the producers not on the
-// lane list: globals, and hashed producers of dense packings).
+// The wave path: one wave per producer of bpWave[0, bpWaveCount) -- the
+// producers not on the lane list (globals, hashed producers of small dense
+// packings), then those the lane path deferred this frame.
 template <bool EMIT>
 __global__ void __launch_bounds__(256) k_bp_pairs(DeviceState S, StepParams P) {
   __shared__ uint64_t fineKeys[4][2][kFineCap];
-  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= P.numBpWave) return;  // whole wave
-  bp_producer<EMIT>(S, P, S.bpWave[w], fineKeys);
+  const uint32_t n = S.bpWaveCount[0];
+  for (uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += gridDim.x * 4)
+    bp_producer<EMIT>(S, P, S.bpWave[w], fineKeys);
 }
 
-// Hashed producers whose candidates are few, one lane each, with the wave
-// path's filters and order: its 27 neighbour cells in (dz, dy, dx) order (a
-// hash met earlier in that order is not walked again), each bucket in
-// collider order, then the globals.  The lane list (built at create) holds
-// every hashed producer when the hash buckets are small (no search grid), and
-// otherwise the soft-group particles of scenes without hashed non-particle
-// colliders, which pair with globals only; dense packings stay on the wave
-// path, where 64 lanes share a producer's candidates.
+// Hashed producers on the lane path, one lane each, with the wave path's
+// filters and order: the 27 neighbour cells in (dz, dy, dx) order (a hash met
+// earlier in that order is not walked again), each bucket in collider order,
+// then the globals.  A soft-group particle with no hashed non-particle
+// collider in the scene skips the cells.  Under the candidate search grid the
+// lane gathers its accepted partners from the 27 search cells, keys each by
+// (position of its hash bucket in the visit order, collider id) and keeps up
+// to kLaneCap of them sorted in registers -- the walk's set and order, as on
+// the wave path.  A producer with more than kLaneCandidates candidates in its
+// cells, or more than kLaneCap accepted search-grid partners, goes to the
+// wave path: the count pass appends it to bpWave (one atomic per wave), the
+// emit pass reaches the same decision and leaves it there.  The lane list
+// (built at create) holds every hashed producer without the search grid or
+// when there are many of them (one lane each hides the walk's latency better
+// than one wave each), else the soft-group particles that scan only globals.
+constexpr uint32_t kLaneCap = 8;
+constexpr uint32_t kLaneCandidates = 64;
+
+__device__ __forceinline__ void bp_defer(const DeviceState& S, bool defer, uint32_t producer) {
+  const uint64_t m = __ballot(defer);
+  if (m == 0ull) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__activemask()) - 1);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(S.bpWaveCount, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader, 64);
+  if (defer) S.bpWave[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = producer;
+}
+
 template <bool EMIT>
 __global__ void __launch_bounds__(256) k_bp_lane(DeviceState S, StepParams P) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,26 +453,90 @@ __global__ void __launch_bounds__(256) k_bp_lane(DeviceState S, StepParams P) {
   const bool aDyn = (ma.z & 2u) != 0;
   const V3 pa = xyz(ca);
   const float ra = ca.w;
+  const bool cells = !(ga != 0 && P.numHashedRigid == 0);
+  const bool fine = P.fineCellSize > 0.0f;
+  // the 27 cells walked (hash cells, or search cells under the search grid)
+  // and their ranges, duplicates and skipped cells empty
+  uint32_t h[27], cs[27], cc[27];
+  uint32_t total = 0;
+  int cx = 0, cy = 0, cz = 0;
+  if (cells) {
+    cell_coord(pa, P.cellSize, cx, cy, cz);
+#pragma unroll
+    for (int q = 0; q < 27; q++)
+      h[q] = cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, grpA, P.gridHashSize);
+    int wx = cx, wy = cy, wz = cz;
+    if (fine) cell_coord(pa, P.fineCellSize, wx, wy, wz);
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      const uint32_t hq =
+          fine ? cell_hash(wx + q % 3 - 1, wy + (q / 3) % 3 - 1, wz + q / 9 - 1, grpA, P.gridHashSize) : h[q];
+      bool dup = false;
+#pragma unroll
+      for (int j = 0; j < q; j++)
+        dup |= (fine ? cell_hash(wx + j % 3 - 1, wy + (j / 3) % 3 - 1, wz + j / 9 - 1, grpA, P.gridHashSize) : h[j]) ==
+               hq;
+      const bool skip = dup || (!fine && ga != 0 && S.cellRigid[hq] == 0);
+      cs[q] = skip ? 0u : (fine ? S.fineStart[hq] : S.cellStart[hq]);
+      cc[q] = skip ? 0u : (fine ? S.fineCount[hq] : S.cellCount[hq]);
+      total += cc[q];
+    }
+  }
+  bool defer = total > kLaneCandidates;
+  uint64_t keys[kLaneCap];
+  uint32_t nAcc = 0;
+  if (!defer && cells && fine) {
+#pragma unroll
+    for (uint32_t k = 0; k < kLaneCap; k++) keys[k] = ~0ull;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      for (uint32_t k = cs[q]; k < cs[q] + cc[q]; k++) {
+        const uint32_t g = S.fineBodies[k];
+        const uint32_t b = S.hashedIdx[g];
+        if (b <= a) continue;
+        const U4 mb = S.hashMeta[g];
+        const F4 cb = S.hashCenter[g];
+        if ((ga != 0 && mb.w != 0) ||
+            !pair_accept(S, P, ownerA, grpA, shA, aDyn, pa, ra, mb.x, mb.y, mb.z & 1u, (mb.z & 2u) != 0, xyz(cb),
+                         cb.w))
+          continue;
+        int bx, by, bz;
+        cell_coord(xyz(cb), P.cellSize, bx, by, bz);
+        const uint32_t hb = cell_hash(bx, by, bz, S.colliderGroup[b], P.gridHashSize);
+        if (ga != 0 && S.cellRigid[hb] == 0) continue;
+        uint32_t pos = 27;
+#pragma unroll
+        for (int j = 26; j >= 0; j--) pos = h[j] == hb ? (uint32_t)j : pos;
+        if (pos == 27) continue;
+        // sorted insertion (ascending keys; static register indices)
+        uint64_t x = ((uint64_t)pos << 32) | b;
+#pragma unroll
+        for (uint32_t m = 0; m < kLaneCap; m++) {
+          const uint64_t lo = keys[m] < x ? keys[m] : x, hi = keys[m] < x ? x : keys[m];
+          keys[m] = lo;
+          x = hi;
+        }
+        nAcc++;
+      }
+    }
+    defer = nAcc > kLaneCap;
+  }
+  if (!EMIT) bp_defer(S, defer, producer);
+  if (defer) return;
   const uint32_t base = EMIT ? S.pairStart[producer] : 0u;
   uint32_t count = 0;
   auto emit = [&](uint32_t x, uint32_t y) {
     if (EMIT && base + count < P.maxPairs) S.pairs[base + count] = U2{x, y};
     count++;
   };
-  if (!(ga != 0 && P.numHashedRigid == 0)) {
-    int cx, cy, cz;
-    cell_coord(pa, P.cellSize, cx, cy, cz);
-    uint32_t h[27];
+  if (cells && fine) {
 #pragma unroll
-    for (int q = 0; q < 27; q++) h[q] = cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, grpA, P.gridHashSize);
+    for (uint32_t k = 0; k < kLaneCap; k++)
+      if (k < nAcc) emit(a, (uint32_t)(keys[k] & 0xFFFFFFFFull));
+  } else if (cells) {
 #pragma unroll
     for (int q = 0; q < 27; q++) {
-      bool dup = false;
-#pragma unroll
-      for (int j = 0; j < q; j++) dup |= h[j] == h[q];
-      if (dup || (ga != 0 && S.cellRigid[h[q]] == 0)) continue;
-      const uint32_t cs = S.cellStart[h[q]], ce = cs + S.cellCount[h[q]];
-      for (uint32_t k = cs; k < ce; k++) {
+      for (uint32_t k = cs[q]; k < cs[q] + cc[q]; k++) {
         const uint32_t b = S.cellBodies[k];
         if (b <= a) continue;
         const U4 mb = S.cellMeta[k];
@@ -539,15 +625,18 @@ void launch_broadphase(hipStream_t st, const DeviceState& S, const StepParams& P
   }
   uint32_t producers = P.numHashed + P.numGlobals;
   if (producers > 0) {
+    // the wave path's grid: its static list plus a bounded share for
+    // deferred lane producers (the kernel loops over its list)
+    const uint32_t deferred = blocks_for(P.numBpLane, 4);
+    const uint32_t waveBlocks = blocks_for(P.numBpWave, 4) + (deferred < 1024u ? deferred : 1024u);
+    launch_fill_u32(st, S.bpWaveCount, 1u, P.numBpWave);
     if (P.numBpLane > 0)
       hipLaunchKernelGGL(k_bp_lane<false>, dim3(blocks_for(P.numBpLane, 256)), dim3(256), 0, st, S, P);
-    if (P.numBpWave > 0)
-      hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for(P.numBpWave, 4)), dim3(256), 0, st, S, P);
+    if (waveBlocks > 0) hipLaunchKernelGGL(k_bp_pairs<false>, dim3(waveBlocks), dim3(256), 0, st, S, P);
     launch_exclusive_scan(st, S.pairCount, S.pairStart, producers, S.scanScratch);
     if (P.numBpLane > 0)
       hipLaunchKernelGGL(k_bp_lane<true>, dim3(blocks_for(P.numBpLane, 256)), dim3(256), 0, st, S, P);
-    if (P.numBpWave > 0)
-      hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(P.numBpWave, 4)), dim3(256), 0, st, S, P);
+    if (waveBlocks > 0) hipLaunchKernelGGL(k_bp_pairs<true>, dim3(waveBlocks), dim3(256), 0, st, S, P);
   }
   hipLaunchKernelGGL(k_bp_finalize, dim3(1), dim3(64), 0, st, S, P);
 }

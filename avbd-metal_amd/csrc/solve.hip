@@ -237,39 +237,56 @@ __global__ void __launch_bounds__(256) k_color_iterate(DeviceState S, StepParams
 // Repair of an unfinished parallel coloring (40_solver.metal:387-436).  The
 // reference walks all bodies serially, giving each the lowest color unused
 // by its lower-index neighbours.  That serial result is the unique fixed
-// point of the same rule applied in parallel: a body's value is final once
-// all lower-index neighbours are final.  So every grid pass re-applies the
-// rule in place, and a pass that changes nothing has reached that fixed
-// point.  Cooperative launch; flags F[21 + p % 3] count changes of pass p.
+// point of the same rule applied in any order: a body's value is final once
+// all lower-index neighbours are final.  Each workgroup owns a contiguous
+// range of bodies and re-applies the rule to it until the range stops
+// changing (workgroup barriers only: chains inside the range resolve
+// there), then the grid meets at a barrier; a round in which no workgroup's
+// first local pass changed anything has reached the fixed point.  Chains
+// that cross ranges cost a grid round each; in index-ordered scenes they are
+// rare.  Cooperative launch; flags F[21 + r % 3] count changing workgroups
+// of round r.
+__device__ __forceinline__ bool repair_pass(const DeviceState& S, uint32_t* colors, uint32_t lo, uint32_t hi) {
+  bool changed = false;
+  for (uint32_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
+    uint32_t cur = __hip_atomic_load(&colors[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t nc = 0;
+    if (S.posLin[v].w > 0.0f) {
+      ColorMasks m = neighbor_masks(S, v, colors, cur);
+      uint64_t freeL = ~m.lower;
+      nc = freeL ? (uint32_t)__builtin_ctzll(freeL) : 63u;
+    }
+    if (nc != cur) {
+      __hip_atomic_store(&colors[v], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      changed = true;
+    }
+  }
+  return __syncthreads_or(changed) != 0;
+}
+
 __global__ void __launch_bounds__(256) k_color_repair_coop(DeviceState S, StepParams P, uint32_t* colors,
                                                            uint32_t finalPass) {
   if (S.colorFlags[finalPass] == 0u) return;  // grid-uniform: the 20 Jacobi passes converged
   uint32_t* bar = &S.counters[CTR_REPAIR_BAR];
   uint32_t* fail = &S.counters[CTR_GBAR_FAIL];
   uint32_t epoch = 0;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
-  for (uint32_t pass = 0; pass <= P.numBodies; pass++) {
-    if (tid == 0) S.colorFlags[21 + (pass + 1) % 3] = 0u;
-    bool changed = false;
-    for (uint32_t v = tid; v < P.numBodies; v += nth) {
-      uint32_t cur = __hip_atomic_load(&colors[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t nc = 0;
-      if (S.posLin[v].w > 0.0f) {
-        ColorMasks m = neighbor_masks(S, v, colors, cur);
-        uint64_t freeL = ~m.lower;
-        nc = freeL ? (uint32_t)__builtin_ctzll(freeL) : 63u;
+  const uint32_t chunk = (P.numBodies + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = blockIdx.x * chunk < P.numBodies ? blockIdx.x * chunk : P.numBodies;
+  const uint32_t hi = lo + chunk < P.numBodies ? lo + chunk : P.numBodies;
+  for (uint32_t round = 0; round <= P.numBodies; round++) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.colorFlags[21 + (round + 1) % 3] = 0u;
+    // local passes: the first one's change is what this round reports
+    bool first = repair_pass(S, colors, lo, hi);
+    if (first) {
+      for (uint32_t k = 1; k <= hi - lo && repair_pass(S, colors, lo, hi); k++) {
       }
-      if (nc != cur) {
-        __hip_atomic_store(&colors[v], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        changed = true;
-      }
+      if (threadIdx.x == 0) atomicAdd(&S.colorFlags[21 + round % 3], 1u);
     }
-    if (__syncthreads_or(changed) && threadIdx.x == 0) atomicAdd(&S.colorFlags[21 + pass % 3], 1u);
     epoch += gridDim.x;
     grid_barrier(bar, fail, epoch);
-    uint32_t any = __hip_atomic_load(&S.colorFlags[21 + pass % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t any = __hip_atomic_load(&S.colorFlags[21 + round % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (any == 0u || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-      if (tid == 0) S.counters[CTR_REPAIR_PASSES] = pass + 1;
+      if (blockIdx.x == 0 && threadIdx.x == 0) S.counters[CTR_REPAIR_PASSES] = round + 1;
       return;
     }
   }

@@ -1615,19 +1615,23 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.largeCells, hashed.size() / 33 + 1)
   }
   uint32_t producers = s->nHashed + s->nGlobal;
-  // pair producers: one lane each where the candidates are few -- every
-  // hashed producer when buckets are small (no search grid), else the
-  // soft-group particles of scenes with no hashed non-particle collider
-  // (they pair with globals only); one wave each for the rest and the globals
+  // pair producers: one lane each where that walk is short or hides its
+  // latency -- every hashed producer without the search grid (small
+  // buckets) or when there are many (kLaneMany), else the soft-group
+  // particles of scenes with no hashed non-particle collider (globals
+  // only); one wave each for the rest and the globals
+  constexpr size_t kLaneMany = 32768;
   std::vector<uint32_t> bpLane, bpWave;
   for (size_t k = 0; k < hashed.size(); k++) {
-    const bool lane = s->fineCellSize <= 0.0f || (clothGroup[co[hashed[k]]] != 0u && s->nHashedRigid == 0);
+    const bool lane = s->fineCellSize <= 0.0f || hashed.size() >= kLaneMany ||
+                      (clothGroup[co[hashed[k]]] != 0u && s->nHashedRigid == 0);
     (lane ? bpLane : bpWave).push_back((uint32_t)k);
   }
   for (uint32_t g = 0; g < s->nGlobal; g++) bpWave.push_back(s->nHashed + g);
   s->numBpLane = (uint32_t)bpLane.size();
   s->numBpWave = (uint32_t)bpWave.size();
-  ALLOC(S.bpLane, std::max<size_t>(1, bpLane.size())) ALLOC(S.bpWave, std::max<size_t>(1, bpWave.size()))
+  ALLOC(S.bpLane, std::max<size_t>(1, bpLane.size())) ALLOC(S.bpWave, std::max<uint32_t>(1, producers))
+  ALLOC(S.bpWaveCount, 1)
   ALLOC(S.pairCount, std::max<uint32_t>(1, producers)) ALLOC(S.pairStart, std::max<uint32_t>(1, producers))
   ALLOC(S.pairs, s->maxPairs) ALLOC(S.exclusions, std::max<size_t>(1, exr.size()))
   ALLOC(S.manifolds, s->maxPairs) ALLOC(S.prevManifolds, s->maxPairs)
@@ -1687,6 +1691,7 @@ avbd_status build_static_and_upload(avbd_solver* s, const avbd_scene_desc* d) {
     ALLOC(S.vEdgeStart, iso ? nb : 1u) ALLOC(S.vEdgeCount, iso ? nb : 1u)
     ALLOC(S.vEdgeList, std::max<size_t>(1, ch.vEdgeList.size())) ALLOC(S.clothVert, iso ? nb : 1u)
     ALLOC(S.vtTrack, iso ? nb : 1u) ALLOC(S.vtPrev, iso ? nb : 1u)
+    ALLOC(S.isoSlotLo, iso ? 2u * s->elemHashSize : 1u) ALLOC(S.isoSlotHi, iso ? 2u * s->elemHashSize : 1u)
     ALLOC(S.eeTrack, iso ? nce : 1u) ALLOC(S.eePrev, iso ? nce : 1u)
     ALLOC(S.ogcBound, iso ? nb : 1u) ALLOC(S.ogcPrev, iso ? nb : 1u) ALLOC(S.isoFrame, 1u)
   }

@@ -49,9 +49,10 @@ struct DeviceState {
   U2* fineSlot;
   uint32_t* largeCells;       // buckets of more than 32 colliders (sorted by a workgroup each)
   uint32_t *pairCount, *pairStart;
-  // pair producers by path (built at create): bpLane one lane each, bpWave
-  // one wave each
-  uint32_t *bpLane, *bpWave;
+  // pair producers by path: bpLane one lane each; bpWave one wave each,
+  // its static part (built at create) followed by the lane producers
+  // deferred this frame, bpWaveCount[0] entries in all
+  uint32_t *bpLane, *bpWave, *bpWaveCount;
   U2* pairs;
   U2* exclusions;
   // contacts
@@ -96,6 +97,10 @@ struct DeviceState {
   uint32_t *vEdgeStart, *vEdgeCount, *vEdgeList;  // incident edges of every body, in edge order
   uint32_t* clothVert;        // per body: 1 for a vertex of an authored triangle
   U4 *vtTrack, *vtPrev;       // per body: tracked triangles of this frame / of the previous one
+  // per element-grid slot (bucket triangles | edges) this frame: the least
+  // and greatest solid key of its entries (a closed surface's group when the
+  // element lies on it and no authored triangle touches it, else all ones)
+  uint32_t *isoSlotLo, *isoSlotHi;
   U4 *eeTrack, *eePrev;       // per contact edge: tracked edges of this frame / of the previous one
   uint32_t* ogcBound;         // per body: squared clearance bound (f32 bits, atomic min)
   F4* ogcPrev;                // per body: the pose the bounds hold around

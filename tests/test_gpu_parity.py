@@ -85,12 +85,32 @@ def _block_with_rigid(seed=13):
     return s
 
 
+def _granular_large(n=40000, seed=14):
+    """40000 free particles (r = 1 cm) in a 0.6 m cube, about six overlaps
+    each, with 60 small rigid bodies: enough producers for the one-lane
+    search-grid walk, and a tail of producers with more than eight accepted
+    partners that it hands to the one-wave walk."""
+    rng = demos.SplitMix64(seed)
+    s = demos.PhysicsScene("granular large", SimSettings(iterations=2))
+    demos.add_ground(s)
+    for i in range(n):
+        s.addParticle(0.01, 0.001, (rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(0.0, 0.6)))
+    for i in range(60):
+        p = (rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(0.0, 0.6))
+        if i % 2:
+            s.addSphere(0.03, 500.0, 0.5, p)
+        else:
+            s.addBody((0.02, 0.02, 0.02), 500.0, 0.5, p)
+    return s
+
+
 def test_hash_cells_and_pairs_bitexact():
     for name, s in [("stack 10", demos.stack_scene(10)), ("soft block 9^3", _block(8, 5)),
                     ("soft block 9^3 + small rigid bodies", _block_with_rigid()),
                     ("soft block 21^3", _block(20, 2)), ("rigid cloud 300", _cloud()),
                     ("granular 1500 + 50 small rigid (search grid)", _granular()),
-                    ("soft block 41^3 (search grid, 64-collider buckets)", _block(40, 1))]:
+                    ("soft block 41^3 (search grid, 64-collider buckets)", _block(40, 1)),
+                    ("granular 40000 + 60 small rigid (search grid walked one lane per producer)", _granular_large())]:
         _case(f"hash + pairs, {name}")
         g = GPUSolver(s)
         o = oracle.OracleSolver(s)
