@@ -1466,14 +1466,96 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
   }
 }
 
+// V-T trackers of pass 1 on 16-lane rows, one surface particle per row: the
+// candidate pool (tracked set, triangles adjacent to its best, the mesh
+// neighbours' best) one candidate per lane, and the 27-cell reseed with the
+// lanes striding over each bucket.  Each lane keeps its own best four; a
+// butterfly merge keeps the four smallest distinct (d2, id) of the union,
+// which is what the serial scan keeps in any candidate order (an element's
+// distance does not depend on who evaluates it), and the bound updates are
+// minima, so the result equals the one-thread tracker's.
+__device__ __forceinline__ void b4_add(Best4& b, uint32_t t, float d2) {
+  if (t != kNone && !b4_has(b, t)) b4_insert(b, t, d2);
+}
+__device__ __forceinline__ void b4_row_merge(Best4& b) {
+  for (int o = 8; o > 0; o >>= 1) {
+    const uint32_t i0 = __shfl_xor(b.id0, o, 16), i1 = __shfl_xor(b.id1, o, 16);
+    const uint32_t i2 = __shfl_xor(b.id2, o, 16), i3 = __shfl_xor(b.id3, o, 16);
+    const float e0 = __shfl_xor(b.d0, o, 16), e1 = __shfl_xor(b.d1, o, 16);
+    const float e2 = __shfl_xor(b.d2, o, 16), e3 = __shfl_xor(b.d3, o, 16);
+    b4_add(b, i0, e0);
+    b4_add(b, i1, e1);
+    b4_add(b, i2, e2);
+    b4_add(b, i3, e3);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_iso_track_vt(DeviceState S, StepParams P) {
+  const uint32_t lane = threadIdx.x & 15u;
+  const uint32_t p = blockIdx.x * 16u + (threadIdx.x >> 4);
+  if (p == 0u && lane == 0u) S.softFlag[P.numParticles + P.numContactEdges] = 0u;
+  if (p >= P.numParticles) return;  // whole row
+  const uint32_t frame = S.isoFrame[0] - 1u;
+  const float cs = P.surfaceContactCellSize;
+  Best4 best;
+  b4_init(best);
+  float own = as_float(kOgcFarBits);
+  VtQuery q;
+  q.v = S.particleIdx[p];
+  q.p = P3(S, q.v);
+  q.gv = S.clothGroup[q.v];
+  q.solid = q.gv != 0u && S.clothVert[q.v] == 0u;
+  q.ns = S.nbrStart[q.v];
+  q.ne = q.ns + S.nbrCount[q.v];
+  q.n2s = S.nbr2Start[q.v];
+  q.n2e = q.n2s + S.nbr2Count[q.v];
+  q.cap2 = cs * cs;
+  const U4 tr = S.vtPrev[q.v];
+  // the pool: 4 tracked, 3 adjacent to the tracked best, 8 neighbours' best
+  uint32_t t = kNone;
+  if (lane < 4u) {
+    t = u4_at(tr, (int)lane);
+  } else if (lane < 7u) {
+    if (tr.x < P.numTris) t = u4_at(S.triAdj[tr.x], (int)lane - 4);
+  } else if (lane < 15u) {
+    const uint32_t k = q.ns + (lane - 7u);
+    if (k < q.ne) t = S.vtPrev[S.nbrList[k]].x;
+  }
+  vt_consider(S, P, q, best, own, t);
+  b4_row_merge(best);
+  V3 relV = xyz(S.velLin[q.v]);
+  if (best.id0 < P.numTris) {
+    const U4 bt = S.tris[best.id0];
+    relV = relV - (xyz(S.velLin[bt.x]) + xyz(S.velLin[bt.y]) + xyz(S.velLin[bt.z])) / 3.0f;
+  }
+  if (iso_reseed(P, q.v, frame, relV, best.id0 == kNone, q.solid)) {
+    const Cell3 cc = cell_of(q.p, cs);
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const uint32_t h = cell_hash(cell_add(cc.x, dx), cell_add(cc.y, dy), cell_add(cc.z, dz), 0u,
+                                       P.elemHashSize);
+          if (own_solid_bucket(S, 2u * h, q.solid, q.gv)) continue;  // only candidates the solid filter rejects
+          const uint32_t s0 = S.elemCellStart[2u * h], e0 = S.elemCellStart[2u * h + 1u];
+          for (uint32_t k = s0 + lane; k < e0; k += 16u) vt_consider(S, P, q, best, own, S.elemCells[k]);
+        }
+    b4_row_merge(best);
+  }
+  for (int o = 8; o > 0; o >>= 1) own = fminf(own, __shfl_xor(own, o, 16));
+  if (lane != 0u) return;
+  const U4 kept = U4{best.id0, best.id1, best.id2, best.id3};
+  S.vtTrack[q.v] = kept;
+  if (own < as_float(kOgcFarBits)) atomicMin(&S.ogcBound[q.v], as_uint(own));
+  S.softFlag[p] = vt_emit<false>(S, P, q.v, kept, 0u);
+}
+
 // Trackers (pass 1: tracked sets, bounds, contact counts into softFlag) and
 // emission (pass 2: records from the tracked sets at the rigid-feature count
 // plus softStart).  One thread per producer.
 template <bool EMIT>
-__global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_iso_track(DeviceState S, StepParams P, uint32_t first) {
+  const uint32_t p = first + blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t nProd = P.numParticles + P.numContactEdges;
-  if (!EMIT && p == 0) S.softFlag[nProd] = 0u;
   if (p >= nProd) return;
   if (EMIT) {
     const uint32_t base = S.counters[CTR_SOFT_RT] + S.softStart[p];
@@ -1767,9 +1849,15 @@ void launch_iso_track(hipStream_t st, const DeviceState& S, const StepParams& P,
                      S.elemCellStart, slots);
   hipLaunchKernelGGL(k_iso_bin<true>, dim3((nElem + 255) / 256), dim3(256), 0, st, S, P);
   const uint32_t nProd = P.numParticles + P.numContactEdges;
-  hipLaunchKernelGGL(k_iso_track<false>, dim3((nProd + 255) / 256), dim3(256), 0, st, S, P);
+  // pass 1: V-T trackers on 16-lane rows (also clearing the scan's total
+  // slot), E-E trackers one thread each
+  hipLaunchKernelGGL(k_iso_track_vt, dim3((P.numParticles + 15) / 16 > 0 ? (P.numParticles + 15) / 16 : 1), dim3(256), 0,
+                     st, S, P);
+  if (P.numContactEdges > 0)
+    hipLaunchKernelGGL(k_iso_track<false>, dim3((P.numContactEdges + 255) / 256), dim3(256), 0, st, S, P,
+                       P.numParticles);
   launch_exclusive_scan(st, S.softFlag, S.softStart, nProd + 1u, S.scanScratch);
-  hipLaunchKernelGGL(k_iso_track<true>, dim3((nProd + 255) / 256), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_iso_track<true>, dim3((nProd + 255) / 256), dim3(256), 0, st, S, P, 0u);
 }
 
 void launch_ogc_refresh(hipStream_t st, const DeviceState& S, const StepParams& P) {
