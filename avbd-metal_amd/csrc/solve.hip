@@ -544,9 +544,18 @@ __device__ __forceinline__ void primal_group(const DeviceState& S, const StepPar
           stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, false, pv, S.initLin, S.initAng, P.alpha, acc, ls);
       }
     } else {
-      for (uint32_t k = lane; k < sc + dc; k += L) {
-        uint32_t entry = k < sc ? S.sAdjList[ss + k] : S.dAdjList[ds + (k - sc)];
-        stamp_entry(S, pv, entry, ss + k, body, xs, ls, rigid, P.alpha, P.dt, acc);
+      // static entries round-robin over the lanes; each contact manifold
+      // split over the lanes contact by contact (a body's few manifolds
+      // carry up to 8 contacts each), other per-frame entries round-robin
+      for (uint32_t k = lane; k < sc; k += L) stamp_entry(S, pv, S.sAdjList[ss + k], ss + k, body, xs, ls, rigid,
+                                                           P.alpha, P.dt, acc);
+      for (uint32_t j = 0; j < dc; j++) {
+        const uint32_t entry = S.dAdjList[ds + j];
+        if ((entry >> kAdjShift) == FK_MANIFOLD)
+          stamp_manifold(S.manifolds[entry & kAdjIndexMask], body, rigid, pv, S.initLin, S.initAng, P.alpha, acc, ls,
+                         lane, (uint32_t)L);
+        else if ((sc + j) % L == lane)
+          stamp_entry(S, pv, entry, ss + sc + j, body, xs, ls, rigid, P.alpha, P.dt, acc);
       }
     }
   }

@@ -155,9 +155,11 @@ __device__ __forceinline__ V3 contact_force(const ManifoldRec& m, int ci, V3 dqA
 
 // loSelf: the low part of the stamping body's position (added to its own
 // displacement from the step start)
+// contacts first, first + stride, ... of the manifold (a lane's share when a
+// body's lanes split one manifold)
 __device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t self, bool rigidSelf, const PoseView& pv,
                                                const F4* initLin, const F4* initAng, float alpha, Acc& acc,
-                                               V3 loSelf) {
+                                               V3 loSelf, uint32_t first = 0u, uint32_t stride = 1u) {
   uint32_t a = m.header.x, b = m.header.y, n = m.header.z;
   bool isA = self == a;
   bool rA_round = (m.header.w & 2u) != 0, rB_round = (m.header.w & 4u) != 0;
@@ -171,7 +173,7 @@ __device__ __forceinline__ void stamp_manifold(const ManifoldRec& m, uint32_t se
   V3 dqBA = q_sub(qB, initAng[b]);
   V3 nrm = xyz(m.basisN), t1 = xyz(m.basisT1), t2 = cross(nrm, t1);
   float s = isA ? 1.0f : -1.0f;
-  for (uint32_t i = 0; i < n; i++) {
+  for (uint32_t i = first; i < n; i += stride) {
     V3 rAW = rA_round ? xyz(m.c[i].rA) : q_rotate(qA, xyz(m.c[i].rA));
     V3 rBW = rB_round ? xyz(m.c[i].rB) : q_rotate(qB, xyz(m.c[i].rB));
     V3 C;
