@@ -147,41 +147,40 @@ __device__ __forceinline__ bool in_range(const CellRange& r, Cell3 c) {
 // element grid: every triangle / edge registers in each distinct hash bucket
 // of its padded AABB cover, at most 4 cells per axis (25_cloth.metal:182-267)
 // ---------------------------------------------------------------------------
-template <bool SCATTER>
-__global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= P.numTris + P.numEdges) return;
-  V3 lo, hi;
-  elem_bounds(S, P, gid, lo, hi);
-  if (!SCATTER) {
-    // lo.w carries the element's surface group when all its vertices share
-    // one (else 0): the queries test the same-component filter from it
-    uint32_t ge;
-    if (gid < P.numTris) {
-      U4 t = S.tris[gid];
-      ge = S.clothGroup[t.x];
-      if (S.clothGroup[t.y] != ge || S.clothGroup[t.z] != ge) ge = 0u;
-    } else {
-      U2 e = S.edges[gid - P.numTris];
-      ge = S.clothGroup[e.x];
-      if (S.clothGroup[e.y] != ge) ge = 0u;
-    }
-    S.elemBox[2 * gid] = f4(lo, as_float(ge));
-    S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
+namespace {
+
+// Bucket counters aggregated per workgroup: an LDS table keyed by bucket slot
+// gathers the block's (element, cell) registrations, then one device atomic
+// per distinct slot adds the block's count (count pass) or reserves the
+// block's run of the bucket (scatter pass).  A block of consecutive elements
+// of a sheet touches a few hundred buckets, so the device-scope atomics drop
+// roughly tenfold.  A slot that finds no table entry within kBinProbes
+// probes goes to the device counter directly (count pass) or turns the whole
+// block to direct per-entry cursors (scatter pass), so a full table costs
+// speed only.  Entries land in a bucket in no particular order, as before;
+// every consumer of the buckets is order independent.
+constexpr uint32_t kBinTable = 2048u, kBinProbes = 32u, kBinEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t bin_entry(uint32_t* key, uint32_t slot, bool insert) {
+  uint32_t i = (slot * 2654435761u) >> 21;
+  for (uint32_t p = 0; p < kBinProbes; p++, i = (i + 1u) & (kBinTable - 1u)) {
+    uint32_t k = __hip_atomic_load(&key[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == slot) return i;
+    if (k != kBinEmpty) continue;
+    if (!insert) return kBinEmpty;
+    uint32_t old = atomicCAS(&key[i], kBinEmpty, slot);
+    if (old == kBinEmpty || old == slot) return i;
   }
-  if (!finite3(lo) || !finite3(hi)) {
-    if (!SCATTER) {
-      atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
-      atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
-    }
-    return;
-  }
-  CellRange r = range_of(lo, hi, P.elemCellSize);
-  if (!SCATTER && r.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+  return kBinEmpty;
+}
+
+// f(slot) once per distinct hash bucket of the cover (a cover of up to 64
+// cells can hash two cells to one bucket; the element registers there once)
+template <typename F>
+__device__ __forceinline__ void for_each_bucket(const StepParams& P, const CellRange& r, bool tri, F&& f) {
   const Cell3 c0 = r.c0;
   const int nx = r.sx + 1, ny = r.sy + 1, nz = r.sz + 1;
   const int n = nx * ny * nz;  // <= 64
-  uint32_t entry = gid < P.numTris ? gid : (kElemEdgeBit | (gid - P.numTris));
   for (int i = 0; i < n; i++) {
     int x = cell_add(c0.x, i % nx), y = cell_add(c0.y, (i / nx) % ny), z = cell_add(c0.z, i / (nx * ny));
     uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
@@ -190,13 +189,82 @@ __global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
       int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
       dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
     }
-    if (dup) continue;
     // counts and cursors are (triangles, edges) per bucket, interleaved, so
     // one scan lays out every bucket as its triangles, then its edges
-    const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
-    if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
-    else atomicAdd(&S.elemCellCount[slot], 1u);
+    if (!dup) f(2u * h + (tri ? 0u : 1u));
   }
+}
+
+}  // namespace
+
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_el_bin(DeviceState S, StepParams P) {
+  __shared__ uint32_t key[kBinTable], cnt[kBinTable];
+  __shared__ uint32_t spill;
+  for (uint32_t i = threadIdx.x; i < kBinTable; i += blockDim.x) {
+    key[i] = kBinEmpty;
+    cnt[i] = 0u;
+  }
+  if (threadIdx.x == 0) spill = 0u;
+  __syncthreads();
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = gid < P.numTris + P.numEdges;
+  const bool tri = gid < P.numTris;
+  CellRange r{};
+  if (live) {
+    V3 lo, hi;
+    elem_bounds(S, P, gid, lo, hi);
+    if (!SCATTER) {
+      // lo.w carries the element's surface group when all its vertices share
+      // one (else 0): the queries test the same-component filter from it
+      uint32_t ge;
+      if (tri) {
+        U4 t = S.tris[gid];
+        ge = S.clothGroup[t.x];
+        if (S.clothGroup[t.y] != ge || S.clothGroup[t.z] != ge) ge = 0u;
+      } else {
+        U2 e = S.edges[gid - P.numTris];
+        ge = S.clothGroup[e.x];
+        if (S.clothGroup[e.y] != ge) ge = 0u;
+      }
+      S.elemBox[2 * gid] = f4(lo, as_float(ge));
+      S.elemBox[2 * gid + 1] = f4(hi, 0.0f);
+    }
+    if (!finite3(lo) || !finite3(hi)) {
+      if (!SCATTER) {
+        atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+        atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      }
+      live = false;
+    } else {
+      r = range_of(lo, hi, P.elemCellSize);
+      if (!SCATTER && r.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+    }
+  }
+  if (live)
+    for_each_bucket(P, r, tri, [&](uint32_t slot) {
+      uint32_t i = bin_entry(key, slot, true);
+      if (i != kBinEmpty) atomicAdd(&cnt[i], 1u);
+      else if (SCATTER) spill = 1u;
+      else atomicAdd(&S.elemCellCount[slot], 1u);
+    });
+  __syncthreads();
+  if (!SCATTER) {
+    for (uint32_t i = threadIdx.x; i < kBinTable; i += blockDim.x)
+      if (key[i] != kBinEmpty) atomicAdd(&S.elemCellCount[key[i]], cnt[i]);
+    return;
+  }
+  const bool direct = spill != 0u;
+  if (!direct)
+    for (uint32_t i = threadIdx.x; i < kBinTable; i += blockDim.x)
+      if (key[i] != kBinEmpty) cnt[i] = atomicAdd(&S.elemCellCursor[key[i]], cnt[i]);
+  __syncthreads();
+  if (!live) return;
+  const uint32_t entry = tri ? gid : (kElemEdgeBit | (gid - P.numTris));
+  for_each_bucket(P, r, tri, [&](uint32_t slot) {
+    uint32_t pos = direct ? atomicAdd(&S.elemCellCursor[slot], 1u) : atomicAdd(&cnt[bin_entry(key, slot, false)], 1u);
+    S.elemCells[pos] = entry;
+  });
 }
 
 __global__ void __launch_bounds__(256) k_copy_u32(uint32_t* dst, const uint32_t* src, uint32_t n) {
