@@ -293,17 +293,28 @@ __device__ __forceinline__ bool boxes_apart(V3 alo, V3 ahi, F4 lo, F4 hi, float 
          alo.y > hi.y + slack || alo.z > hi.z + slack;
 }
 
+// Each test is split in two: the box filter (the candidate's stored padded
+// box and surface group, one 32-byte read) and the rest (connectivity, the
+// owner cell, exact distance, 1-ring topology).  The accept predicate is
+// their conjunction; the count pass queues the box-filter survivors of a
+// wave's producers and runs the rest on full waves.
+//
 // `noSelf`: the producer's surface group when that surface does not
 // self-collide (else 0); a candidate of that group fails the same-component
 // filter, tested first from the group stored with the element's box.
-__device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell, uint32_t ns,
-                          uint32_t ne, uint32_t entry, float q2, uint32_t noSelf) {
+__device__ __forceinline__ bool vt_box(const DeviceState& S, const StepParams& P, V3 p, uint32_t entry,
+                                       uint32_t noSelf) {
   if ((entry & kElemEdgeBit) != 0u || entry >= P.numTris) return false;
   const F4 blo = S.elemBox[2 * entry], bhi = S.elemBox[2 * entry + 1];
   if (noSelf != 0u && as_uint(blo.w) == noSelf) return false;
-  if (outside_box(p, blo, bhi, kBoxSlack)) return false;
+  return !outside_box(p, blo, bhi, kBoxSlack);
+}
+
+__device__ __forceinline__ bool vt_rest(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell,
+                                        uint32_t ns, uint32_t ne, uint32_t entry, float q2) {
   U4 t = S.tris[entry];
   if (t.x == v || t.y == v || t.z == v) return false;
+  const F4 blo = S.elemBox[2 * entry], bhi = S.elemBox[2 * entry + 1];
   if (!in_range(range_of(xyz(blo), xyz(bhi), P.elemCellSize), cell)) return false;
   V3 bary;
   V3 q = closest_pt_triangle(p, P3(S, t.x), P3(S, t.y), P3(S, t.z), bary);
@@ -318,17 +329,27 @@ __device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams
                     nbr_contains(S.nbrList, ns, ne, t.z)));
 }
 
-__device__ __forceinline__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
-                          const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1, Cell3 cell,
-                          uint32_t entry, float q2, uint32_t noSelf) {
+__device__ __forceinline__ bool vt_accept(const DeviceState& S, const StepParams& P, uint32_t v, V3 p, Cell3 cell,
+                                          uint32_t ns, uint32_t ne, uint32_t entry, float q2, uint32_t noSelf) {
+  return vt_box(S, P, p, entry, noSelf) && vt_rest(S, P, v, p, cell, ns, ne, entry, q2);
+}
+
+__device__ __forceinline__ bool ee_box(const DeviceState& S, const StepParams& P, uint32_t gid, V3 a0, V3 a1,
+                                       uint32_t entry, uint32_t noSelf) {
   if ((entry & kElemEdgeBit) == 0u) return false;
   uint32_t other = entry & ~kElemEdgeBit;
   if (other <= gid || other >= P.numEdges) return false;
   const F4 blo = S.elemBox[2 * (P.numTris + other)], bhi = S.elemBox[2 * (P.numTris + other) + 1];
   if (noSelf != 0u && as_uint(blo.w) == noSelf) return false;
-  if (boxes_apart(vmin(a0, a1), vmax(a0, a1), blo, bhi, kBoxSlack)) return false;
+  return !boxes_apart(vmin(a0, a1), vmax(a0, a1), blo, bhi, kBoxSlack);
+}
+
+__device__ __forceinline__ bool ee_rest(const DeviceState& S, const StepParams& P, U2 a, V3 a0, V3 a1,
+                                        const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1,
+                                        Cell3 cell, uint32_t other, float q2) {
   U2 b = S.edges[other];
   if (b.x == a.x || b.x == a.y || b.y == a.x || b.y == a.y) return false;
+  const F4 blo = S.elemBox[2 * (P.numTris + other)], bhi = S.elemBox[2 * (P.numTris + other) + 1];
   CellRange br = range_of(xyz(blo), xyz(bhi), P.elemCellSize);
   // owned by the minimum cell of tight(A) x padded(B)
   Cell3 o0{max(ar.c0.x, br.c0.x), max(ar.c0.y, br.c0.y), max(ar.c0.z, br.c0.z)};
@@ -340,12 +361,19 @@ __device__ __forceinline__ bool ee_accept(const DeviceState& S, const StepParams
   V3 ca = a0 + (a1 - a0) * s, cb = b0 + (b1 - b0) * t;
   V3 d = ca - cb;
   if (dot(d, d) > q2) return false;
-  // topology filters last (see vt_accept)
+  // topology filters last (see vt_rest)
   uint32_t g = S.clothGroup[a.x];
   bool same = g != 0u && S.clothGroup[a.y] == g && S.clothGroup[b.x] == g && S.clothGroup[b.y] == g;
   if (same && S.selfCollision[a.x] == 0u) return false;
   return !(same && (nbr_contains(S.nbrList, ns0, ne0, b.x) || nbr_contains(S.nbrList, ns0, ne0, b.y) ||
                     nbr_contains(S.nbrList, ns1, ne1, b.x) || nbr_contains(S.nbrList, ns1, ne1, b.y)));
+}
+
+__device__ __forceinline__ bool ee_accept(const DeviceState& S, const StepParams& P, uint32_t gid, U2 a, V3 a0, V3 a1,
+                                          const CellRange& ar, uint32_t ns0, uint32_t ne0, uint32_t ns1, uint32_t ne1,
+                                          Cell3 cell, uint32_t entry, float q2, uint32_t noSelf) {
+  return ee_box(S, P, gid, a0, a1, entry, noSelf) &&
+         ee_rest(S, P, a, a0, a1, ar, ns0, ne0, ns1, ne1, cell, entry & ~kElemEdgeBit, q2);
 }
 
 // Visits producer `prod`'s candidates in chunks of `width` positions starting
@@ -479,16 +507,180 @@ __global__ void __launch_bounds__(256) k_dat_pairs(DeviceState S, StepParams P) 
   });
 }
 
-// Count pass: 16 lanes per producer, four producers per wave (a producer's
-// bucket rarely fills 64 lanes), per-lane counts summed within the group.
+// Count pass: 16 lanes per producer, four producers per wave.  The groups
+// walk their producers' buckets 16 candidates at a time through the box
+// filter only; the survivors go to the wave's LDS queue tagged with their
+// producer (and, for E-E, the cell whose bucket held them), and whenever
+// the queue holds 64 of them the remaining tests run on the full wave.  The
+// accepted count is the same as the emit pass's walk (both evaluate the same
+// predicate on the same candidates); the queue only keeps the long tests off
+// lanes the box filter already emptied.
+namespace {
+
+constexpr uint32_t kDatQueue = 128u;  // < 64 held + up to 64 added per step
+
+struct DatProducer {
+  uint32_t kind, id, ax, ay, ns0, ne0, ns1, ne1;  // kind 0: V-T (id = vertex), 1: E-E (id = edge)
+  float p0[3], p1[3];                             // vertex / edge ends
+  int c0[3], span[3];                             // the V-T cell, or the edge's tight cover
+};
+
+__device__ __forceinline__ Cell3 cover_cell(const CellRange& r, uint32_t ci) {
+  const uint32_t nx = (uint32_t)r.sx + 1u, ny = (uint32_t)r.sy + 1u;
+  return Cell3{cell_add(r.c0.x, (int)(ci % nx)), cell_add(r.c0.y, (int)((ci / nx) % ny)),
+               cell_add(r.c0.z, (int)(ci / (nx * ny)))};
+}
+
+}  // namespace
+
 __global__ void __launch_bounds__(256) k_dat_count(DeviceState S, StepParams P) {
-  const uint32_t sub = threadIdx.x & 15u;
-  const uint32_t prod = blockIdx.x * 16u + (threadIdx.x >> 4);
+  __shared__ DatProducer ctx[16];
+  __shared__ uint32_t qEntry[4][kDatQueue], qTag[4][kDatQueue];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, sub = threadIdx.x & 15u, grp = lane >> 4;
+  const uint32_t slot = threadIdx.x >> 4;
+  const uint32_t prod = blockIdx.x * 16u + slot;
   const uint32_t nProd = P.numParticles + P.numEdges;
-  uint32_t n = 0;
-  if (prod < nProd) dat_visit(S, P, prod, sub, 16u, true, [&](bool acc, uint32_t) { n += acc ? 1u : 0u; });
-  for (int o = 8; o > 0; o >>= 1) n += __shfl_xor(n, o, 16);
-  if (prod < nProd && sub == 0) S.datCount[prod] = n;
+  const float q2 = P.datQueryRadius * P.datQueryRadius;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // producer set-up: every lane of a group computes the same values
+  const bool vt = prod < P.numParticles;
+  bool live = prod < nProd;
+  V3 p0 = v3(0.0f, 0.0f, 0.0f), p1 = p0;
+  uint32_t gid = 0, noSelf = 0, nCells = 0;
+  CellRange ar{};
+  if (live && vt) {
+    const uint32_t v = S.particleIdx[prod];
+    p0 = P3(S, v);
+    if (!finite3(p0)) {
+      if (sub == 0) {
+        atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+        atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      }
+      live = false;
+    } else {
+      ar.c0 = cell_of(p0, P.elemCellSize);
+      nCells = 1;
+      noSelf = S.selfCollision[v] == 0u ? S.clothGroup[v] : 0u;
+      if (sub == 0) {
+        DatProducer& c = ctx[slot];
+        const uint32_t ns = S.nbrStart[v];
+        c.kind = 0u; c.id = v; c.ns0 = ns; c.ne0 = ns + S.nbrCount[v];
+        c.p0[0] = p0.x; c.p0[1] = p0.y; c.p0[2] = p0.z;
+        c.c0[0] = ar.c0.x; c.c0[1] = ar.c0.y; c.c0[2] = ar.c0.z;
+      }
+    }
+  } else if (live) {
+    gid = prod - P.numParticles;
+    const U2 a = S.edges[gid];
+    p0 = P3(S, a.x);
+    p1 = P3(S, a.y);
+    if (!finite3(p0) || !finite3(p1)) {
+      if (sub == 0) {
+        atomicAdd(&S.counters[CTR_DAT_INVALID], 1u);
+        atomicAdd(&S.counters[CTR_DAT_NONFINITE], 1u);
+      }
+      live = false;
+    } else {
+      ar = range_of(vmin(p0, p1), vmax(p0, p1), P.elemCellSize);
+      if (sub == 0 && ar.over) atomicAdd(&S.counters[CTR_DAT_OVERFLOW], 1u);
+      nCells = (uint32_t)((ar.sx + 1) * (ar.sy + 1) * (ar.sz + 1));
+      const uint32_t ga = S.clothGroup[a.x];
+      noSelf = (S.selfCollision[a.x] == 0u && S.clothGroup[a.y] == ga) ? ga : 0u;
+      if (sub == 0) {
+        DatProducer& c = ctx[slot];
+        const uint32_t ns0 = S.nbrStart[a.x], ns1 = S.nbrStart[a.y];
+        c.kind = 1u; c.id = gid; c.ax = a.x; c.ay = a.y;
+        c.ns0 = ns0; c.ne0 = ns0 + S.nbrCount[a.x]; c.ns1 = ns1; c.ne1 = ns1 + S.nbrCount[a.y];
+        c.p0[0] = p0.x; c.p0[1] = p0.y; c.p0[2] = p0.z;
+        c.p1[0] = p1.x; c.p1[1] = p1.y; c.p1[2] = p1.z;
+        c.c0[0] = ar.c0.x; c.c0[1] = ar.c0.y; c.c0[2] = ar.c0.z;
+        c.span[0] = ar.sx; c.span[1] = ar.sy; c.span[2] = ar.sz;
+      }
+    }
+  }
+  __syncthreads();
+
+  // a V-T producer reads its cell's triangles, an E-E producer each cover
+  // cell's edges (the buckets lay out triangles, then edges)
+  auto bucket = [&](uint32_t ci, uint32_t& bs, uint32_t& be) {
+    const Cell3 c = cover_cell(ar, ci);
+    const uint32_t h = cell_hash(c.x, c.y, c.z, 0u, P.elemHashSize);
+    const uint32_t o = vt ? 0u : 1u;
+    bs = S.elemCellStart[2u * h + o];
+    be = S.elemCellStart[2u * h + o + 1u];
+  };
+  // the remaining tests on queue items [0, n), then the queue's tail moves
+  // to its head; `count` gathers the accepted items of this lane's producer
+  uint32_t qn = 0, count = 0;
+  auto drain = [&](uint32_t n) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool has = lane < n;
+    const uint32_t entry = has ? qEntry[wv][lane] : 0u, tag = has ? qTag[wv][lane] : 0u;
+    const uint32_t g = tag & 3u;
+    bool acc = false;
+    if (has) {
+      const DatProducer& c = ctx[wv * 4u + g];
+      const V3 x0 = v3(c.p0[0], c.p0[1], c.p0[2]);
+      CellRange cr;
+      cr.c0 = Cell3{c.c0[0], c.c0[1], c.c0[2]};
+      if (c.kind == 0u) {
+        acc = vt_rest(S, P, c.id, x0, cr.c0, c.ns0, c.ne0, entry, q2);
+      } else {
+        cr.sx = c.span[0];
+        cr.sy = c.span[1];
+        cr.sz = c.span[2];
+        cr.over = false;
+        const V3 x1 = v3(c.p1[0], c.p1[1], c.p1[2]);
+        acc = ee_rest(S, P, U2{c.ax, c.ay}, x0, x1, cr, c.ns0, c.ne0, c.ns1, c.ne1, cover_cell(cr, tag >> 2),
+                      entry & ~kElemEdgeBit, q2);
+      }
+    }
+    for (uint32_t gg = 0; gg < 4u; gg++) {
+      const uint64_t m = __ballot(acc && g == gg);
+      if (gg == grp) count += (uint32_t)__popcll(m);
+    }
+    if (qn > n) {
+      const bool mv = lane < qn - n;
+      const uint32_t me = mv ? qEntry[wv][n + lane] : 0u, mt = mv ? qTag[wv][n + lane] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (mv) {
+        qEntry[wv][lane] = me;
+        qTag[wv][lane] = mt;
+      }
+    }
+    qn -= n;
+  };
+
+  uint32_t ci = 0, k0 = 0, ke = 0;
+  bool more = live;
+  if (more) bucket(0u, k0, ke);
+  while (true) {
+    while (more && k0 >= ke) {
+      if (++ci >= nCells) more = false;
+      else bucket(ci, k0, ke);
+    }
+    if (__ballot(more) == 0ull) break;
+    const uint32_t k = k0 + sub;
+    uint32_t entry = 0u;
+    bool surv = false;
+    if (more && k < ke) {
+      entry = S.elemCells[k];
+      surv = vt ? vt_box(S, P, p0, entry, noSelf) : ee_box(S, P, gid, p0, p1, entry, noSelf);
+    }
+    k0 += 16u;
+    const uint64_t m = __ballot(surv);
+    if (surv) {
+      const uint32_t pos = qn + (uint32_t)__popcll(m & below);
+      qEntry[wv][pos] = entry;
+      qTag[wv][pos] = grp | (ci << 2);
+    }
+    qn += (uint32_t)__popcll(m);
+    if (qn >= 64u) drain(64u);
+  }
+  if (qn > 0u) drain(qn);
+  if (prod < nProd && sub == 0) S.datCount[prod] = count;
 }
 
 // The frame's demand counters keep the larger of its two queries (step
