@@ -181,14 +181,21 @@ __device__ __forceinline__ void for_each_bucket(const StepParams& P, const CellR
   const Cell3 c0 = r.c0;
   const int nx = r.sx + 1, ny = r.sy + 1, nz = r.sz + 1;
   const int n = nx * ny * nz;  // <= 64
+  // `seen` holds one bit per bucket (mod 64) already registered: a cell whose
+  // bit is clear cannot repeat a bucket, so only the rare candidates of a
+  // repeat rescan the cover's earlier cells
+  uint64_t seen = 0ull;
   for (int i = 0; i < n; i++) {
     int x = cell_add(c0.x, i % nx), y = cell_add(c0.y, (i / nx) % ny), z = cell_add(c0.z, i / (nx * ny));
     uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
+    const uint64_t bit = 1ull << (h & 63u);
     bool dup = false;
-    for (int j = 0; j < i && !dup; j++) {
-      int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
-      dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
-    }
+    if ((seen & bit) != 0ull)
+      for (int j = 0; j < i && !dup; j++) {
+        int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
+        dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
+      }
+    seen |= bit;
     // counts and cursors are (triangles, edges) per bucket, interleaved, so
     // one scan lays out every bucket as its triangles, then its edges
     if (!dup) f(2u * h + (tri ? 0u : 1u));
@@ -1707,14 +1714,18 @@ __global__ void __launch_bounds__(256) k_iso_bin(DeviceState S, StepParams P) {
   const int nx = span((long long)raw.x - c0.x) + 1, ny = span((long long)raw.y - c0.y) + 1,
             nz = span((long long)raw.z - c0.z) + 1;
   const int n = nx * ny * nz;  // <= 27
+  uint64_t seen = 0ull;  // one bit per registered bucket (mod 64), as in the element grid
   for (int i = 0; i < n; i++) {
     const int x = cell_add(c0.x, i % nx), y = cell_add(c0.y, (i / nx) % ny), z = cell_add(c0.z, i / (nx * ny));
     const uint32_t h = cell_hash(x, y, z, 0u, P.elemHashSize);
+    const uint64_t bit = 1ull << (h & 63u);
     bool dup = false;
-    for (int j = 0; j < i && !dup; j++) {
-      const int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
-      dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
-    }
+    if ((seen & bit) != 0ull)
+      for (int j = 0; j < i && !dup; j++) {
+        const int xj = cell_add(c0.x, j % nx), yj = cell_add(c0.y, (j / nx) % ny), zj = cell_add(c0.z, j / (nx * ny));
+        dup = cell_hash(xj, yj, zj, 0u, P.elemHashSize) == h;
+      }
+    seen |= bit;
     if (dup) continue;
     const uint32_t slot = 2u * h + (gid < P.numTris ? 0u : 1u);
     if (SCATTER) S.elemCells[atomicAdd(&S.elemCellCursor[slot], 1u)] = entry;
