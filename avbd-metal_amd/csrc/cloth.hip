@@ -540,7 +540,10 @@ __device__ __forceinline__ Cell3 cover_cell(const CellRange& r, uint32_t ci) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_dat_count(DeviceState S, StepParams P) {
+// Six waves per SIMD (80 VGPRs, four spilled) measured 1 % faster on the
+// 256^2 cloth step than the compiler's choice of five (85 VGPRs); eight
+// spill 36 registers.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_dat_count(DeviceState S, StepParams P) {
   __shared__ DatProducer ctx[16];
   __shared__ uint32_t qEntry[4][kDatQueue], qTag[4][kDatQueue];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, sub = threadIdx.x & 15u, grp = lane >> 4;
