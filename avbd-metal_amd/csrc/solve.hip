@@ -789,10 +789,14 @@ __device__ __forceinline__ void dual_spring(const DeviceState& S, const StepPara
   if (C > 0.0f) sp.dual.y = fminf(sp.dual.y + C * P.betaLin, fminf(sp.rA.w, kPenaltyMax));
 }
 
-__device__ __forceinline__ void dual_manifold(const DeviceState& S, const StepParams& P, uint32_t gid) {
+// One item per contact slot (kMaxContacts per manifold): a contact's update
+// reads only its own record and the manifold's shared header, basis and
+// body poses, so the slots of a manifold run on adjacent lanes and their
+// records load as one contiguous span instead of a serial loop per thread.
+__device__ __forceinline__ void dual_contact(const DeviceState& S, const StepParams& P, uint32_t gid, uint32_t i) {
   ManifoldRec& m = S.manifolds[gid];
   uint32_t n = m.header.z;
-  if (n == 0) return;
+  if (i >= n) return;
   uint32_t a = m.header.x, b = m.header.y;
   bool rA_round = (m.header.w & 2u) != 0, rB_round = (m.header.w & 4u) != 0;
   V3 dqAL = xyz(S.posLin[a]) - xyz(S.initLin[a]);
@@ -804,7 +808,7 @@ __device__ __forceinline__ void dual_manifold(const DeviceState& S, const StepPa
   float minMass = fminf(mA > 0.0f ? mA : big, mB > 0.0f ? mB : big);
   float lamCap = fminf(P.lambdaMax, fmaxf(10.0f, 1.0e5f * minMass));
   V3 nrm = xyz(m.basisN), t1 = xyz(m.basisT1), t2 = cross(nrm, t1);
-  for (uint32_t i = 0; i < n; i++) {
+  {
     V3 rAW = rA_round ? xyz(m.c[i].rA) : q_rotate(S.posAng[a], xyz(m.c[i].rA));
     V3 rBW = rB_round ? xyz(m.c[i].rB) : q_rotate(S.posAng[b], xyz(m.c[i].rB));
     V3 C;
@@ -857,14 +861,14 @@ __device__ __forceinline__ void dual_item(const DeviceState& S, const StepParams
   uint32_t si = gid - P.numJoints;
   if (si < P.numSprings) { dual_spring(S, P, si); return; }
   uint32_t mi = si - P.numSprings;
-  if (mi < numPairs) { dual_manifold(S, P, mi); return; }
-  dual_soft(S, P, mi - numPairs);
+  if (mi < numPairs * kMaxContacts) { dual_contact(S, P, mi / kMaxContacts, mi % kMaxContacts); return; }
+  dual_soft(S, P, mi - numPairs * kMaxContacts);
 }
 
 __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
   if (P.convCheck && S.counters[CTR_CONV_DONE] != 0u) return;
   uint32_t numPairs = S.counters[CTR_PAIRS];
-  uint32_t total = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
+  uint32_t total = P.numJoints + P.numSprings + numPairs * kMaxContacts + S.counters[CTR_SOFT];
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += stride) dual_item(S, P, gid, numPairs);
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < P.numBodies; b += stride) {
@@ -874,12 +878,12 @@ __global__ void __launch_bounds__(256) k_dual(DeviceState S, StepParams P) {
 }
 
 void launch_dual(hipStream_t st, const DeviceState& S, const StepParams& P, uint32_t gridCap) {
-  uint32_t n = P.numJoints + P.numSprings + P.maxPairs + (P.numTris ? P.maxSoft : 0u);
+  uint64_t n = (uint64_t)P.numJoints + P.numSprings + (uint64_t)P.maxPairs * kMaxContacts + (P.numTris ? P.maxSoft : 0u);
   if (P.numBodies > n) n = P.numBodies;
-  uint32_t blocks = (n + 255) / 256;
+  uint64_t blocks = (n + 255) / 256;
   if (blocks > gridCap) blocks = gridCap;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(256), 0, st, S, P);
+  hipLaunchKernelGGL(k_dual, dim3((uint32_t)blocks), dim3(256), 0, st, S, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -1069,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_persistent(DeviceState S, StepParams P,
   uint32_t epoch = barrierBase;
   const uint32_t colorsUsed = numColors ? numColors : S.colorStart[kMaxColors + 1];
   const uint32_t numPairs = S.counters[CTR_PAIRS];
-  const uint32_t dualTotal = P.numJoints + P.numSprings + numPairs + S.counters[CTR_SOFT];
+  const uint32_t dualTotal = P.numJoints + P.numSprings + numPairs * kMaxContacts + S.counters[CTR_SOFT];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
   for (uint32_t it = 0; it < P.iterations; it++) {
     for (uint32_t c = 0; c < colorsUsed; c++) {
